@@ -1,0 +1,232 @@
+/* weave.h — C ABI of the MI355X dynamic-relaxation solver for woven elastic-rod gridshells.
+ *
+ * This is the drop-in boundary for the reference's batched GPU solver class `GpuSolver`
+ * (web/js/solver_gpu.js:62-523).  Every entry point below replaces one method of that class; the
+ * citation after each declaration names it.  The JavaScript host binds these symbols through the
+ * N-API addon in weave-sim_amd/csrc/napi (a `GpuSolver`-shaped class, weave-sim_amd/js/solver_mi355x.js);
+ * Python binds them with ctypes (weave-sim_amd/__init__.py).
+ *
+ * Conventions
+ *  - plain C: pointers and sizes only; every function returns a status (WS_OK = 0) and leaves a message
+ *    for ws_last_error() on failure;
+ *  - units and array layouts are those of the reference's host model (model.js:10-75): positions X[3n]
+ *    (x, y, z per node), quaternions Q[4m] (w, x, y, z per segment), body angular velocities W[3m];
+ *  - the host model (ws_model) is read on upload; device memory is owned by the library; downloads write
+ *    into caller-owned buffers (ws_state), as GpuSolver.download overwrites the CpuSolver's typed arrays
+ *    (solver_gpu.js:431-503);
+ *  - one host thread per batch handle; a context may hold many batches (they run on one HIP stream each).
+ */
+#ifndef WEAVE_H
+#define WEAVE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+#define WS_EXTERN extern "C"
+#else
+#define WS_EXTERN
+#endif
+#define WS_API WS_EXTERN __attribute__((visibility("default")))
+
+enum ws_status {
+  WS_OK = 0,
+  WS_ERR_ARG = 1,       /* bad argument (the reference throws a JS Error) */
+  WS_ERR_DEVICE = 2,    /* HIP runtime error */
+  WS_ERR_NOMEM = 3,     /* device allocation failed */
+  WS_ERR_TOPOLOGY = 4,  /* instances/clamps incompatible with the batch (solver_gpu.js:70,325) */
+  WS_ERR_CAPACITY = 5,  /* admission refused: over the HBM budget or out of slots */
+  WS_ERR_NOTFOUND = 6,  /* unknown request / instance */
+  WS_ERR_STATE = 7      /* call out of order */
+};
+
+/* arithmetic of the device path */
+enum ws_precision {
+  WS_FP32_FP64 = 0,     /* FP32 forces/velocities; FP64 positions, frames, kinetic energy, residual */
+  WS_FP64 = 1           /* plain FP64 everywhere */
+};
+
+/* Scalar parameters of one instance: the scene's SOLVER and CONTACT groups (params.js:8-23) with the
+ * derived values the reference passes in its IP struct (solver_gpu.js:244-269). */
+typedef struct ws_params {
+  double mass_scale;      /* SOLVER.mass_scale   1.5 */
+  double damping;         /* SOLVER.damping      1.0 */
+  double tol;             /* SOLVER.tol          1e-4 */
+  double mu;              /* CONTACT.mu          0.35 */
+  double strength;        /* CONTACT.strength    1e4   (penalty k_n) */
+  double stick_strength;  /* CONTACT.stick_strength 1e4 (tangential k_t) */
+  double torsion_factor;  /* CONTACT.torsion_factor 1.0 */
+  double epsilon;         /* CONTACT.epsilon     15.0  (slip step clip) */
+  double slip_rate;       /* CONTACT.slip_rate   1.0 */
+  double rot_factor;      /* CONTACT.rot_factor  1.0 */
+  double dent_rate;       /* CONTACT.dent_rate   0.01 */
+  double edge_threshold;  /* CONTACT.edge_threshold 0.5 */
+  double floor_z;         /* scene.floor.z (floor_on only) */
+  double floor_mu;        /* scene.floor.mu */
+  int32_t kinetic_damping;/* SOLVER.kinetic_damping 1 */
+  int32_t warmup_iter;    /* SOLVER.warmup_iter 300 */
+  int32_t max_iter;       /* SOLVER.max_iter 5000 (host loop only) */
+  int32_t contacts;       /* 1 when the instance has a crossing-contact goal (CpuSolver.contact != null) */
+  int32_t ramp_iter;      /* CONTACT.ramp_iter 1500 */
+  int32_t rot_friction;   /* CONTACT.rot_friction (W2) */
+  int32_t seat_law;       /* CONTACT.seat_law != 'off' (W3) */
+  int32_t seat_gap;       /* CONTACT.seat_gap */
+  int32_t dent_after;     /* resolved CONTACT.dent_after_iter (PolylineContact.dentAfterDefault when < 0) */
+  int32_t augmented;      /* CONTACT.augmented (AL) */
+  int32_t al_every;       /* CONTACT.al_every 600 */
+  int32_t floor_on;       /* scene.floor enabled */
+} ws_params;
+
+/* One instance as the host model holds it after goal setup (CpuSolver._buildGoals, solver_cpu.js:670-710).
+ * Topology arrays follow buildState (model.js:10-75); strands are contiguous node/segment/joint ranges. */
+typedef struct ws_model {
+  int32_t n, m, p, K, C, S;           /* nodes, segments, joints, crossings, clamp joints, strands */
+  int32_t n_point_local;              /* frame-local point loads (Loads.pointLocal) */
+  double l_ref;                       /* min(l̄ ∪ l0): torque residual scale (CpuSolver.lRef) */
+  ws_params par;
+  /* strands */
+  const int32_t* node_offset;         /* [S+1] */
+  const int32_t* seg_offset;          /* [S+1] */
+  const int32_t* joint_offset;        /* [S+1] */
+  const uint8_t* strand_closed;       /* [S] */
+  /* connectivity */
+  const int32_t* seg_i0;              /* [m] first node of each segment */
+  const int32_t* seg_i1;              /* [m] second node */
+  const int32_t* j_seg0;              /* [p] */
+  const int32_t* j_seg1;              /* [p] */
+  const int32_t* j_node;              /* [p] */
+  const double* l0;                   /* [m] rest lengths */
+  const double* j_len;                /* [p] Voronoi lengths l̄ */
+  /* rod (CosseratRod) */
+  const double* EA; const double* GA; const double* EIx; const double* EIy; const double* GJ;  /* [m] */
+  const double* omega0;               /* [3p] rest curvature Ω0 */
+  const int32_t* c_seg;               /* [C] clamp joints (setClamps) */
+  const int32_t* c_node;              /* [C] */
+  const double* c_len;                /* [C] */
+  const double* c_B;                  /* [3C] per-axis stiffness, 0 on a free axis */
+  const double* c_Q;                  /* [4C] frozen frames */
+  /* goals */
+  const double* mass;                 /* [n] node masses ms·Σk */
+  const double* krot_static;          /* [3m] static rotational stiffness (addRotStiffness) */
+  const double* load_const;           /* [3n] constant loads (Loads.const) */
+  const int32_t* point_local_node;    /* [n_point_local] */
+  const double* point_local_F;        /* [3 n_point_local] force in the node frame */
+  const uint8_t* node_free;           /* [3n] 1 = free DOF (Anchors.nodeFree) */
+  /* crossings (PolylineContact) */
+  const int32_t* k_strand;            /* [2K] strand of side A / B */
+  const int32_t* k_seg_off;           /* [2K] first segment of that strand */
+  const int32_t* k_nseg;              /* [2K] segments of that strand */
+  const double* k_wk; const double* k_hk;                    /* [2K] section width / height per side */
+  const double* k_fc90; const double* k_E90; const double* k_Hs;  /* [2K] material per side */
+  const double* k_w; const double* k_h;                      /* [K] mean width / height */
+  const uint8_t* k_enabled;           /* [K] */
+  const uint8_t* k_locked;            /* [K] */
+  /* floor (Floor goal): node sections */
+  const double* node_w; const double* node_h;                /* [n] (floor_on only) */
+} ws_model;
+
+/* Dynamic state and outputs of one instance: uploaded from / downloaded into caller buffers.
+ * Arrays marked (out) may be NULL on download; state arrays are read on upload and written on download. */
+typedef struct ws_state {
+  int32_t iteration;        /* st.iteration */
+  int32_t converged;        /* CpuSolver.converged */
+  int32_t pen_n;            /* PolylineContact.penN (AL window length) */
+  int32_t pad0;
+  double ke_prev;           /* CpuSolver.kePrev */
+  double residual;          /* st.residual (inf while warming up) */
+  double res_force, res_torque, fref;               /* residualParts and F_ref of the last iteration (out) */
+  double ke, e_stretch, e_shear, e_bend, e_twist, e_contact;  /* energies of the last iteration (out) */
+  double *X, *X0, *V;       /* [3n] */
+  double *Q, *Q0, *W;       /* [4m], [4m], [3m] */
+  double *u;                /* [2K] contact parameters (segment index + fraction) */
+  double *n_prev;           /* [3K] persistent contact normals */
+  double *slip;             /* [K] accumulated slip length */
+  uint8_t *stuck;           /* [K] */
+  double *lam;              /* [K] AL multipliers */
+  double *pen_acc;          /* [K] AL penetration window sums */
+  double *theta0, *rot_slip;/* [K] W2 state (theta0 NaN = unset) */
+  double *dent, *climb, *tanpsi; /* [2K] W3 state */
+  double *dent_e;           /* [K] W3 state */
+  double *floor_a;          /* [2n] floor stick points (x, y) */
+  /* outputs */
+  double *N, *Vsh;          /* [m], [2m] axial / shear forces (out) */
+  double *M;                /* [3p] joint moments (out) */
+  double *cM;               /* [2C] clamp moments (out) */
+  double *reactions;        /* [3n] (out) */
+  double *Fn, *Ft, *gap, *kstate;  /* [K] contact normal/friction force, gap, state 0 stick / 1 slip (out) */
+  double *theta, *Anom, *pressure, *util, *Mrot;  /* [K] W1 classification, W2 moment (out) */
+  double *phi;              /* [2K] (out) */
+  double *mid;              /* [3K] crossing mid points (out) */
+  int32_t *ctype, *face_side; /* [K] (out) */
+} ws_state;
+
+/* Per-instance control record, the reference's CT struct as readControl returns it (solver_gpu.js:374-386). */
+typedef struct ws_control {
+  int32_t iter, done, rollback, al_update, stop_at, hist_n, pad0, pad1;
+  double ke_prev, ke, resid, fref, e_stretch, e_bend, e_tors, e_cont, e_shear, res_t, res_f;
+} ws_control;
+
+typedef struct ws_batch_opts {
+  int32_t precision;       /* ws_precision */
+  int32_t per_submit;      /* iterations per captured graph (GpuSolver perSubmit, default 64) */
+  int32_t tile;            /* nodes per strand tile: 64 / 128 / 256; 0 = autotune */
+  int32_t history;         /* history ring length per instance (0 = 4096, the reference's HN) */
+} ws_batch_opts;
+
+typedef struct ws_ctx ws_ctx;
+typedef struct ws_batch ws_batch;
+typedef struct ws_server ws_server;
+
+/* ---- device and context ------------------------------------------------------------------------ */
+WS_API int ws_ctx_create(int32_t device, ws_ctx** out);      /* getDevice() (solver_gpu.js:17-29) */
+WS_API void ws_ctx_destroy(ws_ctx* ctx);
+WS_API const char* ws_last_error(const ws_ctx* ctx);         /* message of the last failed call */
+WS_API int ws_device_info(ws_ctx* ctx, char* name, int32_t cap, int64_t* hbm_bytes, int32_t* n_cu);
+
+/* ---- batch (GpuSolver) ---------------------------------------------------------------------------
+ * B instances, heterogeneous topologies allowed (the reference requires one topology per batch,
+ * solver_gpu.js:70; here each instance keeps its own sizes).  Clamp segments are fixed per instance
+ * for the life of the batch (updateParams throws otherwise, solver_gpu.js:325). */
+WS_API int ws_batch_create(ws_ctx* ctx, const ws_model* const* models, const ws_state* const* states,
+                           int32_t B, const ws_batch_opts* opts, ws_batch** out);  /* constructor :65-123 */
+WS_API int ws_batch_destroy(ws_batch* batch);                                       /* dispose() :522 */
+WS_API int ws_batch_size(ws_batch* batch, int32_t* B);
+WS_API int ws_upload(ws_batch* batch, int32_t b, const ws_model* model, const ws_state* state); /* upload(b) :191-198 */
+WS_API int ws_update_params(ws_batch* batch, int32_t b, const ws_model* model);    /* updateParams(b) :323-333 */
+WS_API int ws_step(ws_batch* batch, int32_t n_iter, const int32_t* instances, int32_t n_inst,
+                   int32_t force, ws_control* ctl_out);                             /* step(nIter, opts) :338-361 */
+WS_API int ws_read_control(ws_batch* batch, ws_control* out);                       /* readControl() :374-386 */
+WS_API int ws_read_history(ws_batch* batch, int32_t b, double* out, int32_t cap_rows, int32_t* n_rows); /* readHistory(b) :389-405 */
+WS_API int ws_download(ws_batch* batch, int32_t b, ws_state* out);                  /* download(b) :431-503 */
+WS_API int ws_internalize(ws_batch* batch, int32_t b);                              /* internalize(b) :507-520 */
+WS_API int ws_batch_timing(ws_batch* batch, double* ms_last_step, double* ms_kernel_cross, double* ms_kernel_strand,
+                           double* ms_kernel_ctrl);   /* hipEvent timing of the last step(); per-kernel means of the last profiled step */
+WS_API int ws_batch_bytes(ws_batch* batch, double* alg_bytes_per_iter, double* device_bytes);  /* algorithmic HBM bytes per iteration, device footprint */
+WS_API int ws_batch_profile(ws_batch* batch, int32_t on);  /* 1: step() launches without graphs and times each kernel with hipEvents */
+
+/* ---- serving (continuous batching; no counterpart in the reference) ------------------------------ */
+typedef struct ws_server_opts {
+  int32_t precision;
+  int32_t chunk;           /* iterations per relaxation chunk between admissions */
+  int32_t tile;
+  int32_t max_slots;       /* instances resident at once */
+  int64_t hbm_budget;      /* bytes of device state the server may hold */
+} ws_server_opts;
+
+typedef struct ws_result {
+  int64_t id;              /* request id returned by ws_submit */
+  int32_t status;          /* 0 converged, 1 max_iter reached, 2 cancelled, 3 non-finite state */
+  int32_t iterations;
+  double residual;
+  double t_submit_ms, t_admit_ms, t_done_ms;   /* server clock */
+  int32_t n; int32_t K;
+} ws_result;
+
+WS_API int ws_server_create(ws_ctx* ctx, const ws_server_opts* opts, ws_server** out);
+WS_API int ws_server_destroy(ws_server* srv);
+WS_API int ws_submit(ws_server* srv, const ws_model* model, const ws_state* state, int32_t max_iter, int64_t* id_out);
+WS_API int ws_cancel(ws_server* srv, int64_t id);
+WS_API int ws_pump(ws_server* srv, int32_t* n_active, int32_t* n_queued);   /* one chunk: admit, relax, retire */
+WS_API int ws_poll(ws_server* srv, ws_result* out, int32_t cap, int32_t* n_out);
+WS_API int ws_fetch(ws_server* srv, int64_t id, ws_state* out);             /* final state of a finished request */
+WS_API double ws_server_clock_ms(ws_server* srv);
+
+#endif

@@ -1,0 +1,107 @@
+/* weave_model.h — host-side model builder (C ABI), the caller side of the solver boundary.
+ *
+ * The reference builds every solver instance on the host before the GPU sees it:
+ *   Mesh.fromArrays / weld / edgeTopology        web/js/mesh.js:8-163
+ *   buildTopology (trace, parity, polylines,      web/js/topology.js:78-345
+ *                  frames)
+ *   buildState, Anchors, Loads, strandStiffness   web/js/model.js:10-252
+ *   CosseratRod setup, PolylineContact setup      web/js/rod_cosserat.js:49-111, web/js/solver_cpu.js:62-139
+ *   CpuSolver._buildGoals (masses, inertias)      web/js/solver_cpu.js:670-710
+ * This library restates that pipeline in C++ (f64) so the same instance description (ws_model + ws_state,
+ * weave.h) feeds the device solver, the CPU oracle and the N-API addon.  It also holds the synthetic
+ * scene generator of the benchmark workloads (hexagonal kagome domes, seeded).
+ */
+#ifndef WEAVE_MODEL_H
+#define WEAVE_MODEL_H
+#include "weave.h"
+
+/* lath group: MATERIAL, SECTION and BEAM of params.js:6-19 (one group; hybrid groups are not modelled) */
+typedef struct wsm_group {
+  double E, G, density, E90, fc90, H;      /* N/mm², kg/m³ (density is SI, as units.js) */
+  double width, height;                    /* section (mm) */
+  double Ix, Iy, J;                        /* <= 0: derived from the rectangle (params.js:39-41) */
+  int32_t beam_advanced;                   /* BEAM.mode == 'advanced' */
+  double activation;                       /* simple mode */
+  double stiffness;                        /* simple mode */
+  double act_flat, act_edge, act_twist;    /* advanced mode */
+  double k_axial, k_flat, k_edge, k_twist, k_shear;  /* advanced mode */
+} wsm_group;
+
+typedef struct wsm_scene {
+  wsm_group grp;
+  ws_params par;                           /* SOLVER / CONTACT / floor */
+  int32_t contacts;                        /* CpuSolver option {contacts} */
+  int32_t locked;                          /* CONTACT.locked: every crossing starts locked */
+  int32_t prestress_asbuilt;               /* SOLVER.prestress == 'asbuilt' */
+  /* WEAVE (params.js:24) */
+  int32_t subdivisions;
+  double overhang;
+  int32_t offset_half_height;              /* offset_mode == 'half_height' */
+  int32_t seed_strand_phase;
+  /* units (units.js): scene length → m, scene force → N */
+  double length_to_m, force_to_n;
+  /* loads (scene.js:40) */
+  int32_t gravity;
+  double gravity_dir[3];
+  int32_t udl;                             /* UDL per length */
+  double udl_q[3];
+} wsm_scene;
+
+enum wsm_selector { WSM_FRAME = 0, WSM_ALL_START = 1, WSM_ALL_END = 2, WSM_ALL_ENDS = 3, WSM_STRAND = 4, WSM_BOUNDARY = 5 };
+typedef struct wsm_anchor { int32_t type; int32_t s, k; int32_t fix[6]; } wsm_anchor;     /* model.js:115-189 */
+typedef struct wsm_point_load { int32_t s, k; int32_t local; int32_t pad; double F[3]; } wsm_point_load;
+
+/* synthetic preparation applied after buildState and before the goals: seeded out-of-plane noise */
+typedef struct wsm_prep {
+  double noise;                            /* ± amplitude on node z (scene length) */
+  uint32_t seed;                           /* xoshiro128** seed (rng.js) */
+  int32_t pad;
+} wsm_prep;
+
+/* hexagonal kagome dome of the benchmark workloads */
+typedef struct wsm_hex_spec {
+  double span;          /* developed width of the flat weave (scene length) */
+  double spacing;       /* strand spacing of the flat weave */
+  double segment;       /* target segment length */
+  double contraction;   /* plan contraction of the developed width: sets the rise of the spherical cap */
+  double noise;         /* ± out-of-plane noise on the initial nodes */
+  uint32_t seed;
+  int32_t pad;
+} wsm_hex_spec;
+
+typedef struct wsm_info {
+  int32_t n, m, p, K, C, S;
+  int32_t nV, nF, nE, subdivisions, rings, parity_conflicts;
+  double span_plan;     /* plan diameter of the built model (for the 1e-4·span tolerance) */
+  double rise;
+} wsm_info;
+
+typedef struct wsm_model wsm_model;
+
+WS_API void wsm_scene_defaults(wsm_scene* sc);            /* params.js defaults, mm·N */
+WS_API int wsm_from_mesh(const double* V, int32_t nV, const int32_t* F, int32_t nF, int32_t weld,
+                         const wsm_scene* sc, const wsm_anchor* anchors, int32_t n_anchors,
+                         const wsm_point_load* loads, int32_t n_loads, const int32_t* flip_knots, int32_t n_flips,
+                         const wsm_prep* prep, wsm_model** out);
+/* stand-alone laths (topologyFromPolylines, topology.js:348-363) with optional explicit crossings
+ * (knots[4c..4c+3] = sA, kA, sB, kB frame ids) as the test fixtures of web/tests/cases.mjs build them */
+WS_API int wsm_from_polylines(const double* P, const int32_t* counts, int32_t n_strands,
+                              const int32_t* knot_rows, const int32_t* knot_counts,
+                              const double* normals,          /* NULL = +z, else [3 * total nodes] */
+                              const int32_t* knots, int32_t n_knots,
+                              const wsm_scene* sc, const wsm_anchor* anchors, int32_t n_anchors,
+                              const wsm_point_load* loads, int32_t n_loads, wsm_model** out);
+WS_API int wsm_hex_dome(const wsm_hex_spec* spec, const wsm_scene* sc, wsm_model** out);
+WS_API int wsm_get(wsm_model* mdl, ws_model* model, ws_state* state);   /* pointers into the model's arrays */
+WS_API int wsm_info_get(wsm_model* mdl, wsm_info* info);
+WS_API int wsm_add_load(wsm_model* mdl, int32_t node, const double* F);   /* Loads.const[node] += F (tests) */
+WS_API int wsm_set_strength(wsm_model* mdl, double strength, double stick_strength); /* re-derive masses (rebuildGoals) */
+WS_API int wsm_clone(wsm_model* mdl, wsm_model** out);
+WS_API int wsm_mesh_counts(wsm_model* mdl, int32_t* nV, int32_t* nF);
+WS_API const char* wsm_last_error(void);
+/* sizeof of the ABI structs, for binding checks: ws_params, ws_model, ws_state, ws_control, ws_batch_opts,
+ * ws_server_opts, ws_result, wsm_scene, wsm_anchor, wsm_point_load, wsm_prep, wsm_hex_spec, wsm_info */
+WS_API int32_t wsm_abi_sizes(int64_t* out, int32_t cap);
+WS_API void wsm_free(wsm_model* mdl);
+
+#endif

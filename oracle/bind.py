@@ -1,0 +1,116 @@
+"""ctypes binding of the f64 oracle (oracle/build/liboracle.so) — test infrastructure.
+
+Only tests, the smoke check and the benchmark's CPU-baseline leg use this module.
+"""
+import ctypes as C
+import importlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+A = importlib.import_module("weave-sim_amd._abi")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "build", "liboracle.so")
+        if not os.path.exists(path):
+            raise OSError(f"{path} is not built (make -C oracle)")
+        L = C.CDLL(path)
+        L.oracle_create.argtypes = [C.POINTER(A.ws_model), C.POINTER(A.ws_state), C.c_int32]
+        L.oracle_create.restype = C.c_void_p
+        L.oracle_free.argtypes = [C.c_void_p]
+        L.oracle_free.restype = None
+        for f in ("oracle_step",):
+            getattr(L, f).argtypes = [C.c_void_p, C.c_int32]
+        for f in ("oracle_converged", "oracle_iteration"):
+            getattr(L, f).argtypes = [C.c_void_p]
+        for f in ("oracle_residual", "oracle_ke"):
+            getattr(L, f).argtypes = [C.c_void_p]
+            getattr(L, f).restype = C.c_double
+        L.oracle_get.argtypes = [C.c_void_p, C.POINTER(A.ws_state)]
+        L.oracle_get.restype = None
+        L.oracle_inertia.argtypes = [C.c_void_p, A.P_d]
+        L.oracle_inertia.restype = None
+        L.oracle_history.argtypes = [C.c_void_p, A.P_d, A.P_d, A.P_d, C.c_int32]
+        L.oracle_rod_apply.argtypes = [C.c_void_p, A.P_d, A.P_d, A.P_d, A.P_d]
+        L.oracle_rod_apply.restype = C.c_double
+        L.oracle_pairwise_sum.argtypes = [A.P_d, C.c_int64]
+        L.oracle_pairwise_sum.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+class Oracle:
+    """CpuSolver restated in f64 on a copy of a Model's description and state."""
+
+    def __init__(self, model, history=False, state=None):
+        self.model = model
+        self.h = lib().oracle_create(C.byref(model.md), C.byref(state.st if state is not None else model.st),
+                                     1 if history else 0)
+
+    def __del__(self):
+        try:
+            lib().oracle_free(self.h)
+        except Exception:
+            pass
+
+    def step(self, n):
+        return lib().oracle_step(self.h, int(n))
+
+    def run(self, max_iter, chunk=1000):
+        done = 0
+        while done < max_iter and not self.converged:
+            done += self.step(min(chunk, max_iter - done))
+        return done
+
+    @property
+    def converged(self): return bool(lib().oracle_converged(self.h))
+
+    @property
+    def iteration(self): return lib().oracle_iteration(self.h)
+
+    @property
+    def residual(self): return lib().oracle_residual(self.h)
+
+    @property
+    def ke(self): return lib().oracle_ke(self.h)
+
+    def state(self):
+        buf = self.model.buffers()
+        lib().oracle_get(self.h, C.byref(buf.st))
+        return buf
+
+    def inertia(self):
+        out = np.zeros(3 * self.model.m)
+        lib().oracle_inertia(self.h, out.ctypes.data_as(A.P_d))
+        return out
+
+    def history(self, cap=10 ** 7):
+        n = min(cap, self.iteration)
+        ke, strain, resid = np.zeros(n), np.zeros(n), np.zeros(n)
+        k = lib().oracle_history(self.h, ke.ctypes.data_as(A.P_d), strain.ctypes.data_as(A.P_d),
+                                 resid.ctypes.data_as(A.P_d), n)
+        return ke[:k], strain[:k], resid[:k]
+
+    def rod_apply(self, X=None, Q=None):
+        m = self.model
+        F, T = np.zeros(3 * m.n), np.zeros(3 * m.m)
+        Xp = np.ascontiguousarray(X, dtype=np.float64) if X is not None else None
+        Qp = np.ascontiguousarray(Q, dtype=np.float64) if Q is not None else None
+        e = lib().oracle_rod_apply(self.h, Xp.ctypes.data_as(A.P_d) if Xp is not None else None,
+                                   Qp.ctypes.data_as(A.P_d) if Qp is not None else None,
+                                   F.ctypes.data_as(A.P_d), T.ctypes.data_as(A.P_d))
+        return e, F, T
+
+
+def pairwise_sum(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return lib().oracle_pairwise_sum(a.ctypes.data_as(A.P_d), len(a))

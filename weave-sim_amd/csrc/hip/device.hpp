@@ -1,0 +1,110 @@
+// Device-side layout of the MI355X dynamic-relaxation solver (shared by the kernels and the host ABI code).
+//
+// One instance = one gridshell.  Its arrays live in one device arena, structure-of-arrays:
+//   nodes     x, y, z (f64, double-buffered: [cur] is the state, [cur^1] receives the next iterate or keeps the
+//             previous one for a kinetic-damping roll-back), vx, vy, vz (Real), mass, load (Real), fix bits (u8)
+//   segments  qw, qx, qy, qz (f64, double-buffered), wx, wy, wz (Real), l0 (f64), EA, GA, EIx, EIy, GJ, krot (Real)
+//   joints    jlen, Ω0 (Real)
+//   crossings u (f64 ×2), slip (f64), nprev (Real ×3), flags (u8), contact outputs, side records
+// Real is float in the FP32+FP64 form and double in the plain FP64 form.
+#pragma once
+#include <stdint.h>
+
+namespace wsd {
+
+constexpr int HIST_STRIDE = 9;   // history ring entry: ke, strain, bending, torsion, stretch, shear, contact, residual, iteration
+constexpr int SPART = 12;        // doubles per strand-tile partial
+constexpr int CPART = 4;         // doubles per crossing-block partial
+constexpr int CROSS_BLOCK = 64;  // crossings per crossing-kernel block
+constexpr int MAX_REL = 1024;    // crossing sides a strand tile may gather (checked on the host per strand)
+
+// instance control record, device-written (the reference's CT struct, wgsl.js:154-172)
+struct Ctl {
+  int32_t iter, done, rollback, al_update;
+  int32_t stop_at, hist_n, cur, counter;
+  int32_t pen_n, live_flag, pad1, pad2;
+  double ke_prev, ke, resid, fref;
+  double e_stretch, e_bend, e_tors, e_cont, e_shear, res_t, res_f;
+};
+
+// one crossing side's contribution of this iteration, written by the crossing kernel, gathered per strand tile
+template <typename Real>
+struct SideRec {
+  int32_t seg_pre;      // segment (strand-local) the force acts on: the side's segment before this iteration's slip
+  int32_t seg_post;     // segment after the slip: receives the righting stiffness
+  Real t;               // position on seg_pre (weights 1 − t, t)
+  Real kth;             // contact righting stiffness
+  Real F[3];            // force on the side's centreline point (world)
+  Real T[3];            // lever + rotational-friction torque (world)
+};
+
+template <typename Real>
+struct Inst {
+  // sizes
+  int32_t n, m, p, K, C, S, npl, nclamp_entries;
+  // work-list ranges of this instance
+  int32_t tile0, ntiles, cblk0, ncblk;
+  // scalar parameters (ws_params)
+  double mass_scale, damping, tol, mu, strength, stick, torsion_factor, epsilon, slip_rate, rot_factor, dent_rate;
+  double edge_thr, l_ref, jlen_min;
+  int32_t kin_damp, warmup, contacts, ramp_iter, rot_fric, seat, seat_gap, dent_after, aug, al_every, hist_len, pad;
+  // topology
+  const int32_t* node_off;   // [S+1]
+  const int32_t* seg_off;    // [S+1]
+  const int32_t* joint_off;  // [S+1]
+  const uint8_t* closed;     // [S]
+  const int32_t* seg_i0;     // [m]
+  const int32_t* seg_i1;     // [m]
+  const int32_t* nsa;        // [n] segment starting at the node, -1 if none
+  const int32_t* nsb;        // [n] segment ending at the node, -1 if none
+  const int32_t* side_off;   // [S+1] crossing sides per strand (CSR over the strand-ordered side records)
+  const int32_t* side_pos;   // [2K] slot of side 2c + s in the strand-ordered side records
+  const int32_t* clamp_off;  // [m+1] clamp joints per segment (CSR), null when C == 0
+  const int32_t* clamp_idx;  // [C]
+  // nodes
+  double* x[2]; double* y[2]; double* z[2];
+  Real* vx; Real* vy; Real* vz;
+  const Real* mass;
+  const Real* lx; const Real* ly; const Real* lz;            // constant loads
+  const Real* llx; const Real* lly; const Real* llz;         // frame-local point loads (null when npl == 0)
+  const uint8_t* fix;        // bit0 x, bit1 y, bit2 z fixed; bit3 any fixed (reaction); bit4 all three fixed
+  // segments
+  double* qw[2]; double* qx[2]; double* qy[2]; double* qz[2];
+  Real* wx; Real* wy; Real* wz;
+  const double* l0;
+  const Real* EA; const Real* GA; const Real* EIx; const Real* EIy; const Real* GJ;
+  const Real* krx; const Real* kry; const Real* krz;
+  // joints
+  const Real* jlen; const Real* om0;                          // om0 [3p]
+  // clamps
+  const Real* c_len; const Real* c_B; const double* c_Q;      // [C], [3C], [4C]
+  // crossings (static)
+  const int32_t* k_seg_off; const int32_t* k_nseg;            // [2K]
+  const int32_t* k_noff; const int32_t* k_nn;                 // [2K] first node and node count (bit 30: closed) of the side's strand
+  const Real* wk; const Real* hk;                             // [2K]
+  const Real* fc90; const Real* E90; const Real* Hs;          // [2K]
+  const Real* kw; const Real* kh;                             // [K]
+  const uint8_t* k_en; uint8_t* k_lock;                       // [K]
+  // crossings (state)
+  double* u;                 // [2K]
+  double* slip;              // [K]
+  Real* nprev;               // [3K]
+  uint8_t* kflags;           // [K] bit0 slip, bit1 stuck
+  Real* fn; Real* ft; Real* gap;                              // [K]
+  Real* lam; Real* penacc; Real* theta0; Real* rotslip; Real* mrot;   // [K]
+  Real* dent; Real* climb; Real* tanpsi; Real* dente;         // [2K], [2K], [2K], [K]
+  // crossing classification outputs (W1)
+  Real* theta; Real* anom; Real* pres; Real* util; Real* phi; Real* mid;   // [K], [K], [K], [K], [2K], [3K]
+  int8_t* ctype; int8_t* face;                                // [K]
+  SideRec<Real>* side;       // [2K] in strand order (side_off / side_pos)
+  // control
+  Ctl* ctl;
+  double* hist;              // [hist_len * HIST_STRIDE]
+  double* spart;             // [ntiles * SPART]
+  double* cpart;             // [ncblk * CPART]
+};
+
+struct TileItem { int32_t inst, strand, k0, k1; };   // strand-local node range [k0, k1)
+struct CrossItem { int32_t inst, c0, count, blk; };  // crossings [c0, c0 + count) of one instance, block index in the instance
+
+}  // namespace wsd

@@ -1,0 +1,217 @@
+"""Host models: scenes built by the C++ model builder (libweave_model.so), exposed as ws_model / ws_state.
+
+A `Model` plays the role of the reference's host `CpuSolver` object that `GpuSolver` uploads from
+(web/js/solver_gpu.js:62-123): it owns the f64 topology, goals and state; the device solver copies them in and
+downloads back into a `StateBuffers` (or into the model's own state arrays, as `download` overwrites the
+CpuSolver's typed arrays, solver_gpu.js:431-503).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+
+
+def _lengths(md):
+    return {"n": md.n, "3n": 3 * md.n, "2n": 2 * md.n, "m": md.m, "2m": 2 * md.m, "3m": 3 * md.m, "4m": 4 * md.m,
+            "3p": 3 * md.p, "K": md.K, "2K": 2 * md.K, "3K": 3 * md.K, "2C": 2 * md.C}
+
+
+def _view(ptr, count, dtype):
+    if count == 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(count,))
+
+
+class Scene:
+    """Scene parameters (params.js defaults, mm·N); attributes map onto wsm_scene / ws_params."""
+
+    def __init__(self, **kw):
+        self.c = A.wsm_scene()
+        A.model_lib().wsm_scene_defaults(C.byref(self.c))
+        for k, v in kw.items():
+            self.set(k, v)
+
+    def set(self, key, value):
+        for obj in (self.c.par, self.c.grp, self.c):
+            if hasattr(obj, key) and key not in ("par", "grp"):
+                if key in ("gravity_dir", "udl_q"):
+                    getattr(obj, key)[:] = list(value)
+                else:
+                    setattr(obj, key, value)
+                return self
+        raise KeyError(key)
+
+    def get(self, key):
+        for obj in (self.c.par, self.c.grp, self.c):
+            if hasattr(obj, key):
+                return getattr(obj, key)
+        raise KeyError(key)
+
+
+class StateBuffers:
+    """Caller-owned ws_state buffers (numpy) of one instance, the target of a download."""
+
+    def __init__(self, md):
+        self.st = A.ws_state()
+        self.arrays = {}
+        L = _lengths(md)
+        for name, ptype, key in A.STATE_ARRAYS:
+            dt = {A.P_d: np.float64, A.P_u8: np.uint8, A.P_i32: np.int32}[ptype]
+            arr = np.zeros(L[key], dtype=dt)
+            self.arrays[name] = arr
+            setattr(self.st, name, arr.ctypes.data_as(ptype) if arr.size else ptype())
+
+    def __getitem__(self, k):
+        return self.arrays[k]
+
+    def copy_from(self, other):
+        """copy the arrays and scalars of another StateBuffers / Model state"""
+        for name in self.arrays:
+            self.arrays[name][:] = other[name]
+        for f, _ in A.ws_state._fields_[:15]:
+            setattr(self.st, f, getattr(other.st, f))
+
+
+class Model:
+    """One instance built on the host (the reference's CpuSolver after _buildGoals)."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        self.md = A.ws_model()
+        self.st = A.ws_state()
+        A.model_lib().wsm_get(self.h, C.byref(self.md), C.byref(self.st))
+        self.info = A.wsm_info()
+        A.model_lib().wsm_info_get(self.h, C.byref(self.info))
+        self._L = _lengths(self.md)
+
+    def __del__(self):
+        try:
+            if self.h:
+                A.model_lib().wsm_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ construction
+    @staticmethod
+    def _check(rc, out):
+        if rc != 0:
+            raise ValueError(A.model_lib().wsm_last_error().decode())
+        return Model(out.value)
+
+    @classmethod
+    def hex_dome(cls, span, spacing, segment, contraction=0.15, noise=1.0, seed=1, scene=None):
+        """Kagome weave of a hexagonal patch raised onto a spherical cap (the benchmark scenes)."""
+        sc = scene or Scene()
+        spec = A.wsm_hex_spec(span, spacing, segment, contraction, noise, seed, 0)
+        out = C.c_void_p()
+        return cls._check(A.model_lib().wsm_hex_dome(C.byref(spec), C.byref(sc.c), C.byref(out)), out)
+
+    @classmethod
+    def from_mesh(cls, V, F, scene=None, anchors=(), loads=(), flips=(), noise=0.0, seed=1, weld=True):
+        sc = scene or Scene()
+        V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1)
+        F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1)
+        an = (A.wsm_anchor * max(len(anchors), 1))(*[_anchor(a) for a in anchors])
+        ld = (A.wsm_point_load * max(len(loads), 1))(*[_load(x) for x in loads])
+        fl = np.ascontiguousarray(np.array(flips, dtype=np.int32).reshape(-1))
+        prep = A.wsm_prep(noise, seed, 0)
+        out = C.c_void_p()
+        rc = A.model_lib().wsm_from_mesh(V.ctypes.data_as(A.P_d), len(V) // 3, F.ctypes.data_as(A.P_i32), len(F) // 3,
+                                         1 if weld else 0, C.byref(sc.c), an, len(anchors), ld, len(loads),
+                                         fl.ctypes.data_as(A.P_i32), len(fl) // 2, C.byref(prep), C.byref(out))
+        return cls._check(rc, out)
+
+    @classmethod
+    def from_polylines(cls, polylines, scene=None, knot_rows=None, normals=None, knots=(), anchors=(), loads=()):
+        """Stand-alone laths (topologyFromPolylines) with explicit crossings knots = [(sA, kA, sB, kB), ...]."""
+        sc = scene or Scene()
+        P = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.float64).reshape(-1) for p in polylines]))
+        counts = np.array([len(np.asarray(p).reshape(-1)) // 3 for p in polylines], dtype=np.int32)
+        if knot_rows is None:
+            knot_rows = [[0, c - 1] for c in counts]
+        kr = np.array([r for rows in knot_rows for r in rows], dtype=np.int32)
+        kc = np.array([len(rows) for rows in knot_rows], dtype=np.int32)
+        nrm = None
+        if normals is not None:
+            nrm = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in normals]))
+        kn = np.array(knots, dtype=np.int32).reshape(-1)
+        an = (A.wsm_anchor * max(len(anchors), 1))(*[_anchor(a) for a in anchors])
+        ld = (A.wsm_point_load * max(len(loads), 1))(*[_load(x) for x in loads])
+        out = C.c_void_p()
+        rc = A.model_lib().wsm_from_polylines(
+            P.ctypes.data_as(A.P_d), counts.ctypes.data_as(A.P_i32), len(counts), kr.ctypes.data_as(A.P_i32),
+            kc.ctypes.data_as(A.P_i32), nrm.ctypes.data_as(A.P_d) if nrm is not None else None,
+            kn.ctypes.data_as(A.P_i32), len(kn) // 4, C.byref(sc.c), an, len(anchors), ld, len(loads), C.byref(out))
+        m = cls._check(rc, out)
+        m._keep = (P, counts, kr, kc, nrm, kn)
+        return m
+
+    def clone(self):
+        out = C.c_void_p()
+        A.model_lib().wsm_clone(self.h, C.byref(out))
+        return Model(out.value)
+
+    # ------------------------------------------------------------------ access
+    @property
+    def n(self): return self.md.n
+
+    @property
+    def m(self): return self.md.m
+
+    @property
+    def p(self): return self.md.p
+
+    @property
+    def K(self): return self.md.K
+
+    def __getitem__(self, name):
+        """numpy view of a state array (writes go to the model)"""
+        for nm, ptype, key in A.STATE_ARRAYS:
+            if nm == name:
+                dt = {A.P_d: np.float64, A.P_u8: np.uint8, A.P_i32: np.int32}[ptype]
+                return _view(getattr(self.st, nm), self._L[key], dt)
+        raise KeyError(name)
+
+    def arr(self, name, count=None):
+        """numpy view of a model (ws_model) array"""
+        sizes = {"node_offset": self.md.S + 1, "seg_offset": self.md.S + 1, "joint_offset": self.md.S + 1,
+                 "strand_closed": self.md.S, "seg_i0": self.m, "seg_i1": self.m, "j_seg0": self.p, "j_seg1": self.p,
+                 "j_node": self.p, "l0": self.m, "j_len": self.p, "EA": self.m, "GA": self.m, "EIx": self.m,
+                 "EIy": self.m, "GJ": self.m, "omega0": 3 * self.p, "mass": self.n, "krot_static": 3 * self.m,
+                 "load_const": 3 * self.n, "node_free": 3 * self.n, "k_strand": 2 * self.K, "k_seg_off": 2 * self.K,
+                 "k_nseg": 2 * self.K, "k_wk": 2 * self.K, "k_hk": 2 * self.K, "k_w": self.K, "k_h": self.K,
+                 "k_enabled": self.K, "k_locked": self.K, "c_seg": self.md.C, "c_len": self.md.C, "c_B": 3 * self.md.C,
+                 "c_Q": 4 * self.md.C, "k_fc90": 2 * self.K, "k_E90": 2 * self.K, "k_Hs": 2 * self.K}
+        ptr = getattr(self.md, name)
+        t = type(ptr)
+        dt = {A.P_d: np.float64, A.P_u8: np.uint8, A.P_i32: np.int32}[t]
+        return _view(ptr, count if count is not None else sizes[name], dt)
+
+    def add_load(self, node, F):
+        f = (C.c_double * 3)(*F)
+        A.model_lib().wsm_add_load(self.h, int(node), f)
+
+    def buffers(self):
+        return StateBuffers(self.md)
+
+    @property
+    def span(self):
+        return self.info.span_plan
+
+
+def _anchor(a):
+    """(type, s, k, fix[6]) or dict(type=..., s=, k=, fix=)"""
+    if isinstance(a, dict):
+        t, s, k, fix = a["type"], a.get("s", 0), a.get("k", 0), a["fix"]
+    else:
+        t, s, k, fix = a
+    return A.wsm_anchor(t, s, k, (C.c_int32 * 6)(*[1 if f else 0 for f in fix]))
+
+
+def _load(x):
+    """(s, k, F[3], local)"""
+    s, k, F = x[0], x[1], x[2]
+    local = x[3] if len(x) > 3 else 0
+    return A.wsm_point_load(s, k, 1 if local else 0, 0, (C.c_double * 3)(*F))

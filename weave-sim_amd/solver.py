@@ -1,0 +1,144 @@
+"""The device solver behind the C ABI: `GpuSolver` mirrors the reference's batched solver class
+(web/js/solver_gpu.js:62-523) method for method; `Server` is the continuous-batching request loop.
+
+Every call goes to libweave_hip.so (hand-written HIP for gfx950).  There is no CPU fallback: constructing a
+solver raises when the library is missing or no GPU is visible.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+from .model import StateBuffers
+
+PRECISIONS = {"fp32": A.WS_FP32_FP64, "fp32_fp64": A.WS_FP32_FP64, "fp64": A.WS_FP64}
+
+
+class DeviceError(RuntimeError):
+    pass
+
+
+class Context:
+    """getDevice() (solver_gpu.js:17-29): one HIP device."""
+
+    def __init__(self, device=0):
+        self.lib = A.hip_lib()
+        self.h = C.c_void_p()
+        rc = self.lib.ws_ctx_create(device, C.byref(self.h))
+        if rc != 0:
+            raise DeviceError(f"ws_ctx_create failed ({rc}): no HIP device")
+
+    def check(self, rc):
+        if rc != 0:
+            raise DeviceError(f"status {rc}: {self.lib.ws_last_error(self.h).decode()}")
+
+    def info(self):
+        name = C.create_string_buffer(256)
+        hbm, ncu = C.c_int64(), C.c_int32()
+        self.check(self.lib.ws_device_info(self.h, name, 256, C.byref(hbm), C.byref(ncu)))
+        return {"name": name.value.decode(), "hbm_bytes": hbm.value, "cu": ncu.value}
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.lib.ws_ctx_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _ctl_dict(c):
+    return {f: getattr(c, f) for f, _ in A.ws_control._fields_ if not f.startswith("pad")}
+
+
+class GpuSolver:
+    """B instances (each its own topology allowed) relaxed in the same launches."""
+
+    def __init__(self, models, precision="fp32", tile=0, per_submit=64, history=4096, ctx=None, states=None):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.models = list(models)
+        self.B = len(self.models)
+        self.precision = precision
+        mp = (C.POINTER(A.ws_model) * self.B)(*[C.pointer(m.md) for m in self.models])
+        sts = states or [m for m in self.models]
+        sp = (C.POINTER(A.ws_state) * self.B)(*[C.pointer(s.st) for s in sts])
+        opts = A.ws_batch_opts(PRECISIONS[precision], per_submit, tile, history)
+        self.h = C.c_void_p()
+        self.ctx.check(self.lib.ws_batch_create(self.ctx.h, mp, sp, self.B, C.byref(opts), C.byref(self.h)))
+        self._ctl = (A.ws_control * self.B)()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ws_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n_iter, instances=None, force=True):
+        """run up to n_iter iterations on the selected instances; returns the control records"""
+        if instances is not None:
+            sel = (C.c_int32 * len(instances))(*instances)
+            ns = len(instances)
+        else:
+            sel, ns = None, 0
+        self.ctx.check(self.lib.ws_step(self.h, int(n_iter), sel, ns, 1 if force else 0, self._ctl))
+        return [_ctl_dict(c) for c in self._ctl]
+
+    def read_control(self):
+        self.ctx.check(self.lib.ws_read_control(self.h, self._ctl))
+        return [_ctl_dict(c) for c in self._ctl]
+
+    def download(self, b, into=None):
+        """state of instance b into `into` (StateBuffers) or into the model's own arrays (GpuSolver.download)"""
+        tgt = into if into is not None else self.models[b]
+        self.ctx.check(self.lib.ws_download(self.h, b, C.byref(tgt.st)))
+        return tgt
+
+    def state(self, b):
+        buf = StateBuffers(self.models[b].md)
+        return self.download(b, buf)
+
+    def upload(self, b, model=None, state=None):
+        m = model or self.models[b]
+        st = state or m
+        self.ctx.check(self.lib.ws_upload(self.h, b, C.byref(m.md), C.byref(st.st)))
+
+    def update_params(self, b):
+        self.ctx.check(self.lib.ws_update_params(self.h, b, C.byref(self.models[b].md)))
+
+    def internalize(self, b):
+        self.ctx.check(self.lib.ws_internalize(self.h, b))
+
+    def read_history(self, b, cap=4096):
+        out = np.zeros((cap, 9))
+        n = C.c_int32()
+        self.ctx.check(self.lib.ws_read_history(self.h, b, out.ctypes.data_as(A.P_d), cap, C.byref(n)))
+        return out[:n.value]
+
+    def set_profile(self, on):
+        self.ctx.check(self.lib.ws_batch_profile(self.h, 1 if on else 0))
+
+    def timing(self):
+        a, b, c, e = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        self.ctx.check(self.lib.ws_batch_timing(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(e)))
+        return {"ms_last_step": a.value, "ms_cross": b.value, "ms_strand": c.value, "ms_ctrl": e.value}
+
+    def bytes(self):
+        a, b = C.c_double(), C.c_double()
+        self.ctx.check(self.lib.ws_batch_bytes(self.h, C.byref(a), C.byref(b)))
+        return {"alg_bytes_per_iter": a.value, "device_bytes": b.value}
