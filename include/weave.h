@@ -200,7 +200,10 @@ WS_API int ws_internalize(ws_batch* batch, int32_t b);                          
 WS_API int ws_batch_timing(ws_batch* batch, double* ms_last_step, double* ms_kernel_cross, double* ms_kernel_strand,
                            double* ms_kernel_ctrl);   /* hipEvent timing of the last step(); per-kernel means of the last profiled step */
 WS_API int ws_batch_bytes(ws_batch* batch, double* alg_bytes_per_iter, double* device_bytes);  /* algorithmic HBM bytes per iteration, device footprint */
-WS_API int ws_batch_profile(ws_batch* batch, int32_t on);  /* 1: step() launches without graphs and times each kernel with hipEvents */
+WS_API int ws_batch_profile(ws_batch* batch, int32_t on);
+/* mean duration of one launch of each kernel (crossing, strand, control) from `reps` back-to-back launches of that
+ * kernel alone, bracketed by two hipEvents on the batch's stream; for timing only: it advances the state */
+WS_API int ws_batch_kernel_bench(ws_batch* batch, int32_t reps, double* ms_cross, double* ms_strand, double* ms_ctrl);  /* 1: step() launches without graphs and times each kernel with hipEvents */
 
 /* ---- serving (continuous batching; no counterpart in the reference) ------------------------------ */
 typedef struct ws_server_opts {

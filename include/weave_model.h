@@ -51,9 +51,12 @@ enum wsm_selector { WSM_FRAME = 0, WSM_ALL_START = 1, WSM_ALL_END = 2, WSM_ALL_E
 typedef struct wsm_anchor { int32_t type; int32_t s, k; int32_t fix[6]; } wsm_anchor;     /* model.js:115-189 */
 typedef struct wsm_point_load { int32_t s, k; int32_t local; int32_t pad; double F[3]; } wsm_point_load;
 
-/* synthetic preparation applied after buildState and before the goals: seeded out-of-plane noise */
+/* synthetic preparation applied after buildState and before the goals: seeded out-of-plane noise, a smooth field
+ * z += noise · Σ w_k sin(2π (x cos θ_k + y sin θ_k) / λ_k + φ_k) / Σ w_k of 8 plane waves with λ_k in
+ * [wavelength, 4·wavelength] (bounded by ±noise; drawn from xoshiro128**, rng.js) */
 typedef struct wsm_prep {
   double noise;                            /* ± amplitude on node z (scene length) */
+  double wavelength;                       /* shortest wavelength of the field (<= 0: white noise per node) */
   uint32_t seed;                           /* xoshiro128** seed (rng.js) */
   int32_t pad;
 } wsm_prep;

@@ -101,7 +101,7 @@ class wsm_point_load(C.Structure):
 
 
 class wsm_prep(C.Structure):
-    _fields_ = [("noise", d), ("seed", u32), ("pad", i32)]
+    _fields_ = [("noise", d), ("wavelength", d), ("seed", u32), ("pad", i32)]
 
 
 class wsm_hex_spec(C.Structure):
@@ -178,6 +178,7 @@ def hip_lib():
         L.ws_batch_timing.argtypes = [vp, P_d, P_d, P_d, P_d]
         L.ws_batch_bytes.argtypes = [vp, P_d, P_d]
         L.ws_batch_profile.argtypes = [vp, i32]
+        L.ws_batch_kernel_bench.argtypes = [vp, i32, P_d, P_d, P_d]
         L.ws_server_create.argtypes = [vp, C.POINTER(ws_server_opts), C.POINTER(vp)]
         L.ws_server_destroy.argtypes = [vp]
         L.ws_submit.argtypes = [vp, C.POINTER(ws_model), C.POINTER(ws_state), i32, C.POINTER(i64)]
@@ -195,7 +196,7 @@ def hip_lib():
 WEAVE_H_SYMBOLS = [
     "ws_ctx_create", "ws_ctx_destroy", "ws_last_error", "ws_device_info", "ws_batch_create", "ws_batch_destroy",
     "ws_batch_size", "ws_upload", "ws_update_params", "ws_step", "ws_read_control", "ws_read_history", "ws_download",
-    "ws_internalize", "ws_batch_timing", "ws_batch_bytes", "ws_batch_profile", "ws_server_create", "ws_server_destroy", "ws_submit",
+    "ws_internalize", "ws_batch_timing", "ws_batch_bytes", "ws_batch_profile", "ws_batch_kernel_bench", "ws_server_create", "ws_server_destroy", "ws_submit",
     "ws_cancel", "ws_pump", "ws_poll", "ws_fetch", "ws_server_clock_ms",
 ]
 WEAVE_MODEL_H_SYMBOLS = [

@@ -29,8 +29,12 @@ constexpr double TWIST_RATE_MAX = 0.5 * SUPPORT_SMOOTH;
 constexpr double ROLL_DAMP = 0.5;
 constexpr double AL_FREE_MIN = 0.25;
 
-template <typename R> __device__ __forceinline__ R dmax(R a, R b) { return a > b ? a : b; }
-template <typename R> __device__ __forceinline__ R dmin(R a, R b) { return a < b ? a : b; }
+template <typename A, typename B> __device__ __forceinline__ typename std::common_type<A, B>::type dmax(A a, B b) {
+  typedef typename std::common_type<A, B>::type T; return T(a) > T(b) ? T(a) : T(b);
+}
+template <typename A, typename B> __device__ __forceinline__ typename std::common_type<A, B>::type dmin(A a, B b) {
+  typedef typename std::common_type<A, B>::type T; return T(a) < T(b) ? T(a) : T(b);
+}
 template <typename R> __device__ __forceinline__ R clampv(R v, R lo, R hi) { return dmin(dmax(v, lo), hi); }
 
 __device__ __forceinline__ float fsqrt_(float x) { return sqrtf(x); }
@@ -149,9 +153,9 @@ struct SideEv {
 // i-1 .. i+2 and segments i-1 .. i+1 (modulo the strand for closed laths).  All loads issue together once u is known.
 // Node tangents: normalised sum of the adjacent unit chords, the starting segment first (_prepare); node frames:
 // hemisphere-consistent mean of the adjacent segment quaternions referenced to the starting segment (nodeQuats).
-template <typename Real>
+template <typename Real, typename G>
 __device__ __forceinline__ void eval_side(const Inst<Real>& I, int cur, int c, int side, double uu, double ox, double oy, double oz,
-                                          SideEv<Real>& E) {
+                                          SideEv<G>& E) {
   const int ns = I.k_nseg[2 * c + side];
   const int noff = I.k_noff[2 * c + side];
   const int nnr = I.k_nn[2 * c + side];
@@ -160,7 +164,7 @@ __device__ __forceinline__ void eval_side(const Inst<Real>& I, int cur, int c, i
   const int soff = I.k_seg_off[2 * c + side];
   int i = int(fmin(fmax(floor(uu), 0.0), double(max(ns - 1, 0))));
   double td = fmin(fmax(uu - i, 0.0), 1.0);
-  Real t = Real(td);
+  G t = G(td);
   E.loc = i;
   E.seg = soff + i;
   const int ia = i, ib = closed ? (i + 1) % nn : i + 1;
@@ -179,33 +183,33 @@ __device__ __forceinline__ void eval_side(const Inst<Real>& I, int cur, int c, i
   if (hasPrevSeg) { xm = X[noff + iam]; ym = Y[noff + iam]; zm = Z[noff + iam]; }
   if (hasNextSeg) { xp = X[noff + ibp]; yp = Y[noff + ibp]; zp = Z[noff + ibp]; }
   const double* QW = I.qw[cur]; const double* QX = I.qx[cur]; const double* QY = I.qy[cur]; const double* QZ = I.qz[cur];
-  Real q0[4] = {Real(QW[E.seg]), Real(QX[E.seg]), Real(QY[E.seg]), Real(QZ[E.seg])};
-  Real qm[4] = {0, 0, 0, 0}, qp[4] = {0, 0, 0, 0};
-  if (hasPrevSeg) { int j = soff + sm; qm[0] = Real(QW[j]); qm[1] = Real(QX[j]); qm[2] = Real(QY[j]); qm[3] = Real(QZ[j]); }
-  if (hasNextSeg) { int j = soff + sp; qp[0] = Real(QW[j]); qp[1] = Real(QX[j]); qp[2] = Real(QY[j]); qp[3] = Real(QZ[j]); }
-  auto unitAdd = [](Real* T, double dx, double dy, double dz) {
-    Real ex = Real(dx), ey = Real(dy), ez = Real(dz);
-    Real l = dmax(fsqrt_(ex * ex + ey * ey + ez * ez), Real(1e-12));
+  G q0[4] = {G(QW[E.seg]), G(QX[E.seg]), G(QY[E.seg]), G(QZ[E.seg])};
+  G qm[4] = {0, 0, 0, 0}, qp[4] = {0, 0, 0, 0};
+  if (hasPrevSeg) { int j = soff + sm; qm[0] = G(QW[j]); qm[1] = G(QX[j]); qm[2] = G(QY[j]); qm[3] = G(QZ[j]); }
+  if (hasNextSeg) { int j = soff + sp; qp[0] = G(QW[j]); qp[1] = G(QX[j]); qp[2] = G(QY[j]); qp[3] = G(QZ[j]); }
+  auto unitAdd = [](G* T, double dx, double dy, double dz) {
+    G ex = G(dx), ey = G(dy), ez = G(dz);
+    G l = dmax(fsqrt_(ex * ex + ey * ey + ez * ez), G(1e-12));
     T[0] += ex / l; T[1] += ey / l; T[2] += ez / l;
   };
-  auto normT = [](Real* T) {
-    Real l = dmax(fsqrt_(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]), Real(1e-12));
+  auto normT = [](G* T) {
+    G l = dmax(fsqrt_(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]), G(1e-12));
     T[0] /= l; T[1] /= l; T[2] /= l;
   };
-  auto meanQ = [](Real* out, const Real* qs, bool hasStart, const Real* qe, bool hasEnd) {
-    const Real* r = hasStart ? qs : qe;
-    out[0] = out[1] = out[2] = out[3] = Real(0);
+  auto meanQ = [](G* out, const G* qs, bool hasStart, const G* qe, bool hasEnd) {
+    const G* r = hasStart ? qs : qe;
+    out[0] = out[1] = out[2] = out[3] = G(0);
     if (hasStart) for (int k = 0; k < 4; k++) out[k] += qs[k];   // sign of qs · qs is +1
     if (hasEnd) {
-      Real d = qe[0] * r[0] + qe[1] * r[1] + qe[2] * r[2] + qe[3] * r[3];
-      Real sg = d < Real(0) ? Real(-1) : Real(1);
+      G d = qe[0] * r[0] + qe[1] * r[1] + qe[2] * r[2] + qe[3] * r[3];
+      G sg = d < G(0) ? G(-1) : G(1);
       for (int k = 0; k < 4; k++) out[k] += sg * qe[k];
     }
-    Real nq = fsqrt_(out[0] * out[0] + out[1] * out[1] + out[2] * out[2] + out[3] * out[3]);
-    if (nq == Real(0)) nq = Real(1);
+    G nq = fsqrt_(out[0] * out[0] + out[1] * out[1] + out[2] * out[2] + out[3] * out[3]);
+    if (nq == G(0)) nq = G(1);
     for (int k = 0; k < 4; k++) out[k] /= nq;
   };
-  Real Ta[3] = {0, 0, 0}, Tb[3] = {0, 0, 0}, qa[4], qb[4];
+  G Ta[3] = {0, 0, 0}, Tb[3] = {0, 0, 0}, qa[4], qb[4];
   unitAdd(Ta, xb - xa, yb - ya, zb - za);                           // segment i starts at a
   if (hasPrevSeg) unitAdd(Ta, xa - xm, ya - ym, za - zm);           // segment i-1 ends at a
   normT(Ta);
@@ -214,26 +218,26 @@ __device__ __forceinline__ void eval_side(const Inst<Real>& I, int cur, int c, i
   normT(Tb);
   meanQ(qa, q0, true, qm, hasPrevSeg);
   meanQ(qb, qp, hasNextSeg, q0, true);
-  Real pa[3] = {Real(xa - ox), Real(ya - oy), Real(za - oz)};
-  Real pb[3] = {Real(xb - ox), Real(yb - oy), Real(zb - oz)};
-  Real dx = Real(xb - xa), dy = Real(yb - ya), dz = Real(zb - za);
-  Real L = fsqrt_(dx * dx + dy * dy + dz * dz);
-  Real t2 = t * t, t3 = t * t * t;
-  Real h00 = 2 * t3 - 3 * t2 + 1, h10 = t3 - 2 * t2 + t, h01 = -2 * t3 + 3 * t2, h11 = t3 - t2;
-  Real d00 = 6 * t2 - 6 * t, d10 = 3 * t2 - 4 * t + 1, d01 = -6 * t2 + 6 * t, d11 = 3 * t2 - 2 * t;
-  Real tg[3];
+  G pa[3] = {G(xa - ox), G(ya - oy), G(za - oz)};
+  G pb[3] = {G(xb - ox), G(yb - oy), G(zb - oz)};
+  G dx = G(xb - xa), dy = G(yb - ya), dz = G(zb - za);
+  G L = fsqrt_(dx * dx + dy * dy + dz * dz);
+  G t2 = t * t, t3 = t * t * t;
+  G h00 = 2 * t3 - 3 * t2 + 1, h10 = t3 - 2 * t2 + t, h01 = -2 * t3 + 3 * t2, h11 = t3 - t2;
+  G d00 = 6 * t2 - 6 * t, d10 = 3 * t2 - 4 * t + 1, d01 = -6 * t2 + 6 * t, d11 = 3 * t2 - 2 * t;
+  G tg[3];
   for (int k = 0; k < 3; k++) {
     E.P[k] = h00 * pa[k] + (h10 * L) * Ta[k] + h01 * pb[k] + (h11 * L) * Tb[k];
     tg[k] = d00 * pa[k] + (d10 * L) * Ta[k] + d01 * pb[k] + (d11 * L) * Tb[k];
   }
   normT(tg);
-  Real sgn = (qa[0] * qb[0] + qa[1] * qb[1] + qa[2] * qb[2] + qa[3] * qb[3]) < Real(0) ? Real(-1) : Real(1);
-  Real q[4];
+  G sgn = (qa[0] * qb[0] + qa[1] * qb[1] + qa[2] * qb[2] + qa[3] * qb[3]) < G(0) ? G(-1) : G(1);
+  G q[4];
   for (int k = 0; k < 4; k++) q[k] = (1 - t) * qa[k] + t * sgn * qb[k];
-  Real Rn[9];
+  G Rn[9];
   qframeR(q[0], q[1], q[2], q[3], Rn);   // normalises q first (qnormalize then toFrame)
-  Real y[3] = {Rn[3], Rn[4], Rn[5]};
-  Real yt = y[0] * tg[0] + y[1] * tg[1] + y[2] * tg[2];
+  G y[3] = {Rn[3], Rn[4], Rn[5]};
+  G yt = y[0] * tg[0] + y[1] * tg[1] + y[2] * tg[2];
   y[0] -= yt * tg[0]; y[1] -= yt * tg[1]; y[2] -= yt * tg[2];
   normT(y);
   E.R[0] = y[1] * tg[2] - y[2] * tg[1]; E.R[1] = y[2] * tg[0] - y[0] * tg[2]; E.R[2] = y[0] * tg[1] - y[1] * tg[0];
@@ -264,10 +268,10 @@ __device__ __forceinline__ Real walls(int sideCut, const Real* R, const Real* n,
   return ftan_(facos_(cp));
 }
 
-template <typename Real>
-__device__ __forceinline__ Real free_weight(const Inst<Real>& I, const SideEv<Real>& E) {
+template <typename Real, typename G>
+__device__ __forceinline__ G free_weight(const Inst<Real>& I, const SideEv<G>& E) {
   bool fa = (I.fix[E.a] & 16) != 0, fb = (I.fix[E.b] & 16) != 0;
-  return (fa ? Real(0) : (1 - E.t) * (1 - E.t)) + (fb ? Real(0) : E.t * E.t);
+  return (fa ? G(0) : (1 - E.t) * (1 - E.t)) + (fb ? G(0) : E.t * E.t);
 }
 
 template <typename Real>
@@ -286,221 +290,223 @@ __global__ __launch_bounds__(CROSS_BLOCK) void k_cross(const Inst<Real>* __restr
   const int tid = threadIdx.x;
   double eC = 0.0, mxFn = -INFINITY;
   if (tid < it.count) {
+    using G = double;   // contact kinematics in FP64: the penetration D − g cancels mm-scale lengths
+
     const int c = it.c0 + tid;
     const int rampN = max(I.ramp_iter, 0);
     const double rampD = rampN == 0 ? 1.0 : fmin(1.0, 0.01 + 0.99 * iter / rampN);
-    const Real ramp = Real(rampD);
-    const Real kn = ramp * Real(I.strength), kt = ramp * Real(I.stick);
+    const G ramp = G(rampD);
+    const G kn = ramp * G(I.strength), kt = ramp * G(I.stick);
     const bool enabled = I.k_en[c] != 0;
     const bool locked = I.k_lock[c] != 0;
     const bool seat = I.seat != 0, aug = I.aug != 0;
-    const Real mu0 = Real(I.mu);
+    const G mu0 = G(I.mu);
     // ---- both sides, relative to the first node of A's segment
     const double uA = I.u[2 * c], uB = I.u[2 * c + 1];
     const int o = I.k_noff[2 * c] + int(fmin(fmax(floor(uA), 0.0), double(max(I.k_nseg[2 * c] - 1, 0))));
     const double ox = I.x[cur][o], oy = I.y[cur][o], oz = I.z[cur][o];
-    SideEv<Real> A, B;
+    SideEv<G> A, B;
     eval_side(I, cur, c, 0, uA, ox, oy, oz, A);
     eval_side(I, cur, c, 1, uB, ox, oy, oz, B);
-    const Real wA = I.wk[2 * c], wB = I.wk[2 * c + 1], hA = I.hk[2 * c], hB = I.hk[2 * c + 1];
+    const G wA = I.wk[2 * c], wB = I.wk[2 * c + 1], hA = I.hk[2 * c], hB = I.hk[2 * c + 1];
     // ---- AL multiplier update deferred from the end of the previous iteration (updateMultipliers, :499-506)
     if (aug && al_update) {
-      Real kn1 = Real(I.strength);   // the ramp is complete whenever the multipliers update
-      bool pinned = free_weight(I, A) < Real(AL_FREE_MIN) || free_weight(I, B) < Real(AL_FREE_MIN);
-      Real cap = dmin(I.fc90[2 * c], I.fc90[2 * c + 1]) * wA * wB / dmax(fsin_(I.theta[c]), Real(1e-6));
-      Real v = I.lam[c] + kn1 * (I.penacc[c] / Real(max(pen_n, 1)));
-      if (!locked) v = dmax(v, Real(0));
+      G kn1 = G(I.strength);   // the ramp is complete whenever the multipliers update
+      bool pinned = free_weight(I, A) < G(AL_FREE_MIN) || free_weight(I, B) < G(AL_FREE_MIN);
+      G cap = dmin(I.fc90[2 * c], I.fc90[2 * c + 1]) * wA * wB / dmax(fsin_(I.theta[c]), G(1e-6));
+      G v = I.lam[c] + kn1 * (I.penacc[c] / G(max(pen_n, 1)));
+      if (!locked) v = dmax(v, G(0));
       v = clampv(v, -cap, cap);
-      I.lam[c] = (!enabled || pinned) ? Real(0) : v;
-      I.penacc[c] = Real(0);
+      I.lam[c] = (!enabled || pinned) ? G(0) : v;
+      I.penacc[c] = G(0);
     }
-    const Real* tA = A.tg; const Real* tB = B.tg;
-    Real np[3] = {I.nprev[3 * c], I.nprev[3 * c + 1], I.nprev[3 * c + 2]};
-    Real n[3] = {tA[1] * tB[2] - tA[2] * tB[1], tA[2] * tB[0] - tA[0] * tB[2], tA[0] * tB[1] - tA[1] * tB[0]};
-    Real nn = fsqrt_(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-    if (nn < Real(1e-6)) { n[0] = np[0]; n[1] = np[1]; n[2] = np[2]; }
-    else { Real d = dmax(nn, Real(1e-12)); n[0] /= d; n[1] /= d; n[2] /= d; }
-    if (n[0] * np[0] + n[1] * np[1] + n[2] * np[2] < Real(0)) { n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2]; }
+    const G* tA = A.tg; const G* tB = B.tg;
+    G np[3] = {I.nprev[3 * c], I.nprev[3 * c + 1], I.nprev[3 * c + 2]};
+    G n[3] = {tA[1] * tB[2] - tA[2] * tB[1], tA[2] * tB[0] - tA[0] * tB[2], tA[0] * tB[1] - tA[1] * tB[0]};
+    G nn = fsqrt_(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (nn < G(1e-6)) { n[0] = np[0]; n[1] = np[1]; n[2] = np[2]; }
+    else { G d = dmax(nn, G(1e-12)); n[0] /= d; n[1] /= d; n[2] /= d; }
+    if (n[0] * np[0] + n[1] * np[1] + n[2] * np[2] < G(0)) { n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2]; }
     I.nprev[3 * c] = n[0]; I.nprev[3 * c + 1] = n[1]; I.nprev[3 * c + 2] = n[2];
-    Real S[3] = {B.P[0] - A.P[0], B.P[1] - A.P[1], B.P[2] - A.P[2]};
-    Real g = S[0] * n[0] + S[1] * n[1] + S[2] * n[2];
-    Real rA[3], rB[3];
-    Real supA = support(A.R, n, Real(1), wA, hA, rA);
-    Real supB = support(B.R, n, Real(-1), wB, hB, rB);
-    Real D = supA + supB;
+    G S[3] = {B.P[0] - A.P[0], B.P[1] - A.P[1], B.P[2] - A.P[2]};
+    G g = S[0] * n[0] + S[1] * n[1] + S[2] * n[2];
+    G rA[3], rB[3];
+    G supA = support(A.R, n, G(1), wA, hA, rA);
+    G supB = support(B.R, n, G(-1), wB, hB, rB);
+    G D = supA + supB;
     if (seat && I.seat_gap) D -= I.dent[2 * c] + I.dent[2 * c + 1] + I.dente[c];
-    Real pen = D - g;
+    G pen = D - g;
     if (aug && iter >= rampN) I.penacc[c] += pen;
-    Real Fn = 0; bool act;
+    G Fn = 0; bool act;
     if (aug) {
-      Real trial = I.lam[c] + kn * pen;
-      act = (trial > Real(0) || locked) && enabled;
-      Fn = act ? (locked ? trial : dmax(trial, Real(0))) : Real(0);
+      G trial = I.lam[c] + kn * pen;
+      act = (trial > G(0) || locked) && enabled;
+      Fn = act ? (locked ? trial : dmax(trial, G(0))) : G(0);
     } else {
-      act = (pen > Real(0) || locked) && enabled;
-      Fn = act ? kn * pen : Real(0);
+      act = (pen > G(0) || locked) && enabled;
+      Fn = act ? kn * pen : G(0);
     }
-    Real St[3] = {S[0] - g * n[0], S[1] - g * n[1], S[2] - g * n[2]};
-    Real taa = tA[0] * tA[0] + tA[1] * tA[1] + tA[2] * tA[2], tbb = tB[0] * tB[0] + tB[1] * tB[1] + tB[2] * tB[2];
-    Real tab = tA[0] * tB[0] + tA[1] * tB[1] + tA[2] * tB[2];
-    Real sa = St[0] * tA[0] + St[1] * tA[1] + St[2] * tA[2], sb = St[0] * tB[0] + St[1] * tB[1] + St[2] * tB[2];
-    Real det = taa * tbb - tab * tab;
-    bool ok = det > Real(1e-8);
-    Real ca = ok ? (sa * tbb - sb * tab) / det : Real(0), cb = ok ? (sb * taa - sa * tab) / det : Real(0);
-    Real Ftv = fsqrt_((kt * St[0]) * (kt * St[0]) + (kt * St[1]) * (kt * St[1]) + (kt * St[2]) * (kt * St[2]));
+    G St[3] = {S[0] - g * n[0], S[1] - g * n[1], S[2] - g * n[2]};
+    G taa = tA[0] * tA[0] + tA[1] * tA[1] + tA[2] * tA[2], tbb = tB[0] * tB[0] + tB[1] * tB[1] + tB[2] * tB[2];
+    G tab = tA[0] * tB[0] + tA[1] * tB[1] + tA[2] * tB[2];
+    G sa = St[0] * tA[0] + St[1] * tA[1] + St[2] * tA[2], sb = St[0] * tB[0] + St[1] * tB[1] + St[2] * tB[2];
+    G det = taa * tbb - tab * tab;
+    bool ok = det > G(1e-8);
+    G ca = ok ? (sa * tbb - sb * tab) / det : G(0), cb = ok ? (sb * taa - sa * tab) / det : G(0);
+    G Ftv = fsqrt_((kt * St[0]) * (kt * St[0]) + (kt * St[1]) * (kt * St[1]) + (kt * St[2]) * (kt * St[2]));
     // ---- W3 directional seat friction (_seatFriction, :295-315)
-    Real mu = mu0;
+    G mu = mu0;
     if (seat) {
-      Real norm = fsqrt_(St[0] * St[0] + St[1] * St[1] + St[2] * St[2]);
-      Real dn = dmax(norm, Real(1e-12));
-      Real dh[3] = {St[0] / dn, St[1] / dn, St[2] / dn};
-      bool has = norm > Real(1e-12);
+      G norm = fsqrt_(St[0] * St[0] + St[1] * St[1] + St[2] * St[2]);
+      G dn = dmax(norm, G(1e-12));
+      G dh[3] = {St[0] / dn, St[1] / dn, St[2] / dn};
+      bool has = norm > G(1e-12);
       for (int j = 0; j < 2; j++) {
-        if (!(I.dent[2 * c + j] > Real(0) && has)) continue;
-        const Real* tCut = j == 0 ? tB : tA;
-        const Real* Rcut = j == 0 ? B.R : A.R;
-        Real mm[3] = {n[1] * tCut[2] - n[2] * tCut[1], n[2] * tCut[0] - n[0] * tCut[2], n[0] * tCut[1] - n[1] * tCut[0]};
-        Real ml = dmax(fsqrt_(mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2]), Real(1e-12));
+        if (!(I.dent[2 * c + j] > G(0) && has)) continue;
+        const G* tCut = j == 0 ? tB : tA;
+        const G* Rcut = j == 0 ? B.R : A.R;
+        G mm[3] = {n[1] * tCut[2] - n[2] * tCut[1], n[2] * tCut[0] - n[0] * tCut[2], n[0] * tCut[1] - n[1] * tCut[0]};
+        G ml = dmax(fsqrt_(mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2]), G(1e-12));
         mm[0] /= ml; mm[1] /= ml; mm[2] /= ml;
-        Real sgn = j == 0 ? Real(1) : Real(-1);
-        Real dj = (dh[0] * mm[0] + dh[1] * mm[1] + dh[2] * mm[2]) * sgn;
-        Real wd[3] = {jsSign(dj) * mm[0] * sgn, jsSign(dj) * mm[1] * sgn, jsSign(dj) * mm[2] * sgn};
-        Real tp = walls(1 - j, Rcut, n, wd);
+        G sgn = j == 0 ? G(1) : G(-1);
+        G dj = (dh[0] * mm[0] + dh[1] * mm[1] + dh[2] * mm[2]) * sgn;
+        G wd[3] = {jsSign(dj) * mm[0] * sgn, jsSign(dj) * mm[1] * sgn, jsSign(dj) * mm[2] * sgn};
+        G tp = walls(1 - j, Rcut, n, wd);
         I.tanpsi[2 * c + j] = tp;
-        Real add = fabs(dj) * (mu0 * tp < Real(1) ? (mu0 + tp) / dmax(Real(1) - mu0 * tp, Real(1e-6)) - mu0 : Real(9) * mu0);
-        mu += dmin(add, Real(9) * mu0);
+        G add = fabs(dj) * (mu0 * tp < G(1) ? (mu0 + tp) / dmax(G(1) - mu0 * tp, G(1e-6)) - mu0 : G(9) * mu0);
+        mu += dmin(add, G(9) * mu0);
       }
     }
-    Real limit = locked ? Real(INFINITY) : mu * fabs(Fn);
+    G limit = locked ? G(INFINITY) : mu * fabs(Fn);
     bool slip = act && ok && Ftv > limit;
     bool fre = !act && ok && enabled;
-    Real sc = 0;
-    if (slip) sc = clampv(Real(I.slip_rate), Real(1e-3), Real(1)) * (Real(1) - limit / dmax(Ftv, Real(1e-12)));
-    if (fre) sc = Real(1);
-    Real eps = dmax(Real(I.epsilon), Real(1e-9));
-    Real da = clampv(ca * sc, -eps, eps), db = clampv(cb * sc, -eps, eps);
+    G sc = 0;
+    if (slip) sc = clampv(G(I.slip_rate), G(1e-3), G(1)) * (G(1) - limit / dmax(Ftv, G(1e-12)));
+    if (fre) sc = G(1);
+    G eps = dmax(G(I.epsilon), G(1e-9));
+    G da = clampv(ca * sc, -eps, eps), db = clampv(cb * sc, -eps, eps);
     bool moved = slip || fre;
     if (moved) {
-      double u0 = I.u[2 * c] + double(da) / double(dmax(A.L, Real(1e-12)));
-      double u1 = I.u[2 * c + 1] - double(db) / double(dmax(B.L, Real(1e-12)));
+      double u0 = I.u[2 * c] + double(da) / double(dmax(A.L, G(1e-12)));
+      double u1 = I.u[2 * c + 1] - double(db) / double(dmax(B.L, G(1e-12)));
       I.u[2 * c] = fmin(fmax(u0, 0.0), double(I.k_nseg[2 * c]));
       I.u[2 * c + 1] = fmin(fmax(u1, 0.0), double(I.k_nseg[2 * c + 1]));
     }
     if (slip) {
-      Real q0 = da * tA[0] + db * tB[0], q1 = da * tA[1] + db * tB[1], q2 = da * tA[2] + db * tB[2];
+      G q0 = da * tA[0] + db * tB[0], q1 = da * tA[1] + db * tB[1], q2 = da * tA[2] + db * tB[2];
       I.slip[c] += double(fsqrt_(q0 * q0 + q1 * q1 + q2 * q2));
     }
-    Real Se[3] = {0, 0, 0};
+    G Se[3] = {0, 0, 0};
     if (act) { Se[0] = St[0] - da * tA[0] - db * tB[0]; Se[1] = St[1] - da * tA[1] - db * tB[1]; Se[2] = St[2] - da * tA[2] - db * tB[2]; }
-    Real fT[3] = {kt * Se[0], kt * Se[1], kt * Se[2]};
-    Real FA[3], FB[3];
+    G fT[3] = {kt * Se[0], kt * Se[1], kt * Se[2]};
+    G FA[3], FB[3];
     for (int k = 0; k < 3; k++) { FA[k] = -Fn * n[k] + fT[k]; FB[k] = Fn * n[k] - fT[k]; }
     I.fn[c] = Fn;
     I.ft[c] = fsqrt_(fT[0] * fT[0] + fT[1] * fT[1] + fT[2] * fT[2]);
     I.gap[c] = g;
     I.kflags[c] = uint8_t((slip ? 1 : 0) | (moved ? 0 : 2));
-    for (int k = 0; k < 3; k++) I.mid[3 * c + k] = Real(0.5) * (A.P[k] + B.P[k]) + Real(k == 0 ? ox : (k == 1 ? oy : oz));
+    for (int k = 0; k < 3; k++) I.mid[3 * c + k] = G(0.5) * (A.P[k] + B.P[k]) + G(k == 0 ? ox : (k == 1 ? oy : oz));
     // ---- W1 classification (_classify, :231-271)
-    Real cface[2], z0[2], phiv[2]; int edge[2];
+    G cface[2], z0[2], phiv[2]; int edge[2];
     for (int sd = 0; sd < 2; sd++) {
-      const Real* R = sd == 0 ? A.R : B.R;
-      Real sg = sd == 0 ? Real(1) : Real(-1);
-      Real wv = sd == 0 ? wA : wB, hv = sd == 0 ? hA : hB;
-      Real nx = (n[0] * R[0] + n[1] * R[1] + n[2] * R[2]) * sg, ny = (n[0] * R[3] + n[1] * R[4] + n[2] * R[5]) * sg;
-      Real sx = fabs(ftanh_(nx / Real(SUPPORT_SMOOTH))), sy = fabs(ftanh_(ny / Real(SUPPORT_SMOOTH)));
-      edge[sd] = sx * sy > Real(I.edge_thr) ? 1 : 0;
+      const G* R = sd == 0 ? A.R : B.R;
+      G sg = sd == 0 ? G(1) : G(-1);
+      G wv = sd == 0 ? wA : wB, hv = sd == 0 ? hA : hB;
+      G nx = (n[0] * R[0] + n[1] * R[1] + n[2] * R[2]) * sg, ny = (n[0] * R[3] + n[1] * R[4] + n[2] * R[5]) * sg;
+      G sx = fabs(ftanh_(nx / G(SUPPORT_SMOOTH))), sy = fabs(ftanh_(ny / G(SUPPORT_SMOOTH)));
+      edge[sd] = sx * sy > G(I.edge_thr) ? 1 : 0;
       bool yf = fabs(ny) >= fabs(nx);
       cface[sd] = yf ? wv : hv; z0[sd] = yf ? hv : wv;
-      phiv[sd] = fatan_(dmin(Real(fabs(nx)), Real(fabs(ny))) / dmax(dmax(Real(fabs(nx)), Real(fabs(ny))), Real(1e-12)));
+      phiv[sd] = fatan_(dmin(G(fabs(nx)), G(fabs(ny))) / dmax(dmax(G(fabs(nx)), G(fabs(ny))), G(1e-12)));
     }
-    Real cr[3] = {tA[1] * tB[2] - tA[2] * tB[1], tA[2] * tB[0] - tA[0] * tB[2], tA[0] * tB[1] - tA[1] * tB[0]};
-    Real sinT = clampv(fsqrt_(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]), Real(1e-6), Real(1));
-    Real theta = fasin_(sinT);
+    G cr[3] = {tA[1] * tB[2] - tA[2] * tB[1], tA[2] * tB[0] - tA[0] * tB[2], tA[0] * tB[1] - tA[1] * tB[0]};
+    G sinT = clampv(fsqrt_(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]), G(1e-6), G(1));
+    G theta = fasin_(sinT);
     int nEdge = edge[0] + edge[1];
     int faceSide = nEdge == 1 ? (edge[1] ? 0 : 1) : -1;
-    Real Fa = fabs(Fn);
-    Real Aa = cface[0] * cface[1] / sinT, pres = Fa / dmax(Aa, Real(1e-12));
-    Real ut = pres / dmax(dmin(I.Hs[2 * c], I.Hs[2 * c + 1]), Real(1e-12));
+    G Fa = fabs(Fn);
+    G Aa = cface[0] * cface[1] / sinT, pres = Fa / dmax(Aa, G(1e-12));
+    G ut = pres / dmax(dmin(I.Hs[2 * c], I.Hs[2 * c + 1]), G(1e-12));
     if (nEdge == 1) {
       int fs = faceSide, es = 1 - fs;
-      Real ell = cface[fs] / sinT, kap = I.E90[2 * c + fs] / dmax(z0[fs], Real(1e-12));
-      Real phiE = dmax(phiv[es], Real(1e-4));
-      Real bw = fsqrt_(Real(2) * Fa / dmax(ell * kap * phiE, Real(1e-12)));
+      G ell = cface[fs] / sinT, kap = I.E90[2 * c + fs] / dmax(z0[fs], G(1e-12));
+      G phiE = dmax(phiv[es], G(1e-4));
+      G bw = fsqrt_(G(2) * Fa / dmax(ell * kap * phiE, G(1e-12)));
       bool full = bw >= cface[es];
-      Real Afe = full ? cface[0] * cface[1] / sinT : bw * ell;
-      Real pfe = full ? Fa / dmax(Afe, Real(1e-12)) : kap * phiE * bw;
-      Aa = Afe; pres = pfe; ut = pfe / dmax(I.fc90[2 * c + es], Real(1e-12));
+      G Afe = full ? cface[0] * cface[1] / sinT : bw * ell;
+      G pfe = full ? Fa / dmax(Afe, G(1e-12)) : kap * phiE * bw;
+      Aa = Afe; pres = pfe; ut = pfe / dmax(I.fc90[2 * c + es], G(1e-12));
     } else if (nEdge == 2) {
-      Real fc = dmin(I.fc90[2 * c], I.fc90[2 * c + 1]);
-      Aa = Fa / dmax(fc, Real(1e-12)); pres = Fa > Real(0) ? fc : Real(0); ut = Fa > Real(0) ? Real(1) : Real(0);
+      G fc = dmin(I.fc90[2 * c], I.fc90[2 * c + 1]);
+      Aa = Fa / dmax(fc, G(1e-12)); pres = Fa > G(0) ? fc : G(0); ut = Fa > G(0) ? G(1) : G(0);
     }
     I.theta[c] = theta; I.anom[c] = Aa; I.pres[c] = pres; I.util[c] = ut;
     I.phi[2 * c] = phiv[0]; I.phi[2 * c + 1] = phiv[1];
     I.ctype[c] = int8_t(nEdge); I.face[c] = int8_t(faceSide);
     // ---- W3 seat update (_seatUpdate, :317-344)
     if (seat) {
-      Real inc[3] = {da * tA[0] - db * tB[0], da * tA[1] - db * tB[1], da * tA[2] - db * tB[2]};
+      G inc[3] = {da * tA[0] - db * tB[0], da * tA[1] - db * tB[1], da * tA[2] - db * tB[2]};
       for (int j = 0; j < 2; j++) {
-        const Real* tCut = j == 0 ? tB : tA;
-        Real mm[3] = {n[1] * tCut[2] - n[2] * tCut[1], n[2] * tCut[0] - n[0] * tCut[2], n[0] * tCut[1] - n[1] * tCut[0]};
-        Real ml = dmax(fsqrt_(mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2]), Real(1e-12));
+        const G* tCut = j == 0 ? tB : tA;
+        G mm[3] = {n[1] * tCut[2] - n[2] * tCut[1], n[2] * tCut[0] - n[0] * tCut[2], n[0] * tCut[1] - n[1] * tCut[0]};
+        G ml = dmax(fsqrt_(mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2]), G(1e-12));
         mm[0] /= ml; mm[1] /= ml; mm[2] /= ml;
-        Real stp = (inc[0] * mm[0] + inc[1] * mm[1] + inc[2] * mm[2]) * (j == 0 ? Real(1) : Real(-1));
+        G stp = (inc[0] * mm[0] + inc[1] * mm[1] + inc[2] * mm[2]) * (j == 0 ? G(1) : G(-1));
         int i2 = 2 * c + j;
-        bool has = I.dent[i2] > Real(0);
+        bool has = I.dent[i2] > G(0);
         if (has && slip) I.climb[i2] += stp;
-        if (has && fabs(I.climb[i2]) >= I.dent[i2] / dmax(I.tanpsi[i2], Real(1e-6))) { I.dent[i2] = Real(0); I.climb[i2] = Real(0); }
+        if (has && fabs(I.climb[i2]) >= I.dent[i2] / dmax(I.tanpsi[i2], G(1e-6))) { I.dent[i2] = G(0); I.climb[i2] = G(0); }
       }
       if (iter >= I.dent_after) {
-        Real sp0 = fsin_(phiv[0]) * fcos_(phiv[0]), sp1 = fsin_(phiv[1]) * fcos_(phiv[1]);
-        Real sT = fsin_(theta);
+        G sp0 = fsin_(phiv[0]) * fcos_(phiv[0]), sp1 = fsin_(phiv[1]) * fcos_(phiv[1]);
+        G sT = fsin_(theta);
         if (nEdge == 2 && act && !locked) {
-          Real fc = dmin(I.fc90[2 * c], I.fc90[2 * c + 1]);
-          Real dEst = fsqrt_(Fa / dmax(fc, Real(1e-12)) * sT * sp0 * sp1);
-          for (int j = 0; j < 2; j++) { Real cu = I.dent[2 * c + j]; I.dent[2 * c + j] = dmax(cu, dmin(cu + Real(I.dent_rate) * dEst, dEst)); }
+          G fc = dmin(I.fc90[2 * c], I.fc90[2 * c + 1]);
+          G dEst = fsqrt_(Fa / dmax(fc, G(1e-12)) * sT * sp0 * sp1);
+          for (int j = 0; j < 2; j++) { G cu = I.dent[2 * c + j]; I.dent[2 * c + j] = dmax(cu, dmin(cu + G(I.dent_rate) * dEst, dEst)); }
         }
         if (nEdge == 1 && act && !locked) {
           int fs = faceSide, es = 1 - fs;
-          Real ell = cface[fs] / dmax(sT, Real(1e-6));
-          Real dE = Fa / dmax(I.fc90[2 * c + es] * ell, Real(1e-12)) * (es == 0 ? sp0 : sp1);
+          G ell = cface[fs] / dmax(sT, G(1e-6));
+          G dE = Fa / dmax(I.fc90[2 * c + es] * ell, G(1e-12)) * (es == 0 ? sp0 : sp1);
           I.dente[c] = dmax(I.dente[c], dE);
         }
       }
     }
     // ---- lever torques, W2 rotational friction (_rotationalFriction, :347-379)
-    Real tf = Real(I.torsion_factor);
-    Real TA[3] = {tf * (rA[1] * FA[2] - rA[2] * FA[1]), tf * (rA[2] * FA[0] - rA[0] * FA[2]), tf * (rA[0] * FA[1] - rA[1] * FA[0])};
-    Real TB[3] = {tf * (rB[1] * FB[2] - rB[2] * FB[1]), tf * (rB[2] * FB[0] - rB[0] * FB[2]), tf * (rB[0] * FB[1] - rB[1] * FB[0])};
-    Real eRot = 0;
+    G tf = G(I.torsion_factor);
+    G TA[3] = {tf * (rA[1] * FA[2] - rA[2] * FA[1]), tf * (rA[2] * FA[0] - rA[0] * FA[2]), tf * (rA[0] * FA[1] - rA[1] * FA[0])};
+    G TB[3] = {tf * (rB[1] * FB[2] - rB[2] * FB[1]), tf * (rB[2] * FB[0] - rB[0] * FB[2]), tf * (rB[0] * FB[1] - rB[1] * FB[0])};
+    G eRot = 0;
     if (I.rot_fric) {
-      Real thc = fatan2_(cr[0] * n[0] + cr[1] * n[1] + cr[2] * n[2], tab);
-      Real th0 = I.theta0[c];
+      G thc = fatan2_(cr[0] * n[0] + cr[1] * n[1] + cr[2] * n[2], tab);
+      G th0 = I.theta0[c];
       if (isnan(th0) && act) th0 = thc;
       bool okr = act && !isnan(th0);
-      Real sT2 = dmax(fsin_(theta), Real(1e-6));
-      Real req = Real(0.38) * fsqrt_(wA * wB / sT2);
-      if (nEdge == 1) req = Real(0.25) * (faceSide == 0 ? wA : wB) / sT2;
-      else if (nEdge == 2) req = Real(0.38) * fsqrt_(dmax(Aa, Real(1e-12)));
-      Real kr = kt * req * req;
-      Real dth = okr ? thc - th0 : Real(0);
-      const Real PI = Real(3.141592653589793);
+      G sT2 = dmax(fsin_(theta), G(1e-6));
+      G req = G(0.38) * fsqrt_(wA * wB / sT2);
+      if (nEdge == 1) req = G(0.25) * (faceSide == 0 ? wA : wB) / sT2;
+      else if (nEdge == 2) req = G(0.38) * fsqrt_(dmax(Aa, G(1e-12)));
+      G kr = kt * req * req;
+      G dth = okr ? thc - th0 : G(0);
+      const G PI = G(3.141592653589793);
       dth = ffmod_(ffmod_(dth + PI, 2 * PI) + 2 * PI, 2 * PI) - PI;
-      Real Mc = kr * dth;
-      Real Mlim = locked ? Real(INFINITY) : Real(I.rot_factor) * mu0 * Fa * req;
+      G Mc = kr * dth;
+      G Mlim = locked ? G(INFINITY) : G(I.rot_factor) * mu0 * Fa * req;
       if (okr && fabs(Mc) > Mlim) {
-        Real adv = clampv(clampv(Real(I.slip_rate), Real(1e-3), Real(1)) * (Real(1) - Mlim / dmax(Real(fabs(Mc)), Real(1e-12))) * dth, Real(-0.05), Real(0.05));
+        G adv = clampv(clampv(G(I.slip_rate), G(1e-3), G(1)) * (G(1) - Mlim / dmax(G(fabs(Mc)), G(1e-12))) * dth, G(-0.05), G(0.05));
         th0 += adv; I.rotslip[c] += fabs(adv); dth -= adv;
         Mc = kr * dth;
       }
       I.theta0[c] = th0;
-      Real Mr = okr ? Mc : Real(0);
+      G Mr = okr ? Mc : G(0);
       I.mrot[c] = Mr;
       for (int k = 0; k < 3; k++) { TA[k] += Mr * n[k]; TB[k] -= Mr * n[k]; }
-      eRot = Real(0.5) * kr * dth * dth;
+      eRot = G(0.5) * kr * dth * dth;
     }
     // ---- righting stiffness (addTwistStiffness, :170-176) on the segments after the slip
-    Real tfk = dmax(Real(I.torsion_factor), Real(1e-3));
-    Real kth = tfk * ((Fn + kn * Real(0.25) * I.kh[c]) * (Real(0.5) * I.kw[c]) / Real(SUPPORT_SMOOTH) + kt * (Real(0.5) * I.kh[c]) * (Real(0.5) * I.kh[c]));
+    G tfk = dmax(G(I.torsion_factor), G(1e-3));
+    G kth = tfk * ((Fn + kn * G(0.25) * I.kh[c]) * (G(0.5) * I.kw[c]) / G(SUPPORT_SMOOTH) + kt * (G(0.5) * I.kh[c]) * (G(0.5) * I.kh[c]));
     int postA = int(fmin(fmax(floor(I.u[2 * c]), 0.0), double(max(I.k_nseg[2 * c] - 1, 0))));
     int postB = int(fmin(fmax(floor(I.u[2 * c + 1]), 0.0), double(max(I.k_nseg[2 * c + 1] - 1, 0))));
     SideRec<Real> ra, rb;
@@ -509,7 +515,7 @@ __global__ __launch_bounds__(CROSS_BLOCK) void k_cross(const Inst<Real>* __restr
     for (int k = 0; k < 3; k++) { ra.F[k] = FA[k]; ra.T[k] = TA[k]; rb.F[k] = FB[k]; rb.T[k] = TB[k]; }
     I.side[I.side_pos[2 * c]] = ra;
     I.side[I.side_pos[2 * c + 1]] = rb;
-    Real ePen = act ? (aug ? Fn * Fn / dmax(kn, Real(1e-30)) : kn * pen * pen) : Real(0);
+    G ePen = act ? (aug ? Fn * Fn / dmax(kn, G(1e-30)) : kn * pen * pen) : G(0);
     eC = 0.5 * double(ePen) + 0.5 * double(kt) * double(Se[0] * Se[0] + Se[1] * Se[1] + Se[2] * Se[2]) + double(eRot);
     mxFn = double(Fn);
   }

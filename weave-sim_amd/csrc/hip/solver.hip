@@ -298,6 +298,7 @@ struct BatchBase {
   virtual int read_history(int b, double* out, int cap, int* n) = 0;
   virtual int download(int b, ws_state& out) = 0;
   virtual int internalize(int b) = 0;
+  virtual void kernel_bench(int reps, double* out) = 0;
   virtual double alg_bytes() = 0;
   virtual double dev_bytes() = 0;
 };
@@ -627,6 +628,24 @@ struct Batch : BatchBase {
     return WS_OK;
   }
 
+  void kernel_bench(int reps, double* out) override {
+    HIPCHK(hipStreamSynchronize(stream));
+    // make every instance live for the timing launches (stop_at beyond all of them, convergence flag cleared)
+    std::vector<int32_t> all(B);
+    std::iota(all.begin(), all.end(), 0);
+    HIPCHK(hipMemcpyAsync(d_sel, all.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_begin<Real>, dim3((B + 63) / 64), dim3(64), 0, stream, d_inst, d_sel, B, 3 * reps + 8, 1);
+    HIPCHK(hipGetLastError());
+    for (int k = 0; k < 3; k++) {
+      HIPCHK(hipEventRecord(ev0, stream));
+      for (int r = 0; r < reps; r++) { if (k == 0) launch_cross(stream); else if (k == 1) launch_strand(stream); else launch_ctrl(stream); }
+      HIPCHK(hipEventRecord(ev1, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+      out[k] = double(ms) / reps;
+    }
+  }
   double alg_bytes() override { double s = 0; for (auto& h : inst) s += h->alg_bytes; return s; }
   double dev_bytes() override { double s = 0; for (auto& h : inst) s += double(h->arena.size); return s; }
 };
@@ -737,6 +756,17 @@ WS_API int ws_batch_profile(ws_batch* batch, int32_t on) {
   if (!batch) return WS_ERR_ARG;
   batch->impl->profile = on != 0;
   return WS_OK;
+}
+WS_API int ws_batch_kernel_bench(ws_batch* batch, int32_t reps, double* ms_cross, double* ms_strand, double* ms_ctrl) {
+  if (!batch || reps <= 0) return WS_ERR_ARG;
+  try {
+    double out[3] = {0, 0, 0};
+    batch->impl->kernel_bench(reps, out);
+    if (ms_cross) *ms_cross = out[0];
+    if (ms_strand) *ms_strand = out[1];
+    if (ms_ctrl) *ms_ctrl = out[2];
+    return WS_OK;
+  } catch (const Fail& f) { return fail(batch->impl->ctx, f); }
 }
 WS_API int ws_batch_bytes(ws_batch* batch, double* alg, double* dev) {
   if (!batch) return WS_ERR_ARG;

@@ -704,7 +704,21 @@ std::unique_ptr<Model> buildModel(const Built& tp, const Setup& su) {
   // ---- synthetic preparation: seeded out-of-plane noise on the initial nodes (rng.js generator)
   if (su.hasPrep && su.prep.noise > 0) {
     Rng rng(su.prep.seed);
-    for (int i = 0; i < n; i++) md->X[3 * i + 2] += (2.0 * rng.uniform() - 1.0) * su.prep.noise;
+    if (su.prep.wavelength > 0) {
+      const int NW = 8;
+      double th[NW], lam[NW], ph[NW], w[NW], ws = 0;
+      for (int k = 0; k < NW; k++) {
+        th[k] = 2 * M_PI * rng.uniform(); lam[k] = su.prep.wavelength * (1.0 + 3.0 * rng.uniform());
+        ph[k] = 2 * M_PI * rng.uniform(); w[k] = 0.5 + rng.uniform(); ws += w[k];
+      }
+      for (int i = 0; i < n; i++) {
+        double x = md->X[3 * i], y = md->X[3 * i + 1], z = 0;
+        for (int k = 0; k < NW; k++) z += w[k] * std::sin(2 * M_PI * (x * std::cos(th[k]) + y * std::sin(th[k])) / lam[k] + ph[k]);
+        md->X[3 * i + 2] += su.prep.noise * z / ws;
+      }
+    } else {
+      for (int i = 0; i < n; i++) md->X[3 * i + 2] += (2.0 * rng.uniform() - 1.0) * su.prep.noise;
+    }
   }
   md->X0 = md->X;
 
@@ -1031,7 +1045,7 @@ WS_API int wsm_hex_dome(const wsm_hex_spec* spec, const wsm_scene* sc_in, wsm_mo
     }
     sc.subdivisions = std::max(1, int(std::lround((a / 2.0) / spec->segment)));
     wsm_anchor an[2] = {{WSM_ALL_START, 0, 0, {1, 1, 1, 0, 0, 0}}, {WSM_ALL_END, 0, 0, {1, 1, 1, 0, 0, 0}}};
-    wsm_prep prep{spec->noise, spec->seed, 0};
+    wsm_prep prep{spec->noise, 2.0 * spec->spacing, spec->seed, 0};
     int rc = wsm_from_mesh(V.data(), int(V.size() / 3), F.data(), int(F.size() / 3), 1, &sc, an, 2, nullptr, 0, nullptr, 0, &prep, out);
     if (rc == WS_OK) {
       (*out)->md->info.rings = R;

@@ -109,14 +109,14 @@ class Model:
         return cls._check(A.model_lib().wsm_hex_dome(C.byref(spec), C.byref(sc.c), C.byref(out)), out)
 
     @classmethod
-    def from_mesh(cls, V, F, scene=None, anchors=(), loads=(), flips=(), noise=0.0, seed=1, weld=True):
+    def from_mesh(cls, V, F, scene=None, anchors=(), loads=(), flips=(), noise=0.0, seed=1, weld=True, wavelength=0.0):
         sc = scene or Scene()
         V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1)
         F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1)
         an = (A.wsm_anchor * max(len(anchors), 1))(*[_anchor(a) for a in anchors])
         ld = (A.wsm_point_load * max(len(loads), 1))(*[_load(x) for x in loads])
         fl = np.ascontiguousarray(np.array(flips, dtype=np.int32).reshape(-1))
-        prep = A.wsm_prep(noise, seed, 0)
+        prep = A.wsm_prep(noise, wavelength, seed, 0)
         out = C.c_void_p()
         rc = A.model_lib().wsm_from_mesh(V.ctypes.data_as(A.P_d), len(V) // 3, F.ctypes.data_as(A.P_i32), len(F) // 3,
                                          1 if weld else 0, C.byref(sc.c), an, len(anchors), ld, len(loads),

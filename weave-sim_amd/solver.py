@@ -138,6 +138,12 @@ class GpuSolver:
         self.ctx.check(self.lib.ws_batch_timing(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(e)))
         return {"ms_last_step": a.value, "ms_cross": b.value, "ms_strand": c.value, "ms_ctrl": e.value}
 
+    def kernel_bench(self, reps=50):
+        """mean ms per launch of each kernel alone (back-to-back launches between two hipEvents); advances the state"""
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        self.ctx.check(self.lib.ws_batch_kernel_bench(self.h, reps, C.byref(a), C.byref(b), C.byref(c)))
+        return {"ms_cross": a.value, "ms_strand": b.value, "ms_ctrl": c.value}
+
     def bytes(self):
         a, b = C.c_double(), C.c_double()
         self.ctx.check(self.lib.ws_batch_bytes(self.h, C.byref(a), C.byref(b)))
