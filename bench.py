@@ -37,14 +37,14 @@ def strand_cross_bytes(models, precision):
     return sb, cb
 
 
-def build_models(config):
+def build_models(config, batch=256):
     import cases
     if config == 1:
         return [cases.config1()], "kagome dome 3 m / 0.3 m spacing / 0.05 m segments, 15% contraction, ±1 mm noise"
     if config == 2:
         return [cases.config2()], "kagome dome 10 m / 0.15 m spacing / 0.02 m segments, 12% contraction, ±2 mm noise"
     if config == 3:
-        return [cases.config3_variant(i) for i in range(256)], "256 ragged kagome design variants, 2k-50k nodes each"
+        return [cases.config3_variant(i) for i in range(batch)], f"{batch} ragged kagome design variants, 2k-50k nodes each"
     if config == 5:
         return [cases.config5()], "kagome dome 30 m / 0.2 m spacing / 0.015 m segments, mu 0.05, 18% contraction"
     raise SystemExit(f"unknown config {config}")
@@ -76,13 +76,20 @@ def main():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--batch", type=int, default=256, help="instances of config 3")
+    ap.add_argument("--copy-probe", type=int, default=0, help="only run the copy kernel over this many bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: runs on one GPU only (got --gpus {args.gpus}); instances are independent and would shard "
               "one solve per GPU, which this build does not launch", file=sys.stderr)
         sys.exit(2)
     W = importlib.import_module("weave-sim_amd")
-    models, desc = build_models(args.config)
+    if args.copy_probe:
+        ctx = W.solver.default_context()
+        ms = ctx.copy_probe(args.copy_probe, 20)
+        print(json.dumps({"copy_bytes": args.copy_probe, "ms": ms, "GBps": 2 * args.copy_probe / (ms * 1e-3) / 1e9}))
+        return
+    models, desc = build_models(args.config, args.batch)
     n_nodes = sum(m.n for m in models)
     n_cross = sum(m.K for m in models)
     g = W.GpuSolver(models, precision=args.precision, tile=args.tile)

@@ -180,6 +180,9 @@ WS_API int ws_ctx_create(int32_t device, ws_ctx** out);      /* getDevice() (sol
 WS_API void ws_ctx_destroy(ws_ctx* ctx);
 WS_API const char* ws_last_error(const ws_ctx* ctx);         /* message of the last failed call */
 WS_API int ws_device_info(ws_ctx* ctx, char* name, int32_t cap, int64_t* hbm_bytes, int32_t* n_cu);
+/* a plain device-to-device copy kernel over `bytes` (read once, written once), `reps` launches: the calibration
+ * of the HBM traffic counters and the achievable copy bandwidth; returns the mean ms per launch */
+WS_API int ws_copy_probe(ws_ctx* ctx, int64_t bytes, int32_t reps, double* ms);
 
 /* ---- batch (GpuSolver) ---------------------------------------------------------------------------
  * B instances, heterogeneous topologies allowed (the reference requires one topology per batch,

@@ -164,6 +164,7 @@ def hip_lib():
         L.ws_last_error.argtypes = [vp]
         L.ws_last_error.restype = C.c_char_p
         L.ws_device_info.argtypes = [vp, C.c_char_p, i32, C.POINTER(i64), C.POINTER(i32)]
+        L.ws_copy_probe.argtypes = [vp, i64, i32, P_d]
         L.ws_batch_create.argtypes = [vp, C.POINTER(C.POINTER(ws_model)), C.POINTER(C.POINTER(ws_state)), i32,
                                       C.POINTER(ws_batch_opts), C.POINTER(vp)]
         L.ws_batch_destroy.argtypes = [vp]
@@ -194,7 +195,7 @@ def hip_lib():
 
 # every symbol include/weave.h declares (checked by tests/test_abi.py)
 WEAVE_H_SYMBOLS = [
-    "ws_ctx_create", "ws_ctx_destroy", "ws_last_error", "ws_device_info", "ws_batch_create", "ws_batch_destroy",
+    "ws_ctx_create", "ws_ctx_destroy", "ws_last_error", "ws_device_info", "ws_copy_probe", "ws_batch_create", "ws_batch_destroy",
     "ws_batch_size", "ws_upload", "ws_update_params", "ws_step", "ws_read_control", "ws_read_history", "ws_download",
     "ws_internalize", "ws_batch_timing", "ws_batch_bytes", "ws_batch_profile", "ws_batch_kernel_bench", "ws_server_create", "ws_server_destroy", "ws_submit",
     "ws_cancel", "ws_pump", "ws_poll", "ws_fetch", "ws_server_clock_ms",

@@ -44,6 +44,7 @@ struct Inst {
   int32_t n, m, p, K, C, S, npl, nclamp_entries;
   // work-list ranges of this instance
   int32_t tile0, ntiles, cblk0, ncblk;
+  int32_t tile_waves, pad_w;        // wavefronts per strand tile (partials per tile)
   // scalar parameters (ws_params)
   double mass_scale, damping, tol, mu, strength, stick, torsion_factor, epsilon, slip_rate, rot_factor, dent_rate;
   double edge_thr, l_ref, jlen_min;

@@ -533,19 +533,412 @@ __global__ __launch_bounds__(CROSS_BLOCK) void k_cross(const Inst<Real>* __restr
 // ================================================================================================= k_strand
 template <typename Real, int TILE>
 struct StrandLDS {
-  double X[TILE + 2][3];      // nodes k0-1 .. k1
-  double Q[TILE + 2][4];      // segments k0-1 .. k1
-  Real SF[TILE + 1][3];       // world force of segments k0-1 .. k1-1
-  Real JA[TILE + 1][3];       // joint k0-1+t torque on its first segment (body axes)
-  Real JB[TILE + 1][3];       // ... on its second segment (body axes of that segment)
-  Real KD[TILE + 1][3];       // bending-coupling inertia of joint k0-1+t (both segments)
-  Real ST[TILE][3];           // stretch torque (Rᵀe) × (Rᵀn) of owned segment k0+t (body axes)
-  SideRec<Real> rec[TILE];    // gathered crossing sides (one chunk)
+  // structure-of-arrays: consecutive threads touch consecutive 8-byte words (no bank conflicts)
+  double Xx[TILE + 2], Xy[TILE + 2], Xz[TILE + 2];                   // nodes k0-1 .. k1
+  double Qw[TILE + 2], Qx[TILE + 2], Qy[TILE + 2], Qz[TILE + 2];     // segments k0-1 .. k1
+  Real SFx[TILE + 1], SFy[TILE + 1], SFz[TILE + 1];                  // world force of segments k0-1 .. k1-1
+  Real JAx[TILE + 1], JAy[TILE + 1], JAz[TILE + 1];                  // joint k0-1+i on its first segment (body axes)
+  Real JBx[TILE + 1], JBy[TILE + 1], JBz[TILE + 1];                  // ... on its second segment (its body axes)
+  Real KDx[TILE + 1], KDy[TILE + 1], KDz[TILE + 1];                  // bending-coupling inertia of joint k0-1+i
+  int32_t rpre[TILE], rpost[TILE];                                   // gathered crossing sides: segments
+  SideRec<Real> rec[TILE];                                           // ... and their records
   int wcnt[TILE / 64];
-  double red[13][TILE / 64];
 };
 
-// fixed-order block reduction: v[0, nsum) are sums, v[nsum, NV) maxima; every thread receives the result
+// field-wise copies of a side record (keeps the record in registers: no aggregate copy through private memory)
+template <typename Real>
+__device__ __forceinline__ void side_load(const SideRec<Real>* __restrict__ p, SideRec<Real>& r) {
+  r.seg_pre = p->seg_pre; r.seg_post = p->seg_post; r.t = p->t; r.kth = p->kth;
+  r.F[0] = p->F[0]; r.F[1] = p->F[1]; r.F[2] = p->F[2]; r.T[0] = p->T[0]; r.T[1] = p->T[1]; r.T[2] = p->T[2];
+}
+template <typename Real>
+__device__ __forceinline__ void side_store(const SideRec<Real>& r, SideRec<Real>* p) {
+  p->seg_pre = r.seg_pre; p->seg_post = r.seg_post; p->t = r.t; p->kth = r.kth;
+  p->F[0] = r.F[0]; p->F[1] = r.F[1]; p->F[2] = r.F[2]; p->T[0] = r.T[0]; p->T[1] = r.T[1]; p->T[2] = r.T[2];
+}
+
+// f64 frame [X | Y | Z] of a quaternion with one reciprocal: R = I + 2/|q|² (…) (quat.js toFrame up to rounding)
+__device__ __forceinline__ void qframe_d(double w, double x, double y, double z, double* R) {
+  double n2 = w * w + x * x + y * y + z * z;
+  double s = n2 > 0 ? 2.0 / n2 : 2.0;
+  R[0] = 1 - s * (y * y + z * z); R[1] = s * (x * y + z * w); R[2] = s * (x * z - y * w);
+  R[3] = s * (x * y - z * w); R[4] = 1 - s * (x * x + z * z); R[5] = s * (y * z + x * w);
+  R[6] = s * (x * z + y * w); R[7] = s * (y * z - x * w); R[8] = 1 - s * (x * x + y * y);
+}
+
+// Ψ and θ of the relative rotation d = ±(q_a* ⊗ q_b) (relRotation, rod_cosserat.js:33-45).  The product is formed in
+// f64 (its vector part carries the full precision of a small relative rotation); θ = 2 atan2(|v|, w) in Real.
+template <typename Real>
+__device__ __forceinline__ Real relrot_R(double aw, double ax, double ay, double az, double bw, double bx, double by, double bz,
+                                        Real* psi, double* d) {
+  ax = -ax; ay = -ay; az = -az;
+  double w = aw * bw - ax * bx - ay * by - az * bz;
+  double x = aw * bx + ax * bw + ay * bz - az * by;
+  double y = aw * by - ax * bz + ay * bw + az * bx;
+  double z = aw * bz + ax * by - ay * bx + az * bw;
+  if (w < 0) { w = -w; x = -x; y = -y; z = -z; }
+  d[0] = w; d[1] = x; d[2] = y; d[3] = z;
+  if constexpr (std::is_same<Real, double>::value) {
+    double s = sqrt(x * x + y * y + z * z), th = 2 * atan2(s, w);
+    double f = s > 1e-12 ? th / s : 2 / fmax(w, 1e-12);
+    psi[0] = f * x; psi[1] = f * y; psi[2] = f * z;
+    return th;
+  } else {
+    float xf = float(x), yf = float(y), zf = float(z), wf = float(w);
+    float s = sqrtf(xf * xf + yf * yf + zf * zf), th = 2.0f * atan2f(s, wf);
+    float f = s > 1e-12f ? th / s : 2.0f / fmaxf(wf, 1e-12f);
+    psi[0] = f * xf; psi[1] = f * yf; psi[2] = f * zf;
+    return th;
+  }
+}
+
+// R(d)ᵀ v of a relative rotation d (unit up to rounding)
+template <typename Real>
+__device__ __forceinline__ void rotT2(const double* d, const Real* v, Real* out) {
+  Real w = Real(d[0]), x = Real(d[1]), y = Real(d[2]), z = Real(d[3]);
+  Real n2 = w * w + x * x + y * y + z * z;
+  Real s = n2 > Real(0) ? Real(2) / n2 : Real(2);
+  Real R0 = 1 - s * (y * y + z * z), R1 = s * (x * y + z * w), R2 = s * (x * z - y * w);
+  Real R3 = s * (x * y - z * w), R4 = 1 - s * (x * x + z * z), R5 = s * (y * z + x * w);
+  Real R6 = s * (x * z + y * w), R7 = s * (y * z - x * w), R8 = 1 - s * (x * x + y * y);
+  out[0] = R0 * v[0] + R1 * v[1] + R2 * v[2];
+  out[1] = R3 * v[0] + R4 * v[1] + R5 * v[2];
+  out[2] = R6 * v[0] + R7 * v[1] + R8 * v[2];
+}
+
+// one segment's stretch / shear (CosseratRod.apply, rod_cosserat.js:118-133): world force into SF[i]; body torque
+// (Rᵀe) × (Rᵀn) and the section forces returned for an owned segment.  σ is formed in f64 (it cancels in f32).
+template <typename Real, int TILE>
+__device__ __forceinline__ void seg_force(StrandLDS<Real, TILE>& L, int i, double l0, Real EA, Real GA, Real* st, Real* nb,
+                                          double* eS, double* eSh) {
+  double R[9];
+  qframe_d(L.Qw[i], L.Qx[i], L.Qy[i], L.Qz[i], R);
+  double ex = L.Xx[i + 1] - L.Xx[i], ey = L.Xy[i + 1] - L.Xy[i], ez = L.Xz[i + 1] - L.Xz[i];
+  double b0 = R[0] * ex + R[1] * ey + R[2] * ez, b1 = R[3] * ex + R[4] * ey + R[5] * ez, b2 = R[6] * ex + R[7] * ey + R[8] * ez;
+  double il = 1.0 / l0;
+  double s1 = b0 * il, s2 = b1 * il, s3 = b2 * il - 1.0;
+  Real n1 = GA * Real(s1), n2 = GA * Real(s2), n3 = EA * Real(s3);
+  L.SFx[i] = n1 * Real(R[0]) + n2 * Real(R[3]) + n3 * Real(R[6]);
+  L.SFy[i] = n1 * Real(R[1]) + n2 * Real(R[4]) + n3 * Real(R[7]);
+  L.SFz[i] = n1 * Real(R[2]) + n2 * Real(R[5]) + n3 * Real(R[8]);
+  Real e0 = Real(b0), e1 = Real(b1), e2 = Real(b2);
+  st[0] = e1 * n3 - e2 * n2; st[1] = e2 * n1 - e0 * n3; st[2] = e0 * n2 - e1 * n1;
+  nb[0] = n1; nb[1] = n2; nb[2] = n3;
+  *eS = 0.5 * l0 * double(n3) * s3; *eSh = 0.5 * l0 * (double(n1) * s1 + double(n2) * s2);
+}
+
+// one joint (rod_cosserat.js:136-159) between LDS segments i and i+1: J on the first, −R(Δq)ᵀ J on the second, kDyn
+template <typename Real, int TILE>
+__device__ __forceinline__ void joint_torque(StrandLDS<Real, TILE>& L, int i, Real lb, const Real* om, Real Bx, Real By, Real Bz,
+                                             Real* mm, Real* d, Real* Jo) {
+  Real ps[3]; double dq[4];
+  Real th = relrot_R<Real>(L.Qw[i], L.Qx[i], L.Qy[i], L.Qz[i], L.Qw[i + 1], L.Qx[i + 1], L.Qy[i + 1], L.Qz[i + 1], ps, dq);
+  Real il = Real(1) / lb;
+  d[0] = ps[0] * il - om[0]; d[1] = ps[1] * il - om[1]; d[2] = ps[2] * il - om[2];
+  mm[0] = Bx * d[0]; mm[1] = By * d[1]; mm[2] = Bz * d[2];
+  Real q0 = ps[0] * ps[0], q1 = ps[1] * ps[1], q2 = ps[2] * ps[2], f = Real(0.25) * il;
+  L.KDx[i] = f * (By * q2 + Bz * q1); L.KDy[i] = f * (Bx * q2 + Bz * q0); L.KDz[i] = f * (Bx * q1 + By * q0);
+  Real J[3];
+  jlt(ps, th, mm, J);
+  L.JAx[i] = J[0]; L.JAy[i] = J[1]; L.JAz[i] = J[2];
+  Real Jb[3];
+  rotT2(dq, J, Jb);
+  L.JBx[i] = -Jb[0]; L.JBy[i] = -Jb[1]; L.JBz[i] = -Jb[2];
+  Jo[0] = J[0]; Jo[1] = J[1]; Jo[2] = J[2];
+}
+
+template <typename Real, int TILE, bool OUT>
+__device__ void strand_tile(const Inst<Real>& I, const TileItem tl, StrandLDS<Real, TILE>& L, int tile_local,
+                            double* outN, double* outV, double* outM, double* outCM, double* outR) {
+  const Ctl* ctl = I.ctl;
+  const int tid = threadIdx.x;
+  int cur = ctl->cur;
+  const bool rbk = ctl->rollback != 0;
+  if (OUT) cur = rbk ? cur : (cur ^ 1);            // the state the last iteration evaluated
+  const int iter = ctl->iter;
+  const int s = tl.strand;
+  const int noff = I.node_off[s], nn = I.node_off[s + 1] - noff;
+  const int soff = I.seg_off[s], nseg = I.seg_off[s + 1] - soff;
+  const int joff = I.joint_off[s], njs = I.joint_off[s + 1] - joff;
+  const bool closed = I.closed[s] != 0;
+  const int k0 = tl.k0, k1 = tl.k1, cnt = k1 - k0;
+  auto wrap = [](int v, int n) { int r = v % n; return r < 0 ? r + n : r; };
+  auto node_ok = [&](int k, int& kk) { if (closed) { kk = wrap(k, nn); return true; } kk = k; return k >= 0 && k < nn; };
+  auto seg_ok = [&](int j, int& jj) { if (closed) { if (nseg <= 0) { jj = 0; return false; } jj = wrap(j, nseg); return true; } jj = j; return j >= 0 && j < nseg; };
+  auto joint_ok = [&](int j, int& jj) { if (closed) { if (njs <= 0) { jj = 0; return false; } jj = wrap(j, njs); return true; } jj = j; return j >= 0 && j < njs; };
+
+  // ---------------------------------------------------------------- issue every independent load first
+  for (int h = tid; h < cnt + 2; h += TILE) {
+    int kk;
+    if (node_ok(k0 - 1 + h, kk)) { int i = noff + kk; L.Xx[h] = I.x[cur][i]; L.Xy[h] = I.y[cur][i]; L.Xz[h] = I.z[cur][i]; }
+    else { L.Xx[h] = L.Xy[h] = L.Xz[h] = 0.0; }
+    int jj;
+    if (seg_ok(k0 - 1 + h, jj)) { int j = soff + jj; L.Qw[h] = I.qw[cur][j]; L.Qx[h] = I.qx[cur][j]; L.Qy[h] = I.qy[cur][j]; L.Qz[h] = I.qz[cur][j]; }
+    else { L.Qw[h] = 1.0; L.Qx[h] = L.Qy[h] = L.Qz[h] = 0.0; }
+  }
+  const int t = tid;
+  const int k = k0 + t;
+  const bool ownNode = t < cnt;
+  int kkN = 0, jself = 0, jjo = 0, jjh = 0, jsh = 0;
+  if (ownNode) node_ok(k, kkN);
+  const bool ownSeg = ownNode && seg_ok(k, jself);
+  const bool ownJoint = ownNode && joint_ok(k, jjo);
+  const bool haloSeg = t == 0 && seg_ok(k0 - 1, jsh);
+  const bool haloJoint = t == 0 && joint_ok(k0 - 1, jjh);
+  const int i = noff + kkN, j = soff + jself;
+  // node parameters and velocity
+  uint8_t fx = 0; Real ms = 1, ldx = 0, ldy = 0, ldz = 0, llx = 0, lly = 0, llz = 0, vx = 0, vy = 0, vz = 0;
+  if (ownNode) {
+    fx = I.fix[i]; ms = I.mass[i]; ldx = I.lx[i]; ldy = I.ly[i]; ldz = I.lz[i];
+    if (I.npl) { llx = I.llx[i]; lly = I.lly[i]; llz = I.llz[i]; }
+    if (!OUT && !rbk) { vx = I.vx[i]; vy = I.vy[i]; vz = I.vz[i]; }
+  }
+  // own segment and joint parameters
+  double l0 = 1.0; Real EA = 0, GA = 0, Bx = 0, By = 0, Bz = 0, krx = 0, kry = 0, krz = 0, wx = 0, wy = 0, wz = 0;
+  if (ownSeg) {
+    l0 = I.l0[j]; EA = I.EA[j]; GA = I.GA[j]; krx = I.krx[j]; kry = I.kry[j]; krz = I.krz[j];
+    if (!OUT && !rbk) { wx = I.wx[j]; wy = I.wy[j]; wz = I.wz[j]; }
+  }
+  Real lb = 1, om[3] = {0, 0, 0};
+  if (ownJoint) {
+    const int jg = joff + jjo;
+    lb = I.jlen[jg]; om[0] = I.om0[3 * jg]; om[1] = I.om0[3 * jg + 1]; om[2] = I.om0[3 * jg + 2];
+    Bx = I.EIx[j]; By = I.EIy[j]; Bz = I.GJ[j];
+  }
+  // halo segment k0-1 and joint k0-1 (thread 0)
+  double hl0 = 1.0; Real hEA = 0, hGA = 0, hlb = 1, hom[3] = {0, 0, 0}, hBx = 0, hBy = 0, hBz = 0;
+  if (haloSeg) {
+    const int jh = soff + jsh;
+    hl0 = I.l0[jh]; hEA = I.EA[jh]; hGA = I.GA[jh];
+    hBx = I.EIx[jh]; hBy = I.EIy[jh]; hBz = I.GJ[jh];
+  }
+  if (haloJoint) {
+    const int jg = joff + jjh;
+    hlb = I.jlen[jg]; hom[0] = I.om0[3 * jg]; hom[1] = I.om0[3 * jg + 1]; hom[2] = I.om0[3 * jg + 2];
+  }
+  const int so = I.side_off[s], ns = I.side_off[s + 1] - so;
+  SideRec<Real> r0;
+  if (tid < ns) side_load(I.side + so + tid, r0);
+  __syncthreads();
+
+  // ---------------------------------------------------------------- segments and joints
+  double eS = 0, eSh = 0, eB = 0, eT = 0, mxN = 0, mxM = 0;
+  Real ST[3] = {0, 0, 0};
+  if (ownSeg) {
+    Real nb[3];
+    seg_force(L, t + 1, l0, EA, GA, ST, nb, &eS, &eSh);
+    mxN = fabs(double(nb[2]));
+    if (OUT) { outN[j] = double(nb[2]); outV[2 * j] = double(nb[0]); outV[2 * j + 1] = double(nb[1]); }
+  } else {
+    L.SFx[t + 1] = L.SFy[t + 1] = L.SFz[t + 1] = Real(0);
+  }
+  if (ownJoint) {
+    Real mm[3], d[3], J[3];
+    joint_torque(L, t + 1, lb, om, Bx, By, Bz, mm, d, J);
+    eB = 0.5 * double(lb) * (double(mm[0]) * double(d[0]) + double(mm[1]) * double(d[1]));
+    eT = 0.5 * double(lb) * double(mm[2]) * double(d[2]);
+    mxM = fmax(fabs(double(mm[0])), fmax(fabs(double(mm[1])), fabs(double(mm[2]))));
+    if (OUT) { const int jg = joff + jjo; outM[3 * jg] = double(mm[0]); outM[3 * jg + 1] = double(mm[1]); outM[3 * jg + 2] = double(mm[2]); }
+  } else {
+    L.JAx[t + 1] = L.JAy[t + 1] = L.JAz[t + 1] = Real(0);
+    L.JBx[t + 1] = L.JBy[t + 1] = L.JBz[t + 1] = Real(0);
+    L.KDx[t + 1] = L.KDy[t + 1] = L.KDz[t + 1] = Real(0);
+  }
+  if (t == 0) {
+    if (haloSeg) { Real st2[3], nb2[3]; double a1, a2; seg_force(L, 0, hl0, hEA, hGA, st2, nb2, &a1, &a2); }
+    else { L.SFx[0] = L.SFy[0] = L.SFz[0] = Real(0); }
+    if (haloJoint) { Real mm[3], d[3], J[3]; joint_torque(L, 0, hlb, hom, hBx, hBy, hBz, mm, d, J); }
+    else {
+      L.JAx[0] = L.JAy[0] = L.JAz[0] = Real(0); L.JBx[0] = L.JBy[0] = L.JBz[0] = Real(0); L.KDx[0] = L.KDy[0] = L.KDz[0] = Real(0);
+    }
+  }
+
+  // ---------------------------------------------------------------- crossing sides of this strand touching the tile,
+  // gathered in the strand's fixed side order (independent of the tiling)
+  Real Fc[3] = {0, 0, 0}, Tc[3] = {0, 0, 0}, kTw = 0;
+  const int lane = tid & 63, wv = tid >> 6;
+  const int km1 = closed ? wrap(k - 1, nseg > 0 ? nseg : 1) : k - 1;
+  for (int base = 0; base < ns; base += TILE) {
+    const int q = base + tid;
+    bool rel = false;
+    SideRec<Real> r = r0;
+    if (q < ns) {
+      if (base != 0) side_load(I.side + so + q, r);
+      if (closed) {
+        const int nz = nseg > 0 ? nseg : 1;
+        int dp = wrap(r.seg_pre - (k0 - 1), nz), dq = wrap(r.seg_post - k0, nz);
+        rel = dp < cnt + 1 || dq < cnt;
+      } else {
+        rel = (r.seg_pre >= k0 - 1 && r.seg_pre < k1) || (r.seg_post >= k0 && r.seg_post < k1);
+      }
+    }
+    unsigned long long mask = __ballot(rel);
+    int pos = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) L.wcnt[wv] = __popcll(mask);
+    __syncthreads();
+    int off = 0, tot = 0;
+    for (int w = 0; w < TILE / 64; w++) { if (w < wv) off += L.wcnt[w]; tot += L.wcnt[w]; }
+    if (rel) { side_store(r, L.rec + off + pos); L.rpre[off + pos] = r.seg_pre; L.rpost[off + pos] = r.seg_post; }
+    __syncthreads();
+    if (ownNode) {
+      const bool hasStart = closed || k < nseg, hasEnd = closed || k >= 1;
+      for (int g = 0; g < tot; g++) {
+        const int sp = L.rpre[g], sq = L.rpost[g];
+        const bool asA = hasStart && sp == k, asB = hasEnd && sp == km1;
+        if (asA | asB) {
+          const Real tt = L.rec[g].t;
+          const Real w = asA ? Real(1) - tt : tt;
+          Fc[0] += w * L.rec[g].F[0]; Fc[1] += w * L.rec[g].F[1]; Fc[2] += w * L.rec[g].F[2];
+          if (asA && asB) { Fc[0] += tt * L.rec[g].F[0]; Fc[1] += tt * L.rec[g].F[1]; Fc[2] += tt * L.rec[g].F[2]; }
+        }
+        if (ownSeg) {
+          if (sp == k) { Tc[0] += L.rec[g].T[0]; Tc[1] += L.rec[g].T[1]; Tc[2] += L.rec[g].T[2]; }
+          if (sq == k) kTw += L.rec[g].kth;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---------------------------------------------------------------- nodes: loads, reactions, anchors, v += F/m, x += v
+  double keN = 0, keR = 0, mxF = 0, mxT = 0, mxL = 0;
+  const bool warm = iter < I.warmup && I.contacts;
+  const Real damping = Real(I.damping);
+  if (ownNode) {
+    const bool hasEnd = closed || k >= 1, hasStart = closed || k < nseg;
+    Real F[3];
+    if (hasEnd && hasStart) { F[0] = L.SFx[t + 1] - L.SFx[t]; F[1] = L.SFy[t + 1] - L.SFy[t]; F[2] = L.SFz[t + 1] - L.SFz[t]; }
+    else if (hasStart) { F[0] = L.SFx[t + 1]; F[1] = L.SFy[t + 1]; F[2] = L.SFz[t + 1]; }
+    else if (hasEnd) { F[0] = -L.SFx[t]; F[1] = -L.SFy[t]; F[2] = -L.SFz[t]; }
+    else { F[0] = F[1] = F[2] = Real(0); }
+    Real ld[3] = {ldx, ldy, ldz};
+    if (I.npl && (llx != Real(0) || lly != Real(0) || llz != Real(0))) {
+      // mean node frame from the adjacent segments (nodeQuats: the starting segment is the reference)
+      const int is = hasStart ? t + 1 : t;
+      double acc[4] = {0, 0, 0, 0};
+      if (hasStart) { acc[0] += L.Qw[t + 1]; acc[1] += L.Qx[t + 1]; acc[2] += L.Qy[t + 1]; acc[3] += L.Qz[t + 1]; }
+      if (hasEnd) {
+        double dd = L.Qw[t] * L.Qw[is] + L.Qx[t] * L.Qx[is] + L.Qy[t] * L.Qy[is] + L.Qz[t] * L.Qz[is];
+        double sg = dd < 0 ? -1.0 : 1.0;
+        acc[0] += sg * L.Qw[t]; acc[1] += sg * L.Qx[t]; acc[2] += sg * L.Qy[t]; acc[3] += sg * L.Qz[t];
+      }
+      double Rn[9];
+      qframe_d(acc[0], acc[1], acc[2], acc[3], Rn);
+      for (int c = 0; c < 3; c++) ld[c] += Real(Rn[c]) * llx + Real(Rn[3 + c]) * lly + Real(Rn[6 + c]) * llz;
+    }
+    mxL = fmax(fabs(double(ld[0])), fmax(fabs(double(ld[1])), fabs(double(ld[2]))));
+    for (int c = 0; c < 3; c++) F[c] += ld[c] + Fc[c];
+    if (OUT) {
+      if (fx & 8) { outR[3 * i] = -double(F[0]); outR[3 * i + 1] = -double(F[1]); outR[3 * i + 2] = -double(F[2]); }
+      else { outR[3 * i] = outR[3 * i + 1] = outR[3 * i + 2] = 0.0; }
+    }
+    if (fx & 1) F[0] = Real(0);
+    if (fx & 2) F[1] = Real(0);
+    if (fx & 4) F[2] = Real(0);
+    mxF = sqrt(double(F[0]) * double(F[0]) + double(F[1]) * double(F[1]) + double(F[2]) * double(F[2]));
+    if (!OUT) {
+      Real V[3] = {vx, vy, vz};
+      if (warm) { V[0] = V[1] = V[2] = Real(0); F[0] = F[1] = F[2] = Real(0); }
+      const Real im = Real(1) / ms;
+      for (int c = 0; c < 3; c++) V[c] += F[c] * im;
+      if (damping != Real(1)) for (int c = 0; c < 3; c++) V[c] *= damping;
+      if (fx & 1) V[0] = Real(0);
+      if (fx & 2) V[1] = Real(0);
+      if (fx & 4) V[2] = Real(0);
+      const int nx = cur ^ 1;
+      I.x[nx][i] = L.Xx[t + 1] + double(V[0]);
+      I.y[nx][i] = L.Xy[t + 1] + double(V[1]);
+      I.z[nx][i] = L.Xz[t + 1] + double(V[2]);
+      I.vx[i] = V[0]; I.vy[i] = V[1]; I.vz[i] = V[2];
+      keN = double(ms) * (double(V[0]) * double(V[0]) + double(V[1]) * double(V[1]) + double(V[2]) * double(V[2]));
+    }
+  }
+  // ---------------------------------------------------------------- frames: body torque, inertia, ω, q ← q ⊗ exp(ω)
+  if (ownSeg) {
+    Real tm[3] = {ST[0] + L.JAx[t + 1] + L.JBx[t], ST[1] + L.JAy[t + 1] + L.JBy[t], ST[2] + L.JAz[t + 1] + L.JBz[t]};
+    if (I.C > 0) {
+      for (int e = I.clamp_off[j]; e < I.clamp_off[j + 1]; e++) {
+        const int cc = I.clamp_idx[e];
+        Real ps[3]; double dq[4];
+        Real th = relrot_R<Real>(I.c_Q[4 * cc], I.c_Q[4 * cc + 1], I.c_Q[4 * cc + 2], I.c_Q[4 * cc + 3],
+                                 L.Qw[t + 1], L.Qx[t + 1], L.Qy[t + 1], L.Qz[t + 1], ps, dq);
+        Real lc = I.c_len[cc];
+        Real d[3] = {ps[0] / lc, ps[1] / lc, ps[2] / lc};
+        Real mm[3] = {I.c_B[3 * cc] * d[0], I.c_B[3 * cc + 1] * d[1], I.c_B[3 * cc + 2] * d[2]};
+        Real J[3], Jb[3];
+        jlt(ps, th, mm, J);
+        rotT2(dq, J, Jb);
+        for (int c = 0; c < 3; c++) tm[c] -= Jb[c];
+        eB += 0.5 * double(lc) * (double(mm[0]) * double(d[0]) + double(mm[1]) * double(d[1]));
+        eT += 0.5 * double(lc) * double(mm[2]) * double(d[2]);
+        if (OUT) { outCM[2 * cc] = double(mm[0]); outCM[2 * cc + 1] = double(mm[1]); }
+      }
+    }
+    if (Tc[0] != Real(0) || Tc[1] != Real(0) || Tc[2] != Real(0)) {
+      double Rs[9];
+      qframe_d(L.Qw[t + 1], L.Qx[t + 1], L.Qy[t + 1], L.Qz[t + 1], Rs);
+      tm[0] += Real(Rs[0]) * Tc[0] + Real(Rs[1]) * Tc[1] + Real(Rs[2]) * Tc[2];
+      tm[1] += Real(Rs[3]) * Tc[0] + Real(Rs[4]) * Tc[1] + Real(Rs[5]) * Tc[2];
+      tm[2] += Real(Rs[6]) * Tc[0] + Real(Rs[7]) * Tc[1] + Real(Rs[8]) * Tc[2];
+    }
+    mxT = sqrt(double(tm[0]) * double(tm[0]) + double(tm[1]) * double(tm[1]) + double(tm[2]) * double(tm[2]));
+    if (!OUT) {
+      const Real msc = Real(I.mass_scale);
+      Real I3[3] = {msc * dmax(krx + kTw + (L.KDx[t] + L.KDx[t + 1]), Real(1e-12)),
+                    msc * dmax(kry + kTw + (L.KDy[t] + L.KDy[t + 1]), Real(1e-12)),
+                    msc * dmax(krz + kTw + (L.KDz[t] + L.KDz[t + 1]), Real(1e-12))};
+      Real W[3] = {wx + tm[0] / I3[0], wy + tm[1] / I3[1], wz + tm[2] / I3[2]};
+      W[2] = clampv(W[2], Real(-TWIST_RATE_MAX), Real(TWIST_RATE_MAX));
+      if (kTw > Real(0)) W[2] *= Real(ROLL_DAMP);
+      if (damping != Real(1)) for (int c = 0; c < 3; c++) W[c] *= damping;
+      I.wx[j] = W[0]; I.wy[j] = W[1]; I.wz[j] = W[2];
+      // q ⊗ exp(ω): the increment exp(ω) − 1 = (−2 sin²(a/4), ω sin(a/2)/a) carries the precision of the small
+      // rotation itself (the delta form of wgsl.js:962-980); the product and the renormalisation run in f64
+      double e0, e1, e2, e3;
+      if constexpr (std::is_same<Real, double>::value) {
+        double a = sqrt(W[0] * W[0] + W[1] * W[1] + W[2] * W[2]);
+        double sh = sin(0.25 * a), sc = a > 1e-12 ? sin(0.5 * a) / a : 0.5;
+        e0 = -2.0 * sh * sh; e1 = W[0] * sc; e2 = W[1] * sc; e3 = W[2] * sc;
+      } else {
+        float a = sqrtf(W[0] * W[0] + W[1] * W[1] + W[2] * W[2]);
+        float sh = sinf(0.25f * a), sc = a > 1e-12f ? sinf(0.5f * a) / a : 0.5f;
+        e0 = double(-2.0f * sh * sh); e1 = double(W[0] * sc); e2 = double(W[1] * sc); e3 = double(W[2] * sc);
+      }
+      const double qw = L.Qw[t + 1], qx = L.Qx[t + 1], qy = L.Qy[t + 1], qz = L.Qz[t + 1];
+      double rw = qw + (qw * e0 - qx * e1 - qy * e2 - qz * e3);
+      double rx = qx + (qw * e1 + qx * e0 + qy * e3 - qz * e2);
+      double ry = qy + (qw * e2 - qx * e3 + qy * e0 + qz * e1);
+      double rz = qz + (qw * e3 + qx * e2 - qy * e1 + qz * e0);
+      double n2 = rw * rw + rx * rx + ry * ry + rz * rz;
+      double inv = n2 > 0 ? 1.0 / sqrt(n2) : 1.0;
+      const int nx = cur ^ 1;
+      I.qw[nx][j] = rw * inv; I.qx[nx][j] = rx * inv; I.qy[nx][j] = ry * inv; I.qz[nx][j] = rz * inv;
+      keR = double(I3[0]) * double(W[0]) * double(W[0]) + double(I3[1]) * double(W[1]) * double(W[1]) + double(I3[2]) * double(W[2]) * double(W[2]);
+    }
+  }
+  if (OUT) return;
+
+  // ---------------------------------------------------------------- partial per wavefront (fixed order; reduced by k_ctrl):
+  // sums keN, keR, eS, eSh, eB, eT | maxima |F|, |tm|, |load|, |N|, |M|
+  double v[11] = {keN, keR, eS, eSh, eB, eT, mxF, mxT, mxL, mxN, mxM};
+#pragma unroll
+  for (int q = 0; q < 11; q++) {
+    double x = v[q];
+#pragma unroll
+    for (int o2 = 32; o2 > 0; o2 >>= 1) {
+      double y = __shfl_down(x, o2, 64);
+      x = q < 6 ? x + y : fmax(x, y);
+    }
+    v[q] = x;
+  }
+  if (lane == 0) {
+    double* sp = I.spart + (size_t(tile_local) * (TILE / 64) + wv) * SPART;
+#pragma unroll
+    for (int q = 0; q < 11; q++) sp[q] = v[q];
+  }
+}
+
+// fixed-order block reduction of NV doubles (the first nsum summed, the rest max-reduced); result in every thread
 template <int TILE, int NV>
 __device__ __forceinline__ void block_reduce(double (*red)[TILE / 64], double* v, int nsum) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -566,284 +959,6 @@ __device__ __forceinline__ void block_reduce(double (*red)[TILE / 64], double* v
   __syncthreads();
 }
 
-template <typename Real, int TILE, bool OUT>
-__device__ void strand_tile(const Inst<Real>& I, const TileItem tl, StrandLDS<Real, TILE>& L, int tile_local,
-                            double* outN, double* outV, double* outM, double* outCM, double* outR) {
-  Ctl* ctl = I.ctl;
-  const int tid = threadIdx.x;
-  int cur = ctl->cur;
-  const bool rbk = ctl->rollback != 0;
-  if (OUT) cur = rbk ? cur : (cur ^ 1);            // the state the last iteration evaluated
-  const int iter = ctl->iter;
-  const int s = tl.strand;
-  const int noff = I.node_off[s], nn = I.node_off[s + 1] - noff;
-  const int soff = I.seg_off[s], nseg = I.seg_off[s + 1] - soff;
-  const int joff = I.joint_off[s], njs = I.joint_off[s + 1] - joff;
-  const bool closed = I.closed[s] != 0;
-  const int k0 = tl.k0, k1 = tl.k1, cnt = k1 - k0;
-  auto node_ok = [&](int k, int& kk) { if (closed) { kk = ((k % nn) + nn) % nn; return true; } kk = k; return k >= 0 && k < nn; };
-  auto seg_ok = [&](int j, int& jj) { if (closed) { if (nseg <= 0) { jj = 0; return false; } jj = ((j % nseg) + nseg) % nseg; return true; } jj = j; return j >= 0 && j < nseg; };
-  auto joint_ok = [&](int j, int& jj) { if (closed) { if (njs <= 0) { jj = 0; return false; } jj = ((j % njs) + njs) % njs; return true; } jj = j; return j >= 0 && j < njs; };
-
-  // ---- stage positions and frames of the tile and its halo
-  for (int h = tid; h < cnt + 2; h += TILE) {
-    int kk;
-    if (node_ok(k0 - 1 + h, kk)) { int i = noff + kk; L.X[h][0] = I.x[cur][i]; L.X[h][1] = I.y[cur][i]; L.X[h][2] = I.z[cur][i]; }
-    else { L.X[h][0] = L.X[h][1] = L.X[h][2] = 0.0; }
-    int jj;
-    if (seg_ok(k0 - 1 + h, jj)) { int j = soff + jj; L.Q[h][0] = I.qw[cur][j]; L.Q[h][1] = I.qx[cur][j]; L.Q[h][2] = I.qy[cur][j]; L.Q[h][3] = I.qz[cur][j]; }
-    else { L.Q[h][0] = 1.0; L.Q[h][1] = L.Q[h][2] = L.Q[h][3] = 0.0; }
-  }
-  __syncthreads();
-
-  double eS = 0, eSh = 0, eB = 0, eT = 0, mxN = 0, mxM = 0;
-  // ---- segment stretch / shear (CosseratRod.apply :118-133) and joints (:136-159), k0-1+t for t in [0, cnt]
-  for (int t = tid; t < cnt + 1; t += TILE) {
-    int jj;
-    const int j = k0 - 1 + t;
-    if (seg_ok(j, jj)) {
-      const int sg = soff + jj;
-      double Rd[9];
-      {
-        double w = L.Q[t][0], x = L.Q[t][1], y = L.Q[t][2], z = L.Q[t][3];
-        double nq = sqrt(w * w + x * x + y * y + z * z); if (nq == 0) nq = 1.0;
-        w /= nq; x /= nq; y /= nq; z /= nq;
-        Rd[0] = 1 - 2 * (y * y + z * z); Rd[1] = 2 * (x * y + z * w); Rd[2] = 2 * (x * z - y * w);
-        Rd[3] = 2 * (x * y - z * w); Rd[4] = 1 - 2 * (x * x + z * z); Rd[5] = 2 * (y * z + x * w);
-        Rd[6] = 2 * (x * z + y * w); Rd[7] = 2 * (y * z - x * w); Rd[8] = 1 - 2 * (x * x + y * y);
-      }
-      double ex = L.X[t + 1][0] - L.X[t][0], ey = L.X[t + 1][1] - L.X[t][1], ez = L.X[t + 1][2] - L.X[t][2];
-      double l0 = I.l0[sg];
-      double b0 = Rd[0] * ex + Rd[1] * ey + Rd[2] * ez, b1 = Rd[3] * ex + Rd[4] * ey + Rd[5] * ez, b2 = Rd[6] * ex + Rd[7] * ey + Rd[8] * ez;
-      double s1 = b0 / l0, s2 = b1 / l0, s3 = b2 / l0 - 1.0;           // σ in f64: the strain cancels in f32
-      Real GA = I.GA[sg], EA = I.EA[sg];
-      Real n1 = GA * Real(s1), n2 = GA * Real(s2), n3 = EA * Real(s3);
-      L.SF[t][0] = n1 * Real(Rd[0]) + n2 * Real(Rd[3]) + n3 * Real(Rd[6]);
-      L.SF[t][1] = n1 * Real(Rd[1]) + n2 * Real(Rd[4]) + n3 * Real(Rd[7]);
-      L.SF[t][2] = n1 * Real(Rd[2]) + n2 * Real(Rd[5]) + n3 * Real(Rd[8]);
-      if (t >= 1) {   // owned segment k0 + (t-1)
-        Real e0 = Real(b0), e1 = Real(b1), e2 = Real(b2);
-        L.ST[t - 1][0] = e1 * n3 - e2 * n2; L.ST[t - 1][1] = e2 * n1 - e0 * n3; L.ST[t - 1][2] = e0 * n2 - e1 * n1;
-        eS += 0.5 * l0 * double(n3) * s3; eSh += 0.5 * l0 * (double(n1) * s1 + double(n2) * s2);
-        mxN = fmax(mxN, fabs(double(n3)));
-        if (OUT) { outN[sg] = double(n3); outV[2 * sg] = double(n1); outV[2 * sg + 1] = double(n2); }
-      }
-    } else {
-      L.SF[t][0] = L.SF[t][1] = L.SF[t][2] = Real(0);
-      if (t >= 1) L.ST[t - 1][0] = L.ST[t - 1][1] = L.ST[t - 1][2] = Real(0);
-    }
-    int ji;
-    if (joint_ok(j, ji)) {
-      const int jg = joff + ji;
-      const int sa = soff + (closed ? ((j % nseg) + nseg) % nseg : j);
-      double psi[3], dq[4];
-      double th = relrot(L.Q[t], L.Q[t + 1], psi, dq);
-      Real lb = I.jlen[jg];
-      Real ps[3] = {Real(psi[0]), Real(psi[1]), Real(psi[2])};
-      Real d0 = ps[0] / lb - I.om0[3 * jg], d1 = ps[1] / lb - I.om0[3 * jg + 1], d2 = ps[2] / lb - I.om0[3 * jg + 2];
-      Real Bx = I.EIx[sa], By = I.EIy[sa], Bz = I.GJ[sa];
-      Real mm[3] = {Bx * d0, By * d1, Bz * d2};
-      Real q0 = ps[0] * ps[0], q1 = ps[1] * ps[1], q2 = ps[2] * ps[2], f = Real(0.25) / lb;
-      L.KD[t][0] = f * (By * q2 + Bz * q1); L.KD[t][1] = f * (Bx * q2 + Bz * q0); L.KD[t][2] = f * (Bx * q1 + By * q0);
-      Real J[3];
-      jlt(ps, Real(th), mm, J);
-      L.JA[t][0] = J[0]; L.JA[t][1] = J[1]; L.JA[t][2] = J[2];
-      Real Jb[3];
-      rotT(dq, J, Jb);
-      L.JB[t][0] = -Jb[0]; L.JB[t][1] = -Jb[1]; L.JB[t][2] = -Jb[2];
-      if (t >= 1) {   // owned joint (its first segment is owned)
-        eB += 0.5 * double(lb) * (double(mm[0]) * double(d0) + double(mm[1]) * double(d1));
-        eT += 0.5 * double(lb) * double(mm[2]) * double(d2);
-        mxM = fmax(mxM, fmax(fabs(double(mm[0])), fmax(fabs(double(mm[1])), fabs(double(mm[2])))));
-        if (OUT) { outM[3 * jg] = double(mm[0]); outM[3 * jg + 1] = double(mm[1]); outM[3 * jg + 2] = double(mm[2]); }
-      }
-    } else {
-      for (int k = 0; k < 3; k++) { L.JA[t][k] = Real(0); L.JB[t][k] = Real(0); L.KD[t][k] = Real(0); }
-    }
-  }
-  __syncthreads();
-
-  // ---- gather the crossing sides of this strand that touch the tile, in the strand's fixed side order
-  const int t = tid;
-  const int k = k0 + t;
-  const bool ownNode = t < cnt;
-  int jself;
-  const bool ownSeg = ownNode && seg_ok(k, jself) && (closed || k < nseg);
-  Real Fc[3] = {0, 0, 0}, Tc[3] = {0, 0, 0}, kTw = 0;
-  const int so = I.side_off[s], ns = I.side_off[s + 1] - so;
-  const int lane = tid & 63, wv = tid >> 6;
-  auto modn = [&](int v) { return (closed && nseg > 0) ? ((v % nseg) + nseg) % nseg : v; };
-  for (int base = 0; base < ns; base += TILE) {
-    const int i = base + tid;
-    bool rel = false;
-    SideRec<Real> r;
-    if (i < ns) {
-      r = I.side[so + i];
-      if (closed) {
-        // membership by cyclic distance from k0-1 (the tile covers at most the whole loop)
-        const int nz = nseg > 0 ? nseg : 1;
-        int dp = ((r.seg_pre - (k0 - 1)) % nz + nz) % nz, dq = ((r.seg_post - k0) % nz + nz) % nz;
-        rel = dp < cnt + 1 || dq < cnt;
-      } else {
-        rel = (r.seg_pre >= k0 - 1 && r.seg_pre < k1) || (r.seg_post >= k0 && r.seg_post < k1);
-      }
-    }
-    unsigned long long mask = __ballot(rel);
-    int pos = __popcll(mask & ((1ull << lane) - 1ull));
-    if (lane == 0) L.wcnt[wv] = __popcll(mask);
-    __syncthreads();
-    int off = 0, tot = 0;
-    for (int w = 0; w < TILE / 64; w++) { if (w < wv) off += L.wcnt[w]; tot += L.wcnt[w]; }
-    if (rel) L.rec[off + pos] = r;
-    __syncthreads();
-    if (ownNode) {
-      const int km1 = modn(k - 1);
-      for (int q = 0; q < tot; q++) {
-        const SideRec<Real>& g = L.rec[q];
-        if (g.seg_pre == k && (closed || k < nseg)) { Real w = Real(1) - g.t; Fc[0] += w * g.F[0]; Fc[1] += w * g.F[1]; Fc[2] += w * g.F[2]; }
-        if ((closed || k >= 1) && g.seg_pre == km1) { Real w = g.t; Fc[0] += w * g.F[0]; Fc[1] += w * g.F[1]; Fc[2] += w * g.F[2]; }
-        if (ownSeg) {
-          if (g.seg_pre == k) { Tc[0] += g.T[0]; Tc[1] += g.T[1]; Tc[2] += g.T[2]; }
-          if (g.seg_post == k) kTw += g.kth;
-        }
-      }
-    }
-    __syncthreads();
-  }
-
-  // ---- nodes: loads, reactions, anchors, v += F/m, x += v (solver_cpu.js:753-782)
-  double keN = 0, keR = 0, mxF = 0, mxT = 0, mxL = 0;
-  const bool warm = iter < I.warmup && I.contacts;
-  const Real damping = Real(I.damping);
-  if (ownNode) {
-    int kk; node_ok(k, kk);
-    const int i = noff + kk;
-    const bool hasEnd = closed || k >= 1, hasStart = closed || k < nseg;
-    Real F[3];
-    for (int c = 0; c < 3; c++) {
-      if (hasEnd && hasStart) F[c] = L.SF[t + 1][c] - L.SF[t][c];
-      else if (hasStart) F[c] = L.SF[t + 1][c];
-      else if (hasEnd) F[c] = -L.SF[t][c];
-      else F[c] = Real(0);
-    }
-    Real ld[3] = {I.lx[i], I.ly[i], I.lz[i]};
-    if (I.npl) {
-      Real lx = I.llx[i], ly = I.lly[i], lz = I.llz[i];
-      if (lx != Real(0) || ly != Real(0) || lz != Real(0)) {
-        // mean node frame from the adjacent segments (nodeQuats: the starting segment is the reference)
-        const double* qs = hasStart ? L.Q[t + 1] : L.Q[t];
-        double acc[4] = {0, 0, 0, 0};
-        for (int pass = 0; pass < 2; pass++) {
-          if (pass == 0 && !hasStart) continue;
-          if (pass == 1 && !hasEnd) continue;
-          const double* q = pass == 0 ? L.Q[t + 1] : L.Q[t];
-          double d = q[0] * qs[0] + q[1] * qs[1] + q[2] * qs[2] + q[3] * qs[3];
-          double sg = d < 0 ? -1.0 : 1.0;
-          for (int c = 0; c < 4; c++) acc[c] += sg * q[c];
-        }
-        Real Rn[9];
-        qframe<Real>(acc[0], acc[1], acc[2], acc[3], Rn);
-        for (int c = 0; c < 3; c++) ld[c] += Rn[c] * lx + Rn[3 + c] * ly + Rn[6 + c] * lz;
-      }
-    }
-    mxL = fmax(fabs(double(ld[0])), fmax(fabs(double(ld[1])), fabs(double(ld[2]))));
-    for (int c = 0; c < 3; c++) F[c] += ld[c];
-    for (int c = 0; c < 3; c++) F[c] += Fc[c];
-    const uint8_t fx = I.fix[i];
-    if (OUT) {
-      if (fx & 8) { outR[3 * i] = -double(F[0]); outR[3 * i + 1] = -double(F[1]); outR[3 * i + 2] = -double(F[2]); }
-      else { outR[3 * i] = outR[3 * i + 1] = outR[3 * i + 2] = 0.0; }
-    }
-    if (fx & 1) F[0] = Real(0);
-    if (fx & 2) F[1] = Real(0);
-    if (fx & 4) F[2] = Real(0);
-    mxF = sqrt(double(F[0]) * double(F[0]) + double(F[1]) * double(F[1]) + double(F[2]) * double(F[2]));
-    if (!OUT) {
-      Real V[3];
-      if (rbk || warm) { V[0] = V[1] = V[2] = Real(0); } else { V[0] = I.vx[i]; V[1] = I.vy[i]; V[2] = I.vz[i]; }
-      if (warm) { F[0] = F[1] = F[2] = Real(0); }
-      const Real ms = I.mass[i];
-      for (int c = 0; c < 3; c++) V[c] += F[c] / ms;
-      if (damping != Real(1)) for (int c = 0; c < 3; c++) V[c] *= damping;
-      if (fx & 1) V[0] = Real(0);
-      if (fx & 2) V[1] = Real(0);
-      if (fx & 4) V[2] = Real(0);
-      const int nx = cur ^ 1;
-      I.x[nx][i] = L.X[t + 1][0] + double(V[0]);
-      I.y[nx][i] = L.X[t + 1][1] + double(V[1]);
-      I.z[nx][i] = L.X[t + 1][2] + double(V[2]);
-      I.vx[i] = V[0]; I.vy[i] = V[1]; I.vz[i] = V[2];
-      keN = double(ms) * (double(V[0]) * double(V[0]) + double(V[1]) * double(V[1]) + double(V[2]) * double(V[2]));
-    }
-  }
-  // ---- frames: body torque, inertia, ω += tm / I, roll limit and damping, q ← q ⊗ exp(ω) (:766-808)
-  if (ownSeg) {
-    const int j = soff + jself;
-    Real tm[3] = {L.ST[t][0], L.ST[t][1], L.ST[t][2]};
-    int jx;
-    if (joint_ok(k, jx)) for (int c = 0; c < 3; c++) tm[c] += L.JA[t + 1][c];
-    if (joint_ok(k - 1, jx)) for (int c = 0; c < 3; c++) tm[c] += L.JB[t][c];
-    if (I.C > 0) {
-      for (int e = I.clamp_off[j]; e < I.clamp_off[j + 1]; e++) {
-        const int cc = I.clamp_idx[e];
-        double cq[4] = {I.c_Q[4 * cc], I.c_Q[4 * cc + 1], I.c_Q[4 * cc + 2], I.c_Q[4 * cc + 3]};
-        double psi[3], dq[4];
-        double th = relrot(cq, L.Q[t + 1], psi, dq);
-        Real lc = I.c_len[cc];
-        Real ps[3] = {Real(psi[0]), Real(psi[1]), Real(psi[2])};
-        Real d[3] = {ps[0] / lc, ps[1] / lc, ps[2] / lc};
-        Real mm[3] = {I.c_B[3 * cc] * d[0], I.c_B[3 * cc + 1] * d[1], I.c_B[3 * cc + 2] * d[2]};
-        Real J[3], Jb[3];
-        jlt(ps, Real(th), mm, J);
-        rotT(dq, J, Jb);
-        for (int c = 0; c < 3; c++) tm[c] -= Jb[c];
-        eB += 0.5 * double(lc) * (double(mm[0]) * double(d[0]) + double(mm[1]) * double(d[1]));
-        eT += 0.5 * double(lc) * double(mm[2]) * double(d[2]);
-        if (OUT) { outCM[2 * cc] = double(mm[0]); outCM[2 * cc + 1] = double(mm[1]); }
-      }
-    }
-    Real Rs[9];
-    qframe<Real>(L.Q[t + 1][0], L.Q[t + 1][1], L.Q[t + 1][2], L.Q[t + 1][3], Rs);
-    tm[0] += Rs[0] * Tc[0] + Rs[1] * Tc[1] + Rs[2] * Tc[2];
-    tm[1] += Rs[3] * Tc[0] + Rs[4] * Tc[1] + Rs[5] * Tc[2];
-    tm[2] += Rs[6] * Tc[0] + Rs[7] * Tc[1] + Rs[8] * Tc[2];
-    mxT = sqrt(double(tm[0]) * double(tm[0]) + double(tm[1]) * double(tm[1]) + double(tm[2]) * double(tm[2]));
-    if (!OUT) {
-      const Real msc = Real(I.mass_scale);
-      Real kd[3] = {L.KD[t][0] + L.KD[t + 1][0], L.KD[t][1] + L.KD[t + 1][1], L.KD[t][2] + L.KD[t + 1][2]};
-      Real I3[3] = {msc * dmax(I.krx[j] + kTw + kd[0], Real(1e-12)), msc * dmax(I.kry[j] + kTw + kd[1], Real(1e-12)),
-                    msc * dmax(I.krz[j] + kTw + kd[2], Real(1e-12))};
-      Real W[3];
-      if (rbk) { W[0] = W[1] = W[2] = Real(0); } else { W[0] = I.wx[j]; W[1] = I.wy[j]; W[2] = I.wz[j]; }
-      for (int c = 0; c < 3; c++) W[c] += tm[c] / I3[c];
-      W[2] = clampv(W[2], Real(-TWIST_RATE_MAX), Real(TWIST_RATE_MAX));
-      if (kTw > Real(0)) W[2] *= Real(ROLL_DAMP);
-      if (damping != Real(1)) for (int c = 0; c < 3; c++) W[c] *= damping;
-      I.wx[j] = W[0]; I.wy[j] = W[1]; I.wz[j] = W[2];
-      // q ⊗ exp(ω) in f64 (quat.js qexp / qmul), normalised
-      double wxd = W[0], wyd = W[1], wzd = W[2];
-      double a = sqrt(wxd * wxd + wyd * wyd + wzd * wzd), half = 0.5 * a;
-      double sc = a > 1e-12 ? sin(half) / a : 0.5;
-      double dw = cos(half), dx = wxd * sc, dy = wyd * sc, dz = wzd * sc;
-      const double* q = L.Q[t + 1];
-      double rw = q[0] * dw - q[1] * dx - q[2] * dy - q[3] * dz;
-      double rx = q[0] * dx + q[1] * dw + q[2] * dz - q[3] * dy;
-      double ry = q[0] * dy - q[1] * dz + q[2] * dw + q[3] * dx;
-      double rz = q[0] * dz + q[1] * dy - q[2] * dx + q[3] * dw;
-      double nq = sqrt(rw * rw + rx * rx + ry * ry + rz * rz); if (nq == 0) nq = 1.0;
-      const int nx = cur ^ 1;
-      I.qw[nx][j] = rw / nq; I.qx[nx][j] = rx / nq; I.qy[nx][j] = ry / nq; I.qz[nx][j] = rz / nq;
-      keR = double(I3[0]) * double(W[0]) * double(W[0]) + double(I3[1]) * double(W[1]) * double(W[1]) + double(I3[2]) * double(W[2]) * double(W[2]);
-    }
-  }
-  if (OUT) return;
-
-  // ---- tile partial: sums keN, keR, eS, eSh, eB, eT | maxima |F|, |tm|, |load|, |N|, |M| (reduced by k_ctrl)
-  double v[11] = {keN, keR, eS, eSh, eB, eT, mxF, mxT, mxL, mxN, mxM};
-  block_reduce<TILE, 11>(L.red, v, 6);
-  if (tid < 11) I.spart[size_t(tile_local) * SPART + tid] = v[tid];
-}
-
 // ================================================================================================= k_ctrl
 // one workgroup per instance: fixed-order reduction of the tile and crossing-block partials, then kinetic damping,
 // the residual, the stop test and the history ring (solver_cpu.js:809-837; wgsl.js:990-1047)
@@ -859,7 +974,7 @@ __global__ __launch_bounds__(256) void k_ctrl(const Inst<Real>* __restrict__ ins
   double s12[13];
   for (int q = 0; q < 7; q++) s12[q] = 0.0;
   for (int q = 7; q < 13; q++) s12[q] = q == 12 ? -INFINITY : 0.0;
-  for (int w = tid; w < I.ntiles; w += 256) {
+  for (int w = tid; w < I.ntiles * I.tile_waves; w += 256) {
     const double* sp = I.spart + size_t(w) * SPART;
     double t0 = sp[0], t1 = sp[1], t2 = sp[2], t3 = sp[3], t4 = sp[4], t5 = sp[5];
     double m0 = sp[6], m1 = sp[7], m2 = sp[8], m3 = sp[9], m4 = sp[10];
@@ -944,6 +1059,11 @@ __global__ __launch_bounds__(TILE) void k_outputs(const Inst<Real>* __restrict__
   if (blockIdx.x >= I.ntiles) return;
   TileItem tl = tiles[lt];
   strand_tile<Real, TILE, true>(I, tl, L, blockIdx.x, outN, outV, outM, outCM, outR);
+}
+
+// plain copy of n 16-byte words, grid-stride (HBM counter calibration)
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) dst[i] = src[i];
 }
 
 // explicit instantiations

@@ -23,6 +23,7 @@ template <typename Real> __global__ void k_begin(const Inst<Real>*, const int32_
 template <typename Real> __global__ void k_cross(const Inst<Real>*, const CrossItem*, int32_t);
 template <typename Real, int TILE> __global__ void k_strand(const Inst<Real>*, const TileItem*, int32_t);
 template <typename Real> __global__ void k_ctrl(const Inst<Real>*, int32_t);
+__global__ void k_copy(const uint4*, uint4*, int64_t);
 template <typename Real, int TILE> __global__ void k_outputs(const Inst<Real>*, const TileItem*, int32_t, double*, double*, double*, double*, double*);
 }  // namespace wsd
 
@@ -100,7 +101,7 @@ std::vector<Real> toReal(const double* p, size_t n) {
 
 // build the device image of one host model (GpuSolver.upload, solver_gpu.js:191-310)
 template <typename Real>
-void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, int tile, int hist_len) {
+void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, int tile, int hist_len, Ctl* ctl) {
   const int n = md.n, m = md.m, p = md.p, C = md.C, S = md.S;
   const int K = md.par.contacts ? md.K : 0;   // no contact goal: no crossings on the device
   if (n <= 0 || m < 0 || S <= 0) throw Fail{WS_ERR_ARG, "model has no nodes or strands"};
@@ -156,8 +157,9 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   A.add(d.dent, 2 * K); A.add(d.climb, 2 * K); A.add(d.tanpsi, 2 * K); A.add(d.dente, K);
   A.add(d.theta, K); A.add(d.anom, K); A.add(d.pres, K); A.add(d.util, K); A.add(d.phi, 2 * K); A.add(d.mid, 3 * K);
   A.add(d.ctype, K); A.add(d.face, K); A.add(d.side, 2 * K);
-  A.add(d.ctl, 1); A.add(d.hist, size_t(std::max(hist_len, 1)) * HIST_STRIDE);
-  A.add(d.spart, size_t(d.ntiles) * SPART); A.add(d.cpart, size_t(std::max(H.ncblk, 1)) * CPART);
+  d.ctl = ctl; A.add(d.hist, size_t(std::max(hist_len, 1)) * HIST_STRIDE);
+  d.tile_waves = tile / 64;
+  A.add(d.spart, size_t(d.ntiles) * d.tile_waves * SPART); A.add(d.cpart, size_t(std::max(H.ncblk, 1)) * CPART);
   A.alloc();
   // ---- topology
   h2d(d.node_off, std::vector<int32_t>(md.node_offset, md.node_offset + S + 1));
@@ -309,6 +311,8 @@ struct Batch : BatchBase {
   Inst<Real>* d_inst = nullptr;
   TileItem* d_tiles = nullptr; int ntiles = 0;
   CrossItem* d_cross = nullptr; int ncross = 0;
+  Ctl* d_ctls = nullptr;            // [B] control records
+  std::vector<Ctl> h_ctls;
   std::vector<TileItem> tiles;
 
   ~Batch() override {
@@ -318,6 +322,7 @@ struct Batch : BatchBase {
     if (d_tiles) (void)hipFree(d_tiles);
     if (d_cross) (void)hipFree(d_cross);
     if (d_sel) (void)hipFree(d_sel);
+    if (d_ctls) (void)hipFree(d_ctls);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -327,9 +332,12 @@ struct Batch : BatchBase {
     B = nB;
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&ev0)); HIPCHK(hipEventCreate(&ev1));
+    // the control records of all instances sit side by side: one copy reads every instance's state
+    HIPCHK(hipMalloc(&d_ctls, sizeof(Ctl) * std::max(B, 1)));
+    HIPCHK(hipMemset(d_ctls, 0, sizeof(Ctl) * std::max(B, 1)));
     for (int b = 0; b < B; b++) {
       inst.emplace_back(new InstHost<Real>());
-      build_instance(*inst.back(), *models[b], *states[b], tile, hist_len);
+      build_instance(*inst.back(), *models[b], *states[b], tile, hist_len, d_ctls + b);
       upload_state(*inst.back(), *states[b]);
     }
     relist();
@@ -385,10 +393,15 @@ struct Batch : BatchBase {
     HIPCHK(hipGraphDestroy(g));
   }
 
+  void fetch_ctls() {
+    h_ctls.resize(std::max(B, 1));
+    HIPCHK(hipMemcpy(h_ctls.data(), d_ctls, sizeof(Ctl) * B, hipMemcpyDeviceToHost));
+  }
+
   bool all_done(const int32_t* sel, int nsel) {
+    fetch_ctls();
     for (int q = 0; q < nsel; q++) {
-      Ctl c;
-      HIPCHK(hipMemcpy(&c, inst[sel[q]]->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+      const Ctl& c = h_ctls[sel[q]];
       if (c.done == 0 && c.iter < c.stop_at) return false;
     }
     return true;
@@ -459,11 +472,8 @@ struct Batch : BatchBase {
 
   int read_control(ws_control* out) override {
     HIPCHK(hipStreamSynchronize(stream));
-    for (int b = 0; b < B; b++) {
-      Ctl c;
-      HIPCHK(hipMemcpy(&c, inst[b]->d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
-      out[b] = to_control(c);
-    }
+    fetch_ctls();
+    for (int b = 0; b < B; b++) out[b] = to_control(h_ctls[b]);
     return WS_OK;
   }
 
@@ -566,7 +576,7 @@ struct Batch : BatchBase {
     HIPCHK(hipStreamSynchronize(stream));
     H.arena.release();
     H.arena = Arena();
-    build_instance(H, md, st, tile, hist_len);
+    build_instance(H, md, st, tile, hist_len, d_ctls + b);
     upload_state(H, st);
     relist();
     return WS_OK;
@@ -681,6 +691,31 @@ WS_API int ws_device_info(ws_ctx* ctx, char* name, int32_t cap, int64_t* hbm, in
   if (hbm) *hbm = int64_t(ctx->prop.totalGlobalMem);
   if (ncu) *ncu = ctx->prop.multiProcessorCount;
   return WS_OK;
+}
+
+WS_API int ws_copy_probe(ws_ctx* ctx, int64_t bytes, int32_t reps, double* ms) {
+  if (!ctx || bytes < 16 || reps <= 0 || !ms) return WS_ERR_ARG;
+  try {
+    HIPCHK(hipSetDevice(ctx->device));
+    int64_t n = bytes / 16;
+    uint4 *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, n * 16)); HIPCHK(hipMalloc(&b, n * 16));
+    HIPCHK(hipMemset(a, 1, n * 16));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
+    int grid = int(std::min<int64_t>((n + 255) / 256, 256 * 64));
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n);
+    HIPCHK(hipEventRecord(e0, 0));
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n);
+    HIPCHK(hipEventRecord(e1, 0));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    *ms = double(t) / reps;
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+    (void)hipFree(a); (void)hipFree(b);
+    return WS_OK;
+  } catch (const Fail& f) { return fail(ctx, f); }
 }
 
 WS_API int ws_batch_create(ws_ctx* ctx, const ws_model* const* models, const ws_state* const* states, int32_t B,

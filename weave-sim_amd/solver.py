@@ -38,6 +38,12 @@ class Context:
         self.check(self.lib.ws_device_info(self.h, name, 256, C.byref(hbm), C.byref(ncu)))
         return {"name": name.value.decode(), "hbm_bytes": hbm.value, "cu": ncu.value}
 
+    def copy_probe(self, nbytes, reps=20):
+        """mean ms of a plain copy kernel over nbytes (read once, written once)"""
+        ms = C.c_double()
+        self.check(self.lib.ws_copy_probe(self.h, int(nbytes), int(reps), C.byref(ms)))
+        return ms.value
+
     def __del__(self):
         try:
             if self.h:
