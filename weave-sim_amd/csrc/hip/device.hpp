@@ -13,9 +13,11 @@
 namespace wsd {
 
 constexpr int HIST_STRIDE = 9;   // history ring entry: ke, strain, bending, torsion, stretch, shear, contact, residual, iteration
-constexpr int SPART = 12;        // doubles per strand-tile partial
-constexpr int CPART = 4;         // doubles per crossing-block partial
-constexpr int CROSS_BLOCK = 64;  // crossings per crossing-kernel block
+constexpr int SPART = 11;        // doubles per strand wavefront partial (stored value-major)
+constexpr int CPART = 2;         // doubles per crossing-block partial (stored value-major)
+constexpr int CTRL_THREADS = 256; // lanes of the control kernel's workgroup
+constexpr int CROSS_THREADS = 64;                 // lanes per crossing-kernel block (two per crossing)
+constexpr int CROSS_BLOCK = CROSS_THREADS / 2;    // crossings per crossing-kernel block
 constexpr int MAX_REL = 1024;    // crossing sides a strand tile may gather (checked on the host per strand)
 
 // instance control record, device-written (the reference's CT struct, wgsl.js:154-172)
@@ -101,11 +103,18 @@ struct Inst {
   // control
   Ctl* ctl;
   double* hist;              // [hist_len * HIST_STRIDE]
-  double* spart;             // [ntiles * SPART]
-  double* cpart;             // [ncblk * CPART]
+  double* spart;             // [SPART][ntiles * tile_waves]
+  double* cpart;             // [CPART][ncblk]
 };
 
-struct TileItem { int32_t inst, strand, k0, k1; };   // strand-local node range [k0, k1)
+// one strand tile: strand-local node range [k0, k1) plus the strand's layout, so that a tile reads its geometry in
+// one 64-byte load
+struct TileItem {
+  int32_t inst, strand, k0, k1;
+  int32_t noff, nn, soff, nseg;      // first node / node count, first segment / segment count of the strand
+  int32_t joff, njs, so, ns;         // first joint / joint count, first side record / side count
+  int32_t closed, tile_local, pad0, pad1;
+};
 struct CrossItem { int32_t inst, c0, count, blk; };  // crossings [c0, c0 + count) of one instance, block index in the instance
 
 }  // namespace wsd

@@ -140,6 +140,18 @@ __global__ void k_begin(const Inst<Real>* insts, const int32_t* sel, int32_t nse
   if (force) c->done = 0;
 }
 
+// field-wise copies of a side record (keeps the record in registers: no aggregate copy through private memory)
+template <typename Real>
+__device__ __forceinline__ void side_load(const SideRec<Real>* __restrict__ p, SideRec<Real>& r) {
+  r.seg_pre = p->seg_pre; r.seg_post = p->seg_post; r.t = p->t; r.kth = p->kth;
+  r.F[0] = p->F[0]; r.F[1] = p->F[1]; r.F[2] = p->F[2]; r.T[0] = p->T[0]; r.T[1] = p->T[1]; r.T[2] = p->T[2];
+}
+template <typename Real>
+__device__ __forceinline__ void side_store(const SideRec<Real>& r, SideRec<Real>* p) {
+  p->seg_pre = r.seg_pre; p->seg_post = r.seg_post; p->t = r.t; p->kth = r.kth;
+  p->F[0] = r.F[0]; p->F[1] = r.F[1]; p->F[2] = r.F[2]; p->T[0] = r.T[0]; p->T[1] = r.T[1]; p->T[2] = r.T[2];
+}
+
 // ================================================================================================= k_cross
 template <typename Real>
 struct SideEv {
@@ -246,25 +258,27 @@ __device__ __forceinline__ void eval_side(const Inst<Real>& I, int cur, int c, i
   E.L = L; E.t = t;
 }
 
-// half-extent h(n) and smoothed support point r of a side's section facing sign·n (solver_cpu.js:274-282)
-template <typename Real>
-__device__ __forceinline__ Real support(const Real* R, const Real* n, Real sign, Real w, Real h_, Real* r) {
-  Real nx = (n[0] * R[0] + n[1] * R[1] + n[2] * R[2]) * sign, ny = (n[0] * R[3] + n[1] * R[4] + n[2] * R[5]) * sign;
-  Real h = Real(0.5) * w * fabs(nx) + Real(0.5) * h_ * fabs(ny);
-  Real ax = Real(0.5) * w * ftanh_(nx / Real(SUPPORT_SMOOTH)), ay = Real(0.5) * h_ * ftanh_(ny / Real(SUPPORT_SMOOTH));
-  for (int k = 0; k < 3; k++) r[k] = ax * R[k] + ay * R[3 + k];
+// half-extent h(n) of a side's section facing sign·n (f64: it enters the penetration) and its smoothed support
+// point r (precision H: it only sets the lever of the contact force) (solver_cpu.js:274-282)
+template <typename G, typename H>
+__device__ __forceinline__ G support(const G* R, const G* n, G sign, G w, G h_, H* r) {
+  G nx = (n[0] * R[0] + n[1] * R[1] + n[2] * R[2]) * sign, ny = (n[0] * R[3] + n[1] * R[4] + n[2] * R[5]) * sign;
+  G h = G(0.5) * w * fabs(nx) + G(0.5) * h_ * fabs(ny);
+  H ax = H(0.5) * H(w) * ftanh_(H(nx) / H(SUPPORT_SMOOTH)), ay = H(0.5) * H(h_) * ftanh_(H(ny) / H(SUPPORT_SMOOTH));
+  for (int k = 0; k < 3; k++) r[k] = ax * H(R[k]) + ay * H(R[3 + k]);
   return h;
 }
 
 // W3 wall slope tan ψ (solver_cpu.js:285-293)
-template <typename Real>
-__device__ __forceinline__ Real walls(int sideCut, const Real* R, const Real* n, const Real* mm) {
-  Real sg = sideCut == 0 ? Real(1) : Real(-1);
-  Real nx = (n[0] * R[0] + n[1] * R[1] + n[2] * R[2]) * sg, ny = (n[0] * R[3] + n[1] * R[4] + n[2] * R[5]) * sg;
-  Real fx[3], fy[3];
+template <typename H>
+__device__ __forceinline__ H walls(int sideCut, const H* R, const H* n, const H* mm) {
+  H sg = sideCut == 0 ? H(1) : H(-1);
+  H nx = (n[0] * R[0] + n[1] * R[1] + n[2] * R[2]) * sg, ny = (n[0] * R[3] + n[1] * R[4] + n[2] * R[5]) * sg;
+  H fx[3], fy[3];
   for (int k = 0; k < 3; k++) { fx[k] = jsSign(nx) * R[k]; fy[k] = jsSign(ny) * R[3 + k]; }
-  const Real* f = (fx[0] * mm[0] + fx[1] * mm[1] + fx[2] * mm[2]) > Real(0) ? fx : fy;
-  Real cp = clampv(Real(fabs(f[0] * n[0] + f[1] * n[1] + f[2] * n[2])), Real(1e-6), Real(1));
+  const bool useX = (fx[0] * mm[0] + fx[1] * mm[1] + fx[2] * mm[2]) > H(0);
+  H f0 = useX ? fx[0] : fy[0], f1 = useX ? fx[1] : fy[1], f2 = useX ? fx[2] : fy[2];
+  H cp = clampv(H(fabs(f0 * n[0] + f1 * n[1] + f2 * n[2])), H(1e-6), H(1));
   return ftan_(facos_(cp));
 }
 
@@ -274,260 +288,313 @@ __device__ __forceinline__ G free_weight(const Inst<Real>& I, const SideEv<G>& E
   return (fa ? G(0) : (1 - E.t) * (1 - E.t)) + (fb ? G(0) : E.t * E.t);
 }
 
+__device__ __forceinline__ double xchg(double v) { return __shfl_xor(v, 1, 64); }
+__device__ __forceinline__ float xchg(float v) { return __shfl_xor(v, 1, 64); }
+__device__ __forceinline__ int xchg(int v) { return __shfl_xor(v, 1, 64); }
+
+// Two lanes per crossing: lane 2j evaluates side A of crossing c0 + j, lane 2j + 1 side B.  The side evaluations
+// (the neighbour loads, tangents, node frames and the Hermite point) run in parallel; the lanes then swap the
+// partner's point, tangent and frame and both compute the crossing's normal, penetration and friction law (the same
+// instructions on the same data, so the two copies agree bit for bit).  Each lane writes its own side's u and
+// record; lane 2j writes the crossing's state.  Contact kinematics run in f64 in both forms (the penetration
+// D − g cancels mm-scale lengths); the classification, the lever arms and the W2 / W3 trigonometry run in Real.
 template <typename Real>
-__global__ __launch_bounds__(CROSS_BLOCK) void k_cross(const Inst<Real>* __restrict__ insts, const CrossItem* __restrict__ items,
-                                                      int32_t nitems) {
+__global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __restrict__ insts, const CrossItem* __restrict__ items,
+                                                        int32_t nitems, const Ctl* __restrict__ ctls) {
+  using G = double;
+  using H = Real;
   int bid = blockIdx.x;
   if (bid >= nitems) return;
   CrossItem it = items[bid];
   if (it.inst < 0) return;
   const Inst<Real>& I = insts[it.inst];
-  const Ctl* ctl = I.ctl;
+  const Ctl* ctl = ctls + it.inst;
   if (!inst_live(ctl)) return;
   const int cur = ctl->cur, iter = ctl->iter;
   const bool al_update = ctl->al_update != 0;
   const int pen_n = ctl->pen_n;
   const int tid = threadIdx.x;
+  const int side = tid & 1, jj = tid >> 1;
+  const bool valid = jj < it.count;
+  const int c = it.c0 + (valid ? jj : 0);     // idle lanes shadow the block's first crossing and write nothing
+  const bool lead = valid && side == 0;
   double eC = 0.0, mxFn = -INFINITY;
-  if (tid < it.count) {
-    using G = double;   // contact kinematics in FP64: the penetration D − g cancels mm-scale lengths
 
-    const int c = it.c0 + tid;
-    const int rampN = max(I.ramp_iter, 0);
-    const double rampD = rampN == 0 ? 1.0 : fmin(1.0, 0.01 + 0.99 * iter / rampN);
-    const G ramp = G(rampD);
-    const G kn = ramp * G(I.strength), kt = ramp * G(I.stick);
-    const bool enabled = I.k_en[c] != 0;
-    const bool locked = I.k_lock[c] != 0;
-    const bool seat = I.seat != 0, aug = I.aug != 0;
-    const G mu0 = G(I.mu);
-    // ---- both sides, relative to the first node of A's segment
-    const double uA = I.u[2 * c], uB = I.u[2 * c + 1];
-    const int o = I.k_noff[2 * c] + int(fmin(fmax(floor(uA), 0.0), double(max(I.k_nseg[2 * c] - 1, 0))));
-    const double ox = I.x[cur][o], oy = I.y[cur][o], oz = I.z[cur][o];
-    SideEv<G> A, B;
-    eval_side(I, cur, c, 0, uA, ox, oy, oz, A);
-    eval_side(I, cur, c, 1, uB, ox, oy, oz, B);
-    const G wA = I.wk[2 * c], wB = I.wk[2 * c + 1], hA = I.hk[2 * c], hB = I.hk[2 * c + 1];
-    // ---- AL multiplier update deferred from the end of the previous iteration (updateMultipliers, :499-506)
-    if (aug && al_update) {
-      G kn1 = G(I.strength);   // the ramp is complete whenever the multipliers update
-      bool pinned = free_weight(I, A) < G(AL_FREE_MIN) || free_weight(I, B) < G(AL_FREE_MIN);
-      G cap = dmin(I.fc90[2 * c], I.fc90[2 * c + 1]) * wA * wB / dmax(fsin_(I.theta[c]), G(1e-6));
-      G v = I.lam[c] + kn1 * (I.penacc[c] / G(max(pen_n, 1)));
-      if (!locked) v = dmax(v, G(0));
-      v = clampv(v, -cap, cap);
-      I.lam[c] = (!enabled || pinned) ? G(0) : v;
-      I.penacc[c] = G(0);
+  const int rampN = max(I.ramp_iter, 0);
+  const double rampD = rampN == 0 ? 1.0 : fmin(1.0, 0.01 + 0.99 * iter / rampN);
+  const G kn = G(rampD) * G(I.strength), kt = G(rampD) * G(I.stick);
+  const bool enabled = I.k_en[c] != 0;
+  const bool locked = I.k_lock[c] != 0;
+  const bool seat = I.seat != 0, aug = I.aug != 0;
+  const G mu0 = G(I.mu);
+  // ---- own side, relative to the first node of A's segment
+  const double uA = I.u[2 * c];
+  const double uOwn = side == 0 ? uA : I.u[2 * c + 1];
+  const int o = I.k_noff[2 * c] + int(fmin(fmax(floor(uA), 0.0), double(max(I.k_nseg[2 * c] - 1, 0))));
+  const double ox = I.x[cur][o], oy = I.y[cur][o], oz = I.z[cur][o];
+  SideEv<G> E;
+  eval_side(I, cur, c, side, uOwn, ox, oy, oz, E);
+  // ---- partner's side
+  G oP[3], oT[3], oR[6], oL;
+  for (int k = 0; k < 3; k++) { oP[k] = xchg(E.P[k]); oT[k] = xchg(E.tg[k]); }
+  for (int k = 0; k < 6; k++) oR[k] = xchg(E.R[k]);
+  oL = xchg(E.L);
+  G AP[3], BP[3], tA[3], tB[3], AR[6], BR[6];
+  for (int k = 0; k < 3; k++) {
+    AP[k] = side == 0 ? E.P[k] : oP[k]; BP[k] = side == 0 ? oP[k] : E.P[k];
+    tA[k] = side == 0 ? E.tg[k] : oT[k]; tB[k] = side == 0 ? oT[k] : E.tg[k];
+  }
+  for (int k = 0; k < 6; k++) { AR[k] = side == 0 ? E.R[k] : oR[k]; BR[k] = side == 0 ? oR[k] : E.R[k]; }
+  const G wA = I.wk[2 * c], wB = I.wk[2 * c + 1], hA = I.hk[2 * c], hB = I.hk[2 * c + 1];
+  // ---- AL multiplier update deferred from the end of the previous iteration (updateMultipliers, :499-506)
+  G lam = aug ? G(I.lam[c]) : G(0);
+  G penacc = aug ? G(I.penacc[c]) : G(0);
+  if (aug && al_update) {                        // uniform over the launch: both lanes of every pair reach the swap
+    G fwOwn = free_weight(I, E), fwOth = xchg(fwOwn);
+    G kn1 = G(I.strength);   // the ramp is complete whenever the multipliers update
+    bool pinned = fwOwn < G(AL_FREE_MIN) || fwOth < G(AL_FREE_MIN);
+    G cap = dmin(G(I.fc90[2 * c]), G(I.fc90[2 * c + 1])) * wA * wB / dmax(G(fsin_(I.theta[c])), G(1e-6));
+    G v = lam + kn1 * (penacc / G(max(pen_n, 1)));
+    if (!locked) v = dmax(v, G(0));
+    v = clampv(v, -cap, cap);
+    lam = (!enabled || pinned) ? G(0) : v;
+    penacc = G(0);
+  }
+  G np[3] = {I.nprev[3 * c], I.nprev[3 * c + 1], I.nprev[3 * c + 2]};
+  G n[3] = {tA[1] * tB[2] - tA[2] * tB[1], tA[2] * tB[0] - tA[0] * tB[2], tA[0] * tB[1] - tA[1] * tB[0]};
+  G nn = fsqrt_(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  if (nn < G(1e-6)) { n[0] = np[0]; n[1] = np[1]; n[2] = np[2]; }
+  else { G d = dmax(nn, G(1e-12)); n[0] /= d; n[1] /= d; n[2] /= d; }
+  if (n[0] * np[0] + n[1] * np[1] + n[2] * np[2] < G(0)) { n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2]; }
+  G S[3] = {BP[0] - AP[0], BP[1] - AP[1], BP[2] - AP[2]};
+  G g = S[0] * n[0] + S[1] * n[1] + S[2] * n[2];
+  H rA[3], rB[3];
+  G supA = support(AR, n, G(1), wA, hA, rA);
+  G supB = support(BR, n, G(-1), wB, hB, rB);
+  G D = supA + supB;
+  if (seat && I.seat_gap) D -= G(I.dent[2 * c]) + G(I.dent[2 * c + 1]) + G(I.dente[c]);
+  G pen = D - g;
+  if (aug && iter >= rampN) penacc += pen;
+  G Fn = 0; bool act;
+  if (aug) {
+    G trial = lam + kn * pen;
+    act = (trial > G(0) || locked) && enabled;
+    Fn = act ? (locked ? trial : dmax(trial, G(0))) : G(0);
+  } else {
+    act = (pen > G(0) || locked) && enabled;
+    Fn = act ? kn * pen : G(0);
+  }
+  G St[3] = {S[0] - g * n[0], S[1] - g * n[1], S[2] - g * n[2]};
+  G taa = tA[0] * tA[0] + tA[1] * tA[1] + tA[2] * tA[2], tbb = tB[0] * tB[0] + tB[1] * tB[1] + tB[2] * tB[2];
+  G tab = tA[0] * tB[0] + tA[1] * tB[1] + tA[2] * tB[2];
+  G sa = St[0] * tA[0] + St[1] * tA[1] + St[2] * tA[2], sb = St[0] * tB[0] + St[1] * tB[1] + St[2] * tB[2];
+  G det = taa * tbb - tab * tab;
+  bool ok = det > G(1e-8);
+  G ca = ok ? (sa * tbb - sb * tab) / det : G(0), cb = ok ? (sb * taa - sa * tab) / det : G(0);
+  G Ftv = fsqrt_((kt * St[0]) * (kt * St[0]) + (kt * St[1]) * (kt * St[1]) + (kt * St[2]) * (kt * St[2]));
+  // Real copies of the frames and the normal for the classification and the W2 / W3 branches
+  H nR[3] = {H(n[0]), H(n[1]), H(n[2])}, tAR[3] = {H(tA[0]), H(tA[1]), H(tA[2])}, tBR[3] = {H(tB[0]), H(tB[1]), H(tB[2])};
+  H ARr[6], BRr[6];
+  for (int k = 0; k < 6; k++) { ARr[k] = H(AR[k]); BRr[k] = H(BR[k]); }
+  // ---- W3 directional seat friction (_seatFriction, :295-315)
+  G mu = mu0;
+  H tanpsi[2] = {H(0), H(0)};
+  if (seat) {
+    H Str[3] = {H(St[0]), H(St[1]), H(St[2])};
+    H norm = fsqrt_(Str[0] * Str[0] + Str[1] * Str[1] + Str[2] * Str[2]);
+    H dn = dmax(norm, H(1e-12));
+    H dh[3] = {Str[0] / dn, Str[1] / dn, Str[2] / dn};
+    bool has = norm > H(1e-12);
+    for (int j = 0; j < 2; j++) {
+      tanpsi[j] = I.tanpsi[2 * c + j];
+      if (!(I.dent[2 * c + j] > Real(0) && has)) continue;
+      const H* tCut = j == 0 ? tBR : tAR;
+      const H* Rcut = j == 0 ? BRr : ARr;
+      H mm[3] = {nR[1] * tCut[2] - nR[2] * tCut[1], nR[2] * tCut[0] - nR[0] * tCut[2], nR[0] * tCut[1] - nR[1] * tCut[0]};
+      H ml = dmax(fsqrt_(mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2]), H(1e-12));
+      mm[0] /= ml; mm[1] /= ml; mm[2] /= ml;
+      H sgn = j == 0 ? H(1) : H(-1);
+      H dj = (dh[0] * mm[0] + dh[1] * mm[1] + dh[2] * mm[2]) * sgn;
+      H wd[3] = {jsSign(dj) * mm[0] * sgn, jsSign(dj) * mm[1] * sgn, jsSign(dj) * mm[2] * sgn};
+      H tp = walls(1 - j, Rcut, nR, wd);
+      tanpsi[j] = tp;
+      H m0 = H(mu0);
+      H add = fabs(dj) * (m0 * tp < H(1) ? (m0 + tp) / dmax(H(1) - m0 * tp, H(1e-6)) - m0 : H(9) * m0);
+      mu += G(dmin(add, H(9) * m0));
     }
-    const G* tA = A.tg; const G* tB = B.tg;
-    G np[3] = {I.nprev[3 * c], I.nprev[3 * c + 1], I.nprev[3 * c + 2]};
-    G n[3] = {tA[1] * tB[2] - tA[2] * tB[1], tA[2] * tB[0] - tA[0] * tB[2], tA[0] * tB[1] - tA[1] * tB[0]};
-    G nn = fsqrt_(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-    if (nn < G(1e-6)) { n[0] = np[0]; n[1] = np[1]; n[2] = np[2]; }
-    else { G d = dmax(nn, G(1e-12)); n[0] /= d; n[1] /= d; n[2] /= d; }
-    if (n[0] * np[0] + n[1] * np[1] + n[2] * np[2] < G(0)) { n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2]; }
-    I.nprev[3 * c] = n[0]; I.nprev[3 * c + 1] = n[1]; I.nprev[3 * c + 2] = n[2];
-    G S[3] = {B.P[0] - A.P[0], B.P[1] - A.P[1], B.P[2] - A.P[2]};
-    G g = S[0] * n[0] + S[1] * n[1] + S[2] * n[2];
-    G rA[3], rB[3];
-    G supA = support(A.R, n, G(1), wA, hA, rA);
-    G supB = support(B.R, n, G(-1), wB, hB, rB);
-    G D = supA + supB;
-    if (seat && I.seat_gap) D -= I.dent[2 * c] + I.dent[2 * c + 1] + I.dente[c];
-    G pen = D - g;
-    if (aug && iter >= rampN) I.penacc[c] += pen;
-    G Fn = 0; bool act;
-    if (aug) {
-      G trial = I.lam[c] + kn * pen;
-      act = (trial > G(0) || locked) && enabled;
-      Fn = act ? (locked ? trial : dmax(trial, G(0))) : G(0);
-    } else {
-      act = (pen > G(0) || locked) && enabled;
-      Fn = act ? kn * pen : G(0);
+  }
+  G limit = locked ? G(INFINITY) : mu * fabs(Fn);
+  bool slip = act && ok && Ftv > limit;
+  bool fre = !act && ok && enabled;
+  G sc = 0;
+  if (slip) sc = clampv(G(I.slip_rate), G(1e-3), G(1)) * (G(1) - limit / dmax(Ftv, G(1e-12)));
+  if (fre) sc = G(1);
+  G eps = dmax(G(I.epsilon), G(1e-9));
+  G da = clampv(ca * sc, -eps, eps), db = clampv(cb * sc, -eps, eps);
+  bool moved = slip || fre;
+  // own side's parameter after the slip
+  const int nsOwn = I.k_nseg[2 * c + side];
+  double uNew = uOwn;
+  if (moved) {
+    double du = side == 0 ? double(da) / double(dmax(E.L, G(1e-12))) : -double(db) / double(dmax(E.L, G(1e-12)));
+    uNew = fmin(fmax(uOwn + du, 0.0), double(nsOwn));
+  }
+  G Se[3] = {0, 0, 0};
+  if (act) { Se[0] = St[0] - da * tA[0] - db * tB[0]; Se[1] = St[1] - da * tA[1] - db * tB[1]; Se[2] = St[2] - da * tA[2] - db * tB[2]; }
+  G fT[3] = {kt * Se[0], kt * Se[1], kt * Se[2]};
+  G FA[3], FB[3];
+  for (int k = 0; k < 3; k++) { FA[k] = -Fn * n[k] + fT[k]; FB[k] = Fn * n[k] - fT[k]; }
+  // ---- W1 classification (_classify, :231-271)
+  H cface[2], z0[2], phiv[2]; int edge[2];
+  for (int sd = 0; sd < 2; sd++) {
+    const H* R = sd == 0 ? ARr : BRr;
+    H sg = sd == 0 ? H(1) : H(-1);
+    H wv = H(sd == 0 ? wA : wB), hv = H(sd == 0 ? hA : hB);
+    H nx = (nR[0] * R[0] + nR[1] * R[1] + nR[2] * R[2]) * sg, ny = (nR[0] * R[3] + nR[1] * R[4] + nR[2] * R[5]) * sg;
+    H sx = fabs(ftanh_(nx / H(SUPPORT_SMOOTH))), sy = fabs(ftanh_(ny / H(SUPPORT_SMOOTH)));
+    edge[sd] = sx * sy > H(I.edge_thr) ? 1 : 0;
+    bool yf = fabs(ny) >= fabs(nx);
+    cface[sd] = yf ? wv : hv; z0[sd] = yf ? hv : wv;
+    phiv[sd] = fatan_(dmin(H(fabs(nx)), H(fabs(ny))) / dmax(dmax(H(fabs(nx)), H(fabs(ny))), H(1e-12)));
+  }
+  H cr[3] = {tAR[1] * tBR[2] - tAR[2] * tBR[1], tAR[2] * tBR[0] - tAR[0] * tBR[2], tAR[0] * tBR[1] - tAR[1] * tBR[0]};
+  H sinT = clampv(fsqrt_(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]), H(1e-6), H(1));
+  H theta = fasin_(sinT);
+  int nEdge = edge[0] + edge[1];
+  int faceSide = nEdge == 1 ? (edge[1] ? 0 : 1) : -1;
+  H Fa = H(fabs(Fn));
+  H Aa = cface[0] * cface[1] / sinT, pres = Fa / dmax(Aa, H(1e-12));
+  H ut = pres / dmax(H(dmin(I.Hs[2 * c], I.Hs[2 * c + 1])), H(1e-12));
+  if (nEdge == 1) {
+    int fs = faceSide, es = 1 - fs;
+    H ell = cface[fs] / sinT, kap = H(I.E90[2 * c + fs]) / dmax(z0[fs], H(1e-12));
+    H phiE = dmax(phiv[es], H(1e-4));
+    H bw = fsqrt_(H(2) * Fa / dmax(ell * kap * phiE, H(1e-12)));
+    bool full = bw >= cface[es];
+    H Afe = full ? cface[0] * cface[1] / sinT : bw * ell;
+    H pfe = full ? Fa / dmax(Afe, H(1e-12)) : kap * phiE * bw;
+    Aa = Afe; pres = pfe; ut = pfe / dmax(H(I.fc90[2 * c + es]), H(1e-12));
+  } else if (nEdge == 2) {
+    H fc = H(dmin(I.fc90[2 * c], I.fc90[2 * c + 1]));
+    Aa = Fa / dmax(fc, H(1e-12)); pres = Fa > H(0) ? fc : H(0); ut = Fa > H(0) ? H(1) : H(0);
+  }
+  // ---- W3 seat update (_seatUpdate, :317-344)
+  H dent[2] = {H(0), H(0)}, climb[2] = {H(0), H(0)}, dente = H(0);
+  if (seat) {
+    dent[0] = I.dent[2 * c]; dent[1] = I.dent[2 * c + 1]; climb[0] = I.climb[2 * c]; climb[1] = I.climb[2 * c + 1]; dente = I.dente[c];
+    H dar = H(da), dbr = H(db);
+    H inc[3] = {dar * tAR[0] - dbr * tBR[0], dar * tAR[1] - dbr * tBR[1], dar * tAR[2] - dbr * tBR[2]};
+    for (int j = 0; j < 2; j++) {
+      const H* tCut = j == 0 ? tBR : tAR;
+      H mm[3] = {nR[1] * tCut[2] - nR[2] * tCut[1], nR[2] * tCut[0] - nR[0] * tCut[2], nR[0] * tCut[1] - nR[1] * tCut[0]};
+      H ml = dmax(fsqrt_(mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2]), H(1e-12));
+      mm[0] /= ml; mm[1] /= ml; mm[2] /= ml;
+      H stp = (inc[0] * mm[0] + inc[1] * mm[1] + inc[2] * mm[2]) * (j == 0 ? H(1) : H(-1));
+      bool has = dent[j] > H(0);
+      if (has && slip) climb[j] += stp;
+      if (has && fabs(climb[j]) >= dent[j] / dmax(tanpsi[j], H(1e-6))) { dent[j] = H(0); climb[j] = H(0); }
     }
-    G St[3] = {S[0] - g * n[0], S[1] - g * n[1], S[2] - g * n[2]};
-    G taa = tA[0] * tA[0] + tA[1] * tA[1] + tA[2] * tA[2], tbb = tB[0] * tB[0] + tB[1] * tB[1] + tB[2] * tB[2];
-    G tab = tA[0] * tB[0] + tA[1] * tB[1] + tA[2] * tB[2];
-    G sa = St[0] * tA[0] + St[1] * tA[1] + St[2] * tA[2], sb = St[0] * tB[0] + St[1] * tB[1] + St[2] * tB[2];
-    G det = taa * tbb - tab * tab;
-    bool ok = det > G(1e-8);
-    G ca = ok ? (sa * tbb - sb * tab) / det : G(0), cb = ok ? (sb * taa - sa * tab) / det : G(0);
-    G Ftv = fsqrt_((kt * St[0]) * (kt * St[0]) + (kt * St[1]) * (kt * St[1]) + (kt * St[2]) * (kt * St[2]));
-    // ---- W3 directional seat friction (_seatFriction, :295-315)
-    G mu = mu0;
-    if (seat) {
-      G norm = fsqrt_(St[0] * St[0] + St[1] * St[1] + St[2] * St[2]);
-      G dn = dmax(norm, G(1e-12));
-      G dh[3] = {St[0] / dn, St[1] / dn, St[2] / dn};
-      bool has = norm > G(1e-12);
-      for (int j = 0; j < 2; j++) {
-        if (!(I.dent[2 * c + j] > G(0) && has)) continue;
-        const G* tCut = j == 0 ? tB : tA;
-        const G* Rcut = j == 0 ? B.R : A.R;
-        G mm[3] = {n[1] * tCut[2] - n[2] * tCut[1], n[2] * tCut[0] - n[0] * tCut[2], n[0] * tCut[1] - n[1] * tCut[0]};
-        G ml = dmax(fsqrt_(mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2]), G(1e-12));
-        mm[0] /= ml; mm[1] /= ml; mm[2] /= ml;
-        G sgn = j == 0 ? G(1) : G(-1);
-        G dj = (dh[0] * mm[0] + dh[1] * mm[1] + dh[2] * mm[2]) * sgn;
-        G wd[3] = {jsSign(dj) * mm[0] * sgn, jsSign(dj) * mm[1] * sgn, jsSign(dj) * mm[2] * sgn};
-        G tp = walls(1 - j, Rcut, n, wd);
-        I.tanpsi[2 * c + j] = tp;
-        G add = fabs(dj) * (mu0 * tp < G(1) ? (mu0 + tp) / dmax(G(1) - mu0 * tp, G(1e-6)) - mu0 : G(9) * mu0);
-        mu += dmin(add, G(9) * mu0);
+    if (iter >= I.dent_after) {
+      H sp0 = fsin_(phiv[0]) * fcos_(phiv[0]), sp1 = fsin_(phiv[1]) * fcos_(phiv[1]);
+      H sT = fsin_(theta);
+      if (nEdge == 2 && act && !locked) {
+        H fc = H(dmin(I.fc90[2 * c], I.fc90[2 * c + 1]));
+        H dEst = fsqrt_(Fa / dmax(fc, H(1e-12)) * sT * sp0 * sp1);
+        for (int j = 0; j < 2; j++) { H cu = dent[j]; dent[j] = dmax(cu, dmin(cu + H(I.dent_rate) * dEst, dEst)); }
+      }
+      if (nEdge == 1 && act && !locked) {
+        int fs = faceSide, es = 1 - fs;
+        H ell = cface[fs] / dmax(sT, H(1e-6));
+        H dE = Fa / dmax(H(I.fc90[2 * c + es]) * ell, H(1e-12)) * (es == 0 ? sp0 : sp1);
+        dente = dmax(dente, dE);
       }
     }
-    G limit = locked ? G(INFINITY) : mu * fabs(Fn);
-    bool slip = act && ok && Ftv > limit;
-    bool fre = !act && ok && enabled;
-    G sc = 0;
-    if (slip) sc = clampv(G(I.slip_rate), G(1e-3), G(1)) * (G(1) - limit / dmax(Ftv, G(1e-12)));
-    if (fre) sc = G(1);
-    G eps = dmax(G(I.epsilon), G(1e-9));
-    G da = clampv(ca * sc, -eps, eps), db = clampv(cb * sc, -eps, eps);
-    bool moved = slip || fre;
-    if (moved) {
-      double u0 = I.u[2 * c] + double(da) / double(dmax(A.L, G(1e-12)));
-      double u1 = I.u[2 * c + 1] - double(db) / double(dmax(B.L, G(1e-12)));
-      I.u[2 * c] = fmin(fmax(u0, 0.0), double(I.k_nseg[2 * c]));
-      I.u[2 * c + 1] = fmin(fmax(u1, 0.0), double(I.k_nseg[2 * c + 1]));
+  }
+  // ---- lever torques, W2 rotational friction (_rotationalFriction, :347-379)
+  H tf = H(I.torsion_factor);
+  H FAr[3] = {H(FA[0]), H(FA[1]), H(FA[2])}, FBr[3] = {H(FB[0]), H(FB[1]), H(FB[2])};
+  H TA[3] = {tf * (rA[1] * FAr[2] - rA[2] * FAr[1]), tf * (rA[2] * FAr[0] - rA[0] * FAr[2]), tf * (rA[0] * FAr[1] - rA[1] * FAr[0])};
+  H TB[3] = {tf * (rB[1] * FBr[2] - rB[2] * FBr[1]), tf * (rB[2] * FBr[0] - rB[0] * FBr[2]), tf * (rB[0] * FBr[1] - rB[1] * FBr[0])};
+  H eRot = 0, th0 = H(0), rotslip = H(0), Mr = H(0);
+  if (I.rot_fric) {
+    H tabr = H(tab);
+    H thc = fatan2_(cr[0] * nR[0] + cr[1] * nR[1] + cr[2] * nR[2], tabr);
+    th0 = I.theta0[c];
+    rotslip = I.rotslip[c];
+    if (isnan(th0) && act) th0 = thc;
+    bool okr = act && !isnan(th0);
+    H sT2 = dmax(fsin_(theta), H(1e-6));
+    H wAr = H(wA), wBr = H(wB);
+    H req = H(0.38) * fsqrt_(wAr * wBr / sT2);
+    if (nEdge == 1) req = H(0.25) * (faceSide == 0 ? wAr : wBr) / sT2;
+    else if (nEdge == 2) req = H(0.38) * fsqrt_(dmax(Aa, H(1e-12)));
+    H kr = H(kt) * req * req;
+    H dth = okr ? thc - th0 : H(0);
+    const H PI = H(3.141592653589793);
+    dth = ffmod_(ffmod_(dth + PI, 2 * PI) + 2 * PI, 2 * PI) - PI;
+    H Mc = kr * dth;
+    H Mlim = locked ? H(INFINITY) : H(I.rot_factor) * H(mu0) * Fa * req;
+    if (okr && fabs(Mc) > Mlim) {
+      H adv = clampv(clampv(H(I.slip_rate), H(1e-3), H(1)) * (H(1) - Mlim / dmax(H(fabs(Mc)), H(1e-12))) * dth, H(-0.05), H(0.05));
+      th0 += adv; rotslip += fabs(adv); dth -= adv;
+      Mc = kr * dth;
     }
+    Mr = okr ? Mc : H(0);
+    for (int k = 0; k < 3; k++) { TA[k] += Mr * nR[k]; TB[k] -= Mr * nR[k]; }
+    eRot = H(0.5) * kr * dth * dth;
+  }
+  // ---- righting stiffness (addTwistStiffness, :170-176) on the own side's segment after the slip
+  G tfk = dmax(G(I.torsion_factor), G(1e-3));
+  G kth = tfk * ((Fn + kn * G(0.25) * G(I.kh[c])) * (G(0.5) * G(I.kw[c])) / G(SUPPORT_SMOOTH) + kt * (G(0.5) * G(I.kh[c])) * (G(0.5) * G(I.kh[c])));
+  if (valid) {
+    // own side: parameter, record
+    if (moved) I.u[2 * c + side] = uNew;
+    SideRec<Real> rec;
+    rec.seg_pre = E.loc;
+    rec.seg_post = int(fmin(fmax(floor(uNew), 0.0), double(max(nsOwn - 1, 0))));
+    rec.t = Real(E.t); rec.kth = Real(kth);
+    for (int k = 0; k < 3; k++) {
+      rec.F[k] = Real(side == 0 ? FA[k] : FB[k]);
+      rec.T[k] = Real(side == 0 ? TA[k] : TB[k]);
+    }
+    side_store(rec, I.side + I.side_pos[2 * c + side]);
+  }
+  if (lead) {
+    I.nprev[3 * c] = Real(n[0]); I.nprev[3 * c + 1] = Real(n[1]); I.nprev[3 * c + 2] = Real(n[2]);
     if (slip) {
       G q0 = da * tA[0] + db * tB[0], q1 = da * tA[1] + db * tB[1], q2 = da * tA[2] + db * tB[2];
       I.slip[c] += double(fsqrt_(q0 * q0 + q1 * q1 + q2 * q2));
     }
-    G Se[3] = {0, 0, 0};
-    if (act) { Se[0] = St[0] - da * tA[0] - db * tB[0]; Se[1] = St[1] - da * tA[1] - db * tB[1]; Se[2] = St[2] - da * tA[2] - db * tB[2]; }
-    G fT[3] = {kt * Se[0], kt * Se[1], kt * Se[2]};
-    G FA[3], FB[3];
-    for (int k = 0; k < 3; k++) { FA[k] = -Fn * n[k] + fT[k]; FB[k] = Fn * n[k] - fT[k]; }
-    I.fn[c] = Fn;
-    I.ft[c] = fsqrt_(fT[0] * fT[0] + fT[1] * fT[1] + fT[2] * fT[2]);
-    I.gap[c] = g;
+    if (aug) { I.lam[c] = Real(lam); I.penacc[c] = Real(penacc); }
+    I.fn[c] = Real(Fn);
+    I.ft[c] = Real(fsqrt_(fT[0] * fT[0] + fT[1] * fT[1] + fT[2] * fT[2]));
+    I.gap[c] = Real(g);
     I.kflags[c] = uint8_t((slip ? 1 : 0) | (moved ? 0 : 2));
-    for (int k = 0; k < 3; k++) I.mid[3 * c + k] = G(0.5) * (A.P[k] + B.P[k]) + G(k == 0 ? ox : (k == 1 ? oy : oz));
-    // ---- W1 classification (_classify, :231-271)
-    G cface[2], z0[2], phiv[2]; int edge[2];
-    for (int sd = 0; sd < 2; sd++) {
-      const G* R = sd == 0 ? A.R : B.R;
-      G sg = sd == 0 ? G(1) : G(-1);
-      G wv = sd == 0 ? wA : wB, hv = sd == 0 ? hA : hB;
-      G nx = (n[0] * R[0] + n[1] * R[1] + n[2] * R[2]) * sg, ny = (n[0] * R[3] + n[1] * R[4] + n[2] * R[5]) * sg;
-      G sx = fabs(ftanh_(nx / G(SUPPORT_SMOOTH))), sy = fabs(ftanh_(ny / G(SUPPORT_SMOOTH)));
-      edge[sd] = sx * sy > G(I.edge_thr) ? 1 : 0;
-      bool yf = fabs(ny) >= fabs(nx);
-      cface[sd] = yf ? wv : hv; z0[sd] = yf ? hv : wv;
-      phiv[sd] = fatan_(dmin(G(fabs(nx)), G(fabs(ny))) / dmax(dmax(G(fabs(nx)), G(fabs(ny))), G(1e-12)));
-    }
-    G cr[3] = {tA[1] * tB[2] - tA[2] * tB[1], tA[2] * tB[0] - tA[0] * tB[2], tA[0] * tB[1] - tA[1] * tB[0]};
-    G sinT = clampv(fsqrt_(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]), G(1e-6), G(1));
-    G theta = fasin_(sinT);
-    int nEdge = edge[0] + edge[1];
-    int faceSide = nEdge == 1 ? (edge[1] ? 0 : 1) : -1;
-    G Fa = fabs(Fn);
-    G Aa = cface[0] * cface[1] / sinT, pres = Fa / dmax(Aa, G(1e-12));
-    G ut = pres / dmax(dmin(I.Hs[2 * c], I.Hs[2 * c + 1]), G(1e-12));
-    if (nEdge == 1) {
-      int fs = faceSide, es = 1 - fs;
-      G ell = cface[fs] / sinT, kap = I.E90[2 * c + fs] / dmax(z0[fs], G(1e-12));
-      G phiE = dmax(phiv[es], G(1e-4));
-      G bw = fsqrt_(G(2) * Fa / dmax(ell * kap * phiE, G(1e-12)));
-      bool full = bw >= cface[es];
-      G Afe = full ? cface[0] * cface[1] / sinT : bw * ell;
-      G pfe = full ? Fa / dmax(Afe, G(1e-12)) : kap * phiE * bw;
-      Aa = Afe; pres = pfe; ut = pfe / dmax(I.fc90[2 * c + es], G(1e-12));
-    } else if (nEdge == 2) {
-      G fc = dmin(I.fc90[2 * c], I.fc90[2 * c + 1]);
-      Aa = Fa / dmax(fc, G(1e-12)); pres = Fa > G(0) ? fc : G(0); ut = Fa > G(0) ? G(1) : G(0);
-    }
+    I.mid[3 * c] = Real(G(0.5) * (AP[0] + BP[0]) + ox);
+    I.mid[3 * c + 1] = Real(G(0.5) * (AP[1] + BP[1]) + oy);
+    I.mid[3 * c + 2] = Real(G(0.5) * (AP[2] + BP[2]) + oz);
     I.theta[c] = theta; I.anom[c] = Aa; I.pres[c] = pres; I.util[c] = ut;
     I.phi[2 * c] = phiv[0]; I.phi[2 * c + 1] = phiv[1];
     I.ctype[c] = int8_t(nEdge); I.face[c] = int8_t(faceSide);
-    // ---- W3 seat update (_seatUpdate, :317-344)
     if (seat) {
-      G inc[3] = {da * tA[0] - db * tB[0], da * tA[1] - db * tB[1], da * tA[2] - db * tB[2]};
-      for (int j = 0; j < 2; j++) {
-        const G* tCut = j == 0 ? tB : tA;
-        G mm[3] = {n[1] * tCut[2] - n[2] * tCut[1], n[2] * tCut[0] - n[0] * tCut[2], n[0] * tCut[1] - n[1] * tCut[0]};
-        G ml = dmax(fsqrt_(mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2]), G(1e-12));
-        mm[0] /= ml; mm[1] /= ml; mm[2] /= ml;
-        G stp = (inc[0] * mm[0] + inc[1] * mm[1] + inc[2] * mm[2]) * (j == 0 ? G(1) : G(-1));
-        int i2 = 2 * c + j;
-        bool has = I.dent[i2] > G(0);
-        if (has && slip) I.climb[i2] += stp;
-        if (has && fabs(I.climb[i2]) >= I.dent[i2] / dmax(I.tanpsi[i2], G(1e-6))) { I.dent[i2] = G(0); I.climb[i2] = G(0); }
-      }
-      if (iter >= I.dent_after) {
-        G sp0 = fsin_(phiv[0]) * fcos_(phiv[0]), sp1 = fsin_(phiv[1]) * fcos_(phiv[1]);
-        G sT = fsin_(theta);
-        if (nEdge == 2 && act && !locked) {
-          G fc = dmin(I.fc90[2 * c], I.fc90[2 * c + 1]);
-          G dEst = fsqrt_(Fa / dmax(fc, G(1e-12)) * sT * sp0 * sp1);
-          for (int j = 0; j < 2; j++) { G cu = I.dent[2 * c + j]; I.dent[2 * c + j] = dmax(cu, dmin(cu + G(I.dent_rate) * dEst, dEst)); }
-        }
-        if (nEdge == 1 && act && !locked) {
-          int fs = faceSide, es = 1 - fs;
-          G ell = cface[fs] / dmax(sT, G(1e-6));
-          G dE = Fa / dmax(I.fc90[2 * c + es] * ell, G(1e-12)) * (es == 0 ? sp0 : sp1);
-          I.dente[c] = dmax(I.dente[c], dE);
-        }
-      }
+      I.tanpsi[2 * c] = tanpsi[0]; I.tanpsi[2 * c + 1] = tanpsi[1];
+      I.dent[2 * c] = dent[0]; I.dent[2 * c + 1] = dent[1]; I.climb[2 * c] = climb[0]; I.climb[2 * c + 1] = climb[1];
+      I.dente[c] = dente;
     }
-    // ---- lever torques, W2 rotational friction (_rotationalFriction, :347-379)
-    G tf = G(I.torsion_factor);
-    G TA[3] = {tf * (rA[1] * FA[2] - rA[2] * FA[1]), tf * (rA[2] * FA[0] - rA[0] * FA[2]), tf * (rA[0] * FA[1] - rA[1] * FA[0])};
-    G TB[3] = {tf * (rB[1] * FB[2] - rB[2] * FB[1]), tf * (rB[2] * FB[0] - rB[0] * FB[2]), tf * (rB[0] * FB[1] - rB[1] * FB[0])};
-    G eRot = 0;
-    if (I.rot_fric) {
-      G thc = fatan2_(cr[0] * n[0] + cr[1] * n[1] + cr[2] * n[2], tab);
-      G th0 = I.theta0[c];
-      if (isnan(th0) && act) th0 = thc;
-      bool okr = act && !isnan(th0);
-      G sT2 = dmax(fsin_(theta), G(1e-6));
-      G req = G(0.38) * fsqrt_(wA * wB / sT2);
-      if (nEdge == 1) req = G(0.25) * (faceSide == 0 ? wA : wB) / sT2;
-      else if (nEdge == 2) req = G(0.38) * fsqrt_(dmax(Aa, G(1e-12)));
-      G kr = kt * req * req;
-      G dth = okr ? thc - th0 : G(0);
-      const G PI = G(3.141592653589793);
-      dth = ffmod_(ffmod_(dth + PI, 2 * PI) + 2 * PI, 2 * PI) - PI;
-      G Mc = kr * dth;
-      G Mlim = locked ? G(INFINITY) : G(I.rot_factor) * mu0 * Fa * req;
-      if (okr && fabs(Mc) > Mlim) {
-        G adv = clampv(clampv(G(I.slip_rate), G(1e-3), G(1)) * (G(1) - Mlim / dmax(G(fabs(Mc)), G(1e-12))) * dth, G(-0.05), G(0.05));
-        th0 += adv; I.rotslip[c] += fabs(adv); dth -= adv;
-        Mc = kr * dth;
-      }
-      I.theta0[c] = th0;
-      G Mr = okr ? Mc : G(0);
-      I.mrot[c] = Mr;
-      for (int k = 0; k < 3; k++) { TA[k] += Mr * n[k]; TB[k] -= Mr * n[k]; }
-      eRot = G(0.5) * kr * dth * dth;
-    }
-    // ---- righting stiffness (addTwistStiffness, :170-176) on the segments after the slip
-    G tfk = dmax(G(I.torsion_factor), G(1e-3));
-    G kth = tfk * ((Fn + kn * G(0.25) * I.kh[c]) * (G(0.5) * I.kw[c]) / G(SUPPORT_SMOOTH) + kt * (G(0.5) * I.kh[c]) * (G(0.5) * I.kh[c]));
-    int postA = int(fmin(fmax(floor(I.u[2 * c]), 0.0), double(max(I.k_nseg[2 * c] - 1, 0))));
-    int postB = int(fmin(fmax(floor(I.u[2 * c + 1]), 0.0), double(max(I.k_nseg[2 * c + 1] - 1, 0))));
-    SideRec<Real> ra, rb;
-    ra.seg_pre = A.loc; ra.seg_post = postA; ra.t = A.t; ra.kth = kth;
-    rb.seg_pre = B.loc; rb.seg_post = postB; rb.t = B.t; rb.kth = kth;
-    for (int k = 0; k < 3; k++) { ra.F[k] = FA[k]; ra.T[k] = TA[k]; rb.F[k] = FB[k]; rb.T[k] = TB[k]; }
-    I.side[I.side_pos[2 * c]] = ra;
-    I.side[I.side_pos[2 * c + 1]] = rb;
+    if (I.rot_fric) { I.theta0[c] = th0; I.rotslip[c] = rotslip; I.mrot[c] = Mr; }
     G ePen = act ? (aug ? Fn * Fn / dmax(kn, G(1e-30)) : kn * pen * pen) : G(0);
     eC = 0.5 * double(ePen) + 0.5 * double(kt) * double(Se[0] * Se[0] + Se[1] * Se[1] + Se[2] * Se[2]) + double(eRot);
     mxFn = double(Fn);
   }
   // ---- block partial (fixed tree order)
-  __shared__ double rs[CROSS_BLOCK], rm[CROSS_BLOCK];
+  __shared__ double rs[CROSS_THREADS], rm[CROSS_THREADS];
   rs[tid] = eC; rm[tid] = mxFn;
   __syncthreads();
-  for (int k = CROSS_BLOCK / 2; k > 0; k >>= 1) {
+  for (int k = CROSS_THREADS / 2; k > 0; k >>= 1) {
     if (tid < k) { rs[tid] += rs[tid + k]; rm[tid] = fmax(rm[tid], rm[tid + k]); }
     __syncthreads();
   }
-  if (tid == 0) { I.cpart[it.blk * CPART] = rs[0]; I.cpart[it.blk * CPART + 1] = rm[0]; }
+  if (tid == 0) { I.cpart[it.blk] = rs[0]; I.cpart[I.ncblk + it.blk] = rm[0]; }   // [value][block]
 }
 
 // ================================================================================================= k_strand
@@ -542,20 +609,9 @@ struct StrandLDS {
   Real KDx[TILE + 1], KDy[TILE + 1], KDz[TILE + 1];                  // bending-coupling inertia of joint k0-1+i
   int32_t rpre[TILE], rpost[TILE];                                   // gathered crossing sides: segments
   SideRec<Real> rec[TILE];                                           // ... and their records
+  int32_t glo[TILE], ghi[TILE];                                      // per slot: first / last gathered record it needs
   int wcnt[TILE / 64];
 };
-
-// field-wise copies of a side record (keeps the record in registers: no aggregate copy through private memory)
-template <typename Real>
-__device__ __forceinline__ void side_load(const SideRec<Real>* __restrict__ p, SideRec<Real>& r) {
-  r.seg_pre = p->seg_pre; r.seg_post = p->seg_post; r.t = p->t; r.kth = p->kth;
-  r.F[0] = p->F[0]; r.F[1] = p->F[1]; r.F[2] = p->F[2]; r.T[0] = p->T[0]; r.T[1] = p->T[1]; r.T[2] = p->T[2];
-}
-template <typename Real>
-__device__ __forceinline__ void side_store(const SideRec<Real>& r, SideRec<Real>* p) {
-  p->seg_pre = r.seg_pre; p->seg_post = r.seg_post; p->t = r.t; p->kth = r.kth;
-  p->F[0] = r.F[0]; p->F[1] = r.F[1]; p->F[2] = r.F[2]; p->T[0] = r.T[0]; p->T[1] = r.T[1]; p->T[2] = r.T[2];
-}
 
 // f64 frame [X | Y | Z] of a quaternion with one reciprocal: R = I + 2/|q|² (…) (quat.js toFrame up to rounding)
 __device__ __forceinline__ void qframe_d(double w, double x, double y, double z, double* R) {
@@ -647,20 +703,20 @@ __device__ __forceinline__ void joint_torque(StrandLDS<Real, TILE>& L, int i, Re
   Jo[0] = J[0]; Jo[1] = J[1]; Jo[2] = J[2];
 }
 
+// One strand tile.  A tile owns cnt ≤ TILE − 1 consecutive nodes k0 .. k1−1 of one lath and the segments starting
+// at them; lane t works on "slot" t: node k0−1+t, segment k0−1+t and joint k0−1+t.  Slot 0 is the halo (the segment
+// and joint just before the tile, whose forces the first owned node and frame need): every lane computes one segment
+// and one joint, and lanes 1 .. cnt integrate their node and frame.
 template <typename Real, int TILE, bool OUT>
-__device__ void strand_tile(const Inst<Real>& I, const TileItem tl, StrandLDS<Real, TILE>& L, int tile_local,
+__device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem& tl, StrandLDS<Real, TILE>& L,
                             double* outN, double* outV, double* outM, double* outCM, double* outR) {
-  const Ctl* ctl = I.ctl;
   const int tid = threadIdx.x;
   int cur = ctl->cur;
   const bool rbk = ctl->rollback != 0;
   if (OUT) cur = rbk ? cur : (cur ^ 1);            // the state the last iteration evaluated
   const int iter = ctl->iter;
-  const int s = tl.strand;
-  const int noff = I.node_off[s], nn = I.node_off[s + 1] - noff;
-  const int soff = I.seg_off[s], nseg = I.seg_off[s + 1] - soff;
-  const int joff = I.joint_off[s], njs = I.joint_off[s + 1] - joff;
-  const bool closed = I.closed[s] != 0;
+  const int noff = tl.noff, nn = tl.nn, soff = tl.soff, nseg = tl.nseg, joff = tl.joff, njs = tl.njs;
+  const bool closed = tl.closed != 0;
   const int k0 = tl.k0, k1 = tl.k1, cnt = k1 - k0;
   auto wrap = [](int v, int n) { int r = v % n; return r < 0 ? r + n : r; };
   auto node_ok = [&](int k, int& kk) { if (closed) { kk = wrap(k, nn); return true; } kk = k; return k >= 0 && k < nn; };
@@ -677,14 +733,14 @@ __device__ void strand_tile(const Inst<Real>& I, const TileItem tl, StrandLDS<Re
     else { L.Qw[h] = 1.0; L.Qx[h] = L.Qy[h] = L.Qz[h] = 0.0; }
   }
   const int t = tid;
-  const int k = k0 + t;
-  const bool ownNode = t < cnt;
-  int kkN = 0, jself = 0, jjo = 0, jjh = 0, jsh = 0;
+  const int k = k0 - 1 + t;                        // node / segment / joint of this slot (strand-local)
+  const bool inSlot = t <= cnt;
+  const bool ownNode = t >= 1 && t <= cnt;
+  int kkN = 0, jself = 0, jjo = 0;
   if (ownNode) node_ok(k, kkN);
-  const bool ownSeg = ownNode && seg_ok(k, jself);
-  const bool ownJoint = ownNode && joint_ok(k, jjo);
-  const bool haloSeg = t == 0 && seg_ok(k0 - 1, jsh);
-  const bool haloJoint = t == 0 && joint_ok(k0 - 1, jjh);
+  const bool hasSeg = inSlot && seg_ok(k, jself);
+  const bool hasJoint = inSlot && joint_ok(k, jjo);
+  const bool ownSeg = ownNode && hasSeg;
   const int i = noff + kkN, j = soff + jself;
   // node parameters and velocity
   uint8_t fx = 0; Real ms = 1, ldx = 0, ldy = 0, ldz = 0, llx = 0, lly = 0, llz = 0, vx = 0, vy = 0, vz = 0;
@@ -693,30 +749,22 @@ __device__ void strand_tile(const Inst<Real>& I, const TileItem tl, StrandLDS<Re
     if (I.npl) { llx = I.llx[i]; lly = I.lly[i]; llz = I.llz[i]; }
     if (!OUT && !rbk) { vx = I.vx[i]; vy = I.vy[i]; vz = I.vz[i]; }
   }
-  // own segment and joint parameters
+  // segment and joint parameters of the slot
   double l0 = 1.0; Real EA = 0, GA = 0, Bx = 0, By = 0, Bz = 0, krx = 0, kry = 0, krz = 0, wx = 0, wy = 0, wz = 0;
-  if (ownSeg) {
-    l0 = I.l0[j]; EA = I.EA[j]; GA = I.GA[j]; krx = I.krx[j]; kry = I.kry[j]; krz = I.krz[j];
-    if (!OUT && !rbk) { wx = I.wx[j]; wy = I.wy[j]; wz = I.wz[j]; }
+  if (hasSeg) {
+    l0 = I.l0[j]; EA = I.EA[j]; GA = I.GA[j];
+    if (ownSeg) {
+      krx = I.krx[j]; kry = I.kry[j]; krz = I.krz[j];
+      if (!OUT && !rbk) { wx = I.wx[j]; wy = I.wy[j]; wz = I.wz[j]; }
+    }
   }
   Real lb = 1, om[3] = {0, 0, 0};
-  if (ownJoint) {
+  if (hasJoint) {
     const int jg = joff + jjo;
     lb = I.jlen[jg]; om[0] = I.om0[3 * jg]; om[1] = I.om0[3 * jg + 1]; om[2] = I.om0[3 * jg + 2];
     Bx = I.EIx[j]; By = I.EIy[j]; Bz = I.GJ[j];
   }
-  // halo segment k0-1 and joint k0-1 (thread 0)
-  double hl0 = 1.0; Real hEA = 0, hGA = 0, hlb = 1, hom[3] = {0, 0, 0}, hBx = 0, hBy = 0, hBz = 0;
-  if (haloSeg) {
-    const int jh = soff + jsh;
-    hl0 = I.l0[jh]; hEA = I.EA[jh]; hGA = I.GA[jh];
-    hBx = I.EIx[jh]; hBy = I.EIy[jh]; hBz = I.GJ[jh];
-  }
-  if (haloJoint) {
-    const int jg = joff + jjh;
-    hlb = I.jlen[jg]; hom[0] = I.om0[3 * jg]; hom[1] = I.om0[3 * jg + 1]; hom[2] = I.om0[3 * jg + 2];
-  }
-  const int so = I.side_off[s], ns = I.side_off[s + 1] - so;
+  const int so = tl.so, ns = tl.ns;
   SideRec<Real> r0;
   if (tid < ns) side_load(I.side + so + tid, r0);
   __syncthreads();
@@ -724,40 +772,41 @@ __device__ void strand_tile(const Inst<Real>& I, const TileItem tl, StrandLDS<Re
   // ---------------------------------------------------------------- segments and joints
   double eS = 0, eSh = 0, eB = 0, eT = 0, mxN = 0, mxM = 0;
   Real ST[3] = {0, 0, 0};
-  if (ownSeg) {
-    Real nb[3];
-    seg_force(L, t + 1, l0, EA, GA, ST, nb, &eS, &eSh);
-    mxN = fabs(double(nb[2]));
-    if (OUT) { outN[j] = double(nb[2]); outV[2 * j] = double(nb[0]); outV[2 * j + 1] = double(nb[1]); }
-  } else {
-    L.SFx[t + 1] = L.SFy[t + 1] = L.SFz[t + 1] = Real(0);
-  }
-  if (ownJoint) {
-    Real mm[3], d[3], J[3];
-    joint_torque(L, t + 1, lb, om, Bx, By, Bz, mm, d, J);
-    eB = 0.5 * double(lb) * (double(mm[0]) * double(d[0]) + double(mm[1]) * double(d[1]));
-    eT = 0.5 * double(lb) * double(mm[2]) * double(d[2]);
-    mxM = fmax(fabs(double(mm[0])), fmax(fabs(double(mm[1])), fabs(double(mm[2]))));
-    if (OUT) { const int jg = joff + jjo; outM[3 * jg] = double(mm[0]); outM[3 * jg + 1] = double(mm[1]); outM[3 * jg + 2] = double(mm[2]); }
-  } else {
-    L.JAx[t + 1] = L.JAy[t + 1] = L.JAz[t + 1] = Real(0);
-    L.JBx[t + 1] = L.JBy[t + 1] = L.JBz[t + 1] = Real(0);
-    L.KDx[t + 1] = L.KDy[t + 1] = L.KDz[t + 1] = Real(0);
-  }
-  if (t == 0) {
-    if (haloSeg) { Real st2[3], nb2[3]; double a1, a2; seg_force(L, 0, hl0, hEA, hGA, st2, nb2, &a1, &a2); }
-    else { L.SFx[0] = L.SFy[0] = L.SFz[0] = Real(0); }
-    if (haloJoint) { Real mm[3], d[3], J[3]; joint_torque(L, 0, hlb, hom, hBx, hBy, hBz, mm, d, J); }
-    else {
-      L.JAx[0] = L.JAy[0] = L.JAz[0] = Real(0); L.JBx[0] = L.JBy[0] = L.JBz[0] = Real(0); L.KDx[0] = L.KDy[0] = L.KDz[0] = Real(0);
+  if (hasSeg) {
+    Real nb[3]; double a1, a2;
+    seg_force(L, t, l0, EA, GA, ST, nb, &a1, &a2);
+    if (ownSeg) {
+      eS = a1; eSh = a2;
+      mxN = fabs(double(nb[2]));
+      if (OUT) { outN[j] = double(nb[2]); outV[2 * j] = double(nb[0]); outV[2 * j + 1] = double(nb[1]); }
     }
+  } else if (inSlot) {
+    L.SFx[t] = L.SFy[t] = L.SFz[t] = Real(0);
+  }
+  if (hasJoint) {
+    Real mm[3], d[3], J[3];
+    joint_torque(L, t, lb, om, Bx, By, Bz, mm, d, J);
+    if (ownNode) {
+      eB = 0.5 * double(lb) * (double(mm[0]) * double(d[0]) + double(mm[1]) * double(d[1]));
+      eT = 0.5 * double(lb) * double(mm[2]) * double(d[2]);
+      mxM = fmax(fabs(double(mm[0])), fmax(fabs(double(mm[1])), fabs(double(mm[2]))));
+      if (OUT) { const int jg = joff + jjo; outM[3 * jg] = double(mm[0]); outM[3 * jg + 1] = double(mm[1]); outM[3 * jg + 2] = double(mm[2]); }
+    }
+  } else if (inSlot) {
+    L.JAx[t] = L.JAy[t] = L.JAz[t] = Real(0);
+    L.JBx[t] = L.JBy[t] = L.JBz[t] = Real(0);
+    L.KDx[t] = L.KDy[t] = L.KDz[t] = Real(0);
   }
 
   // ---------------------------------------------------------------- crossing sides of this strand touching the tile,
-  // gathered in the strand's fixed side order (independent of the tiling)
+  // summed in the strand's fixed side order (independent of the tiling).  Each gathered record registers the range
+  // of gathered indices per owned slot (integer min / max, order-free), so a lane visits only the window that can
+  // hold its records, in increasing order: the same sums as a scan over all of them.
   Real Fc[3] = {0, 0, 0}, Tc[3] = {0, 0, 0}, kTw = 0;
   const int lane = tid & 63, wv = tid >> 6;
-  const int km1 = closed ? wrap(k - 1, nseg > 0 ? nseg : 1) : k - 1;
+  const int nz = nseg > 0 ? nseg : 1;
+  const int km1 = closed ? wrap(k - 1, nz) : k - 1;
+  auto slot_of = [&](int v) { return closed ? wrap(v - (k0 - 1), nz) : v - (k0 - 1); };
   for (int base = 0; base < ns; base += TILE) {
     const int q = base + tid;
     bool rel = false;
@@ -765,7 +814,6 @@ __device__ void strand_tile(const Inst<Real>& I, const TileItem tl, StrandLDS<Re
     if (q < ns) {
       if (base != 0) side_load(I.side + so + q, r);
       if (closed) {
-        const int nz = nseg > 0 ? nseg : 1;
         int dp = wrap(r.seg_pre - (k0 - 1), nz), dq = wrap(r.seg_post - k0, nz);
         rel = dp < cnt + 1 || dq < cnt;
       } else {
@@ -775,14 +823,24 @@ __device__ void strand_tile(const Inst<Real>& I, const TileItem tl, StrandLDS<Re
     unsigned long long mask = __ballot(rel);
     int pos = __popcll(mask & ((1ull << lane) - 1ull));
     if (lane == 0) L.wcnt[wv] = __popcll(mask);
+    L.glo[t] = 0x7fffffff; L.ghi[t] = -1;
     __syncthreads();
     int off = 0, tot = 0;
     for (int w = 0; w < TILE / 64; w++) { if (w < wv) off += L.wcnt[w]; tot += L.wcnt[w]; }
-    if (rel) { side_store(r, L.rec + off + pos); L.rpre[off + pos] = r.seg_pre; L.rpost[off + pos] = r.seg_post; }
+    if (rel) {
+      const int g = off + pos;
+      side_store(r, L.rec + g);
+      L.rpre[g] = r.seg_pre; L.rpost[g] = r.seg_post;
+      const int sa = slot_of(r.seg_pre), sb = slot_of(closed ? wrap(r.seg_pre + 1, nz) : r.seg_pre + 1), sc = slot_of(r.seg_post);
+      if (sa >= 1 && sa <= cnt) { atomicMin(&L.glo[sa], g); atomicMax(&L.ghi[sa], g); }
+      if (sb >= 1 && sb <= cnt) { atomicMin(&L.glo[sb], g); atomicMax(&L.ghi[sb], g); }
+      if (sc >= 1 && sc <= cnt) { atomicMin(&L.glo[sc], g); atomicMax(&L.ghi[sc], g); }
+    }
     __syncthreads();
     if (ownNode) {
       const bool hasStart = closed || k < nseg, hasEnd = closed || k >= 1;
-      for (int g = 0; g < tot; g++) {
+      const int glo = L.glo[t], ghi = L.ghi[t];
+      for (int g = glo; g <= ghi; g++) {
         const int sp = L.rpre[g], sq = L.rpost[g];
         const bool asA = hasStart && sp == k, asB = hasEnd && sp == km1;
         if (asA | asB) {
@@ -807,20 +865,20 @@ __device__ void strand_tile(const Inst<Real>& I, const TileItem tl, StrandLDS<Re
   if (ownNode) {
     const bool hasEnd = closed || k >= 1, hasStart = closed || k < nseg;
     Real F[3];
-    if (hasEnd && hasStart) { F[0] = L.SFx[t + 1] - L.SFx[t]; F[1] = L.SFy[t + 1] - L.SFy[t]; F[2] = L.SFz[t + 1] - L.SFz[t]; }
-    else if (hasStart) { F[0] = L.SFx[t + 1]; F[1] = L.SFy[t + 1]; F[2] = L.SFz[t + 1]; }
-    else if (hasEnd) { F[0] = -L.SFx[t]; F[1] = -L.SFy[t]; F[2] = -L.SFz[t]; }
+    if (hasEnd && hasStart) { F[0] = L.SFx[t] - L.SFx[t - 1]; F[1] = L.SFy[t] - L.SFy[t - 1]; F[2] = L.SFz[t] - L.SFz[t - 1]; }
+    else if (hasStart) { F[0] = L.SFx[t]; F[1] = L.SFy[t]; F[2] = L.SFz[t]; }
+    else if (hasEnd) { F[0] = -L.SFx[t - 1]; F[1] = -L.SFy[t - 1]; F[2] = -L.SFz[t - 1]; }
     else { F[0] = F[1] = F[2] = Real(0); }
     Real ld[3] = {ldx, ldy, ldz};
     if (I.npl && (llx != Real(0) || lly != Real(0) || llz != Real(0))) {
       // mean node frame from the adjacent segments (nodeQuats: the starting segment is the reference)
-      const int is = hasStart ? t + 1 : t;
+      const int is = hasStart ? t : t - 1;
       double acc[4] = {0, 0, 0, 0};
-      if (hasStart) { acc[0] += L.Qw[t + 1]; acc[1] += L.Qx[t + 1]; acc[2] += L.Qy[t + 1]; acc[3] += L.Qz[t + 1]; }
+      if (hasStart) { acc[0] += L.Qw[t]; acc[1] += L.Qx[t]; acc[2] += L.Qy[t]; acc[3] += L.Qz[t]; }
       if (hasEnd) {
-        double dd = L.Qw[t] * L.Qw[is] + L.Qx[t] * L.Qx[is] + L.Qy[t] * L.Qy[is] + L.Qz[t] * L.Qz[is];
+        double dd = L.Qw[t - 1] * L.Qw[is] + L.Qx[t - 1] * L.Qx[is] + L.Qy[t - 1] * L.Qy[is] + L.Qz[t - 1] * L.Qz[is];
         double sg = dd < 0 ? -1.0 : 1.0;
-        acc[0] += sg * L.Qw[t]; acc[1] += sg * L.Qx[t]; acc[2] += sg * L.Qy[t]; acc[3] += sg * L.Qz[t];
+        acc[0] += sg * L.Qw[t - 1]; acc[1] += sg * L.Qx[t - 1]; acc[2] += sg * L.Qy[t - 1]; acc[3] += sg * L.Qz[t - 1];
       }
       double Rn[9];
       qframe_d(acc[0], acc[1], acc[2], acc[3], Rn);
@@ -846,22 +904,22 @@ __device__ void strand_tile(const Inst<Real>& I, const TileItem tl, StrandLDS<Re
       if (fx & 2) V[1] = Real(0);
       if (fx & 4) V[2] = Real(0);
       const int nx = cur ^ 1;
-      I.x[nx][i] = L.Xx[t + 1] + double(V[0]);
-      I.y[nx][i] = L.Xy[t + 1] + double(V[1]);
-      I.z[nx][i] = L.Xz[t + 1] + double(V[2]);
+      I.x[nx][i] = L.Xx[t] + double(V[0]);
+      I.y[nx][i] = L.Xy[t] + double(V[1]);
+      I.z[nx][i] = L.Xz[t] + double(V[2]);
       I.vx[i] = V[0]; I.vy[i] = V[1]; I.vz[i] = V[2];
       keN = double(ms) * (double(V[0]) * double(V[0]) + double(V[1]) * double(V[1]) + double(V[2]) * double(V[2]));
     }
   }
   // ---------------------------------------------------------------- frames: body torque, inertia, ω, q ← q ⊗ exp(ω)
   if (ownSeg) {
-    Real tm[3] = {ST[0] + L.JAx[t + 1] + L.JBx[t], ST[1] + L.JAy[t + 1] + L.JBy[t], ST[2] + L.JAz[t + 1] + L.JBz[t]};
+    Real tm[3] = {ST[0] + L.JAx[t] + L.JBx[t - 1], ST[1] + L.JAy[t] + L.JBy[t - 1], ST[2] + L.JAz[t] + L.JBz[t - 1]};
     if (I.C > 0) {
       for (int e = I.clamp_off[j]; e < I.clamp_off[j + 1]; e++) {
         const int cc = I.clamp_idx[e];
         Real ps[3]; double dq[4];
         Real th = relrot_R<Real>(I.c_Q[4 * cc], I.c_Q[4 * cc + 1], I.c_Q[4 * cc + 2], I.c_Q[4 * cc + 3],
-                                 L.Qw[t + 1], L.Qx[t + 1], L.Qy[t + 1], L.Qz[t + 1], ps, dq);
+                                 L.Qw[t], L.Qx[t], L.Qy[t], L.Qz[t], ps, dq);
         Real lc = I.c_len[cc];
         Real d[3] = {ps[0] / lc, ps[1] / lc, ps[2] / lc};
         Real mm[3] = {I.c_B[3 * cc] * d[0], I.c_B[3 * cc + 1] * d[1], I.c_B[3 * cc + 2] * d[2]};
@@ -876,7 +934,7 @@ __device__ void strand_tile(const Inst<Real>& I, const TileItem tl, StrandLDS<Re
     }
     if (Tc[0] != Real(0) || Tc[1] != Real(0) || Tc[2] != Real(0)) {
       double Rs[9];
-      qframe_d(L.Qw[t + 1], L.Qx[t + 1], L.Qy[t + 1], L.Qz[t + 1], Rs);
+      qframe_d(L.Qw[t], L.Qx[t], L.Qy[t], L.Qz[t], Rs);
       tm[0] += Real(Rs[0]) * Tc[0] + Real(Rs[1]) * Tc[1] + Real(Rs[2]) * Tc[2];
       tm[1] += Real(Rs[3]) * Tc[0] + Real(Rs[4]) * Tc[1] + Real(Rs[5]) * Tc[2];
       tm[2] += Real(Rs[6]) * Tc[0] + Real(Rs[7]) * Tc[1] + Real(Rs[8]) * Tc[2];
@@ -884,9 +942,9 @@ __device__ void strand_tile(const Inst<Real>& I, const TileItem tl, StrandLDS<Re
     mxT = sqrt(double(tm[0]) * double(tm[0]) + double(tm[1]) * double(tm[1]) + double(tm[2]) * double(tm[2]));
     if (!OUT) {
       const Real msc = Real(I.mass_scale);
-      Real I3[3] = {msc * dmax(krx + kTw + (L.KDx[t] + L.KDx[t + 1]), Real(1e-12)),
-                    msc * dmax(kry + kTw + (L.KDy[t] + L.KDy[t + 1]), Real(1e-12)),
-                    msc * dmax(krz + kTw + (L.KDz[t] + L.KDz[t + 1]), Real(1e-12))};
+      Real I3[3] = {msc * dmax(krx + kTw + (L.KDx[t - 1] + L.KDx[t]), Real(1e-12)),
+                    msc * dmax(kry + kTw + (L.KDy[t - 1] + L.KDy[t]), Real(1e-12)),
+                    msc * dmax(krz + kTw + (L.KDz[t - 1] + L.KDz[t]), Real(1e-12))};
       Real W[3] = {wx + tm[0] / I3[0], wy + tm[1] / I3[1], wz + tm[2] / I3[2]};
       W[2] = clampv(W[2], Real(-TWIST_RATE_MAX), Real(TWIST_RATE_MAX));
       if (kTw > Real(0)) W[2] *= Real(ROLL_DAMP);
@@ -904,7 +962,7 @@ __device__ void strand_tile(const Inst<Real>& I, const TileItem tl, StrandLDS<Re
         float sh = sinf(0.25f * a), sc = a > 1e-12f ? sinf(0.5f * a) / a : 0.5f;
         e0 = double(-2.0f * sh * sh); e1 = double(W[0] * sc); e2 = double(W[1] * sc); e3 = double(W[2] * sc);
       }
-      const double qw = L.Qw[t + 1], qx = L.Qx[t + 1], qy = L.Qy[t + 1], qz = L.Qz[t + 1];
+      const double qw = L.Qw[t], qx = L.Qx[t], qy = L.Qy[t], qz = L.Qz[t];
       double rw = qw + (qw * e0 - qx * e1 - qy * e2 - qz * e3);
       double rx = qx + (qw * e1 + qx * e0 + qy * e3 - qz * e2);
       double ry = qy + (qw * e2 - qx * e3 + qy * e0 + qz * e1);
@@ -931,10 +989,11 @@ __device__ void strand_tile(const Inst<Real>& I, const TileItem tl, StrandLDS<Re
     }
     v[q] = x;
   }
-  if (lane == 0) {
-    double* sp = I.spart + (size_t(tile_local) * (TILE / 64) + wv) * SPART;
+  if (lane == 0) {   // [value][wavefront] so that the control kernel's loads coalesce
+    const size_t np = size_t(I.ntiles) * I.tile_waves;
+    double* sp = I.spart + size_t(tl.tile_local) * (TILE / 64) + wv;
 #pragma unroll
-    for (int q = 0; q < 11; q++) sp[q] = v[q];
+    for (int q = 0; q < 11; q++) sp[q * np] = v[q];
   }
 }
 
@@ -963,33 +1022,34 @@ __device__ __forceinline__ void block_reduce(double (*red)[TILE / 64], double* v
 // one workgroup per instance: fixed-order reduction of the tile and crossing-block partials, then kinetic damping,
 // the residual, the stop test and the history ring (solver_cpu.js:809-837; wgsl.js:990-1047)
 template <typename Real>
-__global__ __launch_bounds__(256) void k_ctrl(const Inst<Real>* __restrict__ insts, int32_t B) {
-  __shared__ double red[13][4];
+__global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restrict__ insts, int32_t B, Ctl* __restrict__ ctls) {
+  __shared__ double red[13][CTRL_THREADS / 64];
   const int b = blockIdx.x;
   if (b >= B) return;
   const Inst<Real>& I = insts[b];
-  Ctl* ctl = I.ctl;
-  if (!inst_live(ctl)) return;
+  Ctl* ctl = ctls + b;                             // read beside the descriptor, not through it
   const int tid = threadIdx.x;
+  Ctl c = *ctl;
+  // the partial loads are issued before the liveness test (a stopped instance's partials are read and discarded)
   double s12[13];
   for (int q = 0; q < 7; q++) s12[q] = 0.0;
   for (int q = 7; q < 13; q++) s12[q] = q == 12 ? -INFINITY : 0.0;
-  for (int w = tid; w < I.ntiles * I.tile_waves; w += 256) {
-    const double* sp = I.spart + size_t(w) * SPART;
-    double t0 = sp[0], t1 = sp[1], t2 = sp[2], t3 = sp[3], t4 = sp[4], t5 = sp[5];
-    double m0 = sp[6], m1 = sp[7], m2 = sp[8], m3 = sp[9], m4 = sp[10];
-    s12[0] += t0; s12[1] += t1; s12[2] += t2; s12[3] += t3; s12[4] += t4; s12[5] += t5;
-    s12[7] = fmax(s12[7], m0); s12[8] = fmax(s12[8], m1); s12[9] = fmax(s12[9], m2); s12[10] = fmax(s12[10], m3); s12[11] = fmax(s12[11], m4);
+  const int np = I.ntiles * I.tile_waves;
+  for (int w = tid; w < np; w += CTRL_THREADS) {
+    const double* sp = I.spart + w;
+#pragma unroll
+    for (int q = 0; q < 6; q++) s12[q] += sp[size_t(q) * np];
+#pragma unroll
+    for (int q = 6; q < 11; q++) s12[q + 1] = fmax(s12[q + 1], sp[size_t(q) * np]);
   }
-  for (int w = tid; w < I.ncblk; w += 256) {
-    const double* cp = I.cpart + size_t(w) * CPART;
-    s12[6] += cp[0];
-    s12[12] = fmax(s12[12], cp[1]);
+  for (int w = tid; w < I.ncblk; w += CTRL_THREADS) {
+    s12[6] += I.cpart[w];
+    s12[12] = fmax(s12[12], I.cpart[I.ncblk + w]);
   }
-  block_reduce<256, 13>(red, s12, 7);
+  if (!(c.done == 0 && c.iter < c.stop_at)) return;   // uniform over the workgroup
+  block_reduce<CTRL_THREADS, 13>(red, s12, 7);
   if (tid != 0) return;
   const double eCont = s12[6];
-  Ctl c = *ctl;
   double ke = 0.5 * (s12[0] + s12[1]);
   int rollback = 0;
   if (I.kin_damp && ke < c.ke_prev) { rollback = 1; ke = 0.0; }
@@ -1038,15 +1098,16 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
 }
 
 template <typename Real, int TILE>
-__global__ __launch_bounds__(TILE) void k_strand(const Inst<Real>* __restrict__ insts, const TileItem* __restrict__ tiles, int32_t ntiles) {
+__global__ __launch_bounds__(TILE) void k_strand(const Inst<Real>* __restrict__ insts, const TileItem* __restrict__ tiles, int32_t ntiles,
+                                                 const Ctl* __restrict__ ctls) {
   __shared__ StrandLDS<Real, TILE> L;
   int lt = xcd_tile(blockIdx.x, gridDim.x);
   if (lt >= ntiles) return;
-  TileItem tl = tiles[lt];
+  const TileItem tl = tiles[lt];
   if (tl.inst < 0) return;
-  const Inst<Real>& I = insts[tl.inst];
-  if (!inst_live(I.ctl)) return;
-  strand_tile<Real, TILE, false>(I, tl, L, lt - I.tile0, nullptr, nullptr, nullptr, nullptr, nullptr);
+  const Ctl* ctl = ctls + tl.inst;
+  if (!inst_live(ctl)) return;
+  strand_tile<Real, TILE, false>(insts[tl.inst], ctl, tl, L, nullptr, nullptr, nullptr, nullptr, nullptr);
 }
 
 // outputs of the last evaluated state of one instance: N, V, M, clamp moments, reactions (download)
@@ -1055,27 +1116,25 @@ __global__ __launch_bounds__(TILE) void k_outputs(const Inst<Real>* __restrict__
                                                   double* outN, double* outV, double* outM, double* outCM, double* outR) {
   __shared__ StrandLDS<Real, TILE> L;
   const Inst<Real>& I = insts[inst];
-  int lt = I.tile0 + blockIdx.x;
-  if (blockIdx.x >= I.ntiles) return;
-  TileItem tl = tiles[lt];
-  strand_tile<Real, TILE, true>(I, tl, L, blockIdx.x, outN, outV, outM, outCM, outR);
+  if (int(blockIdx.x) >= I.ntiles) return;
+  const TileItem tl = tiles[I.tile0 + blockIdx.x];
+  strand_tile<Real, TILE, true>(I, I.ctl, tl, L, outN, outV, outM, outCM, outR);
 }
 
-// plain copy of n 16-byte words, grid-stride (HBM counter calibration)
 __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
   for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) dst[i] = src[i];
 }
 
 // explicit instantiations
 #define WSD_INST(R, T) \
-  template __global__ void k_strand<R, T>(const Inst<R>*, const TileItem*, int32_t); \
+  template __global__ void k_strand<R, T>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
   template __global__ void k_outputs<R, T>(const Inst<R>*, const TileItem*, int32_t, double*, double*, double*, double*, double*);
 WSD_INST(float, 64) WSD_INST(float, 128) WSD_INST(float, 256)
 WSD_INST(double, 64) WSD_INST(double, 128) WSD_INST(double, 256)
-template __global__ void k_cross<float>(const Inst<float>*, const CrossItem*, int32_t);
-template __global__ void k_cross<double>(const Inst<double>*, const CrossItem*, int32_t);
-template __global__ void k_ctrl<float>(const Inst<float>*, int32_t);
-template __global__ void k_ctrl<double>(const Inst<double>*, int32_t);
+template __global__ void k_cross<float>(const Inst<float>*, const CrossItem*, int32_t, const Ctl*);
+template __global__ void k_cross<double>(const Inst<double>*, const CrossItem*, int32_t, const Ctl*);
+template __global__ void k_ctrl<float>(const Inst<float>*, int32_t, Ctl*);
+template __global__ void k_ctrl<double>(const Inst<double>*, int32_t, Ctl*);
 template __global__ void k_begin<float>(const Inst<float>*, const int32_t*, int32_t, int32_t, int32_t);
 template __global__ void k_begin<double>(const Inst<double>*, const int32_t*, int32_t, int32_t, int32_t);
 

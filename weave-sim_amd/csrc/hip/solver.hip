@@ -20,9 +20,9 @@
 
 namespace wsd {
 template <typename Real> __global__ void k_begin(const Inst<Real>*, const int32_t*, int32_t, int32_t, int32_t);
-template <typename Real> __global__ void k_cross(const Inst<Real>*, const CrossItem*, int32_t);
-template <typename Real, int TILE> __global__ void k_strand(const Inst<Real>*, const TileItem*, int32_t);
-template <typename Real> __global__ void k_ctrl(const Inst<Real>*, int32_t);
+template <typename Real> __global__ void k_cross(const Inst<Real>*, const CrossItem*, int32_t, const Ctl*);
+template <typename Real, int TILE> __global__ void k_strand(const Inst<Real>*, const TileItem*, int32_t, const Ctl*);
+template <typename Real> __global__ void k_ctrl(const Inst<Real>*, int32_t, Ctl*);
 __global__ void k_copy(const uint4*, uint4*, int64_t);
 template <typename Real, int TILE> __global__ void k_outputs(const Inst<Real>*, const TileItem*, int32_t, double*, double*, double*, double*, double*);
 }  // namespace wsd
@@ -128,7 +128,17 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   H.tiles.clear();
   for (int s = 0; s < S; s++) {
     int nn = md.node_offset[s + 1] - md.node_offset[s];
-    for (int k0 = 0; k0 < nn; k0 += tile) H.tiles.push_back(TileItem{0, s, k0, std::min(nn, k0 + tile)});
+    // a tile of `tile` lanes owns tile − 1 nodes: lane 0 computes the halo segment and joint before them
+    for (int k0 = 0; k0 < nn; k0 += tile - 1) {
+      TileItem t{};
+      t.inst = 0; t.strand = s; t.k0 = k0; t.k1 = std::min(nn, k0 + tile - 1);
+      t.noff = md.node_offset[s]; t.nn = nn;
+      t.soff = md.seg_offset[s]; t.nseg = md.seg_offset[s + 1] - md.seg_offset[s];
+      t.joff = md.joint_offset[s]; t.njs = md.joint_offset[s + 1] - md.joint_offset[s];
+      t.closed = md.strand_closed[s] ? 1 : 0;
+      t.tile_local = int(H.tiles.size());
+      H.tiles.push_back(t);
+    }
   }
   d.ntiles = int(H.tiles.size());
   H.ncblk = (K + CROSS_BLOCK - 1) / CROSS_BLOCK;
@@ -178,6 +188,7 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   int slot = 0;
   for (int s = 0; s < S; s++) { std::sort(per[s].begin(), per[s].end()); for (auto& e : per[s]) spos[e.second] = slot++; soff[s + 1] = slot; }
   h2d(d.side_off, soff); h2d(d.side_pos, spos);
+  for (auto& t : H.tiles) { t.so = soff[t.strand]; t.ns = soff[t.strand + 1] - soff[t.strand]; }
   H.cseg.assign(md.c_seg, md.c_seg + C);
   if (C) {
     std::vector<int32_t> coff(m + 1, 0), cidx(C);
@@ -371,16 +382,16 @@ struct Batch : BatchBase {
   }
 
   void launch_cross(hipStream_t s) {
-    if (ncross) hipLaunchKernelGGL(k_cross<Real>, dim3(ncross), dim3(CROSS_BLOCK), 0, s, d_inst, d_cross, ncross);
+    if (ncross) hipLaunchKernelGGL(k_cross<Real>, dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls);
   }
   void launch_strand(hipStream_t s) {
     switch (tile) {
-      case 64: hipLaunchKernelGGL((k_strand<Real, 64>), dim3(ntiles), dim3(64), 0, s, d_inst, d_tiles, ntiles); break;
-      case 128: hipLaunchKernelGGL((k_strand<Real, 128>), dim3(ntiles), dim3(128), 0, s, d_inst, d_tiles, ntiles); break;
-      default: hipLaunchKernelGGL((k_strand<Real, 256>), dim3(ntiles), dim3(256), 0, s, d_inst, d_tiles, ntiles); break;
+      case 64: hipLaunchKernelGGL((k_strand<Real, 64>), dim3(ntiles), dim3(64), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
+      case 128: hipLaunchKernelGGL((k_strand<Real, 128>), dim3(ntiles), dim3(128), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
+      default: hipLaunchKernelGGL((k_strand<Real, 256>), dim3(ntiles), dim3(256), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
     }
   }
-  void launch_ctrl(hipStream_t s) { hipLaunchKernelGGL(k_ctrl<Real>, dim3(B), dim3(256), 0, s, d_inst, B); }
+  void launch_ctrl(hipStream_t s) { hipLaunchKernelGGL(k_ctrl<Real>, dim3(B), dim3(CTRL_THREADS), 0, s, d_inst, B, d_ctls); }
   void launch_iteration(hipStream_t s) { launch_cross(s); launch_strand(s); launch_ctrl(s); }
 
   void ensure_graph() {
