@@ -1,33 +1,60 @@
 """Benchmark of the MI355X dynamic-relaxation path: node-iterations per second of woven kagome gridshells.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--precision fp32|fp64] [--tile 0|64|128|256]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dome_10m] [--precision fp32|fp64]
 
 A step is one dynamic-relaxation iteration (crossing kernel, strand kernel, control kernel) over the whole input.
-Default workload: the single large instance — a kagome dome of 10 m developed width, 0.15 m strand spacing,
-0.02 m segments, 12 % contraction, seeded ±2 mm noise (78,606 nodes, 9,900 crossings), FP32 forces with FP64
-positions, frames, kinetic energy and residual.  Other configurations: 1 (3 m dome, ~1.4k nodes), 3 (256 ragged
-design variants in one batch), 5 (30 m dome, μ = 0.05, ~0.8M nodes).
+The default workload is the single large instance: a kagome dome of 10 m span, 0.15 m lath spacing, 20 mm
+segments, 12 % boundary contraction, seeded ±2 mm imperfection (78,606 nodes, 9,900 crossings), FP32 forces with
+FP64 positions, frames, kinetic energy and residual.  Other workloads: dome_3m (~1.4k nodes), dome_30m (~0.8M
+nodes, μ = 0.05), variants (a ragged batch of design variants, 2k–50k nodes each).
 
-Prints one JSON line.  `value` is whole-job node-iterations/s with the inputs resident in HBM; `roofline` is the
-strand kernel (the dominant launch) against the 8 TB/s HBM peak; `cpu_baseline` is the f64 oracle on this host.
+Prints one JSON line.  `value` is whole-job node-iterations/s with the inputs resident in HBM.  `roofline` is the
+strand kernel (the dominant launch): algorithmic bytes per launch over its mean duration (HIP events on the
+solver's stream) against the 8 TB/s HBM peak, and `traffic` the HBM bytes per launch read by two rocprofv3
+counter passes (FETCH_SIZE, WRITE_SIZE) scaled by what the same counters read for a plain copy kernel.
+`cpu_baseline` is the f64 oracle (a C++ restatement of the reference's CpuSolver) on this host, one thread.
 """
 import argparse
+import csv
+import glob
 import importlib
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 256 << 20          # the calibration copy: 256 MiB read, 256 MiB written
+
+WORKLOADS = {
+    "dome_3m": "kagome dome 3 m span / 0.3 m spacing / 50 mm segments, 15% contraction, +-1 mm imperfection, mu 0.3",
+    "dome_10m": "kagome dome 10 m span / 0.15 m spacing / 20 mm segments, 12% contraction, +-2 mm imperfection, mu 0.3",
+    "dome_30m": "kagome dome 30 m span / 0.2 m spacing / 15 mm segments, 18% contraction, +-2 mm imperfection, mu 0.05",
+    "variants": "ragged batch of seeded design variants (span 2-8 m, spacing 0.2-0.4 m, mu 0.1-0.5, "
+                "contraction 8-20%, thickness 3-6 mm, 2k-50k nodes each)",
+}
+
+
+def scenes():
+    return importlib.import_module("weave-sim_amd.scenes")
+
+
+def build_models(workload, batch):
+    S = scenes()
+    if workload == "variants":
+        return [S.design_variant(i) for i in range(batch)]
+    return [getattr(S, workload)()]
 
 
 def strand_cross_bytes(models, precision):
-    """algorithmic HBM bytes per iteration split by kernel: every per-iteration array read once and written once
-    if it changes (the same accounting as ws_batch_bytes)"""
+    """algorithmic HBM bytes per iteration of the strand and crossing kernels: every per-iteration array read once
+    and, if it changes, written once; parameters read once (the same accounting as ws_batch_bytes)"""
     R = 4.0 if precision == "fp32" else 8.0
     sb = cb = 0.0
     for m in models:
@@ -37,21 +64,8 @@ def strand_cross_bytes(models, precision):
     return sb, cb
 
 
-def build_models(config, batch=256):
-    import cases
-    if config == 1:
-        return [cases.config1()], "kagome dome 3 m / 0.3 m spacing / 0.05 m segments, 15% contraction, ±1 mm noise"
-    if config == 2:
-        return [cases.config2()], "kagome dome 10 m / 0.15 m spacing / 0.02 m segments, 12% contraction, ±2 mm noise"
-    if config == 3:
-        return [cases.config3_variant(i) for i in range(batch)], f"{batch} ragged kagome design variants, 2k-50k nodes each"
-    if config == 5:
-        return [cases.config5()], "kagome dome 30 m / 0.2 m spacing / 0.015 m segments, mu 0.05, 18% contraction"
-    raise SystemExit(f"unknown config {config}")
-
-
 def cpu_baseline(model, budget_s=12.0):
-    """the f64 oracle (a port of the reference's CpuSolver, 1 thread) on the same instance, a bounded sample"""
+    """the f64 oracle on the same instance from its initial state, one host thread, a bounded sample"""
     from oracle.bind import Oracle
     o = Oracle(model)
     t = time.perf_counter()
@@ -63,56 +77,111 @@ def cpu_baseline(model, budget_s=12.0):
     el = time.perf_counter() - t
     return {"value": model.n * its / el, "unit": "node-iterations/s", "cores": 1, "kind": "port",
             "sample": f"{its} iterations of the same {model.n}-node instance from its initial state, f64 oracle "
-                      f"(CpuSolver restated in C++), one host thread, {el:.1f} s"}
+                      f"(CpuSolver restated in C++, -O3, no fast-math), one host thread, {el:.1f} s"}
 
 
+# ------------------------------------------------------------------------------------------------ counter passes
+def kernel_only(args):
+    """the workload of one counter pass: the calibration copy, then warm-up iterations and strand launches"""
+    W = importlib.import_module("weave-sim_amd")
+    ctx = W.solver.default_context()
+    ctx.copy_probe(COPY_BYTES, 3)
+    models = build_models(args.workload, args.batch)
+    g = W.GpuSolver(models, precision=args.precision, tile=args.tile)
+    g.step(64)
+    g.kernel_bench(10)
+    g.close()
+
+
+def counter_pass(args, counter, outdir):
+    """one rocprofv3 pass over kernel_only; returns {kernel: mean counter value per dispatch}"""
+    cmd = ["timeout", "-s", "KILL", "150", "rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", outdir,
+           "-o", "pass", "--", sys.executable, os.path.abspath(__file__), "--kernel-only", "--workload", args.workload,
+           "--precision", args.precision, "--tile", str(args.tile), "--batch", str(args.batch)]
+    env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
+    r = subprocess.run(cmd, cwd=tempfile.gettempdir(), env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError(f"rocprofv3 {counter}: exit {r.returncode}, no counter file")
+    acc = {}
+    with open(files[0]) as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] != counter:
+                continue
+            name = row["Kernel_Name"]
+            key = "k_strand" if "k_strand" in name else "k_copy" if "k_copy" in name else None
+            if key:
+                acc.setdefault(key, []).append(float(row["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def measure_traffic(args):
+    """HBM bytes per strand launch: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the factor that
+    maps the same counter onto the known bytes of the copy kernel"""
+    if shutil.which("rocprofv3") is None:
+        return None, {"error": "rocprofv3 not found"}
+    tmp = tempfile.mkdtemp(prefix="ws_traffic_")
+    try:
+        fetch = counter_pass(args, "FETCH_SIZE", os.path.join(tmp, "fetch"))
+        write = counter_pass(args, "WRITE_SIZE", os.path.join(tmp, "write"))
+        f_fetch = COPY_BYTES / (fetch["k_copy"] * 1024.0)
+        f_write = COPY_BYTES / (write["k_copy"] * 1024.0)
+        t = fetch["k_strand"] * 1024.0 * f_fetch + write["k_strand"] * 1024.0 * f_write
+        return t, {"fetch_kib": fetch["k_strand"], "write_kib": write["k_strand"], "fetch_factor": f_fetch,
+                   "write_factor": f_write, "copy_fetch_kib": fetch["k_copy"], "copy_write_kib": write["k_copy"]}
+    except Exception as e:   # the throughput line stands without the counter passes
+        return None, {"error": str(e)[:200]}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
+
+# ------------------------------------------------------------------------------------------------ main
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3200)
     ap.add_argument("--warmup", type=int, default=320)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--workload", default="dome_10m", choices=sorted(WORKLOADS))
+    ap.add_argument("--batch", type=int, default=64, help="instances of the variants workload")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--tile", type=int, default=0, help="strand tile (64/128/256); 0 = by size")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
-    ap.add_argument("--batch", type=int, default=256, help="instances of config 3")
-    ap.add_argument("--copy-probe", type=int, default=0, help="only run the copy kernel over this many bytes")
+    ap.add_argument("--kernel-only", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: runs on one GPU only (got --gpus {args.gpus}); instances are independent and would shard "
-              "one solve per GPU, which this build does not launch", file=sys.stderr)
+        print(f"bench.py runs on one GPU (got --gpus {args.gpus}): instances are independent solves that would "
+              "shard one per GPU, and this build launches none across GPUs", file=sys.stderr)
         sys.exit(2)
-    W = importlib.import_module("weave-sim_amd")
-    if args.copy_probe:
-        ctx = W.solver.default_context()
-        ms = ctx.copy_probe(args.copy_probe, 20)
-        print(json.dumps({"copy_bytes": args.copy_probe, "ms": ms, "GBps": 2 * args.copy_probe / (ms * 1e-3) / 1e9}))
+    if args.kernel_only:
+        kernel_only(args)
         return
-    models, desc = build_models(args.config, args.batch)
+    # counter passes first, as child processes, before this process touches the GPU
+    traffic, traffic_detail = (None, {"skipped": True}) if args.no_traffic else measure_traffic(args)
+
+    W = importlib.import_module("weave-sim_amd")
+    models = build_models(args.workload, args.batch)
     n_nodes = sum(m.n for m in models)
     n_cross = sum(m.K for m in models)
     g = W.GpuSolver(models, precision=args.precision, tile=args.tile)
     dev = g.ctx.info()
-    # warmup, then exactly K iterations between two synchronisations of the solver's stream
     g.step(args.warmup)
     it0 = sum(c["iter"] for c in g.read_control())
     t0 = time.perf_counter()
-    ctl = g.step(args.steps)
+    ctl = g.step(args.steps)          # returns after a synchronisation of the solver's stream
     t1 = time.perf_counter()
     it1 = sum(c["iter"] for c in ctl)
-    ev_ms = g.timing()["ms_last_step"]
     wall = t1 - t0
-    # every instance must have run exactly K iterations (none may stop inside the timed region)
     if it1 - it0 != args.steps * len(models):
         raise SystemExit("an instance stopped inside the timed region; rerun with fewer steps")
-    node_iters = n_nodes * args.steps
-    value = node_iters / wall
-    # dominant kernel: mean duration of a strand-kernel launch on the solver's stream
+    value = n_nodes * args.steps / wall
+    # dominant kernel: mean duration of a strand launch (HIP events on the solver's stream)
     kb = g.kernel_bench(args.kernel_reps)
     sb, cb = strand_cross_bytes(models, args.precision)
     achieved = sb / (kb["ms_strand"] * 1e-3) / 1e9
     alg = g.bytes()["alg_bytes_per_iter"]
+    tile = g.tile
     g.close()
     line = {
         "metric": "node-iters/s",
@@ -126,19 +195,18 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64" if args.precision == "fp32" else "f64",
-        "data": "synthetic (seeded kagome domes built by the host model builder)",
-        "config": {"workload": f"config {args.config}: {desc}", "instances": len(models), "nodes": n_nodes,
-                   "crossings": n_cross, "precision": args.precision, "device": dev["name"]},
+        "data": "synthetic (seeded kagome domes from the host model builder)",
+        "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]}", "instances": len(models),
+                   "nodes": n_nodes, "crossings": n_cross, "precision": args.precision, "strand_tile": tile,
+                   "device": dev["name"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_strand",
-                     "alg_bytes_per_launch": sb, "ms_per_launch": kb["ms_strand"]},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_strand",
+                     "alg_bytes_per_launch": sb, "ms_per_launch": kb["ms_strand"], "traffic_detail": traffic_detail},
         "kernels_ms": kb,
         "alg_bytes_per_iter": alg,
-        "whole_iteration_hbm_frac": alg / (wall / args.steps) / (HBM_PEAK_GBS * 1e9),
-        "event_ms_timed_region": ev_ms,
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(models[0] if len(models) == 1 else max(models, key=lambda m: m.n))
+        line["cpu_baseline"] = cpu_baseline(max(models, key=lambda m: m.n))
     print(json.dumps(line))
 
 

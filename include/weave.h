@@ -167,9 +167,19 @@ typedef struct ws_control {
 typedef struct ws_batch_opts {
   int32_t precision;       /* ws_precision */
   int32_t per_submit;      /* iterations per captured graph (GpuSolver perSubmit, default 64) */
-  int32_t tile;            /* nodes per strand tile: 64 / 128 / 256; 0 = autotune */
+  int32_t tile;            /* lanes per strand tile: 64 / 128 / 256; 0 = chosen by size; -1 = autotune */
   int32_t history;         /* history ring length per instance (0 = 4096, the reference's HN) */
 } ws_batch_opts;
+
+#define WS_TILE_AUTOTUNE (-1)
+
+/* one autotuner candidate: its options, mean ms per iteration and executed node-iterations per second */
+typedef struct ws_tune_entry {
+  int32_t tile;
+  int32_t per_submit;
+  double ms_per_iter;
+  double node_iters_per_s;
+} ws_tune_entry;
 
 typedef struct ws_ctx ws_ctx;
 typedef struct ws_batch ws_batch;
@@ -203,36 +213,54 @@ WS_API int ws_internalize(ws_batch* batch, int32_t b);                          
 WS_API int ws_batch_timing(ws_batch* batch, double* ms_last_step, double* ms_kernel_cross, double* ms_kernel_strand,
                            double* ms_kernel_ctrl);   /* hipEvent timing of the last step(); per-kernel means of the last profiled step */
 WS_API int ws_batch_bytes(ws_batch* batch, double* alg_bytes_per_iter, double* device_bytes);  /* algorithmic HBM bytes per iteration, device footprint */
+/* on = 1: step() launches the kernels one by one (no graph) and times each with hipEvents (ws_batch_timing) */
 WS_API int ws_batch_profile(ws_batch* batch, int32_t on);
 /* mean duration of one launch of each kernel (crossing, strand, control) from `reps` back-to-back launches of that
  * kernel alone, bracketed by two hipEvents on the batch's stream; for timing only: it advances the state */
-WS_API int ws_batch_kernel_bench(ws_batch* batch, int32_t reps, double* ms_cross, double* ms_strand, double* ms_ctrl);  /* 1: step() launches without graphs and times each kernel with hipEvents */
+WS_API int ws_batch_kernel_bench(ws_batch* batch, int32_t reps, double* ms_cross, double* ms_strand, double* ms_ctrl);
+/* the options in effect (the tile chosen when the batch was created with tile 0 or WS_TILE_AUTOTUNE) */
+WS_API int ws_batch_get_opts(ws_batch* batch, ws_batch_opts* out);
+/* autotuner (no counterpart in the reference): relaxes the inputs `iters` iterations under every strand tile
+ * (64 / 128 / 256) and graph length (32 / 64 / 128 iterations), on temporary batches, and returns the fastest in
+ * node-iterations per second; `table` (optional, `cap` rows) receives every candidate */
+WS_API int ws_autotune(ws_ctx* ctx, const ws_model* const* models, const ws_state* const* states, int32_t B,
+                       int32_t precision, int32_t iters, ws_batch_opts* best, ws_tune_entry* table, int32_t cap,
+                       int32_t* n_entries);
 
-/* ---- serving (continuous batching; no counterpart in the reference) ------------------------------ */
+/* ---- serving (continuous batching; no counterpart in the reference, whose batched caller is the calibration
+ * loop, web/js/calibrate.js:438-520) ------------------------------------------------------------------------
+ * Requests (one instance each) are queued by ws_submit and admitted in arrival order by ws_pump while a slot is
+ * free and their device bytes fit the HBM budget; every pump relaxes all resident instances one chunk in shared
+ * launches and retires the ones that converged, reached their iteration limit or went non-finite, writing the final
+ * state into the request's ws_state.  Model and state stay caller-owned and valid until the result is polled.
+ * One host thread per server. */
 typedef struct ws_server_opts {
-  int32_t precision;
-  int32_t chunk;           /* iterations per relaxation chunk between admissions */
-  int32_t tile;
-  int32_t max_slots;       /* instances resident at once */
-  int64_t hbm_budget;      /* bytes of device state the server may hold */
+  int32_t precision;       /* ws_precision */
+  int32_t chunk;           /* iterations per pump (0 = 64) */
+  int32_t tile;            /* lanes per strand tile: 64 / 128 / 256 (0 = 64) */
+  int32_t max_slots;       /* instances resident at once (0 = 256) */
+  int64_t hbm_budget;      /* bytes of instance state the server may hold (0 = 80 % of the free memory) */
 } ws_server_opts;
 
 typedef struct ws_result {
   int64_t id;              /* request id returned by ws_submit */
-  int32_t status;          /* 0 converged, 1 max_iter reached, 2 cancelled, 3 non-finite state */
-  int32_t iterations;
+  int32_t status;          /* 0 converged, 1 iteration limit reached, 2 cancelled, 3 non-finite state */
+  int32_t iterations;      /* the instance's iteration count at the end */
   double residual;
-  double t_submit_ms, t_admit_ms, t_done_ms;   /* server clock */
+  double t_submit_ms, t_admit_ms, t_done_ms;   /* server clock (ws_server_clock_ms) */
   int32_t n; int32_t K;
 } ws_result;
 
 WS_API int ws_server_create(ws_ctx* ctx, const ws_server_opts* opts, ws_server** out);
 WS_API int ws_server_destroy(ws_server* srv);
-WS_API int ws_submit(ws_server* srv, const ws_model* model, const ws_state* state, int32_t max_iter, int64_t* id_out);
-WS_API int ws_cancel(ws_server* srv, int64_t id);
-WS_API int ws_pump(ws_server* srv, int32_t* n_active, int32_t* n_queued);   /* one chunk: admit, relax, retire */
+/* max_iter: the request's iteration limit (0 = the model's par.max_iter); the request is priced (device bytes) and
+ * validated here, WS_ERR_ARG when it can never fit the budget */
+WS_API int ws_submit(ws_server* srv, const ws_model* model, ws_state* state, int32_t max_iter, int64_t* id_out);
+WS_API int ws_cancel(ws_server* srv, int64_t id);          /* queued or resident; the result has status 2 */
+WS_API int ws_pump(ws_server* srv, int32_t* n_active, int32_t* n_queued);   /* admit, relax one chunk, retire */
 WS_API int ws_poll(ws_server* srv, ws_result* out, int32_t cap, int32_t* n_out);
-WS_API int ws_fetch(ws_server* srv, int64_t id, ws_state* out);             /* final state of a finished request */
+WS_API int ws_fetch(ws_server* srv, int64_t id, ws_state* out);             /* snapshot of a resident request */
 WS_API double ws_server_clock_ms(ws_server* srv);
+WS_API int ws_server_stats(ws_server* srv, int32_t* n_active, int32_t* n_queued, int64_t* bytes_in_use, int64_t* hbm_budget);
 
 #endif

@@ -69,6 +69,13 @@ class ws_batch_opts(C.Structure):
     _fields_ = [("precision", i32), ("per_submit", i32), ("tile", i32), ("history", i32)]
 
 
+WS_TILE_AUTOTUNE = -1
+
+
+class ws_tune_entry(C.Structure):
+    _fields_ = [("tile", i32), ("per_submit", i32), ("ms_per_iter", d), ("node_iters_per_s", d)]
+
+
 class ws_server_opts(C.Structure):
     _fields_ = [("precision", i32), ("chunk", i32), ("tile", i32), ("max_slots", i32), ("hbm_budget", i64)]
 
@@ -180,6 +187,9 @@ def hip_lib():
         L.ws_batch_bytes.argtypes = [vp, P_d, P_d]
         L.ws_batch_profile.argtypes = [vp, i32]
         L.ws_batch_kernel_bench.argtypes = [vp, i32, P_d, P_d, P_d]
+        L.ws_batch_get_opts.argtypes = [vp, C.POINTER(ws_batch_opts)]
+        L.ws_autotune.argtypes = [vp, C.POINTER(C.POINTER(ws_model)), C.POINTER(C.POINTER(ws_state)), i32, i32, i32,
+                                  C.POINTER(ws_batch_opts), C.POINTER(ws_tune_entry), i32, C.POINTER(i32)]
         L.ws_server_create.argtypes = [vp, C.POINTER(ws_server_opts), C.POINTER(vp)]
         L.ws_server_destroy.argtypes = [vp]
         L.ws_submit.argtypes = [vp, C.POINTER(ws_model), C.POINTER(ws_state), i32, C.POINTER(i64)]
@@ -189,6 +199,7 @@ def hip_lib():
         L.ws_fetch.argtypes = [vp, i64, C.POINTER(ws_state)]
         L.ws_server_clock_ms.argtypes = [vp]
         L.ws_server_clock_ms.restype = d
+        L.ws_server_stats.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64), C.POINTER(i64)]
         _hip_lib = L
     return _hip_lib
 
@@ -197,8 +208,9 @@ def hip_lib():
 WEAVE_H_SYMBOLS = [
     "ws_ctx_create", "ws_ctx_destroy", "ws_last_error", "ws_device_info", "ws_copy_probe", "ws_batch_create", "ws_batch_destroy",
     "ws_batch_size", "ws_upload", "ws_update_params", "ws_step", "ws_read_control", "ws_read_history", "ws_download",
-    "ws_internalize", "ws_batch_timing", "ws_batch_bytes", "ws_batch_profile", "ws_batch_kernel_bench", "ws_server_create", "ws_server_destroy", "ws_submit",
-    "ws_cancel", "ws_pump", "ws_poll", "ws_fetch", "ws_server_clock_ms",
+    "ws_internalize", "ws_batch_timing", "ws_batch_bytes", "ws_batch_profile", "ws_batch_kernel_bench",
+    "ws_batch_get_opts", "ws_autotune", "ws_server_create", "ws_server_destroy", "ws_submit",
+    "ws_cancel", "ws_pump", "ws_poll", "ws_fetch", "ws_server_clock_ms", "ws_server_stats",
 ]
 WEAVE_MODEL_H_SYMBOLS = [
     "wsm_scene_defaults", "wsm_from_mesh", "wsm_from_polylines", "wsm_hex_dome", "wsm_get", "wsm_info_get",
@@ -206,4 +218,4 @@ WEAVE_MODEL_H_SYMBOLS = [
     "wsm_abi_sizes",
 ]
 ABI_STRUCTS = ["ws_params", "ws_model", "ws_state", "ws_control", "ws_batch_opts", "ws_server_opts", "ws_result",
-               "wsm_scene", "wsm_anchor", "wsm_point_load", "wsm_prep", "wsm_hex_spec", "wsm_info"]
+               "wsm_scene", "wsm_anchor", "wsm_point_load", "wsm_prep", "wsm_hex_spec", "wsm_info", "ws_tune_entry"]

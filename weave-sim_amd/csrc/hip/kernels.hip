@@ -132,11 +132,11 @@ __device__ __forceinline__ bool inst_live(const Ctl* c) { return c->done == 0 &&
 // ================================================================================================= k_begin
 // step(nIter, {instances, force}) (solver_gpu.js:338-345): stop_at = iter + nIter, clear the convergence flag
 template <typename Real>
-__global__ void k_begin(const Inst<Real>* insts, const int32_t* sel, int32_t nsel, int32_t n_iter, int32_t force) {
+__global__ void k_begin(Ctl* ctls, const int32_t* sel, const int32_t* counts, int32_t nsel, int32_t n_iter, int32_t force) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nsel) return;
-  Ctl* c = insts[sel[t]].ctl;
-  c->stop_at = c->iter + n_iter;
+  Ctl* c = ctls + sel[t];
+  c->stop_at = c->iter + (counts ? counts[t] : n_iter);
   if (force) c->done = 0;
 }
 
@@ -1135,7 +1135,7 @@ template __global__ void k_cross<float>(const Inst<float>*, const CrossItem*, in
 template __global__ void k_cross<double>(const Inst<double>*, const CrossItem*, int32_t, const Ctl*);
 template __global__ void k_ctrl<float>(const Inst<float>*, int32_t, Ctl*);
 template __global__ void k_ctrl<double>(const Inst<double>*, int32_t, Ctl*);
-template __global__ void k_begin<float>(const Inst<float>*, const int32_t*, int32_t, int32_t, int32_t);
-template __global__ void k_begin<double>(const Inst<double>*, const int32_t*, int32_t, int32_t, int32_t);
+template __global__ void k_begin<float>(Ctl*, const int32_t*, const int32_t*, int32_t, int32_t, int32_t);
+template __global__ void k_begin<double>(Ctl*, const int32_t*, const int32_t*, int32_t, int32_t, int32_t);
 
 }  // namespace wsd

@@ -1,681 +1,7 @@
-// C ABI of the device solver (include/weave.h): contexts, batches of instances (GpuSolver, web/js/solver_gpu.js:62-523)
-// and the continuous-batching server.  Host code only; the kernels are in kernels.hip.
-#include <hip/hip_runtime.h>
+// C ABI of the device solver (include/weave.h): contexts and batches of instances (GpuSolver,
+// web/js/solver_gpu.js:62-523).  Host code only; the kernels are in kernels.hip.
+#include "batch.hpp"
 
-#include <algorithm>
-#include <chrono>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <deque>
-#include <map>
-#include <memory>
-#include <numeric>
-#include <string>
-#include <type_traits>
-#include <vector>
-
-#include "../../../include/weave.h"
-#include "device.hpp"
-
-namespace wsd {
-template <typename Real> __global__ void k_begin(const Inst<Real>*, const int32_t*, int32_t, int32_t, int32_t);
-template <typename Real> __global__ void k_cross(const Inst<Real>*, const CrossItem*, int32_t, const Ctl*);
-template <typename Real, int TILE> __global__ void k_strand(const Inst<Real>*, const TileItem*, int32_t, const Ctl*);
-template <typename Real> __global__ void k_ctrl(const Inst<Real>*, int32_t, Ctl*);
-__global__ void k_copy(const uint4*, uint4*, int64_t);
-template <typename Real, int TILE> __global__ void k_outputs(const Inst<Real>*, const TileItem*, int32_t, double*, double*, double*, double*, double*);
-}  // namespace wsd
-
-using namespace wsd;
-
-struct ws_ctx {
-  int device = 0;
-  std::string err;
-  hipDeviceProp_t prop{};
-};
-
-namespace {
-
-struct Fail { int code; std::string msg; };
-#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) throw Fail{WS_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)}; } while (0)
-
-inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
-
-// ------------------------------------------------------------------------------------------------ device arena
-struct Arena {
-  size_t size = 0;
-  char* base = nullptr;
-  std::vector<std::pair<void**, size_t>> pending;   // (pointer slot, offset)
-  template <typename T> void add(T*& slot, size_t count) {
-    size_t off = size;
-    size += align256(std::max<size_t>(count, 1) * sizeof(T));
-    pending.push_back({reinterpret_cast<void**>(const_cast<typename std::remove_const<T>::type**>(&slot)), off});
-  }
-  template <typename T> void addc(const T*& slot, size_t count) {
-    size_t off = size;
-    size += align256(std::max<size_t>(count, 1) * sizeof(T));
-    pending.push_back({reinterpret_cast<void**>(const_cast<T**>(&slot)), off});
-  }
-  void alloc() {
-    HIPCHK(hipMalloc(&base, size));
-    HIPCHK(hipMemset(base, 0, size));
-    for (auto& p : pending) *p.first = base + p.second;
-    pending.clear();
-  }
-  void release() { if (base) (void)hipFree(base); base = nullptr; }
-};
-
-template <typename T> void h2d(T* dst, const std::vector<T>& src) {
-  if (!src.empty()) HIPCHK(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
-}
-template <typename T> void h2d(const T* dst, const std::vector<T>& src) { h2d(const_cast<T*>(dst), src); }
-template <typename T> std::vector<T> d2h(const T* src, size_t n) {
-  std::vector<T> v(n);
-  if (n) HIPCHK(hipMemcpy(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost));
-  return v;
-}
-
-// ------------------------------------------------------------------------------------------------ one instance
-template <typename Real>
-struct InstHost {
-  Inst<Real> d{};                     // descriptor with device pointers (copied to the device table)
-  Arena arena;
-  int n = 0, m = 0, p = 0, K = 0, C = 0, S = 0;
-  std::vector<int32_t> cseg;          // clamp segments (fixed for the batch)
-  std::vector<double> X0, Q0;         // displacement origin (host; internalize moves it)
-  std::vector<TileItem> tiles;
-  int ncblk = 0;
-  double alg_bytes = 0;               // algorithmic HBM bytes per iteration
-  int hist_read = 0;
-  double tol = 0;
-  double outN_dummy = 0;
-};
-
-template <typename Real>
-std::vector<Real> toReal(const double* p, size_t n) {
-  std::vector<Real> v(n);
-  for (size_t i = 0; i < n; i++) v[i] = p ? Real(p[i]) : Real(0);
-  return v;
-}
-
-// build the device image of one host model (GpuSolver.upload, solver_gpu.js:191-310)
-template <typename Real>
-void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, int tile, int hist_len, Ctl* ctl) {
-  const int n = md.n, m = md.m, p = md.p, C = md.C, S = md.S;
-  const int K = md.par.contacts ? md.K : 0;   // no contact goal: no crossings on the device
-  if (n <= 0 || m < 0 || S <= 0) throw Fail{WS_ERR_ARG, "model has no nodes or strands"};
-  if (md.par.floor_on) throw Fail{WS_ERR_ARG, "the ground (scene.floor) is not available on the device path"};
-  for (int q = 0; q < 2 * (md.par.contacts ? md.K : 0); q++)
-    if (md.k_nseg[q] < 1 || md.k_strand[q] < 0 || md.k_strand[q] >= S) throw Fail{WS_ERR_ARG, "a crossing lies on a strand without segments"};
-  for (int s2 = 0; s2 < S; s2++)
-    if (md.node_offset[s2 + 1] - md.node_offset[s2] < 1 || (md.strand_closed[s2] && md.seg_offset[s2 + 1] == md.seg_offset[s2]))
-      throw Fail{WS_ERR_ARG, "a strand has no nodes (or is closed without segments)"};
-  H.n = n; H.m = m; H.p = p; H.K = K; H.C = C; H.S = S;
-  H.tol = md.par.tol;
-  Inst<Real>& d = H.d;
-  d.n = n; d.m = m; d.p = p; d.K = K; d.C = C; d.S = S; d.npl = md.n_point_local > 0 ? 1 : 0;
-  const ws_params& P = md.par;
-  d.mass_scale = P.mass_scale; d.damping = P.damping; d.tol = P.tol; d.mu = P.mu; d.strength = P.strength; d.stick = P.stick_strength;
-  d.torsion_factor = P.torsion_factor; d.epsilon = P.epsilon; d.slip_rate = P.slip_rate; d.rot_factor = P.rot_factor;
-  d.dent_rate = P.dent_rate; d.edge_thr = P.edge_threshold; d.l_ref = md.l_ref;
-  double jmin = INFINITY; for (int j = 0; j < p; j++) jmin = std::min(jmin, md.j_len[j]);
-  d.jlen_min = p ? jmin : 1.0;
-  d.kin_damp = P.kinetic_damping; d.warmup = P.warmup_iter; d.contacts = K > 0 ? 1 : 0; d.ramp_iter = P.ramp_iter;
-  d.rot_fric = P.rot_friction; d.seat = P.seat_law; d.seat_gap = P.seat_gap; d.dent_after = P.dent_after;
-  d.aug = P.augmented; d.al_every = P.al_every; d.hist_len = hist_len;
-  // ---- strand tiles and crossing blocks
-  H.tiles.clear();
-  for (int s = 0; s < S; s++) {
-    int nn = md.node_offset[s + 1] - md.node_offset[s];
-    // a tile of `tile` lanes owns tile − 1 nodes: lane 0 computes the halo segment and joint before them
-    for (int k0 = 0; k0 < nn; k0 += tile - 1) {
-      TileItem t{};
-      t.inst = 0; t.strand = s; t.k0 = k0; t.k1 = std::min(nn, k0 + tile - 1);
-      t.noff = md.node_offset[s]; t.nn = nn;
-      t.soff = md.seg_offset[s]; t.nseg = md.seg_offset[s + 1] - md.seg_offset[s];
-      t.joff = md.joint_offset[s]; t.njs = md.joint_offset[s + 1] - md.joint_offset[s];
-      t.closed = md.strand_closed[s] ? 1 : 0;
-      t.tile_local = int(H.tiles.size());
-      H.tiles.push_back(t);
-    }
-  }
-  d.ntiles = int(H.tiles.size());
-  H.ncblk = (K + CROSS_BLOCK - 1) / CROSS_BLOCK;
-  d.ncblk = H.ncblk;
-  // ---- arena layout
-  Arena& A = H.arena;
-  A.addc(d.node_off, S + 1); A.addc(d.seg_off, S + 1); A.addc(d.joint_off, S + 1); A.addc(d.closed, S);
-  A.addc(d.seg_i0, m); A.addc(d.seg_i1, m); A.addc(d.nsa, n); A.addc(d.nsb, n);
-  A.addc(d.side_off, S + 1); A.addc(d.side_pos, 2 * K);
-  if (C) { A.addc(d.clamp_off, m + 1); A.addc(d.clamp_idx, C); }
-  for (int b = 0; b < 2; b++) { A.add(d.x[b], n); A.add(d.y[b], n); A.add(d.z[b], n); }
-  A.add(d.vx, n); A.add(d.vy, n); A.add(d.vz, n);
-  A.addc(d.mass, n); A.addc(d.lx, n); A.addc(d.ly, n); A.addc(d.lz, n);
-  if (d.npl) { A.addc(d.llx, n); A.addc(d.lly, n); A.addc(d.llz, n); }
-  A.addc(d.fix, n);
-  for (int b = 0; b < 2; b++) { A.add(d.qw[b], m); A.add(d.qx[b], m); A.add(d.qy[b], m); A.add(d.qz[b], m); }
-  A.add(d.wx, m); A.add(d.wy, m); A.add(d.wz, m);
-  A.addc(d.l0, m); A.addc(d.EA, m); A.addc(d.GA, m); A.addc(d.EIx, m); A.addc(d.EIy, m); A.addc(d.GJ, m);
-  A.addc(d.krx, m); A.addc(d.kry, m); A.addc(d.krz, m);
-  A.addc(d.jlen, p); A.addc(d.om0, 3 * p);
-  if (C) { A.addc(d.c_len, C); A.addc(d.c_B, 3 * C); A.addc(d.c_Q, 4 * C); }
-  A.addc(d.k_seg_off, 2 * K); A.addc(d.k_nseg, 2 * K); A.addc(d.k_noff, 2 * K); A.addc(d.k_nn, 2 * K); A.addc(d.wk, 2 * K); A.addc(d.hk, 2 * K);
-  A.addc(d.fc90, 2 * K); A.addc(d.E90, 2 * K); A.addc(d.Hs, 2 * K); A.addc(d.kw, K); A.addc(d.kh, K); A.addc(d.k_en, K); A.add(d.k_lock, K);
-  A.add(d.u, 2 * K); A.add(d.slip, K); A.add(d.nprev, 3 * K); A.add(d.kflags, K); A.add(d.fn, K); A.add(d.ft, K); A.add(d.gap, K);
-  A.add(d.lam, K); A.add(d.penacc, K); A.add(d.theta0, K); A.add(d.rotslip, K); A.add(d.mrot, K);
-  A.add(d.dent, 2 * K); A.add(d.climb, 2 * K); A.add(d.tanpsi, 2 * K); A.add(d.dente, K);
-  A.add(d.theta, K); A.add(d.anom, K); A.add(d.pres, K); A.add(d.util, K); A.add(d.phi, 2 * K); A.add(d.mid, 3 * K);
-  A.add(d.ctype, K); A.add(d.face, K); A.add(d.side, 2 * K);
-  d.ctl = ctl; A.add(d.hist, size_t(std::max(hist_len, 1)) * HIST_STRIDE);
-  d.tile_waves = tile / 64;
-  A.add(d.spart, size_t(d.ntiles) * d.tile_waves * SPART); A.add(d.cpart, size_t(std::max(H.ncblk, 1)) * CPART);
-  A.alloc();
-  // ---- topology
-  h2d(d.node_off, std::vector<int32_t>(md.node_offset, md.node_offset + S + 1));
-  h2d(d.seg_off, std::vector<int32_t>(md.seg_offset, md.seg_offset + S + 1));
-  h2d(d.joint_off, std::vector<int32_t>(md.joint_offset, md.joint_offset + S + 1));
-  h2d(d.closed, std::vector<uint8_t>(md.strand_closed, md.strand_closed + S));
-  h2d(d.seg_i0, std::vector<int32_t>(md.seg_i0, md.seg_i0 + m));
-  h2d(d.seg_i1, std::vector<int32_t>(md.seg_i1, md.seg_i1 + m));
-  std::vector<int32_t> nsa(n, -1), nsb(n, -1);
-  for (int j = 0; j < m; j++) { nsa[md.seg_i0[j]] = j; nsb[md.seg_i1[j]] = j; }
-  h2d(d.nsa, nsa); h2d(d.nsb, nsb);
-  // crossing sides per strand, ordered by their position along the strand (then by side id)
-  std::vector<std::vector<std::pair<double, int>>> per(S);
-  for (int c = 0; c < K; c++) for (int sd = 0; sd < 2; sd++) per[md.k_strand[2 * c + sd]].push_back({st.u[2 * c + sd], 2 * c + sd});
-  std::vector<int32_t> soff(S + 1, 0), spos(2 * K, 0);
-  int slot = 0;
-  for (int s = 0; s < S; s++) { std::sort(per[s].begin(), per[s].end()); for (auto& e : per[s]) spos[e.second] = slot++; soff[s + 1] = slot; }
-  h2d(d.side_off, soff); h2d(d.side_pos, spos);
-  for (auto& t : H.tiles) { t.so = soff[t.strand]; t.ns = soff[t.strand + 1] - soff[t.strand]; }
-  H.cseg.assign(md.c_seg, md.c_seg + C);
-  if (C) {
-    std::vector<int32_t> coff(m + 1, 0), cidx(C);
-    for (int c = 0; c < C; c++) coff[md.c_seg[c] + 1]++;
-    for (int j = 0; j < m; j++) coff[j + 1] += coff[j];
-    std::vector<int32_t> fill(coff.begin(), coff.end() - 1);
-    for (int c = 0; c < C; c++) cidx[fill[md.c_seg[c]]++] = c;
-    h2d(d.clamp_off, coff); h2d(d.clamp_idx, cidx);
-  }
-  // ---- goals, parameters, crossing constants
-  h2d(d.mass, toReal<Real>(md.mass, n));
-  std::vector<Real> lx(n), ly(n), lz(n);
-  for (int i = 0; i < n; i++) { lx[i] = Real(md.load_const[3 * i]); ly[i] = Real(md.load_const[3 * i + 1]); lz[i] = Real(md.load_const[3 * i + 2]); }
-  h2d(d.lx, lx); h2d(d.ly, ly); h2d(d.lz, lz);
-  if (d.npl) {
-    std::vector<double> acc(3 * n, 0.0);
-    for (int q = 0; q < md.n_point_local; q++) for (int c = 0; c < 3; c++) acc[3 * md.point_local_node[q] + c] += md.point_local_F[3 * q + c];
-    std::vector<Real> a(n), b(n), cc(n);
-    for (int i = 0; i < n; i++) { a[i] = Real(acc[3 * i]); b[i] = Real(acc[3 * i + 1]); cc[i] = Real(acc[3 * i + 2]); }
-    h2d(d.llx, a); h2d(d.lly, b); h2d(d.llz, cc);
-  }
-  std::vector<uint8_t> fx(n);
-  for (int i = 0; i < n; i++) {
-    const uint8_t* f = md.node_free + 3 * i;
-    uint8_t bits = uint8_t((f[0] ? 0 : 1) | (f[1] ? 0 : 2) | (f[2] ? 0 : 4));
-    if (bits) bits |= 8;
-    if ((bits & 7) == 7) bits |= 16;
-    fx[i] = bits;
-  }
-  h2d(d.fix, fx);
-  h2d(d.l0, std::vector<double>(md.l0, md.l0 + m));
-  h2d(d.EA, toReal<Real>(md.EA, m)); h2d(d.GA, toReal<Real>(md.GA, m)); h2d(d.EIx, toReal<Real>(md.EIx, m));
-  h2d(d.EIy, toReal<Real>(md.EIy, m)); h2d(d.GJ, toReal<Real>(md.GJ, m));
-  std::vector<Real> kx(m), ky(m), kz(m);
-  for (int j = 0; j < m; j++) { kx[j] = Real(md.krot_static[3 * j]); ky[j] = Real(md.krot_static[3 * j + 1]); kz[j] = Real(md.krot_static[3 * j + 2]); }
-  h2d(d.krx, kx); h2d(d.kry, ky); h2d(d.krz, kz);
-  h2d(d.jlen, toReal<Real>(md.j_len, p)); h2d(d.om0, toReal<Real>(md.omega0, 3 * p));
-  if (C) { h2d(d.c_len, toReal<Real>(md.c_len, C)); h2d(d.c_B, toReal<Real>(md.c_B, 3 * C)); h2d(d.c_Q, std::vector<double>(md.c_Q, md.c_Q + 4 * C)); }
-  if (K) {
-    h2d(d.k_seg_off, std::vector<int32_t>(md.k_seg_off, md.k_seg_off + 2 * K));
-    h2d(d.k_nseg, std::vector<int32_t>(md.k_nseg, md.k_nseg + 2 * K));
-    std::vector<int32_t> knoff(2 * K), knn(2 * K);
-    for (int q = 0; q < 2 * K; q++) {
-      int s2 = md.k_strand[q];
-      knoff[q] = md.node_offset[s2];
-      knn[q] = (md.node_offset[s2 + 1] - md.node_offset[s2]) | (md.strand_closed[s2] ? (1 << 30) : 0);
-    }
-    h2d(d.k_noff, knoff); h2d(d.k_nn, knn);
-    h2d(d.wk, toReal<Real>(md.k_wk, 2 * K)); h2d(d.hk, toReal<Real>(md.k_hk, 2 * K));
-    h2d(d.fc90, toReal<Real>(md.k_fc90, 2 * K)); h2d(d.E90, toReal<Real>(md.k_E90, 2 * K)); h2d(d.Hs, toReal<Real>(md.k_Hs, 2 * K));
-    h2d(d.kw, toReal<Real>(md.k_w, K)); h2d(d.kh, toReal<Real>(md.k_h, K));
-    h2d(d.k_en, std::vector<uint8_t>(md.k_enabled, md.k_enabled + K));
-    h2d(d.k_lock, std::vector<uint8_t>(md.k_locked, md.k_locked + K));
-  }
-  // ---- algorithmic bytes per iteration: every per-iteration array read once, written once if it changes
-  const double R = double(sizeof(Real));
-  H.alg_bytes = n * (48.0 + 6 * R + R + 3 * R + 1) + m * (64.0 + 6 * R + 8 + 5 * R + 3 * R) + p * (4 * R) +
-                K * (32.0 + 6 * R + 16 + R + 1 + 16 + 4 * R + 2 * R + 1);
-}
-
-// dynamic state (GpuSolver._uploadState, solver_gpu.js:271-310)
-template <typename Real>
-void upload_state(InstHost<Real>& H, const ws_state& st) {
-  Inst<Real>& d = H.d;
-  const int n = H.n, m = H.m, K = H.K;
-  std::vector<double> x(n), y(n), z(n);
-  for (int i = 0; i < n; i++) { x[i] = st.X[3 * i]; y[i] = st.X[3 * i + 1]; z[i] = st.X[3 * i + 2]; }
-  for (int b = 0; b < 2; b++) { h2d(d.x[b], x); h2d(d.y[b], y); h2d(d.z[b], z); }
-  std::vector<Real> vx(n), vy(n), vz(n);
-  for (int i = 0; i < n; i++) { vx[i] = st.V ? Real(st.V[3 * i]) : Real(0); vy[i] = st.V ? Real(st.V[3 * i + 1]) : Real(0); vz[i] = st.V ? Real(st.V[3 * i + 2]) : Real(0); }
-  h2d(d.vx, vx); h2d(d.vy, vy); h2d(d.vz, vz);
-  std::vector<double> qw(m), qx(m), qy(m), qz(m);
-  for (int j = 0; j < m; j++) { qw[j] = st.Q[4 * j]; qx[j] = st.Q[4 * j + 1]; qy[j] = st.Q[4 * j + 2]; qz[j] = st.Q[4 * j + 3]; }
-  for (int b = 0; b < 2; b++) { h2d(d.qw[b], qw); h2d(d.qx[b], qx); h2d(d.qy[b], qy); h2d(d.qz[b], qz); }
-  std::vector<Real> wx(m), wy(m), wz(m);
-  for (int j = 0; j < m; j++) { wx[j] = st.W ? Real(st.W[3 * j]) : Real(0); wy[j] = st.W ? Real(st.W[3 * j + 1]) : Real(0); wz[j] = st.W ? Real(st.W[3 * j + 2]) : Real(0); }
-  h2d(d.wx, wx); h2d(d.wy, wy); h2d(d.wz, wz);
-  H.X0.assign(st.X0 ? st.X0 : st.X, (st.X0 ? st.X0 : st.X) + 3 * n);
-  H.Q0.assign(st.Q0 ? st.Q0 : st.Q, (st.Q0 ? st.Q0 : st.Q) + 4 * m);
-  if (K) {
-    h2d(d.u, std::vector<double>(st.u, st.u + 2 * K));
-    h2d(d.slip, st.slip ? std::vector<double>(st.slip, st.slip + K) : std::vector<double>(K, 0.0));
-    h2d(d.nprev, toReal<Real>(st.n_prev, 3 * K));
-    std::vector<uint8_t> fl(K, 2);
-    if (st.stuck) for (int c = 0; c < K; c++) fl[c] = st.stuck[c] ? 2 : 0;
-    h2d(d.kflags, fl);
-    h2d(d.fn, toReal<Real>(st.Fn, K));
-    h2d(d.lam, toReal<Real>(st.lam, K)); h2d(d.penacc, toReal<Real>(st.pen_acc, K));
-    std::vector<Real> th0(K, Real(NAN));
-    if (st.theta0) for (int c = 0; c < K; c++) th0[c] = Real(st.theta0[c]);
-    h2d(d.theta0, th0); h2d(d.rotslip, toReal<Real>(st.rot_slip, K));
-    h2d(d.dent, toReal<Real>(st.dent, 2 * K)); h2d(d.climb, toReal<Real>(st.climb, 2 * K));
-    std::vector<Real> tp(2 * K, Real(1));
-    if (st.tanpsi) for (int c = 0; c < 2 * K; c++) tp[c] = Real(st.tanpsi[c]);
-    h2d(d.tanpsi, tp); h2d(d.dente, toReal<Real>(st.dent_e, K));
-    if (st.theta) h2d(d.theta, toReal<Real>(st.theta, K));
-  }
-  Ctl c{};
-  c.iter = st.iteration; c.stop_at = st.iteration; c.done = 0; c.cur = 0; c.pen_n = st.pen_n;
-  c.ke_prev = st.ke_prev; c.resid = st.residual;
-  HIPCHK(hipMemcpy(d.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice));
-  H.hist_read = 0;
-}
-
-// ------------------------------------------------------------------------------------------------ batch
-struct BatchBase {
-  virtual ~BatchBase() {}
-  ws_ctx* ctx = nullptr;
-  int B = 0, precision = 0, tile = 128, per_submit = 64, hist_len = 4096;
-  hipStream_t stream = nullptr;
-  hipGraphExec_t graph = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double ms_last = 0, ms_cross = 0, ms_strand = 0, ms_ctrl = 0;
-  bool profile = false;
-  int32_t* d_sel = nullptr;
-  virtual int upload(int b, const ws_model& md, const ws_state& st) = 0;
-  virtual int update_params(int b, const ws_model& md) = 0;
-  virtual int step(int n_iter, const int32_t* sel, int nsel, int force, ws_control* out) = 0;
-  virtual int read_control(ws_control* out) = 0;
-  virtual int read_history(int b, double* out, int cap, int* n) = 0;
-  virtual int download(int b, ws_state& out) = 0;
-  virtual int internalize(int b) = 0;
-  virtual void kernel_bench(int reps, double* out) = 0;
-  virtual double alg_bytes() = 0;
-  virtual double dev_bytes() = 0;
-};
-
-template <typename Real>
-struct Batch : BatchBase {
-  std::vector<std::unique_ptr<InstHost<Real>>> inst;
-  Inst<Real>* d_inst = nullptr;
-  TileItem* d_tiles = nullptr; int ntiles = 0;
-  CrossItem* d_cross = nullptr; int ncross = 0;
-  Ctl* d_ctls = nullptr;            // [B] control records
-  std::vector<Ctl> h_ctls;
-  std::vector<TileItem> tiles;
-
-  ~Batch() override {
-    if (graph) (void)hipGraphExecDestroy(graph);
-    for (auto& h : inst) h->arena.release();
-    if (d_inst) (void)hipFree(d_inst);
-    if (d_tiles) (void)hipFree(d_tiles);
-    if (d_cross) (void)hipFree(d_cross);
-    if (d_sel) (void)hipFree(d_sel);
-    if (d_ctls) (void)hipFree(d_ctls);
-    if (ev0) (void)hipEventDestroy(ev0);
-    if (ev1) (void)hipEventDestroy(ev1);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-
-  void create(const ws_model* const* models, const ws_state* const* states, int nB) {
-    B = nB;
-    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    HIPCHK(hipEventCreate(&ev0)); HIPCHK(hipEventCreate(&ev1));
-    // the control records of all instances sit side by side: one copy reads every instance's state
-    HIPCHK(hipMalloc(&d_ctls, sizeof(Ctl) * std::max(B, 1)));
-    HIPCHK(hipMemset(d_ctls, 0, sizeof(Ctl) * std::max(B, 1)));
-    for (int b = 0; b < B; b++) {
-      inst.emplace_back(new InstHost<Real>());
-      build_instance(*inst.back(), *models[b], *states[b], tile, hist_len, d_ctls + b);
-      upload_state(*inst.back(), *states[b]);
-    }
-    relist();
-    HIPCHK(hipMalloc(&d_sel, sizeof(int32_t) * std::max(B, 1)));
-  }
-
-  // work lists and the device descriptor table
-  void relist() {
-    tiles.clear();
-    std::vector<CrossItem> cross;
-    for (int b = 0; b < B; b++) {
-      InstHost<Real>& H = *inst[b];
-      H.d.tile0 = int(tiles.size());
-      for (auto t : H.tiles) { t.inst = b; tiles.push_back(t); }
-      H.d.cblk0 = int(cross.size());
-      for (int q = 0; q < H.ncblk; q++) cross.push_back(CrossItem{b, q * CROSS_BLOCK, std::min(CROSS_BLOCK, H.K - q * CROSS_BLOCK), q});
-    }
-    ntiles = int(tiles.size()); ncross = int(cross.size());
-    if (d_tiles) (void)hipFree(d_tiles);
-    if (d_cross) (void)hipFree(d_cross);
-    if (d_inst) (void)hipFree(d_inst);
-    HIPCHK(hipMalloc(&d_tiles, sizeof(TileItem) * std::max(ntiles, 1)));
-    HIPCHK(hipMalloc(&d_cross, sizeof(CrossItem) * std::max(ncross, 1)));
-    HIPCHK(hipMalloc(&d_inst, sizeof(Inst<Real>) * std::max(B, 1)));
-    if (ntiles) HIPCHK(hipMemcpy(d_tiles, tiles.data(), sizeof(TileItem) * ntiles, hipMemcpyHostToDevice));
-    if (ncross) HIPCHK(hipMemcpy(d_cross, cross.data(), sizeof(CrossItem) * ncross, hipMemcpyHostToDevice));
-    std::vector<Inst<Real>> table(B);
-    for (int b = 0; b < B; b++) table[b] = inst[b]->d;
-    HIPCHK(hipMemcpy(d_inst, table.data(), sizeof(Inst<Real>) * B, hipMemcpyHostToDevice));
-    if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
-  }
-
-  void launch_cross(hipStream_t s) {
-    if (ncross) hipLaunchKernelGGL(k_cross<Real>, dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls);
-  }
-  void launch_strand(hipStream_t s) {
-    switch (tile) {
-      case 64: hipLaunchKernelGGL((k_strand<Real, 64>), dim3(ntiles), dim3(64), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
-      case 128: hipLaunchKernelGGL((k_strand<Real, 128>), dim3(ntiles), dim3(128), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
-      default: hipLaunchKernelGGL((k_strand<Real, 256>), dim3(ntiles), dim3(256), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
-    }
-  }
-  void launch_ctrl(hipStream_t s) { hipLaunchKernelGGL(k_ctrl<Real>, dim3(B), dim3(CTRL_THREADS), 0, s, d_inst, B, d_ctls); }
-  void launch_iteration(hipStream_t s) { launch_cross(s); launch_strand(s); launch_ctrl(s); }
-
-  void ensure_graph() {
-    if (graph) return;
-    hipGraph_t g;
-    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    for (int it = 0; it < per_submit; it++) launch_iteration(stream);
-    HIPCHK(hipStreamEndCapture(stream, &g));
-    HIPCHK(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
-    HIPCHK(hipGraphDestroy(g));
-  }
-
-  void fetch_ctls() {
-    h_ctls.resize(std::max(B, 1));
-    HIPCHK(hipMemcpy(h_ctls.data(), d_ctls, sizeof(Ctl) * B, hipMemcpyDeviceToHost));
-  }
-
-  bool all_done(const int32_t* sel, int nsel) {
-    fetch_ctls();
-    for (int q = 0; q < nsel; q++) {
-      const Ctl& c = h_ctls[sel[q]];
-      if (c.done == 0 && c.iter < c.stop_at) return false;
-    }
-    return true;
-  }
-
-  int step(int n_iter, const int32_t* sel_in, int nsel, int force, ws_control* out) override {
-    std::vector<int32_t> sel;
-    if (sel_in && nsel > 0) sel.assign(sel_in, sel_in + nsel);
-    else { sel.resize(B); std::iota(sel.begin(), sel.end(), 0); }
-    for (int b : sel) if (b < 0 || b >= B) throw Fail{WS_ERR_ARG, "instance index out of range"};
-    if (n_iter > 0 && !sel.empty()) {
-      HIPCHK(hipMemcpyAsync(d_sel, sel.data(), sizeof(int32_t) * sel.size(), hipMemcpyHostToDevice, stream));
-      hipLaunchKernelGGL(k_begin<Real>, dim3((sel.size() + 63) / 64), dim3(64), 0, stream, d_inst, d_sel, int(sel.size()), n_iter, force);
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipEventRecord(ev0, stream));
-      if (profile) {
-        // direct launches with events around each kernel: per-kernel durations on this stream
-        std::vector<hipEvent_t> ev(3 * n_iter + 1);
-        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-        HIPCHK(hipEventRecord(ev[0], stream));
-        for (int it = 0; it < n_iter; it++) {
-          launch_cross(stream);
-          HIPCHK(hipEventRecord(ev[3 * it + 1], stream));
-          launch_strand(stream);
-          HIPCHK(hipEventRecord(ev[3 * it + 2], stream));
-          launch_ctrl(stream);
-          HIPCHK(hipEventRecord(ev[3 * it + 3], stream));
-        }
-        HIPCHK(hipStreamSynchronize(stream));
-        double sc = 0, ss = 0, sk = 0;
-        for (int it = 0; it < n_iter; it++) {
-          float a = 0, b2 = 0, c2 = 0;
-          HIPCHK(hipEventElapsedTime(&a, ev[3 * it], ev[3 * it + 1]));
-          HIPCHK(hipEventElapsedTime(&b2, ev[3 * it + 1], ev[3 * it + 2]));
-          HIPCHK(hipEventElapsedTime(&c2, ev[3 * it + 2], ev[3 * it + 3]));
-          sc += a; ss += b2; sk += c2;
-        }
-        ms_cross = sc / n_iter; ms_strand = ss / n_iter; ms_ctrl = sk / n_iter;
-        for (auto& e : ev) (void)hipEventDestroy(e);
-      } else {
-        ensure_graph();
-        int launches = (n_iter + per_submit - 1) / per_submit;
-        for (int r = 0; r < launches; r++) {
-          HIPCHK(hipGraphLaunch(graph, stream));
-          if (launches > 8 && (r % 8) == 7 && r + 1 < launches) {
-            HIPCHK(hipStreamSynchronize(stream));
-            if (all_done(sel.data(), int(sel.size()))) break;
-          }
-        }
-      }
-      HIPCHK(hipEventRecord(ev1, stream));
-      HIPCHK(hipStreamSynchronize(stream));
-      float ms = 0;
-      HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
-      ms_last = ms;
-    }
-    if (out) return read_control(out);
-    return WS_OK;
-  }
-
-  static ws_control to_control(const Ctl& c) {
-    ws_control o{};
-    o.iter = c.iter; o.done = c.done; o.rollback = c.rollback; o.al_update = c.al_update; o.stop_at = c.stop_at; o.hist_n = c.hist_n;
-    o.ke_prev = c.ke_prev; o.ke = c.ke; o.resid = c.resid; o.fref = c.fref; o.e_stretch = c.e_stretch; o.e_bend = c.e_bend;
-    o.e_tors = c.e_tors; o.e_cont = c.e_cont; o.e_shear = c.e_shear; o.res_t = c.res_t; o.res_f = c.res_f;
-    return o;
-  }
-
-  int read_control(ws_control* out) override {
-    HIPCHK(hipStreamSynchronize(stream));
-    fetch_ctls();
-    for (int b = 0; b < B; b++) out[b] = to_control(h_ctls[b]);
-    return WS_OK;
-  }
-
-  int read_history(int b, double* out, int cap, int* nrows) override {
-    InstHost<Real>& H = *inst[b];
-    Ctl c;
-    HIPCHK(hipStreamSynchronize(stream));
-    HIPCHK(hipMemcpy(&c, H.d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
-    int avail = std::min(c.hist_n - H.hist_read, hist_len);
-    int nr = std::max(0, std::min(avail, cap));
-    if (nr > 0) {
-      auto ring = d2h(H.d.hist, size_t(hist_len) * HIST_STRIDE);
-      // the k-th returned row is iteration c.iter - (avail - 1) + k (readHistory, solver_gpu.js:389-405)
-      int first = c.iter - avail + 1;
-      for (int k = 0; k < nr; k++) {
-        int it = first + k;
-        std::memcpy(out + size_t(k) * HIST_STRIDE, ring.data() + size_t((it - 1) % hist_len) * HIST_STRIDE, sizeof(double) * HIST_STRIDE);
-      }
-    }
-    H.hist_read = c.hist_n;
-    *nrows = nr;
-    return WS_OK;
-  }
-
-  int download(int b, ws_state& o) override {
-    InstHost<Real>& H = *inst[b];
-    Inst<Real>& d = H.d;
-    const int n = H.n, m = H.m, p = H.p, K = H.K, C = H.C;
-    HIPCHK(hipStreamSynchronize(stream));
-    Ctl c;
-    HIPCHK(hipMemcpy(&c, d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
-    const int cur = c.cur;
-    const bool rb = c.rollback != 0;
-    auto x = d2h(d.x[cur], n), y = d2h(d.y[cur], n), z = d2h(d.z[cur], n);
-    if (o.X) for (int i = 0; i < n; i++) { o.X[3 * i] = x[i]; o.X[3 * i + 1] = y[i]; o.X[3 * i + 2] = z[i]; }
-    if (o.X0) std::copy(H.X0.begin(), H.X0.end(), o.X0);
-    if (o.Q0) std::copy(H.Q0.begin(), H.Q0.end(), o.Q0);
-    auto qw = d2h(d.qw[cur], m), qx = d2h(d.qx[cur], m), qy = d2h(d.qy[cur], m), qz = d2h(d.qz[cur], m);
-    if (o.Q) for (int j = 0; j < m; j++) { o.Q[4 * j] = qw[j]; o.Q[4 * j + 1] = qx[j]; o.Q[4 * j + 2] = qy[j]; o.Q[4 * j + 3] = qz[j]; }
-    if (o.V) {
-      if (rb) std::fill(o.V, o.V + 3 * n, 0.0);
-      else { auto a = d2h(d.vx, n), bb = d2h(d.vy, n), cc = d2h(d.vz, n); for (int i = 0; i < n; i++) { o.V[3 * i] = a[i]; o.V[3 * i + 1] = bb[i]; o.V[3 * i + 2] = cc[i]; } }
-    }
-    if (o.W) {
-      if (rb) std::fill(o.W, o.W + 3 * m, 0.0);
-      else { auto a = d2h(d.wx, m), bb = d2h(d.wy, m), cc = d2h(d.wz, m); for (int j = 0; j < m; j++) { o.W[3 * j] = a[j]; o.W[3 * j + 1] = bb[j]; o.W[3 * j + 2] = cc[j]; } }
-    }
-    auto putR = [](double* dst, const Real* src, size_t nn) { if (dst && nn) { auto v = d2h(src, nn); for (size_t i = 0; i < nn; i++) dst[i] = double(v[i]); } };
-    if (K) {
-      if (o.u) { auto v = d2h(d.u, 2 * K); std::copy(v.begin(), v.end(), o.u); }
-      if (o.slip) { auto v = d2h(d.slip, K); std::copy(v.begin(), v.end(), o.slip); }
-      putR(o.n_prev, d.nprev, 3 * K);
-      auto fl = d2h(d.kflags, K);
-      if (o.stuck) for (int q = 0; q < K; q++) o.stuck[q] = (fl[q] >> 1) & 1;
-      if (o.kstate) for (int q = 0; q < K; q++) o.kstate[q] = fl[q] & 1;
-      putR(o.Fn, d.fn, K); putR(o.Ft, d.ft, K); putR(o.gap, d.gap, K);
-      putR(o.lam, d.lam, K); putR(o.pen_acc, d.penacc, K); putR(o.theta0, d.theta0, K); putR(o.rot_slip, d.rotslip, K);
-      putR(o.dent, d.dent, 2 * K); putR(o.climb, d.climb, 2 * K); putR(o.tanpsi, d.tanpsi, 2 * K); putR(o.dent_e, d.dente, K);
-      putR(o.theta, d.theta, K); putR(o.Anom, d.anom, K); putR(o.pressure, d.pres, K); putR(o.util, d.util, K);
-      putR(o.Mrot, d.mrot, K); putR(o.phi, d.phi, 2 * K); putR(o.mid, d.mid, 3 * K);
-      if (o.ctype || o.face_side) {
-        auto ct = d2h(d.ctype, K), fs = d2h(d.face, K);
-        if (o.ctype) for (int q = 0; q < K; q++) o.ctype[q] = ct[q];
-        if (o.face_side) for (int q = 0; q < K; q++) o.face_side[q] = fs[q];
-      }
-    }
-    // section forces, moments and reactions of the last evaluated state
-    if (o.N || o.Vsh || o.M || o.cM || o.reactions) {
-      double *dN = nullptr, *dV = nullptr, *dM = nullptr, *dC = nullptr, *dR = nullptr;
-      HIPCHK(hipMalloc(&dN, sizeof(double) * std::max(m, 1))); HIPCHK(hipMalloc(&dV, sizeof(double) * std::max(2 * m, 1)));
-      HIPCHK(hipMalloc(&dM, sizeof(double) * std::max(3 * p, 1))); HIPCHK(hipMalloc(&dC, sizeof(double) * std::max(2 * C, 1)));
-      HIPCHK(hipMalloc(&dR, sizeof(double) * std::max(3 * n, 1)));
-      HIPCHK(hipMemset(dN, 0, sizeof(double) * std::max(m, 1))); HIPCHK(hipMemset(dV, 0, sizeof(double) * std::max(2 * m, 1)));
-      HIPCHK(hipMemset(dM, 0, sizeof(double) * std::max(3 * p, 1))); HIPCHK(hipMemset(dC, 0, sizeof(double) * std::max(2 * C, 1)));
-      HIPCHK(hipMemset(dR, 0, sizeof(double) * std::max(3 * n, 1)));
-      if (c.live_flag) {
-        switch (tile) {
-          case 64: hipLaunchKernelGGL((k_outputs<Real, 64>), dim3(d.ntiles), dim3(64), 0, stream, d_inst, d_tiles, b, dN, dV, dM, dC, dR); break;
-          case 128: hipLaunchKernelGGL((k_outputs<Real, 128>), dim3(d.ntiles), dim3(128), 0, stream, d_inst, d_tiles, b, dN, dV, dM, dC, dR); break;
-          default: hipLaunchKernelGGL((k_outputs<Real, 256>), dim3(d.ntiles), dim3(256), 0, stream, d_inst, d_tiles, b, dN, dV, dM, dC, dR); break;
-        }
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(stream));
-      }
-      auto cp = [](double* dst, double* src, size_t nn) { if (dst && nn) HIPCHK(hipMemcpy(dst, src, nn * sizeof(double), hipMemcpyDeviceToHost)); };
-      cp(o.N, dN, m); cp(o.Vsh, dV, 2 * m); cp(o.M, dM, 3 * p); cp(o.cM, dC, 2 * C); cp(o.reactions, dR, 3 * n);
-      (void)hipFree(dN); (void)hipFree(dV); (void)hipFree(dM); (void)hipFree(dC); (void)hipFree(dR);
-    }
-    o.iteration = c.iter; o.ke_prev = c.ke_prev; o.residual = c.resid; o.pen_n = c.pen_n;
-    o.converged = (c.done == 1 && c.resid <= H.tol) ? 1 : 0;
-    o.res_force = c.res_f; o.res_torque = c.res_t; o.fref = c.fref; o.ke = c.ke;
-    o.e_stretch = c.e_stretch; o.e_shear = c.e_shear; o.e_bend = c.e_bend; o.e_twist = c.e_tors; o.e_contact = c.e_cont;
-    return WS_OK;
-  }
-
-  int upload(int b, const ws_model& md, const ws_state& st) override {
-    InstHost<Real>& H = *inst[b];
-    if (md.n != H.n || md.m != H.m || md.p != H.p || (md.par.contacts ? md.K : 0) != H.K || md.C != H.C || md.S != H.S)
-      throw Fail{WS_ERR_TOPOLOGY, "upload: the model's sizes differ from the instance's"};
-    HIPCHK(hipStreamSynchronize(stream));
-    H.arena.release();
-    H.arena = Arena();
-    build_instance(H, md, st, tile, hist_len, d_ctls + b);
-    upload_state(H, st);
-    relist();
-    return WS_OK;
-  }
-
-  int update_params(int b, const ws_model& md) override {
-    InstHost<Real>& H = *inst[b];
-    if (md.C != H.C || !std::equal(H.cseg.begin(), H.cseg.end(), md.c_seg))
-      throw Fail{WS_ERR_TOPOLOGY, "the anchors changed the rotation clamps: rebuild the GPU solver"};
-    if (md.n != H.n || md.m != H.m || md.p != H.p) throw Fail{WS_ERR_TOPOLOGY, "update_params: sizes changed"};
-    HIPCHK(hipStreamSynchronize(stream));
-    Inst<Real>& d = H.d;
-    const int n = H.n, m = H.m, p = H.p, C = H.C, K = H.K;
-    const ws_params& P = md.par;
-    d.mass_scale = P.mass_scale; d.damping = P.damping; d.tol = P.tol; d.mu = P.mu; d.strength = P.strength; d.stick = P.stick_strength;
-    d.torsion_factor = P.torsion_factor; d.epsilon = P.epsilon; d.slip_rate = P.slip_rate; d.rot_factor = P.rot_factor;
-    d.dent_rate = P.dent_rate; d.edge_thr = P.edge_threshold; d.l_ref = md.l_ref; d.kin_damp = P.kinetic_damping;
-    d.warmup = P.warmup_iter; d.ramp_iter = P.ramp_iter; d.rot_fric = P.rot_friction; d.seat = P.seat_law; d.seat_gap = P.seat_gap;
-    d.dent_after = P.dent_after; d.aug = P.augmented; d.al_every = P.al_every;
-    H.tol = P.tol;
-    h2d(d.mass, toReal<Real>(md.mass, n));
-    std::vector<Real> lx(n), ly(n), lz(n);
-    for (int i = 0; i < n; i++) { lx[i] = Real(md.load_const[3 * i]); ly[i] = Real(md.load_const[3 * i + 1]); lz[i] = Real(md.load_const[3 * i + 2]); }
-    h2d(d.lx, lx); h2d(d.ly, ly); h2d(d.lz, lz);
-    std::vector<uint8_t> fx(n);
-    for (int i = 0; i < n; i++) {
-      const uint8_t* f = md.node_free + 3 * i;
-      uint8_t bits = uint8_t((f[0] ? 0 : 1) | (f[1] ? 0 : 2) | (f[2] ? 0 : 4));
-      if (bits) bits |= 8;
-      if ((bits & 7) == 7) bits |= 16;
-      fx[i] = bits;
-    }
-    h2d(d.fix, fx);
-    h2d(d.EA, toReal<Real>(md.EA, m)); h2d(d.GA, toReal<Real>(md.GA, m)); h2d(d.EIx, toReal<Real>(md.EIx, m));
-    h2d(d.EIy, toReal<Real>(md.EIy, m)); h2d(d.GJ, toReal<Real>(md.GJ, m));
-    std::vector<Real> kx(m), ky(m), kz(m);
-    for (int j = 0; j < m; j++) { kx[j] = Real(md.krot_static[3 * j]); ky[j] = Real(md.krot_static[3 * j + 1]); kz[j] = Real(md.krot_static[3 * j + 2]); }
-    h2d(d.krx, kx); h2d(d.kry, ky); h2d(d.krz, kz);
-    h2d(d.om0, toReal<Real>(md.omega0, 3 * p));
-    if (C) { h2d(d.c_B, toReal<Real>(md.c_B, 3 * C)); h2d(d.c_Q, std::vector<double>(md.c_Q, md.c_Q + 4 * C)); }
-    if (K) h2d(d.k_lock, std::vector<uint8_t>(md.k_locked, md.k_locked + K));
-    // a rebuild resets the kinetic-energy peak and the convergence flag (solver_gpu.js:331-332)
-    Ctl c;
-    HIPCHK(hipMemcpy(&c, d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
-    c.ke_prev = 0; c.done = 0;
-    HIPCHK(hipMemcpy(d.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice));
-    relist();
-    return WS_OK;
-  }
-
-  int internalize(int b) override {
-    InstHost<Real>& H = *inst[b];
-    ws_state tmp{};
-    std::vector<double> X(3 * H.n), Q(4 * H.m);
-    tmp.X = X.data(); tmp.Q = Q.data();
-    download(b, tmp);
-    H.X0 = X; H.Q0 = Q;
-    if (H.K) HIPCHK(hipMemset(H.d.slip, 0, sizeof(double) * H.K));
-    return WS_OK;
-  }
-
-  void kernel_bench(int reps, double* out) override {
-    HIPCHK(hipStreamSynchronize(stream));
-    // make every instance live for the timing launches (stop_at beyond all of them, convergence flag cleared)
-    std::vector<int32_t> all(B);
-    std::iota(all.begin(), all.end(), 0);
-    HIPCHK(hipMemcpyAsync(d_sel, all.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice, stream));
-    hipLaunchKernelGGL(k_begin<Real>, dim3((B + 63) / 64), dim3(64), 0, stream, d_inst, d_sel, B, 3 * reps + 8, 1);
-    HIPCHK(hipGetLastError());
-    for (int k = 0; k < 3; k++) {
-      HIPCHK(hipEventRecord(ev0, stream));
-      for (int r = 0; r < reps; r++) { if (k == 0) launch_cross(stream); else if (k == 1) launch_strand(stream); else launch_ctrl(stream); }
-      HIPCHK(hipEventRecord(ev1, stream));
-      HIPCHK(hipStreamSynchronize(stream));
-      float ms = 0;
-      HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
-      out[k] = double(ms) / reps;
-    }
-  }
-  double alg_bytes() override { double s = 0; for (auto& h : inst) s += h->alg_bytes; return s; }
-  double dev_bytes() override { double s = 0; for (auto& h : inst) s += double(h->arena.size); return s; }
-};
-
-int fail(ws_ctx* ctx, const Fail& f) { if (ctx) ctx->err = f.msg; return f.code; }
-
-}  // namespace
-
-struct ws_batch { std::unique_ptr<BatchBase> impl; };
 
 // ================================================================================================= C ABI
 extern "C" {
@@ -729,30 +55,51 @@ WS_API int ws_copy_probe(ws_ctx* ctx, int64_t bytes, int32_t reps, double* ms) {
   } catch (const Fail& f) { return fail(ctx, f); }
 }
 
+
 WS_API int ws_batch_create(ws_ctx* ctx, const ws_model* const* models, const ws_state* const* states, int32_t B,
                            const ws_batch_opts* opts, ws_batch** out) {
   if (!ctx || !models || !states || B <= 0 || !out) return ctx ? (ctx->err = "ws_batch_create: bad arguments", WS_ERR_ARG) : WS_ERR_ARG;
   try {
     HIPCHK(hipSetDevice(ctx->device));
     int prec = opts ? opts->precision : WS_FP32_FP64;
-    std::unique_ptr<BatchBase> b;
-    if (prec == WS_FP64) b.reset(new Batch<double>()); else b.reset(new Batch<float>());
-    b->ctx = ctx; b->precision = prec;
-    if (opts && opts->per_submit > 0) b->per_submit = opts->per_submit;
-    if (opts && opts->history > 0) b->hist_len = opts->history;
+    int ps = opts ? opts->per_submit : 0, hist = opts ? opts->history : 0;
     int t = opts ? opts->tile : 0;
-    if (t != 0 && t != 64 && t != 128 && t != 256) throw Fail{WS_ERR_ARG, "tile must be 0, 64, 128 or 256"};
-    if (t == 0) {
-      // choose the tile from the size: enough workgroups to fill 256 CUs, long tiles for long laths
-      long nodes = 0; for (int q = 0; q < B; q++) nodes += models[q]->n;
-      t = nodes >= 2000000 ? 256 : (nodes >= 200000 ? 128 : 64);
+    if (t != WS_TILE_AUTOTUNE && t != 0 && t != 64 && t != 128 && t != 256)
+      throw Fail{WS_ERR_ARG, "tile must be 0 (by size), -1 (autotune), 64, 128 or 256"};
+    if (t == 0) t = size_tile(models, B);
+    if (t == WS_TILE_AUTOTUNE) {
+      std::vector<ws_tune_entry> table;
+      ws_batch_opts best{};
+      autotune(ctx, models, states, B, prec, 256, 16, table, best);
+      t = best.tile; if (ps <= 0) ps = best.per_submit;
     }
-    b->tile = t;
-    if (prec == WS_FP64) static_cast<Batch<double>*>(b.get())->create(models, states, B);
-    else static_cast<Batch<float>*>(b.get())->create(models, states, B);
-    *out = new ws_batch{std::move(b)};
+    *out = new ws_batch{make_batch(ctx, models, states, B, prec, t, ps, hist)};
     return WS_OK;
   } catch (const Fail& f) { return fail(ctx, f); }
+}
+
+WS_API int ws_autotune(ws_ctx* ctx, const ws_model* const* models, const ws_state* const* states, int32_t B,
+                       int32_t precision, int32_t iters, ws_batch_opts* best, ws_tune_entry* table, int32_t cap,
+                       int32_t* n_entries) {
+  if (!ctx || !models || !states || B <= 0 || !best) return ctx ? (ctx->err = "ws_autotune: bad arguments", WS_ERR_ARG) : WS_ERR_ARG;
+  try {
+    HIPCHK(hipSetDevice(ctx->device));
+    std::vector<ws_tune_entry> tab;
+    *best = ws_batch_opts{};
+    autotune(ctx, models, states, B, precision, std::max(iters, 1), 16, tab, *best);
+    best->history = 0;
+    int n = int(tab.size());
+    if (table) for (int q = 0; q < std::min(n, cap); q++) table[q] = tab[q];
+    if (n_entries) *n_entries = n;
+    return WS_OK;
+  } catch (const Fail& f) { return fail(ctx, f); }
+}
+
+WS_API int ws_batch_get_opts(ws_batch* batch, ws_batch_opts* out) {
+  if (!batch || !out) return WS_ERR_ARG;
+  out->precision = batch->impl->precision; out->per_submit = batch->impl->per_submit;
+  out->tile = batch->impl->tile; out->history = batch->impl->hist_len;
+  return WS_OK;
 }
 
 WS_API int ws_batch_destroy(ws_batch* batch) { delete batch; return WS_OK; }

@@ -1095,7 +1095,7 @@ WS_API void wsm_free(wsm_model* mdl) { delete mdl; }
 WS_API int32_t wsm_abi_sizes(int64_t* out, int32_t cap) {
   const int64_t v[] = {sizeof(ws_params), sizeof(ws_model), sizeof(ws_state), sizeof(ws_control), sizeof(ws_batch_opts),
                        sizeof(ws_server_opts), sizeof(ws_result), sizeof(wsm_scene), sizeof(wsm_anchor), sizeof(wsm_point_load),
-                       sizeof(wsm_prep), sizeof(wsm_hex_spec), sizeof(wsm_info)};
+                       sizeof(wsm_prep), sizeof(wsm_hex_spec), sizeof(wsm_info), sizeof(ws_tune_entry)};
   int32_t n = int32_t(sizeof(v) / sizeof(v[0]));
   for (int32_t i = 0; i < n && i < cap; i++) out[i] = v[i];
   return n;

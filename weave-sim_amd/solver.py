@@ -67,10 +67,28 @@ def _ctl_dict(c):
     return {f: getattr(c, f) for f, _ in A.ws_control._fields_ if not f.startswith("pad")}
 
 
+def autotune(models, precision="fp32", iters=256, ctx=None):
+    """the autotuner's pick (tile, per_submit) for these instances and every candidate's rate"""
+    ctx = ctx or default_context()
+    lib = ctx.lib
+    B = len(models)
+    mp = (C.POINTER(A.ws_model) * B)(*[C.pointer(m.md) for m in models])
+    sp = (C.POINTER(A.ws_state) * B)(*[C.pointer(m.st) for m in models])
+    best = A.ws_batch_opts()
+    table = (A.ws_tune_entry * 16)()
+    n = C.c_int32()
+    ctx.check(lib.ws_autotune(ctx.h, mp, sp, B, PRECISIONS[precision], int(iters), C.byref(best), table, 16, C.byref(n)))
+    rows = [{"tile": t.tile, "per_submit": t.per_submit, "ms_per_iter": t.ms_per_iter,
+             "node_iters_per_s": t.node_iters_per_s} for t in table[:n.value]]
+    return {"tile": best.tile, "per_submit": best.per_submit}, rows
+
+
 class GpuSolver:
     """B instances (each its own topology allowed) relaxed in the same launches."""
 
-    def __init__(self, models, precision="fp32", tile=0, per_submit=64, history=4096, ctx=None, states=None):
+    def __init__(self, models, precision="fp32", tile=0, per_submit=0, history=4096, ctx=None, states=None):
+        """tile: 64 / 128 / 256 lanes per strand tile, 0 = chosen by size, "autotune" (or -1) = the autotuner's pick;
+        per_submit: iterations per captured graph (0 = 64, or the autotuner's pick)"""
         self.ctx = ctx or default_context()
         self.lib = self.ctx.lib
         self.models = list(models)
@@ -79,10 +97,23 @@ class GpuSolver:
         mp = (C.POINTER(A.ws_model) * self.B)(*[C.pointer(m.md) for m in self.models])
         sts = states or [m for m in self.models]
         sp = (C.POINTER(A.ws_state) * self.B)(*[C.pointer(s.st) for s in sts])
+        if tile == "autotune":
+            tile = A.WS_TILE_AUTOTUNE
         opts = A.ws_batch_opts(PRECISIONS[precision], per_submit, tile, history)
         self.h = C.c_void_p()
         self.ctx.check(self.lib.ws_batch_create(self.ctx.h, mp, sp, self.B, C.byref(opts), C.byref(self.h)))
         self._ctl = (A.ws_control * self.B)()
+
+    @property
+    def opts(self):
+        """options in effect: precision, per_submit, tile (as chosen by size or by the autotuner), history"""
+        o = A.ws_batch_opts()
+        self.ctx.check(self.lib.ws_batch_get_opts(self.h, C.byref(o)))
+        return {"precision": o.precision, "per_submit": o.per_submit, "tile": o.tile, "history": o.history}
+
+    @property
+    def tile(self):
+        return self.opts["tile"]
 
     def close(self):
         if getattr(self, "h", None):
@@ -154,3 +185,82 @@ class GpuSolver:
         a, b = C.c_double(), C.c_double()
         self.ctx.check(self.lib.ws_batch_bytes(self.h, C.byref(a), C.byref(b)))
         return {"alg_bytes_per_iter": a.value, "device_bytes": b.value}
+
+
+STATUS = {0: "converged", 1: "max_iter", 2: "cancelled", 3: "non_finite"}
+
+
+class Server:
+    """Continuous-batching request server (ws_server_*): submit design-variant solves as they arrive, pump, poll.
+
+    A request's model and state buffers are kept referenced here until its result is polled; the final state is
+    written into them (the model's own arrays unless `into` StateBuffers are given)."""
+
+    def __init__(self, precision="fp32", chunk=64, tile=64, max_slots=256, hbm_budget=0, ctx=None):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        o = A.ws_server_opts(PRECISIONS[precision], chunk, tile, max_slots, int(hbm_budget))
+        self.h = C.c_void_p()
+        self.ctx.check(self.lib.ws_server_create(self.ctx.h, C.byref(o), C.byref(self.h)))
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ws_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, model, max_iter=0, into=None):
+        tgt = into if into is not None else model
+        rid = C.c_int64()
+        self.ctx.check(self.lib.ws_submit(self.h, C.byref(model.md), C.byref(tgt.st), int(max_iter), C.byref(rid)))
+        self._keep[rid.value] = (model, tgt)
+        return rid.value
+
+    def cancel(self, rid):
+        self.ctx.check(self.lib.ws_cancel(self.h, int(rid)))
+
+    def pump(self):
+        """one round: admit, relax one chunk, retire; returns (active, queued)"""
+        a, q = C.c_int32(), C.c_int32()
+        self.ctx.check(self.lib.ws_pump(self.h, C.byref(a), C.byref(q)))
+        return a.value, q.value
+
+    def poll(self, cap=1024):
+        out = (A.ws_result * cap)()
+        n = C.c_int32()
+        self.ctx.check(self.lib.ws_poll(self.h, out, cap, C.byref(n)))
+        res = []
+        for r in out[:n.value]:
+            model, tgt = self._keep.pop(r.id, (None, None))
+            res.append({"id": r.id, "status": STATUS.get(r.status, r.status), "iterations": r.iterations,
+                        "residual": r.residual, "t_submit_ms": r.t_submit_ms, "t_admit_ms": r.t_admit_ms,
+                        "t_done_ms": r.t_done_ms, "n": r.n, "K": r.K, "model": model, "state": tgt})
+        return res
+
+    def fetch(self, rid, into):
+        self.ctx.check(self.lib.ws_fetch(self.h, int(rid), C.byref(into.st)))
+        return into
+
+    def clock_ms(self):
+        return self.lib.ws_server_clock_ms(self.h)
+
+    def stats(self):
+        a, q, u, b = C.c_int32(), C.c_int32(), C.c_int64(), C.c_int64()
+        self.ctx.check(self.lib.ws_server_stats(self.h, C.byref(a), C.byref(q), C.byref(u), C.byref(b)))
+        return {"active": a.value, "queued": q.value, "bytes_in_use": u.value, "hbm_budget": b.value}
+
+    def drain(self, max_rounds=1 << 30):
+        """pump until nothing is resident or queued; returns every result"""
+        out = []
+        for _ in range(max_rounds):
+            a, q = self.pump()
+            out += self.poll()
+            if a == 0 and q == 0:
+                break
+        return out
