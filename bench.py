@@ -176,7 +176,14 @@ def main():
     if it1 - it0 != args.steps * len(models):
         raise SystemExit("an instance stopped inside the timed region; rerun with fewer steps")
     value = n_nodes * args.steps / wall
-    # dominant kernel: mean duration of a strand launch (HIP events on the solver's stream)
+    # per-kernel durations inside the iteration sequence: the kernels launched one by one, a HIP event between
+    # consecutive kernels on the solver's stream
+    g.set_profile(True)
+    g.step(args.kernel_reps)
+    t = g.timing()
+    g.set_profile(False)
+    kin = {"ms_cross": t["ms_cross"], "ms_strand": t["ms_strand"], "ms_ctrl": t["ms_ctrl"]}
+    # and each kernel alone, launched back to back between two events
     kb = g.kernel_bench(args.kernel_reps)
     sb, cb = strand_cross_bytes(models, args.precision)
     achieved = sb / (kb["ms_strand"] * 1e-3) / 1e9
@@ -203,6 +210,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_strand",
                      "alg_bytes_per_launch": sb, "ms_per_launch": kb["ms_strand"], "traffic_detail": traffic_detail},
         "kernels_ms": kb,
+        "kernels_ms_in_iteration": kin,
         "alg_bytes_per_iter": alg,
     }
     if not args.no_cpu:

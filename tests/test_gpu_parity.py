@@ -181,3 +181,49 @@ def test_config2_size_properties(W, gpu_ctx):
     assert c["ke"] >= 0 and np.isfinite(c["resid"])
     assert np.abs(s["X"] - m["X0"]).max() < 0.5 * m.span
     g.close()
+
+
+BRANCHES = {
+    "w2_rotational_friction": dict(rot_friction=1),
+    "w3_seat_law": dict(seat_law=1, dent_after=350),
+    "al_multipliers": dict(augmented=1, ramp_iter=100, al_every=50),
+    "all_three": dict(rot_friction=1, seat_law=1, dent_after=350, augmented=1, ramp_iter=100, al_every=50),
+}
+
+
+@pytest.mark.parametrize("branch", sorted(BRANCHES))
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 1e-4)])
+def test_default_off_branches_match_oracle(W, gpu_ctx, branch, precision, tol):
+    """W2 rotational friction, W3 seat law and the AL multipliers (default off in every workload) on the sliding
+    single crossing and on the small dome's pre-chaos window: positions, contact state and the branch's own state"""
+    kw = BRANCHES[branch]
+    m = cases.crossing(tol=1e-9, **dict(cases.CROSS_CASES["cross_slip"], **kw))
+    (n, a, b, o, c), = run_pair(W, m, [2500], precision)
+    dX = np.abs(a["X"] - b["X"]).max()
+    assert dX <= tol * 1000.0
+    assert np.array_equal(a["stuck"], b["stuck"]) and np.array_equal(a["kstate"], b["kstate"])
+    for k in ("lam", "theta0", "Mrot", "dent", "climb"):
+        scale = max(1.0, np.abs(a[k]).max()) if np.isfinite(a[k]).all() else 1.0
+        atol = (1e-7 if precision == "fp64" else 1e-3) * scale
+        if k == "lam" and precision == "fp32":
+            # λ integrates k_n × penetration: its error is k_n times the position error the FP32 form carries
+            atol = max(atol, 10 * m.md.par.strength * dX)
+        ok = np.isclose(a[k], b[k], rtol=0, atol=atol, equal_nan=True)
+        assert ok.all(), (k, a[k], b[k], dX)
+    # the dome: within tolerance before the branches' discrete events (dents set and released, edge contacts)
+    # amplify rounding; at 600 iterations within the spread of an oracle run perturbed at the form's rounding level
+    # (1e-12 of the span for FP64, 1e-7 for FP32 forces) when that is larger
+    d = cases.config_small(**kw)
+    p = d.clone()
+    eps = 1e-12 if precision == "fp64" else 1e-7
+    p["X"][:] += eps * d.span * np.random.default_rng(0).standard_normal(3 * d.n)
+    op = Oracle(p)
+    for n, a, b, o, c in run_pair(W, d, [100, 400, 600], precision):
+        op.step(n - op.iteration)
+        spread = np.abs(a["X"] - op.state()["X"]).max()
+        err = np.abs(a["X"] - b["X"]).max()
+        if n <= 400:
+            assert err <= tol * d.span, (n, err)
+            assert np.array_equal(a["stuck"], b["stuck"]), n
+        else:
+            assert err <= max(tol * d.span, 20 * spread), (n, err, spread)

@@ -27,6 +27,7 @@ template <typename Real> __global__ void k_cross(const Inst<Real>*, const CrossI
 template <typename Real, int TILE> __global__ void k_strand(const Inst<Real>*, const TileItem*, int32_t, const Ctl*);
 template <typename Real> __global__ void k_ctrl(const Inst<Real>*, int32_t, Ctl*);
 __global__ void k_copy(const uint4*, uint4*, int64_t);
+template <typename Real> __global__ void k_al_preview(const Inst<Real>*, int32_t, double*);
 template <typename Real, int TILE> __global__ void k_outputs(const Inst<Real>*, const TileItem*, int32_t, double*, double*, double*, double*, double*);
 }  // namespace wsd
 
@@ -289,10 +290,10 @@ void upload_state(InstHost<Real>& H, const ws_state& st) {
     if (st.stuck) for (int c = 0; c < K; c++) fl[c] = st.stuck[c] ? 2 : 0;
     h2d(d.kflags, fl);
     h2d(d.fn, toReal<Real>(st.Fn, K));
-    h2d(d.lam, toReal<Real>(st.lam, K)); h2d(d.penacc, toReal<Real>(st.pen_acc, K));
-    std::vector<Real> th0(K, Real(NAN));
-    if (st.theta0) for (int c = 0; c < K; c++) th0[c] = Real(st.theta0[c]);
-    h2d(d.theta0, th0); h2d(d.rotslip, toReal<Real>(st.rot_slip, K));
+    h2d(d.lam, toReal<double>(st.lam, K)); h2d(d.penacc, toReal<double>(st.pen_acc, K));
+    std::vector<double> th0(K, NAN);
+    if (st.theta0) for (int c = 0; c < K; c++) th0[c] = st.theta0[c];
+    h2d(d.theta0, th0); h2d(d.rotslip, toReal<double>(st.rot_slip, K));
     h2d(d.dent, toReal<Real>(st.dent, 2 * K)); h2d(d.climb, toReal<Real>(st.climb, 2 * K));
     std::vector<Real> tp(2 * K, Real(1));
     if (st.tanpsi) for (int c = 0; c < 2 * K; c++) tp[c] = Real(st.tanpsi[c]);
@@ -613,7 +614,23 @@ struct Batch : BatchBase {
       if (o.stuck) for (int q = 0; q < K; q++) o.stuck[q] = (fl[q] >> 1) & 1;
       if (o.kstate) for (int q = 0; q < K; q++) o.kstate[q] = fl[q] & 1;
       putR(o.Fn, d.fn, K); putR(o.Ft, d.ft, K); putR(o.gap, d.gap, K);
-      putR(o.lam, d.lam, K); putR(o.pen_acc, d.penacc, K); putR(o.theta0, d.theta0, K); putR(o.rot_slip, d.rotslip, K);
+      if (c.al_update && d.aug) {
+        // the multiplier update decided at the end of the last iteration runs at the start of the next one here;
+        // the host solver has already applied it (updateMultipliers, solver_cpu.js:499-506)
+        double* dl = nullptr;
+        HIPCHK(hipMalloc(&dl, sizeof(double) * K));
+        hipLaunchKernelGGL(k_al_preview<Real>, dim3((K + 255) / 256), dim3(256), 0, stream, d_inst, b, dl);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
+        if (o.lam) HIPCHK(hipMemcpy(o.lam, dl, sizeof(double) * K, hipMemcpyDeviceToHost));
+        (void)hipFree(dl);
+        if (o.pen_acc) std::fill(o.pen_acc, o.pen_acc + K, 0.0);
+      } else {
+        if (o.lam) { auto v = d2h(d.lam, K); std::copy(v.begin(), v.end(), o.lam); }
+        if (o.pen_acc) { auto v = d2h(d.penacc, K); std::copy(v.begin(), v.end(), o.pen_acc); }
+      }
+      if (o.theta0) { auto v = d2h(d.theta0, K); std::copy(v.begin(), v.end(), o.theta0); }
+      if (o.rot_slip) { auto v = d2h(d.rotslip, K); std::copy(v.begin(), v.end(), o.rot_slip); }
       putR(o.dent, d.dent, 2 * K); putR(o.climb, d.climb, 2 * K); putR(o.tanpsi, d.tanpsi, 2 * K); putR(o.dent_e, d.dente, K);
       putR(o.theta, d.theta, K); putR(o.Anom, d.anom, K); putR(o.pressure, d.pres, K); putR(o.util, d.util, K);
       putR(o.Mrot, d.mrot, K); putR(o.phi, d.phi, 2 * K); putR(o.mid, d.mid, 3 * K);
@@ -645,7 +662,7 @@ struct Batch : BatchBase {
       cp(o.N, dN, m); cp(o.Vsh, dV, 2 * m); cp(o.M, dM, 3 * p); cp(o.cM, dC, 2 * C); cp(o.reactions, dR, 3 * n);
       (void)hipFree(dN); (void)hipFree(dV); (void)hipFree(dM); (void)hipFree(dC); (void)hipFree(dR);
     }
-    o.iteration = c.iter; o.ke_prev = c.ke_prev; o.residual = c.resid; o.pen_n = c.pen_n;
+    o.iteration = c.iter; o.ke_prev = c.ke_prev; o.residual = c.resid; o.pen_n = (c.al_update && d.aug && K) ? 0 : c.pen_n;
     o.converged = (c.done == 1 && c.resid <= H.tol) ? 1 : 0;
     o.res_force = c.res_f; o.res_torque = c.res_t; o.fref = c.fref; o.ke = c.ke;
     o.e_stretch = c.e_stretch; o.e_shear = c.e_shear; o.e_bend = c.e_bend; o.e_twist = c.e_tors; o.e_contact = c.e_cont;

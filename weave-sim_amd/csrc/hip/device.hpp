@@ -94,7 +94,9 @@ struct Inst {
   Real* nprev;               // [3K]
   uint8_t* kflags;           // [K] bit0 slip, bit1 stuck
   Real* fn; Real* ft; Real* gap;                              // [K]
-  Real* lam; Real* penacc; Real* theta0; Real* rotslip; Real* mrot;   // [K]
+  double* lam; double* penacc;                                // [K] AL multipliers and accumulated penetration (f64 in both forms)
+  double* theta0; double* rotslip;                            // [K] W2 reference angle and rotational slip (f64)
+  Real* mrot;                                                 // [K]
   Real* dent; Real* climb; Real* tanpsi; Real* dente;         // [2K], [2K], [2K], [K]
   // crossing classification outputs (W1)
   Real* theta; Real* anom; Real* pres; Real* util; Real* phi; Real* mid;   // [K], [K], [K], [K], [2K], [3K]

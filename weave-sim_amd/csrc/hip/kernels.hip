@@ -282,10 +282,33 @@ __device__ __forceinline__ H walls(int sideCut, const H* R, const H* n, const H*
   return ftan_(facos_(cp));
 }
 
-template <typename Real, typename G>
-__device__ __forceinline__ G free_weight(const Inst<Real>& I, const SideEv<G>& E) {
-  bool fa = (I.fix[E.a] & 16) != 0, fb = (I.fix[E.b] & 16) != 0;
-  return (fa ? G(0) : (1 - E.t) * (1 - E.t)) + (fb ? G(0) : E.t * E.t);
+// share of a side's force carried by free nodes, (1 − t)² + t² over the nodes not fixed in all three directions, at
+// the side's current parameter u (freeWeight, solver_cpu.js:483-487)
+template <typename Real>
+__device__ __forceinline__ double side_free_weight(const Inst<Real>& I, int q) {
+  const int ns = I.k_nseg[q], nnr = I.k_nn[q], nn = nnr & 0x3fffffff;
+  const bool closed = (nnr >> 30) & 1;
+  const double uu = I.u[q];
+  const int i = int(fmin(fmax(floor(uu), 0.0), double(max(ns - 1, 0))));
+  const double t = fmin(fmax(uu - i, 0.0), 1.0);
+  const int a = I.k_noff[q] + i, b = I.k_noff[q] + (closed ? (i + 1) % nn : i + 1);
+  const bool fa = (I.fix[a] & 16) != 0, fb = (I.fix[b] & 16) != 0;
+  return (fa ? 0.0 : (1 - t) * (1 - t)) + (fb ? 0.0 : t * t);
+}
+
+// the multiplier after an AL update (alValue + updateMultipliers, solver_cpu.js:492-506): λ + k_n · mean penetration
+// since the last update, kept ≥ 0 unless locked, clipped to the crushing cap, zero when disabled or pinned.  The
+// ramp is complete whenever the multipliers update (updates start after the ramp).
+template <typename Real>
+__device__ __forceinline__ double al_updated(const Inst<Real>& I, int c, double lam, double penacc, int pen_n) {
+  const bool enabled = I.k_en[c] != 0, locked = I.k_lock[c] != 0;
+  const bool pinned = side_free_weight(I, 2 * c) < AL_FREE_MIN || side_free_weight(I, 2 * c + 1) < AL_FREE_MIN;
+  const double cap = fmin(double(I.fc90[2 * c]), double(I.fc90[2 * c + 1])) * double(I.wk[2 * c]) * double(I.wk[2 * c + 1]) /
+                     fmax(sin(double(I.theta[c])), 1e-6);
+  double v = lam + I.strength * (penacc / double(max(pen_n, 1)));
+  if (!locked) v = fmax(v, 0.0);
+  v = fmin(fmax(v, -cap), cap);
+  return (!enabled || pinned) ? 0.0 : v;
 }
 
 __device__ __forceinline__ double xchg(double v) { return __shfl_xor(v, 1, 64); }
@@ -349,15 +372,8 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
   // ---- AL multiplier update deferred from the end of the previous iteration (updateMultipliers, :499-506)
   G lam = aug ? G(I.lam[c]) : G(0);
   G penacc = aug ? G(I.penacc[c]) : G(0);
-  if (aug && al_update) {                        // uniform over the launch: both lanes of every pair reach the swap
-    G fwOwn = free_weight(I, E), fwOth = xchg(fwOwn);
-    G kn1 = G(I.strength);   // the ramp is complete whenever the multipliers update
-    bool pinned = fwOwn < G(AL_FREE_MIN) || fwOth < G(AL_FREE_MIN);
-    G cap = dmin(G(I.fc90[2 * c]), G(I.fc90[2 * c + 1])) * wA * wB / dmax(G(fsin_(I.theta[c])), G(1e-6));
-    G v = lam + kn1 * (penacc / G(max(pen_n, 1)));
-    if (!locked) v = dmax(v, G(0));
-    v = clampv(v, -cap, cap);
-    lam = (!enabled || pinned) ? G(0) : v;
+  if (aug && al_update) {
+    lam = al_updated(I, c, lam, penacc, pen_n);
     penacc = G(0);
   }
   G np[3] = {I.nprev[3 * c], I.nprev[3 * c + 1], I.nprev[3 * c + 2]};
@@ -515,33 +531,34 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
   H FAr[3] = {H(FA[0]), H(FA[1]), H(FA[2])}, FBr[3] = {H(FB[0]), H(FB[1]), H(FB[2])};
   H TA[3] = {tf * (rA[1] * FAr[2] - rA[2] * FAr[1]), tf * (rA[2] * FAr[0] - rA[0] * FAr[2]), tf * (rA[0] * FAr[1] - rA[1] * FAr[0])};
   H TB[3] = {tf * (rB[1] * FBr[2] - rB[2] * FBr[1]), tf * (rB[2] * FBr[0] - rB[0] * FBr[2]), tf * (rB[0] * FBr[1] - rB[1] * FBr[0])};
-  H eRot = 0, th0 = H(0), rotslip = H(0), Mr = H(0);
+  // the relative rotation θ − θ0 cancels two angles near each other, and k_r is large: the W2 angle law runs in f64
+  H eRot = 0, Mr = H(0);
+  double th0 = 0, rotslip = 0;
   if (I.rot_fric) {
-    H tabr = H(tab);
-    H thc = fatan2_(cr[0] * nR[0] + cr[1] * nR[1] + cr[2] * nR[2], tabr);
+    G crd[3] = {tA[1] * tB[2] - tA[2] * tB[1], tA[2] * tB[0] - tA[0] * tB[2], tA[0] * tB[1] - tA[1] * tB[0]};
+    G thc = atan2(crd[0] * n[0] + crd[1] * n[1] + crd[2] * n[2], tab);
     th0 = I.theta0[c];
     rotslip = I.rotslip[c];
     if (isnan(th0) && act) th0 = thc;
     bool okr = act && !isnan(th0);
-    H sT2 = dmax(fsin_(theta), H(1e-6));
-    H wAr = H(wA), wBr = H(wB);
-    H req = H(0.38) * fsqrt_(wAr * wBr / sT2);
-    if (nEdge == 1) req = H(0.25) * (faceSide == 0 ? wAr : wBr) / sT2;
-    else if (nEdge == 2) req = H(0.38) * fsqrt_(dmax(Aa, H(1e-12)));
-    H kr = H(kt) * req * req;
-    H dth = okr ? thc - th0 : H(0);
-    const H PI = H(3.141592653589793);
-    dth = ffmod_(ffmod_(dth + PI, 2 * PI) + 2 * PI, 2 * PI) - PI;
-    H Mc = kr * dth;
-    H Mlim = locked ? H(INFINITY) : H(I.rot_factor) * H(mu0) * Fa * req;
+    G sT2 = fmax(sin(G(theta)), G(1e-6));
+    G req = G(0.38) * sqrt(wA * wB / sT2);
+    if (nEdge == 1) req = G(0.25) * (faceSide == 0 ? wA : wB) / sT2;
+    else if (nEdge == 2) req = G(0.38) * sqrt(fmax(G(Aa), G(1e-12)));
+    G kr = kt * req * req;
+    G dth = okr ? thc - th0 : G(0);
+    const G PI = 3.141592653589793;
+    dth = fmod(fmod(dth + PI, 2 * PI) + 2 * PI, 2 * PI) - PI;
+    G Mc = kr * dth;
+    G Mlim = locked ? G(INFINITY) : G(I.rot_factor) * mu0 * G(Fa) * req;
     if (okr && fabs(Mc) > Mlim) {
-      H adv = clampv(clampv(H(I.slip_rate), H(1e-3), H(1)) * (H(1) - Mlim / dmax(H(fabs(Mc)), H(1e-12))) * dth, H(-0.05), H(0.05));
+      G adv = clampv(clampv(G(I.slip_rate), G(1e-3), G(1)) * (G(1) - Mlim / fmax(fabs(Mc), G(1e-12))) * dth, G(-0.05), G(0.05));
       th0 += adv; rotslip += fabs(adv); dth -= adv;
       Mc = kr * dth;
     }
-    Mr = okr ? Mc : H(0);
+    Mr = H(okr ? Mc : G(0));
     for (int k = 0; k < 3; k++) { TA[k] += Mr * nR[k]; TB[k] -= Mr * nR[k]; }
-    eRot = H(0.5) * kr * dth * dth;
+    eRot = H(0.5 * kr * dth * dth);
   }
   // ---- righting stiffness (addTwistStiffness, :170-176) on the own side's segment after the slip
   G tfk = dmax(G(I.torsion_factor), G(1e-3));
@@ -565,7 +582,7 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
       G q0 = da * tA[0] + db * tB[0], q1 = da * tA[1] + db * tB[1], q2 = da * tA[2] + db * tB[2];
       I.slip[c] += double(fsqrt_(q0 * q0 + q1 * q1 + q2 * q2));
     }
-    if (aug) { I.lam[c] = Real(lam); I.penacc[c] = Real(penacc); }
+    if (aug) { I.lam[c] = lam; I.penacc[c] = penacc; }
     I.fn[c] = Real(Fn);
     I.ft[c] = Real(fsqrt_(fT[0] * fT[0] + fT[1] * fT[1] + fT[2] * fT[2]));
     I.gap[c] = Real(g);
@@ -1121,6 +1138,18 @@ __global__ __launch_bounds__(TILE) void k_outputs(const Inst<Real>* __restrict__
   strand_tile<Real, TILE, true>(I, I.ctl, tl, L, outN, outV, outM, outCM, outR);
 }
 
+// the multipliers as the host solver holds them after an iteration: an AL update decided by the control kernel is
+// applied at the start of the next iteration on the device, so a download between the two applies it here
+template <typename Real>
+__global__ __launch_bounds__(256) void k_al_preview(const Inst<Real>* __restrict__ insts, int32_t inst, double* out) {
+  const Inst<Real>& I = insts[inst];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= I.K) return;
+  const Ctl* ctl = I.ctl;
+  const double lam = I.lam[c];
+  out[c] = (I.aug && I.contacts && ctl->al_update) ? al_updated(I, c, lam, I.penacc[c], ctl->pen_n) : lam;
+}
+
 __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
   for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) dst[i] = src[i];
 }
@@ -1135,6 +1164,8 @@ template __global__ void k_cross<float>(const Inst<float>*, const CrossItem*, in
 template __global__ void k_cross<double>(const Inst<double>*, const CrossItem*, int32_t, const Ctl*);
 template __global__ void k_ctrl<float>(const Inst<float>*, int32_t, Ctl*);
 template __global__ void k_ctrl<double>(const Inst<double>*, int32_t, Ctl*);
+template __global__ void k_al_preview<float>(const Inst<float>*, int32_t, double*);
+template __global__ void k_al_preview<double>(const Inst<double>*, int32_t, double*);
 template __global__ void k_begin<float>(Ctl*, const int32_t*, const int32_t*, int32_t, int32_t, int32_t);
 template __global__ void k_begin<double>(Ctl*, const int32_t*, const int32_t*, int32_t, int32_t, int32_t);
 

@@ -24,7 +24,11 @@ WS_API const char* ws_last_error(const ws_ctx* ctx) { return ctx ? ctx->err.c_st
 
 WS_API int ws_device_info(ws_ctx* ctx, char* name, int32_t cap, int64_t* hbm, int32_t* ncu) {
   if (!ctx) return WS_ERR_ARG;
-  if (name && cap > 0) { std::snprintf(name, cap, "%s (%s)", ctx->prop.name, ctx->prop.gcnArchName); }
+  if (name && cap > 0) {
+    char dn[256] = {0};
+    if (hipDeviceGetName(dn, sizeof(dn) - 1, ctx->device) != hipSuccess || !dn[0]) std::snprintf(dn, sizeof(dn), "%s", ctx->prop.name);
+    std::snprintf(name, cap, "%s (%s)", dn, ctx->prop.gcnArchName);
+  }
   if (hbm) *hbm = int64_t(ctx->prop.totalGlobalMem);
   if (ncu) *ncu = ctx->prop.multiProcessorCount;
   return WS_OK;
