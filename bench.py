@@ -135,6 +135,125 @@ def measure_traffic(args):
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+# ------------------------------------------------------------------------------------------------ solves/s
+def run_solves(args):
+    """design variants relaxed to the stop criterion (residual ≤ tol, or the iteration limit): the batched FP32+FP64
+    form (all variants in one solver, autotuned tile) against the plain single-instance FP64 path, timed on a
+    sample of the same variants (per-iteration time × iterations to the stop, summed over the set)"""
+    W = importlib.import_module("weave-sim_amd")
+    S = scenes()
+    models = [S.design_variant(i) for i in range(args.batch)]
+    B = len(models)
+    limit = args.max_iter
+    g = W.GpuSolver(models, precision="fp32", tile="autotune")
+    opts = g.opts
+    t0 = time.perf_counter()
+    ctl = g.step(limit)
+    wall = time.perf_counter() - t0
+    iters = [c["iter"] for c in ctl]
+    converged = sum(1 for c in ctl if c["done"])
+    g.close()
+    batched = B / wall
+    # plain path: one instance per solver, FP64, one after another
+    idx = sorted(set(int(round(k * (B - 1) / max(args.plain_sample - 1, 1))) for k in range(args.plain_sample)))
+    per_solve = []
+    for i in idx:
+        p = W.GpuSolver([models[i]], precision="fp64", tile=0)
+        p.step(64)
+        t = time.perf_counter()
+        c = p.step(args.plain_iters)[0]
+        dt = (time.perf_counter() - t) / max(c["iter"] - 64, 1)
+        p.close()
+        per_solve.append(dt * iters[i])
+    plain = 1.0 / (sum(per_solve) / len(per_solve))
+    nodes = sum(m.n for m in models)
+    return {
+        "metric": "solves/s", "value": batched, "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded design variants)",
+        "config": {"workload": f"{B} design variants ({WORKLOADS['variants']}), relaxed to residual <= tol or "
+                               f"{limit} iterations", "instances": B, "nodes": nodes, "tile": opts["tile"],
+                   "per_submit": opts["per_submit"]},
+        "wall_s": wall, "node_iters_per_s": sum(m.n * k for m, k in zip(models, iters)) / wall,
+        "converged": converged, "iterations_mean": sum(iters) / B,
+        "plain_single_instance_fp64_solves_per_s": plain,
+        "plain_sample": {"instances": idx, "timed_iterations_each": args.plain_iters},
+        "batched_over_plain": batched / plain,
+    }
+
+
+# ------------------------------------------------------------------------------------------------ serving
+def run_serve(args):
+    """a Poisson stream of design-variant requests (1k–200k nodes) through the continuous-batching server: each
+    request relaxed to the stop criterion (residual ≤ tol or the iteration limit); latency = completion − arrival"""
+    import queue
+    import threading
+    W = importlib.import_module("weave-sim_amd")
+    S = scenes()
+    arrivals = S.request_stream(rate=args.rate, duration=args.duration, seed=4)
+    built = {}
+    lock = threading.Lock()
+    ready = threading.Condition(lock)
+    todo = queue.Queue()
+    for i in range(len(arrivals)):
+        todo.put(i)
+    inflight = threading.Semaphore(args.prefetch)
+
+    def builder():
+        while True:
+            try:
+                i = todo.get_nowait()
+            except queue.Empty:
+                return
+            inflight.acquire()
+            m = S.request_variant(i)
+            with lock:
+                built[i] = m
+                ready.notify_all()
+
+    workers = [threading.Thread(target=builder, daemon=True) for _ in range(args.builders)]
+    for w in workers:
+        w.start()
+    srv = W.Server(precision="fp32", chunk=args.chunk, tile=64, max_slots=args.slots)
+    lat, nodes, next_i, done, results = [], 0, 0, 0, []
+    t0 = time.perf_counter()
+    while done < len(arrivals):
+        now = time.perf_counter() - t0
+        while next_i < len(arrivals) and arrivals[next_i] <= now:
+            with lock:
+                m = built.pop(next_i, None)
+                if m is None and srv.stats()["active"] == 0:
+                    ready.wait(0.01)          # nothing to relax meanwhile: wait for the scene builders
+                    m = built.pop(next_i, None)
+            if m is None:
+                break
+            srv.submit(m, max_iter=args.max_iter)
+            next_i += 1
+        a, q = srv.pump()
+        for r in srv.poll():
+            idx = r["id"] - 1
+            lat.append((time.perf_counter() - t0 - arrivals[idx]) * 1e3)
+            nodes += r["n"] * r["iterations"]
+            results.append(r["status"])
+            done += 1
+            inflight.release()
+        if a == 0 and q == 0 and next_i < len(arrivals):
+            time.sleep(max(0.0, min(arrivals[next_i] - (time.perf_counter() - t0), 0.002)))
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat.sort()
+    pct = lambda p: lat[min(len(lat) - 1, int(p / 100.0 * len(lat)))]
+    return {
+        "metric": "solves/s", "value": len(lat) / wall, "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded request stream)",
+        "config": {"workload": f"Poisson stream at {args.rate}/s over {args.duration} s of design variants "
+                               f"(1k-200k nodes), each relaxed to residual <= tol or {args.max_iter} iterations",
+                   "requests": len(arrivals), "chunk": args.chunk, "max_slots": args.slots},
+        "latency_ms": {"p50": pct(50), "p99": pct(99), "max": lat[-1]},
+        "wall_s": wall, "node_iters_per_s": nodes / wall,
+        "status": {k: results.count(k) for k in set(results)},
+    }
+
+
 # ------------------------------------------------------------------------------------------------ main
 def main():
     ap = argparse.ArgumentParser()
@@ -149,6 +268,18 @@ def main():
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--kernel-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--mode", default="throughput", choices=["throughput", "solves", "serve"],
+                    help="throughput: node-iterations/s of one workload; solves: design variants to the stop "
+                         "criterion, batched vs plain; serve: a Poisson request stream through the server")
+    ap.add_argument("--max-iter", type=int, default=5000, help="iteration limit of a solve (solves, serve)")
+    ap.add_argument("--plain-sample", type=int, default=16, help="variants timed on the plain path (solves)")
+    ap.add_argument("--plain-iters", type=int, default=512, help="iterations timed per plain-path variant")
+    ap.add_argument("--rate", type=float, default=60.0, help="requests per second (serve)")
+    ap.add_argument("--duration", type=float, default=120.0, help="seconds of arrivals (serve)")
+    ap.add_argument("--chunk", type=int, default=64, help="iterations per server round (serve)")
+    ap.add_argument("--slots", type=int, default=256, help="resident instances at most (serve)")
+    ap.add_argument("--builders", type=int, default=4, help="host threads building request scenes (serve)")
+    ap.add_argument("--prefetch", type=int, default=320, help="request scenes held at once: built ahead, queued or resident (serve)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU (got --gpus {args.gpus}): instances are independent solves that would "
@@ -156,6 +287,14 @@ def main():
         sys.exit(2)
     if args.kernel_only:
         kernel_only(args)
+        return
+    if args.mode == "solves":
+        if args.batch == 64 and "--batch" not in sys.argv:
+            args.batch = 256
+        print(json.dumps(run_solves(args)))
+        return
+    if args.mode == "serve":
+        print(json.dumps(run_serve(args)))
         return
     # counter passes first, as child processes, before this process touches the GPU
     traffic, traffic_detail = (None, {"skipped": True}) if args.no_traffic else measure_traffic(args)
