@@ -169,6 +169,8 @@ typedef struct ws_batch_opts {
   int32_t per_submit;      /* iterations per captured graph (GpuSolver perSubmit, default 64) */
   int32_t tile;            /* lanes per strand tile: 64 / 128 / 256; 0 = chosen by size; -1 = autotune */
   int32_t history;         /* history ring length per instance (0 = 4096, the reference's HN) */
+  int32_t ctrl_chunk;      /* strand partials per control workgroup (0 = by size: one workgroup up to 4096, else 2048
+                              each, the last to finish combining them) */
 } ws_batch_opts;
 
 #define WS_TILE_AUTOTUNE (-1)

@@ -227,3 +227,25 @@ def test_default_off_branches_match_oracle(W, gpu_ctx, branch, precision, tol):
             assert np.array_equal(a["stuck"], b["stuck"]), n
         else:
             assert err <= max(tol * d.span, 20 * spread), (n, err, spread)
+
+
+def test_control_reduction_over_several_workgroups(W, gpu_ctx):
+    """the 10 m dome in strand tiles of 64 leaves 1,248 wavefront partials, reduced here by three control workgroups
+    of 512 (the last to finish combines them): kinetic energy, strain energies and the residual parts against the
+    oracle"""
+    m = cases.config2()
+    its = 320                                   # past the 300-iteration warm-up: velocities are live
+    o = Oracle(m)
+    o.step(its)
+    a = o.state()
+    g = W.GpuSolver([m], precision="fp64", tile=64, ctrl_chunk=512)
+    c = g.step(its)[0]
+    g.close()
+    assert c["iter"] == its
+    assert abs(c["ke"] - o.ke) <= 1e-6 * o.ke
+    for k_dev, k_ora in (("e_stretch", "e_stretch"), ("e_bend", "e_bend"), ("e_tors", "e_twist"), ("e_shear", "e_shear"),
+                         ("e_cont", "e_contact")):
+        ref = getattr(a.st, k_ora)
+        assert abs(c[k_dev] - ref) <= 1e-6 * max(abs(ref), 1e-12), (k_dev, c[k_dev], ref)
+    assert abs(c["res_f"] - a.st.res_force) <= 1e-6 * a.st.res_force
+    assert abs(c["res_t"] - a.st.res_torque) <= 1e-6 * a.st.res_torque

@@ -66,7 +66,7 @@ class ws_control(C.Structure):
 
 
 class ws_batch_opts(C.Structure):
-    _fields_ = [("precision", i32), ("per_submit", i32), ("tile", i32), ("history", i32)]
+    _fields_ = [("precision", i32), ("per_submit", i32), ("tile", i32), ("history", i32), ("ctrl_chunk", i32)]
 
 
 WS_TILE_AUTOTUNE = -1

@@ -25,7 +25,7 @@ namespace wsd {
 template <typename Real> __global__ void k_begin(Ctl*, const int32_t*, const int32_t*, int32_t, int32_t, int32_t);
 template <typename Real> __global__ void k_cross(const Inst<Real>*, const CrossItem*, int32_t, const Ctl*);
 template <typename Real, int TILE> __global__ void k_strand(const Inst<Real>*, const TileItem*, int32_t, const Ctl*);
-template <typename Real> __global__ void k_ctrl(const Inst<Real>*, int32_t, Ctl*);
+template <typename Real> __global__ void k_ctrl(const Inst<Real>*, const CtrlItem*, int32_t, Ctl*);
 __global__ void k_copy(const uint4*, uint4*, int64_t);
 template <typename Real> __global__ void k_al_preview(const Inst<Real>*, int32_t, double*);
 template <typename Real, int TILE> __global__ void k_outputs(const Inst<Real>*, const TileItem*, int32_t, double*, double*, double*, double*, double*);
@@ -106,7 +106,7 @@ std::vector<Real> toReal(const double* p, size_t n) {
 // build the device image of one host model (GpuSolver.upload, solver_gpu.js:191-310)
 template <typename Real>
 void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, int tile, int hist_len, Ctl* ctl,
-                    bool layout_only = false) {
+                    int ctrl_chunk = 0, bool layout_only = false) {
   const int n = md.n, m = md.m, p = md.p, C = md.C, S = md.S;
   const int K = md.par.contacts ? md.K : 0;   // no contact goal: no crossings on the device
   if (n <= 0 || m < 0 || S <= 0) throw Fail{WS_ERR_ARG, "model has no nodes or strands"};
@@ -174,7 +174,13 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   A.add(d.ctype, K); A.add(d.face, K); A.add(d.side, 2 * K);
   d.ctl = ctl; A.add(d.hist, size_t(std::max(hist_len, 1)) * HIST_STRIDE);
   d.tile_waves = tile / 64;
-  A.add(d.spart, size_t(d.ntiles) * d.tile_waves * SPART); A.add(d.cpart, size_t(std::max(H.ncblk, 1)) * CPART);
+  A.add(d.spart, size_t(d.ntiles) * SPART); A.add(d.cpart, size_t(std::max(H.ncblk, 1)) * CPART);
+  {
+    const int np = std::max(1, d.ntiles);
+    d.ctrl_chunk = ctrl_chunk > 0 ? ctrl_chunk : (np <= CTRL_ONE ? np : CTRL_CHUNK);
+    d.ctrl_chunks = (np + d.ctrl_chunk - 1) / d.ctrl_chunk;
+  }
+  A.add(d.rpart, size_t(d.ctrl_chunks) * RPART);
   if (layout_only) return;   // A.size: the device bytes the instance would take
   A.alloc();
   // ---- topology
@@ -260,7 +266,7 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
 template <typename Real>
 size_t instance_bytes(const ws_model& md, const ws_state& st, int tile, int hist_len) {
   InstHost<Real> H;
-  build_instance(H, md, st, tile, hist_len, nullptr, true);
+  build_instance(H, md, st, tile, hist_len, nullptr, 0, true);
   return H.arena.size;
 }
 
@@ -312,6 +318,7 @@ struct BatchBase {
   virtual ~BatchBase() {}
   ws_ctx* ctx = nullptr;
   int B = 0, precision = 0, tile = 128, per_submit = 64, hist_len = 4096;
+  int ctrl_chunk = 0;                   // strand partials per control workgroup (0: by size)
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -344,6 +351,7 @@ struct Batch : BatchBase {
   Inst<Real>* d_inst = nullptr;
   TileItem* d_tiles = nullptr; int ntiles = 0;
   CrossItem* d_cross = nullptr; int ncross = 0;
+  CtrlItem* d_ctrl = nullptr; int nctrl = 0;
   Ctl* d_ctls = nullptr;            // [B] control records
   std::vector<Ctl> h_ctls;
   std::vector<TileItem> tiles;
@@ -354,6 +362,7 @@ struct Batch : BatchBase {
     if (d_inst) (void)hipFree(d_inst);
     if (d_tiles) (void)hipFree(d_tiles);
     if (d_cross) (void)hipFree(d_cross);
+    if (d_ctrl) (void)hipFree(d_ctrl);
     if (d_sel) (void)hipFree(d_sel);
     if (d_cnt) (void)hipFree(d_cnt);
     if (d_ctls) (void)hipFree(d_ctls);
@@ -371,7 +380,7 @@ struct Batch : BatchBase {
     HIPCHK(hipMemset(d_ctls, 0, sizeof(Ctl) * std::max(B, 1)));
     for (int b = 0; b < B; b++) {
       inst.emplace_back(new InstHost<Real>());
-      build_instance(*inst.back(), *models[b], *states[b], tile, hist_len, d_ctls + b);
+      build_instance(*inst.back(), *models[b], *states[b], tile, hist_len, d_ctls + b, ctrl_chunk);
       upload_state(*inst.back(), *states[b]);
     }
     relist();
@@ -398,7 +407,7 @@ struct Batch : BatchBase {
     if (slot < 0 || slot >= B || inst[slot]) throw Fail{WS_ERR_ARG, "add_instance: slot out of range or occupied"};
     HIPCHK(hipStreamSynchronize(stream));
     std::unique_ptr<InstHost<Real>> h(new InstHost<Real>());
-    build_instance(*h, md, st, tile, hist_len, d_ctls + slot);
+    build_instance(*h, md, st, tile, hist_len, d_ctls + slot, ctrl_chunk);
     upload_state(*h, st);
     inst[slot] = std::move(h);
   }
@@ -421,9 +430,22 @@ struct Batch : BatchBase {
 
 
   // work lists and the device descriptor table
+  // device work lists and descriptor table (buffers grow, never shrink: the server relists every round)
+  template <typename T> void put_list(T*& dptr, int& cap, const std::vector<T>& v) {
+    const int need = std::max(int(v.size()), 1);
+    if (need > cap) {
+      if (dptr) HIPCHK(hipFree(dptr));
+      HIPCHK(hipMalloc(&dptr, sizeof(T) * need));
+      cap = need;
+    }
+    if (!v.empty()) HIPCHK(hipMemcpy(dptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  }
+  int cap_tiles = 0, cap_cross = 0, cap_ctrl = 0, cap_inst = 0;
+
   void relist() override {
     tiles.clear();
     std::vector<CrossItem> cross;
+    std::vector<CtrlItem> ctrl;
     for (int b = 0; b < B; b++) {
       if (!inst[b]) continue;
       InstHost<Real>& H = at(b);
@@ -431,21 +453,19 @@ struct Batch : BatchBase {
       for (auto t : H.tiles) { t.inst = b; tiles.push_back(t); }
       H.d.cblk0 = int(cross.size());
       for (int q = 0; q < H.ncblk; q++) cross.push_back(CrossItem{b, q * CROSS_BLOCK, std::min(CROSS_BLOCK, H.K - q * CROSS_BLOCK), q});
+      for (int q = 0; q < H.d.ctrl_chunks; q++) ctrl.push_back(CtrlItem{b, q, H.d.ctrl_chunks, 0});
     }
-    ntiles = int(tiles.size()); ncross = int(cross.size());
-    if (d_tiles) (void)hipFree(d_tiles);
-    if (d_cross) (void)hipFree(d_cross);
-    if (d_inst) (void)hipFree(d_inst);
-    HIPCHK(hipMalloc(&d_tiles, sizeof(TileItem) * std::max(ntiles, 1)));
-    HIPCHK(hipMalloc(&d_cross, sizeof(CrossItem) * std::max(ncross, 1)));
-    HIPCHK(hipMalloc(&d_inst, sizeof(Inst<Real>) * std::max(B, 1)));
-    if (ntiles) HIPCHK(hipMemcpy(d_tiles, tiles.data(), sizeof(TileItem) * ntiles, hipMemcpyHostToDevice));
-    if (ncross) HIPCHK(hipMemcpy(d_cross, cross.data(), sizeof(CrossItem) * ncross, hipMemcpyHostToDevice));
+    ntiles = int(tiles.size()); ncross = int(cross.size()); nctrl = int(ctrl.size());
+    HIPCHK(hipStreamSynchronize(stream));
+    put_list(d_tiles, cap_tiles, tiles);
+    put_list(d_cross, cap_cross, cross);
+    put_list(d_ctrl, cap_ctrl, ctrl);
     std::vector<Inst<Real>> table(B);
     for (int b = 0; b < B; b++) table[b] = inst[b] ? inst[b]->d : Inst<Real>{};
-    HIPCHK(hipMemcpy(d_inst, table.data(), sizeof(Inst<Real>) * B, hipMemcpyHostToDevice));
+    put_list(d_inst, cap_inst, table);
     if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
   }
+
 
   void launch_cross(hipStream_t s) {
     if (ncross) hipLaunchKernelGGL(k_cross<Real>, dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls);
@@ -457,7 +477,9 @@ struct Batch : BatchBase {
       default: hipLaunchKernelGGL((k_strand<Real, 256>), dim3(ntiles), dim3(256), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
     }
   }
-  void launch_ctrl(hipStream_t s) { hipLaunchKernelGGL(k_ctrl<Real>, dim3(B), dim3(CTRL_THREADS), 0, s, d_inst, B, d_ctls); }
+  void launch_ctrl(hipStream_t s) {
+    if (nctrl) hipLaunchKernelGGL(k_ctrl<Real>, dim3(nctrl), dim3(CTRL_THREADS), 0, s, d_inst, d_ctrl, nctrl, d_ctls);
+  }
   void launch_iteration(hipStream_t s) { launch_cross(s); launch_strand(s); launch_ctrl(s); }
 
   void ensure_graph() {
@@ -676,7 +698,7 @@ struct Batch : BatchBase {
     HIPCHK(hipStreamSynchronize(stream));
     H.arena.release();
     H.arena = Arena();
-    build_instance(H, md, st, tile, hist_len, d_ctls + b);
+    build_instance(H, md, st, tile, hist_len, d_ctls + b, ctrl_chunk);
     upload_state(H, st);
     relist();
     return WS_OK;
@@ -767,12 +789,13 @@ int fail(ws_ctx* ctx, const Fail& f) { if (ctx) ctx->err = f.msg; return f.code;
 
 // the batch of the given options (tile already resolved to 64 / 128 / 256)
 std::unique_ptr<BatchBase> make_batch(ws_ctx* ctx, const ws_model* const* models, const ws_state* const* states, int B,
-                                      int prec, int tile, int per_submit, int history) {
+                                      int prec, int tile, int per_submit, int history, int ctrl_chunk = 0) {
   std::unique_ptr<BatchBase> b;
   if (prec == WS_FP64) b.reset(new Batch<double>()); else b.reset(new Batch<float>());
   b->ctx = ctx; b->precision = prec;
   if (per_submit > 0) b->per_submit = per_submit;
   if (history > 0) b->hist_len = history;
+  b->ctrl_chunk = std::max(ctrl_chunk, 0);
   b->tile = tile;
   if (prec == WS_FP64) static_cast<Batch<double>*>(b.get())->create(models, states, B);
   else static_cast<Batch<float>*>(b.get())->create(models, states, B);

@@ -13,9 +13,13 @@
 namespace wsd {
 
 constexpr int HIST_STRIDE = 9;   // history ring entry: ke, strain, bending, torsion, stretch, shear, contact, residual, iteration
-constexpr int SPART = 11;        // doubles per strand wavefront partial (stored value-major)
+constexpr int SPART = 11;        // doubles per strand-tile partial (stored value-major)
 constexpr int CPART = 2;         // doubles per crossing-block partial (stored value-major)
-constexpr int CTRL_THREADS = 256; // lanes of the control kernel's workgroup
+constexpr int CTRL_THREADS = 256;  // lanes of the control kernel's workgroup
+constexpr int CTRL_PER = 4;         // strand partials per lane per round (all loads of a round in flight together)
+constexpr int CTRL_ONE = 4096;    // up to this many strand partials one control workgroup reduces them all
+constexpr int CTRL_CHUNK = 2048;  // beyond: partials per control workgroup (the last to finish combines them)
+constexpr int RPART = 13;         // doubles per control-chunk partial
 constexpr int CROSS_THREADS = 64;                 // lanes per crossing-kernel block (two per crossing)
 constexpr int CROSS_BLOCK = CROSS_THREADS / 2;    // crossings per crossing-kernel block
 constexpr int MAX_REL = 1024;    // crossing sides a strand tile may gather (checked on the host per strand)
@@ -46,7 +50,8 @@ struct Inst {
   int32_t n, m, p, K, C, S, npl, nclamp_entries;
   // work-list ranges of this instance
   int32_t tile0, ntiles, cblk0, ncblk;
-  int32_t tile_waves, pad_w;        // wavefronts per strand tile (partials per tile)
+  int32_t tile_waves, ctrl_chunks;  // wavefronts per strand tile; control workgroups of the instance
+  int32_t ctrl_chunk, pad_c;        // strand partials per control workgroup
   // scalar parameters (ws_params)
   double mass_scale, damping, tol, mu, strength, stick, torsion_factor, epsilon, slip_rate, rot_factor, dent_rate;
   double edge_thr, l_ref, jlen_min;
@@ -105,8 +110,9 @@ struct Inst {
   // control
   Ctl* ctl;
   double* hist;              // [hist_len * HIST_STRIDE]
-  double* spart;             // [SPART][ntiles * tile_waves]
+  double* spart;             // [SPART][ntiles] one partial per strand tile
   double* cpart;             // [CPART][ncblk]
+  double* rpart;             // [ctrl_chunks][RPART] control-chunk partials (ctrl_chunks > 1)
 };
 
 // one strand tile: strand-local node range [k0, k1) plus the strand's layout, so that a tile reads its geometry in
@@ -118,5 +124,6 @@ struct TileItem {
   int32_t closed, tile_local, pad0, pad1;
 };
 struct CrossItem { int32_t inst, c0, count, blk; };  // crossings [c0, c0 + count) of one instance, block index in the instance
+struct CtrlItem { int32_t inst, chunk, nchunks, pad; };   // one control workgroup: a chunk of an instance's partials
 
 }  // namespace wsd

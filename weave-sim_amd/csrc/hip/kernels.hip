@@ -627,6 +627,7 @@ struct StrandLDS {
   int32_t rpre[TILE], rpost[TILE];                                   // gathered crossing sides: segments
   SideRec<Real> rec[TILE];                                           // ... and their records
   int32_t glo[TILE], ghi[TILE];                                      // per slot: first / last gathered record it needs
+  double wred[11][TILE / 64];                                        // wavefront partials of the tile
   int wcnt[TILE / 64];
 };
 
@@ -993,7 +994,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   }
   if (OUT) return;
 
-  // ---------------------------------------------------------------- partial per wavefront (fixed order; reduced by k_ctrl):
+  // ---------------------------------------------------------------- partial of the tile (fixed order; reduced by k_ctrl):
   // sums keN, keR, eS, eSh, eB, eT | maxima |F|, |tm|, |load|, |N|, |M|
   double v[11] = {keN, keR, eS, eSh, eB, eT, mxF, mxT, mxL, mxN, mxM};
 #pragma unroll
@@ -1006,9 +1007,24 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
     }
     v[q] = x;
   }
-  if (lane == 0) {   // [value][wavefront] so that the control kernel's loads coalesce
-    const size_t np = size_t(I.ntiles) * I.tile_waves;
-    double* sp = I.spart + size_t(tl.tile_local) * (TILE / 64) + wv;
+  if (TILE > 64) {
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < 11; q++) L.wred[q][wv] = v[q];
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+      for (int q = 0; q < 11; q++) {
+        double x = L.wred[q][0];
+        for (int w = 1; w < TILE / 64; w++) x = q < 6 ? x + L.wred[q][w] : fmax(x, L.wred[q][w]);
+        v[q] = x;
+      }
+    }
+  }
+  if (tid == 0) {   // [value][tile] so that the control kernel's loads coalesce
+    const size_t np = size_t(I.ntiles);
+    double* sp = I.spart + size_t(tl.tile_local);
 #pragma unroll
     for (int q = 0; q < 11; q++) sp[q * np] = v[q];
   }
@@ -1039,32 +1055,74 @@ __device__ __forceinline__ void block_reduce(double (*red)[TILE / 64], double* v
 // one workgroup per instance: fixed-order reduction of the tile and crossing-block partials, then kinetic damping,
 // the residual, the stop test and the history ring (solver_cpu.js:809-837; wgsl.js:990-1047)
 template <typename Real>
-__global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restrict__ insts, int32_t B, Ctl* __restrict__ ctls) {
+__global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restrict__ insts, const CtrlItem* __restrict__ items,
+                                                      int32_t nitems, Ctl* __restrict__ ctls) {
   __shared__ double red[13][CTRL_THREADS / 64];
-  const int b = blockIdx.x;
-  if (b >= B) return;
-  const Inst<Real>& I = insts[b];
-  Ctl* ctl = ctls + b;                             // read beside the descriptor, not through it
+  __shared__ int last;
+  if (int(blockIdx.x) >= nitems) return;
+  const CtrlItem it = items[blockIdx.x];
+  const Inst<Real>& I = insts[it.inst];
+  Ctl* ctl = ctls + it.inst;                       // read beside the descriptor, not through it
   const int tid = threadIdx.x;
   Ctl c = *ctl;
-  // the partial loads are issued before the liveness test (a stopped instance's partials are read and discarded)
+  // this workgroup's share of the strand-tile and crossing-block partials (value-major: coalesced loads).  A round
+  // issues every load of CTRL_PER partials per lane before using any, so a round costs one memory latency; the
+  // loads are issued before the liveness test (a stopped instance's partials are read and discarded)
   double s12[13];
   for (int q = 0; q < 7; q++) s12[q] = 0.0;
   for (int q = 7; q < 13; q++) s12[q] = q == 12 ? -INFINITY : 0.0;
-  const int np = I.ntiles * I.tile_waves;
-  for (int w = tid; w < np; w += CTRL_THREADS) {
-    const double* sp = I.spart + w;
+  const int np = I.ntiles;
+  const int p0 = it.chunk * I.ctrl_chunk, p1 = min(np, p0 + I.ctrl_chunk);
+  for (int w0 = p0; w0 < p1; w0 += CTRL_THREADS * CTRL_PER) {
+    double x[CTRL_PER][11];
 #pragma unroll
-    for (int q = 0; q < 6; q++) s12[q] += sp[size_t(q) * np];
+    for (int k = 0; k < CTRL_PER; k++) {
+      const int w = w0 + k * CTRL_THREADS + tid;
+      const bool in = w < p1;
+      const double* sp = I.spart + (in ? w : p0);
 #pragma unroll
-    for (int q = 6; q < 11; q++) s12[q + 1] = fmax(s12[q + 1], sp[size_t(q) * np]);
+      for (int q = 0; q < 11; q++) x[k][q] = sp[size_t(q) * np];
+      if (!in) {
+#pragma unroll
+        for (int q = 0; q < 11; q++) x[k][q] = q < 6 ? 0.0 : -INFINITY;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < CTRL_PER; k++) {
+#pragma unroll
+      for (int q = 0; q < 6; q++) s12[q] += x[k][q];
+#pragma unroll
+      for (int q = 6; q < 11; q++) s12[q + 1] = fmax(s12[q + 1], x[k][q]);
+    }
   }
-  for (int w = tid; w < I.ncblk; w += CTRL_THREADS) {
+  const int cper = (I.ncblk + it.nchunks - 1) / it.nchunks;
+  const int c0 = it.chunk * cper, c1 = min(I.ncblk, c0 + cper);
+  for (int w = c0 + tid; w < c1; w += CTRL_THREADS) {
     s12[6] += I.cpart[w];
     s12[12] = fmax(s12[12], I.cpart[I.ncblk + w]);
   }
   if (!(c.done == 0 && c.iter < c.stop_at)) return;   // uniform over the workgroup
   block_reduce<CTRL_THREADS, 13>(red, s12, 7);
+  if (it.nchunks > 1) {
+    // several workgroups: each leaves its partial; the last to arrive reduces them in chunk order and goes on
+    if (tid == 0) {
+      for (int q = 0; q < 13; q++) I.rpart[size_t(it.chunk) * RPART + q] = s12[q];
+      __threadfence();
+      last = atomicAdd(&ctl->counter, 1) == it.nchunks - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    if (tid != 0) return;
+    __threadfence();
+    const volatile double* rp = I.rpart;
+    for (int q = 0; q < 7; q++) s12[q] = 0.0;
+    for (int q = 7; q < 13; q++) s12[q] = q == 12 ? -INFINITY : 0.0;
+    for (int k = 0; k < it.nchunks; k++) {
+      for (int q = 0; q < 7; q++) s12[q] += rp[size_t(k) * RPART + q];
+      for (int q = 7; q < 13; q++) s12[q] = fmax(s12[q], rp[size_t(k) * RPART + q]);
+    }
+    c.counter = 0;                                 // the other workgroups change nothing else in the record
+  }
   if (tid != 0) return;
   const double eCont = s12[6];
   double ke = 0.5 * (s12[0] + s12[1]);
@@ -1162,8 +1220,8 @@ WSD_INST(float, 64) WSD_INST(float, 128) WSD_INST(float, 256)
 WSD_INST(double, 64) WSD_INST(double, 128) WSD_INST(double, 256)
 template __global__ void k_cross<float>(const Inst<float>*, const CrossItem*, int32_t, const Ctl*);
 template __global__ void k_cross<double>(const Inst<double>*, const CrossItem*, int32_t, const Ctl*);
-template __global__ void k_ctrl<float>(const Inst<float>*, int32_t, Ctl*);
-template __global__ void k_ctrl<double>(const Inst<double>*, int32_t, Ctl*);
+template __global__ void k_ctrl<float>(const Inst<float>*, const CtrlItem*, int32_t, Ctl*);
+template __global__ void k_ctrl<double>(const Inst<double>*, const CtrlItem*, int32_t, Ctl*);
 template __global__ void k_al_preview<float>(const Inst<float>*, int32_t, double*);
 template __global__ void k_al_preview<double>(const Inst<double>*, int32_t, double*);
 template __global__ void k_begin<float>(Ctl*, const int32_t*, const int32_t*, int32_t, int32_t, int32_t);

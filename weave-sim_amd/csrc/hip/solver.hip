@@ -77,7 +77,7 @@ WS_API int ws_batch_create(ws_ctx* ctx, const ws_model* const* models, const ws_
       autotune(ctx, models, states, B, prec, 256, 16, table, best);
       t = best.tile; if (ps <= 0) ps = best.per_submit;
     }
-    *out = new ws_batch{make_batch(ctx, models, states, B, prec, t, ps, hist)};
+    *out = new ws_batch{make_batch(ctx, models, states, B, prec, t, ps, hist, opts ? opts->ctrl_chunk : 0)};
     return WS_OK;
   } catch (const Fail& f) { return fail(ctx, f); }
 }
@@ -102,7 +102,7 @@ WS_API int ws_autotune(ws_ctx* ctx, const ws_model* const* models, const ws_stat
 WS_API int ws_batch_get_opts(ws_batch* batch, ws_batch_opts* out) {
   if (!batch || !out) return WS_ERR_ARG;
   out->precision = batch->impl->precision; out->per_submit = batch->impl->per_submit;
-  out->tile = batch->impl->tile; out->history = batch->impl->hist_len;
+  out->tile = batch->impl->tile; out->history = batch->impl->hist_len; out->ctrl_chunk = batch->impl->ctrl_chunk;
   return WS_OK;
 }
 

@@ -86,7 +86,7 @@ def autotune(models, precision="fp32", iters=256, ctx=None):
 class GpuSolver:
     """B instances (each its own topology allowed) relaxed in the same launches."""
 
-    def __init__(self, models, precision="fp32", tile=0, per_submit=0, history=4096, ctx=None, states=None):
+    def __init__(self, models, precision="fp32", tile=0, per_submit=0, history=4096, ctx=None, states=None, ctrl_chunk=0):
         """tile: 64 / 128 / 256 lanes per strand tile, 0 = chosen by size, "autotune" (or -1) = the autotuner's pick;
         per_submit: iterations per captured graph (0 = 64, or the autotuner's pick)"""
         self.ctx = ctx or default_context()
@@ -99,7 +99,7 @@ class GpuSolver:
         sp = (C.POINTER(A.ws_state) * self.B)(*[C.pointer(s.st) for s in sts])
         if tile == "autotune":
             tile = A.WS_TILE_AUTOTUNE
-        opts = A.ws_batch_opts(PRECISIONS[precision], per_submit, tile, history)
+        opts = A.ws_batch_opts(PRECISIONS[precision], per_submit, tile, history, ctrl_chunk)
         self.h = C.c_void_p()
         self.ctx.check(self.lib.ws_batch_create(self.ctx.h, mp, sp, self.B, C.byref(opts), C.byref(self.h)))
         self._ctl = (A.ws_control * self.B)()
@@ -109,7 +109,8 @@ class GpuSolver:
         """options in effect: precision, per_submit, tile (as chosen by size or by the autotuner), history"""
         o = A.ws_batch_opts()
         self.ctx.check(self.lib.ws_batch_get_opts(self.h, C.byref(o)))
-        return {"precision": o.precision, "per_submit": o.per_submit, "tile": o.tile, "history": o.history}
+        return {"precision": o.precision, "per_submit": o.per_submit, "tile": o.tile, "history": o.history,
+                "ctrl_chunk": o.ctrl_chunk}
 
     @property
     def tile(self):
