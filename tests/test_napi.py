@@ -1,0 +1,74 @@
+"""The Node.js host: the N-API addon over the C ABI and the JavaScript GpuSolver (the interface of
+web/js/solver_gpu.js).  Node runs as a child process; the GPU case compares the JS path with the Python path
+through the same library, bitwise."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import cases
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+JS = os.path.join(ROOT, "weave-sim_amd", "js", "solver_mi355x.js")
+ADDON = os.path.join(ROOT, "weave-sim_amd", "lib", "weave_napi.node")
+NODE = shutil.which("node")
+
+pytestmark = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON), reason="node or the N-API addon is absent")
+
+SCENE = "{E:10000,G:600,density:500,width:40,height:4,mu:0.3,tol:1e-6}"
+
+
+def run_node(script, timeout=240):
+    r = subprocess.run([NODE, "-e", f"const W=require({json.dumps(JS)});\n" + script], capture_output=True, text=True,
+                       timeout=timeout, cwd=ROOT)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return r.stdout
+
+
+def test_addon_exports_and_model_builder():
+    """the addon loads in Node, the JS model builder matches the Python one, and the GpuSolver class carries the
+    reference's method set"""
+    out = run_node(f"""
+const m = W.Model.hexDome({{span:3000,spacing:300,segment:50,contraction:0.15,noise:1,seed:1}}, {SCENE});
+const names = Object.getOwnPropertyNames(W.GpuSolver.prototype).sort();
+console.log(JSON.stringify({{info: m.info, x: Array.from(m.st.X.slice(0, 6)), u: Array.from(m.st.u.slice(0, 4)),
+  names, scales: W.GpuSolver.computeScales([m])}}));""")
+    d = json.loads(out)
+    py = cases.config1()
+    assert d["info"]["n"] == py.n and d["info"]["K"] == py.K
+    assert np.array_equal(np.array(d["x"]), py["X"][:6]) and np.array_equal(np.array(d["u"]), py["u"][:4])
+    for name in ("upload", "updateParams", "step", "readControl", "readHistory", "download", "downloadAll",
+                 "internalize", "dispose", "compileInfo"):
+        assert name in d["names"], name
+    assert d["scales"] == {"PB": 0, "FB": 0, "TB": 0, "KB": 0}
+
+
+@pytest.mark.gpu
+def test_js_gpu_solver_matches_python_path(W, gpu_ctx):
+    """600 iterations of the 3 m dome through the JS GpuSolver (Promises, in-place download) equal the Python path"""
+    out = run_node(f"""
+(async () => {{
+  const dev = await W.getDevice();
+  if (!dev) throw new Error('no device');
+  const m = W.Model.hexDome({{span:3000,spacing:300,segment:50,contraction:0.15,noise:1,seed:1}}, {SCENE});
+  const g = new W.GpuSolver(dev, [m], {{ perSubmit: 64, tile: 64 }});
+  const c = await g.step(600);
+  await g.download(0);
+  const n = await g.readHistory(0);
+  g.dispose();
+  console.log(JSON.stringify({{ctl: c[0], X: Array.from(m.st.X), stuck: Array.from(m.st.stuck), n,
+                               it: m.st.iteration, hist: m.st.history.kinetic.length}}));
+}})().catch((e) => {{ console.error(e); process.exit(1); }});""")
+    d = json.loads(out)
+    m = cases.config1()
+    g = W.GpuSolver([m], precision="fp32", tile=64)
+    c = g.step(600)[0]
+    s = g.state(0)
+    g.close()
+    assert d["ctl"]["iter"] == c["iter"] == 600 and d["it"] == 600
+    assert np.array_equal(np.array(d["X"]), s["X"])
+    assert np.array_equal(np.array(d["stuck"], dtype=np.uint8), s["stuck"])
+    assert d["ctl"]["ke"] == c["ke"] and d["hist"] == d["n"] == 600

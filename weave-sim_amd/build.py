@@ -45,7 +45,7 @@ def build_napi():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "weave_napi.node")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + INC, "-I" + node_inc,
-          "-DNAPI_VERSION=8", os.path.join(HERE, "csrc", "napi", "addon.cpp"), "-o", out,
+          "-DNAPI_VERSION=8", "-DNODE_GYP_MODULE_NAME=weave_napi", os.path.join(HERE, "csrc", "napi", "addon.cpp"), "-o", out,
           "-L" + LIB, "-lweave_hip", "-lweave_model", "-Wl,-rpath,$ORIGIN"])
     return out
 

@@ -186,6 +186,7 @@ WS_API int ws_cancel(ws_server* srv, int64_t id) {
 WS_API int ws_pump(ws_server* srv, int32_t* n_active, int32_t* n_queued) {
   if (!srv) return WS_ERR_ARG;
   try {
+    HIPCHK(hipSetDevice(srv->ctx->device));
     srv->pump(n_active, n_queued);
     return WS_OK;
   } catch (const Fail& f) { return fail(srv->ctx, f); }

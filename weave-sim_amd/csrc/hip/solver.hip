@@ -109,9 +109,10 @@ WS_API int ws_batch_get_opts(ws_batch* batch, ws_batch_opts* out) {
 WS_API int ws_batch_destroy(ws_batch* batch) { delete batch; return WS_OK; }
 WS_API int ws_batch_size(ws_batch* batch, int32_t* B) { if (!batch || !B) return WS_ERR_ARG; *B = batch->impl->B; return WS_OK; }
 
+// every call makes the batch's device current on the calling thread (the N-API addon calls from worker threads)
 #define BATCH_CALL(expr) \
   if (!batch) return WS_ERR_ARG; \
-  try { return (expr); } catch (const Fail& f) { return fail(batch->impl->ctx, f); }
+  try { HIPCHK(hipSetDevice(batch->impl->ctx->device)); return (expr); } catch (const Fail& f) { return fail(batch->impl->ctx, f); }
 
 WS_API int ws_upload(ws_batch* batch, int32_t b, const ws_model* model, const ws_state* state) {
   if (!batch || !model || !state || b < 0 || b >= batch->impl->B) return WS_ERR_ARG;

@@ -1,0 +1,528 @@
+// Node N-API addon: the thin binding between the JavaScript host (weave-sim_amd/js/solver_mi355x.js, the GpuSolver
+// interface of web/js/solver_gpu.js) and the C ABI (include/weave.h, include/weave_model.h).  Device calls that
+// wait for the GPU (step, readControl, readHistory, download, server pump) run on libuv worker threads and resolve
+// Promises, as the reference's mapAsync-based methods do; the rest are synchronous.  Model state arrays are exposed
+// to JavaScript as typed arrays over the model's own memory, so a download overwrites them in place.
+#include <node_api.h>
+
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../../include/weave.h"
+#include "../../../include/weave_model.h"
+
+namespace {
+
+#define NAPI_CALL(env, call)                                                         \
+  do {                                                                               \
+    if ((call) != napi_ok) {                                                         \
+      const napi_extended_error_info* ei = nullptr;                                  \
+      napi_get_last_error_info((env), &ei);                                          \
+      napi_throw_error((env), nullptr, ei && ei->error_message ? ei->error_message : "napi call failed"); \
+      return nullptr;                                                                \
+    }                                                                                \
+  } while (0)
+
+struct Model {
+  wsm_model* h = nullptr;
+  ws_model md{};
+  ws_state st{};
+  wsm_info info{};
+};
+struct Batch { ws_ctx* ctx = nullptr; ws_batch* b = nullptr; int32_t B = 0; };
+struct Server { ws_ctx* ctx = nullptr; ws_server* s = nullptr; };
+
+// ---------------------------------------------------------------------------------------------- helpers
+napi_value undef(napi_env env) { napi_value v; napi_get_undefined(env, &v); return v; }
+napi_value num(napi_env env, double x) { napi_value v; napi_create_double(env, x, &v); return v; }
+napi_value i32(napi_env env, int32_t x) { napi_value v; napi_create_int32(env, x, &v); return v; }
+napi_value str(napi_env env, const char* s) { napi_value v; napi_create_string_utf8(env, s, NAPI_AUTO_LENGTH, &v); return v; }
+void set(napi_env env, napi_value o, const char* k, napi_value v) { napi_set_named_property(env, o, k, v); }
+
+bool get_args(napi_env env, napi_callback_info info, size_t n, napi_value* argv) {
+  size_t argc = n;
+  if (napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr) != napi_ok) return false;
+  for (size_t i = argc; i < n; i++) argv[i] = undef(env);
+  return true;
+}
+template <typename T> T* ext(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok) return nullptr;
+  return static_cast<T*>(p);
+}
+double getd(napi_env env, napi_value o, const char* k, double dflt) {
+  bool has = false; napi_value v; double x = dflt;
+  if (napi_has_named_property(env, o, k, &has) == napi_ok && has && napi_get_named_property(env, o, k, &v) == napi_ok) {
+    napi_valuetype t; napi_typeof(env, v, &t);
+    if (t == napi_number) napi_get_value_double(env, v, &x);
+    else if (t == napi_boolean) { bool b; napi_get_value_bool(env, v, &b); x = b ? 1 : 0; }
+  }
+  return x;
+}
+bool is_object(napi_env env, napi_value v) { napi_valuetype t; napi_typeof(env, v, &t); return t == napi_object; }
+
+napi_value throw_status(napi_env env, int rc, const char* what) {
+  std::string m = std::string(what) + " failed with status " + std::to_string(rc);
+  napi_throw_error(env, nullptr, m.c_str());
+  return nullptr;
+}
+
+napi_value ctl_object(napi_env env, const ws_control& c) {
+  napi_value o; napi_create_object(env, &o);
+  set(env, o, "iter", i32(env, c.iter)); set(env, o, "done", i32(env, c.done)); set(env, o, "rollback", i32(env, c.rollback));
+  set(env, o, "alUpdate", i32(env, c.al_update)); set(env, o, "stopAt", i32(env, c.stop_at)); set(env, o, "histN", i32(env, c.hist_n));
+  set(env, o, "kePrev", num(env, c.ke_prev)); set(env, o, "ke", num(env, c.ke)); set(env, o, "resid", num(env, c.resid));
+  set(env, o, "fref", num(env, c.fref)); set(env, o, "eStretch", num(env, c.e_stretch)); set(env, o, "eBend", num(env, c.e_bend));
+  set(env, o, "eTors", num(env, c.e_tors)); set(env, o, "eCont", num(env, c.e_cont)); set(env, o, "eShear", num(env, c.e_shear));
+  set(env, o, "resT", num(env, c.res_t)); set(env, o, "resF", num(env, c.res_f));
+  return o;
+}
+
+// ---------------------------------------------------------------------------------------------- async work
+struct Job {
+  napi_async_work work = nullptr;
+  napi_deferred deferred = nullptr;
+  std::function<int()> run;                               // worker thread: C ABI calls only
+  std::function<napi_value(napi_env)> done;               // main thread: builds the resolved value
+  std::function<std::string()> error;                     // main thread: message of a failed run
+  int rc = 0;
+};
+
+void job_execute(napi_env, void* data) { Job* j = static_cast<Job*>(data); j->rc = j->run(); }
+void job_complete(napi_env env, napi_status, void* data) {
+  Job* j = static_cast<Job*>(data);
+  if (j->rc == 0) {
+    napi_resolve_deferred(env, j->deferred, j->done(env));
+  } else {
+    napi_value msg, err;
+    std::string m = j->error ? j->error() : ("status " + std::to_string(j->rc));
+    napi_create_string_utf8(env, m.c_str(), NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, nullptr, msg, &err);
+    napi_reject_deferred(env, j->deferred, err);
+  }
+  napi_delete_async_work(env, j->work);
+  delete j;
+}
+napi_value queue_job(napi_env env, Job* j, const char* name) {
+  napi_value promise, res;
+  NAPI_CALL(env, napi_create_promise(env, &j->deferred, &promise));
+  NAPI_CALL(env, napi_create_string_utf8(env, name, NAPI_AUTO_LENGTH, &res));
+  NAPI_CALL(env, napi_create_async_work(env, nullptr, res, job_execute, job_complete, j, &j->work));
+  NAPI_CALL(env, napi_queue_async_work(env, j->work));
+  return promise;
+}
+std::function<std::string()> ctx_error(ws_ctx* ctx) { return [ctx]() { return std::string(ws_last_error(ctx)); }; }
+
+// ---------------------------------------------------------------------------------------------- device
+napi_value DeviceCreate(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  int32_t dev = 0; napi_get_value_int32(env, a[0], &dev);
+  ws_ctx* ctx = nullptr;
+  if (ws_ctx_create(dev, &ctx) != WS_OK) { napi_value n; napi_get_null(env, &n); return n; }   // getDevice() → null
+  napi_value v; NAPI_CALL(env, napi_create_external(env, ctx, [](napi_env, void* p, void*) { ws_ctx_destroy(static_cast<ws_ctx*>(p)); }, nullptr, &v));
+  return v;
+}
+napi_value DeviceInfo(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  ws_ctx* ctx = ext<ws_ctx>(env, a[0]);
+  char name[256] = {0}; int64_t hbm = 0; int32_t ncu = 0;
+  int rc = ws_device_info(ctx, name, 256, &hbm, &ncu);
+  if (rc) return throw_status(env, rc, "ws_device_info");
+  napi_value o; napi_create_object(env, &o);
+  set(env, o, "name", str(env, name)); set(env, o, "hbmBytes", num(env, double(hbm))); set(env, o, "cu", i32(env, ncu));
+  return o;
+}
+
+// ---------------------------------------------------------------------------------------------- host models
+void scene_from(napi_env env, napi_value o, wsm_scene& sc) {
+  wsm_scene_defaults(&sc);
+  if (!is_object(env, o)) return;
+#define D(field, key) sc.field = getd(env, o, key, sc.field)
+#define I(field, key) sc.field = int32_t(getd(env, o, key, sc.field))
+  D(grp.E, "E"); D(grp.G, "G"); D(grp.density, "density"); D(grp.width, "width"); D(grp.height, "height");
+  D(grp.E90, "E90"); D(grp.fc90, "fc90"); D(grp.H, "H");
+  D(par.mass_scale, "mass_scale"); D(par.damping, "damping"); D(par.tol, "tol"); D(par.mu, "mu");
+  D(par.strength, "strength"); D(par.stick_strength, "stick_strength"); D(par.torsion_factor, "torsion_factor");
+  D(par.epsilon, "epsilon"); D(par.slip_rate, "slip_rate"); D(par.rot_factor, "rot_factor"); D(par.dent_rate, "dent_rate");
+  D(par.edge_threshold, "edge_threshold");
+  I(par.kinetic_damping, "kinetic_damping"); I(par.warmup_iter, "warmup_iter"); I(par.max_iter, "max_iter");
+  I(par.ramp_iter, "ramp_iter"); I(par.rot_friction, "rot_friction"); I(par.seat_law, "seat_law"); I(par.seat_gap, "seat_gap");
+  I(par.dent_after, "dent_after"); I(par.augmented, "augmented"); I(par.al_every, "al_every");
+  I(contacts, "contacts"); I(locked, "locked"); I(gravity, "gravity");
+#undef D
+#undef I
+}
+
+napi_value wrap_model(napi_env env, wsm_model* h) {
+  Model* m = new Model();
+  m->h = h;
+  wsm_get(h, &m->md, &m->st);
+  wsm_info_get(h, &m->info);
+  napi_value v;
+  NAPI_CALL(env, napi_create_external(env, m, [](napi_env, void* p, void*) {
+    Model* mm = static_cast<Model*>(p); wsm_free(mm->h); delete mm; }, nullptr, &v));
+  return v;
+}
+
+napi_value ModelHexDome(napi_env env, napi_callback_info info) {
+  napi_value a[2]; get_args(env, info, 2, a);
+  wsm_hex_spec spec{};
+  spec.span = getd(env, a[0], "span", 3000); spec.spacing = getd(env, a[0], "spacing", 300);
+  spec.segment = getd(env, a[0], "segment", 50); spec.contraction = getd(env, a[0], "contraction", 0.15);
+  spec.noise = getd(env, a[0], "noise", 1.0); spec.seed = uint32_t(getd(env, a[0], "seed", 1));
+  wsm_scene sc; scene_from(env, a[1], sc);
+  wsm_model* h = nullptr;
+  int rc = wsm_hex_dome(&spec, &sc, &h);
+  if (rc) { napi_throw_error(env, nullptr, wsm_last_error()); return nullptr; }
+  return wrap_model(env, h);
+}
+
+napi_value ModelClone(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  Model* m = ext<Model>(env, a[0]);
+  if (!m) { napi_throw_type_error(env, nullptr, "not a model"); return nullptr; }
+  wsm_model* h = nullptr;
+  if (wsm_clone(m->h, &h)) { napi_throw_error(env, nullptr, wsm_last_error()); return nullptr; }
+  return wrap_model(env, h);
+}
+
+napi_value ModelInfo(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  Model* m = ext<Model>(env, a[0]);
+  if (!m) { napi_throw_type_error(env, nullptr, "not a model"); return nullptr; }
+  napi_value o; napi_create_object(env, &o);
+  const wsm_info& f = m->info;
+  set(env, o, "n", i32(env, f.n)); set(env, o, "m", i32(env, f.m)); set(env, o, "p", i32(env, f.p));
+  set(env, o, "K", i32(env, f.K)); set(env, o, "C", i32(env, f.C)); set(env, o, "S", i32(env, f.S));
+  set(env, o, "rings", i32(env, f.rings)); set(env, o, "subdivisions", i32(env, f.subdivisions));
+  set(env, o, "span", num(env, f.span_plan)); set(env, o, "rise", num(env, f.rise));
+  return o;
+}
+
+// typed arrays over the model's state (the arrays a download overwrites), and its scalars
+napi_value ModelState(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  Model* m = ext<Model>(env, a[0]);
+  if (!m) { napi_throw_type_error(env, nullptr, "not a model"); return nullptr; }
+  const ws_model& md = m->md;
+  const size_t n = md.n, mm = md.m, p = md.p, K = md.K, C = md.C;
+  napi_value o; napi_create_object(env, &o);
+  auto arr = [&](const char* name, void* ptr, size_t count, napi_typedarray_type t, size_t esz) {
+    if (!ptr || count == 0) return;
+    napi_value ab, ta;
+    if (napi_create_external_arraybuffer(env, ptr, count * esz, nullptr, nullptr, &ab) != napi_ok) return;
+    if (napi_create_typedarray(env, t, count, ab, 0, &ta) != napi_ok) return;
+    set(env, o, name, ta);
+  };
+  const ws_state& s = m->st;
+  arr("X", s.X, 3 * n, napi_float64_array, 8); arr("X0", s.X0, 3 * n, napi_float64_array, 8);
+  arr("V", s.V, 3 * n, napi_float64_array, 8); arr("Q", s.Q, 4 * mm, napi_float64_array, 8);
+  arr("Q0", s.Q0, 4 * mm, napi_float64_array, 8); arr("Wr", s.W, 3 * mm, napi_float64_array, 8);
+  arr("u", s.u, 2 * K, napi_float64_array, 8); arr("nPrev", s.n_prev, 3 * K, napi_float64_array, 8);
+  arr("slip", s.slip, K, napi_float64_array, 8); arr("stuck", s.stuck, K, napi_uint8_array, 1);
+  arr("lam", s.lam, K, napi_float64_array, 8); arr("penAcc", s.pen_acc, K, napi_float64_array, 8);
+  arr("N", s.N, mm, napi_float64_array, 8); arr("V2", s.Vsh, 2 * mm, napi_float64_array, 8);
+  arr("M", s.M, 3 * p, napi_float64_array, 8); arr("cM", s.cM, 2 * C, napi_float64_array, 8);
+  arr("reactions", s.reactions, 3 * n, napi_float64_array, 8);
+  arr("Fn", s.Fn, K, napi_float64_array, 8); arr("Ft", s.Ft, K, napi_float64_array, 8); arr("gap", s.gap, K, napi_float64_array, 8);
+  arr("state", s.kstate, K, napi_float64_array, 8); arr("theta", s.theta, K, napi_float64_array, 8);
+  arr("Anom", s.Anom, K, napi_float64_array, 8); arr("pressure", s.pressure, K, napi_float64_array, 8);
+  arr("util", s.util, K, napi_float64_array, 8); arr("phi", s.phi, 2 * K, napi_float64_array, 8);
+  arr("mid", s.mid, 3 * K, napi_float64_array, 8); arr("ctype", s.ctype, K, napi_int32_array, 4);
+  arr("faceSide", s.face_side, K, napi_int32_array, 4);
+  set(env, o, "iteration", i32(env, s.iteration)); set(env, o, "residual", num(env, s.residual));
+  set(env, o, "converged", i32(env, s.converged));
+  return o;
+}
+
+napi_value scalars(napi_env env, const ws_state& s) {
+  napi_value o; napi_create_object(env, &o);
+  set(env, o, "iteration", i32(env, s.iteration)); set(env, o, "residual", num(env, s.residual));
+  set(env, o, "converged", i32(env, s.converged)); set(env, o, "kePrev", num(env, s.ke_prev)); set(env, o, "ke", num(env, s.ke));
+  set(env, o, "resForce", num(env, s.res_force)); set(env, o, "resTorque", num(env, s.res_torque)); set(env, o, "fref", num(env, s.fref));
+  set(env, o, "eStretch", num(env, s.e_stretch)); set(env, o, "eShear", num(env, s.e_shear)); set(env, o, "eBend", num(env, s.e_bend));
+  set(env, o, "eTwist", num(env, s.e_twist)); set(env, o, "eContact", num(env, s.e_contact));
+  return o;
+}
+
+// ---------------------------------------------------------------------------------------------- batches
+bool models_from(napi_env env, napi_value arr, std::vector<Model*>& out) {
+  bool isarr = false; napi_is_array(env, arr, &isarr);
+  if (!isarr) return false;
+  uint32_t n = 0; napi_get_array_length(env, arr, &n);
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value e; napi_get_element(env, arr, i, &e);
+    Model* m = ext<Model>(env, e);
+    if (!m) return false;
+    out.push_back(m);
+  }
+  return !out.empty();
+}
+
+napi_value BatchCreate(napi_env env, napi_callback_info info) {
+  napi_value a[3]; get_args(env, info, 3, a);
+  ws_ctx* ctx = ext<ws_ctx>(env, a[0]);
+  std::vector<Model*> ms;
+  if (!ctx || !models_from(env, a[1], ms)) { napi_throw_type_error(env, nullptr, "batchCreate(ctx, models[], opts)"); return nullptr; }
+  ws_batch_opts o{};
+  o.precision = int32_t(getd(env, a[2], "precision", WS_FP32_FP64)); o.per_submit = int32_t(getd(env, a[2], "perSubmit", 0));
+  o.tile = int32_t(getd(env, a[2], "tile", 0)); o.history = int32_t(getd(env, a[2], "history", 0));
+  o.ctrl_chunk = int32_t(getd(env, a[2], "ctrlChunk", 0));
+  std::vector<const ws_model*> mp; std::vector<const ws_state*> sp;
+  for (Model* m : ms) { mp.push_back(&m->md); sp.push_back(&m->st); }
+  Batch* b = new Batch(); b->ctx = ctx; b->B = int32_t(ms.size());
+  int rc = ws_batch_create(ctx, mp.data(), sp.data(), b->B, &o, &b->b);
+  if (rc) { std::string e = ws_last_error(ctx); delete b; napi_throw_error(env, nullptr, e.c_str()); return nullptr; }
+  napi_value v;
+  NAPI_CALL(env, napi_create_external(env, b, [](napi_env, void* p, void*) {
+    Batch* bb = static_cast<Batch*>(p); if (bb->b) ws_batch_destroy(bb->b); delete bb; }, nullptr, &v));
+  return v;
+}
+
+napi_value BatchDestroy(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  Batch* b = ext<Batch>(env, a[0]);
+  if (b && b->b) { ws_batch_destroy(b->b); b->b = nullptr; }
+  return undef(env);
+}
+
+napi_value BatchOpts(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  Batch* b = ext<Batch>(env, a[0]);
+  ws_batch_opts o{};
+  if (!b || !b->b || ws_batch_get_opts(b->b, &o)) { napi_throw_error(env, nullptr, "batchOpts: no batch"); return nullptr; }
+  napi_value r; napi_create_object(env, &r);
+  set(env, r, "precision", i32(env, o.precision)); set(env, r, "perSubmit", i32(env, o.per_submit));
+  set(env, r, "tile", i32(env, o.tile)); set(env, r, "history", i32(env, o.history));
+  return r;
+}
+
+napi_value BatchStep(napi_env env, napi_callback_info info) {
+  napi_value a[4]; get_args(env, info, 4, a);
+  Batch* b = ext<Batch>(env, a[0]);
+  if (!b || !b->b) { napi_throw_type_error(env, nullptr, "step: no batch"); return nullptr; }
+  int32_t n_iter = 0; napi_get_value_int32(env, a[1], &n_iter);
+  std::vector<int32_t> sel;
+  bool isarr = false; napi_is_array(env, a[2], &isarr);
+  if (isarr) {
+    uint32_t n = 0; napi_get_array_length(env, a[2], &n);
+    for (uint32_t i = 0; i < n; i++) { napi_value e; napi_get_element(env, a[2], i, &e); int32_t x; napi_get_value_int32(env, e, &x); sel.push_back(x); }
+  }
+  bool force = true; napi_valuetype t; napi_typeof(env, a[3], &t);
+  if (t == napi_boolean) napi_get_value_bool(env, a[3], &force);
+  auto ctl = std::make_shared<std::vector<ws_control>>(b->B);
+  Job* j = new Job();
+  ws_batch* bh = b->b;
+  j->run = [=]() { return ws_step(bh, n_iter, sel.empty() ? nullptr : sel.data(), int32_t(sel.size()), force ? 1 : 0, ctl->data()); };
+  j->done = [=](napi_env e) {
+    napi_value arr; napi_create_array_with_length(e, ctl->size(), &arr);
+    for (size_t i = 0; i < ctl->size(); i++) napi_set_element(e, arr, uint32_t(i), ctl_object(e, (*ctl)[i]));
+    return arr;
+  };
+  j->error = ctx_error(b->ctx);
+  return queue_job(env, j, "ws_step");
+}
+
+napi_value BatchReadControl(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  Batch* b = ext<Batch>(env, a[0]);
+  if (!b || !b->b) { napi_throw_type_error(env, nullptr, "readControl: no batch"); return nullptr; }
+  auto ctl = std::make_shared<std::vector<ws_control>>(b->B);
+  Job* j = new Job();
+  ws_batch* bh = b->b;
+  j->run = [=]() { return ws_read_control(bh, ctl->data()); };
+  j->done = [=](napi_env e) {
+    napi_value arr; napi_create_array_with_length(e, ctl->size(), &arr);
+    for (size_t i = 0; i < ctl->size(); i++) napi_set_element(e, arr, uint32_t(i), ctl_object(e, (*ctl)[i]));
+    return arr;
+  };
+  j->error = ctx_error(b->ctx);
+  return queue_job(env, j, "ws_read_control");
+}
+
+napi_value BatchReadHistory(napi_env env, napi_callback_info info) {
+  napi_value a[3]; get_args(env, info, 3, a);
+  Batch* b = ext<Batch>(env, a[0]);
+  int32_t idx = 0, cap = 4096; napi_get_value_int32(env, a[1], &idx);
+  napi_valuetype t; napi_typeof(env, a[2], &t); if (t == napi_number) napi_get_value_int32(env, a[2], &cap);
+  if (!b || !b->b) { napi_throw_type_error(env, nullptr, "readHistory: no batch"); return nullptr; }
+  auto rows = std::make_shared<std::vector<double>>(size_t(cap) * 9);
+  auto nrows = std::make_shared<int32_t>(0);
+  Job* j = new Job();
+  ws_batch* bh = b->b;
+  j->run = [=]() { return ws_read_history(bh, idx, rows->data(), cap, nrows.get()); };
+  j->done = [=](napi_env e) {
+    napi_value ab, ta; void* data = nullptr;
+    size_t bytes = size_t(*nrows) * 9 * sizeof(double);
+    napi_create_arraybuffer(e, bytes, &data, &ab);
+    if (bytes) std::memcpy(data, rows->data(), bytes);
+    napi_create_typedarray(e, napi_float64_array, size_t(*nrows) * 9, ab, 0, &ta);
+    return ta;
+  };
+  j->error = ctx_error(b->ctx);
+  return queue_job(env, j, "ws_read_history");
+}
+
+napi_value BatchDownload(napi_env env, napi_callback_info info) {
+  napi_value a[3]; get_args(env, info, 3, a);
+  Batch* b = ext<Batch>(env, a[0]);
+  int32_t idx = 0; napi_get_value_int32(env, a[1], &idx);
+  Model* m = ext<Model>(env, a[2]);
+  if (!b || !b->b || !m) { napi_throw_type_error(env, nullptr, "download(batch, b, model)"); return nullptr; }
+  Job* j = new Job();
+  ws_batch* bh = b->b;
+  j->run = [=]() { return ws_download(bh, idx, &m->st); };
+  j->done = [=](napi_env e) { return scalars(e, m->st); };
+  j->error = ctx_error(b->ctx);
+  return queue_job(env, j, "ws_download");
+}
+
+napi_value BatchUpload(napi_env env, napi_callback_info info) {
+  napi_value a[3]; get_args(env, info, 3, a);
+  Batch* b = ext<Batch>(env, a[0]);
+  int32_t idx = 0; napi_get_value_int32(env, a[1], &idx);
+  Model* m = ext<Model>(env, a[2]);
+  if (!b || !b->b || !m) { napi_throw_type_error(env, nullptr, "upload(batch, b, model)"); return nullptr; }
+  int rc = ws_upload(b->b, idx, &m->md, &m->st);
+  if (rc) { napi_throw_error(env, nullptr, ws_last_error(b->ctx)); return nullptr; }
+  return undef(env);
+}
+
+napi_value BatchUpdateParams(napi_env env, napi_callback_info info) {
+  napi_value a[3]; get_args(env, info, 3, a);
+  Batch* b = ext<Batch>(env, a[0]);
+  int32_t idx = 0; napi_get_value_int32(env, a[1], &idx);
+  Model* m = ext<Model>(env, a[2]);
+  if (!b || !b->b || !m) { napi_throw_type_error(env, nullptr, "updateParams(batch, b, model)"); return nullptr; }
+  int rc = ws_update_params(b->b, idx, &m->md);
+  if (rc) { napi_throw_error(env, nullptr, ws_last_error(b->ctx)); return nullptr; }
+  return undef(env);
+}
+
+napi_value BatchInternalize(napi_env env, napi_callback_info info) {
+  napi_value a[2]; get_args(env, info, 2, a);
+  Batch* b = ext<Batch>(env, a[0]);
+  int32_t idx = 0; napi_get_value_int32(env, a[1], &idx);
+  if (!b || !b->b) { napi_throw_type_error(env, nullptr, "internalize: no batch"); return nullptr; }
+  int rc = ws_internalize(b->b, idx);
+  if (rc) { napi_throw_error(env, nullptr, ws_last_error(b->ctx)); return nullptr; }
+  return undef(env);
+}
+
+// ---------------------------------------------------------------------------------------------- server
+napi_value ServerCreate(napi_env env, napi_callback_info info) {
+  napi_value a[2]; get_args(env, info, 2, a);
+  ws_ctx* ctx = ext<ws_ctx>(env, a[0]);
+  if (!ctx) { napi_throw_type_error(env, nullptr, "serverCreate(ctx, opts)"); return nullptr; }
+  ws_server_opts o{};
+  o.precision = int32_t(getd(env, a[1], "precision", WS_FP32_FP64)); o.chunk = int32_t(getd(env, a[1], "chunk", 0));
+  o.tile = int32_t(getd(env, a[1], "tile", 0)); o.max_slots = int32_t(getd(env, a[1], "maxSlots", 0));
+  o.hbm_budget = int64_t(getd(env, a[1], "hbmBudget", 0));
+  Server* s = new Server(); s->ctx = ctx;
+  int rc = ws_server_create(ctx, &o, &s->s);
+  if (rc) { std::string e = ws_last_error(ctx); delete s; napi_throw_error(env, nullptr, e.c_str()); return nullptr; }
+  napi_value v;
+  NAPI_CALL(env, napi_create_external(env, s, [](napi_env, void* p, void*) {
+    Server* ss = static_cast<Server*>(p); if (ss->s) ws_server_destroy(ss->s); delete ss; }, nullptr, &v));
+  return v;
+}
+
+napi_value ServerDestroy(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  Server* s = ext<Server>(env, a[0]);
+  if (s && s->s) { ws_server_destroy(s->s); s->s = nullptr; }
+  return undef(env);
+}
+
+napi_value ServerSubmit(napi_env env, napi_callback_info info) {
+  napi_value a[3]; get_args(env, info, 3, a);
+  Server* s = ext<Server>(env, a[0]);
+  Model* m = ext<Model>(env, a[1]);
+  int32_t max_iter = 0; napi_valuetype t; napi_typeof(env, a[2], &t); if (t == napi_number) napi_get_value_int32(env, a[2], &max_iter);
+  if (!s || !s->s || !m) { napi_throw_type_error(env, nullptr, "submit(server, model, maxIter)"); return nullptr; }
+  int64_t id = 0;
+  int rc = ws_submit(s->s, &m->md, &m->st, max_iter, &id);
+  if (rc) { napi_throw_error(env, nullptr, ws_last_error(s->ctx)); return nullptr; }
+  return num(env, double(id));
+}
+
+napi_value ServerPump(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  Server* s = ext<Server>(env, a[0]);
+  if (!s || !s->s) { napi_throw_type_error(env, nullptr, "pump: no server"); return nullptr; }
+  auto out = std::make_shared<std::pair<int32_t, int32_t>>(0, 0);
+  Job* j = new Job();
+  ws_server* sh = s->s;
+  j->run = [=]() { return ws_pump(sh, &out->first, &out->second); };
+  j->done = [=](napi_env e) {
+    napi_value o; napi_create_object(e, &o);
+    set(e, o, "active", i32(e, out->first)); set(e, o, "queued", i32(e, out->second));
+    return o;
+  };
+  j->error = ctx_error(s->ctx);
+  return queue_job(env, j, "ws_pump");
+}
+
+napi_value ServerPoll(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  Server* s = ext<Server>(env, a[0]);
+  if (!s || !s->s) { napi_throw_type_error(env, nullptr, "poll: no server"); return nullptr; }
+  std::vector<ws_result> r(1024); int32_t n = 0;
+  int rc = ws_poll(s->s, r.data(), int32_t(r.size()), &n);
+  if (rc) { napi_throw_error(env, nullptr, ws_last_error(s->ctx)); return nullptr; }
+  static const char* names[4] = {"converged", "max_iter", "cancelled", "non_finite"};
+  napi_value arr; napi_create_array_with_length(env, n, &arr);
+  for (int32_t i = 0; i < n; i++) {
+    napi_value o; napi_create_object(env, &o);
+    set(env, o, "id", num(env, double(r[i].id)));
+    set(env, o, "status", str(env, r[i].status >= 0 && r[i].status < 4 ? names[r[i].status] : "unknown"));
+    set(env, o, "iterations", i32(env, r[i].iterations)); set(env, o, "residual", num(env, r[i].residual));
+    set(env, o, "tSubmitMs", num(env, r[i].t_submit_ms)); set(env, o, "tAdmitMs", num(env, r[i].t_admit_ms));
+    set(env, o, "tDoneMs", num(env, r[i].t_done_ms)); set(env, o, "n", i32(env, r[i].n)); set(env, o, "K", i32(env, r[i].K));
+    napi_set_element(env, arr, uint32_t(i), o);
+  }
+  return arr;
+}
+
+napi_value ServerCancel(napi_env env, napi_callback_info info) {
+  napi_value a[2]; get_args(env, info, 2, a);
+  Server* s = ext<Server>(env, a[0]);
+  double id = 0; napi_get_value_double(env, a[1], &id);
+  if (!s || !s->s) { napi_throw_type_error(env, nullptr, "cancel: no server"); return nullptr; }
+  int rc = ws_cancel(s->s, int64_t(id));
+  if (rc) { napi_throw_error(env, nullptr, ws_last_error(s->ctx)); return nullptr; }
+  return undef(env);
+}
+
+napi_value ServerClock(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  Server* s = ext<Server>(env, a[0]);
+  return num(env, s && s->s ? ws_server_clock_ms(s->s) : 0.0);
+}
+
+napi_value Init(napi_env env, napi_value exports) {
+  struct Fn { const char* name; napi_callback cb; };
+  static const Fn fns[] = {
+      {"deviceCreate", DeviceCreate}, {"deviceInfo", DeviceInfo},
+      {"modelHexDome", ModelHexDome}, {"modelClone", ModelClone}, {"modelInfo", ModelInfo}, {"modelState", ModelState},
+      {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy}, {"batchOpts", BatchOpts}, {"batchStep", BatchStep},
+      {"batchReadControl", BatchReadControl}, {"batchReadHistory", BatchReadHistory}, {"batchDownload", BatchDownload},
+      {"batchUpload", BatchUpload}, {"batchUpdateParams", BatchUpdateParams}, {"batchInternalize", BatchInternalize},
+      {"serverCreate", ServerCreate}, {"serverDestroy", ServerDestroy}, {"serverSubmit", ServerSubmit},
+      {"serverPump", ServerPump}, {"serverPoll", ServerPoll}, {"serverCancel", ServerCancel}, {"serverClockMs", ServerClock},
+  };
+  for (const Fn& f : fns) {
+    napi_value v;
+    NAPI_CALL(env, napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &v));
+    NAPI_CALL(env, napi_set_named_property(env, exports, f.name, v));
+  }
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)

@@ -1,0 +1,169 @@
+'use strict';
+// MI355X GpuSolver for Node.js: the interface of the reference's WebGPU solver (web/js/solver_gpu.js:17-523) over
+// the C ABI (include/weave.h) through the N-API addon (weave_napi.node).  Plain CommonJS for Node 12+.
+//
+//   const W = require('./solver_mi355x.js');
+//   const device = await W.getDevice();                 // null without a HIP device (solver_gpu.js:17-29)
+//   const m = W.Model.hexDome({ span: 3000, spacing: 300, segment: 50, contraction: 0.15, noise: 1, seed: 1 },
+//                             { E: 10000, G: 600, density: 500, width: 40, height: 4, mu: 0.3, tol: 1e-6 });
+//   const g = new W.GpuSolver(device, [m], { perSubmit: 64 });
+//   const ctl = await g.step(600);                      // [{ iter, done, rollback, ke, resid, ... }]
+//   await g.download(0);                                // m.st.X, m.st.Q, ... overwritten in place
+//   g.dispose();
+//
+// Device calls that wait for the GPU return Promises; calls on one solver are serialised (one host thread per
+// batch handle), as the reference's app serialises them (app.js:164-166).
+
+const path = require('path');
+
+let addon = null;
+function native() {
+  if (!addon) addon = require(path.join(__dirname, '..', 'lib', 'weave_napi.node'));
+  return addon;
+}
+
+const FP32_FP64 = 0;
+const FP64 = 1;
+
+/** getDevice() (solver_gpu.js:17-29): the HIP device, or null when there is none */
+async function getDevice(index = 0) {
+  const ctx = native().deviceCreate(index);
+  if (!ctx) return null;
+  return { ctx, info: native().deviceInfo(ctx) };
+}
+
+/** A host model: one gridshell built by the C++ model builder; `st` holds typed arrays over its state (the arrays a
+ *  download overwrites, named as the reference's CpuSolver state: X, X0, V, Q, Q0, Wr, u, nPrev, slip, stuck, lam) */
+class Model {
+  constructor(handle) {
+    this.h = handle;
+    this.info = native().modelInfo(handle);
+    this.st = native().modelState(handle);
+    this.st.history = { kinetic: [], strain: [], bending: [], torsion: [], stretch: [], shear: [], contact: [], residual: [] };
+  }
+  static hexDome(spec, scene = {}) { return new Model(native().modelHexDome(spec, scene)); }
+  clone() { return new Model(native().modelClone(this.h)); }
+  get n() { return this.info.n; }
+  get m() { return this.info.m; }
+  get K() { return this.info.K; }
+  get span() { return this.info.span; }
+}
+
+class GpuSolver {
+  /** new GpuSolver(device, models, { perSubmit, tile, precision }) (solver_gpu.js:65-123).  Instances may differ in
+   *  topology; precision 'fp32' (FP32 forces, FP64 positions / frames / energy / residual) or 'fp64' */
+  constructor(device, models, opts = {}) {
+    if (!device || !device.ctx) throw new Error('GpuSolver: no device');
+    this.dev = device;
+    this.models = models;
+    this.B = models.length;
+    const precision = opts.precision === 'fp64' ? FP64 : FP32_FP64;
+    this.h = native().batchCreate(device.ctx, models.map((m) => m.h),
+      { precision, perSubmit: opts.perSubmit || 0, tile: opts.tile === 'autotune' ? -1 : (opts.tile || 0), history: opts.history || 0 });
+    this.perSubmit = native().batchOpts(this.h).perSubmit;
+    this.ctlHost = new Array(this.B).fill(null);
+    this._chain = Promise.resolve();
+    this.sizes = models.map((m) => ({ n: m.n, m: m.m, K: m.K }));
+    // fixed-point scales of the reference (solver_gpu.js:132-159) have no use here: positions and frames are FP64
+    this.PB = 0; this.FB = 0; this.TB = 0; this.KB = 0;
+  }
+
+  /** fixed-point scales (solver_gpu.js:132): none on this device */
+  static computeScales() { return { PB: 0, FB: 0, TB: 0, KB: 0 }; }
+
+  /** compileInfo() (solver_gpu.js:126): the kernels are compiled ahead of time */
+  async compileInfo() { return []; }
+
+  _serial(fn) {
+    const p = this._chain.then(fn);
+    this._chain = p.catch(() => {});
+    return p;
+  }
+
+  /** upload(b) (solver_gpu.js:191): the model's state and goals to the device */
+  upload(b) { native().batchUpload(this.h, b, this.models[b].h); }
+
+  /** updateParams(b) (solver_gpu.js:323-333): new goals and parameters; throws if the rotation clamps changed */
+  updateParams(b) { native().batchUpdateParams(this.h, b, this.models[b].h); }
+  _uploadParams(b) { this.updateParams(b); }
+
+  /** step(nIter, { instances, force }) (solver_gpu.js:338-361) → control records */
+  step(nIter, opts = {}) {
+    const inst = opts.instances || null;
+    const force = opts.force !== false;
+    return this._serial(() => native().batchStep(this.h, nIter, inst, force).then((c) => { this.ctlHost = c; return c; }));
+  }
+
+  /** readControl() (solver_gpu.js:374-386) */
+  readControl() {
+    return this._serial(() => native().batchReadControl(this.h).then((c) => { this.ctlHost = c; return c; }));
+  }
+
+  /** readHistory(b) (solver_gpu.js:389-405): the rows written since the last call, appended to model.st.history */
+  readHistory(b) {
+    return this._serial(() => native().batchReadHistory(this.h, b, 4096).then((rows) => {
+      const H = this.models[b].st.history;
+      const n = rows.length / 9;
+      for (let k = 0; k < n; k++) {
+        const o = 9 * k;
+        H.kinetic.push(rows[o]); H.strain.push(rows[o + 1]); H.bending.push(rows[o + 2]); H.torsion.push(rows[o + 3]);
+        H.stretch.push(rows[o + 4]); H.shear.push(rows[o + 5]); H.contact.push(rows[o + 6]); H.residual.push(rows[o + 7]);
+      }
+      return n;
+    }));
+  }
+
+  /** download(b) (solver_gpu.js:431-503): the device state of instance b into its model's arrays, in place */
+  download(b) {
+    const m = this.models[b];
+    return this._serial(() => native().batchDownload(this.h, b, m.h).then((s) => {
+      Object.assign(m.st, { iteration: s.iteration, residual: s.residual });
+      m.converged = !!s.converged; m.kePrev = s.kePrev;
+      m.residualParts = { force: s.resForce, torque: s.resTorque, fref: s.fref };
+      m.energies = { stretch: s.eStretch, shear: s.eShear, bending: s.eBend, torsion: s.eTwist, contact: s.eContact };
+      return m;
+    }));
+  }
+
+  /** downloadAll(instances) (solver_gpu.js:418-428) */
+  async downloadAll(instances = null) {
+    const out = [];
+    for (const b of instances || [...Array(this.B).keys()]) out.push(await this.download(b));
+    return out;
+  }
+
+  /** internalize(b) (solver_gpu.js:507-520): the current shape becomes the displacement origin, slip lengths reset */
+  internalize(b) { native().batchInternalize(this.h, b); }
+
+  /** readOutputs / readAllOutputs (solver_gpu.js:408-416) are the UTM protocol replays' outputs, not on this path */
+  async readOutputs() { throw new Error('protocol replays are not part of the MI355X solver'); }
+  async readAllOutputs() { throw new Error('protocol replays are not part of the MI355X solver'); }
+
+  /** dispose() (solver_gpu.js:522) */
+  dispose() { if (this.h) { native().batchDestroy(this.h); this.h = null; } }
+}
+
+/** Continuous-batching request server (include/weave.h, serving): submit solves as they arrive, pump, poll */
+class Server {
+  constructor(device, opts = {}) {
+    this.h = native().serverCreate(device.ctx, {
+      precision: opts.precision === 'fp64' ? FP64 : FP32_FP64, chunk: opts.chunk || 0, tile: opts.tile || 0,
+      maxSlots: opts.maxSlots || 0, hbmBudget: opts.hbmBudget || 0,
+    });
+    this.keep = new Map();
+  }
+  submit(model, maxIter = 0) {
+    const id = native().serverSubmit(this.h, model.h, maxIter);
+    this.keep.set(id, model);
+    return id;
+  }
+  cancel(id) { native().serverCancel(this.h, id); }
+  pump() { return native().serverPump(this.h); }
+  poll() {
+    return native().serverPoll(this.h).map((r) => { r.model = this.keep.get(r.id); this.keep.delete(r.id); return r; });
+  }
+  clockMs() { return native().serverClockMs(this.h); }
+  dispose() { if (this.h) { native().serverDestroy(this.h); this.h = null; } }
+}
+
+module.exports = { getDevice, Model, GpuSolver, Server, FP32_FP64, FP64 };
