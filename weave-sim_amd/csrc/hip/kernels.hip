@@ -60,6 +60,47 @@ __device__ __forceinline__ double ffmod_(double a, double b) { return fmod(a, b)
 
 template <typename R> __device__ __forceinline__ R jsSign(R v) { return v > R(0) ? R(1) : (v < R(0) ? R(-1) : v); }
 
+
+// ------------------------------------------------------------------------------------------------ fast arithmetic
+// Reciprocals: FP32 uses the hardware reciprocal (1 ulp; the FP32 form's forces carry 1e-7 anyway); FP64 refines
+// the hardware estimate with two Newton steps to full double precision.
+__device__ __forceinline__ float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0); r = fma(r, e, r);
+  e = fma(-x, r, 1.0); return fma(r, e, r);
+}
+// 1/√x in f64: hardware estimate and two Newton steps
+__device__ __forceinline__ double rsqrt_d(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  return y * fma(-h * y, y, 1.5);
+}
+
+// Wavefront reductions with DPP row shifts and row broadcasts (no LDS traffic).  Inclusive scan within each row of
+// 16 lanes (row_shr 1, 2, 4, 8), then row 0 → row 1 and row 2 → row 3 (row_bcast:15), rows 0–1 → rows 2–3
+// (row_bcast:31): lane 63 holds the wavefront's result.  Lanes whose source lies outside the row read 0, the neutral
+// value of both the sums and the maxima taken here (all maxima are of non-negative quantities).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROWS, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROWS, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
+template <bool SUM>
+__device__ __forceinline__ double op_(double a, double b) { return SUM ? a + b : fmax(a, b); }
+template <bool SUM>
+__device__ __forceinline__ double wave_reduce63(double x) {
+  x = op_<SUM>(x, dpp_<0x111, 0xF>(x));
+  x = op_<SUM>(x, dpp_<0x112, 0xF>(x));
+  x = op_<SUM>(x, dpp_<0x114, 0xF>(x));
+  x = op_<SUM>(x, dpp_<0x118, 0xF>(x));
+  x = op_<SUM>(x, dpp_<0x142, 0xA>(x));
+  x = op_<SUM>(x, dpp_<0x143, 0xC>(x));
+  return x;
+}
+
 // ------------------------------------------------------------------------------------------------ quaternions
 // frame of a quaternion, normalised first (quat.js:122-130); columns X | Y | Z
 template <typename R>
@@ -129,6 +170,13 @@ __device__ __forceinline__ void rotT(const double* d, const R* v, R* out) {
 
 __device__ __forceinline__ bool inst_live(const Ctl* c) { return c->done == 0 && c->iter < c->stop_at; }
 
+// An array pointer read from an instance descriptor, in the global address space: its loads and stores become
+// global_* instead of flat_* (a flat access also counts against the LDS wait counter, so every LDS wait would wait
+// for it too).  The descriptor's pointers all come from hipMalloc.
+template <typename T> __device__ __forceinline__ __attribute__((address_space(1))) T* gl(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 // ================================================================================================= k_begin
 // step(nIter, {instances, force}) (solver_gpu.js:338-345): stop_at = iter + nIter, clear the convergence flag
 template <typename Real>
@@ -141,13 +189,13 @@ __global__ void k_begin(Ctl* ctls, const int32_t* sel, const int32_t* counts, in
 }
 
 // field-wise copies of a side record (keeps the record in registers: no aggregate copy through private memory)
-template <typename Real>
-__device__ __forceinline__ void side_load(const SideRec<Real>* __restrict__ p, SideRec<Real>& r) {
+template <typename P, typename Real>
+__device__ __forceinline__ void side_load(P p, SideRec<Real>& r) {
   r.seg_pre = p->seg_pre; r.seg_post = p->seg_post; r.t = p->t; r.kth = p->kth;
   r.F[0] = p->F[0]; r.F[1] = p->F[1]; r.F[2] = p->F[2]; r.T[0] = p->T[0]; r.T[1] = p->T[1]; r.T[2] = p->T[2];
 }
-template <typename Real>
-__device__ __forceinline__ void side_store(const SideRec<Real>& r, SideRec<Real>* p) {
+template <typename P, typename Real>
+__device__ __forceinline__ void side_store(const SideRec<Real>& r, P p) {
   p->seg_pre = r.seg_pre; p->seg_post = r.seg_post; p->t = r.t; p->kth = r.kth;
   p->F[0] = r.F[0]; p->F[1] = r.F[1]; p->F[2] = r.F[2]; p->T[0] = r.T[0]; p->T[1] = r.T[1]; p->T[2] = r.T[2];
 }
@@ -168,12 +216,12 @@ struct SideEv {
 template <typename Real, typename G>
 __device__ __forceinline__ void eval_side(const Inst<Real>& I, int cur, int c, int side, double uu, double ox, double oy, double oz,
                                           SideEv<G>& E) {
-  const int ns = I.k_nseg[2 * c + side];
-  const int noff = I.k_noff[2 * c + side];
-  const int nnr = I.k_nn[2 * c + side];
+  const int ns = gl(I.k_nseg)[2 * c + side];
+  const int noff = gl(I.k_noff)[2 * c + side];
+  const int nnr = gl(I.k_nn)[2 * c + side];
   const bool closed = (nnr >> 30) & 1;
   const int nn = nnr & 0x3fffffff;
-  const int soff = I.k_seg_off[2 * c + side];
+  const int soff = gl(I.k_seg_off)[2 * c + side];
   int i = int(fmin(fmax(floor(uu), 0.0), double(max(ns - 1, 0))));
   double td = fmin(fmax(uu - i, 0.0), 1.0);
   G t = G(td);
@@ -188,13 +236,13 @@ __device__ __forceinline__ void eval_side(const Inst<Real>& I, int cur, int c, i
   const int ibp = closed ? (ib + 1) % nn : ib + 1;
   const int sm = closed ? (i - 1 + ns) % ns : i - 1;
   const int sp = closed ? (i + 1) % ns : i + 1;
-  const double* X = I.x[cur]; const double* Y = I.y[cur]; const double* Z = I.z[cur];
+  const auto X = gl(I.x[cur]); const auto Y = gl(I.y[cur]); const auto Z = gl(I.z[cur]);
   const double xa = X[E.a], ya = Y[E.a], za = Z[E.a];
   const double xb = X[E.b], yb = Y[E.b], zb = Z[E.b];
   double xm = 0, ym = 0, zm = 0, xp = 0, yp = 0, zp = 0;
   if (hasPrevSeg) { xm = X[noff + iam]; ym = Y[noff + iam]; zm = Z[noff + iam]; }
   if (hasNextSeg) { xp = X[noff + ibp]; yp = Y[noff + ibp]; zp = Z[noff + ibp]; }
-  const double* QW = I.qw[cur]; const double* QX = I.qx[cur]; const double* QY = I.qy[cur]; const double* QZ = I.qz[cur];
+  const auto QW = gl(I.qw[cur]); const auto QX = gl(I.qx[cur]); const auto QY = gl(I.qy[cur]); const auto QZ = gl(I.qz[cur]);
   G q0[4] = {G(QW[E.seg]), G(QX[E.seg]), G(QY[E.seg]), G(QZ[E.seg])};
   G qm[4] = {0, 0, 0, 0}, qp[4] = {0, 0, 0, 0};
   if (hasPrevSeg) { int j = soff + sm; qm[0] = G(QW[j]); qm[1] = G(QX[j]); qm[2] = G(QY[j]); qm[3] = G(QZ[j]); }
@@ -286,13 +334,13 @@ __device__ __forceinline__ H walls(int sideCut, const H* R, const H* n, const H*
 // the side's current parameter u (freeWeight, solver_cpu.js:483-487)
 template <typename Real>
 __device__ __forceinline__ double side_free_weight(const Inst<Real>& I, int q) {
-  const int ns = I.k_nseg[q], nnr = I.k_nn[q], nn = nnr & 0x3fffffff;
+  const int ns = gl(I.k_nseg)[q], nnr = gl(I.k_nn)[q], nn = nnr & 0x3fffffff;
   const bool closed = (nnr >> 30) & 1;
-  const double uu = I.u[q];
+  const double uu = gl(I.u)[q];
   const int i = int(fmin(fmax(floor(uu), 0.0), double(max(ns - 1, 0))));
   const double t = fmin(fmax(uu - i, 0.0), 1.0);
-  const int a = I.k_noff[q] + i, b = I.k_noff[q] + (closed ? (i + 1) % nn : i + 1);
-  const bool fa = (I.fix[a] & 16) != 0, fb = (I.fix[b] & 16) != 0;
+  const int a = gl(I.k_noff)[q] + i, b = gl(I.k_noff)[q] + (closed ? (i + 1) % nn : i + 1);
+  const bool fa = (gl(I.fix)[a] & 16) != 0, fb = (gl(I.fix)[b] & 16) != 0;
   return (fa ? 0.0 : (1 - t) * (1 - t)) + (fb ? 0.0 : t * t);
 }
 
@@ -301,10 +349,10 @@ __device__ __forceinline__ double side_free_weight(const Inst<Real>& I, int q) {
 // ramp is complete whenever the multipliers update (updates start after the ramp).
 template <typename Real>
 __device__ __forceinline__ double al_updated(const Inst<Real>& I, int c, double lam, double penacc, int pen_n) {
-  const bool enabled = I.k_en[c] != 0, locked = I.k_lock[c] != 0;
+  const bool enabled = gl(I.k_en)[c] != 0, locked = gl(I.k_lock)[c] != 0;
   const bool pinned = side_free_weight(I, 2 * c) < AL_FREE_MIN || side_free_weight(I, 2 * c + 1) < AL_FREE_MIN;
-  const double cap = fmin(double(I.fc90[2 * c]), double(I.fc90[2 * c + 1])) * double(I.wk[2 * c]) * double(I.wk[2 * c + 1]) /
-                     fmax(sin(double(I.theta[c])), 1e-6);
+  const double cap = fmin(double(gl(I.fc90)[2 * c]), double(gl(I.fc90)[2 * c + 1])) * double(gl(I.wk)[2 * c]) * double(gl(I.wk)[2 * c + 1]) /
+                     fmax(sin(double(gl(I.theta)[c])), 1e-6);
   double v = lam + I.strength * (penacc / double(max(pen_n, 1)));
   if (!locked) v = fmax(v, 0.0);
   v = fmin(fmax(v, -cap), cap);
@@ -346,15 +394,15 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
   const int rampN = max(I.ramp_iter, 0);
   const double rampD = rampN == 0 ? 1.0 : fmin(1.0, 0.01 + 0.99 * iter / rampN);
   const G kn = G(rampD) * G(I.strength), kt = G(rampD) * G(I.stick);
-  const bool enabled = I.k_en[c] != 0;
-  const bool locked = I.k_lock[c] != 0;
+  const bool enabled = gl(I.k_en)[c] != 0;
+  const bool locked = gl(I.k_lock)[c] != 0;
   const bool seat = I.seat != 0, aug = I.aug != 0;
   const G mu0 = G(I.mu);
   // ---- own side, relative to the first node of A's segment
-  const double uA = I.u[2 * c];
-  const double uOwn = side == 0 ? uA : I.u[2 * c + 1];
-  const int o = I.k_noff[2 * c] + int(fmin(fmax(floor(uA), 0.0), double(max(I.k_nseg[2 * c] - 1, 0))));
-  const double ox = I.x[cur][o], oy = I.y[cur][o], oz = I.z[cur][o];
+  const double uA = gl(I.u)[2 * c];
+  const double uOwn = side == 0 ? uA : gl(I.u)[2 * c + 1];
+  const int o = gl(I.k_noff)[2 * c] + int(fmin(fmax(floor(uA), 0.0), double(max(gl(I.k_nseg)[2 * c] - 1, 0))));
+  const double ox = gl(I.x[cur])[o], oy = gl(I.y[cur])[o], oz = gl(I.z[cur])[o];
   SideEv<G> E;
   eval_side(I, cur, c, side, uOwn, ox, oy, oz, E);
   // ---- partner's side
@@ -368,15 +416,15 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
     tA[k] = side == 0 ? E.tg[k] : oT[k]; tB[k] = side == 0 ? oT[k] : E.tg[k];
   }
   for (int k = 0; k < 6; k++) { AR[k] = side == 0 ? E.R[k] : oR[k]; BR[k] = side == 0 ? oR[k] : E.R[k]; }
-  const G wA = I.wk[2 * c], wB = I.wk[2 * c + 1], hA = I.hk[2 * c], hB = I.hk[2 * c + 1];
+  const G wA = gl(I.wk)[2 * c], wB = gl(I.wk)[2 * c + 1], hA = gl(I.hk)[2 * c], hB = gl(I.hk)[2 * c + 1];
   // ---- AL multiplier update deferred from the end of the previous iteration (updateMultipliers, :499-506)
-  G lam = aug ? G(I.lam[c]) : G(0);
-  G penacc = aug ? G(I.penacc[c]) : G(0);
+  G lam = aug ? G(gl(I.lam)[c]) : G(0);
+  G penacc = aug ? G(gl(I.penacc)[c]) : G(0);
   if (aug && al_update) {
     lam = al_updated(I, c, lam, penacc, pen_n);
     penacc = G(0);
   }
-  G np[3] = {I.nprev[3 * c], I.nprev[3 * c + 1], I.nprev[3 * c + 2]};
+  G np[3] = {gl(I.nprev)[3 * c], gl(I.nprev)[3 * c + 1], gl(I.nprev)[3 * c + 2]};
   G n[3] = {tA[1] * tB[2] - tA[2] * tB[1], tA[2] * tB[0] - tA[0] * tB[2], tA[0] * tB[1] - tA[1] * tB[0]};
   G nn = fsqrt_(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
   if (nn < G(1e-6)) { n[0] = np[0]; n[1] = np[1]; n[2] = np[2]; }
@@ -388,7 +436,7 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
   G supA = support(AR, n, G(1), wA, hA, rA);
   G supB = support(BR, n, G(-1), wB, hB, rB);
   G D = supA + supB;
-  if (seat && I.seat_gap) D -= G(I.dent[2 * c]) + G(I.dent[2 * c + 1]) + G(I.dente[c]);
+  if (seat && I.seat_gap) D -= G(gl(I.dent)[2 * c]) + G(gl(I.dent)[2 * c + 1]) + G(gl(I.dente)[c]);
   G pen = D - g;
   if (aug && iter >= rampN) penacc += pen;
   G Fn = 0; bool act;
@@ -422,8 +470,8 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
     H dh[3] = {Str[0] / dn, Str[1] / dn, Str[2] / dn};
     bool has = norm > H(1e-12);
     for (int j = 0; j < 2; j++) {
-      tanpsi[j] = I.tanpsi[2 * c + j];
-      if (!(I.dent[2 * c + j] > Real(0) && has)) continue;
+      tanpsi[j] = gl(I.tanpsi)[2 * c + j];
+      if (!(gl(I.dent)[2 * c + j] > Real(0) && has)) continue;
       const H* tCut = j == 0 ? tBR : tAR;
       const H* Rcut = j == 0 ? BRr : ARr;
       H mm[3] = {nR[1] * tCut[2] - nR[2] * tCut[1], nR[2] * tCut[0] - nR[0] * tCut[2], nR[0] * tCut[1] - nR[1] * tCut[0]};
@@ -449,7 +497,7 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
   G da = clampv(ca * sc, -eps, eps), db = clampv(cb * sc, -eps, eps);
   bool moved = slip || fre;
   // own side's parameter after the slip
-  const int nsOwn = I.k_nseg[2 * c + side];
+  const int nsOwn = gl(I.k_nseg)[2 * c + side];
   double uNew = uOwn;
   if (moved) {
     double du = side == 0 ? double(da) / double(dmax(E.L, G(1e-12))) : -double(db) / double(dmax(E.L, G(1e-12)));
@@ -480,24 +528,24 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
   int faceSide = nEdge == 1 ? (edge[1] ? 0 : 1) : -1;
   H Fa = H(fabs(Fn));
   H Aa = cface[0] * cface[1] / sinT, pres = Fa / dmax(Aa, H(1e-12));
-  H ut = pres / dmax(H(dmin(I.Hs[2 * c], I.Hs[2 * c + 1])), H(1e-12));
+  H ut = pres / dmax(H(dmin(gl(I.Hs)[2 * c], gl(I.Hs)[2 * c + 1])), H(1e-12));
   if (nEdge == 1) {
     int fs = faceSide, es = 1 - fs;
-    H ell = cface[fs] / sinT, kap = H(I.E90[2 * c + fs]) / dmax(z0[fs], H(1e-12));
+    H ell = cface[fs] / sinT, kap = H(gl(I.E90)[2 * c + fs]) / dmax(z0[fs], H(1e-12));
     H phiE = dmax(phiv[es], H(1e-4));
     H bw = fsqrt_(H(2) * Fa / dmax(ell * kap * phiE, H(1e-12)));
     bool full = bw >= cface[es];
     H Afe = full ? cface[0] * cface[1] / sinT : bw * ell;
     H pfe = full ? Fa / dmax(Afe, H(1e-12)) : kap * phiE * bw;
-    Aa = Afe; pres = pfe; ut = pfe / dmax(H(I.fc90[2 * c + es]), H(1e-12));
+    Aa = Afe; pres = pfe; ut = pfe / dmax(H(gl(I.fc90)[2 * c + es]), H(1e-12));
   } else if (nEdge == 2) {
-    H fc = H(dmin(I.fc90[2 * c], I.fc90[2 * c + 1]));
+    H fc = H(dmin(gl(I.fc90)[2 * c], gl(I.fc90)[2 * c + 1]));
     Aa = Fa / dmax(fc, H(1e-12)); pres = Fa > H(0) ? fc : H(0); ut = Fa > H(0) ? H(1) : H(0);
   }
   // ---- W3 seat update (_seatUpdate, :317-344)
   H dent[2] = {H(0), H(0)}, climb[2] = {H(0), H(0)}, dente = H(0);
   if (seat) {
-    dent[0] = I.dent[2 * c]; dent[1] = I.dent[2 * c + 1]; climb[0] = I.climb[2 * c]; climb[1] = I.climb[2 * c + 1]; dente = I.dente[c];
+    dent[0] = gl(I.dent)[2 * c]; dent[1] = gl(I.dent)[2 * c + 1]; climb[0] = gl(I.climb)[2 * c]; climb[1] = gl(I.climb)[2 * c + 1]; dente = gl(I.dente)[c];
     H dar = H(da), dbr = H(db);
     H inc[3] = {dar * tAR[0] - dbr * tBR[0], dar * tAR[1] - dbr * tBR[1], dar * tAR[2] - dbr * tBR[2]};
     for (int j = 0; j < 2; j++) {
@@ -514,14 +562,14 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
       H sp0 = fsin_(phiv[0]) * fcos_(phiv[0]), sp1 = fsin_(phiv[1]) * fcos_(phiv[1]);
       H sT = fsin_(theta);
       if (nEdge == 2 && act && !locked) {
-        H fc = H(dmin(I.fc90[2 * c], I.fc90[2 * c + 1]));
+        H fc = H(dmin(gl(I.fc90)[2 * c], gl(I.fc90)[2 * c + 1]));
         H dEst = fsqrt_(Fa / dmax(fc, H(1e-12)) * sT * sp0 * sp1);
         for (int j = 0; j < 2; j++) { H cu = dent[j]; dent[j] = dmax(cu, dmin(cu + H(I.dent_rate) * dEst, dEst)); }
       }
       if (nEdge == 1 && act && !locked) {
         int fs = faceSide, es = 1 - fs;
         H ell = cface[fs] / dmax(sT, H(1e-6));
-        H dE = Fa / dmax(H(I.fc90[2 * c + es]) * ell, H(1e-12)) * (es == 0 ? sp0 : sp1);
+        H dE = Fa / dmax(H(gl(I.fc90)[2 * c + es]) * ell, H(1e-12)) * (es == 0 ? sp0 : sp1);
         dente = dmax(dente, dE);
       }
     }
@@ -537,8 +585,8 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
   if (I.rot_fric) {
     G crd[3] = {tA[1] * tB[2] - tA[2] * tB[1], tA[2] * tB[0] - tA[0] * tB[2], tA[0] * tB[1] - tA[1] * tB[0]};
     G thc = atan2(crd[0] * n[0] + crd[1] * n[1] + crd[2] * n[2], tab);
-    th0 = I.theta0[c];
-    rotslip = I.rotslip[c];
+    th0 = gl(I.theta0)[c];
+    rotslip = gl(I.rotslip)[c];
     if (isnan(th0) && act) th0 = thc;
     bool okr = act && !isnan(th0);
     G sT2 = fmax(sin(G(theta)), G(1e-6));
@@ -562,10 +610,10 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
   }
   // ---- righting stiffness (addTwistStiffness, :170-176) on the own side's segment after the slip
   G tfk = dmax(G(I.torsion_factor), G(1e-3));
-  G kth = tfk * ((Fn + kn * G(0.25) * G(I.kh[c])) * (G(0.5) * G(I.kw[c])) / G(SUPPORT_SMOOTH) + kt * (G(0.5) * G(I.kh[c])) * (G(0.5) * G(I.kh[c])));
+  G kth = tfk * ((Fn + kn * G(0.25) * G(gl(I.kh)[c])) * (G(0.5) * G(gl(I.kw)[c])) / G(SUPPORT_SMOOTH) + kt * (G(0.5) * G(gl(I.kh)[c])) * (G(0.5) * G(gl(I.kh)[c])));
   if (valid) {
     // own side: parameter, record
-    if (moved) I.u[2 * c + side] = uNew;
+    if (moved) gl(I.u)[2 * c + side] = uNew;
     SideRec<Real> rec;
     rec.seg_pre = E.loc;
     rec.seg_post = int(fmin(fmax(floor(uNew), 0.0), double(max(nsOwn - 1, 0))));
@@ -574,44 +622,40 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
       rec.F[k] = Real(side == 0 ? FA[k] : FB[k]);
       rec.T[k] = Real(side == 0 ? TA[k] : TB[k]);
     }
-    side_store(rec, I.side + I.side_pos[2 * c + side]);
+    side_store(rec, gl(I.side) + gl(I.side_pos)[2 * c + side]);
   }
   if (lead) {
-    I.nprev[3 * c] = Real(n[0]); I.nprev[3 * c + 1] = Real(n[1]); I.nprev[3 * c + 2] = Real(n[2]);
+    gl(I.nprev)[3 * c] = Real(n[0]); gl(I.nprev)[3 * c + 1] = Real(n[1]); gl(I.nprev)[3 * c + 2] = Real(n[2]);
     if (slip) {
       G q0 = da * tA[0] + db * tB[0], q1 = da * tA[1] + db * tB[1], q2 = da * tA[2] + db * tB[2];
-      I.slip[c] += double(fsqrt_(q0 * q0 + q1 * q1 + q2 * q2));
+      gl(I.slip)[c] += double(fsqrt_(q0 * q0 + q1 * q1 + q2 * q2));
     }
-    if (aug) { I.lam[c] = lam; I.penacc[c] = penacc; }
-    I.fn[c] = Real(Fn);
-    I.ft[c] = Real(fsqrt_(fT[0] * fT[0] + fT[1] * fT[1] + fT[2] * fT[2]));
-    I.gap[c] = Real(g);
-    I.kflags[c] = uint8_t((slip ? 1 : 0) | (moved ? 0 : 2));
-    I.mid[3 * c] = Real(G(0.5) * (AP[0] + BP[0]) + ox);
-    I.mid[3 * c + 1] = Real(G(0.5) * (AP[1] + BP[1]) + oy);
-    I.mid[3 * c + 2] = Real(G(0.5) * (AP[2] + BP[2]) + oz);
-    I.theta[c] = theta; I.anom[c] = Aa; I.pres[c] = pres; I.util[c] = ut;
-    I.phi[2 * c] = phiv[0]; I.phi[2 * c + 1] = phiv[1];
-    I.ctype[c] = int8_t(nEdge); I.face[c] = int8_t(faceSide);
+    if (aug) { gl(I.lam)[c] = lam; gl(I.penacc)[c] = penacc; }
+    gl(I.fn)[c] = Real(Fn);
+    gl(I.ft)[c] = Real(fsqrt_(fT[0] * fT[0] + fT[1] * fT[1] + fT[2] * fT[2]));
+    gl(I.gap)[c] = Real(g);
+    gl(I.kflags)[c] = uint8_t((slip ? 1 : 0) | (moved ? 0 : 2));
+    gl(I.mid)[3 * c] = Real(G(0.5) * (AP[0] + BP[0]) + ox);
+    gl(I.mid)[3 * c + 1] = Real(G(0.5) * (AP[1] + BP[1]) + oy);
+    gl(I.mid)[3 * c + 2] = Real(G(0.5) * (AP[2] + BP[2]) + oz);
+    gl(I.theta)[c] = theta; gl(I.anom)[c] = Aa; gl(I.pres)[c] = pres; gl(I.util)[c] = ut;
+    gl(I.phi)[2 * c] = phiv[0]; gl(I.phi)[2 * c + 1] = phiv[1];
+    gl(I.ctype)[c] = int8_t(nEdge); gl(I.face)[c] = int8_t(faceSide);
     if (seat) {
-      I.tanpsi[2 * c] = tanpsi[0]; I.tanpsi[2 * c + 1] = tanpsi[1];
-      I.dent[2 * c] = dent[0]; I.dent[2 * c + 1] = dent[1]; I.climb[2 * c] = climb[0]; I.climb[2 * c + 1] = climb[1];
-      I.dente[c] = dente;
+      gl(I.tanpsi)[2 * c] = tanpsi[0]; gl(I.tanpsi)[2 * c + 1] = tanpsi[1];
+      gl(I.dent)[2 * c] = dent[0]; gl(I.dent)[2 * c + 1] = dent[1]; gl(I.climb)[2 * c] = climb[0]; gl(I.climb)[2 * c + 1] = climb[1];
+      gl(I.dente)[c] = dente;
     }
-    if (I.rot_fric) { I.theta0[c] = th0; I.rotslip[c] = rotslip; I.mrot[c] = Mr; }
+    if (I.rot_fric) { gl(I.theta0)[c] = th0; gl(I.rotslip)[c] = rotslip; gl(I.mrot)[c] = Mr; }
     G ePen = act ? (aug ? Fn * Fn / dmax(kn, G(1e-30)) : kn * pen * pen) : G(0);
     eC = 0.5 * double(ePen) + 0.5 * double(kt) * double(Se[0] * Se[0] + Se[1] * Se[1] + Se[2] * Se[2]) + double(eRot);
     mxFn = double(Fn);
   }
-  // ---- block partial (fixed tree order)
-  __shared__ double rs[CROSS_THREADS], rm[CROSS_THREADS];
-  rs[tid] = eC; rm[tid] = mxFn;
-  __syncthreads();
-  for (int k = CROSS_THREADS / 2; k > 0; k >>= 1) {
-    if (tid < k) { rs[tid] += rs[tid + k]; rm[tid] = fmax(rm[tid], rm[tid + k]); }
-    __syncthreads();
-  }
-  if (tid == 0) { I.cpart[it.blk] = rs[0]; I.cpart[I.ncblk + it.blk] = rm[0]; }   // [value][block]
+  // ---- block partial (one wavefront; fixed DPP order)
+  static_assert(CROSS_THREADS == 64, "the crossing partial is a single-wavefront reduction");
+  eC = wave_reduce63<true>(eC);
+  mxFn = wave_reduce63<false>(fmax(mxFn, 0.0));
+  if (tid == 63) { gl(I.cpart)[it.blk] = eC; gl(I.cpart)[I.ncblk + it.blk] = mxFn; }   // [value][block]
 }
 
 // ================================================================================================= k_strand
@@ -634,7 +678,7 @@ struct StrandLDS {
 // f64 frame [X | Y | Z] of a quaternion with one reciprocal: R = I + 2/|q|² (…) (quat.js toFrame up to rounding)
 __device__ __forceinline__ void qframe_d(double w, double x, double y, double z, double* R) {
   double n2 = w * w + x * x + y * y + z * z;
-  double s = n2 > 0 ? 2.0 / n2 : 2.0;
+  double s = n2 > 0 ? 2.0 * rcp_(n2) : 2.0;
   R[0] = 1 - s * (y * y + z * z); R[1] = s * (x * y + z * w); R[2] = s * (x * z - y * w);
   R[3] = s * (x * y - z * w); R[4] = 1 - s * (x * x + z * z); R[5] = s * (y * z + x * w);
   R[6] = s * (x * z + y * w); R[7] = s * (y * z - x * w); R[8] = 1 - s * (x * x + y * y);
@@ -660,7 +704,7 @@ __device__ __forceinline__ Real relrot_R(double aw, double ax, double ay, double
   } else {
     float xf = float(x), yf = float(y), zf = float(z), wf = float(w);
     float s = sqrtf(xf * xf + yf * yf + zf * zf), th = 2.0f * atan2f(s, wf);
-    float f = s > 1e-12f ? th / s : 2.0f / fmaxf(wf, 1e-12f);
+    float f = s > 1e-12f ? th * rcp_(s) : 2.0f * rcp_(fmaxf(wf, 1e-12f));
     psi[0] = f * xf; psi[1] = f * yf; psi[2] = f * zf;
     return th;
   }
@@ -671,7 +715,7 @@ template <typename Real>
 __device__ __forceinline__ void rotT2(const double* d, const Real* v, Real* out) {
   Real w = Real(d[0]), x = Real(d[1]), y = Real(d[2]), z = Real(d[3]);
   Real n2 = w * w + x * x + y * y + z * z;
-  Real s = n2 > Real(0) ? Real(2) / n2 : Real(2);
+  Real s = n2 > Real(0) ? Real(2) * rcp_(n2) : Real(2);
   Real R0 = 1 - s * (y * y + z * z), R1 = s * (x * y + z * w), R2 = s * (x * z - y * w);
   Real R3 = s * (x * y - z * w), R4 = 1 - s * (x * x + z * z), R5 = s * (y * z + x * w);
   Real R6 = s * (x * z + y * w), R7 = s * (y * z - x * w), R8 = 1 - s * (x * x + y * y);
@@ -689,7 +733,7 @@ __device__ __forceinline__ void seg_force(StrandLDS<Real, TILE>& L, int i, doubl
   qframe_d(L.Qw[i], L.Qx[i], L.Qy[i], L.Qz[i], R);
   double ex = L.Xx[i + 1] - L.Xx[i], ey = L.Xy[i + 1] - L.Xy[i], ez = L.Xz[i + 1] - L.Xz[i];
   double b0 = R[0] * ex + R[1] * ey + R[2] * ez, b1 = R[3] * ex + R[4] * ey + R[5] * ez, b2 = R[6] * ex + R[7] * ey + R[8] * ez;
-  double il = 1.0 / l0;
+  double il = rcp_(l0);
   double s1 = b0 * il, s2 = b1 * il, s3 = b2 * il - 1.0;
   Real n1 = GA * Real(s1), n2 = GA * Real(s2), n3 = EA * Real(s3);
   L.SFx[i] = n1 * Real(R[0]) + n2 * Real(R[3]) + n3 * Real(R[6]);
@@ -707,7 +751,7 @@ __device__ __forceinline__ void joint_torque(StrandLDS<Real, TILE>& L, int i, Re
                                              Real* mm, Real* d, Real* Jo) {
   Real ps[3]; double dq[4];
   Real th = relrot_R<Real>(L.Qw[i], L.Qx[i], L.Qy[i], L.Qz[i], L.Qw[i + 1], L.Qx[i + 1], L.Qy[i + 1], L.Qz[i + 1], ps, dq);
-  Real il = Real(1) / lb;
+  Real il = rcp_(lb);
   d[0] = ps[0] * il - om[0]; d[1] = ps[1] * il - om[1]; d[2] = ps[2] * il - om[2];
   mm[0] = Bx * d[0]; mm[1] = By * d[1]; mm[2] = Bz * d[2];
   Real q0 = ps[0] * ps[0], q1 = ps[1] * ps[1], q2 = ps[2] * ps[2], f = Real(0.25) * il;
@@ -744,10 +788,10 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   // ---------------------------------------------------------------- issue every independent load first
   for (int h = tid; h < cnt + 2; h += TILE) {
     int kk;
-    if (node_ok(k0 - 1 + h, kk)) { int i = noff + kk; L.Xx[h] = I.x[cur][i]; L.Xy[h] = I.y[cur][i]; L.Xz[h] = I.z[cur][i]; }
+    if (node_ok(k0 - 1 + h, kk)) { int i = noff + kk; L.Xx[h] = gl(I.x[cur])[i]; L.Xy[h] = gl(I.y[cur])[i]; L.Xz[h] = gl(I.z[cur])[i]; }
     else { L.Xx[h] = L.Xy[h] = L.Xz[h] = 0.0; }
     int jj;
-    if (seg_ok(k0 - 1 + h, jj)) { int j = soff + jj; L.Qw[h] = I.qw[cur][j]; L.Qx[h] = I.qx[cur][j]; L.Qy[h] = I.qy[cur][j]; L.Qz[h] = I.qz[cur][j]; }
+    if (seg_ok(k0 - 1 + h, jj)) { int j = soff + jj; L.Qw[h] = gl(I.qw[cur])[j]; L.Qx[h] = gl(I.qx[cur])[j]; L.Qy[h] = gl(I.qy[cur])[j]; L.Qz[h] = gl(I.qz[cur])[j]; }
     else { L.Qw[h] = 1.0; L.Qx[h] = L.Qy[h] = L.Qz[h] = 0.0; }
   }
   const int t = tid;
@@ -763,28 +807,28 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   // node parameters and velocity
   uint8_t fx = 0; Real ms = 1, ldx = 0, ldy = 0, ldz = 0, llx = 0, lly = 0, llz = 0, vx = 0, vy = 0, vz = 0;
   if (ownNode) {
-    fx = I.fix[i]; ms = I.mass[i]; ldx = I.lx[i]; ldy = I.ly[i]; ldz = I.lz[i];
-    if (I.npl) { llx = I.llx[i]; lly = I.lly[i]; llz = I.llz[i]; }
-    if (!OUT && !rbk) { vx = I.vx[i]; vy = I.vy[i]; vz = I.vz[i]; }
+    fx = gl(I.fix)[i]; ms = gl(I.mass)[i]; ldx = gl(I.lx)[i]; ldy = gl(I.ly)[i]; ldz = gl(I.lz)[i];
+    if (I.npl) { llx = gl(I.llx)[i]; lly = gl(I.lly)[i]; llz = gl(I.llz)[i]; }
+    if (!OUT && !rbk) { vx = gl(I.vx)[i]; vy = gl(I.vy)[i]; vz = gl(I.vz)[i]; }
   }
   // segment and joint parameters of the slot
   double l0 = 1.0; Real EA = 0, GA = 0, Bx = 0, By = 0, Bz = 0, krx = 0, kry = 0, krz = 0, wx = 0, wy = 0, wz = 0;
   if (hasSeg) {
-    l0 = I.l0[j]; EA = I.EA[j]; GA = I.GA[j];
+    l0 = gl(I.l0)[j]; EA = gl(I.EA)[j]; GA = gl(I.GA)[j];
     if (ownSeg) {
-      krx = I.krx[j]; kry = I.kry[j]; krz = I.krz[j];
-      if (!OUT && !rbk) { wx = I.wx[j]; wy = I.wy[j]; wz = I.wz[j]; }
+      krx = gl(I.krx)[j]; kry = gl(I.kry)[j]; krz = gl(I.krz)[j];
+      if (!OUT && !rbk) { wx = gl(I.wx)[j]; wy = gl(I.wy)[j]; wz = gl(I.wz)[j]; }
     }
   }
   Real lb = 1, om[3] = {0, 0, 0};
   if (hasJoint) {
     const int jg = joff + jjo;
-    lb = I.jlen[jg]; om[0] = I.om0[3 * jg]; om[1] = I.om0[3 * jg + 1]; om[2] = I.om0[3 * jg + 2];
-    Bx = I.EIx[j]; By = I.EIy[j]; Bz = I.GJ[j];
+    lb = gl(I.jlen)[jg]; om[0] = gl(I.om0)[3 * jg]; om[1] = gl(I.om0)[3 * jg + 1]; om[2] = gl(I.om0)[3 * jg + 2];
+    Bx = gl(I.EIx)[j]; By = gl(I.EIy)[j]; Bz = gl(I.GJ)[j];
   }
   const int so = tl.so, ns = tl.ns;
   SideRec<Real> r0;
-  if (tid < ns) side_load(I.side + so + tid, r0);
+  if (tid < ns) side_load(gl(I.side) + so + tid, r0);
   __syncthreads();
 
   // ---------------------------------------------------------------- segments and joints
@@ -830,7 +874,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
     bool rel = false;
     SideRec<Real> r = r0;
     if (q < ns) {
-      if (base != 0) side_load(I.side + so + q, r);
+      if (base != 0) side_load(gl(I.side) + so + q, r);
       if (closed) {
         int dp = wrap(r.seg_pre - (k0 - 1), nz), dq = wrap(r.seg_post - k0, nz);
         rel = dp < cnt + 1 || dq < cnt;
@@ -915,17 +959,17 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
     if (!OUT) {
       Real V[3] = {vx, vy, vz};
       if (warm) { V[0] = V[1] = V[2] = Real(0); F[0] = F[1] = F[2] = Real(0); }
-      const Real im = Real(1) / ms;
+      const Real im = rcp_(ms);
       for (int c = 0; c < 3; c++) V[c] += F[c] * im;
       if (damping != Real(1)) for (int c = 0; c < 3; c++) V[c] *= damping;
       if (fx & 1) V[0] = Real(0);
       if (fx & 2) V[1] = Real(0);
       if (fx & 4) V[2] = Real(0);
       const int nx = cur ^ 1;
-      I.x[nx][i] = L.Xx[t] + double(V[0]);
-      I.y[nx][i] = L.Xy[t] + double(V[1]);
-      I.z[nx][i] = L.Xz[t] + double(V[2]);
-      I.vx[i] = V[0]; I.vy[i] = V[1]; I.vz[i] = V[2];
+      gl(I.x[nx])[i] = L.Xx[t] + double(V[0]);
+      gl(I.y[nx])[i] = L.Xy[t] + double(V[1]);
+      gl(I.z[nx])[i] = L.Xz[t] + double(V[2]);
+      gl(I.vx)[i] = V[0]; gl(I.vy)[i] = V[1]; gl(I.vz)[i] = V[2];
       keN = double(ms) * (double(V[0]) * double(V[0]) + double(V[1]) * double(V[1]) + double(V[2]) * double(V[2]));
     }
   }
@@ -933,14 +977,14 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   if (ownSeg) {
     Real tm[3] = {ST[0] + L.JAx[t] + L.JBx[t - 1], ST[1] + L.JAy[t] + L.JBy[t - 1], ST[2] + L.JAz[t] + L.JBz[t - 1]};
     if (I.C > 0) {
-      for (int e = I.clamp_off[j]; e < I.clamp_off[j + 1]; e++) {
-        const int cc = I.clamp_idx[e];
+      for (int e = gl(I.clamp_off)[j]; e < gl(I.clamp_off)[j + 1]; e++) {
+        const int cc = gl(I.clamp_idx)[e];
         Real ps[3]; double dq[4];
-        Real th = relrot_R<Real>(I.c_Q[4 * cc], I.c_Q[4 * cc + 1], I.c_Q[4 * cc + 2], I.c_Q[4 * cc + 3],
+        Real th = relrot_R<Real>(gl(I.c_Q)[4 * cc], gl(I.c_Q)[4 * cc + 1], gl(I.c_Q)[4 * cc + 2], gl(I.c_Q)[4 * cc + 3],
                                  L.Qw[t], L.Qx[t], L.Qy[t], L.Qz[t], ps, dq);
-        Real lc = I.c_len[cc];
+        Real lc = gl(I.c_len)[cc];
         Real d[3] = {ps[0] / lc, ps[1] / lc, ps[2] / lc};
-        Real mm[3] = {I.c_B[3 * cc] * d[0], I.c_B[3 * cc + 1] * d[1], I.c_B[3 * cc + 2] * d[2]};
+        Real mm[3] = {gl(I.c_B)[3 * cc] * d[0], gl(I.c_B)[3 * cc + 1] * d[1], gl(I.c_B)[3 * cc + 2] * d[2]};
         Real J[3], Jb[3];
         jlt(ps, th, mm, J);
         rotT2(dq, J, Jb);
@@ -963,11 +1007,11 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       Real I3[3] = {msc * dmax(krx + kTw + (L.KDx[t - 1] + L.KDx[t]), Real(1e-12)),
                     msc * dmax(kry + kTw + (L.KDy[t - 1] + L.KDy[t]), Real(1e-12)),
                     msc * dmax(krz + kTw + (L.KDz[t - 1] + L.KDz[t]), Real(1e-12))};
-      Real W[3] = {wx + tm[0] / I3[0], wy + tm[1] / I3[1], wz + tm[2] / I3[2]};
+      Real W[3] = {wx + tm[0] * rcp_(I3[0]), wy + tm[1] * rcp_(I3[1]), wz + tm[2] * rcp_(I3[2])};
       W[2] = clampv(W[2], Real(-TWIST_RATE_MAX), Real(TWIST_RATE_MAX));
       if (kTw > Real(0)) W[2] *= Real(ROLL_DAMP);
       if (damping != Real(1)) for (int c = 0; c < 3; c++) W[c] *= damping;
-      I.wx[j] = W[0]; I.wy[j] = W[1]; I.wz[j] = W[2];
+      gl(I.wx)[j] = W[0]; gl(I.wy)[j] = W[1]; gl(I.wz)[j] = W[2];
       // q ⊗ exp(ω): the increment exp(ω) − 1 = (−2 sin²(a/4), ω sin(a/2)/a) carries the precision of the small
       // rotation itself (the delta form of wgsl.js:962-980); the product and the renormalisation run in f64
       double e0, e1, e2, e3;
@@ -977,7 +1021,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
         e0 = -2.0 * sh * sh; e1 = W[0] * sc; e2 = W[1] * sc; e3 = W[2] * sc;
       } else {
         float a = sqrtf(W[0] * W[0] + W[1] * W[1] + W[2] * W[2]);
-        float sh = sinf(0.25f * a), sc = a > 1e-12f ? sinf(0.5f * a) / a : 0.5f;
+        float sh = sinf(0.25f * a), sc = a > 1e-12f ? sinf(0.5f * a) * rcp_(a) : 0.5f;
         e0 = double(-2.0f * sh * sh); e1 = double(W[0] * sc); e2 = double(W[1] * sc); e3 = double(W[2] * sc);
       }
       const double qw = L.Qw[t], qx = L.Qx[t], qy = L.Qy[t], qz = L.Qz[t];
@@ -986,9 +1030,9 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       double ry = qy + (qw * e2 - qx * e3 + qy * e0 + qz * e1);
       double rz = qz + (qw * e3 + qx * e2 - qy * e1 + qz * e0);
       double n2 = rw * rw + rx * rx + ry * ry + rz * rz;
-      double inv = n2 > 0 ? 1.0 / sqrt(n2) : 1.0;
+      double inv = n2 > 0 ? rsqrt_d(n2) : 1.0;
       const int nx = cur ^ 1;
-      I.qw[nx][j] = rw * inv; I.qx[nx][j] = rx * inv; I.qy[nx][j] = ry * inv; I.qz[nx][j] = rz * inv;
+      gl(I.qw[nx])[j] = rw * inv; gl(I.qx[nx])[j] = rx * inv; gl(I.qy[nx])[j] = ry * inv; gl(I.qz[nx])[j] = rz * inv;
       keR = double(I3[0]) * double(W[0]) * double(W[0]) + double(I3[1]) * double(W[1]) * double(W[1]) + double(I3[2]) * double(W[2]) * double(W[2]);
     }
   }
@@ -998,22 +1042,16 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   // sums keN, keR, eS, eSh, eB, eT | maxima |F|, |tm|, |load|, |N|, |M|
   double v[11] = {keN, keR, eS, eSh, eB, eT, mxF, mxT, mxL, mxN, mxM};
 #pragma unroll
-  for (int q = 0; q < 11; q++) {
-    double x = v[q];
+  for (int q = 0; q < 6; q++) v[q] = wave_reduce63<true>(v[q]);
 #pragma unroll
-    for (int o2 = 32; o2 > 0; o2 >>= 1) {
-      double y = __shfl_down(x, o2, 64);
-      x = q < 6 ? x + y : fmax(x, y);
-    }
-    v[q] = x;
-  }
+  for (int q = 6; q < 11; q++) v[q] = wave_reduce63<false>(v[q]);
   if (TILE > 64) {
-    if (lane == 0) {
+    if (lane == 63) {
 #pragma unroll
       for (int q = 0; q < 11; q++) L.wred[q][wv] = v[q];
     }
     __syncthreads();
-    if (tid == 0) {
+    if (tid == 63) {
 #pragma unroll
       for (int q = 0; q < 11; q++) {
         double x = L.wred[q][0];
@@ -1022,27 +1060,26 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       }
     }
   }
-  if (tid == 0) {   // [value][tile] so that the control kernel's loads coalesce
+  if (tid == 63) {   // [value][tile] so that the control kernel's loads coalesce
     const size_t np = size_t(I.ntiles);
-    double* sp = I.spart + size_t(tl.tile_local);
+    const auto sp = gl(I.spart) + size_t(tl.tile_local);
 #pragma unroll
     for (int q = 0; q < 11; q++) sp[q * np] = v[q];
   }
 }
 
-// fixed-order block reduction of NV doubles (the first nsum summed, the rest max-reduced); result in every thread
+// fixed-order block reduction of NV doubles (the first nsum summed, the rest max-reduced over non-negative values);
+// the result in every thread
 template <int TILE, int NV>
 __device__ __forceinline__ void block_reduce(double (*red)[TILE / 64], double* v, int nsum) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
   for (int k = 0; k < NV; k++) {
-    double x = v[k];
-    for (int off = 32; off > 0; off >>= 1) {
-      double y = __shfl_down(x, off, 64);
-      x = k < nsum ? x + y : fmax(x, y);
-    }
-    if (lane == 0) red[k][wv] = x;
+    const double x = k < nsum ? wave_reduce63<true>(v[k]) : wave_reduce63<false>(v[k]);
+    if (lane == 63) red[k][wv] = x;
   }
   __syncthreads();
+#pragma unroll
   for (int k = 0; k < NV; k++) {
     double x = red[k][0];
     for (int w = 1; w < TILE / 64; w++) x = k < nsum ? x + red[k][w] : fmax(x, red[k][w]);
@@ -1070,7 +1107,7 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   // loads are issued before the liveness test (a stopped instance's partials are read and discarded)
   double s12[13];
   for (int q = 0; q < 7; q++) s12[q] = 0.0;
-  for (int q = 7; q < 13; q++) s12[q] = q == 12 ? -INFINITY : 0.0;
+  for (int q = 7; q < 13; q++) s12[q] = 0.0;   // maxima of non-negative quantities (Fn clamped at 0: F_ref ≥ its other terms)
   const int np = I.ntiles;
   const int p0 = it.chunk * I.ctrl_chunk, p1 = min(np, p0 + I.ctrl_chunk);
   for (int w0 = p0; w0 < p1; w0 += CTRL_THREADS * CTRL_PER) {
@@ -1079,12 +1116,12 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
     for (int k = 0; k < CTRL_PER; k++) {
       const int w = w0 + k * CTRL_THREADS + tid;
       const bool in = w < p1;
-      const double* sp = I.spart + (in ? w : p0);
+      const auto sp = gl(I.spart) + (in ? w : p0);
 #pragma unroll
       for (int q = 0; q < 11; q++) x[k][q] = sp[size_t(q) * np];
       if (!in) {
 #pragma unroll
-        for (int q = 0; q < 11; q++) x[k][q] = q < 6 ? 0.0 : -INFINITY;
+        for (int q = 0; q < 11; q++) x[k][q] = 0.0;
       }
     }
 #pragma unroll
@@ -1098,15 +1135,15 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   const int cper = (I.ncblk + it.nchunks - 1) / it.nchunks;
   const int c0 = it.chunk * cper, c1 = min(I.ncblk, c0 + cper);
   for (int w = c0 + tid; w < c1; w += CTRL_THREADS) {
-    s12[6] += I.cpart[w];
-    s12[12] = fmax(s12[12], I.cpart[I.ncblk + w]);
+    s12[6] += gl(I.cpart)[w];
+    s12[12] = fmax(s12[12], gl(I.cpart)[I.ncblk + w]);
   }
   if (!(c.done == 0 && c.iter < c.stop_at)) return;   // uniform over the workgroup
   block_reduce<CTRL_THREADS, 13>(red, s12, 7);
   if (it.nchunks > 1) {
     // several workgroups: each leaves its partial; the last to arrive reduces them in chunk order and goes on
     if (tid == 0) {
-      for (int q = 0; q < 13; q++) I.rpart[size_t(it.chunk) * RPART + q] = s12[q];
+      for (int q = 0; q < 13; q++) gl(I.rpart)[size_t(it.chunk) * RPART + q] = s12[q];
       __threadfence();
       last = atomicAdd(&ctl->counter, 1) == it.nchunks - 1;
     }
@@ -1116,7 +1153,7 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
     __threadfence();
     const volatile double* rp = I.rpart;
     for (int q = 0; q < 7; q++) s12[q] = 0.0;
-    for (int q = 7; q < 13; q++) s12[q] = q == 12 ? -INFINITY : 0.0;
+    for (int q = 7; q < 13; q++) s12[q] = 0.0;
     for (int k = 0; k < it.nchunks; k++) {
       for (int q = 0; q < 7; q++) s12[q] += rp[size_t(k) * RPART + q];
       for (int q = 7; q < 13; q++) s12[q] = fmax(s12[q], rp[size_t(k) * RPART + q]);
@@ -1153,7 +1190,7 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   if (!rollback) c.cur = c.cur ^ 1;
   c.live_flag = 1;   // an iteration has been evaluated
   if (I.hist_len > 0) {
-    double* h = I.hist + size_t((c.iter - 1) % I.hist_len) * HIST_STRIDE;
+    const auto h = gl(I.hist) + size_t((c.iter - 1) % I.hist_len) * HIST_STRIDE;
     h[0] = ke; h[1] = s12[2] + s12[3] + s12[4] + s12[5] + eCont; h[2] = s12[4]; h[3] = s12[5];
     h[4] = s12[2]; h[5] = s12[3]; h[6] = eCont; h[7] = resid; h[8] = double(c.iter);
   }
@@ -1204,8 +1241,8 @@ __global__ __launch_bounds__(256) void k_al_preview(const Inst<Real>* __restrict
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= I.K) return;
   const Ctl* ctl = I.ctl;
-  const double lam = I.lam[c];
-  out[c] = (I.aug && I.contacts && ctl->al_update) ? al_updated(I, c, lam, I.penacc[c], ctl->pen_n) : lam;
+  const double lam = gl(I.lam)[c];
+  out[c] = (I.aug && I.contacts && ctl->al_update) ? al_updated(I, c, lam, gl(I.penacc)[c], ctl->pen_n) : lam;
 }
 
 __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
