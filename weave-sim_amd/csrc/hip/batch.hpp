@@ -126,6 +126,7 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   d.dent_rate = P.dent_rate; d.edge_thr = P.edge_threshold; d.l_ref = md.l_ref;
   double jmin = INFINITY; for (int j = 0; j < p; j++) jmin = std::min(jmin, md.j_len[j]);
   d.jlen_min = p ? jmin : 1.0;
+  d.inv_jlen_min = p ? 1.0 / std::max(jmin, 1e-12) : 0.0;
   d.kin_damp = P.kinetic_damping; d.warmup = P.warmup_iter; d.contacts = K > 0 ? 1 : 0; d.ramp_iter = P.ramp_iter;
   d.rot_fric = P.rot_friction; d.seat = P.seat_law; d.seat_gap = P.seat_gap; d.dent_after = P.dent_after;
   d.aug = P.augmented; d.al_every = P.al_every; d.hist_len = hist_len;

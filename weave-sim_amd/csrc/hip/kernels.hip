@@ -25,6 +25,7 @@
 namespace wsd {
 
 constexpr double SUPPORT_SMOOTH = 0.03;              // solver_cpu.js:16
+constexpr double INV_SMOOTH = 1.0 / SUPPORT_SMOOTH;
 constexpr double TWIST_RATE_MAX = 0.5 * SUPPORT_SMOOTH;
 constexpr double ROLL_DAMP = 0.5;
 constexpr double AL_FREE_MIN = 0.25;
@@ -114,12 +115,12 @@ __device__ __forceinline__ void qframe(double w, double x, double y, double z, R
 }
 template <typename R>
 __device__ __forceinline__ void qframeR(R w, R x, R y, R z, R* F) {
-  R n = fsqrt_(w * w + x * x + y * y + z * z);
-  if (n == R(0)) n = R(1);
-  w /= n; x /= n; y /= n; z /= n;
-  F[0] = 1 - 2 * (y * y + z * z); F[1] = 2 * (x * y + z * w); F[2] = 2 * (x * z - y * w);
-  F[3] = 2 * (x * y - z * w); F[4] = 1 - 2 * (x * x + z * z); F[5] = 2 * (y * z + x * w);
-  F[6] = 2 * (x * z + y * w); F[7] = 2 * (y * z - x * w); F[8] = 1 - 2 * (x * x + y * y);
+  // the frame of q/|q| as I + (2/|q|²)(…): normalisation folded into one reciprocal, no square root
+  const R n2 = w * w + x * x + y * y + z * z;
+  const R s = n2 > R(0) ? R(2) * rcp_(n2) : R(2);
+  F[0] = 1 - s * (y * y + z * z); F[1] = s * (x * y + z * w); F[2] = s * (x * z - y * w);
+  F[3] = s * (x * y - z * w); F[4] = 1 - s * (x * x + z * z); F[5] = s * (y * z + x * w);
+  F[6] = s * (x * z + y * w); F[7] = s * (y * z - x * w); F[8] = 1 - s * (x * x + y * y);
 }
 
 // Ψ = log(q_a* ⊗ q_b) in f64 (rod_cosserat.js:33-45); also returns the relative quaternion d = ±(q_a* ⊗ q_b)
@@ -249,12 +250,12 @@ __device__ __forceinline__ void eval_side(const Inst<Real>& I, int cur, int c, i
   if (hasNextSeg) { int j = soff + sp; qp[0] = G(QW[j]); qp[1] = G(QX[j]); qp[2] = G(QY[j]); qp[3] = G(QZ[j]); }
   auto unitAdd = [](G* T, double dx, double dy, double dz) {
     G ex = G(dx), ey = G(dy), ez = G(dz);
-    G l = dmax(fsqrt_(ex * ex + ey * ey + ez * ez), G(1e-12));
-    T[0] += ex / l; T[1] += ey / l; T[2] += ez / l;
+    G il = rcp_(dmax(fsqrt_(ex * ex + ey * ey + ez * ez), G(1e-12)));
+    T[0] += ex * il; T[1] += ey * il; T[2] += ez * il;
   };
   auto normT = [](G* T) {
-    G l = dmax(fsqrt_(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]), G(1e-12));
-    T[0] /= l; T[1] /= l; T[2] /= l;
+    G il = rcp_(dmax(fsqrt_(T[0] * T[0] + T[1] * T[1] + T[2] * T[2]), G(1e-12)));
+    T[0] *= il; T[1] *= il; T[2] *= il;
   };
   auto meanQ = [](G* out, const G* qs, bool hasStart, const G* qe, bool hasEnd) {
     const G* r = hasStart ? qs : qe;
@@ -267,7 +268,8 @@ __device__ __forceinline__ void eval_side(const Inst<Real>& I, int cur, int c, i
     }
     G nq = fsqrt_(out[0] * out[0] + out[1] * out[1] + out[2] * out[2] + out[3] * out[3]);
     if (nq == G(0)) nq = G(1);
-    for (int k = 0; k < 4; k++) out[k] /= nq;
+    const G inq = rcp_(nq);
+    for (int k = 0; k < 4; k++) out[k] *= inq;
   };
   G Ta[3] = {0, 0, 0}, Tb[3] = {0, 0, 0}, qa[4], qb[4];
   unitAdd(Ta, xb - xa, yb - ya, zb - za);                           // segment i starts at a
@@ -312,7 +314,7 @@ template <typename G, typename H>
 __device__ __forceinline__ G support(const G* R, const G* n, G sign, G w, G h_, H* r) {
   G nx = (n[0] * R[0] + n[1] * R[1] + n[2] * R[2]) * sign, ny = (n[0] * R[3] + n[1] * R[4] + n[2] * R[5]) * sign;
   G h = G(0.5) * w * fabs(nx) + G(0.5) * h_ * fabs(ny);
-  H ax = H(0.5) * H(w) * ftanh_(H(nx) / H(SUPPORT_SMOOTH)), ay = H(0.5) * H(h_) * ftanh_(H(ny) / H(SUPPORT_SMOOTH));
+  H ax = H(0.5) * H(w) * ftanh_(H(nx) * H(INV_SMOOTH)), ay = H(0.5) * H(h_) * ftanh_(H(ny) * H(INV_SMOOTH));
   for (int k = 0; k < 3; k++) r[k] = ax * H(R[k]) + ay * H(R[3 + k]);
   return h;
 }
@@ -428,7 +430,7 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
   G n[3] = {tA[1] * tB[2] - tA[2] * tB[1], tA[2] * tB[0] - tA[0] * tB[2], tA[0] * tB[1] - tA[1] * tB[0]};
   G nn = fsqrt_(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
   if (nn < G(1e-6)) { n[0] = np[0]; n[1] = np[1]; n[2] = np[2]; }
-  else { G d = dmax(nn, G(1e-12)); n[0] /= d; n[1] /= d; n[2] /= d; }
+  else { G id = rcp_(dmax(nn, G(1e-12))); n[0] *= id; n[1] *= id; n[2] *= id; }
   if (n[0] * np[0] + n[1] * np[1] + n[2] * np[2] < G(0)) { n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2]; }
   G S[3] = {BP[0] - AP[0], BP[1] - AP[1], BP[2] - AP[2]};
   G g = S[0] * n[0] + S[1] * n[1] + S[2] * n[2];
@@ -454,7 +456,8 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
   G sa = St[0] * tA[0] + St[1] * tA[1] + St[2] * tA[2], sb = St[0] * tB[0] + St[1] * tB[1] + St[2] * tB[2];
   G det = taa * tbb - tab * tab;
   bool ok = det > G(1e-8);
-  G ca = ok ? (sa * tbb - sb * tab) / det : G(0), cb = ok ? (sb * taa - sa * tab) / det : G(0);
+  const G idet = ok ? rcp_(det) : G(0);
+  G ca = ok ? (sa * tbb - sb * tab) * idet : G(0), cb = ok ? (sb * taa - sa * tab) * idet : G(0);
   G Ftv = fsqrt_((kt * St[0]) * (kt * St[0]) + (kt * St[1]) * (kt * St[1]) + (kt * St[2]) * (kt * St[2]));
   // Real copies of the frames and the normal for the classification and the W2 / W3 branches
   H nR[3] = {H(n[0]), H(n[1]), H(n[2])}, tAR[3] = {H(tA[0]), H(tA[1]), H(tA[2])}, tBR[3] = {H(tB[0]), H(tB[1]), H(tB[2])};
@@ -467,7 +470,8 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
     H Str[3] = {H(St[0]), H(St[1]), H(St[2])};
     H norm = fsqrt_(Str[0] * Str[0] + Str[1] * Str[1] + Str[2] * Str[2]);
     H dn = dmax(norm, H(1e-12));
-    H dh[3] = {Str[0] / dn, Str[1] / dn, Str[2] / dn};
+    const H idn = rcp_(dn);
+    H dh[3] = {Str[0] * idn, Str[1] * idn, Str[2] * idn};
     bool has = norm > H(1e-12);
     for (int j = 0; j < 2; j++) {
       tanpsi[j] = gl(I.tanpsi)[2 * c + j];
@@ -476,7 +480,8 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
       const H* Rcut = j == 0 ? BRr : ARr;
       H mm[3] = {nR[1] * tCut[2] - nR[2] * tCut[1], nR[2] * tCut[0] - nR[0] * tCut[2], nR[0] * tCut[1] - nR[1] * tCut[0]};
       H ml = dmax(fsqrt_(mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2]), H(1e-12));
-      mm[0] /= ml; mm[1] /= ml; mm[2] /= ml;
+      const H iml = rcp_(ml);
+      mm[0] *= iml; mm[1] *= iml; mm[2] *= iml;
       H sgn = j == 0 ? H(1) : H(-1);
       H dj = (dh[0] * mm[0] + dh[1] * mm[1] + dh[2] * mm[2]) * sgn;
       H wd[3] = {jsSign(dj) * mm[0] * sgn, jsSign(dj) * mm[1] * sgn, jsSign(dj) * mm[2] * sgn};
@@ -500,7 +505,8 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
   const int nsOwn = gl(I.k_nseg)[2 * c + side];
   double uNew = uOwn;
   if (moved) {
-    double du = side == 0 ? double(da) / double(dmax(E.L, G(1e-12))) : -double(db) / double(dmax(E.L, G(1e-12)));
+    const double iL = rcp_(double(dmax(E.L, G(1e-12))));
+    double du = side == 0 ? double(da) * iL : -double(db) * iL;
     uNew = fmin(fmax(uOwn + du, 0.0), double(nsOwn));
   }
   G Se[3] = {0, 0, 0};
@@ -515,7 +521,7 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
     H sg = sd == 0 ? H(1) : H(-1);
     H wv = H(sd == 0 ? wA : wB), hv = H(sd == 0 ? hA : hB);
     H nx = (nR[0] * R[0] + nR[1] * R[1] + nR[2] * R[2]) * sg, ny = (nR[0] * R[3] + nR[1] * R[4] + nR[2] * R[5]) * sg;
-    H sx = fabs(ftanh_(nx / H(SUPPORT_SMOOTH))), sy = fabs(ftanh_(ny / H(SUPPORT_SMOOTH)));
+    H sx = fabs(ftanh_(nx * H(INV_SMOOTH))), sy = fabs(ftanh_(ny * H(INV_SMOOTH)));
     edge[sd] = sx * sy > H(I.edge_thr) ? 1 : 0;
     bool yf = fabs(ny) >= fabs(nx);
     cface[sd] = yf ? wv : hv; z0[sd] = yf ? hv : wv;
@@ -552,7 +558,8 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
       const H* tCut = j == 0 ? tBR : tAR;
       H mm[3] = {nR[1] * tCut[2] - nR[2] * tCut[1], nR[2] * tCut[0] - nR[0] * tCut[2], nR[0] * tCut[1] - nR[1] * tCut[0]};
       H ml = dmax(fsqrt_(mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2]), H(1e-12));
-      mm[0] /= ml; mm[1] /= ml; mm[2] /= ml;
+      const H iml = rcp_(ml);
+      mm[0] *= iml; mm[1] *= iml; mm[2] *= iml;
       H stp = (inc[0] * mm[0] + inc[1] * mm[1] + inc[2] * mm[2]) * (j == 0 ? H(1) : H(-1));
       bool has = dent[j] > H(0);
       if (has && slip) climb[j] += stp;
@@ -671,7 +678,7 @@ struct StrandLDS {
   int32_t rpre[TILE], rpost[TILE];                                   // gathered crossing sides: segments
   SideRec<Real> rec[TILE];                                           // ... and their records
   int32_t glo[TILE], ghi[TILE];                                      // per slot: first / last gathered record it needs
-  double wred[11][TILE / 64];                                        // wavefront partials of the tile
+  double wred[SPART][TILE / 64];                                     // wavefront partials of the tile
   int wcnt[TILE / 64];
 };
 
@@ -703,8 +710,18 @@ __device__ __forceinline__ Real relrot_R(double aw, double ax, double ay, double
     return th;
   } else {
     float xf = float(x), yf = float(y), zf = float(z), wf = float(w);
-    float s = sqrtf(xf * xf + yf * yf + zf * zf), th = 2.0f * atan2f(s, wf);
-    float f = s > 1e-12f ? th * rcp_(s) : 2.0f * rcp_(fmaxf(wf, 1e-12f));
+    const float s2 = xf * xf + yf * yf + zf * zf, s = sqrtf(s2);
+    const float iw = rcp_(fmaxf(wf, 1e-12f));
+    const float r2 = s2 * iw * iw;                 // (|v| / w)²: w ≥ 0 here
+    float f, th;
+    if (r2 < 0.01f) {
+      // small relative rotation (θ < 0.2): Ψ = (2/w)·atan(r)/r · v with the series of atan(r)/r (error < r⁸/9)
+      f = 2.0f * iw * (1.0f + r2 * (-1.0f / 3 + r2 * (1.0f / 5 + r2 * (-1.0f / 7))));
+      th = f * s;
+    } else {
+      th = 2.0f * atan2f(s, wf);
+      f = th * rcp_(s);
+    }
     psi[0] = f * xf; psi[1] = f * yf; psi[2] = f * zf;
     return th;
   }
@@ -1020,9 +1037,18 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
         double sh = sin(0.25 * a), sc = a > 1e-12 ? sin(0.5 * a) / a : 0.5;
         e0 = -2.0 * sh * sh; e1 = W[0] * sc; e2 = W[1] * sc; e3 = W[2] * sc;
       } else {
-        float a = sqrtf(W[0] * W[0] + W[1] * W[1] + W[2] * W[2]);
-        float sh = sinf(0.25f * a), sc = a > 1e-12f ? sinf(0.5f * a) * rcp_(a) : 0.5f;
-        e0 = double(-2.0f * sh * sh); e1 = double(W[0] * sc); e2 = double(W[1] * sc); e3 = double(W[2] * sc);
+        const float a2 = W[0] * W[0] + W[1] * W[1] + W[2] * W[2];
+        float c0, sc;
+        if (a2 < 0.0625f) {
+          // a < 0.25 rad per iteration: cos(a/2) − 1 and sin(a/2)/a by their series (truncation below FP32 rounding)
+          c0 = a2 * (-1.0f / 8 + a2 * (1.0f / 384 + a2 * (-1.0f / 46080)));
+          sc = 0.5f + a2 * (-1.0f / 48 + a2 * (1.0f / 3840));
+        } else {
+          const float a = sqrtf(a2), sh = sinf(0.25f * a);
+          c0 = -2.0f * sh * sh;
+          sc = sinf(0.5f * a) * rcp_(a);
+        }
+        e0 = double(c0); e1 = double(W[0] * sc); e2 = double(W[1] * sc); e3 = double(W[2] * sc);
       }
       const double qw = L.Qw[t], qx = L.Qx[t], qy = L.Qy[t], qz = L.Qz[t];
       double rw = qw + (qw * e0 - qx * e1 - qy * e2 - qz * e3);
@@ -1039,23 +1065,25 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   if (OUT) return;
 
   // ---------------------------------------------------------------- partial of the tile (fixed order; reduced by k_ctrl):
-  // sums keN, keR, eS, eSh, eB, eT | maxima |F|, |tm|, |load|, |N|, |M|
-  double v[11] = {keN, keR, eS, eSh, eB, eT, mxF, mxT, mxL, mxN, mxM};
+  // sums Σm|v|² + ΣI3ω², stretch, shear, bending, torsion energies | maxima |F_free|, |t_m|, and the node / segment /
+  // joint terms of F_ref: max(|load|, |N|, |M| / l̄_min) (forceRef, solver_cpu.js:843-853)
+  const double fr = fmax(mxL, fmax(mxN, mxM * I.inv_jlen_min));
+  double v[SPART] = {keN + keR, eS, eSh, eB, eT, mxF, mxT, fr};
 #pragma unroll
-  for (int q = 0; q < 6; q++) v[q] = wave_reduce63<true>(v[q]);
+  for (int q = 0; q < 5; q++) v[q] = wave_reduce63<true>(v[q]);
 #pragma unroll
-  for (int q = 6; q < 11; q++) v[q] = wave_reduce63<false>(v[q]);
+  for (int q = 5; q < SPART; q++) v[q] = wave_reduce63<false>(v[q]);
   if (TILE > 64) {
     if (lane == 63) {
 #pragma unroll
-      for (int q = 0; q < 11; q++) L.wred[q][wv] = v[q];
+      for (int q = 0; q < SPART; q++) L.wred[q][wv] = v[q];
     }
     __syncthreads();
     if (tid == 63) {
 #pragma unroll
-      for (int q = 0; q < 11; q++) {
+      for (int q = 0; q < SPART; q++) {
         double x = L.wred[q][0];
-        for (int w = 1; w < TILE / 64; w++) x = q < 6 ? x + L.wred[q][w] : fmax(x, L.wred[q][w]);
+        for (int w = 1; w < TILE / 64; w++) x = q < 5 ? x + L.wred[q][w] : fmax(x, L.wred[q][w]);
         v[q] = x;
       }
     }
@@ -1064,7 +1092,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
     const size_t np = size_t(I.ntiles);
     const auto sp = gl(I.spart) + size_t(tl.tile_local);
 #pragma unroll
-    for (int q = 0; q < 11; q++) sp[q * np] = v[q];
+    for (int q = 0; q < SPART; q++) sp[q * np] = v[q];
   }
 }
 
@@ -1094,7 +1122,10 @@ __device__ __forceinline__ void block_reduce(double (*red)[TILE / 64], double* v
 template <typename Real>
 __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restrict__ insts, const CtrlItem* __restrict__ items,
                                                       int32_t nitems, Ctl* __restrict__ ctls) {
-  __shared__ double red[13][CTRL_THREADS / 64];
+  // accumulator: [0] Σm|v|² + ΣI3ω², [1..4] stretch / shear / bending / torsion energy, [5] contact energy (sums);
+  // [6] max |F_free|, [7] max |t_m|, [8] max(|load|, |N|, |M|/l̄_min), [9] max Fn (maxima)
+  constexpr int NACC = RPART, NSUM = 6;
+  __shared__ double red[NACC][CTRL_THREADS / 64];
   __shared__ int last;
   if (int(blockIdx.x) >= nitems) return;
   const CtrlItem it = items[blockIdx.x];
@@ -1104,46 +1135,43 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   Ctl c = *ctl;
   // this workgroup's share of the strand-tile and crossing-block partials (value-major: coalesced loads).  A round
   // issues every load of CTRL_PER partials per lane before using any, so a round costs one memory latency; the
-  // loads are issued before the liveness test (a stopped instance's partials are read and discarded)
-  double s12[13];
-  for (int q = 0; q < 7; q++) s12[q] = 0.0;
-  for (int q = 7; q < 13; q++) s12[q] = 0.0;   // maxima of non-negative quantities (Fn clamped at 0: F_ref ≥ its other terms)
+  // loads are issued before the liveness test (a stopped instance's partials are read and discarded).  All maxima
+  // are of non-negative quantities (Fn enters clamped at 0, below F_ref's other terms), so 0 is their neutral value.
+  double acc[NACC];
+#pragma unroll
+  for (int q = 0; q < NACC; q++) acc[q] = 0.0;
   const int np = I.ntiles;
   const int p0 = it.chunk * I.ctrl_chunk, p1 = min(np, p0 + I.ctrl_chunk);
   for (int w0 = p0; w0 < p1; w0 += CTRL_THREADS * CTRL_PER) {
-    double x[CTRL_PER][11];
+    double x[CTRL_PER][SPART];
 #pragma unroll
     for (int k = 0; k < CTRL_PER; k++) {
       const int w = w0 + k * CTRL_THREADS + tid;
       const bool in = w < p1;
       const auto sp = gl(I.spart) + (in ? w : p0);
 #pragma unroll
-      for (int q = 0; q < 11; q++) x[k][q] = sp[size_t(q) * np];
-      if (!in) {
-#pragma unroll
-        for (int q = 0; q < 11; q++) x[k][q] = 0.0;
-      }
+      for (int q = 0; q < SPART; q++) x[k][q] = in ? sp[size_t(q) * np] : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < CTRL_PER; k++) {
 #pragma unroll
-      for (int q = 0; q < 6; q++) s12[q] += x[k][q];
+      for (int q = 0; q < 5; q++) acc[q] += x[k][q];
 #pragma unroll
-      for (int q = 6; q < 11; q++) s12[q + 1] = fmax(s12[q + 1], x[k][q]);
+      for (int q = 5; q < SPART; q++) acc[q + 1] = fmax(acc[q + 1], x[k][q]);
     }
   }
   const int cper = (I.ncblk + it.nchunks - 1) / it.nchunks;
   const int c0 = it.chunk * cper, c1 = min(I.ncblk, c0 + cper);
   for (int w = c0 + tid; w < c1; w += CTRL_THREADS) {
-    s12[6] += gl(I.cpart)[w];
-    s12[12] = fmax(s12[12], gl(I.cpart)[I.ncblk + w]);
+    acc[5] += gl(I.cpart)[w];
+    acc[9] = fmax(acc[9], gl(I.cpart)[I.ncblk + w]);
   }
   if (!(c.done == 0 && c.iter < c.stop_at)) return;   // uniform over the workgroup
-  block_reduce<CTRL_THREADS, 13>(red, s12, 7);
+  block_reduce<CTRL_THREADS, NACC>(red, acc, NSUM);
   if (it.nchunks > 1) {
     // several workgroups: each leaves its partial; the last to arrive reduces them in chunk order and goes on
     if (tid == 0) {
-      for (int q = 0; q < 13; q++) gl(I.rpart)[size_t(it.chunk) * RPART + q] = s12[q];
+      for (int q = 0; q < NACC; q++) gl(I.rpart)[size_t(it.chunk) * RPART + q] = acc[q];
       __threadfence();
       last = atomicAdd(&ctl->counter, 1) == it.nchunks - 1;
     }
@@ -1152,17 +1180,16 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
     if (tid != 0) return;
     __threadfence();
     const volatile double* rp = I.rpart;
-    for (int q = 0; q < 7; q++) s12[q] = 0.0;
-    for (int q = 7; q < 13; q++) s12[q] = 0.0;
+    for (int q = 0; q < NACC; q++) acc[q] = 0.0;
     for (int k = 0; k < it.nchunks; k++) {
-      for (int q = 0; q < 7; q++) s12[q] += rp[size_t(k) * RPART + q];
-      for (int q = 7; q < 13; q++) s12[q] = fmax(s12[q], rp[size_t(k) * RPART + q]);
+      for (int q = 0; q < NSUM; q++) acc[q] += rp[size_t(k) * RPART + q];
+      for (int q = NSUM; q < NACC; q++) acc[q] = fmax(acc[q], rp[size_t(k) * RPART + q]);
     }
     c.counter = 0;                                 // the other workgroups change nothing else in the record
   }
   if (tid != 0) return;
-  const double eCont = s12[6];
-  double ke = 0.5 * (s12[0] + s12[1]);
+  const double eCont = acc[5];
+  double ke = 0.5 * acc[0];
   int rollback = 0;
   if (I.kin_damp && ke < c.ke_prev) { rollback = 1; ke = 0.0; }
   const int iterPre = c.iter;
@@ -1177,22 +1204,21 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   } else {
     c.al_update = 0;
   }
-  double fref = fmax(s12[9], s12[10]);
-  if (I.p > 0) fref = fmax(fref, s12[11] / fmax(I.jlen_min, 1e-12));
-  if (I.contacts) fref = fmax(fref, s12[12]);
+  double fref = acc[8];
+  if (I.contacts) fref = fmax(fref, acc[9]);
   fref = fmax(fref, 1e-12);
-  const double rf = I.n ? s12[7] / fref : 0.0, rt = I.m ? s12[8] / (fref * I.l_ref) : 0.0;
+  const double rf = I.n ? acc[6] / fref : 0.0, rt = I.m ? acc[7] / (fref * I.l_ref) : 0.0;
   double resid = fmax(rf, rt);
   if (!(c.iter > I.warmup)) resid = INFINITY;
   c.resid = resid; c.fref = fref; c.res_f = rf; c.res_t = rt;
-  c.e_stretch = s12[2]; c.e_shear = s12[3]; c.e_bend = s12[4]; c.e_tors = s12[5]; c.e_cont = eCont;
+  c.e_stretch = acc[1]; c.e_shear = acc[2]; c.e_bend = acc[3]; c.e_tors = acc[4]; c.e_cont = eCont;
   if (resid <= I.tol) c.done = 1;
   if (!rollback) c.cur = c.cur ^ 1;
   c.live_flag = 1;   // an iteration has been evaluated
   if (I.hist_len > 0) {
     const auto h = gl(I.hist) + size_t((c.iter - 1) % I.hist_len) * HIST_STRIDE;
-    h[0] = ke; h[1] = s12[2] + s12[3] + s12[4] + s12[5] + eCont; h[2] = s12[4]; h[3] = s12[5];
-    h[4] = s12[2]; h[5] = s12[3]; h[6] = eCont; h[7] = resid; h[8] = double(c.iter);
+    h[0] = ke; h[1] = acc[1] + acc[2] + acc[3] + acc[4] + eCont; h[2] = acc[3]; h[3] = acc[4];
+    h[4] = acc[1]; h[5] = acc[2]; h[6] = eCont; h[7] = resid; h[8] = double(c.iter);
   }
   c.hist_n += 1;
   *ctl = c;
