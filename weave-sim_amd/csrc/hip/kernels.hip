@@ -669,18 +669,29 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
 template <typename Real, int TILE>
 struct StrandLDS {
   // structure-of-arrays: consecutive threads touch consecutive 8-byte words (no bank conflicts)
-  double Xx[TILE + 2], Xy[TILE + 2], Xz[TILE + 2];                   // nodes k0-1 .. k1
-  double Qw[TILE + 2], Qx[TILE + 2], Qy[TILE + 2], Qz[TILE + 2];     // segments k0-1 .. k1
-  Real SFx[TILE + 1], SFy[TILE + 1], SFz[TILE + 1];                  // world force of segments k0-1 .. k1-1
-  Real JAx[TILE + 1], JAy[TILE + 1], JAz[TILE + 1];                  // joint k0-1+i on its first segment (body axes)
-  Real JBx[TILE + 1], JBy[TILE + 1], JBz[TILE + 1];                  // ... on its second segment (its body axes)
-  Real KDx[TILE + 1], KDy[TILE + 1], KDz[TILE + 1];                  // bending-coupling inertia of joint k0-1+i
+  union {
+    struct {
+      double Xx[TILE + 2], Xy[TILE + 2], Xz[TILE + 2];                   // nodes k0-1 .. k1
+      double Qw[TILE + 2], Qx[TILE + 2], Qy[TILE + 2], Qz[TILE + 2];     // segments k0-1 .. k1
+    };
+    double part[SPART][TILE];                                            // the lanes' partials, once the state is dead
+  };
   int32_t rpre[TILE], rpost[TILE];                                   // gathered crossing sides: segments
   SideRec<Real> rec[TILE];                                           // ... and their records
   int32_t glo[TILE], ghi[TILE];                                      // per slot: first / last gathered record it needs
-  double wred[SPART][TILE / 64];                                     // wavefront partials of the tile
   int wcnt[TILE / 64];
+  Real bnd[TILE / 64][9];            // last lane of each wavefront: segment force, second-segment torque, kDyn
 };
+
+// the value of lane − 1 (DPP wave_shr:1; lane 0 reads 0)
+__device__ __forceinline__ float from_left(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ double from_left(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x138, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x138, 0xF, 0xF, true);
+  return __hiloint2double(hi, lo);
+}
 
 // f64 frame [X | Y | Z] of a quaternion with one reciprocal: R = I + 2/|q|² (…) (quat.js toFrame up to rounding)
 __device__ __forceinline__ void qframe_d(double w, double x, double y, double z, double* R) {
@@ -741,11 +752,11 @@ __device__ __forceinline__ void rotT2(const double* d, const Real* v, Real* out)
   out[2] = R6 * v[0] + R7 * v[1] + R8 * v[2];
 }
 
-// one segment's stretch / shear (CosseratRod.apply, rod_cosserat.js:118-133): world force into SF[i]; body torque
+// one segment's stretch / shear (CosseratRod.apply, rod_cosserat.js:118-133): world force sf; body torque
 // (Rᵀe) × (Rᵀn) and the section forces returned for an owned segment.  σ is formed in f64 (it cancels in f32).
 template <typename Real, int TILE>
-__device__ __forceinline__ void seg_force(StrandLDS<Real, TILE>& L, int i, double l0, Real EA, Real GA, Real* st, Real* nb,
-                                          double* eS, double* eSh) {
+__device__ __forceinline__ void seg_force(StrandLDS<Real, TILE>& L, int i, double l0, Real EA, Real GA, Real* sf, Real* st,
+                                          Real* nb, double* eS, double* eSh) {
   double R[9];
   qframe_d(L.Qw[i], L.Qx[i], L.Qy[i], L.Qz[i], R);
   double ex = L.Xx[i + 1] - L.Xx[i], ey = L.Xy[i + 1] - L.Xy[i], ez = L.Xz[i + 1] - L.Xz[i];
@@ -753,33 +764,31 @@ __device__ __forceinline__ void seg_force(StrandLDS<Real, TILE>& L, int i, doubl
   double il = rcp_(l0);
   double s1 = b0 * il, s2 = b1 * il, s3 = b2 * il - 1.0;
   Real n1 = GA * Real(s1), n2 = GA * Real(s2), n3 = EA * Real(s3);
-  L.SFx[i] = n1 * Real(R[0]) + n2 * Real(R[3]) + n3 * Real(R[6]);
-  L.SFy[i] = n1 * Real(R[1]) + n2 * Real(R[4]) + n3 * Real(R[7]);
-  L.SFz[i] = n1 * Real(R[2]) + n2 * Real(R[5]) + n3 * Real(R[8]);
+  sf[0] = n1 * Real(R[0]) + n2 * Real(R[3]) + n3 * Real(R[6]);
+  sf[1] = n1 * Real(R[1]) + n2 * Real(R[4]) + n3 * Real(R[7]);
+  sf[2] = n1 * Real(R[2]) + n2 * Real(R[5]) + n3 * Real(R[8]);
   Real e0 = Real(b0), e1 = Real(b1), e2 = Real(b2);
   st[0] = e1 * n3 - e2 * n2; st[1] = e2 * n1 - e0 * n3; st[2] = e0 * n2 - e1 * n1;
   nb[0] = n1; nb[1] = n2; nb[2] = n3;
   *eS = 0.5 * l0 * double(n3) * s3; *eSh = 0.5 * l0 * (double(n1) * s1 + double(n2) * s2);
 }
 
-// one joint (rod_cosserat.js:136-159) between LDS segments i and i+1: J on the first, −R(Δq)ᵀ J on the second, kDyn
+// one joint (rod_cosserat.js:136-159) between LDS segments i and i+1: J on the first (ja), −R(Δq)ᵀ J on the second
+// (jb), the bending-coupling inertia kDyn (kd)
 template <typename Real, int TILE>
 __device__ __forceinline__ void joint_torque(StrandLDS<Real, TILE>& L, int i, Real lb, const Real* om, Real Bx, Real By, Real Bz,
-                                             Real* mm, Real* d, Real* Jo) {
+                                             Real* mm, Real* d, Real* ja, Real* jb, Real* kd) {
   Real ps[3]; double dq[4];
   Real th = relrot_R<Real>(L.Qw[i], L.Qx[i], L.Qy[i], L.Qz[i], L.Qw[i + 1], L.Qx[i + 1], L.Qy[i + 1], L.Qz[i + 1], ps, dq);
   Real il = rcp_(lb);
   d[0] = ps[0] * il - om[0]; d[1] = ps[1] * il - om[1]; d[2] = ps[2] * il - om[2];
   mm[0] = Bx * d[0]; mm[1] = By * d[1]; mm[2] = Bz * d[2];
   Real q0 = ps[0] * ps[0], q1 = ps[1] * ps[1], q2 = ps[2] * ps[2], f = Real(0.25) * il;
-  L.KDx[i] = f * (By * q2 + Bz * q1); L.KDy[i] = f * (Bx * q2 + Bz * q0); L.KDz[i] = f * (Bx * q1 + By * q0);
-  Real J[3];
-  jlt(ps, th, mm, J);
-  L.JAx[i] = J[0]; L.JAy[i] = J[1]; L.JAz[i] = J[2];
+  kd[0] = f * (By * q2 + Bz * q1); kd[1] = f * (Bx * q2 + Bz * q0); kd[2] = f * (Bx * q1 + By * q0);
+  jlt(ps, th, mm, ja);
   Real Jb[3];
-  rotT2(dq, J, Jb);
-  L.JBx[i] = -Jb[0]; L.JBy[i] = -Jb[1]; L.JBz[i] = -Jb[2];
-  Jo[0] = J[0]; Jo[1] = J[1]; Jo[2] = J[2];
+  rotT2(dq, ja, Jb);
+  jb[0] = -Jb[0]; jb[1] = -Jb[1]; jb[2] = -Jb[2];
 }
 
 // One strand tile.  A tile owns cnt ≤ TILE − 1 consecutive nodes k0 .. k1−1 of one lath and the segments starting
@@ -850,31 +859,41 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
 
   // ---------------------------------------------------------------- segments and joints
   double eS = 0, eSh = 0, eB = 0, eT = 0, mxN = 0, mxM = 0;
-  Real ST[3] = {0, 0, 0};
+  Real ST[3] = {0, 0, 0}, SF[3] = {0, 0, 0}, JA[3] = {0, 0, 0}, JB[3] = {0, 0, 0}, KD[3] = {0, 0, 0};
   if (hasSeg) {
     Real nb[3]; double a1, a2;
-    seg_force(L, t, l0, EA, GA, ST, nb, &a1, &a2);
+    seg_force(L, t, l0, EA, GA, SF, ST, nb, &a1, &a2);
     if (ownSeg) {
       eS = a1; eSh = a2;
       mxN = fabs(double(nb[2]));
       if (OUT) { outN[j] = double(nb[2]); outV[2 * j] = double(nb[0]); outV[2 * j + 1] = double(nb[1]); }
     }
-  } else if (inSlot) {
-    L.SFx[t] = L.SFy[t] = L.SFz[t] = Real(0);
   }
   if (hasJoint) {
-    Real mm[3], d[3], J[3];
-    joint_torque(L, t, lb, om, Bx, By, Bz, mm, d, J);
+    Real mm[3], d[3];
+    joint_torque(L, t, lb, om, Bx, By, Bz, mm, d, JA, JB, KD);
     if (ownNode) {
       eB = 0.5 * double(lb) * (double(mm[0]) * double(d[0]) + double(mm[1]) * double(d[1]));
       eT = 0.5 * double(lb) * double(mm[2]) * double(d[2]);
       mxM = fmax(fabs(double(mm[0])), fmax(fabs(double(mm[1])), fabs(double(mm[2]))));
       if (OUT) { const int jg = joff + jjo; outM[3 * jg] = double(mm[0]); outM[3 * jg + 1] = double(mm[1]); outM[3 * jg + 2] = double(mm[2]); }
     }
-  } else if (inSlot) {
-    L.JAx[t] = L.JAy[t] = L.JAz[t] = Real(0);
-    L.JBx[t] = L.JBy[t] = L.JBz[t] = Real(0);
-    L.KDx[t] = L.KDy[t] = L.KDz[t] = Real(0);
+  }
+  // what lane t needs from lane t − 1: the force of segment t − 1, joint t − 1's torque on segment t and its kDyn —
+  // a one-lane shift inside the wavefront, through LDS across a wavefront boundary
+  Real pSF[3], pJB[3], pKD[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) { pSF[c] = from_left(SF[c]); pJB[c] = from_left(JB[c]); pKD[c] = from_left(KD[c]); }
+  if (TILE > 64) {
+    if ((tid & 63) == 63) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) { L.bnd[tid >> 6][c] = SF[c]; L.bnd[tid >> 6][3 + c] = JB[c]; L.bnd[tid >> 6][6 + c] = KD[c]; }
+    }
+    __syncthreads();
+    if ((tid & 63) == 0 && tid > 0) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) { pSF[c] = L.bnd[(tid >> 6) - 1][c]; pJB[c] = L.bnd[(tid >> 6) - 1][3 + c]; pKD[c] = L.bnd[(tid >> 6) - 1][6 + c]; }
+    }
   }
 
   // ---------------------------------------------------------------- crossing sides of this strand touching the tile,
@@ -944,9 +963,9 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   if (ownNode) {
     const bool hasEnd = closed || k >= 1, hasStart = closed || k < nseg;
     Real F[3];
-    if (hasEnd && hasStart) { F[0] = L.SFx[t] - L.SFx[t - 1]; F[1] = L.SFy[t] - L.SFy[t - 1]; F[2] = L.SFz[t] - L.SFz[t - 1]; }
-    else if (hasStart) { F[0] = L.SFx[t]; F[1] = L.SFy[t]; F[2] = L.SFz[t]; }
-    else if (hasEnd) { F[0] = -L.SFx[t - 1]; F[1] = -L.SFy[t - 1]; F[2] = -L.SFz[t - 1]; }
+    if (hasEnd && hasStart) { F[0] = SF[0] - pSF[0]; F[1] = SF[1] - pSF[1]; F[2] = SF[2] - pSF[2]; }
+    else if (hasStart) { F[0] = SF[0]; F[1] = SF[1]; F[2] = SF[2]; }
+    else if (hasEnd) { F[0] = -pSF[0]; F[1] = -pSF[1]; F[2] = -pSF[2]; }
     else { F[0] = F[1] = F[2] = Real(0); }
     Real ld[3] = {ldx, ldy, ldz};
     if (I.npl && (llx != Real(0) || lly != Real(0) || llz != Real(0))) {
@@ -972,7 +991,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
     if (fx & 1) F[0] = Real(0);
     if (fx & 2) F[1] = Real(0);
     if (fx & 4) F[2] = Real(0);
-    mxF = sqrt(double(F[0]) * double(F[0]) + double(F[1]) * double(F[1]) + double(F[2]) * double(F[2]));
+    mxF = double(F[0]) * double(F[0]) + double(F[1]) * double(F[1]) + double(F[2]) * double(F[2]);   // squared: √ after the max
     if (!OUT) {
       Real V[3] = {vx, vy, vz};
       if (warm) { V[0] = V[1] = V[2] = Real(0); F[0] = F[1] = F[2] = Real(0); }
@@ -992,7 +1011,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   }
   // ---------------------------------------------------------------- frames: body torque, inertia, ω, q ← q ⊗ exp(ω)
   if (ownSeg) {
-    Real tm[3] = {ST[0] + L.JAx[t] + L.JBx[t - 1], ST[1] + L.JAy[t] + L.JBy[t - 1], ST[2] + L.JAz[t] + L.JBz[t - 1]};
+    Real tm[3] = {ST[0] + JA[0] + pJB[0], ST[1] + JA[1] + pJB[1], ST[2] + JA[2] + pJB[2]};
     if (I.C > 0) {
       for (int e = gl(I.clamp_off)[j]; e < gl(I.clamp_off)[j + 1]; e++) {
         const int cc = gl(I.clamp_idx)[e];
@@ -1018,12 +1037,12 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       tm[1] += Real(Rs[3]) * Tc[0] + Real(Rs[4]) * Tc[1] + Real(Rs[5]) * Tc[2];
       tm[2] += Real(Rs[6]) * Tc[0] + Real(Rs[7]) * Tc[1] + Real(Rs[8]) * Tc[2];
     }
-    mxT = sqrt(double(tm[0]) * double(tm[0]) + double(tm[1]) * double(tm[1]) + double(tm[2]) * double(tm[2]));
+    mxT = double(tm[0]) * double(tm[0]) + double(tm[1]) * double(tm[1]) + double(tm[2]) * double(tm[2]);   // squared
     if (!OUT) {
       const Real msc = Real(I.mass_scale);
-      Real I3[3] = {msc * dmax(krx + kTw + (L.KDx[t - 1] + L.KDx[t]), Real(1e-12)),
-                    msc * dmax(kry + kTw + (L.KDy[t - 1] + L.KDy[t]), Real(1e-12)),
-                    msc * dmax(krz + kTw + (L.KDz[t - 1] + L.KDz[t]), Real(1e-12))};
+      Real I3[3] = {msc * dmax(krx + kTw + (pKD[0] + KD[0]), Real(1e-12)),
+                    msc * dmax(kry + kTw + (pKD[1] + KD[1]), Real(1e-12)),
+                    msc * dmax(krz + kTw + (pKD[2] + KD[2]), Real(1e-12))};
       Real W[3] = {wx + tm[0] * rcp_(I3[0]), wy + tm[1] * rcp_(I3[1]), wz + tm[2] * rcp_(I3[2])};
       W[2] = clampv(W[2], Real(-TWIST_RATE_MAX), Real(TWIST_RATE_MAX));
       if (kTw > Real(0)) W[2] *= Real(ROLL_DAMP);
@@ -1065,34 +1084,27 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   if (OUT) return;
 
   // ---------------------------------------------------------------- partial of the tile (fixed order; reduced by k_ctrl):
-  // sums Σm|v|² + ΣI3ω², stretch, shear, bending, torsion energies | maxima |F_free|, |t_m|, and the node / segment /
+  // sums Σm|v|² + ΣI3ω², stretch, shear, bending, torsion energies | maxima |F_free|², |t_m|², and the node / segment /
   // joint terms of F_ref: max(|load|, |N|, |M| / l̄_min) (forceRef, solver_cpu.js:843-853)
+  // Transposed through LDS: every lane leaves its eight values; then lane l of the first wavefront takes value
+  // l / 8 over lanes l % 8, l % 8 + 8, … in order, and the eight lanes of each value combine by a butterfly within
+  // the 8-lane group (xor 1, xor 2 within quads, then the half-row mirror).  Fixed order, no atomics.
   const double fr = fmax(mxL, fmax(mxN, mxM * I.inv_jlen_min));
-  double v[SPART] = {keN + keR, eS, eSh, eB, eT, mxF, mxT, fr};
+  const double v[SPART] = {keN + keR, eS, eSh, eB, eT, mxF, mxT, fr};
+  __syncthreads();                                 // the staged state is dead: its space takes the partials
 #pragma unroll
-  for (int q = 0; q < 5; q++) v[q] = wave_reduce63<true>(v[q]);
-#pragma unroll
-  for (int q = 5; q < SPART; q++) v[q] = wave_reduce63<false>(v[q]);
-  if (TILE > 64) {
-    if (lane == 63) {
-#pragma unroll
-      for (int q = 0; q < SPART; q++) L.wred[q][wv] = v[q];
-    }
-    __syncthreads();
-    if (tid == 63) {
-#pragma unroll
-      for (int q = 0; q < SPART; q++) {
-        double x = L.wred[q][0];
-        for (int w = 1; w < TILE / 64; w++) x = q < 5 ? x + L.wred[q][w] : fmax(x, L.wred[q][w]);
-        v[q] = x;
-      }
-    }
-  }
-  if (tid == 63) {   // [value][tile] so that the control kernel's loads coalesce
-    const size_t np = size_t(I.ntiles);
-    const auto sp = gl(I.spart) + size_t(tl.tile_local);
-#pragma unroll
-    for (int q = 0; q < SPART; q++) sp[q * np] = v[q];
+  for (int q = 0; q < SPART; q++) L.part[q][t] = v[q];
+  __syncthreads();
+  if (tid < 64) {
+    static_assert(SPART == 8, "one 8-lane group per partial value");
+    const int q = lane >> 3, k = lane & 7;
+    const bool sum = q < 5;
+    double x = L.part[q][k];
+    for (int i = k + 8; i < TILE; i += 8) { const double y = L.part[q][i]; x = sum ? x + y : fmax(x, y); }
+    double y = dpp_<0xB1, 0xF>(x); x = sum ? x + y : fmax(x, y);     // quad_perm [1,0,3,2]
+    y = dpp_<0x4E, 0xF>(x); x = sum ? x + y : fmax(x, y);            // quad_perm [2,3,0,1]
+    y = dpp_<0x141, 0xF>(x); x = sum ? x + y : fmax(x, y);           // row_half_mirror
+    if (k == 0) gl(I.spart)[size_t(q) * I.ntiles + tl.tile_local] = x;   // [value][tile]: coalesced for k_ctrl
   }
 }
 
@@ -1123,7 +1135,7 @@ template <typename Real>
 __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restrict__ insts, const CtrlItem* __restrict__ items,
                                                       int32_t nitems, Ctl* __restrict__ ctls) {
   // accumulator: [0] Σm|v|² + ΣI3ω², [1..4] stretch / shear / bending / torsion energy, [5] contact energy (sums);
-  // [6] max |F_free|, [7] max |t_m|, [8] max(|load|, |N|, |M|/l̄_min), [9] max Fn (maxima)
+  // [6] max |F_free|², [7] max |t_m|², [8] max(|load|, |N|, |M|/l̄_min), [9] max Fn (maxima)
   constexpr int NACC = RPART, NSUM = 6;
   __shared__ double red[NACC][CTRL_THREADS / 64];
   __shared__ int last;
@@ -1207,7 +1219,8 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   double fref = acc[8];
   if (I.contacts) fref = fmax(fref, acc[9]);
   fref = fmax(fref, 1e-12);
-  const double rf = I.n ? acc[6] / fref : 0.0, rt = I.m ? acc[7] / (fref * I.l_ref) : 0.0;
+  // the strand partials carry max |F_free|² and max |t_m|²: the square root of the maximum is the maximum norm
+  const double rf = I.n ? sqrt(acc[6]) / fref : 0.0, rt = I.m ? sqrt(acc[7]) / (fref * I.l_ref) : 0.0;
   double resid = fmax(rf, rt);
   if (!(c.iter > I.warmup)) resid = INFINITY;
   c.resid = resid; c.fref = fref; c.res_f = rf; c.res_t = rt;
