@@ -39,6 +39,8 @@ def lib():
         L.oracle_get.restype = None
         L.oracle_inertia.argtypes = [C.c_void_p, A.P_d]
         L.oracle_inertia.restype = None
+        L.oracle_floor.argtypes = [C.c_void_p, A.P_d, C.POINTER(C.c_uint8), A.P_d]
+        L.oracle_floor.restype = C.c_int32
         L.oracle_history.argtypes = [C.c_void_p, A.P_d, A.P_d, A.P_d, C.c_int32]
         L.oracle_rod_apply.argtypes = [C.c_void_p, A.P_d, A.P_d, A.P_d, A.P_d]
         L.oracle_rod_apply.restype = C.c_double
@@ -92,6 +94,15 @@ class Oracle:
         out = np.zeros(3 * self.model.m)
         lib().oracle_inertia(self.h, out.ctypes.data_as(A.P_d))
         return out
+
+    def floor(self):
+        """the ground's per-node forces (n × 3), state (0 off, 1 stick, 2 slide) and the rigid-body residual part"""
+        F = np.zeros(3 * self.model.n)
+        st = np.zeros(self.model.n, dtype=np.uint8)
+        rr = C.c_double()
+        if not lib().oracle_floor(self.h, F.ctypes.data_as(A.P_d), st.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(rr)):
+            return None
+        return F.reshape(-1, 3), st, rr.value
 
     def history(self, cap=10 ** 7):
         n = min(cap, self.iteration)

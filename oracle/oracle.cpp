@@ -825,6 +825,14 @@ WS_API double oracle_residual(oracle_t* h) { return h->o.residual; }
 WS_API double oracle_ke(oracle_t* h) { return h->o.ke; }
 WS_API void oracle_get(oracle_t* h, ws_state* out) { h->o.exportState(*out); }
 WS_API void oracle_inertia(oracle_t* h, double* out) { std::copy(h->o.inertia3.begin(), h->o.inertia3.end(), out); }
+// the ground's per-node forces of the last iteration and state (0 off, 1 stick, 2 slide) (Floor.F / Floor.state)
+WS_API int32_t oracle_floor(oracle_t* h, double* F, uint8_t* state, double* rigid_residual) {
+  if (!h->o.floorOn) return 0;
+  std::copy(h->o.fF.begin(), h->o.fF.end(), F);
+  std::copy(h->o.fState.begin(), h->o.fState.end(), state);
+  if (rigid_residual) *rigid_residual = h->o.resRigid;
+  return 1;
+}
 WS_API int32_t oracle_history(oracle_t* h, double* ke, double* strain, double* resid, int32_t cap) {
   int32_t nn = int32_t(std::min<size_t>(h->o.histKE.size(), size_t(cap)));
   for (int i = 0; i < nn; i++) { if (ke) ke[i] = h->o.histKE[i]; if (strain) strain[i] = h->o.histStrain[i]; if (resid) resid[i] = h->o.histResid[i]; }

@@ -53,3 +53,18 @@ config1 = _S.dome_3m
 config2 = _S.dome_10m
 config5 = _S.dome_30m
 config3_variant = _S.design_variant
+
+
+def ground_lath(push=0.0, mu=0.4, anchor=False, tilt=0.05, lift=2.0, push_dir=(1.0, 0.0, 0.0), **scene):
+    """one straight lath above the ground, cm·N (floor_check.mjs:12-25): 100 cm, 10 segments, 6 × 3 section,
+    E = 1e6, G = 6e4, self-weight, optional sideways push (a UDL along x), optionally pinned at its start"""
+    L, nseg = 100.0, 10
+    P = [[L * i / nseg - L / 2, 0.0, lift + tilt * (L * i / nseg)] for i in range(nseg + 1)]
+    kw = dict(width=6.0, height=3.0, E=1e6, G=6e4, strength=3000.0, stick_strength=1000.0, ramp_iter=0, gravity=1,
+              tol=1e-6, contacts=0, floor_on=1, floor_z=0.0, floor_mu=mu, length_to_m=0.01)
+    if push:
+        kw.update(udl=1, udl_q=[push / L * c for c in push_dir])
+    kw.update(scene)
+    sc = W.Scene(**kw)
+    anchors = [(A.WSM_ALL_START, 0, 0, [1, 1, 1, 0, 0, 0])] if anchor else []
+    return W.Model.from_polylines([P], scene=sc, anchors=anchors)

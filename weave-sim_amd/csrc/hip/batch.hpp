@@ -110,7 +110,12 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   const int n = md.n, m = md.m, p = md.p, C = md.C, S = md.S;
   const int K = md.par.contacts ? md.K : 0;   // no contact goal: no crossings on the device
   if (n <= 0 || m < 0 || S <= 0) throw Fail{WS_ERR_ARG, "model has no nodes or strands"};
-  if (md.par.floor_on) throw Fail{WS_ERR_ARG, "the ground (scene.floor) is not available on the device path"};
+  if (md.par.floor_on) {
+    // the ground's rigid-body freedoms (a structure with no anchor resting on it) are not on the device
+    bool anchored = md.C > 0;
+    for (int i = 0; i < md.n && !anchored; i++) anchored = !(md.node_free[3 * i] && md.node_free[3 * i + 1] && md.node_free[3 * i + 2]);
+    if (!anchored) throw Fail{WS_ERR_ARG, "the ground without an anchor (rigid-body freedoms) is not on the device path"};
+  }
   for (int q = 0; q < 2 * (md.par.contacts ? md.K : 0); q++)
     if (md.k_nseg[q] < 1 || md.k_strand[q] < 0 || md.k_strand[q] >= S) throw Fail{WS_ERR_ARG, "a crossing lies on a strand without segments"};
   for (int s2 = 0; s2 < S; s2++)
@@ -130,6 +135,7 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   d.kin_damp = P.kinetic_damping; d.warmup = P.warmup_iter; d.contacts = K > 0 ? 1 : 0; d.ramp_iter = P.ramp_iter;
   d.rot_fric = P.rot_friction; d.seat = P.seat_law; d.seat_gap = P.seat_gap; d.dent_after = P.dent_after;
   d.aug = P.augmented; d.al_every = P.al_every; d.hist_len = hist_len;
+  d.floor_on = P.floor_on ? 1 : 0; d.floor_z = P.floor_z; d.floor_mu = std::max(P.floor_mu, 0.0);
   // ---- strand tiles and crossing blocks
   H.tiles.clear();
   for (int s = 0; s < S; s++) {
@@ -164,6 +170,7 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   A.add(d.wx, m); A.add(d.wy, m); A.add(d.wz, m);
   A.addc(d.l0, m); A.addc(d.EA, m); A.addc(d.GA, m); A.addc(d.EIx, m); A.addc(d.EIy, m); A.addc(d.GJ, m);
   A.addc(d.krx, m); A.addc(d.kry, m); A.addc(d.krz, m);
+  if (d.floor_on) { A.addc(d.nodew, n); A.addc(d.nodeh, n); A.add(d.fax, n); A.add(d.fay, n); }
   A.addc(d.jlen, p); A.addc(d.om0, 3 * p);
   if (C) { A.addc(d.c_len, C); A.addc(d.c_B, 3 * C); A.addc(d.c_Q, 4 * C); }
   A.addc(d.k_seg_off, 2 * K); A.addc(d.k_nseg, 2 * K); A.addc(d.k_noff, 2 * K); A.addc(d.k_nn, 2 * K); A.addc(d.wk, 2 * K); A.addc(d.hk, 2 * K);
@@ -213,6 +220,7 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   }
   // ---- goals, parameters, crossing constants
   h2d(d.mass, toReal<Real>(md.mass, n));
+  if (d.floor_on) { h2d(d.nodew, toReal<Real>(md.node_w, n)); h2d(d.nodeh, toReal<Real>(md.node_h, n)); }
   std::vector<Real> lx(n), ly(n), lz(n);
   for (int i = 0; i < n; i++) { lx[i] = Real(md.load_const[3 * i]); ly[i] = Real(md.load_const[3 * i + 1]); lz[i] = Real(md.load_const[3 * i + 2]); }
   h2d(d.lx, lx); h2d(d.ly, ly); h2d(d.lz, lz);
@@ -281,6 +289,11 @@ void upload_state(InstHost<Real>& H, const ws_state& st) {
   std::vector<Real> vx(n), vy(n), vz(n);
   for (int i = 0; i < n; i++) { vx[i] = st.V ? Real(st.V[3 * i]) : Real(0); vy[i] = st.V ? Real(st.V[3 * i + 1]) : Real(0); vz[i] = st.V ? Real(st.V[3 * i + 2]) : Real(0); }
   h2d(d.vx, vx); h2d(d.vy, vy); h2d(d.vz, vz);
+  if (d.floor_on) {   // stick points: the state's, else under the nodes (Floor constructor, solver_cpu.js:540-545)
+    std::vector<double> ax(n), ay(n);
+    for (int i = 0; i < n; i++) { ax[i] = st.floor_a ? st.floor_a[2 * i] : x[i]; ay[i] = st.floor_a ? st.floor_a[2 * i + 1] : y[i]; }
+    h2d(d.fax, ax); h2d(d.fay, ay);
+  }
   std::vector<double> qw(m), qx(m), qy(m), qz(m);
   for (int j = 0; j < m; j++) { qw[j] = st.Q[4 * j]; qx[j] = st.Q[4 * j + 1]; qy[j] = st.Q[4 * j + 2]; qz[j] = st.Q[4 * j + 3]; }
   for (int b = 0; b < 2; b++) { h2d(d.qw[b], qw); h2d(d.qx[b], qx); h2d(d.qy[b], qy); h2d(d.qz[b], qz); }
@@ -616,6 +629,10 @@ struct Batch : BatchBase {
     const bool rb = c.rollback != 0;
     auto x = d2h(d.x[cur], n), y = d2h(d.y[cur], n), z = d2h(d.z[cur], n);
     if (o.X) for (int i = 0; i < n; i++) { o.X[3 * i] = x[i]; o.X[3 * i + 1] = y[i]; o.X[3 * i + 2] = z[i]; }
+    if (o.floor_a && d.floor_on) {
+      auto ax = d2h(d.fax, n), ay = d2h(d.fay, n);
+      for (int i = 0; i < n; i++) { o.floor_a[2 * i] = ax[i]; o.floor_a[2 * i + 1] = ay[i]; }
+    }
     if (o.X0) std::copy(H.X0.begin(), H.X0.end(), o.X0);
     if (o.Q0) std::copy(H.Q0.begin(), H.Q0.end(), o.Q0);
     auto qw = d2h(d.qw[cur], m), qx = d2h(d.qx[cur], m), qy = d2h(d.qy[cur], m), qz = d2h(d.qz[cur], m);
@@ -719,6 +736,8 @@ struct Batch : BatchBase {
     d.dent_rate = P.dent_rate; d.edge_thr = P.edge_threshold; d.l_ref = md.l_ref; d.kin_damp = P.kinetic_damping;
     d.warmup = P.warmup_iter; d.ramp_iter = P.ramp_iter; d.rot_fric = P.rot_friction; d.seat = P.seat_law; d.seat_gap = P.seat_gap;
     d.dent_after = P.dent_after; d.aug = P.augmented; d.al_every = P.al_every;
+    if ((P.floor_on != 0) != (d.floor_on != 0)) throw Fail{WS_ERR_TOPOLOGY, "the ground was switched on or off: rebuild the GPU solver"};
+    d.floor_z = P.floor_z; d.floor_mu = std::max(P.floor_mu, 0.0);
     H.tol = P.tol;
     h2d(d.mass, toReal<Real>(md.mass, n));
     std::vector<Real> lx(n), ly(n), lz(n);

@@ -13,13 +13,13 @@
 namespace wsd {
 
 constexpr int HIST_STRIDE = 9;   // history ring entry: ke, strain, bending, torsion, stretch, shear, contact, residual, iteration
-constexpr int SPART = 8;         // doubles per strand-tile partial (stored value-major)
+constexpr int SPART = 9;         // doubles per strand-tile partial (stored value-major)
 constexpr int CPART = 2;         // doubles per crossing-block partial (stored value-major)
 constexpr int CTRL_THREADS = 256;  // lanes of the control kernel's workgroup
 constexpr int CTRL_PER = 4;         // strand partials per lane per round (all loads of a round in flight together)
 constexpr int CTRL_ONE = 4096;    // up to this many strand partials one control workgroup reduces them all
 constexpr int CTRL_CHUNK = 2048;  // beyond: partials per control workgroup (the last to finish combines them)
-constexpr int RPART = 10;         // doubles per control-chunk partial
+constexpr int RPART = 11;         // doubles per control-chunk partial
 constexpr int CROSS_THREADS = 64;                 // lanes per crossing-kernel block (two per crossing)
 constexpr int CROSS_BLOCK = CROSS_THREADS / 2;    // crossings per crossing-kernel block
 constexpr int MAX_REL = 1024;    // crossing sides a strand tile may gather (checked on the host per strand)
@@ -55,7 +55,8 @@ struct Inst {
   // scalar parameters (ws_params)
   double mass_scale, damping, tol, mu, strength, stick, torsion_factor, epsilon, slip_rate, rot_factor, dent_rate;
   double edge_thr, l_ref, jlen_min, inv_jlen_min;   // inv_jlen_min: 1/l̄_min, 0 without joints
-  int32_t kin_damp, warmup, contacts, ramp_iter, rot_fric, seat, seat_gap, dent_after, aug, al_every, hist_len, pad;
+  int32_t kin_damp, warmup, contacts, ramp_iter, rot_fric, seat, seat_gap, dent_after, aug, al_every, hist_len, floor_on;
+  double floor_z, floor_mu;          // the ground plane (anchored structures only)
   // topology
   const int32_t* node_off;   // [S+1]
   const int32_t* seg_off;    // [S+1]
@@ -82,6 +83,9 @@ struct Inst {
   const double* l0;
   const Real* EA; const Real* GA; const Real* EIx; const Real* EIy; const Real* GJ;
   const Real* krx; const Real* kry; const Real* krz;
+  // ground (floor_on): node section width / height, stick points (f64)
+  const Real* nodew; const Real* nodeh;
+  double* fax; double* fay;
   // joints
   const Real* jlen; const Real* om0;                          // om0 [3p]
   // clamps

@@ -957,7 +957,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   }
 
   // ---------------------------------------------------------------- nodes: loads, reactions, anchors, v += F/m, x += v
-  double keN = 0, keR = 0, mxF = 0, mxT = 0, mxL = 0;
+  double keN = 0, keR = 0, mxF = 0, mxT = 0, mxL = 0, eG = 0, mxG = 0;
   const bool warm = iter < I.warmup && I.contacts;
   const Real damping = Real(I.damping);
   if (ownNode) {
@@ -968,7 +968,9 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
     else if (hasEnd) { F[0] = -pSF[0]; F[1] = -pSF[1]; F[2] = -pSF[2]; }
     else { F[0] = F[1] = F[2] = Real(0); }
     Real ld[3] = {ldx, ldy, ldz};
-    if (I.npl && (llx != Real(0) || lly != Real(0) || llz != Real(0))) {
+    const bool localLoad = I.npl && (llx != Real(0) || lly != Real(0) || llz != Real(0));
+    double Rn[9];
+    if (localLoad || I.floor_on) {
       // mean node frame from the adjacent segments (nodeQuats: the starting segment is the reference)
       const int is = hasStart ? t : t - 1;
       double acc[4] = {0, 0, 0, 0};
@@ -978,12 +980,37 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
         double sg = dd < 0 ? -1.0 : 1.0;
         acc[0] += sg * L.Qw[t - 1]; acc[1] += sg * L.Qx[t - 1]; acc[2] += sg * L.Qy[t - 1]; acc[3] += sg * L.Qz[t - 1];
       }
-      double Rn[9];
       qframe_d(acc[0], acc[1], acc[2], acc[3], Rn);
-      for (int c = 0; c < 3; c++) ld[c] += Real(Rn[c]) * llx + Real(Rn[3 + c]) * lly + Real(Rn[6 + c]) * llz;
     }
+    if (localLoad) for (int c = 0; c < 3; c++) ld[c] += Real(Rn[c]) * llx + Real(Rn[3 + c]) * lly + Real(Rn[6 + c]) * llz;
     mxL = fmax(fabs(double(ld[0])), fmax(fabs(double(ld[1])), fabs(double(ld[2]))));
     for (int c = 0; c < 3; c++) F[c] += ld[c] + Fc[c];
+    if (I.floor_on) {
+      // the ground (Floor.apply, solver_cpu.js:583-627): penalty against the plane under the section's lowest
+      // point, Coulomb friction against a stick point that follows a sliding node.  In f64 (positions, stick
+      // points); the output pass recomputes the force from the moved stick point, which gives the same force.
+      const int rampN = max(I.ramp_iter, 0);
+      const double rmp = rampN <= 0 ? 1.0 : fmin(1.0, 0.01 + 0.99 * iter / rampN);
+      const double kn = rmp * I.strength, kt = rmp * I.stick;
+      const double pen = (I.floor_z - L.Xz[t]) + (0.5 * double(gl(I.nodew)[i]) * fabs(Rn[2]) + 0.5 * double(gl(I.nodeh)[i]) * fabs(Rn[5]));
+      double fg[3] = {0, 0, 0};
+      if (pen > 0) {
+        const double fn = kn * pen;
+        double ax = gl(I.fax)[i], ay = gl(I.fay)[i];
+        const double sx = L.Xx[t] - ax, sy = L.Xy[t] - ay;
+        const double Ftv = kt * sqrt(sx * sx + sy * sy), limit = I.floor_mu * fn;
+        const double scale = Ftv > limit ? 1.0 - limit / fmax(Ftv, 1e-12) : 0.0;
+        const double dax = sx * scale, day = sy * scale;
+        if (!OUT && scale > 0) { gl(I.fax)[i] = ax + dax; gl(I.fay)[i] = ay + day; }
+        const double ex = sx - dax, ey = sy - day;
+        fg[0] = -kt * ex; fg[1] = -kt * ey; fg[2] = fn;
+        eG = 0.5 * kn * pen * pen + 0.5 * kt * (ex * ex + ey * ey);
+        mxG = fn;
+      } else if (!OUT) {
+        gl(I.fax)[i] = L.Xx[t]; gl(I.fay)[i] = L.Xy[t];
+      }
+      for (int c = 0; c < 3; c++) F[c] += Real(fg[c]);
+    }
     if (OUT) {
       if (fx & 8) { outR[3 * i] = -double(F[0]); outR[3 * i + 1] = -double(F[1]); outR[3 * i + 2] = -double(F[2]); }
       else { outR[3 * i] = outR[3 * i + 1] = outR[3 * i + 2] = 0.0; }
@@ -1084,27 +1111,37 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   if (OUT) return;
 
   // ---------------------------------------------------------------- partial of the tile (fixed order; reduced by k_ctrl):
-  // sums Σm|v|² + ΣI3ω², stretch, shear, bending, torsion energies | maxima |F_free|², |t_m|², and the node / segment /
-  // joint terms of F_ref: max(|load|, |N|, |M| / l̄_min) (forceRef, solver_cpu.js:843-853)
-  // Transposed through LDS: every lane leaves its eight values; then lane l of the first wavefront takes value
-  // l / 8 over lanes l % 8, l % 8 + 8, … in order, and the eight lanes of each value combine by a butterfly within
-  // the 8-lane group (xor 1, xor 2 within quads, then the half-row mirror).  Fixed order, no atomics.
-  const double fr = fmax(mxL, fmax(mxN, mxM * I.inv_jlen_min));
-  const double v[SPART] = {keN + keR, eS, eSh, eB, eT, mxF, mxT, fr};
+  // sums Σm|v|² + ΣI3ω², stretch, shear, bending, torsion energies | maxima |F_free|², |t_m|², and the node /
+  // segment / joint / ground terms of F_ref: max(|load|, |N|, |M| / l̄_min, ground Fn) (forceRef,
+  // solver_cpu.js:843-853) | the ground's energy (written only when the ground is on)
+  // Transposed through LDS: every lane leaves its values; then lane l of the first wavefront takes value l / 8
+  // over lanes l % 8, l % 8 + 8, … in order, and the eight lanes of each value combine by a butterfly within the
+  // 8-lane group (xor 1, xor 2 within quads, then the half-row mirror).  The ninth value (the ground's energy) is
+  // reduced the same way by lanes 0–7 when the ground is on.  Fixed order, no atomics.
+  const double fr = fmax(fmax(mxL, mxG), fmax(mxN, mxM * I.inv_jlen_min));
+  const double v[SPART] = {keN + keR, eS, eSh, eB, eT, mxF, mxT, fr, eG};
   __syncthreads();                                 // the staged state is dead: its space takes the partials
 #pragma unroll
   for (int q = 0; q < SPART; q++) L.part[q][t] = v[q];
   __syncthreads();
   if (tid < 64) {
-    static_assert(SPART == 8, "one 8-lane group per partial value");
-    const int q = lane >> 3, k = lane & 7;
-    const bool sum = q < 5;
-    double x = L.part[q][k];
-    for (int i = k + 8; i < TILE; i += 8) { const double y = L.part[q][i]; x = sum ? x + y : fmax(x, y); }
-    double y = dpp_<0xB1, 0xF>(x); x = sum ? x + y : fmax(x, y);     // quad_perm [1,0,3,2]
-    y = dpp_<0x4E, 0xF>(x); x = sum ? x + y : fmax(x, y);            // quad_perm [2,3,0,1]
-    y = dpp_<0x141, 0xF>(x); x = sum ? x + y : fmax(x, y);           // row_half_mirror
+    static_assert(SPART == 9, "eight values in 8-lane groups, the ninth in a second pass");
+    const int k = lane & 7;
+    auto group = [&](int q, bool sum) {
+      double x = L.part[q][k];
+      for (int i = k + 8; i < TILE; i += 8) { const double y = L.part[q][i]; x = sum ? x + y : fmax(x, y); }
+      double y = dpp_<0xB1, 0xF>(x); x = sum ? x + y : fmax(x, y);   // quad_perm [1,0,3,2]
+      y = dpp_<0x4E, 0xF>(x); x = sum ? x + y : fmax(x, y);          // quad_perm [2,3,0,1]
+      y = dpp_<0x141, 0xF>(x); x = sum ? x + y : fmax(x, y);         // row_half_mirror
+      return x;
+    };
+    const int q = lane >> 3;
+    const double x = group(q, q < 5);
     if (k == 0) gl(I.spart)[size_t(q) * I.ntiles + tl.tile_local] = x;   // [value][tile]: coalesced for k_ctrl
+    if (I.floor_on) {
+      const double g = group(8, true);
+      if (lane == 0) gl(I.spart)[size_t(8) * I.ntiles + tl.tile_local] = g;
+    }
   }
 }
 
@@ -1134,9 +1171,10 @@ __device__ __forceinline__ void block_reduce(double (*red)[TILE / 64], double* v
 template <typename Real>
 __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restrict__ insts, const CtrlItem* __restrict__ items,
                                                       int32_t nitems, Ctl* __restrict__ ctls) {
-  // accumulator: [0] Σm|v|² + ΣI3ω², [1..4] stretch / shear / bending / torsion energy, [5] contact energy (sums);
-  // [6] max |F_free|², [7] max |t_m|², [8] max(|load|, |N|, |M|/l̄_min), [9] max Fn (maxima)
-  constexpr int NACC = RPART, NSUM = 6;
+  // accumulator: [0] Σm|v|² + ΣI3ω², [1..4] stretch / shear / bending / torsion energy, [5] ground energy,
+  // [6] contact energy (sums); [7] max |F_free|², [8] max |t_m|², [9] max(|load|, |N|, |M|/l̄_min, ground Fn),
+  // [10] max Fn (maxima)
+  constexpr int NACC = RPART, NSUM = 7;
   __shared__ double red[NACC][CTRL_THREADS / 64];
   __shared__ int last;
   if (int(blockIdx.x) >= nitems) return;
@@ -1162,21 +1200,23 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
       const bool in = w < p1;
       const auto sp = gl(I.spart) + (in ? w : p0);
 #pragma unroll
-      for (int q = 0; q < SPART; q++) x[k][q] = in ? sp[size_t(q) * np] : 0.0;
+      for (int q = 0; q < SPART - 1; q++) x[k][q] = in ? sp[size_t(q) * np] : 0.0;
+      x[k][SPART - 1] = (in && I.floor_on) ? sp[size_t(SPART - 1) * np] : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < CTRL_PER; k++) {
 #pragma unroll
       for (int q = 0; q < 5; q++) acc[q] += x[k][q];
 #pragma unroll
-      for (int q = 5; q < SPART; q++) acc[q + 1] = fmax(acc[q + 1], x[k][q]);
+      for (int q = 5; q < 8; q++) acc[q + 2] = fmax(acc[q + 2], x[k][q]);
+      acc[5] += x[k][8];
     }
   }
   const int cper = (I.ncblk + it.nchunks - 1) / it.nchunks;
   const int c0 = it.chunk * cper, c1 = min(I.ncblk, c0 + cper);
   for (int w = c0 + tid; w < c1; w += CTRL_THREADS) {
-    acc[5] += gl(I.cpart)[w];
-    acc[9] = fmax(acc[9], gl(I.cpart)[I.ncblk + w]);
+    acc[6] += gl(I.cpart)[w];
+    acc[10] = fmax(acc[10], gl(I.cpart)[I.ncblk + w]);
   }
   if (!(c.done == 0 && c.iter < c.stop_at)) return;   // uniform over the workgroup
   block_reduce<CTRL_THREADS, NACC>(red, acc, NSUM);
@@ -1200,7 +1240,7 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
     c.counter = 0;                                 // the other workgroups change nothing else in the record
   }
   if (tid != 0) return;
-  const double eCont = acc[5];
+  const double eCont = acc[6], eGround = acc[5];
   double ke = 0.5 * acc[0];
   int rollback = 0;
   if (I.kin_damp && ke < c.ke_prev) { rollback = 1; ke = 0.0; }
@@ -1216,11 +1256,11 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   } else {
     c.al_update = 0;
   }
-  double fref = acc[8];
-  if (I.contacts) fref = fmax(fref, acc[9]);
+  double fref = acc[9];
+  if (I.contacts) fref = fmax(fref, acc[10]);
   fref = fmax(fref, 1e-12);
   // the strand partials carry max |F_free|² and max |t_m|²: the square root of the maximum is the maximum norm
-  const double rf = I.n ? sqrt(acc[6]) / fref : 0.0, rt = I.m ? sqrt(acc[7]) / (fref * I.l_ref) : 0.0;
+  const double rf = I.n ? sqrt(acc[7]) / fref : 0.0, rt = I.m ? sqrt(acc[8]) / (fref * I.l_ref) : 0.0;
   double resid = fmax(rf, rt);
   if (!(c.iter > I.warmup)) resid = INFINITY;
   c.resid = resid; c.fref = fref; c.res_f = rf; c.res_t = rt;
@@ -1230,7 +1270,7 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   c.live_flag = 1;   // an iteration has been evaluated
   if (I.hist_len > 0) {
     const auto h = gl(I.hist) + size_t((c.iter - 1) % I.hist_len) * HIST_STRIDE;
-    h[0] = ke; h[1] = acc[1] + acc[2] + acc[3] + acc[4] + eCont; h[2] = acc[3]; h[3] = acc[4];
+    h[0] = ke; h[1] = acc[1] + acc[2] + acc[3] + acc[4] + eCont + eGround; h[2] = acc[3]; h[3] = acc[4];
     h[4] = acc[1]; h[5] = acc[2]; h[6] = eCont; h[7] = resid; h[8] = double(c.iter);
   }
   c.hist_n += 1;
