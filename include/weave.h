@@ -171,6 +171,9 @@ typedef struct ws_batch_opts {
   int32_t history;         /* history ring length per instance (0 = 4096, the reference's HN) */
   int32_t ctrl_chunk;      /* strand partials per control workgroup (0 = by size: one workgroup up to 4096, else 2048
                               each, the last to finish combining them) */
+  int32_t cross_lanes;     /* lanes per crossing in the contact kernel: 2 (the sides split, shorter chains), 1 (one lane
+                              takes both sides, fewer instructions per crossing), 0 = by size (1 from 131,072
+                              crossings in the batch) or the autotuner's pick */
 } ws_batch_opts;
 
 #define WS_TILE_AUTOTUNE (-1)
@@ -181,6 +184,7 @@ typedef struct ws_tune_entry {
   int32_t per_submit;
   double ms_per_iter;
   double node_iters_per_s;
+  int32_t cross_lanes;
 } ws_tune_entry;
 
 typedef struct ws_ctx ws_ctx;
@@ -223,8 +227,9 @@ WS_API int ws_batch_kernel_bench(ws_batch* batch, int32_t reps, double* ms_cross
 /* the options in effect (the tile chosen when the batch was created with tile 0 or WS_TILE_AUTOTUNE) */
 WS_API int ws_batch_get_opts(ws_batch* batch, ws_batch_opts* out);
 /* autotuner (no counterpart in the reference): relaxes the inputs `iters` iterations under every strand tile
- * (64 / 128 / 256) and graph length (32 / 64 / 128 iterations), on temporary batches, and returns the fastest in
- * node-iterations per second; `table` (optional, `cap` rows) receives every candidate */
+ * (64 / 128 / 256) and graph length (32 / 64 / 128 iterations), then the other crossing layout (one or two lanes
+ * per crossing) under the best of those, on temporary batches, and returns the fastest in node-iterations per
+ * second; `table` (optional, `cap` rows) receives every candidate */
 WS_API int ws_autotune(ws_ctx* ctx, const ws_model* const* models, const ws_state* const* states, int32_t B,
                        int32_t precision, int32_t iters, ws_batch_opts* best, ws_tune_entry* table, int32_t cap,
                        int32_t* n_entries);

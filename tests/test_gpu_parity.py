@@ -20,9 +20,10 @@ class Snap:
         self.iteration, self.residual, self.converged = o.iteration, o.residual, o.converged
 
 
-def run_pair(W, model, iters, precision):
+def run_pair(W, model, iters, precision, cross_lanes=0):
     o = Oracle(model)
-    g = W.GpuSolver([model], precision=precision)
+    g = W.GpuSolver([model], precision=precision, cross_lanes=cross_lanes)
+    assert cross_lanes == 0 or g.opts["cross_lanes"] == cross_lanes
     done = 0
     out = []
     for n in iters:
@@ -66,13 +67,15 @@ def test_cantilever_converged_matches_oracle_and_timoshenko(W, gpu_ctx, precisio
     g.close()
 
 
+@pytest.mark.parametrize("lanes", [1, 2])
 @pytest.mark.parametrize("name", ["cross_press", "cross_stick", "cross_locked", "cross_slip"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_crossing_cases_match_oracle(W, gpu_ctx, name, precision):
-    """the reference's single-crossing cases: same stick/slip state, same friction force, positions within 1e-4 span"""
+def test_crossing_cases_match_oracle(W, gpu_ctx, name, precision, lanes):
+    """the reference's single-crossing cases under both crossing layouts (one lane per crossing side, one lane per
+    crossing): same stick/slip state, same friction force, positions within 1e-4 span"""
     m = cases.crossing(tol=1e-6, **cases.CROSS_CASES[name])
     its = 20000 if name != "cross_slip" else 3000
-    (n, a, b, o, c), = run_pair(W, m, [its], precision)
+    (n, a, b, o, c), = run_pair(W, m, [its], precision, lanes)
     assert a["kstate"][0] == b["kstate"][0] and a["stuck"][0] == b["stuck"][0]
     assert np.abs(a["X"] - b["X"]).max() <= 1e-4 * 1000.0
     assert abs(a["Ft"][0] - b["Ft"][0]) <= 1e-3 * max(1.0, a["Ft"][0])
@@ -81,12 +84,14 @@ def test_crossing_cases_match_oracle(W, gpu_ctx, name, precision):
         assert bool(c["done"]) == o.converged
 
 
+@pytest.mark.parametrize("lanes", [1, 2])
 @pytest.mark.parametrize("precision,tol,window", [("fp64", 1e-9, [1, 100, 300, 600]), ("fp32", 1e-4, [1, 100, 300, 600])])
-def test_dome_trajectory_window(W, gpu_ctx, precision, tol, window):
-    """config-1 dome (1.4k nodes, 240 crossings) before chaos: positions, contact parameters, stick/slip"""
+def test_dome_trajectory_window(W, gpu_ctx, precision, tol, window, lanes):
+    """config-1 dome (1.4k nodes, 240 crossings) before chaos, under both crossing layouts: positions, contact
+    parameters, stick/slip"""
     m = cases.config1()
     span = m.span
-    for n, a, b, o, c in run_pair(W, m, window, precision):
+    for n, a, b, o, c in run_pair(W, m, window, precision, lanes):
         assert np.abs(a["X"] - b["X"]).max() <= tol * span, (n, np.abs(a["X"] - b["X"]).max())
         assert np.array_equal(a["stuck"], b["stuck"]), n
         assert c["iter"] == o.iteration == n
@@ -191,14 +196,15 @@ BRANCHES = {
 }
 
 
-@pytest.mark.parametrize("branch", sorted(BRANCHES))
+@pytest.mark.parametrize("branch,lanes", [(b, 2) for b in sorted(BRANCHES)] + [("all_three", 1)])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 1e-4)])
-def test_default_off_branches_match_oracle(W, gpu_ctx, branch, precision, tol):
+def test_default_off_branches_match_oracle(W, gpu_ctx, branch, lanes, precision, tol):
     """W2 rotational friction, W3 seat law and the AL multipliers (default off in every workload) on the sliding
-    single crossing and on the small dome's pre-chaos window: positions, contact state and the branch's own state"""
+    single crossing and on the small dome's pre-chaos window: positions, contact state and the branch's own state
+    (all three together also under one lane per crossing)"""
     kw = BRANCHES[branch]
     m = cases.crossing(tol=1e-9, **dict(cases.CROSS_CASES["cross_slip"], **kw))
-    (n, a, b, o, c), = run_pair(W, m, [2500], precision)
+    (n, a, b, o, c), = run_pair(W, m, [2500], precision, lanes)
     dX = np.abs(a["X"] - b["X"]).max()
     assert dX <= tol * 1000.0
     assert np.array_equal(a["stuck"], b["stuck"]) and np.array_equal(a["kstate"], b["kstate"])
@@ -218,7 +224,7 @@ def test_default_off_branches_match_oracle(W, gpu_ctx, branch, precision, tol):
     eps = 1e-12 if precision == "fp64" else 1e-7
     p["X"][:] += eps * d.span * np.random.default_rng(0).standard_normal(3 * d.n)
     op = Oracle(p)
-    for n, a, b, o, c in run_pair(W, d, [100, 400, 600], precision):
+    for n, a, b, o, c in run_pair(W, d, [100, 400, 600], precision, lanes):
         op.step(n - op.iteration)
         spread = np.abs(a["X"] - op.state()["X"]).max()
         err = np.abs(a["X"] - b["X"]).max()

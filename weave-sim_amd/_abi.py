@@ -66,14 +66,16 @@ class ws_control(C.Structure):
 
 
 class ws_batch_opts(C.Structure):
-    _fields_ = [("precision", i32), ("per_submit", i32), ("tile", i32), ("history", i32), ("ctrl_chunk", i32)]
+    _fields_ = [("precision", i32), ("per_submit", i32), ("tile", i32), ("history", i32), ("ctrl_chunk", i32),
+                ("cross_lanes", i32)]
 
 
 WS_TILE_AUTOTUNE = -1
 
 
 class ws_tune_entry(C.Structure):
-    _fields_ = [("tile", i32), ("per_submit", i32), ("ms_per_iter", d), ("node_iters_per_s", d)]
+    _fields_ = [("tile", i32), ("per_submit", i32), ("ms_per_iter", d), ("node_iters_per_s", d),
+                ("cross_lanes", i32)]
 
 
 class ws_server_opts(C.Structure):

@@ -23,7 +23,7 @@
 
 namespace wsd {
 template <typename Real> __global__ void k_begin(Ctl*, const int32_t*, const int32_t*, int32_t, int32_t, int32_t);
-template <typename Real> __global__ void k_cross(const Inst<Real>*, const CrossItem*, int32_t, const Ctl*);
+template <typename Real, int LANES> __global__ void k_cross(const Inst<Real>*, const CrossItem*, int32_t, const Ctl*);
 template <typename Real, int TILE> __global__ void k_strand(const Inst<Real>*, const TileItem*, int32_t, const Ctl*);
 template <typename Real> __global__ void k_ctrl(const Inst<Real>*, const CtrlItem*, int32_t, Ctl*);
 __global__ void k_copy(const uint4*, uint4*, int64_t);
@@ -153,7 +153,7 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
     }
   }
   d.ntiles = int(H.tiles.size());
-  H.ncblk = (K + CROSS_BLOCK - 1) / CROSS_BLOCK;
+  H.ncblk = (K + CROSS_BLOCK - 1) / CROSS_BLOCK;   // at two lanes per crossing: the most blocks a launch uses
   d.ncblk = H.ncblk;
   // ---- arena layout
   Arena& A = H.arena;
@@ -333,6 +333,8 @@ struct BatchBase {
   ws_ctx* ctx = nullptr;
   int B = 0, precision = 0, tile = 128, per_submit = 64, hist_len = 4096;
   int ctrl_chunk = 0;                   // strand partials per control workgroup (0: by size)
+  int cross_lanes = 0;                  // lanes per crossing (0: by size at every relist)
+  int lanes = 2;                        // ... as the current work lists use it
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -460,13 +462,18 @@ struct Batch : BatchBase {
     tiles.clear();
     std::vector<CrossItem> cross;
     std::vector<CtrlItem> ctrl;
+    long total_k = 0;
+    for (int b = 0; b < B; b++) if (inst[b]) total_k += inst[b]->K;
+    lanes = cross_lanes > 0 ? cross_lanes : (total_k >= CROSS_ONE_LANE ? 1 : 2);
+    const int per_blk = CROSS_THREADS / lanes;
     for (int b = 0; b < B; b++) {
       if (!inst[b]) continue;
       InstHost<Real>& H = at(b);
       H.d.tile0 = int(tiles.size());
       for (auto t : H.tiles) { t.inst = b; tiles.push_back(t); }
       H.d.cblk0 = int(cross.size());
-      for (int q = 0; q < H.ncblk; q++) cross.push_back(CrossItem{b, q * CROSS_BLOCK, std::min(CROSS_BLOCK, H.K - q * CROSS_BLOCK), q});
+      H.d.ncblk = (H.K + per_blk - 1) / per_blk;   // cpart holds H.ncblk ≥ this many blocks
+      for (int q = 0; q < H.d.ncblk; q++) cross.push_back(CrossItem{b, q * per_blk, std::min(per_blk, H.K - q * per_blk), q});
       for (int q = 0; q < H.d.ctrl_chunks; q++) ctrl.push_back(CtrlItem{b, q, H.d.ctrl_chunks, 0});
     }
     ntiles = int(tiles.size()); ncross = int(cross.size()); nctrl = int(ctrl.size());
@@ -482,7 +489,9 @@ struct Batch : BatchBase {
 
 
   void launch_cross(hipStream_t s) {
-    if (ncross) hipLaunchKernelGGL(k_cross<Real>, dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls);
+    if (!ncross) return;
+    if (lanes == 1) hipLaunchKernelGGL((k_cross<Real, 1>), dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls);
+    else hipLaunchKernelGGL((k_cross<Real, 2>), dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls);
   }
   void launch_strand(hipStream_t s) {
     switch (tile) {
@@ -809,13 +818,15 @@ int fail(ws_ctx* ctx, const Fail& f) { if (ctx) ctx->err = f.msg; return f.code;
 
 // the batch of the given options (tile already resolved to 64 / 128 / 256)
 std::unique_ptr<BatchBase> make_batch(ws_ctx* ctx, const ws_model* const* models, const ws_state* const* states, int B,
-                                      int prec, int tile, int per_submit, int history, int ctrl_chunk = 0) {
+                                      int prec, int tile, int per_submit, int history, int ctrl_chunk = 0,
+                                      int cross_lanes = 0) {
   std::unique_ptr<BatchBase> b;
   if (prec == WS_FP64) b.reset(new Batch<double>()); else b.reset(new Batch<float>());
   b->ctx = ctx; b->precision = prec;
   if (per_submit > 0) b->per_submit = per_submit;
   if (history > 0) b->hist_len = history;
   b->ctrl_chunk = std::max(ctrl_chunk, 0);
+  b->cross_lanes = cross_lanes;
   b->tile = tile;
   if (prec == WS_FP64) static_cast<Batch<double>*>(b.get())->create(models, states, B);
   else static_cast<Batch<float>*>(b.get())->create(models, states, B);
@@ -846,8 +857,22 @@ void autotune(ws_ctx* ctx, const ws_model* const* models, const ws_state* const*
     for (int q = 0; q < B; q++) node_iters += double(models[q]->n) * double(c1[q].iter - c0[q].iter);
     const double ms = b->ms_last;
     const double rate = ms > 0 ? node_iters / (ms * 1e-3) : 0;
-    table.push_back(ws_tune_entry{t, ps, ms / n, rate});
-    if (rate > best_rate) { best_rate = rate; best.tile = t; best.per_submit = ps; }
+    table.push_back(ws_tune_entry{t, ps, ms / n, rate, b->lanes});
+    if (rate > best_rate) { best_rate = rate; best.tile = t; best.per_submit = ps; best.cross_lanes = b->lanes; }
+  }
+  // the other crossing layout under the best tile and graph length
+  {
+    const int other = best.cross_lanes == 1 ? 2 : 1, t = best.tile, ps = best.per_submit;
+    auto b = make_batch(ctx, models, states, B, prec, t, ps, std::max(history, 1), 0, other);
+    b->step(2 * ps, nullptr, 0, 1, c0.data());
+    const int n = std::max(ps, (iters + ps - 1) / ps * ps);
+    b->step(n, nullptr, 0, 1, c1.data());
+    double node_iters = 0;
+    for (int q = 0; q < B; q++) node_iters += double(models[q]->n) * double(c1[q].iter - c0[q].iter);
+    const double ms = b->ms_last;
+    const double rate = ms > 0 ? node_iters / (ms * 1e-3) : 0;
+    table.push_back(ws_tune_entry{t, ps, ms / n, rate, other});
+    if (rate > best_rate) { best_rate = rate; best.cross_lanes = other; }
   }
   best.precision = prec;
   best.history = history;

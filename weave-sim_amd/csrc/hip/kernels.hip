@@ -1,6 +1,6 @@
 // HIP kernels (gfx950) of the dynamic-relaxation iteration.  One iteration is two launches:
 //
-//   k_cross   one thread per crossing: node tangents and mean node frames around both sides' contact points,
+//   k_cross   one or two lanes per crossing: node tangents and mean node frames around both sides' contact points,
 //             C¹ Hermite point / tangent / frame on each side, contact normal with side persistence, penalty normal
 //             force, Coulomb stick / slip with the u advance, W1 classification, W2 / W3 / AL branches; writes the
 //             crossing state and one record per crossing side (force, lever torque, righting stiffness) — no atomics.
@@ -365,15 +365,19 @@ __device__ __forceinline__ double xchg(double v) { return __shfl_xor(v, 1, 64); 
 __device__ __forceinline__ float xchg(float v) { return __shfl_xor(v, 1, 64); }
 __device__ __forceinline__ int xchg(int v) { return __shfl_xor(v, 1, 64); }
 
-// Two lanes per crossing: lane 2j evaluates side A of crossing c0 + j, lane 2j + 1 side B.  The side evaluations
-// (the neighbour loads, tangents, node frames and the Hermite point) run in parallel; the lanes then swap the
-// partner's point, tangent and frame and both compute the crossing's normal, penetration and friction law (the same
-// instructions on the same data, so the two copies agree bit for bit).  Each lane writes its own side's u and
-// record; lane 2j writes the crossing's state.  Contact kinematics run in f64 in both forms (the penetration
+// LANES = 2: lane 2j evaluates side A of crossing c0 + j, lane 2j + 1 side B.  The side evaluations (the neighbour
+// loads, tangents, node frames and the Hermite point) run in parallel; the lanes then swap the partner's point,
+// tangent and frame and both compute the crossing's normal, penetration and friction law (the same instructions on
+// the same data, so the two copies agree bit for bit).  Each lane writes its own side's u and record; lane 2j writes
+// the crossing's state.  Short dependency chains for batches too small to fill the chip.
+// LANES = 1: lane j evaluates both sides of crossing c0 + j one after the other and writes everything: half the
+// instructions per crossing, for batches with enough crossings to fill the chip.  Both layouts run the same
+// arithmetic on each side, so they give the same results.  Contact kinematics run in f64 in both forms (the penetration
 // D − g cancels mm-scale lengths); the classification, the lever arms and the W2 / W3 trigonometry run in Real.
-template <typename Real>
-__global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __restrict__ insts, const CrossItem* __restrict__ items,
+template <typename Real, int LANES>
+__global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_cross(const Inst<Real>* __restrict__ insts, const CrossItem* __restrict__ items,
                                                         int32_t nitems, const Ctl* __restrict__ ctls) {
+  static_assert(LANES == 1 || LANES == 2, "one or two lanes per crossing");
   using G = double;
   using H = Real;
   int bid = blockIdx.x;
@@ -387,7 +391,7 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
   const bool al_update = ctl->al_update != 0;
   const int pen_n = ctl->pen_n;
   const int tid = threadIdx.x;
-  const int side = tid & 1, jj = tid >> 1;
+  const int side = LANES == 2 ? (tid & 1) : 0, jj = LANES == 2 ? (tid >> 1) : tid;
   const bool valid = jj < it.count;
   const int c = it.c0 + (valid ? jj : 0);     // idle lanes shadow the block's first crossing and write nothing
   const bool lead = valid && side == 0;
@@ -401,17 +405,22 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
   const bool seat = I.seat != 0, aug = I.aug != 0;
   const G mu0 = G(I.mu);
   // ---- own side, relative to the first node of A's segment
-  const double uA = gl(I.u)[2 * c];
-  const double uOwn = side == 0 ? uA : gl(I.u)[2 * c + 1];
+  const double uA = gl(I.u)[2 * c], uB = gl(I.u)[2 * c + 1];
+  const double uOwn = side == 0 ? uA : uB;
   const int o = gl(I.k_noff)[2 * c] + int(fmin(fmax(floor(uA), 0.0), double(max(gl(I.k_nseg)[2 * c] - 1, 0))));
   const double ox = gl(I.x[cur])[o], oy = gl(I.y[cur])[o], oz = gl(I.z[cur])[o];
-  SideEv<G> E;
+  SideEv<G> E, E2;                                // own side; side B when one lane takes both
   eval_side(I, cur, c, side, uOwn, ox, oy, oz, E);
+  if (LANES == 1) eval_side(I, cur, c, 1, uB, ox, oy, oz, E2);
   // ---- partner's side
-  G oP[3], oT[3], oR[6], oL;
-  for (int k = 0; k < 3; k++) { oP[k] = xchg(E.P[k]); oT[k] = xchg(E.tg[k]); }
-  for (int k = 0; k < 6; k++) oR[k] = xchg(E.R[k]);
-  oL = xchg(E.L);
+  G oP[3], oT[3], oR[6];
+  if (LANES == 2) {
+    for (int k = 0; k < 3; k++) { oP[k] = xchg(E.P[k]); oT[k] = xchg(E.tg[k]); }
+    for (int k = 0; k < 6; k++) oR[k] = xchg(E.R[k]);
+  } else {
+    for (int k = 0; k < 3; k++) { oP[k] = E2.P[k]; oT[k] = E2.tg[k]; }
+    for (int k = 0; k < 6; k++) oR[k] = E2.R[k];
+  }
   G AP[3], BP[3], tA[3], tB[3], AR[6], BR[6];
   for (int k = 0; k < 3; k++) {
     AP[k] = side == 0 ? E.P[k] : oP[k]; BP[k] = side == 0 ? oP[k] : E.P[k];
@@ -501,14 +510,6 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
   G eps = dmax(G(I.epsilon), G(1e-9));
   G da = clampv(ca * sc, -eps, eps), db = clampv(cb * sc, -eps, eps);
   bool moved = slip || fre;
-  // own side's parameter after the slip
-  const int nsOwn = gl(I.k_nseg)[2 * c + side];
-  double uNew = uOwn;
-  if (moved) {
-    const double iL = rcp_(double(dmax(E.L, G(1e-12))));
-    double du = side == 0 ? double(da) * iL : -double(db) * iL;
-    uNew = fmin(fmax(uOwn + du, 0.0), double(nsOwn));
-  }
   G Se[3] = {0, 0, 0};
   if (act) { Se[0] = St[0] - da * tA[0] - db * tB[0]; Se[1] = St[1] - da * tA[1] - db * tB[1]; Se[2] = St[2] - da * tA[2] - db * tB[2]; }
   G fT[3] = {kt * Se[0], kt * Se[1], kt * Se[2]};
@@ -618,18 +619,29 @@ __global__ __launch_bounds__(CROSS_THREADS) void k_cross(const Inst<Real>* __res
   // ---- righting stiffness (addTwistStiffness, :170-176) on the own side's segment after the slip
   G tfk = dmax(G(I.torsion_factor), G(1e-3));
   G kth = tfk * ((Fn + kn * G(0.25) * G(gl(I.kh)[c])) * (G(0.5) * G(gl(I.kw)[c])) / G(SUPPORT_SMOOTH) + kt * (G(0.5) * G(gl(I.kh)[c])) * (G(0.5) * G(gl(I.kh)[c])));
-  if (valid) {
-    // own side: parameter, record
-    if (moved) gl(I.u)[2 * c + side] = uNew;
-    SideRec<Real> rec;
-    rec.seg_pre = E.loc;
-    rec.seg_post = int(fmin(fmax(floor(uNew), 0.0), double(max(nsOwn - 1, 0))));
-    rec.t = Real(E.t); rec.kth = Real(kth);
-    for (int k = 0; k < 3; k++) {
-      rec.F[k] = Real(side == 0 ? FA[k] : FB[k]);
-      rec.T[k] = Real(side == 0 ? TA[k] : TB[k]);
+  // a side's parameter after the slip and its record
+  auto finish_side = [&](int sd, const SideEv<G>& Es, double uOld) {
+    const int nsS = gl(I.k_nseg)[2 * c + sd];
+    double uNew = uOld;
+    if (moved) {
+      const double iL = rcp_(double(dmax(Es.L, G(1e-12))));
+      const double du = sd == 0 ? double(da) * iL : -double(db) * iL;
+      uNew = fmin(fmax(uOld + du, 0.0), double(nsS));
+      gl(I.u)[2 * c + sd] = uNew;
     }
-    side_store(rec, gl(I.side) + gl(I.side_pos)[2 * c + side]);
+    SideRec<Real> rec;
+    rec.seg_pre = Es.loc;
+    rec.seg_post = int(fmin(fmax(floor(uNew), 0.0), double(max(nsS - 1, 0))));
+    rec.t = Real(Es.t); rec.kth = Real(kth);
+    for (int k = 0; k < 3; k++) {
+      rec.F[k] = Real(sd == 0 ? FA[k] : FB[k]);
+      rec.T[k] = Real(sd == 0 ? TA[k] : TB[k]);
+    }
+    side_store(rec, gl(I.side) + gl(I.side_pos)[2 * c + sd]);
+  };
+  if (valid) {
+    finish_side(side, E, uOwn);
+    if (LANES == 1) finish_side(1, E2, uB);
   }
   if (lead) {
     gl(I.nprev)[3 * c] = Real(n[0]); gl(I.nprev)[3 * c + 1] = Real(n[1]); gl(I.nprev)[3 * c + 2] = Real(n[2]);
@@ -1334,8 +1346,10 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
   template __global__ void k_outputs<R, T>(const Inst<R>*, const TileItem*, int32_t, double*, double*, double*, double*, double*);
 WSD_INST(float, 64) WSD_INST(float, 128) WSD_INST(float, 256)
 WSD_INST(double, 64) WSD_INST(double, 128) WSD_INST(double, 256)
-template __global__ void k_cross<float>(const Inst<float>*, const CrossItem*, int32_t, const Ctl*);
-template __global__ void k_cross<double>(const Inst<double>*, const CrossItem*, int32_t, const Ctl*);
+template __global__ void k_cross<float, 1>(const Inst<float>*, const CrossItem*, int32_t, const Ctl*);
+template __global__ void k_cross<float, 2>(const Inst<float>*, const CrossItem*, int32_t, const Ctl*);
+template __global__ void k_cross<double, 1>(const Inst<double>*, const CrossItem*, int32_t, const Ctl*);
+template __global__ void k_cross<double, 2>(const Inst<double>*, const CrossItem*, int32_t, const Ctl*);
 template __global__ void k_ctrl<float>(const Inst<float>*, const CtrlItem*, int32_t, Ctl*);
 template __global__ void k_ctrl<double>(const Inst<double>*, const CtrlItem*, int32_t, Ctl*);
 template __global__ void k_al_preview<float>(const Inst<float>*, int32_t, double*);

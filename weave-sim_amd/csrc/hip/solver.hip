@@ -70,14 +70,17 @@ WS_API int ws_batch_create(ws_ctx* ctx, const ws_model* const* models, const ws_
     int t = opts ? opts->tile : 0;
     if (t != WS_TILE_AUTOTUNE && t != 0 && t != 64 && t != 128 && t != 256)
       throw Fail{WS_ERR_ARG, "tile must be 0 (by size), -1 (autotune), 64, 128 or 256"};
+    int lanes = opts ? opts->cross_lanes : 0;
+    if (lanes < 0 || lanes > 2) throw Fail{WS_ERR_ARG, "cross_lanes must be 0 (by size), 1 or 2"};
     if (t == 0) t = size_tile(models, B);
     if (t == WS_TILE_AUTOTUNE) {
       std::vector<ws_tune_entry> table;
       ws_batch_opts best{};
       autotune(ctx, models, states, B, prec, 256, 16, table, best);
       t = best.tile; if (ps <= 0) ps = best.per_submit;
+      if (lanes == 0) lanes = best.cross_lanes;
     }
-    *out = new ws_batch{make_batch(ctx, models, states, B, prec, t, ps, hist, opts ? opts->ctrl_chunk : 0)};
+    *out = new ws_batch{make_batch(ctx, models, states, B, prec, t, ps, hist, opts ? opts->ctrl_chunk : 0, lanes)};
     return WS_OK;
   } catch (const Fail& f) { return fail(ctx, f); }
 }
@@ -103,6 +106,7 @@ WS_API int ws_batch_get_opts(ws_batch* batch, ws_batch_opts* out) {
   if (!batch || !out) return WS_ERR_ARG;
   out->precision = batch->impl->precision; out->per_submit = batch->impl->per_submit;
   out->tile = batch->impl->tile; out->history = batch->impl->hist_len; out->ctrl_chunk = batch->impl->ctrl_chunk;
+  out->cross_lanes = batch->impl->lanes;
   return WS_OK;
 }
 

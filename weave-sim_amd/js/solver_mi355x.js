@@ -59,7 +59,8 @@ class GpuSolver {
     this.B = models.length;
     const precision = opts.precision === 'fp64' ? FP64 : FP32_FP64;
     this.h = native().batchCreate(device.ctx, models.map((m) => m.h),
-      { precision, perSubmit: opts.perSubmit || 0, tile: opts.tile === 'autotune' ? -1 : (opts.tile || 0), history: opts.history || 0 });
+      { precision, perSubmit: opts.perSubmit || 0, tile: opts.tile === 'autotune' ? -1 : (opts.tile || 0), history: opts.history || 0,
+        crossLanes: opts.crossLanes || 0 });
     this.perSubmit = native().batchOpts(this.h).perSubmit;
     this.ctlHost = new Array(this.B).fill(null);
     this._chain = Promise.resolve();

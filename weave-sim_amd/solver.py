@@ -68,7 +68,7 @@ def _ctl_dict(c):
 
 
 def autotune(models, precision="fp32", iters=256, ctx=None):
-    """the autotuner's pick (tile, per_submit) for these instances and every candidate's rate"""
+    """the autotuner's pick (tile, per_submit, cross_lanes) for these instances and every candidate's rate"""
     ctx = ctx or default_context()
     lib = ctx.lib
     B = len(models)
@@ -78,17 +78,19 @@ def autotune(models, precision="fp32", iters=256, ctx=None):
     table = (A.ws_tune_entry * 16)()
     n = C.c_int32()
     ctx.check(lib.ws_autotune(ctx.h, mp, sp, B, PRECISIONS[precision], int(iters), C.byref(best), table, 16, C.byref(n)))
-    rows = [{"tile": t.tile, "per_submit": t.per_submit, "ms_per_iter": t.ms_per_iter,
+    rows = [{"tile": t.tile, "per_submit": t.per_submit, "cross_lanes": t.cross_lanes, "ms_per_iter": t.ms_per_iter,
              "node_iters_per_s": t.node_iters_per_s} for t in table[:n.value]]
-    return {"tile": best.tile, "per_submit": best.per_submit}, rows
+    return {"tile": best.tile, "per_submit": best.per_submit, "cross_lanes": best.cross_lanes}, rows
 
 
 class GpuSolver:
     """B instances (each its own topology allowed) relaxed in the same launches."""
 
-    def __init__(self, models, precision="fp32", tile=0, per_submit=0, history=4096, ctx=None, states=None, ctrl_chunk=0):
+    def __init__(self, models, precision="fp32", tile=0, per_submit=0, history=4096, ctx=None, states=None, ctrl_chunk=0,
+                 cross_lanes=0):
         """tile: 64 / 128 / 256 lanes per strand tile, 0 = chosen by size, "autotune" (or -1) = the autotuner's pick;
-        per_submit: iterations per captured graph (0 = 64, or the autotuner's pick)"""
+        per_submit: iterations per captured graph (0 = 64, or the autotuner's pick);
+        cross_lanes: lanes per crossing in the contact kernel, 1 or 2 (0 = by size, or the autotuner's pick)"""
         self.ctx = ctx or default_context()
         self.lib = self.ctx.lib
         self.models = list(models)
@@ -99,18 +101,19 @@ class GpuSolver:
         sp = (C.POINTER(A.ws_state) * self.B)(*[C.pointer(s.st) for s in sts])
         if tile == "autotune":
             tile = A.WS_TILE_AUTOTUNE
-        opts = A.ws_batch_opts(PRECISIONS[precision], per_submit, tile, history, ctrl_chunk)
+        opts = A.ws_batch_opts(PRECISIONS[precision], per_submit, tile, history, ctrl_chunk, cross_lanes)
         self.h = C.c_void_p()
         self.ctx.check(self.lib.ws_batch_create(self.ctx.h, mp, sp, self.B, C.byref(opts), C.byref(self.h)))
         self._ctl = (A.ws_control * self.B)()
 
     @property
     def opts(self):
-        """options in effect: precision, per_submit, tile (as chosen by size or by the autotuner), history"""
+        """options in effect: precision, per_submit, tile and cross_lanes (as chosen by size or by the autotuner),
+        history"""
         o = A.ws_batch_opts()
         self.ctx.check(self.lib.ws_batch_get_opts(self.h, C.byref(o)))
         return {"precision": o.precision, "per_submit": o.per_submit, "tile": o.tile, "history": o.history,
-                "ctrl_chunk": o.ctrl_chunk}
+                "ctrl_chunk": o.ctrl_chunk, "cross_lanes": o.cross_lanes}
 
     @property
     def tile(self):
