@@ -263,7 +263,7 @@ def main():
     ap.add_argument("--workload", default="dome_10m", choices=sorted(WORKLOADS))
     ap.add_argument("--batch", type=int, default=64, help="instances of the variants workload")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--tile", type=int, default=0, help="strand tile (64/128/256); 0 = by size")
+    ap.add_argument("--tile", type=int, default=0, help="strand tile (64/128/256); 0 = the default (64)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)

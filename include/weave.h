@@ -167,9 +167,9 @@ typedef struct ws_control {
 typedef struct ws_batch_opts {
   int32_t precision;       /* ws_precision */
   int32_t per_submit;      /* iterations per captured graph (GpuSolver perSubmit, default 64) */
-  int32_t tile;            /* lanes per strand tile: 64 / 128 / 256; 0 = chosen by size; -1 = autotune */
+  int32_t tile;            /* lanes per strand tile: 64 / 128 / 256; 0 = default (64); -1 = autotune */
   int32_t history;         /* history ring length per instance (0 = 4096, the reference's HN) */
-  int32_t ctrl_chunk;      /* strand partials per control workgroup (0 = by size: one workgroup up to 4096, else 2048
+  int32_t ctrl_chunk;      /* strand partials per control workgroup (0 = by size: one workgroup up to 2048, else 1024
                               each, the last to finish combining them) */
   int32_t cross_lanes;     /* lanes per crossing in the contact kernel: 2 (the sides split, shorter chains), 1 (one lane
                               takes both sides, fewer instructions per crossing), 0 = by size (1 from 131,072

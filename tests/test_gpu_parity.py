@@ -236,7 +236,7 @@ def test_default_off_branches_match_oracle(W, gpu_ctx, branch, lanes, precision,
 
 
 def test_control_reduction_over_several_workgroups(W, gpu_ctx):
-    """the 10 m dome in strand tiles of 64 leaves 1,248 wavefront partials, reduced here by three control workgroups
+    """the 10 m dome in strand tiles of 64 leaves 1,350 tile partials, reduced here by three control workgroups
     of 512 (the last to finish combines them): kinetic energy, strain energies and the residual parts against the
     oracle"""
     m = cases.config2()
@@ -255,3 +255,36 @@ def test_control_reduction_over_several_workgroups(W, gpu_ctx):
         assert abs(c[k_dev] - ref) <= 1e-6 * max(abs(ref), 1e-12), (k_dev, c[k_dev], ref)
     assert abs(c["res_f"] - a.st.res_force) <= 1e-6 * a.st.res_force
     assert abs(c["res_t"] - a.st.res_torque) <= 1e-6 * a.st.res_torque
+
+
+def test_wide_control_reduction_and_mixed_widths(W, gpu_ctx):
+    """the 30 m dome in strand tiles of 64 leaves 13,080 tile partials, reduced by 13 control workgroups of 1,024
+    (the last to finish combines their partials through LDS): energies, kinetic energy and residual parts against
+    the oracle; a small dome batched with it is bitwise the same as alone"""
+    big = cases.config5()
+    big.md.par.warmup_iter = 0                  # residual parts from the first iteration on
+    its = 5                                     # past a kinetic-damping roll-back at iteration 3
+    o = Oracle(big)
+    o.step(its)
+    a = o.state()
+    small = cases.config1()
+    alone = W.GpuSolver([small.clone()], precision="fp64", tile=64)
+    c_alone = alone.step(its)[0]
+    s_alone = alone.state(0)
+    alone.close()
+    g = W.GpuSolver([small, big], precision="fp64", tile=64)
+    c_small, c = g.step(its)
+    s_small = g.state(0)
+    g.close()
+    assert c["iter"] == its
+    assert o.ke > 0 and abs(c["ke"] - o.ke) <= 1e-9 * o.ke
+    for k_dev, k_ora in (("e_stretch", "e_stretch"), ("e_bend", "e_bend"), ("e_tors", "e_twist"), ("e_shear", "e_shear"),
+                         ("e_cont", "e_contact")):
+        ref = getattr(a.st, k_ora)
+        assert abs(c[k_dev] - ref) <= 1e-9 * max(abs(ref), 1e-12), (k_dev, c[k_dev], ref)
+    assert a.st.res_force > 0 and a.st.res_torque > 0
+    assert abs(c["res_f"] - a.st.res_force) <= 1e-9 * a.st.res_force
+    assert abs(c["res_t"] - a.st.res_torque) <= 1e-9 * a.st.res_torque
+    for k in ("ke", "e_stretch", "e_bend", "res_f", "res_t"):
+        assert c_small[k] == c_alone[k], k
+    assert np.array_equal(s_small["X"], s_alone["X"])

@@ -833,11 +833,10 @@ std::unique_ptr<BatchBase> make_batch(ws_ctx* ctx, const ws_model* const* models
   return b;
 }
 
-int size_tile(const ws_model* const* models, int B) {
-  // enough workgroups to fill 256 CUs at small sizes, long tiles (fewer halo lanes) at large ones
-  long nodes = 0; for (int q = 0; q < B; q++) nodes += models[q]->n;
-  return nodes >= 2000000 ? 256 : (nodes >= 200000 ? 128 : 64);
-}
+// the default strand tile: one wavefront (the neighbour exchange is DPP alone, no barrier between wavefronts, and the
+// last tile of a lath wastes fewest lanes); fastest or tied with 128 / 256 on the 10 m dome, the 30 m dome and a
+// batch of 64 design variants
+constexpr int DEFAULT_TILE = 64;
 
 // Autotuner: every candidate (strand tile × iterations per captured graph) relaxes the same inputs from the same
 // state for `iters` iterations after a warm-up of two graph replays; the score is executed node-iterations per

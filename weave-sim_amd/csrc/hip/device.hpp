@@ -17,8 +17,12 @@ constexpr int SPART = 9;         // doubles per strand-tile partial (stored valu
 constexpr int CPART = 2;         // doubles per crossing-block partial (stored value-major)
 constexpr int CTRL_THREADS = 256;  // lanes of the control kernel's workgroup
 constexpr int CTRL_PER = 4;         // strand partials per lane per round (all loads of a round in flight together)
-constexpr int CTRL_ONE = 4096;    // up to this many strand partials one control workgroup reduces them all
-constexpr int CTRL_CHUNK = 2048;  // beyond: partials per control workgroup (the last to finish combines them)
+// up to CTRL_ONE strand partials one control workgroup reduces them all; beyond, workgroups of CTRL_CHUNK each (the
+// last to finish combines them): one CU streams partials at ~80 GB/s, so a large instance's partials are spread
+// over several CUs (measured on the 10 m and 30 m domes)
+constexpr int CTRL_ONE = 2048;
+constexpr int CTRL_CHUNK = 1024;
+constexpr int CTRL_STAGE = 64;    // chunk partials the combining workgroup stages in LDS at a time
 constexpr int RPART = 11;         // doubles per control-chunk partial
 constexpr int CROSS_THREADS = 64;                 // lanes per crossing-kernel block (two per crossing)
 constexpr int CROSS_BLOCK = CROSS_THREADS / 2;    // crossings per crossing-kernel block at two lanes per crossing

@@ -1188,6 +1188,7 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   // [10] max Fn (maxima)
   constexpr int NACC = RPART, NSUM = 7;
   __shared__ double red[NACC][CTRL_THREADS / 64];
+  __shared__ double stage[CTRL_STAGE * RPART];
   __shared__ int last;
   if (int(blockIdx.x) >= nitems) return;
   const CtrlItem it = items[blockIdx.x];
@@ -1236,18 +1237,28 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
     // several workgroups: each leaves its partial; the last to arrive reduces them in chunk order and goes on
     if (tid == 0) {
       for (int q = 0; q < NACC; q++) gl(I.rpart)[size_t(it.chunk) * RPART + q] = acc[q];
-      __threadfence();
+      __threadfence();                             // release the partial before counting in
       last = atomicAdd(&ctl->counter, 1) == it.nchunks - 1;
     }
     __syncthreads();
     if (!last) return;
-    if (tid != 0) return;
-    __threadfence();
-    const volatile double* rp = I.rpart;
+    __threadfence();                               // acquire: every lane reads the other workgroups' partials
+    // the chunk partials come in through LDS, every lane loading one value (one memory latency for a batch of
+    // CTRL_STAGE chunks, where one lane reading them in turn would pay one per value); lane 0 combines them in
+    // chunk order
     for (int q = 0; q < NACC; q++) acc[q] = 0.0;
-    for (int k = 0; k < it.nchunks; k++) {
-      for (int q = 0; q < NSUM; q++) acc[q] += rp[size_t(k) * RPART + q];
-      for (int q = NSUM; q < NACC; q++) acc[q] = fmax(acc[q], rp[size_t(k) * RPART + q]);
+    for (int k0 = 0; k0 < it.nchunks; k0 += CTRL_STAGE) {
+      const int nk = min(CTRL_STAGE, it.nchunks - k0);
+      for (int v = tid; v < nk * RPART; v += CTRL_THREADS)
+        stage[v] = __hip_atomic_load(I.rpart + size_t(k0) * RPART + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __syncthreads();
+      if (tid == 0) {
+        for (int k = 0; k < nk; k++) {
+          for (int q = 0; q < NSUM; q++) acc[q] += stage[k * RPART + q];
+          for (int q = NSUM; q < NACC; q++) acc[q] = fmax(acc[q], stage[k * RPART + q]);
+        }
+      }
+      __syncthreads();
     }
     c.counter = 0;                                 // the other workgroups change nothing else in the record
   }

@@ -69,10 +69,10 @@ WS_API int ws_batch_create(ws_ctx* ctx, const ws_model* const* models, const ws_
     int ps = opts ? opts->per_submit : 0, hist = opts ? opts->history : 0;
     int t = opts ? opts->tile : 0;
     if (t != WS_TILE_AUTOTUNE && t != 0 && t != 64 && t != 128 && t != 256)
-      throw Fail{WS_ERR_ARG, "tile must be 0 (by size), -1 (autotune), 64, 128 or 256"};
+      throw Fail{WS_ERR_ARG, "tile must be 0 (default), -1 (autotune), 64, 128 or 256"};
     int lanes = opts ? opts->cross_lanes : 0;
     if (lanes < 0 || lanes > 2) throw Fail{WS_ERR_ARG, "cross_lanes must be 0 (by size), 1 or 2"};
-    if (t == 0) t = size_tile(models, B);
+    if (t == 0) t = DEFAULT_TILE;
     if (t == WS_TILE_AUTOTUNE) {
       std::vector<ws_tune_entry> table;
       ws_batch_opts best{};

@@ -88,7 +88,7 @@ class GpuSolver:
 
     def __init__(self, models, precision="fp32", tile=0, per_submit=0, history=4096, ctx=None, states=None, ctrl_chunk=0,
                  cross_lanes=0):
-        """tile: 64 / 128 / 256 lanes per strand tile, 0 = chosen by size, "autotune" (or -1) = the autotuner's pick;
+        """tile: 64 / 128 / 256 lanes per strand tile, 0 = the default (64), "autotune" (or -1) = the autotuner's pick;
         per_submit: iterations per captured graph (0 = 64, or the autotuner's pick);
         cross_lanes: lanes per crossing in the contact kernel, 1 or 2 (0 = by size, or the autotuner's pick)"""
         self.ctx = ctx or default_context()
@@ -108,7 +108,7 @@ class GpuSolver:
 
     @property
     def opts(self):
-        """options in effect: precision, per_submit, tile and cross_lanes (as chosen by size or by the autotuner),
+        """options in effect: precision, per_submit, tile and cross_lanes (as resolved by default, size or autotuner),
         history"""
         o = A.ws_batch_opts()
         self.ctx.check(self.lib.ws_batch_get_opts(self.h, C.byref(o)))
