@@ -134,6 +134,7 @@ typedef struct ws_state {
   double residual;          /* st.residual (inf while warming up) */
   double res_force, res_torque, fref;               /* residualParts and F_ref of the last iteration (out) */
   double ke, e_stretch, e_shear, e_bend, e_twist, e_contact;  /* energies of the last iteration (out) */
+  double res_rigid;         /* residualParts.rigid (out; the rigid velocities restart at rest on an upload) */
   double *X, *X0, *V;       /* [3n] */
   double *Q, *Q0, *W;       /* [4m], [4m], [3m] */
   double *u;                /* [2K] contact parameters (segment index + fraction) */
@@ -162,6 +163,7 @@ typedef struct ws_state {
 typedef struct ws_control {
   int32_t iter, done, rollback, al_update, stop_at, hist_n, pad0, pad1;
   double ke_prev, ke, resid, fref, e_stretch, e_bend, e_tors, e_cont, e_shear, res_t, res_f;
+  double res_rb, rv[3], rw[3];   /* rigid-body freedoms (ground only): residual part, translation / rotation per iteration */
 } ws_control;
 
 typedef struct ws_batch_opts {

@@ -1,6 +1,7 @@
-"""The ground on the device (anchored structures): the pinned lath of the reference's floor checks
-(floor_check.mjs:55-59) against the oracle — trajectory, stick points, reactions and the converged load share — and a
-pushed, pinned lath whose free end slides.  A structure on the ground with no anchor (rigid-body freedoms) is refused."""
+"""The ground on the device: the pinned lath of the reference's floor checks (floor_check.mjs:55-59) against the
+oracle — trajectory, stick points, reactions and the converged load share — a pushed, pinned lath whose free end
+slides, and the unanchored laths (rigid-body freedoms on) that fall and rest, stick, slide and slide without
+friction."""
 import numpy as np
 import pytest
 
@@ -61,6 +62,55 @@ def test_pushed_pinned_lath_slides(W, gpu_ctx, precision, tol):
     g.close()
 
 
-def test_ground_without_anchor_is_refused(W, gpu_ctx):
-    with pytest.raises(W.DeviceError):
-        W.GpuSolver([cases.ground_lath()], precision="fp64")
+# the reference's unanchored cases (floor_check.mjs): the ground alone holds the lath, so its rigid-body freedoms are on
+RIGID = {
+    "falls_and_rests": dict(),                                  # tilted and lifted: falls, turns flat, rests
+    "pushed_sticks": dict(push=3.0),                            # pushed below μ·W
+    "pushed_slides": dict(push=6.0),                            # pushed above μ·W: slides as a whole
+    "frictionless": dict(push=3.0, mu=0.0),
+}
+
+
+@pytest.mark.parametrize("case", sorted(RIGID))
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 1e-5)])
+def test_unanchored_lath_matches_oracle(W, gpu_ctx, case, precision, tol):
+    """positions, stick points, the rigid part of the residual and the rigid velocities' effect against the oracle
+    over the fall / push windows"""
+    m = cases.ground_lath(**RIGID[case])
+    o = Oracle(m)
+    g = W.GpuSolver([m], precision=precision)
+    for n in (50, 400, 2000):
+        o.step(n - o.iteration)
+        c = g.step(n - g.read_control()[0]["iter"])[0]
+        a, b = o.state(), g.state(0)
+        dX = np.abs(a["X"] - b["X"]).max()
+        assert dX <= tol * 100.0, (case, n, dX)
+        assert np.abs(a["floor_a"] - b["floor_a"]).max() <= tol * 100.0
+        # the rigid part |Q_t|/F_ref, |Q_r|/(F_ref·R): near rest the resultant cancels, and its absolute error follows
+        # the position error times the ground stiffness
+        rigid = o.floor()[2]
+        rtol, atol = (1e-6, 1e-8) if precision == "fp64" else (1e-2, 1e-5)
+        assert abs(c["res_rb"] - rigid) <= rtol * rigid + atol, (n, c["res_rb"], rigid)
+        assert abs(b.st.res_rigid - c["res_rb"]) == 0.0
+    g.close()
+
+
+def test_unanchored_lath_rests_on_the_ground(W, gpu_ctx):
+    """relaxed to the 1e-6 residual with the rigid freedoms on: the ground carries the whole weight (floor_check.mjs
+    rest case) and the device stops where the oracle does"""
+    m = cases.ground_lath()
+    Wt = m.arr("load_const").reshape(-1, 3).sum(0)
+    o = Oracle(m)
+    o.run(20000)
+    g = W.GpuSolver([m], precision="fp64")
+    for _ in range(10):
+        if g.step(2000)[0]["done"]:
+            break
+    c = g.read_control()[0]
+    b = g.state(0)
+    g.close()
+    assert o.converged and b.st.converged == 1
+    assert abs(c["iter"] - o.iteration) <= 0.02 * o.iteration
+    assert np.abs(o.state()["X"] - b["X"]).max() <= 1e-4 * 100.0
+    X = b["X"].reshape(-1, 3)
+    assert abs(X[0, 2] - X[-1, 2]) < 1e-4

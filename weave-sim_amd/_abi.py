@@ -55,14 +55,15 @@ STATE_ARRAYS = [
 class ws_state(C.Structure):
     _fields_ = [(n, i32) for n in ("iteration", "converged", "pen_n", "pad0")] + \
                [(n, d) for n in ("ke_prev", "residual", "res_force", "res_torque", "fref", "ke", "e_stretch",
-                                 "e_shear", "e_bend", "e_twist", "e_contact")] + \
+                                 "e_shear", "e_bend", "e_twist", "e_contact", "res_rigid")] + \
                [(n, t) for (n, t, _) in STATE_ARRAYS]
 
 
 class ws_control(C.Structure):
     _fields_ = [(n, i32) for n in ("iter", "done", "rollback", "al_update", "stop_at", "hist_n", "pad0", "pad1")] + \
                [(n, d) for n in ("ke_prev", "ke", "resid", "fref", "e_stretch", "e_bend", "e_tors", "e_cont",
-                                 "e_shear", "res_t", "res_f")]
+                                 "e_shear", "res_t", "res_f", "res_rb")] + \
+               [("rv", d * 3), ("rw", d * 3)]
 
 
 class ws_batch_opts(C.Structure):

@@ -24,7 +24,8 @@
 namespace wsd {
 template <typename Real> __global__ void k_begin(Ctl*, const int32_t*, const int32_t*, int32_t, int32_t, int32_t);
 template <typename Real, int LANES> __global__ void k_cross(const Inst<Real>*, const CrossItem*, int32_t, const Ctl*);
-template <typename Real, int TILE> __global__ void k_strand(const Inst<Real>*, const TileItem*, int32_t, const Ctl*);
+template <typename Real, int TILE, bool RB> __global__ void k_strand(const Inst<Real>*, const TileItem*, int32_t, const Ctl*);
+template <typename Real, int TILE> __global__ void k_rigid(const Inst<Real>*, const TileItem*, int32_t, const Ctl*);
 template <typename Real> __global__ void k_ctrl(const Inst<Real>*, const CtrlItem*, int32_t, Ctl*);
 __global__ void k_copy(const uint4*, uint4*, int64_t);
 template <typename Real> __global__ void k_al_preview(const Inst<Real>*, int32_t, double*);
@@ -110,12 +111,11 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   const int n = md.n, m = md.m, p = md.p, C = md.C, S = md.S;
   const int K = md.par.contacts ? md.K : 0;   // no contact goal: no crossings on the device
   if (n <= 0 || m < 0 || S <= 0) throw Fail{WS_ERR_ARG, "model has no nodes or strands"};
-  if (md.par.floor_on) {
-    // the ground's rigid-body freedoms (a structure with no anchor resting on it) are not on the device
-    bool anchored = md.C > 0;
-    for (int i = 0; i < md.n && !anchored; i++) anchored = !(md.node_free[3 * i] && md.node_free[3 * i + 1] && md.node_free[3 * i + 2]);
-    if (!anchored) throw Fail{WS_ERR_ARG, "the ground without an anchor (rigid-body freedoms) is not on the device path"};
-  }
+  // the rigid-body freedoms: on when the ground alone holds the structure (no node with a fixed freedom, no clamp;
+  // CpuSolver.step, solver_cpu.js:745)
+  bool anchored = md.C > 0;
+  for (int i = 0; i < md.n && !anchored; i++) anchored = !(md.node_free[3 * i] && md.node_free[3 * i + 1] && md.node_free[3 * i + 2]);
+  const bool rigid = md.par.floor_on && !anchored;
   for (int q = 0; q < 2 * (md.par.contacts ? md.K : 0); q++)
     if (md.k_nseg[q] < 1 || md.k_strand[q] < 0 || md.k_strand[q] >= S) throw Fail{WS_ERR_ARG, "a crossing lies on a strand without segments"};
   for (int s2 = 0; s2 < S; s2++)
@@ -136,6 +136,22 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   d.rot_fric = P.rot_friction; d.seat = P.seat_law; d.seat_gap = P.seat_gap; d.dent_after = P.dent_after;
   d.aug = P.augmented; d.al_every = P.al_every; d.hist_len = hist_len;
   d.floor_on = P.floor_on ? 1 : 0; d.floor_z = P.floor_z; d.floor_mu = std::max(P.floor_mu, 0.0);
+  d.rigid = rigid ? 1 : 0;
+  if (rigid) {
+    // model radius about the centroid and the rigid speed limits, from the state uploaded at construction (the
+    // Floor constructor, solver_cpu.js:546-562)
+    double c[3] = {0, 0, 0};
+    for (int i = 0; i < n; i++) for (int k = 0; k < 3; k++) c[k] += st.X[3 * i + k] / std::max(n, 1);
+    double R2 = 0, hmin = INFINITY;
+    for (int i = 0; i < n; i++) {
+      const double dx = st.X[3 * i] - c[0], dy = st.X[3 * i + 1] - c[1], dz = st.X[3 * i + 2] - c[2];
+      R2 = std::max(R2, dx * dx + dy * dy + dz * dz);
+      hmin = std::min(hmin, std::min(md.node_h[i], md.node_w[i]));
+    }
+    d.rb_R = std::max(std::sqrt(R2), 1e-9);
+    d.rb_vmax = RB_VMAX * (std::isfinite(hmin) && hmin > 0 ? hmin : 1.0);
+    d.rb_wmax = d.rb_vmax / d.rb_R;
+  }
   // ---- strand tiles and crossing blocks
   H.tiles.clear();
   for (int s = 0; s < S; s++) {
@@ -189,6 +205,7 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
     d.ctrl_chunks = (np + d.ctrl_chunk - 1) / d.ctrl_chunk;
   }
   A.add(d.rpart, size_t(d.ctrl_chunks) * RPART);
+  if (d.rigid) A.add(d.rbpart, size_t(d.ntiles) * RBP);
   if (layout_only) return;   // A.size: the device bytes the instance would take
   A.alloc();
   // ---- topology
@@ -323,6 +340,13 @@ void upload_state(InstHost<Real>& H, const ws_state& st) {
   Ctl c{};
   c.iter = st.iteration; c.stop_at = st.iteration; c.done = 0; c.cur = 0; c.pen_n = st.pen_n;
   c.ke_prev = st.ke_prev; c.resid = st.residual;
+  if (d.rigid) {
+    // the reference point of the first evaluation: the centroid of the uploaded nodes (Floor.apply, :588); the rigid
+    // velocities start at rest
+    double s[3] = {0, 0, 0};
+    for (int i = 0; i < n; i++) for (int k = 0; k < 3; k++) s[k] += st.X[3 * i + k];
+    for (int k = 0; k < 3; k++) { c.rc[k] = s[k] / n; c.rc_prev[k] = c.rc[k]; }
+  }
   HIPCHK(hipMemcpy(d.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice));
   H.hist_read = 0;
 }
@@ -333,6 +357,7 @@ struct BatchBase {
   ws_ctx* ctx = nullptr;
   int B = 0, precision = 0, tile = 128, per_submit = 64, hist_len = 4096;
   int ctrl_chunk = 0;                   // strand partials per control workgroup (0: by size)
+  bool has_rigid = false;               // an instance that only the ground holds: the rigid step runs
   int cross_lanes = 0;                  // lanes per crossing (0: by size at every relist)
   int lanes = 2;                        // ... as the current work lists use it
   hipStream_t stream = nullptr;
@@ -477,6 +502,8 @@ struct Batch : BatchBase {
       for (int q = 0; q < H.d.ctrl_chunks; q++) ctrl.push_back(CtrlItem{b, q, H.d.ctrl_chunks, 0});
     }
     ntiles = int(tiles.size()); ncross = int(cross.size()); nctrl = int(ctrl.size());
+    has_rigid = false;
+    for (int b = 0; b < B; b++) if (inst[b] && inst[b]->d.rigid) has_rigid = true;
     HIPCHK(hipStreamSynchronize(stream));
     put_list(d_tiles, cap_tiles, tiles);
     put_list(d_cross, cap_cross, cross);
@@ -495,13 +522,29 @@ struct Batch : BatchBase {
   }
   void launch_strand(hipStream_t s) {
     switch (tile) {
-      case 64: hipLaunchKernelGGL((k_strand<Real, 64>), dim3(ntiles), dim3(64), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
-      case 128: hipLaunchKernelGGL((k_strand<Real, 128>), dim3(ntiles), dim3(128), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
-      default: hipLaunchKernelGGL((k_strand<Real, 256>), dim3(ntiles), dim3(256), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
+      case 64:
+        if (has_rigid) hipLaunchKernelGGL((k_strand<Real, 64, true>), dim3(ntiles), dim3(64), 0, s, d_inst, d_tiles, ntiles, d_ctls);
+        else hipLaunchKernelGGL((k_strand<Real, 64, false>), dim3(ntiles), dim3(64), 0, s, d_inst, d_tiles, ntiles, d_ctls);
+        break;
+      case 128:
+        if (has_rigid) hipLaunchKernelGGL((k_strand<Real, 128, true>), dim3(ntiles), dim3(128), 0, s, d_inst, d_tiles, ntiles, d_ctls);
+        else hipLaunchKernelGGL((k_strand<Real, 128, false>), dim3(ntiles), dim3(128), 0, s, d_inst, d_tiles, ntiles, d_ctls);
+        break;
+      default:
+        if (has_rigid) hipLaunchKernelGGL((k_strand<Real, 256, true>), dim3(ntiles), dim3(256), 0, s, d_inst, d_tiles, ntiles, d_ctls);
+        else hipLaunchKernelGGL((k_strand<Real, 256, false>), dim3(ntiles), dim3(256), 0, s, d_inst, d_tiles, ntiles, d_ctls);
+        break;
     }
   }
   void launch_ctrl(hipStream_t s) {
     if (nctrl) hipLaunchKernelGGL(k_ctrl<Real>, dim3(nctrl), dim3(CTRL_THREADS), 0, s, d_inst, d_ctrl, nctrl, d_ctls);
+    if (has_rigid && ntiles) {
+      switch (tile) {
+        case 64: hipLaunchKernelGGL((k_rigid<Real, 64>), dim3(ntiles), dim3(64), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
+        case 128: hipLaunchKernelGGL((k_rigid<Real, 128>), dim3(ntiles), dim3(128), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
+        default: hipLaunchKernelGGL((k_rigid<Real, 256>), dim3(ntiles), dim3(256), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
+      }
+    }
   }
   void launch_iteration(hipStream_t s) { launch_cross(s); launch_strand(s); launch_ctrl(s); }
 
@@ -596,6 +639,8 @@ struct Batch : BatchBase {
     o.iter = c.iter; o.done = c.done; o.rollback = c.rollback; o.al_update = c.al_update; o.stop_at = c.stop_at; o.hist_n = c.hist_n;
     o.ke_prev = c.ke_prev; o.ke = c.ke; o.resid = c.resid; o.fref = c.fref; o.e_stretch = c.e_stretch; o.e_bend = c.e_bend;
     o.e_tors = c.e_tors; o.e_cont = c.e_cont; o.e_shear = c.e_shear; o.res_t = c.res_t; o.res_f = c.res_f;
+    o.res_rb = c.res_rb;
+    for (int k = 0; k < 3; k++) { o.rv[k] = c.rv[k]; o.rw[k] = c.rw[k]; }
     return o;
   }
 
@@ -713,7 +758,7 @@ struct Batch : BatchBase {
     }
     o.iteration = c.iter; o.ke_prev = c.ke_prev; o.residual = c.resid; o.pen_n = (c.al_update && d.aug && K) ? 0 : c.pen_n;
     o.converged = (c.done == 1 && c.resid <= H.tol) ? 1 : 0;
-    o.res_force = c.res_f; o.res_torque = c.res_t; o.fref = c.fref; o.ke = c.ke;
+    o.res_force = c.res_f; o.res_torque = c.res_t; o.fref = c.fref; o.ke = c.ke; o.res_rigid = c.res_rb;
     o.e_stretch = c.e_stretch; o.e_shear = c.e_shear; o.e_bend = c.e_bend; o.e_twist = c.e_tors; o.e_contact = c.e_cont;
     return WS_OK;
   }

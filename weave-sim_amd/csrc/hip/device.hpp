@@ -24,6 +24,10 @@ constexpr int CTRL_ONE = 2048;
 constexpr int CTRL_CHUNK = 1024;
 constexpr int CTRL_STAGE = 64;    // chunk partials the combining workgroup stages in LDS at a time
 constexpr int RPART = 11;         // doubles per control-chunk partial
+// rigid-body sums per strand tile: resultant of loads + ground (3), its moment about the centroid (3), Σ d² of the
+// ground-contact nodes per axis (3), their count (1), Σ x of the new positions (3)
+constexpr int RBP = 13;
+constexpr double RB_MASS = 2.0, RB_VMAX = 0.05, RB_LEVER = 0.05;   // solver_cpu.js:37-39
 constexpr int CROSS_THREADS = 64;                 // lanes per crossing-kernel block (two per crossing)
 constexpr int CROSS_BLOCK = CROSS_THREADS / 2;    // crossings per crossing-kernel block at two lanes per crossing
 // crossings in a batch from which one lane takes both sides: from here the one-lane layout alone fills every SIMD
@@ -36,9 +40,13 @@ constexpr int MAX_REL = 1024;    // crossing sides a strand tile may gather (che
 struct Ctl {
   int32_t iter, done, rollback, al_update;
   int32_t stop_at, hist_n, cur, counter;
-  int32_t pen_n, live_flag, pad1, pad2;
+  int32_t pen_n, live_flag, rb_move, pad2;   // rb_move: the rigid step of the iteration just evaluated is pending
   double ke_prev, ke, resid, fref;
   double e_stretch, e_bend, e_tors, e_cont, e_shear, res_t, res_f;
+  // rigid-body freedoms of a structure only the ground holds (Floor, solver_cpu.js:546-551): the reference point
+  // (centroid) the next iteration uses, the one the last iteration used, translation / rotation velocities, and the
+  // rigid part of the residual
+  double rc[3], rc_prev[3], rv[3], rw[3], res_rb;
 };
 
 // one crossing side's contribution of this iteration, written by the crossing kernel, gathered per strand tile
@@ -64,7 +72,9 @@ struct Inst {
   double mass_scale, damping, tol, mu, strength, stick, torsion_factor, epsilon, slip_rate, rot_factor, dent_rate;
   double edge_thr, l_ref, jlen_min, inv_jlen_min;   // inv_jlen_min: 1/l̄_min, 0 without joints
   int32_t kin_damp, warmup, contacts, ramp_iter, rot_fric, seat, seat_gap, dent_after, aug, al_every, hist_len, floor_on;
-  double floor_z, floor_mu;          // the ground plane (anchored structures only)
+  int32_t rigid, pad_r;              // rigid: the ground holds the structure alone (no anchor, no clamp)
+  double floor_z, floor_mu;          // the ground plane
+  double rb_R, rb_vmax, rb_wmax;     // rigid body: model radius about the centroid, speed limits per iteration
   // topology
   const int32_t* node_off;   // [S+1]
   const int32_t* seg_off;    // [S+1]
@@ -125,6 +135,7 @@ struct Inst {
   double* spart;             // [SPART][ntiles] one partial per strand tile
   double* cpart;             // [CPART][ncblk]
   double* rpart;             // [ctrl_chunks][RPART] control-chunk partials (ctrl_chunks > 1)
+  double* rbpart;            // [RBP][ntiles] rigid-body sums per strand tile (rigid only)
 };
 
 // one strand tile: strand-local node range [k0, k1) plus the strand's layout, so that a tile reads its geometry in

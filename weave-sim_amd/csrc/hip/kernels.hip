@@ -693,6 +693,7 @@ struct StrandLDS {
   int32_t glo[TILE], ghi[TILE];                                      // per slot: first / last gathered record it needs
   int wcnt[TILE / 64];
   Real bnd[TILE / 64][9];            // last lane of each wavefront: segment force, second-segment torque, kDyn
+  double rbw[RBP][TILE / 64];        // rigid-body sums per wavefront (rigid instances)
 };
 
 // the value of lane − 1 (DPP wave_shr:1; lane 0 reads 0)
@@ -807,7 +808,7 @@ __device__ __forceinline__ void joint_torque(StrandLDS<Real, TILE>& L, int i, Re
 // at them; lane t works on "slot" t: node k0−1+t, segment k0−1+t and joint k0−1+t.  Slot 0 is the halo (the segment
 // and joint just before the tile, whose forces the first owned node and frame need): every lane computes one segment
 // and one joint, and lanes 1 .. cnt integrate their node and frame.
-template <typename Real, int TILE, bool OUT>
+template <typename Real, int TILE, bool OUT, bool RB>
 __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem& tl, StrandLDS<Real, TILE>& L,
                             double* outN, double* outV, double* outM, double* outCM, double* outR) {
   const int tid = threadIdx.x;
@@ -970,6 +971,9 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
 
   // ---------------------------------------------------------------- nodes: loads, reactions, anchors, v += F/m, x += v
   double keN = 0, keR = 0, mxF = 0, mxT = 0, mxL = 0, eG = 0, mxG = 0;
+  double rb[RBP];                                  // rigid-body sums of the lane's node (rigid instances)
+#pragma unroll
+  for (int q = 0; q < RBP; q++) rb[q] = 0.0;
   const bool warm = iter < I.warmup && I.contacts;
   const Real damping = Real(I.damping);
   if (ownNode) {
@@ -1006,6 +1010,13 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       const double kn = rmp * I.strength, kt = rmp * I.stick;
       const double pen = (I.floor_z - L.Xz[t]) + (0.5 * double(gl(I.nodew)[i]) * fabs(Rn[2]) + 0.5 * double(gl(I.nodeh)[i]) * fabs(Rn[5]));
       double fg[3] = {0, 0, 0};
+      if (RB && I.rigid && !OUT) {
+        // the resultant of loads + ground and its moment about the centroid; Σ d² and count of the nodes on the
+        // ground (the rigid masses)
+        const double dx = L.Xx[t] - ctl->rc[0], dy = L.Xy[t] - ctl->rc[1], dz = L.Xz[t] - ctl->rc[2];
+        if (pen > 0) { rb[6] = dx * dx; rb[7] = dy * dy; rb[8] = dz * dz; rb[9] = 1.0; }
+        rb[3] = dx; rb[4] = dy; rb[5] = dz;          // lever, until the force is known
+      }
       if (pen > 0) {
         const double fn = kn * pen;
         double ax = gl(I.fax)[i], ay = gl(I.fay)[i];
@@ -1022,6 +1033,12 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
         gl(I.fax)[i] = L.Xx[t]; gl(I.fay)[i] = L.Xy[t];
       }
       for (int c = 0; c < 3; c++) F[c] += Real(fg[c]);
+      if (RB && I.rigid && !OUT) {
+        const double gx = double(ld[0]) + fg[0], gy = double(ld[1]) + fg[1], gz = double(ld[2]) + fg[2];
+        const double dx = rb[3], dy = rb[4], dz = rb[5];
+        rb[0] = gx; rb[1] = gy; rb[2] = gz;
+        rb[3] = dy * gz - dz * gy; rb[4] = dz * gx - dx * gz; rb[5] = dx * gy - dy * gx;
+      }
     }
     if (OUT) {
       if (fx & 8) { outR[3 * i] = -double(F[0]); outR[3 * i + 1] = -double(F[1]); outR[3 * i + 2] = -double(F[2]); }
@@ -1041,9 +1058,11 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       if (fx & 2) V[1] = Real(0);
       if (fx & 4) V[2] = Real(0);
       const int nx = cur ^ 1;
-      gl(I.x[nx])[i] = L.Xx[t] + double(V[0]);
-      gl(I.y[nx])[i] = L.Xy[t] + double(V[1]);
-      gl(I.z[nx])[i] = L.Xz[t] + double(V[2]);
+      const double xn = L.Xx[t] + double(V[0]), yn = L.Xy[t] + double(V[1]), zn = L.Xz[t] + double(V[2]);
+      gl(I.x[nx])[i] = xn;
+      gl(I.y[nx])[i] = yn;
+      gl(I.z[nx])[i] = zn;
+      if (RB) { rb[10] = xn; rb[11] = yn; rb[12] = zn; }
       gl(I.vx)[i] = V[0]; gl(I.vy)[i] = V[1]; gl(I.vz)[i] = V[2];
       keN = double(ms) * (double(V[0]) * double(V[0]) + double(V[1]) * double(V[1]) + double(V[2]) * double(V[2]));
     }
@@ -1155,6 +1174,20 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       if (lane == 0) gl(I.spart)[size_t(8) * I.ntiles + tl.tile_local] = g;
     }
   }
+  if (RB && I.rigid) {
+    // the tile's rigid-body sums: each wavefront's in a fixed DPP order, then the wavefronts in order
+#pragma unroll
+    for (int q = 0; q < RBP; q++) {
+      const double x = wave_reduce63<true>(rb[q]);
+      if (lane == 63) L.rbw[q][wv] = x;
+    }
+    __syncthreads();
+    if (tid < RBP) {
+      double x = L.rbw[tid][0];
+      for (int w = 1; w < TILE / 64; w++) x += L.rbw[tid][w];
+      gl(I.rbpart)[size_t(tid) * I.ntiles + tl.tile_local] = x;
+    }
+  }
 }
 
 // fixed-order block reduction of NV doubles (the first nsum summed, the rest max-reduced over non-negative values);
@@ -1189,6 +1222,7 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   constexpr int NACC = RPART, NSUM = 7;
   __shared__ double red[NACC][CTRL_THREADS / 64];
   __shared__ double stage[CTRL_STAGE * RPART];
+  __shared__ double red_rb[RBP][CTRL_THREADS / 64];
   __shared__ int last;
   if (int(blockIdx.x) >= nitems) return;
   const CtrlItem it = items[blockIdx.x];
@@ -1231,7 +1265,10 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
     acc[6] += gl(I.cpart)[w];
     acc[10] = fmax(acc[10], gl(I.cpart)[I.ncblk + w]);
   }
-  if (!(c.done == 0 && c.iter < c.stop_at)) return;   // uniform over the workgroup
+  if (!(c.done == 0 && c.iter < c.stop_at)) {        // uniform over the workgroup
+    if (I.rigid && it.chunk == 0 && tid == 0 && c.rb_move) ctl->rb_move = 0;   // no rigid step without an iteration
+    return;
+  }
   block_reduce<CTRL_THREADS, NACC>(red, acc, NSUM);
   if (it.nchunks > 1) {
     // several workgroups: each leaves its partial; the last to arrive reduces them in chunk order and goes on
@@ -1262,12 +1299,47 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
     }
     c.counter = 0;                                 // the other workgroups change nothing else in the record
   }
+  // rigid-body sums of the instance's tiles (a structure only the ground holds), in tile order per lane, then the
+  // fixed block order
+  double rbs[RBP];
+#pragma unroll
+  for (int q = 0; q < RBP; q++) rbs[q] = 0.0;
+  if (I.rigid) {
+    for (int w = tid; w < np; w += CTRL_THREADS)
+#pragma unroll
+      for (int q = 0; q < RBP; q++) rbs[q] += gl(I.rbpart)[size_t(q) * np + w];
+    block_reduce<CTRL_THREADS, RBP>(red_rb, rbs, RBP);
+  }
   if (tid != 0) return;
   const double eCont = acc[6], eGround = acc[5];
   double ke = 0.5 * acc[0];
-  int rollback = 0;
-  if (I.kin_damp && ke < c.ke_prev) { rollback = 1; ke = 0.0; }
   const int iterPre = c.iter;
+  // rigid-body step (CpuSolver.step, solver_cpu.js:784-790; rigidMasses :565-572): velocities from this iteration's
+  // resultant, v ← clip((v + Q/M)·damping), zero while warming up; their kinetic energy counts for the roll-back
+  double rv[3] = {0, 0, 0}, rw[3] = {0, 0, 0};
+  if (I.rigid) {
+    const int rampN = max(I.ramp_iter, 0);
+    const double rmp = rampN <= 0 ? 1.0 : fmin(1.0, 0.01 + 0.99 * iterPre / rampN);
+    const double kn = rmp * I.strength, kt = rmp * I.stick, f = I.mass_scale * RB_MASS;
+    const double nc = fmax(rbs[9], 1.0), rmin = (RB_LEVER * I.rb_R) * (RB_LEVER * I.rb_R) * kn;
+    double Mr[6];
+    Mr[0] = fmax(f * nc * kt, 1e-30); Mr[1] = Mr[0]; Mr[2] = fmax(f * nc * kn, 1e-30);
+    Mr[3] = fmax(f * fmax(kn * rbs[7] + kt * rbs[8], rmin), 1e-30);
+    Mr[4] = fmax(f * fmax(kn * rbs[6] + kt * rbs[8], rmin), 1e-30);
+    Mr[5] = fmax(f * fmax(kt * (rbs[6] + rbs[7]), rmin), 1e-30);
+    if (!(iterPre < I.warmup && I.contacts)) {
+      for (int k = 0; k < 3; k++) {
+        rv[k] = fmin(fmax((c.rv[k] + rbs[k] / Mr[k]) * I.damping, -I.rb_vmax), I.rb_vmax);
+        rw[k] = fmin(fmax((c.rw[k] + rbs[3 + k] / Mr[3 + k]) * I.damping, -I.rb_wmax), I.rb_wmax);
+      }
+    }
+    for (int k = 0; k < 3; k++) ke += 0.5 * (Mr[k] * rv[k] * rv[k] + Mr[3 + k] * rw[k] * rw[k]);
+  }
+  int rollback = 0;
+  if (I.kin_damp && ke < c.ke_prev) {
+    rollback = 1; ke = 0.0;
+    for (int k = 0; k < 3; k++) { rv[k] = 0.0; rw[k] = 0.0; }
+  }
   c.rollback = rollback;
   c.ke_prev = ke; c.ke = ke;
   c.iter = iterPre + 1;
@@ -1285,6 +1357,21 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   // the strand partials carry max |F_free|² and max |t_m|²: the square root of the maximum is the maximum norm
   const double rf = I.n ? sqrt(acc[7]) / fref : 0.0, rt = I.m ? sqrt(acc[8]) / (fref * I.l_ref) : 0.0;
   double resid = fmax(rf, rt);
+  if (I.rigid) {
+    // the rigid part of the residual (rigidResidual, :575), then the rigid record: the next reference point is the
+    // centroid of the new positions, moved by the rigid translation (the rotation turns them about the old one)
+    const double rrb = fmax(sqrt(rbs[0] * rbs[0] + rbs[1] * rbs[1] + rbs[2] * rbs[2]) / fref,
+                            sqrt(rbs[3] * rbs[3] + rbs[4] * rbs[4] + rbs[5] * rbs[5]) / (fref * I.rb_R));
+    resid = fmax(resid, rrb);
+    const bool move = rv[0] != 0 || rv[1] != 0 || rv[2] != 0 || rw[0] != 0 || rw[1] != 0 || rw[2] != 0;
+    c.rb_move = move ? 1 : 0;
+    for (int k = 0; k < 3; k++) {
+      c.rc_prev[k] = c.rc[k];
+      if (!rollback) c.rc[k] = rbs[10 + k] / I.n + (move ? rv[k] : 0.0);
+      c.rv[k] = rv[k]; c.rw[k] = rw[k];
+    }
+    c.res_rb = rrb;
+  }
   if (!(c.iter > I.warmup)) resid = INFINITY;
   c.resid = resid; c.fref = fref; c.res_f = rf; c.res_t = rt;
   c.e_stretch = acc[1]; c.e_shear = acc[2]; c.e_bend = acc[3]; c.e_tors = acc[4]; c.e_cont = eCont;
@@ -1311,7 +1398,9 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
   return base + r;
 }
 
-template <typename Real, int TILE>
+// RB: the batch holds an instance that only the ground holds (its tiles also leave the rigid-body sums); the other
+// batches run the form without them
+template <typename Real, int TILE, bool RB>
 __global__ __launch_bounds__(TILE) void k_strand(const Inst<Real>* __restrict__ insts, const TileItem* __restrict__ tiles, int32_t ntiles,
                                                  const Ctl* __restrict__ ctls) {
   __shared__ StrandLDS<Real, TILE> L;
@@ -1321,7 +1410,47 @@ __global__ __launch_bounds__(TILE) void k_strand(const Inst<Real>* __restrict__ 
   if (tl.inst < 0) return;
   const Ctl* ctl = ctls + tl.inst;
   if (!inst_live(ctl)) return;
-  strand_tile<Real, TILE, false>(insts[tl.inst], ctl, tl, L, nullptr, nullptr, nullptr, nullptr, nullptr);
+  strand_tile<Real, TILE, false, RB>(insts[tl.inst], ctl, tl, L, nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+// The rigid step of a structure only the ground holds (CpuSolver.step, solver_cpu.js:791-805), after the control
+// kernel has decided the rigid velocities and kept the iteration: every owned node moves by v + w × (x − c) from
+// its position before the node step, every frame turns by exp(w) in world axes, q ← normalize(exp(w) ⊗ q), in f64.
+// Same tiles and lanes as the strand kernel; nothing to do when the instance is not rigid or did not move.
+template <typename Real, int TILE>
+__global__ __launch_bounds__(TILE) void k_rigid(const Inst<Real>* __restrict__ insts, const TileItem* __restrict__ tiles, int32_t ntiles,
+                                                const Ctl* __restrict__ ctls) {
+  const int lt = blockIdx.x;
+  if (lt >= ntiles) return;
+  const TileItem tl = tiles[lt];
+  if (tl.inst < 0) return;
+  const Inst<Real>& I = insts[tl.inst];
+  const Ctl* ctl = ctls + tl.inst;
+  if (!I.rigid || !ctl->rb_move) return;
+  const int t = threadIdx.x, cnt = tl.k1 - tl.k0;
+  if (t < 1 || t > cnt) return;                    // lane t owns node and segment k0 − 1 + t
+  const int k = tl.k0 - 1 + t;
+  const int cur = ctl->cur, old = cur ^ 1;          // the kept iteration flipped the buffers: cur holds the new state
+  const double v0 = ctl->rv[0], v1 = ctl->rv[1], v2 = ctl->rv[2], w0 = ctl->rw[0], w1 = ctl->rw[1], w2 = ctl->rw[2];
+  const int i = tl.noff + k;
+  const double dx = gl(I.x[old])[i] - ctl->rc_prev[0], dy = gl(I.y[old])[i] - ctl->rc_prev[1], dz = gl(I.z[old])[i] - ctl->rc_prev[2];
+  gl(I.x[cur])[i] += v0 + w1 * dz - w2 * dy;
+  gl(I.y[cur])[i] += v1 + w2 * dx - w0 * dz;
+  gl(I.z[cur])[i] += v2 + w0 * dy - w1 * dx;
+  if (tl.nseg > 0 && (tl.closed || k < tl.nseg)) {
+    const int j = tl.soff + k;
+    const double a = sqrt(w0 * w0 + w1 * w1 + w2 * w2), half = 0.5 * a;
+    const double sc = a > 1e-12 ? sin(half) / a : 0.5;
+    const double ew = cos(half), ex = w0 * sc, ey = w1 * sc, ez = w2 * sc;      // exp(w) (quat.js qexp)
+    const double qw = gl(I.qw[cur])[j], qx = gl(I.qx[cur])[j], qy = gl(I.qy[cur])[j], qz = gl(I.qz[cur])[j];
+    const double rw = ew * qw - ex * qx - ey * qy - ez * qz;
+    const double rx = ew * qx + ex * qw + ey * qz - ez * qy;
+    const double ry = ew * qy - ex * qz + ey * qw + ez * qx;
+    const double rz = ew * qz + ex * qy - ey * qx + ez * qw;
+    const double n2 = rw * rw + rx * rx + ry * ry + rz * rz;
+    const double inv = n2 > 0 ? 1.0 / sqrt(n2) : 1.0;
+    gl(I.qw[cur])[j] = rw * inv; gl(I.qx[cur])[j] = rx * inv; gl(I.qy[cur])[j] = ry * inv; gl(I.qz[cur])[j] = rz * inv;
+  }
 }
 
 // outputs of the last evaluated state of one instance: N, V, M, clamp moments, reactions (download)
@@ -1332,7 +1461,7 @@ __global__ __launch_bounds__(TILE) void k_outputs(const Inst<Real>* __restrict__
   const Inst<Real>& I = insts[inst];
   if (int(blockIdx.x) >= I.ntiles) return;
   const TileItem tl = tiles[I.tile0 + blockIdx.x];
-  strand_tile<Real, TILE, true>(I, I.ctl, tl, L, outN, outV, outM, outCM, outR);
+  strand_tile<Real, TILE, true, false>(I, I.ctl, tl, L, outN, outV, outM, outCM, outR);
 }
 
 // the multipliers as the host solver holds them after an iteration: an AL update decided by the control kernel is
@@ -1353,7 +1482,9 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
 
 // explicit instantiations
 #define WSD_INST(R, T) \
-  template __global__ void k_strand<R, T>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
+  template __global__ void k_strand<R, T, false>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
+  template __global__ void k_strand<R, T, true>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
+  template __global__ void k_rigid<R, T>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
   template __global__ void k_outputs<R, T>(const Inst<R>*, const TileItem*, int32_t, double*, double*, double*, double*, double*);
 WSD_INST(float, 64) WSD_INST(float, 128) WSD_INST(float, 256)
 WSD_INST(double, 64) WSD_INST(double, 128) WSD_INST(double, 256)

@@ -78,7 +78,10 @@ napi_value ctl_object(napi_env env, const ws_control& c) {
   set(env, o, "kePrev", num(env, c.ke_prev)); set(env, o, "ke", num(env, c.ke)); set(env, o, "resid", num(env, c.resid));
   set(env, o, "fref", num(env, c.fref)); set(env, o, "eStretch", num(env, c.e_stretch)); set(env, o, "eBend", num(env, c.e_bend));
   set(env, o, "eTors", num(env, c.e_tors)); set(env, o, "eCont", num(env, c.e_cont)); set(env, o, "eShear", num(env, c.e_shear));
-  set(env, o, "resT", num(env, c.res_t)); set(env, o, "resF", num(env, c.res_f));
+  set(env, o, "resT", num(env, c.res_t)); set(env, o, "resF", num(env, c.res_f)); set(env, o, "resRb", num(env, c.res_rb));
+  napi_value rv, rw; napi_create_array_with_length(env, 3, &rv); napi_create_array_with_length(env, 3, &rw);
+  for (uint32_t k = 0; k < 3; k++) { napi_set_element(env, rv, k, num(env, c.rv[k])); napi_set_element(env, rw, k, num(env, c.rw[k])); }
+  set(env, o, "rv", rv); set(env, o, "rw", rw);
   return o;
 }
 
@@ -246,6 +249,7 @@ napi_value scalars(napi_env env, const ws_state& s) {
   set(env, o, "resForce", num(env, s.res_force)); set(env, o, "resTorque", num(env, s.res_torque)); set(env, o, "fref", num(env, s.fref));
   set(env, o, "eStretch", num(env, s.e_stretch)); set(env, o, "eShear", num(env, s.e_shear)); set(env, o, "eBend", num(env, s.e_bend));
   set(env, o, "eTwist", num(env, s.e_twist)); set(env, o, "eContact", num(env, s.e_contact));
+  set(env, o, "resRigid", num(env, s.res_rigid));
   return o;
 }
 

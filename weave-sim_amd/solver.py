@@ -64,7 +64,9 @@ def default_context():
 
 
 def _ctl_dict(c):
-    return {f: getattr(c, f) for f, _ in A.ws_control._fields_ if not f.startswith("pad")}
+    def val(v):
+        return list(v) if isinstance(v, C.Array) else v
+    return {f: val(getattr(c, f)) for f, _ in A.ws_control._fields_ if not f.startswith("pad")}
 
 
 def autotune(models, precision="fp32", iters=256, ctx=None):
