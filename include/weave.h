@@ -121,6 +121,14 @@ typedef struct ws_model {
   const uint8_t* k_locked;            /* [K] */
   /* floor (Floor goal): node sections */
   const double* node_w; const double* node_h;                /* [n] (floor_on only) */
+  /* UTM load-test replay (md.protocol, solver_gpu.js:256-318; calibrate.js replayForce / replayDisplacement):
+   * 0 off (relax to the residual), 1 force control (the probe node carries axis·value), 2 displacement control (the
+   * probe node pinned at P0 + value·axis, P0 its uploaded position).  Per schedule value: settle (at most `settle`
+   * iterations, ending early at the residual), then average the probe's travel and reaction along the axis over `avg`
+   * iterations and record them (ws_read_outputs); the instance is done after the last value. */
+  int32_t proto_mode, proto_probe, proto_n, proto_settle, proto_avg, proto_pad;
+  double proto_axis[3];
+  const double* proto_sched;          /* [proto_n] forces (mode 1) or probe displacements (mode 2) */
 } ws_model;
 
 /* Dynamic state and outputs of one instance: uploaded from / downloaded into caller buffers.
@@ -161,7 +169,8 @@ typedef struct ws_state {
 
 /* Per-instance control record, the reference's CT struct as readControl returns it (solver_gpu.js:374-386). */
 typedef struct ws_control {
-  int32_t iter, done, rollback, al_update, stop_at, hist_n, pad0, pad1;
+  int32_t iter, done, rollback, al_update, stop_at, hist_n, phase, inc;   /* phase / inc: replay settle 0, average 1,
+                                                                             finished 2; schedule values recorded */
   double ke_prev, ke, resid, fref, e_stretch, e_bend, e_tors, e_cont, e_shear, res_t, res_f;
   double res_rb, rv[3], rw[3];   /* rigid-body freedoms (ground only): residual part, translation / rotation per iteration */
 } ws_control;
@@ -217,6 +226,8 @@ WS_API int ws_step(ws_batch* batch, int32_t n_iter, const int32_t* instances, in
 WS_API int ws_read_control(ws_batch* batch, ws_control* out);                       /* readControl() :374-386 */
 WS_API int ws_read_history(ws_batch* batch, int32_t b, double* out, int32_t cap_rows, int32_t* n_rows); /* readHistory(b) :389-405 */
 WS_API int ws_download(ws_batch* batch, int32_t b, ws_state* out);                  /* download(b) :431-503 */
+WS_API int ws_read_outputs(ws_batch* batch, int32_t b, double* out, int32_t cap_rows, int32_t* n_rows);
+                          /* readOutputs(b) :408-411: per recorded schedule value (travel, reaction, value, iteration) */
 WS_API int ws_internalize(ws_batch* batch, int32_t b);                              /* internalize(b) :507-520 */
 WS_API int ws_batch_timing(ws_batch* batch, double* ms_last_step, double* ms_kernel_cross, double* ms_kernel_strand,
                            double* ms_kernel_ctrl);   /* hipEvent timing of the last step(); per-kernel means of the last profiled step */

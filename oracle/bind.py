@@ -44,6 +44,8 @@ def lib():
         L.oracle_history.argtypes = [C.c_void_p, A.P_d, A.P_d, A.P_d, C.c_int32]
         L.oracle_rod_apply.argtypes = [C.c_void_p, A.P_d, A.P_d, A.P_d, A.P_d]
         L.oracle_rod_apply.restype = C.c_double
+        L.oracle_replay.argtypes = [C.c_void_p, C.c_int32, C.c_int32, A.P_d, A.P_d, C.c_int32, C.c_int32, C.c_int32, A.P_d]
+        L.oracle_replay.restype = None
         L.oracle_pairwise_sum.argtypes = [A.P_d, C.c_int64]
         L.oracle_pairwise_sum.restype = C.c_double
         _lib = L
@@ -66,6 +68,16 @@ class Oracle:
 
     def step(self, n):
         return lib().oracle_step(self.h, int(n))
+
+    def replay(self, mode, probe, axis, sched, settle, avg):
+        """UTM load-test replay (calibrate.js replayForce / replayDisplacement) from the current state: one row per
+        schedule value of (travel, reaction, value, iteration)"""
+        sch = np.ascontiguousarray(sched, dtype=np.float64)
+        ax = np.ascontiguousarray(axis, dtype=np.float64)
+        out = np.zeros((len(sch), 4))
+        lib().oracle_replay(self.h, {"force": 1, "displacement": 2}[mode], int(probe), ax.ctypes.data_as(A.P_d),
+                            sch.ctypes.data_as(A.P_d), len(sch), int(settle), int(avg), out.ctypes.data_as(A.P_d))
+        return out
 
     def run(self, max_iter, chunk=1000):
         done = 0

@@ -684,6 +684,40 @@ struct Oracle {
     return std::max(std::max(std::max(a, b), std::max(c, d)), std::max(e, 1e-12));
   }
 
+  // UTM load-test replays (calibrate.js:154-215): force control (replayForce: the probe's constant load is axis·F
+  // for each schedule value) or displacement control (replayDisplacement: the probe pinned, placed at P0 + d·axis at
+  // rest).  Per value: settle (runAsync, calibrate.js:119-130: at most `settle` iterations, ending at convergence),
+  // then `avg` single iterations averaging the probe's travel (X − P0)·axis and its reaction·axis.  One row per value:
+  // travel, reaction, value, iteration.  The schedule is used as given (the caller flattens the load cycles,
+  // forceSequence :147-151, and drops non-positive displacements as the reference's job setup does).
+  void replay(int mode, int probe, const double* ax, const double* sched, int nsched, int settle, int avg, double* out) {
+    const double P0[3] = {X[3 * probe], X[3 * probe + 1], X[3 * probe + 2]};
+    double applied[3] = {0, 0, 0};
+    if (mode == 2) {   // pin the probe (calibrate.js:190-193): reactions reported, the rigid freedoms off
+      for (int k = 0; k < 3; k++) nodeFree[3 * probe + k] = 0;
+      fixedAny[probe] = 1; anyAnchor = true;
+    }
+    for (int q = 0; q < nsched; q++) {
+      const double v = sched[q];
+      if (mode == 1) {
+        for (int k = 0; k < 3; k++) loadConst[3 * probe + k] -= applied[k];
+        for (int k = 0; k < 3; k++) { applied[k] = ax[k] * v; loadConst[3 * probe + k] += applied[k]; }
+      } else {
+        for (int k = 0; k < 3; k++) { X[3 * probe + k] = P0[k] + ax[k] * v; V[3 * probe + k] = 0.0; }
+      }
+      converged = false;
+      step(std::max(settle, 1));
+      double accD = 0, accR = 0;
+      const int na = std::max(avg, 1);
+      for (int i = 0; i < na; i++) {
+        step(1);
+        accD += (X[3 * probe] - P0[0]) * ax[0] + (X[3 * probe + 1] - P0[1]) * ax[1] + (X[3 * probe + 2] - P0[2]) * ax[2];
+        accR += reactions[3 * probe] * ax[0] + reactions[3 * probe + 1] * ax[1] + reactions[3 * probe + 2] * ax[2];
+      }
+      out[4 * q] = accD / na; out[4 * q + 1] = accR / na; out[4 * q + 2] = v; out[4 * q + 3] = iteration;
+    }
+  }
+
   // one or more iterations (solver_cpu.js:737-841); returns the number of iterations run
   int step(int nIter) {
     double ms = par.mass_scale;
@@ -819,6 +853,10 @@ WS_API oracle_t* oracle_create(const ws_model* md, const ws_state* st, int32_t k
 }
 WS_API void oracle_free(oracle_t* h) { delete h; }
 WS_API int32_t oracle_step(oracle_t* h, int32_t n_iter) { return h->o.step(n_iter); }
+WS_API void oracle_replay(oracle_t* h, int32_t mode, int32_t probe, const double* axis, const double* sched, int32_t n,
+                          int32_t settle, int32_t avg, double* out) {
+  h->o.replay(mode, probe, axis, sched, n, settle, avg, out);
+}
 WS_API int32_t oracle_converged(oracle_t* h) { return h->o.converged ? 1 : 0; }
 WS_API int32_t oracle_iteration(oracle_t* h) { return h->o.iteration; }
 WS_API double oracle_residual(oracle_t* h) { return h->o.residual; }

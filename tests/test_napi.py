@@ -41,7 +41,7 @@ console.log(JSON.stringify({{info: m.info, x: Array.from(m.st.X.slice(0, 6)), u:
     assert d["info"]["n"] == py.n and d["info"]["K"] == py.K
     assert np.array_equal(np.array(d["x"]), py["X"][:6]) and np.array_equal(np.array(d["u"]), py["u"][:4])
     for name in ("upload", "updateParams", "step", "readControl", "readHistory", "download", "downloadAll",
-                 "internalize", "dispose", "compileInfo"):
+                 "internalize", "dispose", "compileInfo", "readOutputs", "readAllOutputs"):
         assert name in d["names"], name
     assert d["scales"] == {"PB": 0, "FB": 0, "TB": 0, "KB": 0}
 
@@ -72,3 +72,35 @@ def test_js_gpu_solver_matches_python_path(W, gpu_ctx):
     assert np.array_equal(np.array(d["X"]), s["X"])
     assert np.array_equal(np.array(d["stuck"], dtype=np.uint8), s["stuck"])
     assert d["ctl"]["ke"] == c["ke"] and d["hist"] == d["n"] == 600
+
+
+@pytest.mark.gpu
+def test_js_replay_matches_python_path(W, gpu_ctx):
+    """a force-controlled replay on a free node of the 3 m dome (Model.setProtocol, GpuSolver.readOutputs) equals the
+    same replay through the Python path, bitwise"""
+    py = cases.config1()
+    probe = py.n // 2
+    assert py.md.node_free[3 * probe] and py.md.node_free[3 * probe + 2]
+    out = run_node(f"""
+(async () => {{
+  const dev = await W.getDevice();
+  const m = W.Model.hexDome({{span:3000,spacing:300,segment:50,contraction:0.15,noise:1,seed:1}}, {SCENE});
+  m.setProtocol({{ mode: 'force', probe: {probe}, axis: [0, 0, -1], sched: [2.0, 4.0], settle: 300, avg: 20 }});
+  const g = new W.GpuSolver(dev, [m], {{ perSubmit: 64, tile: 64 }});
+  let c;
+  for (let k = 0; k < 20; k++) {{ c = await g.step(128); if (c[0].done) break; }}
+  const rows = await g.readOutputs(0);
+  const all = await g.readAllOutputs();
+  g.dispose();
+  console.log(JSON.stringify({{ rows: Array.from(rows), n: all[0].length, phase: c[0].phase, inc: c[0].inc }}));
+}})().catch((e) => {{ console.error(e); process.exit(1); }});""")
+    d = json.loads(out)
+    py.set_protocol("force", probe, [0, 0, -1], [2.0, 4.0], 300, 20)
+    g = W.GpuSolver([py], precision="fp32", tile=64, per_submit=64)
+    for _ in range(20):
+        if g.step(128)[0]["done"]:
+            break
+    rows = g.read_outputs(0)
+    g.close()
+    assert d["phase"] == 2 and d["inc"] == 2 and d["n"] == 8
+    assert np.array_equal(np.array(d["rows"]).reshape(-1, 4), rows)

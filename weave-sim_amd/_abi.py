@@ -36,6 +36,8 @@ class ws_model(C.Structure):
         ("k_wk", P_d), ("k_hk", P_d), ("k_fc90", P_d), ("k_E90", P_d), ("k_Hs", P_d), ("k_w", P_d), ("k_h", P_d),
         ("k_enabled", P_u8), ("k_locked", P_u8),
         ("node_w", P_d), ("node_h", P_d),
+        ("proto_mode", i32), ("proto_probe", i32), ("proto_n", i32), ("proto_settle", i32), ("proto_avg", i32),
+        ("proto_pad", i32), ("proto_axis", d * 3), ("proto_sched", P_d),
     ]
 
 
@@ -60,7 +62,7 @@ class ws_state(C.Structure):
 
 
 class ws_control(C.Structure):
-    _fields_ = [(n, i32) for n in ("iter", "done", "rollback", "al_update", "stop_at", "hist_n", "pad0", "pad1")] + \
+    _fields_ = [(n, i32) for n in ("iter", "done", "rollback", "al_update", "stop_at", "hist_n", "phase", "inc")] + \
                [(n, d) for n in ("ke_prev", "ke", "resid", "fref", "e_stretch", "e_bend", "e_tors", "e_cont",
                                  "e_shear", "res_t", "res_f", "res_rb")] + \
                [("rv", d * 3), ("rw", d * 3)]
@@ -184,6 +186,7 @@ def hip_lib():
         L.ws_step.argtypes = [vp, i32, P_i32, i32, i32, C.POINTER(ws_control)]
         L.ws_read_control.argtypes = [vp, C.POINTER(ws_control)]
         L.ws_read_history.argtypes = [vp, i32, P_d, i32, C.POINTER(i32)]
+        L.ws_read_outputs.argtypes = [vp, i32, P_d, i32, C.POINTER(i32)]
         L.ws_download.argtypes = [vp, i32, C.POINTER(ws_state)]
         L.ws_internalize.argtypes = [vp, i32]
         L.ws_batch_timing.argtypes = [vp, P_d, P_d, P_d, P_d]
@@ -210,7 +213,7 @@ def hip_lib():
 # every symbol include/weave.h declares (checked by tests/test_abi.py)
 WEAVE_H_SYMBOLS = [
     "ws_ctx_create", "ws_ctx_destroy", "ws_last_error", "ws_device_info", "ws_copy_probe", "ws_batch_create", "ws_batch_destroy",
-    "ws_batch_size", "ws_upload", "ws_update_params", "ws_step", "ws_read_control", "ws_read_history", "ws_download",
+    "ws_batch_size", "ws_upload", "ws_update_params", "ws_step", "ws_read_control", "ws_read_history", "ws_read_outputs", "ws_download",
     "ws_internalize", "ws_batch_timing", "ws_batch_bytes", "ws_batch_profile", "ws_batch_kernel_bench",
     "ws_batch_get_opts", "ws_autotune", "ws_server_create", "ws_server_destroy", "ws_submit",
     "ws_cancel", "ws_pump", "ws_poll", "ws_fetch", "ws_server_clock_ms", "ws_server_stats",

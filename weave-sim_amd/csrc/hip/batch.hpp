@@ -104,6 +104,18 @@ std::vector<Real> toReal(const double* p, size_t n) {
   return v;
 }
 
+// a node's fixed freedoms: bit0..2 x, y, z fixed, bit3 any (a reaction is reported), bit4 all three (the contact's
+// setFixed); a replay's pinned probe (displacement control) has bits 0..3 only: pinned after the contact goal was set
+// up, as calibrate.js:190-193 pins it (solver_gpu.js:215)
+inline uint8_t fix_bits(const ws_model& md, int i) {
+  const uint8_t* f = md.node_free + 3 * i;
+  uint8_t bits = uint8_t((f[0] ? 0 : 1) | (f[1] ? 0 : 2) | (f[2] ? 0 : 4));
+  if (bits) bits |= 8;
+  if ((bits & 7) == 7) bits |= 16;
+  if (md.proto_mode == 2 && i == md.proto_probe) bits = uint8_t(bits | 15);
+  return bits;
+}
+
 // build the device image of one host model (GpuSolver.upload, solver_gpu.js:191-310)
 template <typename Real>
 void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, int tile, int hist_len, Ctl* ctl,
@@ -115,7 +127,14 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   // CpuSolver.step, solver_cpu.js:745)
   bool anchored = md.C > 0;
   for (int i = 0; i < md.n && !anchored; i++) anchored = !(md.node_free[3 * i] && md.node_free[3 * i + 1] && md.node_free[3 * i + 2]);
+  if (md.proto_mode == 2) anchored = true;          // the pinned probe (solver_gpu.js:224)
   const bool rigid = md.par.floor_on && !anchored;
+  if (md.proto_mode < 0 || md.proto_mode > 2) throw Fail{WS_ERR_ARG, "protocol mode must be 0, 1 or 2"};
+  if (md.proto_mode) {
+    if (md.proto_probe < 0 || md.proto_probe >= md.n) throw Fail{WS_ERR_ARG, "protocol probe node out of range"};
+    if (md.proto_n < 1 || !md.proto_sched) throw Fail{WS_ERR_ARG, "GPU replay: empty load schedule"};
+    if (md.proto_settle < 1 || md.proto_avg < 1) throw Fail{WS_ERR_ARG, "protocol settle and avg must be at least 1"};
+  }
   for (int q = 0; q < 2 * (md.par.contacts ? md.K : 0); q++)
     if (md.k_nseg[q] < 1 || md.k_strand[q] < 0 || md.k_strand[q] >= S) throw Fail{WS_ERR_ARG, "a crossing lies on a strand without segments"};
   for (int s2 = 0; s2 < S; s2++)
@@ -137,6 +156,10 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   d.aug = P.augmented; d.al_every = P.al_every; d.hist_len = hist_len;
   d.floor_on = P.floor_on ? 1 : 0; d.floor_z = P.floor_z; d.floor_mu = std::max(P.floor_mu, 0.0);
   d.rigid = rigid ? 1 : 0;
+  d.proto = md.proto_mode; d.probe = md.proto_probe; d.n_inc = md.proto_mode ? md.proto_n : 0;
+  d.settle = md.proto_settle; d.avg = md.proto_avg;
+  d.pax = md.proto_axis[0]; d.pay = md.proto_axis[1]; d.paz = md.proto_axis[2];
+  if (md.proto_mode) { d.p0x = st.X[3 * md.proto_probe]; d.p0y = st.X[3 * md.proto_probe + 1]; d.p0z = st.X[3 * md.proto_probe + 2]; }
   if (rigid) {
     // model radius about the centroid and the rigid speed limits, from the state uploaded at construction (the
     // Floor constructor, solver_cpu.js:546-562)
@@ -206,8 +229,10 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   }
   A.add(d.rpart, size_t(d.ctrl_chunks) * RPART);
   if (d.rigid) A.add(d.rbpart, size_t(d.ntiles) * RBP);
+  if (d.proto) { A.addc(d.sched, d.n_inc); A.add(d.pout, 4 * size_t(d.n_inc)); A.add(d.preact, 3); }
   if (layout_only) return;   // A.size: the device bytes the instance would take
   A.alloc();
+  if (d.proto) h2d(d.sched, std::vector<double>(md.proto_sched, md.proto_sched + d.n_inc));
   // ---- topology
   h2d(d.node_off, std::vector<int32_t>(md.node_offset, md.node_offset + S + 1));
   h2d(d.seg_off, std::vector<int32_t>(md.seg_offset, md.seg_offset + S + 1));
@@ -250,11 +275,7 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   }
   std::vector<uint8_t> fx(n);
   for (int i = 0; i < n; i++) {
-    const uint8_t* f = md.node_free + 3 * i;
-    uint8_t bits = uint8_t((f[0] ? 0 : 1) | (f[1] ? 0 : 2) | (f[2] ? 0 : 4));
-    if (bits) bits |= 8;
-    if ((bits & 7) == 7) bits |= 16;
-    fx[i] = bits;
+    fx[i] = fix_bits(md, i);
   }
   h2d(d.fix, fx);
   h2d(d.l0, std::vector<double>(md.l0, md.l0 + m));
@@ -302,9 +323,15 @@ void upload_state(InstHost<Real>& H, const ws_state& st) {
   const int n = H.n, m = H.m, K = H.K;
   std::vector<double> x(n), y(n), z(n);
   for (int i = 0; i < n; i++) { x[i] = st.X[3 * i]; y[i] = st.X[3 * i + 1]; z[i] = st.X[3 * i + 2]; }
+  double v0 = 0;
+  if (d.proto) HIPCHK(hipMemcpy(&v0, d.sched, sizeof(double), hipMemcpyDeviceToHost));
+  if (d.proto == 2) {   // displacement control starts with the probe at P0 + sched[0]·axis (solver_gpu.js:309, 314-318)
+    x[d.probe] = d.p0x + d.pax * v0; y[d.probe] = d.p0y + d.pay * v0; z[d.probe] = d.p0z + d.paz * v0;
+  }
   for (int b = 0; b < 2; b++) { h2d(d.x[b], x); h2d(d.y[b], y); h2d(d.z[b], z); }
   std::vector<Real> vx(n), vy(n), vz(n);
   for (int i = 0; i < n; i++) { vx[i] = st.V ? Real(st.V[3 * i]) : Real(0); vy[i] = st.V ? Real(st.V[3 * i + 1]) : Real(0); vz[i] = st.V ? Real(st.V[3 * i + 2]) : Real(0); }
+  if (d.proto == 2) { vx[d.probe] = vy[d.probe] = vz[d.probe] = Real(0); }
   h2d(d.vx, vx); h2d(d.vy, vy); h2d(d.vz, vz);
   if (d.floor_on) {   // stick points: the state's, else under the nodes (Floor constructor, solver_cpu.js:540-545)
     std::vector<double> ax(n), ay(n);
@@ -340,6 +367,10 @@ void upload_state(InstHost<Real>& H, const ws_state& st) {
   Ctl c{};
   c.iter = st.iteration; c.stop_at = st.iteration; c.done = 0; c.cur = 0; c.pen_n = st.pen_n;
   c.ke_prev = st.ke_prev; c.resid = st.residual;
+  if (d.proto) {   // the replay starts settling under its first schedule value (solver_gpu.js:296-304)
+    c.cur_f = v0;
+    if (d.proto == 1) { c.pf[0] = d.pax * v0; c.pf[1] = d.pay * v0; c.pf[2] = d.paz * v0; }
+  }
   if (d.rigid) {
     // the reference point of the first evaluation: the centroid of the uploaded nodes (Floor.apply, :588); the rigid
     // velocities start at rest
@@ -373,6 +404,7 @@ struct BatchBase {
   virtual int step(int n_iter, const int32_t* sel, int nsel, int force, ws_control* out, const int32_t* counts = nullptr) = 0;
   virtual int read_control(ws_control* out) = 0;
   virtual int read_history(int b, double* out, int cap, int* n) = 0;
+  virtual int read_outputs(int b, double* out, int cap, int* n) = 0;
   virtual int download(int b, ws_state& out) = 0;
   virtual int internalize(int b) = 0;
   virtual void kernel_bench(int reps, double* out) = 0;
@@ -641,6 +673,7 @@ struct Batch : BatchBase {
     o.e_tors = c.e_tors; o.e_cont = c.e_cont; o.e_shear = c.e_shear; o.res_t = c.res_t; o.res_f = c.res_f;
     o.res_rb = c.res_rb;
     for (int k = 0; k < 3; k++) { o.rv[k] = c.rv[k]; o.rw[k] = c.rw[k]; }
+    o.phase = c.phase; o.inc = c.inc;
     return o;
   }
 
@@ -648,6 +681,18 @@ struct Batch : BatchBase {
     HIPCHK(hipStreamSynchronize(stream));
     fetch_ctls();
     for (int b = 0; b < B; b++) out[b] = to_control(h_ctls[b]);
+    return WS_OK;
+  }
+
+  // replay outputs recorded so far (readOutputs, solver_gpu.js:408-411): rows of travel, reaction, value, iteration
+  int read_outputs(int b, double* out, int cap, int* nrows) override {
+    InstHost<Real>& H = at(b);
+    Ctl c;
+    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipMemcpy(&c, H.d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
+    const int nr = H.d.proto ? std::max(0, std::min(c.inc, cap)) : 0;
+    if (nr > 0 && out) HIPCHK(hipMemcpy(out, H.d.pout, sizeof(double) * 4 * nr, hipMemcpyDeviceToHost));
+    if (nrows) *nrows = nr;
     return WS_OK;
   }
 
@@ -799,11 +844,7 @@ struct Batch : BatchBase {
     h2d(d.lx, lx); h2d(d.ly, ly); h2d(d.lz, lz);
     std::vector<uint8_t> fx(n);
     for (int i = 0; i < n; i++) {
-      const uint8_t* f = md.node_free + 3 * i;
-      uint8_t bits = uint8_t((f[0] ? 0 : 1) | (f[1] ? 0 : 2) | (f[2] ? 0 : 4));
-      if (bits) bits |= 8;
-      if ((bits & 7) == 7) bits |= 16;
-      fx[i] = bits;
+      fx[i] = fix_bits(md, i);
     }
     h2d(d.fix, fx);
     h2d(d.EA, toReal<Real>(md.EA, m)); h2d(d.GA, toReal<Real>(md.GA, m)); h2d(d.EIx, toReal<Real>(md.EIx, m));

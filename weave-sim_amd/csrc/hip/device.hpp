@@ -47,6 +47,11 @@ struct Ctl {
   // (centroid) the next iteration uses, the one the last iteration used, translation / rotation velocities, and the
   // rigid part of the residual
   double rc[3], rc_prev[3], rv[3], rw[3], res_rb;
+  // UTM replay: phase (0 settle, 1 average, 2 finished), iterations in the phase, schedule values recorded,
+  // samples averaged; the probe force (force control), the current schedule value, Σ probe travel per axis and
+  // Σ reaction along the axis over the averaging window
+  int32_t phase, it_phase, inc, acc_n;
+  double pf[3], cur_f, ad[3], ar;
 };
 
 // one crossing side's contribution of this iteration, written by the crossing kernel, gathered per strand tile
@@ -75,6 +80,8 @@ struct Inst {
   int32_t rigid, pad_r;              // rigid: the ground holds the structure alone (no anchor, no clamp)
   double floor_z, floor_mu;          // the ground plane
   double rb_R, rb_vmax, rb_wmax;     // rigid body: model radius about the centroid, speed limits per iteration
+  int32_t proto, probe, n_inc, settle, avg, pad_p;   // UTM replay: mode (0 off, 1 force, 2 displacement), probe node, …
+  double pax, pay, paz, p0x, p0y, p0z;               // … load axis, probe position at upload
   // topology
   const int32_t* node_off;   // [S+1]
   const int32_t* seg_off;    // [S+1]
@@ -136,6 +143,9 @@ struct Inst {
   double* cpart;             // [CPART][ncblk]
   double* rpart;             // [ctrl_chunks][RPART] control-chunk partials (ctrl_chunks > 1)
   double* rbpart;            // [RBP][ntiles] rigid-body sums per strand tile (rigid only)
+  const double* sched;       // [n_inc] replay schedule (proto != 0)
+  double* pout;              // [4 n_inc] replay outputs: travel, reaction, value, iteration
+  double* preact;            // [3] the pinned probe's reaction of the last iteration (displacement control)
 };
 
 // one strand tile: strand-local node range [k0, k1) plus the strand's layout, so that a tile reads its geometry in

@@ -999,6 +999,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       qframe_d(acc[0], acc[1], acc[2], acc[3], Rn);
     }
     if (localLoad) for (int c = 0; c < 3; c++) ld[c] += Real(Rn[c]) * llx + Real(Rn[3 + c]) * lly + Real(Rn[6 + c]) * llz;
+    if (I.proto == 1 && i == I.probe) for (int c = 0; c < 3; c++) ld[c] += Real(ctl->pf[c]);   // the replay's probe force
     mxL = fmax(fabs(double(ld[0])), fmax(fabs(double(ld[1])), fabs(double(ld[2]))));
     for (int c = 0; c < 3; c++) F[c] += ld[c] + Fc[c];
     if (I.floor_on) {
@@ -1043,6 +1044,8 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
     if (OUT) {
       if (fx & 8) { outR[3 * i] = -double(F[0]); outR[3 * i + 1] = -double(F[1]); outR[3 * i + 2] = -double(F[2]); }
       else { outR[3 * i] = outR[3 * i + 1] = outR[3 * i + 2] = 0.0; }
+    } else if (I.proto == 2 && i == I.probe) {
+      gl(I.preact)[0] = -double(F[0]); gl(I.preact)[1] = -double(F[1]); gl(I.preact)[2] = -double(F[2]);
     }
     if (fx & 1) F[0] = Real(0);
     if (fx & 2) F[1] = Real(0);
@@ -1375,8 +1378,45 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   if (!(c.iter > I.warmup)) resid = INFINITY;
   c.resid = resid; c.fref = fref; c.res_f = rf; c.res_t = rt;
   c.e_stretch = acc[1]; c.e_shear = acc[2]; c.e_bend = acc[3]; c.e_tors = acc[4]; c.e_cont = eCont;
-  if (resid <= I.tol) c.done = 1;
+  const bool conv = resid <= I.tol;
+  if (I.proto == 0 && conv) c.done = 1;
   if (!rollback) c.cur = c.cur ^ 1;
+  if (I.proto != 0) {
+    // UTM replay (the protocol branch of the reference's control kernel, wgsl.js:1048-1093; calibrate.js:154-215):
+    // settle at most `settle` iterations (ending at the residual), then average the probe's travel from P0 and its
+    // reaction along the axis over `avg` iterations — the state after each iteration, i.e. before it when rolled
+    // back — record them, and go on with the next schedule value
+    c.it_phase += 1;
+    if (c.phase == 0) {
+      if (conv || c.it_phase >= I.settle) {
+        c.phase = 1; c.it_phase = 0; c.acc_n = 0;
+        c.ad[0] = c.ad[1] = c.ad[2] = 0.0; c.ar = 0.0;
+      }
+    } else if (c.phase == 1) {
+      const int pr = I.probe;
+      c.ad[0] += gl(I.x[c.cur])[pr] - I.p0x; c.ad[1] += gl(I.y[c.cur])[pr] - I.p0y; c.ad[2] += gl(I.z[c.cur])[pr] - I.p0z;
+      if (I.proto == 2) c.ar += gl(I.preact)[0] * I.pax + gl(I.preact)[1] * I.pay + gl(I.preact)[2] * I.paz;
+      c.acc_n += 1;
+      if (c.it_phase >= I.avg) {
+        const double nn = double(c.acc_n);
+        const auto o = gl(I.pout) + size_t(c.inc) * 4;
+        o[0] = (c.ad[0] * I.pax + c.ad[1] * I.pay + c.ad[2] * I.paz) / nn;
+        o[1] = c.ar / nn; o[2] = c.cur_f; o[3] = double(c.iter);
+        c.inc += 1; c.it_phase = 0; c.phase = 0;
+        if (c.inc >= I.n_inc) { c.phase = 2; c.done = 1; }
+        else {
+          const double v = gl(I.sched)[c.inc];
+          c.cur_f = v;
+          if (I.proto == 1) { c.pf[0] = I.pax * v; c.pf[1] = I.pay * v; c.pf[2] = I.paz * v; }
+          else {
+            // displacement control: the probe node at P0 + v·axis, at rest, in the state the next iteration reads
+            gl(I.x[c.cur])[pr] = I.p0x + I.pax * v; gl(I.y[c.cur])[pr] = I.p0y + I.pay * v; gl(I.z[c.cur])[pr] = I.p0z + I.paz * v;
+            gl(I.vx)[pr] = Real(0); gl(I.vy)[pr] = Real(0); gl(I.vz)[pr] = Real(0);
+          }
+        }
+      }
+    }
+  }
   c.live_flag = 1;   // an iteration has been evaluated
   if (I.hist_len > 0) {
     const auto h = gl(I.hist) + size_t((c.iter - 1) % I.hist_len) * HIST_STRIDE;

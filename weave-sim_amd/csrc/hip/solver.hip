@@ -138,6 +138,10 @@ WS_API int ws_read_history(ws_batch* batch, int32_t b, double* out, int32_t cap,
   if (!batch || !out || !n_rows || b < 0 || b >= batch->impl->B) return WS_ERR_ARG;
   BATCH_CALL(batch->impl->read_history(b, out, cap, n_rows));
 }
+WS_API int ws_read_outputs(ws_batch* batch, int32_t b, double* out, int32_t cap, int32_t* n_rows) {
+  if (!batch || !n_rows || b < 0 || b >= batch->impl->B || cap < 0) return WS_ERR_ARG;
+  BATCH_CALL(batch->impl->read_outputs(b, out, cap, n_rows));
+}
 WS_API int ws_download(ws_batch* batch, int32_t b, ws_state* out) {
   if (!batch || !out || b < 0 || b >= batch->impl->B) return WS_ERR_ARG;
   BATCH_CALL(batch->impl->download(b, *out));

@@ -32,6 +32,7 @@ struct Model {
   ws_model md{};
   ws_state st{};
   wsm_info info{};
+  std::vector<double> sched;   // the replay schedule md.proto_sched points at
 };
 struct Batch { ws_ctx* ctx = nullptr; ws_batch* b = nullptr; int32_t B = 0; };
 struct Server { ws_ctx* ctx = nullptr; ws_server* s = nullptr; };
@@ -75,6 +76,7 @@ napi_value ctl_object(napi_env env, const ws_control& c) {
   napi_value o; napi_create_object(env, &o);
   set(env, o, "iter", i32(env, c.iter)); set(env, o, "done", i32(env, c.done)); set(env, o, "rollback", i32(env, c.rollback));
   set(env, o, "alUpdate", i32(env, c.al_update)); set(env, o, "stopAt", i32(env, c.stop_at)); set(env, o, "histN", i32(env, c.hist_n));
+  set(env, o, "phase", i32(env, c.phase)); set(env, o, "inc", i32(env, c.inc));
   set(env, o, "kePrev", num(env, c.ke_prev)); set(env, o, "ke", num(env, c.ke)); set(env, o, "resid", num(env, c.resid));
   set(env, o, "fref", num(env, c.fref)); set(env, o, "eStretch", num(env, c.e_stretch)); set(env, o, "eBend", num(env, c.e_bend));
   set(env, o, "eTors", num(env, c.e_tors)); set(env, o, "eCont", num(env, c.e_cont)); set(env, o, "eShear", num(env, c.e_shear));
@@ -191,6 +193,39 @@ napi_value ModelClone(napi_env env, napi_callback_info info) {
   wsm_model* h = nullptr;
   if (wsm_clone(m->h, &h)) { napi_throw_error(env, nullptr, wsm_last_error()); return nullptr; }
   return wrap_model(env, h);
+}
+
+// modelSetProtocol(model, { mode: 'force' | 'displacement' | null, probe, axis: [x, y, z], sched: [...], settle, avg })
+// (md.protocol of the reference's GPU solver, solver_gpu.js:256-318)
+napi_value ModelSetProtocol(napi_env env, napi_callback_info info) {
+  napi_value a[2]; get_args(env, info, 2, a);
+  Model* m = ext<Model>(env, a[0]);
+  if (!m) { napi_throw_type_error(env, nullptr, "not a model"); return nullptr; }
+  napi_value v; bool has = false;
+  std::string mode;
+  if (napi_has_named_property(env, a[1], "mode", &has) == napi_ok && has && napi_get_named_property(env, a[1], "mode", &v) == napi_ok) {
+    napi_valuetype t; napi_typeof(env, v, &t);
+    if (t == napi_string) { char buf[32]; size_t len = 0; napi_get_value_string_utf8(env, v, buf, sizeof buf, &len); mode.assign(buf, len); }
+  }
+  if (mode.empty()) { m->md.proto_mode = 0; m->sched.clear(); m->md.proto_sched = nullptr; return undef(env); }
+  if (mode != "force" && mode != "displacement") { napi_throw_type_error(env, nullptr, "mode: 'force' or 'displacement'"); return nullptr; }
+  m->sched.clear();
+  if (napi_get_named_property(env, a[1], "sched", &v) == napi_ok) {
+    uint32_t len = 0; napi_get_array_length(env, v, &len);
+    for (uint32_t k = 0; k < len; k++) { napi_value e; double x = 0; napi_get_element(env, v, k, &e); napi_get_value_double(env, e, &x); m->sched.push_back(x); }
+  }
+  double ax[3] = {0, 0, 0};
+  if (napi_get_named_property(env, a[1], "axis", &v) == napi_ok) {
+    for (uint32_t k = 0; k < 3; k++) { napi_value e; napi_get_element(env, v, k, &e); napi_get_value_double(env, e, &ax[k]); }
+  }
+  m->md.proto_mode = mode == "force" ? 1 : 2;
+  m->md.proto_probe = int32_t(getd(env, a[1], "probe", 0));
+  m->md.proto_settle = int32_t(getd(env, a[1], "settle", 1));
+  m->md.proto_avg = int32_t(getd(env, a[1], "avg", 1));
+  m->md.proto_n = int32_t(m->sched.size());
+  m->md.proto_sched = m->sched.data();
+  for (int k = 0; k < 3; k++) m->md.proto_axis[k] = ax[k];
+  return undef(env);
 }
 
 napi_value ModelInfo(napi_env env, napi_callback_info info) {
@@ -373,6 +408,29 @@ napi_value BatchReadHistory(napi_env env, napi_callback_info info) {
   return queue_job(env, j, "ws_read_history");
 }
 
+napi_value BatchReadOutputs(napi_env env, napi_callback_info info) {
+  napi_value a[2]; get_args(env, info, 2, a);
+  Batch* b = ext<Batch>(env, a[0]);
+  int32_t idx = 0; napi_get_value_int32(env, a[1], &idx);
+  if (!b || !b->b) { napi_throw_type_error(env, nullptr, "readOutputs: no batch"); return nullptr; }
+  const int32_t cap = 4096;
+  auto rows = std::make_shared<std::vector<double>>(size_t(cap) * 4);
+  auto nrows = std::make_shared<int32_t>(0);
+  Job* j = new Job();
+  ws_batch* bh = b->b;
+  j->run = [=]() { return ws_read_outputs(bh, idx, rows->data(), cap, nrows.get()); };
+  j->done = [=](napi_env e) {
+    napi_value ab, ta; void* data = nullptr;
+    size_t bytes = size_t(*nrows) * 4 * sizeof(double);
+    napi_create_arraybuffer(e, bytes, &data, &ab);
+    if (bytes) std::memcpy(data, rows->data(), bytes);
+    napi_create_typedarray(e, napi_float64_array, size_t(*nrows) * 4, ab, 0, &ta);
+    return ta;
+  };
+  j->error = ctx_error(b->ctx);
+  return queue_job(env, j, "ws_read_outputs");
+}
+
 napi_value BatchDownload(napi_env env, napi_callback_info info) {
   napi_value a[3]; get_args(env, info, 3, a);
   Batch* b = ext<Batch>(env, a[0]);
@@ -517,6 +575,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"modelHexDome", ModelHexDome}, {"modelClone", ModelClone}, {"modelInfo", ModelInfo}, {"modelState", ModelState},
       {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy}, {"batchOpts", BatchOpts}, {"batchStep", BatchStep},
       {"batchReadControl", BatchReadControl}, {"batchReadHistory", BatchReadHistory}, {"batchDownload", BatchDownload},
+      {"batchReadOutputs", BatchReadOutputs}, {"modelSetProtocol", ModelSetProtocol},
       {"batchUpload", BatchUpload}, {"batchUpdateParams", BatchUpdateParams}, {"batchInternalize", BatchInternalize},
       {"serverCreate", ServerCreate}, {"serverDestroy", ServerDestroy}, {"serverSubmit", ServerSubmit},
       {"serverPump", ServerPump}, {"serverPoll", ServerPoll}, {"serverCancel", ServerCancel}, {"serverClockMs", ServerClock},

@@ -42,6 +42,10 @@ class Model {
     this.st.history = { kinetic: [], strain: [], bending: [], torsion: [], stretch: [], shear: [], contact: [], residual: [] };
   }
   static hexDome(spec, scene = {}) { return new Model(native().modelHexDome(spec, scene)); }
+  /** md.protocol of the reference (solver_gpu.js:256-318, calibrate.js:345-365): { mode: 'force' | 'displacement' |
+   *  null, probe, axis, sched, settle, avg }; the schedule is used as given (flatten the load cycles and drop
+   *  non-positive displacements first, as CalibrationSet's jobs do) */
+  setProtocol(p) { native().modelSetProtocol(this.h, p || {}); this.protocol = p || null; return this; }
   clone() { return new Model(native().modelClone(this.h)); }
   get n() { return this.info.n; }
   get m() { return this.info.m; }
@@ -136,9 +140,16 @@ class GpuSolver {
   /** internalize(b) (solver_gpu.js:507-520): the current shape becomes the displacement origin, slip lengths reset */
   internalize(b) { native().batchInternalize(this.h, b); }
 
-  /** readOutputs / readAllOutputs (solver_gpu.js:408-416) are the UTM protocol replays' outputs, not on this path */
-  async readOutputs() { throw new Error('protocol replays are not part of the MI355X solver'); }
-  async readAllOutputs() { throw new Error('protocol replays are not part of the MI355X solver'); }
+  /** readOutputs(b) (solver_gpu.js:408-411): the replay outputs recorded so far, a Float64Array of rows (travel,
+   *  reaction, value, iteration) */
+  readOutputs(b) { return this._serial(() => native().batchReadOutputs(this.h, b)); }
+
+  /** readAllOutputs() (solver_gpu.js:412-416): every instance's outputs */
+  async readAllOutputs() {
+    const out = [];
+    for (let b = 0; b < this.B; b++) out.push(await this.readOutputs(b));
+    return out;
+  }
 
   /** dispose() (solver_gpu.js:522) */
   dispose() { if (this.h) { native().batchDestroy(this.h); this.h = null; } }

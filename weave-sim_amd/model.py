@@ -153,6 +153,29 @@ class Model:
         A.model_lib().wsm_clone(self.h, C.byref(out))
         return Model(out.value)
 
+    def set_protocol(self, mode, probe, axis, sched, settle, avg):
+        """UTM load-test replay on the device (md.protocol of the reference's GPU solver): mode "force" (the probe
+        node carries axis·value) or "displacement" (the probe pinned at P0 + value·axis); per schedule value at most
+        `settle` iterations (ending at the residual), then the probe's travel and reaction along the axis averaged over
+        `avg` iterations (GpuSolver.read_outputs).  mode None switches it off."""
+        if mode is None:
+            self.md.proto_mode = 0
+            self._proto = None
+            return self
+        codes = {"force": 1, "displacement": 2}
+        sch = np.ascontiguousarray(sched, dtype=np.float64)
+        ax = np.asarray(axis, dtype=np.float64)
+        self.md.proto_mode = codes[mode]
+        self.md.proto_probe = int(probe)
+        self.md.proto_n = len(sch)
+        self.md.proto_settle = int(settle)
+        self.md.proto_avg = int(avg)
+        for k in range(3):
+            self.md.proto_axis[k] = float(ax[k])
+        self.md.proto_sched = sch.ctypes.data_as(A.P_d)
+        self._proto = sch                      # the schedule stays alive with the model
+        return self
+
     # ------------------------------------------------------------------ access
     @property
     def n(self): return self.md.n

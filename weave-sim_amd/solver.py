@@ -167,6 +167,13 @@ class GpuSolver:
     def internalize(self, b):
         self.ctx.check(self.lib.ws_internalize(self.h, b))
 
+    def read_outputs(self, b, cap=4096):
+        """replay outputs of instance b recorded so far: rows of (travel, reaction, value, iteration)"""
+        out = np.zeros((max(cap, 1), 4))
+        n = C.c_int32()
+        self.ctx.check(self.lib.ws_read_outputs(self.h, b, out.ctypes.data_as(A.P_d), cap, C.byref(n)))
+        return out[:n.value]
+
     def read_history(self, b, cap=4096):
         out = np.zeros((cap, 9))
         n = C.c_int32()
