@@ -216,8 +216,14 @@ def run_serve(args):
     srv = W.Server(precision="fp32", chunk=args.chunk, tile=64, max_slots=args.slots)
     lat, nodes, next_i, done, results = [], 0, 0, 0, []
     t0 = time.perf_counter()
+    t_report = 30.0
     while done < len(arrivals):
         now = time.perf_counter() - t0
+        if now >= t_report:                   # progress of the stream on stderr (the JSON line stays on stdout)
+            st = srv.stats()
+            print(f"serve: {now:.0f} s, {next_i} submitted, {done} done, {st['active']} resident, {st['queued']} queued",
+                  file=sys.stderr, flush=True)
+            t_report += 30.0
         while next_i < len(arrivals) and arrivals[next_i] <= now:
             with lock:
                 m = built.pop(next_i, None)

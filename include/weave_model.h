@@ -100,6 +100,10 @@ WS_API int wsm_info_get(wsm_model* mdl, wsm_info* info);
 WS_API int wsm_add_load(wsm_model* mdl, int32_t node, const double* F);   /* Loads.const[node] += F (tests) */
 WS_API int wsm_set_strength(wsm_model* mdl, double strength, double stick_strength); /* re-derive masses (rebuildGoals) */
 WS_API int wsm_clone(wsm_model* mdl, wsm_model** out);
+/* segment moments of the results (buildResults, solver_cpu.js:878-891): node moments from the joint moments M [3p] and
+ * the clamp moments cM [2C] of a downloaded state, averaged over each segment's two nodes, an end without a joint
+ * (without a joint or clamp for Mx, My) taking its neighbour's value; Mx, My, T [m] */
+WS_API int wsm_segment_moments(const ws_model* md, const double* M, const double* cM, double* Mx, double* My, double* T);
 WS_API int wsm_mesh_counts(wsm_model* mdl, int32_t* nV, int32_t* nF);
 WS_API const char* wsm_last_error(void);
 /* sizeof of the ABI structs, for binding checks: ws_params, ws_model, ws_state, ws_control, ws_batch_opts,

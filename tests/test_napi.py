@@ -104,3 +104,35 @@ def test_js_replay_matches_python_path(W, gpu_ctx):
     g.close()
     assert d["phase"] == 2 and d["inc"] == 2 and d["n"] == 8
     assert np.array_equal(np.array(d["rows"]).reshape(-1, 4), rows)
+
+
+@pytest.mark.gpu
+def test_js_build_results_match_python(W, gpu_ctx):
+    """buildResults / resultsJSON in Node over a downloaded 3 m dome equal the Python results over the same download"""
+    import importlib
+    R = importlib.import_module("weave-sim_amd.results")
+    out = run_node(f"""
+(async () => {{
+  const dev = await W.getDevice();
+  const m = W.Model.hexDome({{span:3000,spacing:300,segment:50,contraction:0.15,noise:1,seed:1}}, {SCENE});
+  const g = new W.GpuSolver(dev, [m], {{ perSubmit: 64, tile: 64 }});
+  await g.step(400);
+  await g.download(0);
+  g.dispose();
+  const r = W.buildResults(m);
+  const doc = W.resultsJSON(m, r);
+  console.log(JSON.stringify({{ Mx: Array.from(r.segment.Mx), T: Array.from(r.segment.T), N: Array.from(r.segment.N),
+    Fn: Array.from(r.knot.Fn), disp: Array.from(r.disp), nreact: r.reactions.size, doc }}));
+}})().catch((e) => {{ console.error(e); process.exit(1); }});""")
+    d = json.loads(out)
+    py = cases.config1()
+    g = W.GpuSolver([py], precision="fp32", tile=64, per_submit=64)
+    g.step(400)
+    s = g.state(0)
+    g.close()
+    r = R.build_results(py, s)
+    assert np.array_equal(np.array(d["Mx"]), r["segment"]["Mx"]) and np.array_equal(np.array(d["T"]), r["segment"]["T"])
+    assert np.array_equal(np.array(d["N"]), r["segment"]["N"]) and np.array_equal(np.array(d["Fn"]), r["knot"]["Fn"])
+    assert np.array_equal(np.array(d["disp"]), r["disp"]) and d["nreact"] == len(r["reactions"])
+    doc = R.results_json(py, r)
+    assert d["doc"]["segment"]["My"] == doc["segment"]["My"] and d["doc"]["reactions"].keys() == doc["reactions"].keys()

@@ -1,0 +1,54 @@
+"""Results and the export document (buildResults, web/js/solver_cpu.js:878-922; resultsJSON, web/js/app.js:360-390)
+over the oracle's converged cantilever, pinned by statics: the bending moment of a tip-loaded cantilever is
+F·(L − x) along it (each segment the mean of its two nodes, the free end's segment its inner node's value), no
+torsion, the clamp carries the load, and the tip deflection is the Timoshenko value within the 10-segment rod's
+0.5 %."""
+import importlib
+import json
+
+import numpy as np
+
+import cases
+from oracle.bind import Oracle
+
+R = importlib.import_module("weave-sim_amd.results")
+F, L = 10.0, 1000.0
+
+
+def converged_cantilever():
+    m = cases.cantilever(F=F, tol=1e-10)
+    o = Oracle(m)
+    o.run(60000)
+    assert o.converged
+    return m, o.state()
+
+
+def test_segment_moments_follow_statics():
+    m, st = converged_cantilever()
+    r = R.build_results(m, st)
+    x = m["X"].reshape(-1, 3)[:, 0]                         # undeformed node positions along the beam
+    Mb = np.hypot(r["segment"]["Mx"], r["segment"]["My"])
+    expect = F * (L - 0.5 * (x[:-1] + x[1:]))
+    expect[-1] = F * (L - x[-2])                            # the free end has no joint: its segment takes the inner node's
+    assert np.allclose(Mb, expect, rtol=1e-4)
+    assert np.abs(r["segment"]["T"]).max() < 1e-6 * F * L
+    (key, react), = r["reactions"].items()
+    assert key == (0, 0) and abs(react[2] - F) < 1e-6 * F
+    E, G, w, h = 10000.0, 600.0, 60.0, 30.0
+    tip = F * (L ** 3 / (3 * E * w * h ** 3 / 12) + L / (5.0 / 6.0 * G * w * h))
+    assert abs(-r["disp"][-1] / tip - 1) < 0.006
+
+
+def test_results_json_document():
+    m, st = converged_cantilever()
+    r = R.build_results(m, st)
+    doc = json.loads(json.dumps(R.results_json(m, r)))
+    assert doc["converged"] and set(doc["segment"]) == set(R.SEG_LAYERS)
+    assert len(doc["segment"]["Mx"]) == 1 and len(doc["segment"]["Mx"][0]) == 10
+    assert len(doc["disp"][0]) == 11 and doc["reactions"]["0,0"][2] > 0
+    c = cases.crossing(**cases.CROSS_CASES["cross_stick"])
+    o = Oracle(c)
+    o.step(500)
+    rc = R.build_results(c, o.state())
+    assert set(rc["knot"]) == {"Fn", "Ft", "slip", "state", "gap", "ctype", "pressure", "util", "theta", "phiA", "phiB"}
+    assert len(rc["knot"]["Fn"]) == c.md.K and rc["knot"]["Fn"][0] > 0

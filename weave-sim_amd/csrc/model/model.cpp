@@ -1065,6 +1065,42 @@ WS_API int wsm_get(wsm_model* mdl, ws_model* model, ws_state* state) {
 
 WS_API int wsm_info_get(wsm_model* mdl, wsm_info* info) { if (!mdl || !info) return WS_ERR_ARG; *info = mdl->md->info; return WS_OK; }
 
+WS_API int wsm_segment_moments(const ws_model* md, const double* M, const double* cM, double* Mx, double* My, double* T) {
+  if (!md || !Mx || !My || !T || (md->p && !M) || (md->C && !cM)) { g_err = "wsm_segment_moments: bad arguments"; return WS_ERR_ARG; }
+  const int n = md->n, m = md->m;
+  std::vector<double> nodeM(3 * size_t(n), 0.0), segM(3 * size_t(m), 0.0);
+  std::vector<uint8_t> has0(n, 0);
+  for (int j = 0; j < md->p; j++) {
+    const int i = md->j_node[j];
+    for (int k = 0; k < 3; k++) nodeM[3 * i + k] = M[3 * j + k];
+    has0[i] = 1;
+  }
+  std::vector<uint8_t> hasB = has0;
+  for (int c = 0; c < md->C; c++) {
+    const int i = md->c_node[c];
+    nodeM[3 * i] = cM[2 * c]; nodeM[3 * i + 1] = cM[2 * c + 1];
+    hasB[i] = 1;
+  }
+  for (int j = 0; j < m; j++)
+    for (int k = 0; k < 3; k++) segM[3 * j + k] = 0.5 * (nodeM[3 * md->seg_i0[j] + k] + nodeM[3 * md->seg_i1[j] + k]);
+  // the bending pair falls back where an end has neither joint nor clamp, the torsion where it has no joint: first
+  // over the start nodes, then over the end nodes (the later assignment wins when both ends lack one)
+  for (int pass = 0; pass < 2; pass++) {
+    const std::vector<uint8_t>& has = pass == 0 ? hasB : has0;
+    const int k0 = pass == 0 ? 0 : 2, k1 = pass == 0 ? 2 : 3;
+    for (int j = 0; j < m; j++) {
+      const int a = md->seg_i0[j], b = md->seg_i1[j];
+      for (int k = k0; k < k1; k++) if (!has[a]) segM[3 * j + k] = nodeM[3 * b + k];
+    }
+    for (int j = 0; j < m; j++) {
+      const int a = md->seg_i0[j], b = md->seg_i1[j];
+      for (int k = k0; k < k1; k++) if (!has[b]) segM[3 * j + k] = nodeM[3 * a + k];
+    }
+  }
+  for (int j = 0; j < m; j++) { Mx[j] = segM[3 * j]; My[j] = segM[3 * j + 1]; T[j] = segM[3 * j + 2]; }
+  return WS_OK;
+}
+
 WS_API int wsm_add_load(wsm_model* mdl, int32_t node, const double* F) {
   if (!mdl || !F || node < 0 || node >= mdl->md->n) { g_err = "wsm_add_load: bad node"; return WS_ERR_ARG; }
   for (int c = 0; c < 3; c++) mdl->md->loadConst[3 * node + c] += F[c];

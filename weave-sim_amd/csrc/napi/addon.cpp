@@ -5,6 +5,7 @@
 // to JavaScript as typed arrays over the model's own memory, so a download overwrites them in place.
 #include <node_api.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <functional>
@@ -228,6 +229,26 @@ napi_value ModelSetProtocol(napi_env env, napi_callback_info info) {
   return undef(env);
 }
 
+// modelSegmentMoments(model) → { Mx, My, T } (Float64Array[m]) from the model's downloaded joint and clamp moments
+napi_value ModelSegmentMoments(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  Model* m = ext<Model>(env, a[0]);
+  if (!m) { napi_throw_type_error(env, nullptr, "not a model"); return nullptr; }
+  const size_t nm = size_t(m->md.m);
+  napi_value o; napi_create_object(env, &o);
+  double* out[3] = {nullptr, nullptr, nullptr};
+  napi_value arrs[3];
+  for (int k = 0; k < 3; k++) {
+    napi_value ab; void* data = nullptr;
+    napi_create_arraybuffer(env, std::max<size_t>(nm, 1) * sizeof(double), &data, &ab);
+    out[k] = static_cast<double*>(data);
+    napi_create_typedarray(env, napi_float64_array, nm, ab, 0, &arrs[k]);
+  }
+  if (wsm_segment_moments(&m->md, m->st.M, m->st.cM, out[0], out[1], out[2])) { napi_throw_error(env, nullptr, wsm_last_error()); return nullptr; }
+  set(env, o, "Mx", arrs[0]); set(env, o, "My", arrs[1]); set(env, o, "T", arrs[2]);
+  return o;
+}
+
 napi_value ModelInfo(napi_env env, napi_callback_info info) {
   napi_value a[1]; get_args(env, info, 1, a);
   Model* m = ext<Model>(env, a[0]);
@@ -238,6 +259,20 @@ napi_value ModelInfo(napi_env env, napi_callback_info info) {
   set(env, o, "K", i32(env, f.K)); set(env, o, "C", i32(env, f.C)); set(env, o, "S", i32(env, f.S));
   set(env, o, "rings", i32(env, f.rings)); set(env, o, "subdivisions", i32(env, f.subdivisions));
   set(env, o, "span", num(env, f.span_plan)); set(env, o, "rise", num(env, f.rise));
+  const ws_params& P = m->md.par;
+  set(env, o, "contacts", i32(env, P.contacts)); set(env, o, "seatLaw", i32(env, P.seat_law));
+  set(env, o, "rotFriction", i32(env, P.rot_friction)); set(env, o, "augmented", i32(env, P.augmented));
+  // strand layout and free freedoms (copies): what the results need to split arrays per strand and find anchors
+  auto copy = [&](const char* name, const void* src, size_t count, napi_typedarray_type t, size_t esz) {
+    napi_value ab, ta; void* data = nullptr;
+    napi_create_arraybuffer(env, std::max<size_t>(count, 1) * esz, &data, &ab);
+    if (count && src) std::memcpy(data, src, count * esz);
+    napi_create_typedarray(env, t, count, ab, 0, &ta);
+    set(env, o, name, ta);
+  };
+  copy("nodeOffset", m->md.node_offset, size_t(f.S) + 1, napi_int32_array, 4);
+  copy("segOffset", m->md.seg_offset, size_t(f.S) + 1, napi_int32_array, 4);
+  copy("nodeFree", m->md.node_free, 3 * size_t(f.n), napi_uint8_array, 1);
   return o;
 }
 
@@ -576,6 +611,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy}, {"batchOpts", BatchOpts}, {"batchStep", BatchStep},
       {"batchReadControl", BatchReadControl}, {"batchReadHistory", BatchReadHistory}, {"batchDownload", BatchDownload},
       {"batchReadOutputs", BatchReadOutputs}, {"modelSetProtocol", ModelSetProtocol},
+      {"modelSegmentMoments", ModelSegmentMoments},
       {"batchUpload", BatchUpload}, {"batchUpdateParams", BatchUpdateParams}, {"batchInternalize", BatchInternalize},
       {"serverCreate", ServerCreate}, {"serverDestroy", ServerDestroy}, {"serverSubmit", ServerSubmit},
       {"serverPump", ServerPump}, {"serverPoll", ServerPoll}, {"serverCancel", ServerCancel}, {"serverClockMs", ServerClock},

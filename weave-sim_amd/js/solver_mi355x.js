@@ -155,6 +155,52 @@ class GpuSolver {
   dispose() { if (this.h) { native().batchDestroy(this.h); this.h = null; } }
 }
 
+/** buildResults (solver_cpu.js:878-922) over a downloaded model: per-segment Mx My T N Vx Vy, per-crossing contact
+ *  layers, per-node displacement, reactions of the anchored nodes keyed "strand,node", energies.  Knots are indexed
+ *  by crossing; the ground's per-node force is not part of a download. */
+function buildResults(model) {
+  const st = model.st, info = model.info, n = info.n, m = info.m, K = info.K;
+  const mom = native().modelSegmentMoments(model.h);
+  const V2 = st.V2 || new Float64Array(2 * m);
+  const segment = { Mx: mom.Mx, My: mom.My, T: mom.T, N: Float64Array.from(st.N || new Float64Array(m)),
+    Vx: Float64Array.from({ length: m }, (_, j) => V2[2 * j]), Vy: Float64Array.from({ length: m }, (_, j) => V2[2 * j + 1]) };
+  const disp = new Float64Array(3 * n);
+  for (let i = 0; i < 3 * n; i++) disp[i] = st.X[i] - st.X0[i];
+  const knot = {};
+  if (info.contacts && K) {
+    const pick = (a) => Float64Array.from(a || new Float64Array(K));
+    Object.assign(knot, { Fn: pick(st.Fn), Ft: pick(st.Ft), slip: pick(st.slip), state: pick(st.state), gap: pick(st.gap),
+      ctype: pick(st.ctype), pressure: pick(st.pressure), util: pick(st.util), theta: pick(st.theta),
+      phiA: Float64Array.from({ length: K }, (_, k) => st.phi[2 * k]), phiB: Float64Array.from({ length: K }, (_, k) => st.phi[2 * k + 1]) });
+    if (info.augmented) knot.lambda = pick(st.lam);
+  } else for (const k of ['Fn', 'Ft', 'slip', 'state']) knot[k] = new Float64Array(K);
+  const reactions = new Map();
+  for (let s = 0; s < info.S; s++) {
+    for (let i = info.nodeOffset[s]; i < info.nodeOffset[s + 1]; i++) {
+      if (info.nodeFree[3 * i] && info.nodeFree[3 * i + 1] && info.nodeFree[3 * i + 2]) continue;
+      reactions.set(`${s},${i - info.nodeOffset[s]}`, [st.reactions[3 * i], st.reactions[3 * i + 1], st.reactions[3 * i + 2], 0, 0, 0]);
+    }
+  }
+  const e = model.energies || {};
+  const beam = (e.bending || 0) + (e.torsion || 0) + (e.stretch || 0) + (e.shear || 0);
+  return { iterations: st.iteration, converged: !!model.converged, residual: st.residual, segment, knot, disp, reactions,
+    energies: { kinetic: model.kePrev || 0, bending: e.bending || 0, torsion: e.torsion || 0, stretch: e.stretch || 0,
+      shear: e.shear || 0, contact: e.contact || 0, beam, strain: beam + (e.contact || 0) }, rod: 'cosserat' };
+}
+
+/** resultsJSON (app.js:360-390): per-strand segment arrays, knot values by crossing index, per-strand displacements,
+ *  reactions */
+function resultsJSON(model, r) {
+  const info = model.info;
+  const perStrand = (a) => { const out = []; for (let s = 0; s < info.S; s++) out.push(Array.from(a.slice(info.segOffset[s], info.segOffset[s + 1]))); return out; };
+  const segment = {}; for (const k of ['Mx', 'My', 'T', 'N', 'Vx', 'Vy']) segment[k] = perStrand(r.segment[k]);
+  const knot = {}; for (const [k, v] of Object.entries(r.knot)) { const o = {}; v.forEach((x, c) => { o[String(c)] = x; }); knot[k] = o; }
+  const disp = [];
+  for (let s = 0; s < info.S; s++) { const rows = []; for (let i = info.nodeOffset[s]; i < info.nodeOffset[s + 1]; i++) rows.push([r.disp[3 * i], r.disp[3 * i + 1], r.disp[3 * i + 2]]); disp.push(rows); }
+  const reactions = {}; for (const [k, v] of r.reactions) reactions[k] = v;
+  return { iterations: r.iterations, converged: r.converged, residual: r.residual, segment, knot, disp, reactions, energies: r.energies };
+}
+
 /** Continuous-batching request server (include/weave.h, serving): submit solves as they arrive, pump, poll */
 class Server {
   constructor(device, opts = {}) {
@@ -178,4 +224,4 @@ class Server {
   dispose() { if (this.h) { native().serverDestroy(this.h); this.h = null; } }
 }
 
-module.exports = { getDevice, Model, GpuSolver, Server, FP32_FP64, FP64 };
+module.exports = { getDevice, Model, GpuSolver, Server, buildResults, resultsJSON, FP32_FP64, FP64 };

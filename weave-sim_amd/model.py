@@ -69,8 +69,10 @@ class StateBuffers:
         """copy the arrays and scalars of another StateBuffers / Model state"""
         for name in self.arrays:
             self.arrays[name][:] = other[name]
-        for f, _ in A.ws_state._fields_[:15]:
-            setattr(self.st, f, getattr(other.st, f))
+        arrays = {name for name, _, _ in A.STATE_ARRAYS}
+        for f, _ in A.ws_state._fields_:
+            if f not in arrays:
+                setattr(self.st, f, getattr(other.st, f))
 
 
 class Model:

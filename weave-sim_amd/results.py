@@ -1,0 +1,103 @@
+"""Results of a relaxed instance, as the reference's app reports them: buildResults (web/js/solver_cpu.js:878-922)
+and the results JSON export (resultsJSON, web/js/app.js:360-390), over a downloaded state.
+
+    r = build_results(model, state)        # state: a StateBuffers filled by GpuSolver.download (or the model itself)
+    doc = results_json(model, r)           # per-strand segment arrays, knot values, displacements, reactions
+
+Differences from the reference, by design: knot values are keyed by crossing index (the reference keys them by the
+mesh edge id of the knot); reactions by "strand,node" (the strand-local node index; the reference by frame key).
+The ground's per-node force is not part of a download, so `floor` is absent."""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+
+SEG_LAYERS = ("Mx", "My", "T", "N", "Vx", "Vy")
+
+
+def _arr(st, name, n, dtype=np.float64):
+    p = getattr(st.st if hasattr(st, "st") else st, name)
+    if not p or n == 0:
+        return np.zeros(n, dtype=dtype)
+    return np.ctypeslib.as_array(p, shape=(n,)).astype(dtype, copy=True)
+
+
+def segment_moments(model, state):
+    """Mx, My, T per segment from the joint and clamp moments of a downloaded state (buildResults :878-891)"""
+    md = model.md
+    m = md.m
+    M = _arr(state, "M", 3 * md.p)
+    cM = _arr(state, "cM", 2 * md.C)
+    Mx, My, T = np.zeros(m), np.zeros(m), np.zeros(m)
+    P = A.P_d
+    rc = A.model_lib().wsm_segment_moments(C.byref(md), M.ctypes.data_as(P), cM.ctypes.data_as(P), Mx.ctypes.data_as(P),
+                                           My.ctypes.data_as(P), T.ctypes.data_as(P))
+    if rc != 0:
+        raise ValueError(A.model_lib().wsm_last_error().decode())
+    return Mx, My, T
+
+
+def build_results(model, state, contact_options=None):
+    """per-segment Mx My T N Vx Vy, per-crossing contact layers, per-node displacement, reactions, energies
+    (buildResults, solver_cpu.js:878-922).  contact_options: which default-off layers to report
+    ({"seat": bool, "rot_friction": bool, "augmented": bool}; by default those the model's parameters switch on)"""
+    md = model.md
+    n, m, K = md.n, md.m, md.K
+    st = state.st if hasattr(state, "st") else state
+    Mx, My, T = segment_moments(model, state)
+    V = _arr(state, "Vsh", 2 * m)
+    seg = {"Mx": Mx, "My": My, "T": T, "N": _arr(state, "N", m), "Vx": V[0::2].copy(), "Vy": V[1::2].copy()}
+    X, X0 = _arr(state, "X", 3 * n), _arr(state, "X0", 3 * n)
+    disp = X - X0
+    par = md.par
+    opt = {"seat": bool(par.seat_law), "rot_friction": bool(par.rot_friction), "augmented": bool(par.augmented)}
+    opt.update(contact_options or {})
+    knot = {}
+    if par.contacts and K:
+        phi = _arr(state, "phi", 2 * K)
+        knot = {"Fn": _arr(state, "Fn", K), "Ft": _arr(state, "Ft", K), "slip": _arr(state, "slip", K),
+                "state": _arr(state, "kstate", K), "gap": _arr(state, "gap", K),
+                "ctype": _arr(state, "ctype", K), "pressure": _arr(state, "pressure", K),
+                "util": _arr(state, "util", K), "theta": _arr(state, "theta", K),
+                "phiA": phi[0::2].copy(), "phiB": phi[1::2].copy()}
+        if opt["seat"]:
+            dent = _arr(state, "dent", 2 * K)
+            knot["dent"] = dent[0::2] + dent[1::2] + _arr(state, "dent_e", K)
+        if opt["rot_friction"]:
+            knot["M_rot"] = _arr(state, "Mrot", K)
+            knot["rot_slip"] = _arr(state, "rot_slip", K)
+        if opt["augmented"]:
+            knot["lambda"] = _arr(state, "lam", K)
+    else:
+        knot = {k: np.zeros(K) for k in ("Fn", "Ft", "slip", "state")}
+    free = np.ctypeslib.as_array(md.node_free, shape=(3 * n,)).reshape(n, 3)
+    R = _arr(state, "reactions", 3 * n).reshape(n, 3)
+    noff = np.ctypeslib.as_array(md.node_offset, shape=(md.S + 1,))
+    reactions = {}
+    for i in np.flatnonzero(~free.all(axis=1)):
+        s = int(np.searchsorted(noff, i, side="right") - 1)
+        reactions[(s, int(i - noff[s]))] = [float(R[i, 0]), float(R[i, 1]), float(R[i, 2]), 0.0, 0.0, 0.0]
+    energies = {"kinetic": st.ke, "bending": st.e_bend, "torsion": st.e_twist, "stretch": st.e_stretch,
+                "shear": st.e_shear, "contact": st.e_contact,
+                "beam": st.e_bend + st.e_twist + st.e_stretch + st.e_shear}
+    energies["strain"] = energies["beam"] + st.e_contact
+    return {"iterations": st.iteration, "converged": bool(st.converged), "residual": st.residual,
+            "segment": seg, "knot": knot, "disp": disp, "reactions": reactions, "energies": energies,
+            "rod": "cosserat"}
+
+
+def results_json(model, r):
+    """the export document (resultsJSON, app.js:360-390): per-strand segment arrays, knot values by crossing index,
+    per-strand node displacements, reactions by "strand,node" """
+    md = model.md
+    soff = np.ctypeslib.as_array(md.seg_offset, shape=(md.S + 1,))
+    noff = np.ctypeslib.as_array(md.node_offset, shape=(md.S + 1,))
+    per_strand = lambda a: [a[soff[s]:soff[s + 1]].tolist() for s in range(md.S)]
+    seg = {k: per_strand(r["segment"][k]) for k in SEG_LAYERS if k in r["segment"]}
+    knot = {k: {str(c): float(v[c]) for c in range(len(v))} for k, v in r["knot"].items()}
+    d = r["disp"].reshape(-1, 3)
+    disp = [d[noff[s]:noff[s + 1]].tolist() for s in range(md.S)]
+    reactions = {f"{s},{k}": v for (s, k), v in r["reactions"].items()}
+    return {"iterations": int(r["iterations"]), "converged": r["converged"], "residual": float(r["residual"]),
+            "segment": seg, "knot": knot, "disp": disp, "reactions": reactions, "energies": r["energies"]}
