@@ -260,6 +260,45 @@ def run_serve(args):
     }
 
 
+def run_stability(args):
+    """the stability check (stability.js) of design variant 6 (7,800 nodes, 156 crossings) after 1,200 iterations:
+    the perturbed copies' generalised forces evaluated on the device in batches vs by the oracle on one host thread,
+    the same host colouring and factorisation after both"""
+    W = importlib.import_module("weave-sim_amd")
+    ST = importlib.import_module("weave-sim_amd.stability")
+    from oracle.bind import Oracle
+    m = importlib.import_module("weave-sim_amd.scenes").design_variant(6)
+    o = Oracle(m)
+    o.step(1200)
+    st = o.state()
+    X, Q, u = st["X"].copy(), st["Q"].copy(), st["u"].copy()
+    timing = {}
+
+    def timed(name, ev):
+        def f(states):
+            t = time.perf_counter()
+            out = ev(states)
+            timing[name] = (time.perf_counter() - t, len(states))
+            return out
+        return f
+    ev = ST.device_evaluator(W, m, base=st, batch=256)
+    ST.stability(m, timed("setup", ev), X=X, Q=Q, u=u)     # builds the batch of copies (a one-time upload)
+    sd = ST.stability(m, timed("device", ev), X=X, Q=Q, u=u)
+    so = ST.stability(m, timed("oracle", lambda states: [o.evaluate(x, q) for x, q in states]), X=X, Q=Q, u=u)
+    td, nd = timing["device"]
+    tc, nc = timing["oracle"]
+    return {"metric": "stability evaluations/s", "value": nd / td, "unit": "evaluations/s", "n_gpus": 1,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded design variant, 1,200 iterations)",
+            "config": {"workload": f"stability check of design variant 6 ({m.n} nodes, {m.md.K} crossings) after 1,200 iterations",
+                       "dofs": sd["ndof"], "colours": sd["colours"], "evaluations": nd},
+            "device": {"lam": sd["lam"], "neg": sd["neg"], "eval_s": td, "total_ms": sd["ms"],
+                       "first_call_s_with_upload": timing["setup"][0]},
+            "cpu_baseline": {"value": nc / tc, "unit": "evaluations/s", "cores": 1, "kind": "port",
+                             "sample": f"the same {nc} perturbed states evaluated by the f64 oracle, one host thread",
+                             "lam": so["lam"], "neg": so["neg"], "total_ms": so["ms"]}}
+
+
 # ------------------------------------------------------------------------------------------------ main
 def main():
     ap = argparse.ArgumentParser()
@@ -274,9 +313,10 @@ def main():
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--kernel-only", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--mode", default="throughput", choices=["throughput", "solves", "serve"],
+    ap.add_argument("--mode", default="throughput", choices=["throughput", "solves", "serve", "stability"],
                     help="throughput: node-iterations/s of one workload; solves: design variants to the stop "
-                         "criterion, batched vs plain; serve: a Poisson request stream through the server")
+                         "criterion, batched vs plain; serve: a Poisson request stream through the server; "
+                         "stability: the stability check of a relaxed dome, device-evaluated vs the oracle")
     ap.add_argument("--max-iter", type=int, default=5000, help="iteration limit of a solve (solves, serve)")
     ap.add_argument("--plain-sample", type=int, default=16, help="variants timed on the plain path (solves)")
     ap.add_argument("--plain-iters", type=int, default=512, help="iterations timed per plain-path variant")
@@ -301,6 +341,9 @@ def main():
         return
     if args.mode == "serve":
         print(json.dumps(run_serve(args)))
+        return
+    if args.mode == "stability":
+        print(json.dumps(run_stability(args)))
         return
     # counter passes first, as child processes, before this process touches the GPU
     traffic, traffic_detail = (None, {"skipped": True}) if args.no_traffic else measure_traffic(args)

@@ -229,6 +229,14 @@ WS_API int ws_download(ws_batch* batch, int32_t b, ws_state* out);              
 WS_API int ws_read_outputs(ws_batch* batch, int32_t b, double* out, int32_t cap_rows, int32_t* n_rows);
                           /* readOutputs(b) :408-411: per recorded schedule value (travel, reaction, value, iteration) */
 WS_API int ws_internalize(ws_batch* batch, int32_t b);                              /* internalize(b) :507-520 */
+/* generalised forces of every instance at its current state — the evaluator of the reference's stability check
+ * (stability.js:27-52, on the CPU there; no GpuSolver counterpart): node forces F [3n] (rod, constant and frame-local
+ * loads, contacts, ground; before the anchors), body-axis torques T [3m] and the frame inertia I3 [3m] of that state,
+ * concatenated in instance order.  The batch's state is unchanged by the call. */
+WS_API int ws_evaluate(ws_batch* batch, double* F, double* T, double* I3);
+/* positions and frames of instances 0 .. count−1, the rest of their state kept (the stability check's perturbed
+ * copies; no counterpart in the reference): X holds each instance's [3n] one after the other, Q each [4m] */
+WS_API int ws_set_positions(ws_batch* batch, int32_t count, const double* X, const double* Q);
 WS_API int ws_batch_timing(ws_batch* batch, double* ms_last_step, double* ms_kernel_cross, double* ms_kernel_strand,
                            double* ms_kernel_ctrl);   /* hipEvent timing of the last step(); per-kernel means of the last profiled step */
 WS_API int ws_batch_bytes(ws_batch* batch, double* alg_bytes_per_iter, double* device_bytes);  /* algorithmic HBM bytes per iteration, device footprint */

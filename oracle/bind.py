@@ -44,6 +44,8 @@ def lib():
         L.oracle_history.argtypes = [C.c_void_p, A.P_d, A.P_d, A.P_d, C.c_int32]
         L.oracle_rod_apply.argtypes = [C.c_void_p, A.P_d, A.P_d, A.P_d, A.P_d]
         L.oracle_rod_apply.restype = C.c_double
+        L.oracle_evaluate.argtypes = [C.c_void_p, A.P_d, A.P_d, A.P_d, A.P_d, A.P_d]
+        L.oracle_evaluate.restype = None
         L.oracle_replay.argtypes = [C.c_void_p, C.c_int32, C.c_int32, A.P_d, A.P_d, C.c_int32, C.c_int32, C.c_int32, A.P_d]
         L.oracle_replay.restype = None
         L.oracle_pairwise_sum.argtypes = [A.P_d, C.c_int64]
@@ -68,6 +70,17 @@ class Oracle:
 
     def step(self, n):
         return lib().oracle_step(self.h, int(n))
+
+    def evaluate(self, X, Q):
+        """generalised forces at positions X and frames Q (the stability check's evaluator): F (n × 3) before the
+        anchors, body-axis torques T (m × 3), frame inertia I3 (m × 3); the state is unchanged"""
+        n, m = self.model.n, self.model.md.m
+        X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+        Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(-1)
+        F, T, I3 = np.zeros(3 * n), np.zeros(3 * m), np.zeros(3 * m)
+        lib().oracle_evaluate(self.h, X.ctypes.data_as(A.P_d), Q.ctypes.data_as(A.P_d), F.ctypes.data_as(A.P_d),
+                              T.ctypes.data_as(A.P_d), I3.ctypes.data_as(A.P_d))
+        return F.reshape(-1, 3), T.reshape(-1, 3), I3.reshape(-1, 3)
 
     def replay(self, mode, probe, axis, sched, settle, avg):
         """UTM load-test replay (calibrate.js replayForce / replayDisplacement) from the current state: one row per

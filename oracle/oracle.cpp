@@ -718,6 +718,45 @@ struct Oracle {
     }
   }
 
+  // The evaluator of the stability check (stability.js:27-52) at positions Xe and frames Qe: node forces before the
+  // anchors (rod, constant and frame-local loads, contacts, ground), body-axis torques Rᵀ T, and the frame inertia of
+  // that state (static + contact righting + bending coupling).  The contact and ground passes run as in an iteration
+  // (ramp at the current iteration, as the device's crossing kernel has it) and every piece of state is restored.
+  void evaluate(const double* Xe, const double* Qe, double* Fo, double* To, double* Io) {
+    const Oracle saved = *this;
+    const int rampN = std::max(par.ramp_iter, 0);
+    const bool contact = par.contacts != 0 && K > 0;
+    std::copy(Xe, Xe + 3 * n, X.begin());
+    std::copy(Qe, Qe + 4 * m, Q.begin());
+    if (contact) ramp = rampN == 0 ? 1.0 : std::min(1.0, 0.01 + 0.99 * iteration / rampN);
+    std::fill(F.begin(), F.end(), 0.0); std::fill(T.begin(), T.end(), 0.0);
+    rodApply(F, T);
+    Ltotal = loadConst;
+    if (npl) {
+      std::vector<double> nq; nodeQuats(nq); double Rl[9];
+      for (int q = 0; q < npl; q++) {
+        int nid = plNode[q]; const double* Fv = &plF[3 * q];
+        toFrame(&nq[4 * nid], Rl);
+        for (int k = 0; k < 3; k++) Ltotal[3 * nid + k] += Rl[k] * Fv[0] + Rl[3 + k] * Fv[1] + Rl[6 + k] * Fv[2];
+      }
+    }
+    for (int i = 0; i < 3 * n; i++) F[i] += Ltotal[i];
+    if (contact) contactApply(F, T);
+    if (floorOn) floorApply(F, rampN <= 0 ? 1.0 : std::min(1.0, 0.01 + 0.99 * iteration / rampN), nullptr);
+    std::copy(F.begin(), F.end(), Fo);
+    double R[9];
+    std::vector<double> kTw(m, 0.0);
+    if (contact) addTwistStiffness(kTw);
+    for (int j = 0; j < m; j++) {
+      toFrame(&Q[4 * j], R);
+      for (int k = 0; k < 3; k++) {
+        To[3 * j + k] = R[3 * k] * T[3 * j] + R[3 * k + 1] * T[3 * j + 1] + R[3 * k + 2] * T[3 * j + 2];
+        Io[3 * j + k] = par.mass_scale * std::max(krot[3 * j + k] + kTw[j] + kDyn[3 * j + k], 1e-12);
+      }
+    }
+    *this = saved;
+  }
+
   // one or more iterations (solver_cpu.js:737-841); returns the number of iterations run
   int step(int nIter) {
     double ms = par.mass_scale;
@@ -853,6 +892,9 @@ WS_API oracle_t* oracle_create(const ws_model* md, const ws_state* st, int32_t k
 }
 WS_API void oracle_free(oracle_t* h) { delete h; }
 WS_API int32_t oracle_step(oracle_t* h, int32_t n_iter) { return h->o.step(n_iter); }
+WS_API void oracle_evaluate(oracle_t* h, const double* X, const double* Q, double* F, double* T, double* I3) {
+  h->o.evaluate(X, Q, F, T, I3);
+}
 WS_API void oracle_replay(oracle_t* h, int32_t mode, int32_t probe, const double* axis, const double* sched, int32_t n,
                           int32_t settle, int32_t avg, double* out) {
   h->o.replay(mode, probe, axis, sched, n, settle, avg, out);

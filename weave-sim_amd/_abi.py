@@ -188,6 +188,8 @@ def hip_lib():
         L.ws_read_control.argtypes = [vp, C.POINTER(ws_control)]
         L.ws_read_history.argtypes = [vp, i32, P_d, i32, C.POINTER(i32)]
         L.ws_read_outputs.argtypes = [vp, i32, P_d, i32, C.POINTER(i32)]
+        L.ws_evaluate.argtypes = [vp, P_d, P_d, P_d]
+        L.ws_set_positions.argtypes = [vp, i32, P_d, P_d]
         L.ws_download.argtypes = [vp, i32, C.POINTER(ws_state)]
         L.ws_internalize.argtypes = [vp, i32]
         L.ws_batch_timing.argtypes = [vp, P_d, P_d, P_d, P_d]
@@ -214,7 +216,7 @@ def hip_lib():
 # every symbol include/weave.h declares (checked by tests/test_abi.py)
 WEAVE_H_SYMBOLS = [
     "ws_ctx_create", "ws_ctx_destroy", "ws_last_error", "ws_device_info", "ws_copy_probe", "ws_batch_create", "ws_batch_destroy",
-    "ws_batch_size", "ws_upload", "ws_update_params", "ws_step", "ws_read_control", "ws_read_history", "ws_read_outputs", "ws_download",
+    "ws_batch_size", "ws_upload", "ws_update_params", "ws_step", "ws_read_control", "ws_read_history", "ws_read_outputs", "ws_evaluate", "ws_set_positions", "ws_download",
     "ws_internalize", "ws_batch_timing", "ws_batch_bytes", "ws_batch_profile", "ws_batch_kernel_bench",
     "ws_batch_get_opts", "ws_autotune", "ws_server_create", "ws_server_destroy", "ws_submit",
     "ws_cancel", "ws_pump", "ws_poll", "ws_fetch", "ws_server_clock_ms", "ws_server_stats",

@@ -29,7 +29,9 @@ template <typename Real, int TILE> __global__ void k_rigid(const Inst<Real>*, co
 template <typename Real> __global__ void k_ctrl(const Inst<Real>*, const CtrlItem*, int32_t, Ctl*);
 __global__ void k_copy(const uint4*, uint4*, int64_t);
 template <typename Real> __global__ void k_al_preview(const Inst<Real>*, int32_t, double*);
-template <typename Real, int TILE> __global__ void k_outputs(const Inst<Real>*, const TileItem*, int32_t, double*, double*, double*, double*, double*);
+template <typename Real> __global__ void k_set_xq(const Inst<Real>*, int32_t, const double*, const double*);
+template <typename Real, int TILE> __global__ void k_outputs(const Inst<Real>*, const TileItem*, int32_t, OutPtrs);
+template <typename Real, int TILE> __global__ void k_evaluate(const Inst<Real>*, const TileItem*, int32_t, double*, double*, double*, const int64_t*, const int64_t*);
 }  // namespace wsd
 
 using namespace wsd;
@@ -405,6 +407,8 @@ struct BatchBase {
   virtual int read_control(ws_control* out) = 0;
   virtual int read_history(int b, double* out, int cap, int* n) = 0;
   virtual int read_outputs(int b, double* out, int cap, int* n) = 0;
+  virtual int evaluate(double* F, double* T, double* I3) = 0;
+  virtual int set_xq(int count, const double* X, const double* Q) = 0;
   virtual int download(int b, ws_state& out) = 0;
   virtual int internalize(int b) = 0;
   virtual void kernel_bench(int reps, double* out) = 0;
@@ -432,6 +436,7 @@ struct Batch : BatchBase {
   ~Batch() override {
     if (graph) (void)hipGraphExecDestroy(graph);
     for (auto& h : inst) if (h) h->arena.release();
+    for (void* p : scratch_ptr) if (p) (void)hipFree(p);
     if (d_inst) (void)hipFree(d_inst);
     if (d_tiles) (void)hipFree(d_tiles);
     if (d_cross) (void)hipFree(d_cross);
@@ -684,6 +689,91 @@ struct Batch : BatchBase {
     return WS_OK;
   }
 
+  // generalised forces of every instance at its current state (the evaluator of the reference's stability check,
+  // stability.js:27-52): the crossing pass and the strand pass in output mode, node forces F [3n] (rod, loads,
+  // contacts, ground; before the anchors), body-axis torques T [3m] and frame inertia I3 [3m], concatenated in
+  // instance order.  The crossing pass advances the contact state as one iteration would; every arena and control
+  // record is restored afterwards, so the batch is as before the call.
+  int evaluate(double* F, double* T, double* I3) override {
+    HIPCHK(hipStreamSynchronize(stream));
+    std::vector<int32_t> all;
+    std::vector<int64_t> fofs(std::max(B, 1), 0), tofs(std::max(B, 1), 0);
+    size_t total = 0, nF = 0, nT = 0;
+    for (int b = 0; b < B; b++) if (inst[b]) {
+      all.push_back(b); total += inst[b]->arena.size;
+      fofs[b] = int64_t(nF); tofs[b] = int64_t(nT);
+      nF += 3 * size_t(inst[b]->n); nT += 3 * size_t(inst[b]->m);
+    }
+    if (all.empty()) return WS_OK;
+    // scratch kept by the batch between calls (grown when needed): the arena snapshot, the outputs, the offsets
+    char* save = static_cast<char*>(eval_scratch(0, total));
+    double* dF = static_cast<double*>(eval_scratch(1, sizeof(double) * nF));
+    double* dT = static_cast<double*>(eval_scratch(2, sizeof(double) * nT));
+    double* dI = static_cast<double*>(eval_scratch(3, sizeof(double) * nT));
+    int64_t* dofs = static_cast<int64_t*>(eval_scratch(4, sizeof(int64_t) * 2 * fofs.size()));
+    std::vector<int64_t> ofs(fofs); ofs.insert(ofs.end(), tofs.begin(), tofs.end());
+    HIPCHK(hipMemcpyAsync(dofs, ofs.data(), sizeof(int64_t) * ofs.size(), hipMemcpyHostToDevice, stream));
+    std::vector<Ctl> ctl_save(B);
+    HIPCHK(hipMemcpy(ctl_save.data(), d_ctls, sizeof(Ctl) * B, hipMemcpyDeviceToHost));
+    size_t off = 0;
+    for (int b : all) { HIPCHK(hipMemcpyAsync(save + off, inst[b]->arena.base, inst[b]->arena.size, hipMemcpyDeviceToDevice, stream)); off += inst[b]->arena.size; }
+    HIPCHK(hipMemcpyAsync(d_sel, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_begin<Real>, dim3((all.size() + 63) / 64), dim3(64), 0, stream, d_ctls, d_sel, nullptr, int(all.size()), 1, 1);
+    launch_cross(stream);
+    const int64_t* dfo = dofs; const int64_t* dto = dofs + fofs.size();
+    switch (tile) {
+      case 64: hipLaunchKernelGGL((k_evaluate<Real, 64>), dim3(ntiles), dim3(64), 0, stream, d_inst, d_tiles, ntiles, dF, dT, dI, dfo, dto); break;
+      case 128: hipLaunchKernelGGL((k_evaluate<Real, 128>), dim3(ntiles), dim3(128), 0, stream, d_inst, d_tiles, ntiles, dF, dT, dI, dfo, dto); break;
+      default: hipLaunchKernelGGL((k_evaluate<Real, 256>), dim3(ntiles), dim3(256), 0, stream, d_inst, d_tiles, ntiles, dF, dT, dI, dfo, dto); break;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(F, dF, sizeof(double) * nF, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(T, dT, sizeof(double) * nT, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(I3, dI, sizeof(double) * nT, hipMemcpyDeviceToHost, stream));
+    off = 0;
+    for (int b : all) { HIPCHK(hipMemcpyAsync(inst[b]->arena.base, save + off, inst[b]->arena.size, hipMemcpyDeviceToDevice, stream)); off += inst[b]->arena.size; }
+    HIPCHK(hipMemcpyAsync(d_ctls, ctl_save.data(), sizeof(Ctl) * B, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    return WS_OK;
+  }
+
+  void* scratch_ptr[7] = {};
+  size_t scratch_cap[7] = {};
+  void* eval_scratch(int k, size_t bytes) {
+    bytes = std::max<size_t>(bytes, 256);
+    if (bytes > scratch_cap[k]) {
+      if (scratch_ptr[k]) HIPCHK(hipFree(scratch_ptr[k]));
+      scratch_ptr[k] = nullptr; scratch_cap[k] = 0;
+      HIPCHK(hipMalloc(&scratch_ptr[k], bytes));
+      scratch_cap[k] = bytes;
+    }
+    return scratch_ptr[k];
+  }
+
+  // positions and frames of instances 0 .. count−1 (both buffers), the rest of their state as it is (the stability
+  // check's perturbed copies): X and Q hold the instances' arrays one after the other; one copy to the device, one
+  // scatter launch per instance
+  int set_xq(int count, const double* X, const double* Q) override {
+    if (count < 0 || count > B) throw Fail{WS_ERR_ARG, "set_positions: count out of range"};
+    HIPCHK(hipStreamSynchronize(stream));
+    size_t nx = 0, nq = 0;
+    for (int b = 0; b < count; b++) { InstHost<Real>& H = at(b); nx += 3 * size_t(H.n); nq += 4 * size_t(H.m); }
+    double* dX = static_cast<double*>(eval_scratch(5, sizeof(double) * nx));
+    double* dQ = static_cast<double*>(eval_scratch(6, sizeof(double) * nq));
+    HIPCHK(hipMemcpyAsync(dX, X, sizeof(double) * nx, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(dQ, Q, sizeof(double) * nq, hipMemcpyHostToDevice, stream));
+    size_t ox = 0, oq = 0;
+    for (int b = 0; b < count; b++) {
+      const InstHost<Real>& H = *inst[b];
+      const int nt = std::max(H.n, H.m);
+      hipLaunchKernelGGL(k_set_xq<Real>, dim3((nt + 255) / 256), dim3(256), 0, stream, d_inst, b, dX + ox, dQ + oq);
+      ox += 3 * size_t(H.n); oq += 4 * size_t(H.m);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
+    return WS_OK;
+  }
+
   // replay outputs recorded so far (readOutputs, solver_gpu.js:408-411): rows of travel, reaction, value, iteration
   int read_outputs(int b, double* out, int cap, int* nrows) override {
     InstHost<Real>& H = at(b);
@@ -790,9 +880,9 @@ struct Batch : BatchBase {
       HIPCHK(hipMemset(dR, 0, sizeof(double) * std::max(3 * n, 1)));
       if (c.live_flag) {
         switch (tile) {
-          case 64: hipLaunchKernelGGL((k_outputs<Real, 64>), dim3(d.ntiles), dim3(64), 0, stream, d_inst, d_tiles, b, dN, dV, dM, dC, dR); break;
-          case 128: hipLaunchKernelGGL((k_outputs<Real, 128>), dim3(d.ntiles), dim3(128), 0, stream, d_inst, d_tiles, b, dN, dV, dM, dC, dR); break;
-          default: hipLaunchKernelGGL((k_outputs<Real, 256>), dim3(d.ntiles), dim3(256), 0, stream, d_inst, d_tiles, b, dN, dV, dM, dC, dR); break;
+          case 64: hipLaunchKernelGGL((k_outputs<Real, 64>), dim3(d.ntiles), dim3(64), 0, stream, d_inst, d_tiles, b, OutPtrs{dN, dV, dM, dC, dR}); break;
+          case 128: hipLaunchKernelGGL((k_outputs<Real, 128>), dim3(d.ntiles), dim3(128), 0, stream, d_inst, d_tiles, b, OutPtrs{dN, dV, dM, dC, dR}); break;
+          default: hipLaunchKernelGGL((k_outputs<Real, 256>), dim3(d.ntiles), dim3(256), 0, stream, d_inst, d_tiles, b, OutPtrs{dN, dV, dM, dC, dR}); break;
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream));

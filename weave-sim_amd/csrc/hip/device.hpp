@@ -148,6 +148,11 @@ struct Inst {
   double* preact;            // [3] the pinned probe's reaction of the last iteration (displacement control)
 };
 
+// output arrays of the strand tile in output mode (downloads, evaluations): N [m], V [2m], M [3p], clamp moments
+// [2C], reactions [3n]; for an evaluation also the node forces F [3n], body-axis torques T [3m] and frame inertia
+// I3 [3m] (null otherwise)
+struct OutPtrs { double *N, *V, *M, *CM, *R, *F, *T, *I3; };
+
 // one strand tile: strand-local node range [k0, k1) plus the strand's layout, so that a tile reads its geometry in
 // one 64-byte load
 struct TileItem {

@@ -810,11 +810,12 @@ __device__ __forceinline__ void joint_torque(StrandLDS<Real, TILE>& L, int i, Re
 // and one joint, and lanes 1 .. cnt integrate their node and frame.
 template <typename Real, int TILE, bool OUT, bool RB>
 __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem& tl, StrandLDS<Real, TILE>& L,
-                            double* outN, double* outV, double* outM, double* outCM, double* outR) {
+                            const OutPtrs& out) {
   const int tid = threadIdx.x;
   int cur = ctl->cur;
   const bool rbk = ctl->rollback != 0;
-  if (OUT) cur = rbk ? cur : (cur ^ 1);            // the state the last iteration evaluated
+  // the state the last iteration evaluated (downloads), or the current one (evaluations: out.F set)
+  if (OUT) cur = (rbk || out.F) ? cur : (cur ^ 1);
   const int iter = ctl->iter;
   const int noff = tl.noff, nn = tl.nn, soff = tl.soff, nseg = tl.nseg, joff = tl.joff, njs = tl.njs;
   const bool closed = tl.closed != 0;
@@ -879,7 +880,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
     if (ownSeg) {
       eS = a1; eSh = a2;
       mxN = fabs(double(nb[2]));
-      if (OUT) { outN[j] = double(nb[2]); outV[2 * j] = double(nb[0]); outV[2 * j + 1] = double(nb[1]); }
+      if (OUT && out.N) { out.N[j] = double(nb[2]); out.V[2 * j] = double(nb[0]); out.V[2 * j + 1] = double(nb[1]); }
     }
   }
   if (hasJoint) {
@@ -889,7 +890,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       eB = 0.5 * double(lb) * (double(mm[0]) * double(d[0]) + double(mm[1]) * double(d[1]));
       eT = 0.5 * double(lb) * double(mm[2]) * double(d[2]);
       mxM = fmax(fabs(double(mm[0])), fmax(fabs(double(mm[1])), fabs(double(mm[2]))));
-      if (OUT) { const int jg = joff + jjo; outM[3 * jg] = double(mm[0]); outM[3 * jg + 1] = double(mm[1]); outM[3 * jg + 2] = double(mm[2]); }
+      if (OUT && out.M) { const int jg = joff + jjo; out.M[3 * jg] = double(mm[0]); out.M[3 * jg + 1] = double(mm[1]); out.M[3 * jg + 2] = double(mm[2]); }
     }
   }
   // what lane t needs from lane t − 1: the force of segment t − 1, joint t − 1's torque on segment t and its kDyn —
@@ -1042,8 +1043,11 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       }
     }
     if (OUT) {
-      if (fx & 8) { outR[3 * i] = -double(F[0]); outR[3 * i + 1] = -double(F[1]); outR[3 * i + 2] = -double(F[2]); }
-      else { outR[3 * i] = outR[3 * i + 1] = outR[3 * i + 2] = 0.0; }
+      if (out.R) {
+        if (fx & 8) { out.R[3 * i] = -double(F[0]); out.R[3 * i + 1] = -double(F[1]); out.R[3 * i + 2] = -double(F[2]); }
+        else { out.R[3 * i] = out.R[3 * i + 1] = out.R[3 * i + 2] = 0.0; }
+      }
+      if (out.F) { out.F[3 * i] = double(F[0]); out.F[3 * i + 1] = double(F[1]); out.F[3 * i + 2] = double(F[2]); }
     } else if (I.proto == 2 && i == I.probe) {
       gl(I.preact)[0] = -double(F[0]); gl(I.preact)[1] = -double(F[1]); gl(I.preact)[2] = -double(F[2]);
     }
@@ -1088,7 +1092,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
         for (int c = 0; c < 3; c++) tm[c] -= Jb[c];
         eB += 0.5 * double(lc) * (double(mm[0]) * double(d[0]) + double(mm[1]) * double(d[1]));
         eT += 0.5 * double(lc) * double(mm[2]) * double(d[2]);
-        if (OUT) { outCM[2 * cc] = double(mm[0]); outCM[2 * cc + 1] = double(mm[1]); }
+        if (OUT && out.CM) { out.CM[2 * cc] = double(mm[0]); out.CM[2 * cc + 1] = double(mm[1]); }
       }
     }
     if (Tc[0] != Real(0) || Tc[1] != Real(0) || Tc[2] != Real(0)) {
@@ -1099,6 +1103,14 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       tm[2] += Real(Rs[6]) * Tc[0] + Real(Rs[7]) * Tc[1] + Real(Rs[8]) * Tc[2];
     }
     mxT = double(tm[0]) * double(tm[0]) + double(tm[1]) * double(tm[1]) + double(tm[2]) * double(tm[2]);   // squared
+    if (OUT && out.T) {
+      // the generalised forces of an evaluation: body-axis torques, and the frame inertia of this state
+      out.T[3 * j] = double(tm[0]); out.T[3 * j + 1] = double(tm[1]); out.T[3 * j + 2] = double(tm[2]);
+      const double msc = I.mass_scale;
+      out.I3[3 * j] = msc * fmax(double(krx + kTw + (pKD[0] + KD[0])), 1e-12);
+      out.I3[3 * j + 1] = msc * fmax(double(kry + kTw + (pKD[1] + KD[1])), 1e-12);
+      out.I3[3 * j + 2] = msc * fmax(double(krz + kTw + (pKD[2] + KD[2])), 1e-12);
+    }
     if (!OUT) {
       const Real msc = Real(I.mass_scale);
       Real I3[3] = {msc * dmax(krx + kTw + (pKD[0] + KD[0]), Real(1e-12)),
@@ -1450,7 +1462,7 @@ __global__ __launch_bounds__(TILE) void k_strand(const Inst<Real>* __restrict__ 
   if (tl.inst < 0) return;
   const Ctl* ctl = ctls + tl.inst;
   if (!inst_live(ctl)) return;
-  strand_tile<Real, TILE, false, RB>(insts[tl.inst], ctl, tl, L, nullptr, nullptr, nullptr, nullptr, nullptr);
+  strand_tile<Real, TILE, false, RB>(insts[tl.inst], ctl, tl, L, OutPtrs{});
 }
 
 // The rigid step of a structure only the ground holds (CpuSolver.step, solver_cpu.js:791-805), after the control
@@ -1493,15 +1505,48 @@ __global__ __launch_bounds__(TILE) void k_rigid(const Inst<Real>* __restrict__ i
   }
 }
 
+// generalised forces of every instance of the batch at its current state in one launch over all strand tiles (the
+// stability check's evaluations): node forces, body-axis torques and frame inertia into one buffer each, instance b
+// at element offsets fofs[b] (3n per instance) and tofs[b] (3m)
+template <typename Real, int TILE>
+__global__ __launch_bounds__(TILE) void k_evaluate(const Inst<Real>* __restrict__ insts, const TileItem* __restrict__ tiles,
+                                                   int32_t ntiles, double* F, double* T, double* I3,
+                                                   const int64_t* __restrict__ fofs, const int64_t* __restrict__ tofs) {
+  __shared__ StrandLDS<Real, TILE> L;
+  if (int(blockIdx.x) >= ntiles) return;
+  const TileItem tl = tiles[blockIdx.x];
+  if (tl.inst < 0) return;
+  const Inst<Real>& I = insts[tl.inst];
+  OutPtrs o{};
+  o.F = F + fofs[tl.inst]; o.T = T + tofs[tl.inst]; o.I3 = I3 + tofs[tl.inst];
+  strand_tile<Real, TILE, true, false>(I, I.ctl, tl, L, o);
+}
+
+// positions X [3n] and frames Q [4m] of instance b (interleaved, as the host holds them) into both of its buffers
+template <typename Real>
+__global__ __launch_bounds__(256) void k_set_xq(const Inst<Real>* __restrict__ insts, int32_t b, const double* __restrict__ X,
+                                               const double* __restrict__ Q) {
+  const Inst<Real>& I = insts[b];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < I.n) {
+    const double x = X[3 * t], y = X[3 * t + 1], z = X[3 * t + 2];
+    for (int k = 0; k < 2; k++) { gl(I.x[k])[t] = x; gl(I.y[k])[t] = y; gl(I.z[k])[t] = z; }
+  }
+  if (t < I.m) {
+    const double w = Q[4 * t], qx = Q[4 * t + 1], qy = Q[4 * t + 2], qz = Q[4 * t + 3];
+    for (int k = 0; k < 2; k++) { gl(I.qw[k])[t] = w; gl(I.qx[k])[t] = qx; gl(I.qy[k])[t] = qy; gl(I.qz[k])[t] = qz; }
+  }
+}
+
 // outputs of the last evaluated state of one instance: N, V, M, clamp moments, reactions (download)
 template <typename Real, int TILE>
 __global__ __launch_bounds__(TILE) void k_outputs(const Inst<Real>* __restrict__ insts, const TileItem* __restrict__ tiles, int32_t inst,
-                                                  double* outN, double* outV, double* outM, double* outCM, double* outR) {
+                                                  OutPtrs out) {
   __shared__ StrandLDS<Real, TILE> L;
   const Inst<Real>& I = insts[inst];
   if (int(blockIdx.x) >= I.ntiles) return;
   const TileItem tl = tiles[I.tile0 + blockIdx.x];
-  strand_tile<Real, TILE, true, false>(I, I.ctl, tl, L, outN, outV, outM, outCM, outR);
+  strand_tile<Real, TILE, true, false>(I, I.ctl, tl, L, out);
 }
 
 // the multipliers as the host solver holds them after an iteration: an AL update decided by the control kernel is
@@ -1525,7 +1570,8 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
   template __global__ void k_strand<R, T, false>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
   template __global__ void k_strand<R, T, true>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
   template __global__ void k_rigid<R, T>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
-  template __global__ void k_outputs<R, T>(const Inst<R>*, const TileItem*, int32_t, double*, double*, double*, double*, double*);
+  template __global__ void k_outputs<R, T>(const Inst<R>*, const TileItem*, int32_t, OutPtrs); \
+  template __global__ void k_evaluate<R, T>(const Inst<R>*, const TileItem*, int32_t, double*, double*, double*, const int64_t*, const int64_t*);
 WSD_INST(float, 64) WSD_INST(float, 128) WSD_INST(float, 256)
 WSD_INST(double, 64) WSD_INST(double, 128) WSD_INST(double, 256)
 template __global__ void k_cross<float, 1>(const Inst<float>*, const CrossItem*, int32_t, const Ctl*);
@@ -1534,6 +1580,8 @@ template __global__ void k_cross<double, 1>(const Inst<double>*, const CrossItem
 template __global__ void k_cross<double, 2>(const Inst<double>*, const CrossItem*, int32_t, const Ctl*);
 template __global__ void k_ctrl<float>(const Inst<float>*, const CtrlItem*, int32_t, Ctl*);
 template __global__ void k_ctrl<double>(const Inst<double>*, const CtrlItem*, int32_t, Ctl*);
+template __global__ void k_set_xq<float>(const Inst<float>*, int32_t, const double*, const double*);
+template __global__ void k_set_xq<double>(const Inst<double>*, int32_t, const double*, const double*);
 template __global__ void k_al_preview<float>(const Inst<float>*, int32_t, double*);
 template __global__ void k_al_preview<double>(const Inst<double>*, int32_t, double*);
 template __global__ void k_begin<float>(Ctl*, const int32_t*, const int32_t*, int32_t, int32_t, int32_t);

@@ -138,6 +138,14 @@ WS_API int ws_read_history(ws_batch* batch, int32_t b, double* out, int32_t cap,
   if (!batch || !out || !n_rows || b < 0 || b >= batch->impl->B) return WS_ERR_ARG;
   BATCH_CALL(batch->impl->read_history(b, out, cap, n_rows));
 }
+WS_API int ws_evaluate(ws_batch* batch, double* F, double* T, double* I3) {
+  if (!batch || !F || !T || !I3) return WS_ERR_ARG;
+  BATCH_CALL(batch->impl->evaluate(F, T, I3));
+}
+WS_API int ws_set_positions(ws_batch* batch, int32_t count, const double* X, const double* Q) {
+  if (!batch || !X || !Q || count < 0 || count > batch->impl->B) return WS_ERR_ARG;
+  BATCH_CALL(batch->impl->set_xq(count, X, Q));
+}
 WS_API int ws_read_outputs(ws_batch* batch, int32_t b, double* out, int32_t cap, int32_t* n_rows) {
   if (!batch || !n_rows || b < 0 || b >= batch->impl->B || cap < 0) return WS_ERR_ARG;
   BATCH_CALL(batch->impl->read_outputs(b, out, cap, n_rows));

@@ -167,6 +167,28 @@ class GpuSolver:
     def internalize(self, b):
         self.ctx.check(self.lib.ws_internalize(self.h, b))
 
+    def evaluate(self):
+        """generalised forces of every instance at its current state (ws_evaluate): lists of F (n × 3), body-axis
+        torques T (m × 3) and frame inertia I3 (m × 3) per instance; the state is unchanged"""
+        ns = [m.md.n for m in self.models]
+        ms = [m.md.m for m in self.models]
+        F = np.zeros(3 * sum(ns))
+        T = np.zeros(3 * sum(ms))
+        I3 = np.zeros(3 * sum(ms))
+        self.ctx.check(self.lib.ws_evaluate(self.h, F.ctypes.data_as(A.P_d), T.ctypes.data_as(A.P_d),
+                                            I3.ctypes.data_as(A.P_d)))
+        fo = np.concatenate([[0], np.cumsum(ns)]) * 3
+        to = np.concatenate([[0], np.cumsum(ms)]) * 3
+        return ([F[fo[b]:fo[b + 1]].reshape(-1, 3) for b in range(self.B)],
+                [T[to[b]:to[b + 1]].reshape(-1, 3) for b in range(self.B)],
+                [I3[to[b]:to[b + 1]].reshape(-1, 3) for b in range(self.B)])
+
+    def set_positions(self, XQ):
+        """positions and frames [(X, Q), …] of instances 0, 1, … (both buffers), the rest of their state kept"""
+        X = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x, _ in XQ]))
+        Q = np.ascontiguousarray(np.concatenate([np.asarray(q, dtype=np.float64).reshape(-1) for _, q in XQ]))
+        self.ctx.check(self.lib.ws_set_positions(self.h, len(XQ), X.ctypes.data_as(A.P_d), Q.ctypes.data_as(A.P_d)))
+
     def read_outputs(self, b, cap=4096):
         """replay outputs of instance b recorded so far: rows of (travel, reaction, value, iteration)"""
         out = np.zeros((max(cap, 1), 4))
