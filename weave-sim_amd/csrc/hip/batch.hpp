@@ -202,12 +202,12 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   A.addc(d.seg_i0, m); A.addc(d.seg_i1, m); A.addc(d.nsa, n); A.addc(d.nsb, n);
   A.addc(d.side_off, S + 1); A.addc(d.side_pos, 2 * K);
   if (C) { A.addc(d.clamp_off, m + 1); A.addc(d.clamp_idx, C); }
-  for (int b = 0; b < 2; b++) { A.add(d.x[b], n); A.add(d.y[b], n); A.add(d.z[b], n); }
+  for (int b = 0; b < 2; b++) A.add(d.X[b], 3 * size_t(n));
   A.add(d.vx, n); A.add(d.vy, n); A.add(d.vz, n);
   A.addc(d.mass, n); A.addc(d.lx, n); A.addc(d.ly, n); A.addc(d.lz, n);
   if (d.npl) { A.addc(d.llx, n); A.addc(d.lly, n); A.addc(d.llz, n); }
   A.addc(d.fix, n);
-  for (int b = 0; b < 2; b++) { A.add(d.qw[b], m); A.add(d.qx[b], m); A.add(d.qy[b], m); A.add(d.qz[b], m); }
+  for (int b = 0; b < 2; b++) A.add(d.Q[b], 4 * size_t(m));
   A.add(d.wx, m); A.add(d.wy, m); A.add(d.wz, m);
   A.addc(d.l0, m); A.addc(d.EA, m); A.addc(d.GA, m); A.addc(d.EIx, m); A.addc(d.EIy, m); A.addc(d.GJ, m);
   A.addc(d.krx, m); A.addc(d.kry, m); A.addc(d.krz, m);
@@ -323,26 +323,24 @@ template <typename Real>
 void upload_state(InstHost<Real>& H, const ws_state& st) {
   Inst<Real>& d = H.d;
   const int n = H.n, m = H.m, K = H.K;
-  std::vector<double> x(n), y(n), z(n);
-  for (int i = 0; i < n; i++) { x[i] = st.X[3 * i]; y[i] = st.X[3 * i + 1]; z[i] = st.X[3 * i + 2]; }
+  std::vector<double> xs(st.X, st.X + 3 * size_t(n));
   double v0 = 0;
   if (d.proto) HIPCHK(hipMemcpy(&v0, d.sched, sizeof(double), hipMemcpyDeviceToHost));
   if (d.proto == 2) {   // displacement control starts with the probe at P0 + sched[0]·axis (solver_gpu.js:309, 314-318)
-    x[d.probe] = d.p0x + d.pax * v0; y[d.probe] = d.p0y + d.pay * v0; z[d.probe] = d.p0z + d.paz * v0;
+    xs[3 * d.probe] = d.p0x + d.pax * v0; xs[3 * d.probe + 1] = d.p0y + d.pay * v0; xs[3 * d.probe + 2] = d.p0z + d.paz * v0;
   }
-  for (int b = 0; b < 2; b++) { h2d(d.x[b], x); h2d(d.y[b], y); h2d(d.z[b], z); }
+  for (int b = 0; b < 2; b++) h2d(d.X[b], xs);
   std::vector<Real> vx(n), vy(n), vz(n);
   for (int i = 0; i < n; i++) { vx[i] = st.V ? Real(st.V[3 * i]) : Real(0); vy[i] = st.V ? Real(st.V[3 * i + 1]) : Real(0); vz[i] = st.V ? Real(st.V[3 * i + 2]) : Real(0); }
   if (d.proto == 2) { vx[d.probe] = vy[d.probe] = vz[d.probe] = Real(0); }
   h2d(d.vx, vx); h2d(d.vy, vy); h2d(d.vz, vz);
   if (d.floor_on) {   // stick points: the state's, else under the nodes (Floor constructor, solver_cpu.js:540-545)
     std::vector<double> ax(n), ay(n);
-    for (int i = 0; i < n; i++) { ax[i] = st.floor_a ? st.floor_a[2 * i] : x[i]; ay[i] = st.floor_a ? st.floor_a[2 * i + 1] : y[i]; }
+    for (int i = 0; i < n; i++) { ax[i] = st.floor_a ? st.floor_a[2 * i] : xs[3 * i]; ay[i] = st.floor_a ? st.floor_a[2 * i + 1] : xs[3 * i + 1]; }
     h2d(d.fax, ax); h2d(d.fay, ay);
   }
-  std::vector<double> qw(m), qx(m), qy(m), qz(m);
-  for (int j = 0; j < m; j++) { qw[j] = st.Q[4 * j]; qx[j] = st.Q[4 * j + 1]; qy[j] = st.Q[4 * j + 2]; qz[j] = st.Q[4 * j + 3]; }
-  for (int b = 0; b < 2; b++) { h2d(d.qw[b], qw); h2d(d.qx[b], qx); h2d(d.qy[b], qy); h2d(d.qz[b], qz); }
+  const std::vector<double> qs(st.Q, st.Q + 4 * size_t(m));
+  for (int b = 0; b < 2; b++) h2d(d.Q[b], qs);
   std::vector<Real> wx(m), wy(m), wz(m);
   for (int j = 0; j < m; j++) { wx[j] = st.W ? Real(st.W[3 * j]) : Real(0); wy[j] = st.W ? Real(st.W[3 * j + 1]) : Real(0); wz[j] = st.W ? Real(st.W[3 * j + 2]) : Real(0); }
   h2d(d.wx, wx); h2d(d.wy, wy); h2d(d.wz, wz);
@@ -816,16 +814,14 @@ struct Batch : BatchBase {
     HIPCHK(hipMemcpy(&c, d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
     const int cur = c.cur;
     const bool rb = c.rollback != 0;
-    auto x = d2h(d.x[cur], n), y = d2h(d.y[cur], n), z = d2h(d.z[cur], n);
-    if (o.X) for (int i = 0; i < n; i++) { o.X[3 * i] = x[i]; o.X[3 * i + 1] = y[i]; o.X[3 * i + 2] = z[i]; }
+    if (o.X) HIPCHK(hipMemcpy(o.X, d.X[cur], sizeof(double) * 3 * size_t(n), hipMemcpyDeviceToHost));
     if (o.floor_a && d.floor_on) {
       auto ax = d2h(d.fax, n), ay = d2h(d.fay, n);
       for (int i = 0; i < n; i++) { o.floor_a[2 * i] = ax[i]; o.floor_a[2 * i + 1] = ay[i]; }
     }
     if (o.X0) std::copy(H.X0.begin(), H.X0.end(), o.X0);
     if (o.Q0) std::copy(H.Q0.begin(), H.Q0.end(), o.Q0);
-    auto qw = d2h(d.qw[cur], m), qx = d2h(d.qx[cur], m), qy = d2h(d.qy[cur], m), qz = d2h(d.qz[cur], m);
-    if (o.Q) for (int j = 0; j < m; j++) { o.Q[4 * j] = qw[j]; o.Q[4 * j + 1] = qx[j]; o.Q[4 * j + 2] = qy[j]; o.Q[4 * j + 3] = qz[j]; }
+    if (o.Q) HIPCHK(hipMemcpy(o.Q, d.Q[cur], sizeof(double) * 4 * size_t(m), hipMemcpyDeviceToHost));
     if (o.V) {
       if (rb) std::fill(o.V, o.V + 3 * n, 0.0);
       else { auto a = d2h(d.vx, n), bb = d2h(d.vy, n), cc = d2h(d.vz, n); for (int i = 0; i < n; i++) { o.V[3 * i] = a[i]; o.V[3 * i + 1] = bb[i]; o.V[3 * i + 2] = cc[i]; } }

@@ -16,7 +16,7 @@ constexpr int HIST_STRIDE = 9;   // history ring entry: ke, strain, bending, tor
 constexpr int SPART = 9;         // doubles per strand-tile partial (stored value-major)
 constexpr int CPART = 2;         // doubles per crossing-block partial (stored value-major)
 constexpr int CTRL_THREADS = 256;  // lanes of the control kernel's workgroup
-constexpr int CTRL_PER = 4;         // strand partials per lane per round (all loads of a round in flight together)
+constexpr int CTRL_PER = 8;         // strand partials per lane per round (all loads of a round in flight together)
 // up to CTRL_ONE strand partials one control workgroup reduces them all; beyond, workgroups of CTRL_CHUNK each (the
 // last to finish combines them): one CU streams partials at ~80 GB/s, so a large instance's partials are spread
 // over several CUs (measured on the 10 m and 30 m domes)
@@ -96,14 +96,14 @@ struct Inst {
   const int32_t* clamp_off;  // [m+1] clamp joints per segment (CSR), null when C == 0
   const int32_t* clamp_idx;  // [C]
   // nodes
-  double* x[2]; double* y[2]; double* z[2];
+  double* X[2];              // [3n] positions, x y z interleaved (double buffer)
   Real* vx; Real* vy; Real* vz;
   const Real* mass;
   const Real* lx; const Real* ly; const Real* lz;            // constant loads
   const Real* llx; const Real* lly; const Real* llz;         // frame-local point loads (null when npl == 0)
   const uint8_t* fix;        // bit0 x, bit1 y, bit2 z fixed; bit3 any fixed (reaction); bit4 all three fixed
   // segments
-  double* qw[2]; double* qx[2]; double* qy[2]; double* qz[2];
+  double* Q[2];              // [4m] frames, w x y z interleaved (double buffer)
   Real* wx; Real* wy; Real* wz;
   const double* l0;
   const Real* EA; const Real* GA; const Real* EIx; const Real* EIy; const Real* GJ;

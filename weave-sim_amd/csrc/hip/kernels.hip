@@ -237,17 +237,19 @@ __device__ __forceinline__ void eval_side(const Inst<Real>& I, int cur, int c, i
   const int ibp = closed ? (ib + 1) % nn : ib + 1;
   const int sm = closed ? (i - 1 + ns) % ns : i - 1;
   const int sp = closed ? (i + 1) % ns : i + 1;
-  const auto X = gl(I.x[cur]); const auto Y = gl(I.y[cur]); const auto Z = gl(I.z[cur]);
-  const double xa = X[E.a], ya = Y[E.a], za = Z[E.a];
-  const double xb = X[E.b], yb = Y[E.b], zb = Z[E.b];
+  // positions and frames are interleaved (x y z per node, w x y z per segment): a side's four nodes and three
+  // segments are two short contiguous runs, a few cache lines
+  const auto P = gl(I.X[cur]);
+  const double xa = P[3 * E.a], ya = P[3 * E.a + 1], za = P[3 * E.a + 2];
+  const double xb = P[3 * E.b], yb = P[3 * E.b + 1], zb = P[3 * E.b + 2];
   double xm = 0, ym = 0, zm = 0, xp = 0, yp = 0, zp = 0;
-  if (hasPrevSeg) { xm = X[noff + iam]; ym = Y[noff + iam]; zm = Z[noff + iam]; }
-  if (hasNextSeg) { xp = X[noff + ibp]; yp = Y[noff + ibp]; zp = Z[noff + ibp]; }
-  const auto QW = gl(I.qw[cur]); const auto QX = gl(I.qx[cur]); const auto QY = gl(I.qy[cur]); const auto QZ = gl(I.qz[cur]);
-  G q0[4] = {G(QW[E.seg]), G(QX[E.seg]), G(QY[E.seg]), G(QZ[E.seg])};
+  if (hasPrevSeg) { const int k = 3 * (noff + iam); xm = P[k]; ym = P[k + 1]; zm = P[k + 2]; }
+  if (hasNextSeg) { const int k = 3 * (noff + ibp); xp = P[k]; yp = P[k + 1]; zp = P[k + 2]; }
+  const auto Qv = gl(I.Q[cur]);
+  G q0[4] = {G(Qv[4 * E.seg]), G(Qv[4 * E.seg + 1]), G(Qv[4 * E.seg + 2]), G(Qv[4 * E.seg + 3])};
   G qm[4] = {0, 0, 0, 0}, qp[4] = {0, 0, 0, 0};
-  if (hasPrevSeg) { int j = soff + sm; qm[0] = G(QW[j]); qm[1] = G(QX[j]); qm[2] = G(QY[j]); qm[3] = G(QZ[j]); }
-  if (hasNextSeg) { int j = soff + sp; qp[0] = G(QW[j]); qp[1] = G(QX[j]); qp[2] = G(QY[j]); qp[3] = G(QZ[j]); }
+  if (hasPrevSeg) { const int j = 4 * (soff + sm); qm[0] = G(Qv[j]); qm[1] = G(Qv[j + 1]); qm[2] = G(Qv[j + 2]); qm[3] = G(Qv[j + 3]); }
+  if (hasNextSeg) { const int j = 4 * (soff + sp); qp[0] = G(Qv[j]); qp[1] = G(Qv[j + 1]); qp[2] = G(Qv[j + 2]); qp[3] = G(Qv[j + 3]); }
   auto unitAdd = [](G* T, double dx, double dy, double dz) {
     G ex = G(dx), ey = G(dy), ez = G(dz);
     G il = rcp_(dmax(fsqrt_(ex * ex + ey * ey + ez * ez), G(1e-12)));
@@ -408,7 +410,7 @@ __global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(2
   const double uA = gl(I.u)[2 * c], uB = gl(I.u)[2 * c + 1];
   const double uOwn = side == 0 ? uA : uB;
   const int o = gl(I.k_noff)[2 * c] + int(fmin(fmax(floor(uA), 0.0), double(max(gl(I.k_nseg)[2 * c] - 1, 0))));
-  const double ox = gl(I.x[cur])[o], oy = gl(I.y[cur])[o], oz = gl(I.z[cur])[o];
+  const double ox = gl(I.X[cur])[3 * o], oy = gl(I.X[cur])[3 * o + 1], oz = gl(I.X[cur])[3 * o + 2];
   SideEv<G> E, E2;                                // own side; side B when one lane takes both
   eval_side(I, cur, c, side, uOwn, ox, oy, oz, E);
   if (LANES == 1) eval_side(I, cur, c, 1, uB, ox, oy, oz, E2);
@@ -828,10 +830,10 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   // ---------------------------------------------------------------- issue every independent load first
   for (int h = tid; h < cnt + 2; h += TILE) {
     int kk;
-    if (node_ok(k0 - 1 + h, kk)) { int i = noff + kk; L.Xx[h] = gl(I.x[cur])[i]; L.Xy[h] = gl(I.y[cur])[i]; L.Xz[h] = gl(I.z[cur])[i]; }
+    if (node_ok(k0 - 1 + h, kk)) { const int i = 3 * (noff + kk); L.Xx[h] = gl(I.X[cur])[i]; L.Xy[h] = gl(I.X[cur])[i + 1]; L.Xz[h] = gl(I.X[cur])[i + 2]; }
     else { L.Xx[h] = L.Xy[h] = L.Xz[h] = 0.0; }
     int jj;
-    if (seg_ok(k0 - 1 + h, jj)) { int j = soff + jj; L.Qw[h] = gl(I.qw[cur])[j]; L.Qx[h] = gl(I.qx[cur])[j]; L.Qy[h] = gl(I.qy[cur])[j]; L.Qz[h] = gl(I.qz[cur])[j]; }
+    if (seg_ok(k0 - 1 + h, jj)) { const int j = 4 * (soff + jj); L.Qw[h] = gl(I.Q[cur])[j]; L.Qx[h] = gl(I.Q[cur])[j + 1]; L.Qy[h] = gl(I.Q[cur])[j + 2]; L.Qz[h] = gl(I.Q[cur])[j + 3]; }
     else { L.Qw[h] = 1.0; L.Qx[h] = L.Qy[h] = L.Qz[h] = 0.0; }
   }
   const int t = tid;
@@ -1066,9 +1068,9 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       if (fx & 4) V[2] = Real(0);
       const int nx = cur ^ 1;
       const double xn = L.Xx[t] + double(V[0]), yn = L.Xy[t] + double(V[1]), zn = L.Xz[t] + double(V[2]);
-      gl(I.x[nx])[i] = xn;
-      gl(I.y[nx])[i] = yn;
-      gl(I.z[nx])[i] = zn;
+      gl(I.X[nx])[3 * i] = xn;
+      gl(I.X[nx])[3 * i + 1] = yn;
+      gl(I.X[nx])[3 * i + 2] = zn;
       if (RB) { rb[10] = xn; rb[11] = yn; rb[12] = zn; }
       gl(I.vx)[i] = V[0]; gl(I.vy)[i] = V[1]; gl(I.vz)[i] = V[2];
       keN = double(ms) * (double(V[0]) * double(V[0]) + double(V[1]) * double(V[1]) + double(V[2]) * double(V[2]));
@@ -1150,7 +1152,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       double n2 = rw * rw + rx * rx + ry * ry + rz * rz;
       double inv = n2 > 0 ? rsqrt_d(n2) : 1.0;
       const int nx = cur ^ 1;
-      gl(I.qw[nx])[j] = rw * inv; gl(I.qx[nx])[j] = rx * inv; gl(I.qy[nx])[j] = ry * inv; gl(I.qz[nx])[j] = rz * inv;
+      gl(I.Q[nx])[4 * j] = rw * inv; gl(I.Q[nx])[4 * j + 1] = rx * inv; gl(I.Q[nx])[4 * j + 2] = ry * inv; gl(I.Q[nx])[4 * j + 3] = rz * inv;
       keR = double(I3[0]) * double(W[0]) * double(W[0]) + double(I3[1]) * double(W[1]) * double(W[1]) + double(I3[2]) * double(W[2]) * double(W[2]);
     }
   }
@@ -1254,6 +1256,12 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   for (int q = 0; q < NACC; q++) acc[q] = 0.0;
   const int np = I.ntiles;
   const int p0 = it.chunk * I.ctrl_chunk, p1 = min(np, p0 + I.ctrl_chunk);
+  // the first crossing-block partials of the lane go out with the first round of strand partials (independent loads;
+  // added below in the same order as the rest)
+  const int cper = (I.ncblk + it.nchunks - 1) / it.nchunks;
+  const int c0 = it.chunk * cper, c1 = min(I.ncblk, c0 + cper);
+  const bool cin = c0 + tid < c1;
+  const double ce0 = cin ? gl(I.cpart)[c0 + tid] : 0.0, cm0 = cin ? gl(I.cpart)[I.ncblk + c0 + tid] : 0.0;
   for (int w0 = p0; w0 < p1; w0 += CTRL_THREADS * CTRL_PER) {
     double x[CTRL_PER][SPART];
 #pragma unroll
@@ -1274,9 +1282,8 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
       acc[5] += x[k][8];
     }
   }
-  const int cper = (I.ncblk + it.nchunks - 1) / it.nchunks;
-  const int c0 = it.chunk * cper, c1 = min(I.ncblk, c0 + cper);
-  for (int w = c0 + tid; w < c1; w += CTRL_THREADS) {
+  if (cin) { acc[6] += ce0; acc[10] = fmax(acc[10], cm0); }
+  for (int w = c0 + tid + CTRL_THREADS; w < c1; w += CTRL_THREADS) {
     acc[6] += gl(I.cpart)[w];
     acc[10] = fmax(acc[10], gl(I.cpart)[I.ncblk + w]);
   }
@@ -1406,7 +1413,7 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
       }
     } else if (c.phase == 1) {
       const int pr = I.probe;
-      c.ad[0] += gl(I.x[c.cur])[pr] - I.p0x; c.ad[1] += gl(I.y[c.cur])[pr] - I.p0y; c.ad[2] += gl(I.z[c.cur])[pr] - I.p0z;
+      c.ad[0] += gl(I.X[c.cur])[3 * pr] - I.p0x; c.ad[1] += gl(I.X[c.cur])[3 * pr + 1] - I.p0y; c.ad[2] += gl(I.X[c.cur])[3 * pr + 2] - I.p0z;
       if (I.proto == 2) c.ar += gl(I.preact)[0] * I.pax + gl(I.preact)[1] * I.pay + gl(I.preact)[2] * I.paz;
       c.acc_n += 1;
       if (c.it_phase >= I.avg) {
@@ -1422,7 +1429,7 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
           if (I.proto == 1) { c.pf[0] = I.pax * v; c.pf[1] = I.pay * v; c.pf[2] = I.paz * v; }
           else {
             // displacement control: the probe node at P0 + v·axis, at rest, in the state the next iteration reads
-            gl(I.x[c.cur])[pr] = I.p0x + I.pax * v; gl(I.y[c.cur])[pr] = I.p0y + I.pay * v; gl(I.z[c.cur])[pr] = I.p0z + I.paz * v;
+            gl(I.X[c.cur])[3 * pr] = I.p0x + I.pax * v; gl(I.X[c.cur])[3 * pr + 1] = I.p0y + I.pay * v; gl(I.X[c.cur])[3 * pr + 2] = I.p0z + I.paz * v;
             gl(I.vx)[pr] = Real(0); gl(I.vy)[pr] = Real(0); gl(I.vz)[pr] = Real(0);
           }
         }
@@ -1485,23 +1492,23 @@ __global__ __launch_bounds__(TILE) void k_rigid(const Inst<Real>* __restrict__ i
   const int cur = ctl->cur, old = cur ^ 1;          // the kept iteration flipped the buffers: cur holds the new state
   const double v0 = ctl->rv[0], v1 = ctl->rv[1], v2 = ctl->rv[2], w0 = ctl->rw[0], w1 = ctl->rw[1], w2 = ctl->rw[2];
   const int i = tl.noff + k;
-  const double dx = gl(I.x[old])[i] - ctl->rc_prev[0], dy = gl(I.y[old])[i] - ctl->rc_prev[1], dz = gl(I.z[old])[i] - ctl->rc_prev[2];
-  gl(I.x[cur])[i] += v0 + w1 * dz - w2 * dy;
-  gl(I.y[cur])[i] += v1 + w2 * dx - w0 * dz;
-  gl(I.z[cur])[i] += v2 + w0 * dy - w1 * dx;
+  const double dx = gl(I.X[old])[3 * i] - ctl->rc_prev[0], dy = gl(I.X[old])[3 * i + 1] - ctl->rc_prev[1], dz = gl(I.X[old])[3 * i + 2] - ctl->rc_prev[2];
+  gl(I.X[cur])[3 * i] += v0 + w1 * dz - w2 * dy;
+  gl(I.X[cur])[3 * i + 1] += v1 + w2 * dx - w0 * dz;
+  gl(I.X[cur])[3 * i + 2] += v2 + w0 * dy - w1 * dx;
   if (tl.nseg > 0 && (tl.closed || k < tl.nseg)) {
     const int j = tl.soff + k;
     const double a = sqrt(w0 * w0 + w1 * w1 + w2 * w2), half = 0.5 * a;
     const double sc = a > 1e-12 ? sin(half) / a : 0.5;
     const double ew = cos(half), ex = w0 * sc, ey = w1 * sc, ez = w2 * sc;      // exp(w) (quat.js qexp)
-    const double qw = gl(I.qw[cur])[j], qx = gl(I.qx[cur])[j], qy = gl(I.qy[cur])[j], qz = gl(I.qz[cur])[j];
+    const double qw = gl(I.Q[cur])[4 * j], qx = gl(I.Q[cur])[4 * j + 1], qy = gl(I.Q[cur])[4 * j + 2], qz = gl(I.Q[cur])[4 * j + 3];
     const double rw = ew * qw - ex * qx - ey * qy - ez * qz;
     const double rx = ew * qx + ex * qw + ey * qz - ez * qy;
     const double ry = ew * qy - ex * qz + ey * qw + ez * qx;
     const double rz = ew * qz + ex * qy - ey * qx + ez * qw;
     const double n2 = rw * rw + rx * rx + ry * ry + rz * rz;
     const double inv = n2 > 0 ? 1.0 / sqrt(n2) : 1.0;
-    gl(I.qw[cur])[j] = rw * inv; gl(I.qx[cur])[j] = rx * inv; gl(I.qy[cur])[j] = ry * inv; gl(I.qz[cur])[j] = rz * inv;
+    gl(I.Q[cur])[4 * j] = rw * inv; gl(I.Q[cur])[4 * j + 1] = rx * inv; gl(I.Q[cur])[4 * j + 2] = ry * inv; gl(I.Q[cur])[4 * j + 3] = rz * inv;
   }
 }
 
@@ -1530,11 +1537,11 @@ __global__ __launch_bounds__(256) void k_set_xq(const Inst<Real>* __restrict__ i
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < I.n) {
     const double x = X[3 * t], y = X[3 * t + 1], z = X[3 * t + 2];
-    for (int k = 0; k < 2; k++) { gl(I.x[k])[t] = x; gl(I.y[k])[t] = y; gl(I.z[k])[t] = z; }
+    for (int k = 0; k < 2; k++) { gl(I.X[k])[3 * t] = x; gl(I.X[k])[3 * t + 1] = y; gl(I.X[k])[3 * t + 2] = z; }
   }
   if (t < I.m) {
     const double w = Q[4 * t], qx = Q[4 * t + 1], qy = Q[4 * t + 2], qz = Q[4 * t + 3];
-    for (int k = 0; k < 2; k++) { gl(I.qw[k])[t] = w; gl(I.qx[k])[t] = qx; gl(I.qy[k])[t] = qy; gl(I.qz[k])[t] = qz; }
+    for (int k = 0; k < 2; k++) { gl(I.Q[k])[4 * t] = w; gl(I.Q[k])[4 * t + 1] = qx; gl(I.Q[k])[4 * t + 2] = qy; gl(I.Q[k])[4 * t + 3] = qz; }
   }
 }
 
