@@ -12,7 +12,8 @@ Prints one JSON line.  `value` is whole-job node-iterations/s with the inputs re
 strand kernel (the dominant launch): algorithmic bytes per launch over its mean duration (HIP events on the
 solver's stream) against the 8 TB/s HBM peak, and `traffic` the HBM bytes per launch read by two rocprofv3
 counter passes (FETCH_SIZE, WRITE_SIZE) scaled by what the same counters read for a plain copy kernel.
-`cpu_baseline` is the f64 oracle (a C++ restatement of the reference's CpuSolver) on this host, one thread.
+`cpu_baseline` is the f64 oracle (a C++ restatement of the reference's CpuSolver) on this host: one thread on the
+single-instance workloads, every host thread (one instance each, at most 16) in `--mode solves`.
 """
 import argparse
 import csv
@@ -78,6 +79,55 @@ def cpu_baseline(model, budget_s=12.0):
     return {"value": model.n * its / el, "unit": "node-iterations/s", "cores": 1, "kind": "port",
             "sample": f"{its} iterations of the same {model.n}-node instance from its initial state, f64 oracle "
                       f"(CpuSolver restated in C++, -O3, no fast-math), one host thread, {el:.1f} s"}
+
+
+def host_threads():
+    """host threads for the all-cores baseline: the CPUs this process may run on, at most 16 (one GPU's share of a
+    shared host)"""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_solves_baseline(models, iters, budget_s=20.0, per_thread=2):
+    """the f64 oracle on all host cores, one instance per thread (a sample of the same variants, evenly spaced over
+    the set): each sampled instance runs a bounded number of iterations from its initial state; its solve time is its
+    measured time per iteration × the iterations the batched run needed to stop.  All-cores solves/s = threads ÷ the
+    mean solve time (every thread busy with one solve after another)"""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle.bind import Oracle
+    T = host_threads()
+    B = len(models)
+    k = min(B, T * per_thread)
+    idx = sorted(set(int(round(q * (B - 1) / max(k - 1, 1))) for q in range(k)))
+    per_inst = budget_s / per_thread
+
+    def one(i):
+        o = Oracle(models[i])
+        t = time.perf_counter()
+        o.step(3)
+        dt = max(time.perf_counter() - t, 1e-6) / 3
+        its = int(min(max(per_inst / dt, 10), iters[i]))
+        t = time.perf_counter()
+        o.step(its)
+        el = time.perf_counter() - t
+        return i, its, el
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=T) as ex:      # the oracle's calls release the interpreter lock
+        res = list(ex.map(one, idx))
+    wall = time.perf_counter() - t0
+    solve_s = [el / its * iters[i] for i, its, el in res]
+    mean_solve = sum(solve_s) / len(solve_s)
+    return {"value": T / mean_solve, "unit": "solves/s", "cores": T, "kind": "port",
+            "node_iters_per_s": sum(models[i].n * its for i, its, _ in res) / wall,
+            "single_core_solves_per_s": 1.0 / mean_solve,
+            "sample": f"{len(idx)} of the {B} variants (evenly spaced), {T} host threads, one instance per thread, "
+                      f"each run from its initial state for up to {per_inst:.0f} s of f64 oracle iterations (CpuSolver "
+                      f"restated in C++, -O3, no fast-math); solve time = time per iteration x the batched run's "
+                      f"iterations to the stop; {wall:.1f} s wall"}
 
 
 # ------------------------------------------------------------------------------------------------ counter passes
@@ -167,7 +217,7 @@ def run_solves(args):
         per_solve.append(dt * iters[i])
     plain = 1.0 / (sum(per_solve) / len(per_solve))
     nodes = sum(m.n for m in models)
-    return {
+    out = {
         "metric": "solves/s", "value": batched, "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded design variants)",
         "config": {"workload": f"{B} design variants ({WORKLOADS['variants']}), relaxed to residual <= tol or "
@@ -179,6 +229,9 @@ def run_solves(args):
         "plain_sample": {"instances": idx, "timed_iterations_each": args.plain_iters},
         "batched_over_plain": batched / plain,
     }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_solves_baseline(models, iters)
+    return out
 
 
 # ------------------------------------------------------------------------------------------------ serving
@@ -214,7 +267,7 @@ def run_serve(args):
     for w in workers:
         w.start()
     srv = W.Server(precision="fp32", chunk=args.chunk, tile=64, max_slots=args.slots)
-    lat, nodes, next_i, done, results = [], 0, 0, 0, []
+    lat, nodes, next_i, done, results, its_of = [], 0, 0, 0, [], {}
     t0 = time.perf_counter()
     t_report = 30.0
     while done < len(arrivals):
@@ -238,6 +291,7 @@ def run_serve(args):
         for r in srv.poll():
             idx = r["id"] - 1
             lat.append((time.perf_counter() - t0 - arrivals[idx]) * 1e3)
+            its_of[idx] = r["iterations"]
             nodes += r["n"] * r["iterations"]
             results.append(r["status"])
             done += 1
@@ -248,7 +302,7 @@ def run_serve(args):
     srv.close()
     lat.sort()
     pct = lambda p: lat[min(len(lat) - 1, int(p / 100.0 * len(lat)))]
-    return {
+    out = {
         "metric": "solves/s", "value": len(lat) / wall, "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded request stream)",
         "config": {"workload": f"Poisson stream at {args.rate}/s over {args.duration} s of design variants "
@@ -258,6 +312,14 @@ def run_serve(args):
         "wall_s": wall, "node_iters_per_s": nodes / wall,
         "status": {k: results.count(k) for k in set(results)},
     }
+    if not args.no_cpu:   # all host cores on a sample of the same requests, after the stream
+        R = len(arrivals)
+        k = min(R, 2 * host_threads())
+        idx = sorted(set(int(round(q * (R - 1) / max(k - 1, 1))) for q in range(k)))
+        out["cpu_baseline"] = cpu_solves_baseline([S.request_variant(i) for i in idx],
+                                                  [its_of.get(i, args.max_iter) for i in idx])
+        out["cpu_baseline"]["sample"] += f" (requests {idx[0]}..{idx[-1]} of {R})"
+    return out
 
 
 def run_stability(args):

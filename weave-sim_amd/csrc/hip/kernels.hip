@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 #include <type_traits>
 
 #include "device.hpp"
@@ -171,6 +172,26 @@ __device__ __forceinline__ void rotT(const double* d, const R* v, R* out) {
 
 __device__ __forceinline__ bool inst_live(const Ctl* c) { return c->done == 0 && c->iter < c->stop_at; }
 
+// the first eight words of a control record (iter, done, rollback, al_update, stop_at, hist_n, cur, counter) in one
+// scalar load: separate narrow loads of the same record can be split over two memory latencies
+struct CtlHead { int32_t iter, done, rollback, al_update, stop_at, hist_n, cur, counter; };
+__device__ __forceinline__ CtlHead ctl_head(const Ctl* c) {
+  typedef int32_t v8 __attribute__((ext_vector_type(8))) __attribute__((aligned(8)));
+  const v8 v = *reinterpret_cast<const v8*>(c);
+  return CtlHead{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]};
+}
+static_assert(offsetof(Ctl, iter) == 0 && offsetof(Ctl, done) == 4 && offsetof(Ctl, rollback) == 8 &&
+              offsetof(Ctl, al_update) == 12 && offsetof(Ctl, stop_at) == 16 && offsetof(Ctl, cur) == 24,
+              "control record head layout");
+
+// one of a descriptor's two state buffers: both pointers are read and one is selected, so the read does not wait for
+// the control record that names the buffer (a read indexed by it would)
+template <typename T> __device__ __forceinline__ T* buf(T* const (&p)[2], int k) {
+  T* const a = p[0];
+  T* const b = p[1];
+  return k ? b : a;
+}
+
 // An array pointer read from an instance descriptor, in the global address space: its loads and stores become
 // global_* instead of flat_* (a flat access also counts against the LDS wait counter, so every LDS wait would wait
 // for it too).  The descriptor's pointers all come from hipMalloc.
@@ -239,13 +260,13 @@ __device__ __forceinline__ void eval_side(const Inst<Real>& I, int cur, int c, i
   const int sp = closed ? (i + 1) % ns : i + 1;
   // positions and frames are interleaved (x y z per node, w x y z per segment): a side's four nodes and three
   // segments are two short contiguous runs, a few cache lines
-  const auto P = gl(I.X[cur]);
+  const auto P = gl(buf(I.X, cur));
   const double xa = P[3 * E.a], ya = P[3 * E.a + 1], za = P[3 * E.a + 2];
   const double xb = P[3 * E.b], yb = P[3 * E.b + 1], zb = P[3 * E.b + 2];
   double xm = 0, ym = 0, zm = 0, xp = 0, yp = 0, zp = 0;
   if (hasPrevSeg) { const int k = 3 * (noff + iam); xm = P[k]; ym = P[k + 1]; zm = P[k + 2]; }
   if (hasNextSeg) { const int k = 3 * (noff + ibp); xp = P[k]; yp = P[k + 1]; zp = P[k + 2]; }
-  const auto Qv = gl(I.Q[cur]);
+  const auto Qv = gl(buf(I.Q, cur));
   G q0[4] = {G(Qv[4 * E.seg]), G(Qv[4 * E.seg + 1]), G(Qv[4 * E.seg + 2]), G(Qv[4 * E.seg + 3])};
   G qm[4] = {0, 0, 0, 0}, qp[4] = {0, 0, 0, 0};
   if (hasPrevSeg) { const int j = 4 * (soff + sm); qm[0] = G(Qv[j]); qm[1] = G(Qv[j + 1]); qm[2] = G(Qv[j + 2]); qm[3] = G(Qv[j + 3]); }
@@ -384,13 +405,16 @@ __global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(2
   using H = Real;
   int bid = blockIdx.x;
   if (bid >= nitems) return;
-  CrossItem it = items[bid];
-  if (it.inst < 0) return;
+  // the work list holds blocks of occupied slots only (it.inst >= 0); the liveness test waits until the first
+  // crossing loads are out, so the item, the control record with the descriptor, and the crossing state cost three
+  // memory latencies, not one per test
+  const CrossItem it = items[bid];
   const Inst<Real>& I = insts[it.inst];
   const Ctl* ctl = ctls + it.inst;
-  if (!inst_live(ctl)) return;
-  const int cur = ctl->cur, iter = ctl->iter;
-  const bool al_update = ctl->al_update != 0;
+  const CtlHead ch = ctl_head(ctl);
+  const int cur = ch.cur, iter = ch.iter;
+  const bool live = (ch.done == 0) & (iter < ch.stop_at);
+  const bool al_update = ch.al_update != 0;
   const int pen_n = ctl->pen_n;
   const int tid = threadIdx.x;
   const int side = LANES == 2 ? (tid & 1) : 0, jj = LANES == 2 ? (tid >> 1) : tid;
@@ -408,9 +432,12 @@ __global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(2
   const G mu0 = G(I.mu);
   // ---- own side, relative to the first node of A's segment
   const double uA = gl(I.u)[2 * c], uB = gl(I.u)[2 * c + 1];
+  const int noffA = gl(I.k_noff)[2 * c], nsegA = gl(I.k_nseg)[2 * c];
+  if (!live) return;                               // uniform over the workgroup (one instance per block)
   const double uOwn = side == 0 ? uA : uB;
-  const int o = gl(I.k_noff)[2 * c] + int(fmin(fmax(floor(uA), 0.0), double(max(gl(I.k_nseg)[2 * c] - 1, 0))));
-  const double ox = gl(I.X[cur])[3 * o], oy = gl(I.X[cur])[3 * o + 1], oz = gl(I.X[cur])[3 * o + 2];
+  const int o = noffA + int(fmin(fmax(floor(uA), 0.0), double(max(nsegA - 1, 0))));
+  const auto Xc = gl(buf(I.X, cur));
+  const double ox = Xc[3 * o], oy = Xc[3 * o + 1], oz = Xc[3 * o + 2];
   SideEv<G> E, E2;                                // own side; side B when one lane takes both
   eval_side(I, cur, c, side, uOwn, ox, oy, oz, E);
   if (LANES == 1) eval_side(I, cur, c, 1, uB, ox, oy, oz, E2);
@@ -814,11 +841,14 @@ template <typename Real, int TILE, bool OUT, bool RB>
 __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem& tl, StrandLDS<Real, TILE>& L,
                             const OutPtrs& out) {
   const int tid = threadIdx.x;
-  int cur = ctl->cur;
-  const bool rbk = ctl->rollback != 0;
+  const CtlHead ch = ctl_head(ctl);
+  int cur = ch.cur;
+  const bool rbk = ch.rollback != 0;
   // the state the last iteration evaluated (downloads), or the current one (evaluations: out.F set)
   if (OUT) cur = (rbk || out.F) ? cur : (cur ^ 1);
-  const int iter = ctl->iter;
+  const int iter = ch.iter;
+  // an iteration runs only on a live instance (not stopped, below its stop); tested after the first loads are issued
+  const bool live = OUT || ((ch.done == 0) & (iter < ch.stop_at));
   const int noff = tl.noff, nn = tl.nn, soff = tl.soff, nseg = tl.nseg, joff = tl.joff, njs = tl.njs;
   const bool closed = tl.closed != 0;
   const int k0 = tl.k0, k1 = tl.k1, cnt = k1 - k0;
@@ -828,12 +858,13 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   auto joint_ok = [&](int j, int& jj) { if (closed) { if (njs <= 0) { jj = 0; return false; } jj = wrap(j, njs); return true; } jj = j; return j >= 0 && j < njs; };
 
   // ---------------------------------------------------------------- issue every independent load first
+  const auto Xc = gl(buf(I.X, cur)), Qc = gl(buf(I.Q, cur));
   for (int h = tid; h < cnt + 2; h += TILE) {
     int kk;
-    if (node_ok(k0 - 1 + h, kk)) { const int i = 3 * (noff + kk); L.Xx[h] = gl(I.X[cur])[i]; L.Xy[h] = gl(I.X[cur])[i + 1]; L.Xz[h] = gl(I.X[cur])[i + 2]; }
+    if (node_ok(k0 - 1 + h, kk)) { const int i = 3 * (noff + kk); L.Xx[h] = Xc[i]; L.Xy[h] = Xc[i + 1]; L.Xz[h] = Xc[i + 2]; }
     else { L.Xx[h] = L.Xy[h] = L.Xz[h] = 0.0; }
     int jj;
-    if (seg_ok(k0 - 1 + h, jj)) { const int j = 4 * (soff + jj); L.Qw[h] = gl(I.Q[cur])[j]; L.Qx[h] = gl(I.Q[cur])[j + 1]; L.Qy[h] = gl(I.Q[cur])[j + 2]; L.Qz[h] = gl(I.Q[cur])[j + 3]; }
+    if (seg_ok(k0 - 1 + h, jj)) { const int j = 4 * (soff + jj); L.Qw[h] = Qc[j]; L.Qx[h] = Qc[j + 1]; L.Qy[h] = Qc[j + 2]; L.Qz[h] = Qc[j + 3]; }
     else { L.Qw[h] = 1.0; L.Qx[h] = L.Qy[h] = L.Qz[h] = 0.0; }
   }
   const int t = tid;
@@ -871,6 +902,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   const int so = tl.so, ns = tl.ns;
   SideRec<Real> r0;
   if (tid < ns) side_load(gl(I.side) + so + tid, r0);
+  if (!live) return;                               // uniform over the workgroup (one instance per tile)
   __syncthreads();
 
   // ---------------------------------------------------------------- segments and joints
@@ -1068,9 +1100,10 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       if (fx & 4) V[2] = Real(0);
       const int nx = cur ^ 1;
       const double xn = L.Xx[t] + double(V[0]), yn = L.Xy[t] + double(V[1]), zn = L.Xz[t] + double(V[2]);
-      gl(I.X[nx])[3 * i] = xn;
-      gl(I.X[nx])[3 * i + 1] = yn;
-      gl(I.X[nx])[3 * i + 2] = zn;
+      const auto Xn = gl(buf(I.X, nx));
+      Xn[3 * i] = xn;
+      Xn[3 * i + 1] = yn;
+      Xn[3 * i + 2] = zn;
       if (RB) { rb[10] = xn; rb[11] = yn; rb[12] = zn; }
       gl(I.vx)[i] = V[0]; gl(I.vy)[i] = V[1]; gl(I.vz)[i] = V[2];
       keN = double(ms) * (double(V[0]) * double(V[0]) + double(V[1]) * double(V[1]) + double(V[2]) * double(V[2]));
@@ -1152,7 +1185,8 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       double n2 = rw * rw + rx * rx + ry * ry + rz * rz;
       double inv = n2 > 0 ? rsqrt_d(n2) : 1.0;
       const int nx = cur ^ 1;
-      gl(I.Q[nx])[4 * j] = rw * inv; gl(I.Q[nx])[4 * j + 1] = rx * inv; gl(I.Q[nx])[4 * j + 2] = ry * inv; gl(I.Q[nx])[4 * j + 3] = rz * inv;
+      const auto Qn = gl(buf(I.Q, nx));
+      Qn[4 * j] = rw * inv; Qn[4 * j + 1] = rx * inv; Qn[4 * j + 2] = ry * inv; Qn[4 * j + 3] = rz * inv;
       keR = double(I3[0]) * double(W[0]) * double(W[0]) + double(I3[1]) * double(W[1]) * double(W[1]) + double(I3[2]) * double(W[2]) * double(W[2]);
     }
   }
@@ -1261,7 +1295,11 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   const int cper = (I.ncblk + it.nchunks - 1) / it.nchunks;
   const int c0 = it.chunk * cper, c1 = min(I.ncblk, c0 + cper);
   const bool cin = c0 + tid < c1;
-  const double ce0 = cin ? gl(I.cpart)[c0 + tid] : 0.0, cm0 = cin ? gl(I.cpart)[I.ncblk + c0 + tid] : 0.0;
+  // unconditional loads at an index kept in range (the array holds at least one block's partials), so the
+  // descriptor's pointer is not read behind a branch
+  const int ci = cin ? c0 + tid : 0;
+  const double ce0v = gl(I.cpart)[ci], cm0v = gl(I.cpart)[I.ncblk + ci];
+  const double ce0 = cin ? ce0v : 0.0, cm0 = cin ? cm0v : 0.0;
   for (int w0 = p0; w0 < p1; w0 += CTRL_THREADS * CTRL_PER) {
     double x[CTRL_PER][SPART];
 #pragma unroll
@@ -1465,11 +1503,11 @@ __global__ __launch_bounds__(TILE) void k_strand(const Inst<Real>* __restrict__ 
   __shared__ StrandLDS<Real, TILE> L;
   int lt = xcd_tile(blockIdx.x, gridDim.x);
   if (lt >= ntiles) return;
+  // the work list holds tiles of occupied slots only (tl.inst >= 0); the liveness test waits until the tile's first
+  // loads are out (strand_tile), so the item, the control record, the descriptor and the state cost three memory
+  // latencies, not one per test
   const TileItem tl = tiles[lt];
-  if (tl.inst < 0) return;
-  const Ctl* ctl = ctls + tl.inst;
-  if (!inst_live(ctl)) return;
-  strand_tile<Real, TILE, false, RB>(insts[tl.inst], ctl, tl, L, OutPtrs{});
+  strand_tile<Real, TILE, false, RB>(insts[tl.inst], ctls + tl.inst, tl, L, OutPtrs{});
 }
 
 // The rigid step of a structure only the ground holds (CpuSolver.step, solver_cpu.js:791-805), after the control
