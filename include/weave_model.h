@@ -43,7 +43,7 @@ typedef struct wsm_scene {
   /* loads (scene.js:40) */
   int32_t gravity;
   double gravity_dir[3];
-  int32_t udl;                             /* UDL per length */
+  int32_t udl;                             /* UDL: 0 off, 1 per length (model.js:215), 2 per area of the mesh (:216-219) */
   double udl_q[3];
 } wsm_scene;
 

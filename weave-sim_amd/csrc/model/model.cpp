@@ -502,7 +502,22 @@ struct Setup {
   std::vector<wsm_point_load> loads;
   wsm_prep prep{};
   bool hasPrep = false;
+  double meshArea = 0;     // area of the welded mesh (a UDL per area spreads over it); 0 without a mesh
 };
+
+// Σ of the triangle areas (Mesh.area, mesh.js:50-58)
+double meshArea(const Mesh& mesh) {
+  double a = 0;
+  for (int f = 0; f < mesh.nF; f++) {
+    const double* Vi = &mesh.V[3 * size_t(mesh.F[3 * f])];
+    const double* Vj = &mesh.V[3 * size_t(mesh.F[3 * f + 1])];
+    const double* Vk = &mesh.V[3 * size_t(mesh.F[3 * f + 2])];
+    const double ux = Vj[0] - Vi[0], uy = Vj[1] - Vi[1], uz = Vj[2] - Vi[2];
+    const double vx = Vk[0] - Vi[0], vy = Vk[1] - Vi[1], vz = Vk[2] - Vi[2];
+    a += 0.5 * std::hypot(uy * vz - uz * vy, uz * vx - ux * vz, ux * vy - uy * vx);
+  }
+  return a;
+}
 
 // section stiffness (params.js:33-45) × beam factors (:49-57)
 struct Stiff { double EA, EIx, EIy, GJ, GA; double act[3]; };
@@ -751,7 +766,17 @@ std::unique_ptr<Model> buildModel(const Built& tp, const Setup& su) {
       for (int i : {md->segI0[j], md->segI1[j]}) for (int c = 0; c < 3; c++) md->loadConst[3 * i + c] += q * g[c];
     }
   }
-  if (sc.udl) for (int i = 0; i < n; i++) for (int c = 0; c < 3; c++) md->loadConst[3 * i + c] += trib[i] * sc.udl_q[c];
+  if (sc.udl == 2) {
+    // per area: the mesh's area shared out in proportion to the nodes' tributary lengths (model.js:216-219)
+    double total = 0;
+    for (int i = 0; i < n; i++) total += trib[i];
+    for (int i = 0; i < n; i++) {
+      const double w = trib[i] / std::max(total, 1e-12) * su.meshArea;
+      for (int c = 0; c < 3; c++) md->loadConst[3 * i + c] += w * sc.udl_q[c];
+    }
+  } else if (sc.udl) {
+    for (int i = 0; i < n; i++) for (int c = 0; c < 3; c++) md->loadConst[3 * i + c] += trib[i] * sc.udl_q[c];
+  }
   for (const auto& pl : su.loads) {
     auto it = md->nodeFrameIds.find(fkey(pl.s, pl.k));
     if (it == md->nodeFrameIds.end()) continue;
@@ -961,6 +986,7 @@ WS_API int wsm_from_mesh(const double* V, int32_t nV, const int32_t* F, int32_t 
     Setup su; su.sc = *sc;
     su.anchors.assign(anchors, anchors + n_anchors); su.loads.assign(loads, loads + n_loads);
     if (prep) { su.prep = *prep; su.hasPrep = true; }
+    su.meshArea = meshArea(mesh);
     auto md = buildModel(fromTopology(topo), su);
     md->info.nV = mesh.nV; md->info.nF = mesh.nF; md->info.nE = mesh.nE; md->info.subdivisions = sc->subdivisions;
     md->info.parity_conflicts = topo.parityConflicts;
