@@ -136,3 +136,36 @@ def test_js_build_results_match_python(W, gpu_ctx):
     assert np.array_equal(np.array(d["disp"]), r["disp"]) and d["nreact"] == len(r["reactions"])
     doc = R.results_json(py, r)
     assert d["doc"]["segment"]["My"] == doc["segment"]["My"] and d["doc"]["reactions"].keys() == doc["reactions"].keys()
+
+
+def test_js_scene_reader_matches_python():
+    """the reference's example scenes read by the JS reader (scene_mi355x.js: scene.json, STL) and by the Python one
+    build the same model: sizes, initial state, anchors, loads and solver parameters, bit for bit"""
+    import importlib
+    SC = importlib.import_module("weave-sim_amd.scene")
+    golden = os.path.join(ROOT, "tests", "golden", "scenes")
+    names = ["dome_basic", "v5dome_cm", "tree_cm", "dualv5dome_cm"]
+    scene_js = os.path.join(ROOT, "weave-sim_amd", "js", "scene_mi355x.js")
+    out = run_node(f"""
+const S = require({json.dumps(scene_js)});
+const res = {{}};
+for (const name of {json.dumps(names)}) {{
+  const m = S.loadScene(require('path').join({json.dumps(golden)}, name + '.json'));
+  res[name] = {{ info: {{ n: m.info.n, m: m.info.m, K: m.info.K, S: m.info.S, params: m.info.params }},
+    X: Array.from(m.st.X), u: Array.from(m.st.u), loads: Array.from(m.info.loadConst), free: Array.from(m.info.nodeFree) }};
+}}
+let refused = '';
+try {{ S.loadScene(require('path').join({json.dumps(golden)}, 'dualv5dome_cm_hybrid.json')); }} catch (e) {{ refused = e.message; }}
+res.refused = refused;
+console.log(JSON.stringify(res));""")
+    d = json.loads(out)
+    assert "hybrid" in d["refused"]
+    for name in names:
+        py = SC.load_scene(os.path.join(golden, name + ".json"))
+        js = d[name]
+        assert (js["info"]["n"], js["info"]["m"], js["info"]["K"], js["info"]["S"]) == (py.n, py.m, py.K, py.md.S)
+        assert np.array_equal(np.array(js["X"]), py["X"]) and np.array_equal(np.array(js["u"]), py["u"])
+        assert np.array_equal(np.array(js["loads"]), py.arr("load_const"))
+        assert np.array_equal(np.array(js["free"], dtype=np.uint8), py.arr("node_free"))
+        for k, v in js["info"]["params"].items():
+            assert v == getattr(py.md.par, k), (name, k, v, getattr(py.md.par, k))

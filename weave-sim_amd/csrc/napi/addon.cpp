@@ -144,6 +144,16 @@ napi_value DeviceInfo(napi_env env, napi_callback_info info) {
 }
 
 // ---------------------------------------------------------------------------------------------- host models
+void get3(napi_env env, napi_value o, const char* k, double* out) {
+  bool has = false; napi_value v; bool is = false; uint32_t n = 0;
+  if (napi_has_named_property(env, o, k, &has) != napi_ok || !has || napi_get_named_property(env, o, k, &v) != napi_ok) return;
+  if (napi_is_array(env, v, &is) != napi_ok || !is || napi_get_array_length(env, v, &n) != napi_ok || n != 3) return;
+  for (uint32_t i = 0; i < 3; i++) {
+    napi_value e; double x = 0;
+    if (napi_get_element(env, v, i, &e) == napi_ok && napi_get_value_double(env, e, &x) == napi_ok) out[i] = x;
+  }
+}
+
 void scene_from(napi_env env, napi_value o, wsm_scene& sc) {
   wsm_scene_defaults(&sc);
   if (!is_object(env, o)) return;
@@ -159,9 +169,36 @@ void scene_from(napi_env env, napi_value o, wsm_scene& sc) {
   I(par.ramp_iter, "ramp_iter"); I(par.rot_friction, "rot_friction"); I(par.seat_law, "seat_law"); I(par.seat_gap, "seat_gap");
   I(par.dent_after, "dent_after"); I(par.augmented, "augmented"); I(par.al_every, "al_every");
   I(contacts, "contacts"); I(locked, "locked"); I(gravity, "gravity");
+  // the rest of a scene (scene.json through the JS reader): section constants, beam factors, ground, weave, units,
+  // UDL; vectors as 3-element arrays
+  D(grp.Ix, "Ix"); D(grp.Iy, "Iy"); D(grp.J, "J"); I(grp.beam_advanced, "beam_advanced");
+  D(grp.activation, "activation"); D(grp.stiffness, "stiffness"); D(grp.act_flat, "act_flat"); D(grp.act_edge, "act_edge");
+  D(grp.act_twist, "act_twist"); D(grp.k_axial, "k_axial"); D(grp.k_flat, "k_flat"); D(grp.k_edge, "k_edge");
+  D(grp.k_twist, "k_twist"); D(grp.k_shear, "k_shear");
+  I(par.floor_on, "floor_on"); D(par.floor_z, "floor_z"); D(par.floor_mu, "floor_mu");
+  I(prestress_asbuilt, "prestress_asbuilt"); I(subdivisions, "subdivisions"); D(overhang, "overhang");
+  I(offset_half_height, "offset_half_height"); I(seed_strand_phase, "seed_strand_phase");
+  D(length_to_m, "length_to_m"); D(force_to_n, "force_to_n"); I(udl, "udl");
+  get3(env, o, "gravity_dir", sc.gravity_dir); get3(env, o, "udl_q", sc.udl_q);
 #undef D
 #undef I
 }
+
+// a typed array's data and element count (null / 0 when v is not one of the expected type)
+template <typename T> T* typed(napi_env env, napi_value v, napi_typedarray_type want, size_t* n) {
+  bool is = false; *n = 0;
+  if (napi_is_typedarray(env, v, &is) != napi_ok || !is) return nullptr;
+  napi_typedarray_type t; void* data = nullptr; napi_value ab; size_t off = 0;
+  if (napi_get_typedarray_info(env, v, &t, n, &data, &ab, &off) != napi_ok || t != want) { *n = 0; return nullptr; }
+  return static_cast<T*>(data);
+}
+uint32_t array_len(napi_env env, napi_value v) {
+  bool is = false; uint32_t n = 0;
+  if (napi_is_array(env, v, &is) == napi_ok && is) napi_get_array_length(env, v, &n);
+  return n;
+}
+napi_value elem(napi_env env, napi_value v, uint32_t i) { napi_value e; napi_get_element(env, v, i, &e); return e; }
+double num_at(napi_env env, napi_value v, uint32_t i) { double x = 0; napi_get_value_double(env, elem(env, v, i), &x); return x; }
 
 napi_value wrap_model(napi_env env, wsm_model* h) {
   Model* m = new Model();
@@ -183,6 +220,51 @@ napi_value ModelHexDome(napi_env env, napi_callback_info info) {
   wsm_scene sc; scene_from(env, a[1], sc);
   wsm_model* h = nullptr;
   int rc = wsm_hex_dome(&spec, &sc, &h);
+  if (rc) { napi_throw_error(env, nullptr, wsm_last_error()); return nullptr; }
+  return wrap_model(env, h);
+}
+
+// modelFromMesh(V: Float64Array [3nV], F: Int32Array [3nF], scene, anchors: [{type, s, k, fix[6]}],
+//               loads: [{s, k, F[3], local}], flips: [[s, k], ...], weld) — buildTopology … _buildGoals on a mesh
+napi_value ModelFromMesh(napi_env env, napi_callback_info info) {
+  napi_value a[7]; get_args(env, info, 7, a);
+  size_t nV = 0, nF = 0;
+  const double* V = typed<double>(env, a[0], napi_float64_array, &nV);
+  const int32_t* F = typed<int32_t>(env, a[1], napi_int32_array, &nF);
+  if (!V || !F || nV % 3 || nF % 3 || !nV || !nF) {
+    napi_throw_type_error(env, nullptr, "modelFromMesh: V must be a Float64Array of 3·nV, F an Int32Array of 3·nF");
+    return nullptr;
+  }
+  wsm_scene sc; scene_from(env, a[2], sc);
+  std::vector<wsm_anchor> an(array_len(env, a[3]));
+  for (uint32_t i = 0; i < an.size(); i++) {
+    napi_value e = elem(env, a[3], i);
+    an[i].type = int32_t(getd(env, e, "type", 0)); an[i].s = int32_t(getd(env, e, "s", 0)); an[i].k = int32_t(getd(env, e, "k", 0));
+    napi_value fx; bool has = false;
+    napi_has_named_property(env, e, "fix", &has);
+    if (has && napi_get_named_property(env, e, "fix", &fx) == napi_ok)
+      for (uint32_t c = 0; c < 6; c++) { bool b = false; napi_value x = elem(env, fx, c); napi_valuetype t; napi_typeof(env, x, &t);
+        if (t == napi_boolean) napi_get_value_bool(env, x, &b); else if (t == napi_number) { double d = 0; napi_get_value_double(env, x, &d); b = d != 0; }
+        an[i].fix[c] = b ? 1 : 0; }
+  }
+  std::vector<wsm_point_load> ld(array_len(env, a[4]));
+  for (uint32_t i = 0; i < ld.size(); i++) {
+    napi_value e = elem(env, a[4], i);
+    ld[i].s = int32_t(getd(env, e, "s", 0)); ld[i].k = int32_t(getd(env, e, "k", 0)); ld[i].local = getd(env, e, "local", 0) != 0;
+    napi_value f;
+    if (napi_get_named_property(env, e, "F", &f) == napi_ok && array_len(env, f) == 3)
+      for (uint32_t c = 0; c < 3; c++) ld[i].F[c] = num_at(env, f, c);
+  }
+  std::vector<int32_t> fl;
+  for (uint32_t i = 0, n = array_len(env, a[5]); i < n; i++) {
+    napi_value e = elem(env, a[5], i);
+    fl.push_back(int32_t(num_at(env, e, 0))); fl.push_back(int32_t(num_at(env, e, 1)));
+  }
+  bool weld = true; napi_valuetype wt; napi_typeof(env, a[6], &wt);
+  if (wt == napi_boolean) napi_get_value_bool(env, a[6], &weld);
+  wsm_model* h = nullptr;
+  int rc = wsm_from_mesh(V, int32_t(nV / 3), F, int32_t(nF / 3), weld ? 1 : 0, &sc, an.data(), int32_t(an.size()),
+                         ld.data(), int32_t(ld.size()), fl.data(), int32_t(fl.size() / 2), nullptr, &h);
   if (rc) { napi_throw_error(env, nullptr, wsm_last_error()); return nullptr; }
   return wrap_model(env, h);
 }
@@ -273,6 +355,16 @@ napi_value ModelInfo(napi_env env, napi_callback_info info) {
   copy("nodeOffset", m->md.node_offset, size_t(f.S) + 1, napi_int32_array, 4);
   copy("segOffset", m->md.seg_offset, size_t(f.S) + 1, napi_int32_array, 4);
   copy("nodeFree", m->md.node_free, 3 * size_t(f.n), napi_uint8_array, 1);
+  copy("loadConst", m->md.load_const, 3 * size_t(f.n), napi_float64_array, 8);
+  napi_value pr; napi_create_object(env, &pr);
+  set(env, pr, "mass_scale", num(env, P.mass_scale)); set(env, pr, "damping", num(env, P.damping)); set(env, pr, "tol", num(env, P.tol));
+  set(env, pr, "mu", num(env, P.mu)); set(env, pr, "strength", num(env, P.strength)); set(env, pr, "stick_strength", num(env, P.stick_strength));
+  set(env, pr, "torsion_factor", num(env, P.torsion_factor)); set(env, pr, "epsilon", num(env, P.epsilon));
+  set(env, pr, "slip_rate", num(env, P.slip_rate)); set(env, pr, "rot_factor", num(env, P.rot_factor)); set(env, pr, "dent_rate", num(env, P.dent_rate));
+  set(env, pr, "edge_threshold", num(env, P.edge_threshold)); set(env, pr, "warmup_iter", i32(env, P.warmup_iter));
+  set(env, pr, "ramp_iter", i32(env, P.ramp_iter)); set(env, pr, "dent_after", i32(env, P.dent_after)); set(env, pr, "al_every", i32(env, P.al_every));
+  set(env, pr, "seat_gap", i32(env, P.seat_gap)); set(env, pr, "floor_on", i32(env, P.floor_on)); set(env, pr, "kinetic_damping", i32(env, P.kinetic_damping));
+  set(env, o, "params", pr);
   return o;
 }
 
@@ -607,7 +699,7 @@ napi_value Init(napi_env env, napi_value exports) {
   struct Fn { const char* name; napi_callback cb; };
   static const Fn fns[] = {
       {"deviceCreate", DeviceCreate}, {"deviceInfo", DeviceInfo},
-      {"modelHexDome", ModelHexDome}, {"modelClone", ModelClone}, {"modelInfo", ModelInfo}, {"modelState", ModelState},
+      {"modelHexDome", ModelHexDome}, {"modelFromMesh", ModelFromMesh}, {"modelClone", ModelClone}, {"modelInfo", ModelInfo}, {"modelState", ModelState},
       {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy}, {"batchOpts", BatchOpts}, {"batchStep", BatchStep},
       {"batchReadControl", BatchReadControl}, {"batchReadHistory", BatchReadHistory}, {"batchDownload", BatchDownload},
       {"batchReadOutputs", BatchReadOutputs}, {"modelSetProtocol", ModelSetProtocol},

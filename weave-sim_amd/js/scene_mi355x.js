@@ -1,0 +1,173 @@
+'use strict';
+// Scenes as the reference stores them (scene.json version 1 and the mesh it names) read into a host model, for the
+// Node.js host: Scene.fromJSON (web/js/scene.js:136-187) — every section from the params.js defaults plus the keys
+// the file holds, E90 / fc90 / H of older files from their N/mm² defaults in the scene's stress unit — and
+// Mesh.fromBytes (web/js/mesh.js:24-31): STL (binary when the size says so, else ASCII) or OBJ (fan-triangulated),
+// scaled, then welded by the model builder.  Refused with an error rather than approximated: lath groups with their
+// own material / section / beam (hybrid weaves) and the 'region' anchor selector.
+//
+//   const S = require('./scene_mi355x.js');
+//   const m = S.loadScene('examples/scenes/dome_basic.json');        // a W.Model, ready for new W.GpuSolver(...)
+
+const fs = require('fs');
+const path = require('path');
+const { Model } = require('./solver_mi355x.js');
+
+const SCENE_VERSION = 1;
+const LENGTH_TO_M = { mm: 1e-3, cm: 1e-2, m: 1.0 };
+const FORCE_TO_N = { N: 1.0, kN: 1e3 };
+const FLOOR_MU = 0.4;
+const MATERIAL_MM_N = { E90: 800.0, fc90: 8.0, H: 30.0 };
+const SELECTOR = { frame: 0, all_start: 1, all_end: 2, all_ends: 3, strand: 4, boundary: 5 };
+
+/** triangle soup { V, F } of an STL file's bytes (Buffer / Uint8Array) */
+function parseSTL(buf) {
+  const u8 = buf instanceof Uint8Array ? buf : new Uint8Array(buf);
+  const dv = new DataView(u8.buffer, u8.byteOffset, u8.byteLength);
+  if (u8.byteLength >= 84) {
+    const t = dv.getUint32(80, true);
+    if (84 + 50 * t === u8.byteLength) {
+      const V = new Float64Array(9 * t), F = new Int32Array(3 * t);
+      for (let q = 0; q < t; q++) {
+        const o = 84 + 50 * q + 12;
+        for (let k = 0; k < 9; k++) V[9 * q + k] = dv.getFloat32(o + 4 * k, true);
+      }
+      for (let i = 0; i < 3 * t; i++) F[i] = i;
+      return { V, F };
+    }
+  }
+  const text = Buffer.from(u8).toString('utf8');
+  const re = /vertex\s+([-+0-9.eE]+)\s+([-+0-9.eE]+)\s+([-+0-9.eE]+)/g;
+  const vals = [];
+  let m;
+  while ((m = re.exec(text))) vals.push(+m[1], +m[2], +m[3]);
+  const t = Math.floor(vals.length / 9);
+  if (!t) throw new Error('no triangles found (binary or ASCII STL expected)');
+  const F = new Int32Array(3 * t);
+  for (let i = 0; i < 3 * t; i++) F[i] = i;
+  return { V: Float64Array.from(vals.slice(0, 9 * t)), F };
+}
+
+/** vertices and fan-triangulated faces of an OBJ file's text */
+function parseOBJ(text) {
+  const V = [], F = [];
+  for (const raw of text.split(/\r?\n/)) {
+    const line = raw.trim();
+    if (line.startsWith('v ')) { const p = line.split(/\s+/); V.push(+p[1], +p[2], +p[3]); }
+    else if (line.startsWith('f ')) {
+      const nv = V.length / 3;
+      const idx = line.split(/\s+/).slice(1).map((tok) => { const i = parseInt(tok.split('/')[0], 10); return i < 0 ? nv + i : i - 1; });
+      for (let k = 1; k + 1 < idx.length; k++) F.push(idx[0], idx[k], idx[k + 1]);
+    }
+  }
+  if (!F.length) throw new Error('no faces in the OBJ file');
+  return { V: Float64Array.from(V), F: Int32Array.from(F) };
+}
+
+/** { V, F } of a mesh file's bytes, scaled before welding */
+function readMesh(bytes, name, scale = 1.0) {
+  const r = String(name).toLowerCase().endsWith('.obj') ? parseOBJ(Buffer.from(bytes).toString('utf8')) : parseSTL(bytes);
+  for (let i = 0; i < r.V.length; i++) r.V[i] *= scale;
+  return r;
+}
+
+const has = (o, k) => o != null && Object.prototype.hasOwnProperty.call(o, k) && o[k] !== null && o[k] !== undefined;
+const flag = (v) => (v ? 1 : 0);
+
+function sameGroup(d, g) {
+  for (const key of ['material', 'section', 'beam']) {
+    const a = d[key] || {}, b = g[key] || {};
+    for (const k of Object.keys(a)) if (has(b, k) && b[k] !== a[k]) return false;
+  }
+  return true;
+}
+
+/** { scene, anchors, loads, flips, mesh } of a scene.json object: the scene as the addon's builder reads it */
+function sceneFromJSON(d) {
+  d = d || {};
+  const v = has(d, 'version') ? d.version : SCENE_VERSION;
+  if (v > SCENE_VERSION) throw new Error(`scene version ${v} newer than supported ${SCENE_VERSION}`);
+  const units = d.units || {};
+  const lu = units.length || 'mm', fu = units.force || 'N';
+  if (!(lu in LENGTH_TO_M) || !(fu in FORCE_TO_N)) throw new Error(`unknown units ${lu}, ${fu}`);
+  const L = LENGTH_TO_M[lu], Fn = FORCE_TO_N[fu];
+  const h = d.hybrid || {};
+  if (h.enabled) for (const g of h.groups || []) {
+    if (!sameGroup(d, g)) throw new Error('lath groups with their own material, section or beam (hybrid weaves) are not modelled by the builder');
+  }
+  const sc = { length_to_m: L, force_to_n: Fn };
+  const mat = d.material || {};
+  for (const k of ['E', 'G', 'density', 'E90', 'fc90', 'H']) if (has(mat, k)) sc[k] = +mat[k];
+  const stress = Fn / (L * L) / 1e6;
+  for (const k of Object.keys(MATERIAL_MM_N)) if (!(k in mat)) sc[k] = MATERIAL_MM_N[k] / stress;
+  const sec = d.section || {};
+  for (const k of ['width', 'height']) if (has(sec, k)) sc[k] = +sec[k];
+  for (const k of ['Ix', 'Iy', 'J']) sc[k] = has(sec, k) ? +sec[k] : 0;
+  const beam = d.beam || {};
+  sc.beam_advanced = flag(beam.mode === 'advanced');
+  const bmap = { activation: 'activation', stiffness: 'stiffness', activation_flat: 'act_flat', activation_edge: 'act_edge',
+    activation_twist: 'act_twist', stiffness_axial: 'k_axial', stiffness_flat: 'k_flat', stiffness_edge: 'k_edge',
+    stiffness_twist: 'k_twist', stiffness_shear: 'k_shear' };
+  for (const k of Object.keys(bmap)) if (has(beam, k)) sc[bmap[k]] = +beam[k];
+  const sol = d.solver || {};
+  for (const k of ['mass_scale', 'damping', 'tol', 'warmup_iter', 'max_iter']) if (has(sol, k)) sc[k] = +sol[k];
+  if (has(sol, 'kinetic_damping')) sc.kinetic_damping = flag(sol.kinetic_damping);
+  sc.prestress_asbuilt = flag(sol.prestress === 'asbuilt');
+  const con = d.contact || {};
+  for (const k of ['mu', 'strength', 'stick_strength', 'torsion_factor', 'epsilon', 'slip_rate', 'rot_factor', 'dent_rate',
+    'edge_threshold', 'ramp_iter', 'al_every']) if (has(con, k)) sc[k] = +con[k];
+  for (const k of ['rot_friction', 'seat_gap', 'augmented']) if (has(con, k)) sc[k] = flag(con[k]);
+  if (has(con, 'seat_law')) sc.seat_law = flag(con.seat_law !== 'off');
+  if (has(con, 'dent_after_iter')) sc.dent_after = +con.dent_after_iter;
+  sc.locked = flag(con.locked);
+  const fl = d.floor || {};
+  const fz = has(fl, 'z') ? +fl.z : null;
+  const fmu = has(fl, 'mu') && +fl.mu >= 0 ? +fl.mu : FLOOR_MU;
+  sc.floor_on = flag(fl.enabled && fz !== null && Number.isFinite(fz));
+  sc.floor_z = fz !== null && Number.isFinite(fz) ? fz : 0;
+  sc.floor_mu = fmu;
+  const wv = d.weave || {};
+  if (has(wv, 'subdivisions')) sc.subdivisions = +wv.subdivisions;
+  if (has(wv, 'overhang')) sc.overhang = +wv.overhang;
+  sc.offset_half_height = flag((wv.offset_mode || 'half_height') === 'half_height');
+  if (has(wv, 'seed_strand_phase')) sc.seed_strand_phase = +wv.seed_strand_phase;
+  const flips = (wv.flip_knots || []).map((f) => [f[0] | 0, f[1] | 0]);
+  const lo = d.loads || {};
+  const gr = lo.gravity || {};
+  sc.gravity = flag(has(gr, 'enabled') ? gr.enabled : true);
+  sc.gravity_dir = (gr.direction || [0, 0, -1]).map(Number);
+  const udl = lo.udl || {};
+  sc.udl = !udl.enabled ? 0 : ((udl.per || 'length') === 'length' ? 1 : 2);
+  sc.udl_q = (udl.q || [0, 0, 0]).map(Number);
+  const loads = (lo.point || []).map((p) => ({ s: p.frame[0] | 0, k: p.frame[1] | 0, F: p.F.map(Number), local: (p.space || 'world') === 'local' ? 1 : 0 }));
+  const anchors = [];
+  for (const a of d.anchors || []) {
+    const sel = a.select || {}, t = sel.type || 'frame', fix = (a.fix || [1, 1, 1, 0, 0, 0]).map((f) => (f ? 1 : 0));
+    if (t === 'frame') anchors.push({ type: SELECTOR.frame, s: sel.id[0] | 0, k: sel.id[1] | 0, fix });
+    else if (t === 'knots') for (const i of sel.ids || []) anchors.push({ type: SELECTOR.frame, s: i[0] | 0, k: i[1] | 0, fix });
+    else if (t === 'strand') anchors.push({ type: SELECTOR.strand, s: sel.strand | 0, k: 0, fix });
+    else if (t in SELECTOR) anchors.push({ type: SELECTOR[t], s: 0, k: 0, fix });
+    else if (t === 'region') throw new Error("the 'region' anchor selector (graph distance on the mesh) is not modelled");
+    else throw new Error(`unknown selector type ${t}`);
+  }
+  const mesh = d.mesh || {};
+  return { scene: sc, anchors, loads, flips, mesh: { path: mesh.path || '', scale: has(mesh, 'scale') ? +mesh.scale : 1.0 } };
+}
+
+/** the host model of a scene.json object and its mesh file's bytes */
+function modelFromScene(d, meshBytes, meshName) {
+  const s = sceneFromJSON(d);
+  const { V, F } = readMesh(meshBytes, meshName || s.mesh.path, s.mesh.scale);
+  return Model.fromMesh(V, F, s.scene, { anchors: s.anchors, loads: s.loads, flips: s.flips, weld: true });
+}
+
+/** the host model of a scene.json file; the mesh path inside it is relative to the file's own folder */
+function loadScene(file) {
+  const d = JSON.parse(fs.readFileSync(file, 'utf8'));
+  const s = sceneFromJSON(d);
+  if (!s.mesh.path) throw new Error('the scene names no mesh');
+  const meshPath = path.resolve(path.dirname(file), s.mesh.path);
+  return modelFromScene(d, fs.readFileSync(meshPath), meshPath);
+}
+
+module.exports = { parseSTL, parseOBJ, readMesh, sceneFromJSON, modelFromScene, loadScene };

@@ -42,6 +42,13 @@ class Model {
     this.st.history = { kinetic: [], strain: [], bending: [], torsion: [], stretch: [], shear: [], contact: [], residual: [] };
   }
   static hexDome(spec, scene = {}) { return new Model(native().modelHexDome(spec, scene)); }
+  /** a model on a triangle mesh (buildTopology … CpuSolver._buildGoals): V Float64Array [3nV], F Int32Array [3nF],
+   *  scene fields as the addon names them, anchors [{ type, s, k, fix }], point loads [{ s, k, F, local }], flipped
+   *  knots [[s, k], ...]; duplicate vertices are welded unless weld is false (scene_mi355x.js reads scene.json) */
+  static fromMesh(V, F, scene = {}, opts = {}) {
+    return new Model(native().modelFromMesh(Float64Array.from(V), Int32Array.from(F), scene, opts.anchors || [],
+      opts.loads || [], opts.flips || [], opts.weld !== false));
+  }
   /** md.protocol of the reference (solver_gpu.js:256-318, calibrate.js:345-365): { mode: 'force' | 'displacement' |
    *  null, probe, axis, sched, settle, avg }; the schedule is used as given (flatten the load cycles and drop
    *  non-positive displacements first, as CalibrationSet's jobs do) */
