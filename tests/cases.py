@@ -1,6 +1,7 @@
 """Scene fixtures of the reference's test suite (web/tests/cases.mjs:10-66, rod_bench.mjs, rod_check.mjs) and the
 benchmark domes, built with the host model builder."""
 import importlib
+import math
 
 W = importlib.import_module("weave-sim_amd")
 A = W._abi
@@ -20,6 +21,18 @@ def cantilever(nseg=10, L=1000.0, F=10.0, edge=False, w=60.0, h=30.0, tol=1e-10,
     m = W.Model.from_polylines([straight(L, nseg)], scene=sc, anchors=[(A.WSM_FRAME, 0, 0, ALL6)])
     m.add_load(nseg, [0.0, -F, 0.0] if edge else [0.0, 0.0, -F])
     return m
+
+
+def torsion(nseg=10, L=1000.0, tol=1e-10):
+    """a straight lath whose section normals turn by 90° along it, clamped at its start and held in place and about
+    its axis at its end (cases.mjs:38-42)"""
+    normals = []
+    for i in range(nseg + 1):
+        a = math.pi / 2 * i / nseg
+        normals.append([0.0, -math.sin(a), math.cos(a)])
+    sc = W.Scene(gravity=0, contacts=0, tol=tol)
+    return W.Model.from_polylines([straight(L, nseg)], scene=sc, normals=[normals],
+                                  anchors=[(A.WSM_FRAME, 0, 0, ALL6), (A.WSM_FRAME, 0, 1, [1, 1, 1, 0, 0, 1])])
 
 
 def crossing(load=-50.0, pull=0.0, mu=0.35, free_b=True, nseg=10, L=1000.0, h=30.0, locked=False, **scene):

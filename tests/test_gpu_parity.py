@@ -44,6 +44,19 @@ def test_cantilever_trajectory(W, gpu_ctx, precision, tol):
         assert c["iter"] == o.iteration
 
 
+@pytest.mark.parametrize("case", ["cantilever_edge", "torsion"])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 1e-5)])
+def test_edgewise_and_torsion_trajectories(W, gpu_ctx, case, precision, tol):
+    """the reference's other two beam checks (gpu_parity.html, cases.mjs:32-42): the cantilever loaded edgewise and
+    the lath with a quarter turn of built-in twist, positions and frames against the oracle along the relaxation"""
+    m = cases.cantilever(nseg=10, edge=True) if case == "cantilever_edge" else cases.torsion(nseg=10)
+    for n, a, b, o, c in run_pair(W, m, [1, 10, 100, 1000, 3000], precision):
+        assert np.abs(a["X"] - b["X"]).max() <= tol * 1000.0, (case, n, np.abs(a["X"] - b["X"]).max())
+        sgn = np.sign(np.sum(a["Q"].reshape(-1, 4) * b["Q"].reshape(-1, 4), axis=1))[:, None]    # q and −q: one frame
+        assert np.abs(a["Q"].reshape(-1, 4) - sgn * b["Q"].reshape(-1, 4)).max() <= 10 * tol, (case, n)
+        assert c["iter"] == o.iteration
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_cantilever_converged_matches_oracle_and_timoshenko(W, gpu_ctx, precision):
     """both run to the residual stop; tip deflections agree to 1e-4·span and sit 0.50 % from Timoshenko"""

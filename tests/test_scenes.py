@@ -101,6 +101,13 @@ def test_hybrid_groups_that_differ_are_refused_and_equal_ones_accepted():
         SC.scene_from_dict(d)
     with pytest.raises(ValueError):
         SC.scene_from_dict({"version": 2})
+    # a group that leaves a key out takes its default, not the scene's value; an unassigned group is not used
+    d = scene_json("v5dome_cm")
+    del d["hybrid"]["groups"][0]["section"]["width"]                  # default 60 ≠ the scene's 2.0
+    with pytest.raises(NotImplementedError):
+        SC.scene_from_dict(d)
+    d["hybrid"]["assign"] = {}
+    SC.scene_from_dict(d)
 
 
 def test_binary_and_ascii_stl_agree():

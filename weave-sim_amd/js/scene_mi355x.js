@@ -74,12 +74,33 @@ function readMesh(bytes, name, scale = 1.0) {
 const has = (o, k) => o != null && Object.prototype.hasOwnProperty.call(o, k) && o[k] !== null && o[k] !== undefined;
 const flag = (v) => (v ? 1 : 0);
 
-function sameGroup(d, g) {
-  for (const key of ['material', 'section', 'beam']) {
-    const a = d[key] || {}, b = g[key] || {};
-    for (const k of Object.keys(a)) if (has(b, k) && b[k] !== a[k]) return false;
+// params.js:6-19: what a section of a scene or of a lath group holds when the file leaves a key out
+const DEFAULTS = {
+  material: { E: 10000.0, G: 625.0, density: 700.0, E90: 800.0, fc90: 8.0, H: 30.0 },
+  section: { width: 60.0, height: 30.0, Ix: null, Iy: null, J: null },
+  beam: { mode: 'simple', activation: 1.0, stiffness: 1.0, activation_flat: 1.0, activation_edge: 1.0, activation_twist: 1.0,
+    stiffness_flat: 1.0, stiffness_edge: 1.0, stiffness_twist: 1.0, stiffness_axial: 1.0, stiffness_shear: 1.0 },
+};
+
+/** a lath group equal to the scene's own material / section / beam: every key at its default when left out, and the
+ *  scene's own E90 / fc90 / H defaults in its stress unit (a group's are not rescaled, scene.js:144-155) */
+function sameGroup(d, g, stress) {
+  for (const key of Object.keys(DEFAULTS)) {
+    const a = Object.assign({}, DEFAULTS[key], d[key] || {}), b = Object.assign({}, DEFAULTS[key], g[key] || {});
+    if (key === 'material') for (const k of Object.keys(MATERIAL_MM_N)) if (!(k in (d[key] || {}))) a[k] = MATERIAL_MM_N[k] / stress;
+    const keys = new Set(Object.keys(a).concat(Object.keys(b)));
+    for (const k of keys) if (a[k] !== b[k] && !(a[k] == null && b[k] == null)) return false;
   }
   return true;
+}
+
+/** no strand or piece is assigned to a lath group that differs from the scene's own (scene.js:53-72) */
+function hybridIsOneGroup(d, stress) {
+  const h = d.hybrid || {};
+  if (!h.enabled) return true;
+  const groups = h.groups || [];
+  const used = Object.values(h.assign || {}).concat(Object.values(h.pieces || {})).map((v) => v | 0);
+  return used.every((g) => g < 2 || g > 1 + groups.length || sameGroup(d, groups[g - 2], stress));
 }
 
 /** { scene, anchors, loads, flips, mesh } of a scene.json object: the scene as the addon's builder reads it */
@@ -91,10 +112,7 @@ function sceneFromJSON(d) {
   const lu = units.length || 'mm', fu = units.force || 'N';
   if (!(lu in LENGTH_TO_M) || !(fu in FORCE_TO_N)) throw new Error(`unknown units ${lu}, ${fu}`);
   const L = LENGTH_TO_M[lu], Fn = FORCE_TO_N[fu];
-  const h = d.hybrid || {};
-  if (h.enabled) for (const g of h.groups || []) {
-    if (!sameGroup(d, g)) throw new Error('lath groups with their own material, section or beam (hybrid weaves) are not modelled by the builder');
-  }
+  if (!hybridIsOneGroup(d, Fn / (L * L) / 1e6)) throw new Error('lath groups with their own material, section or beam (hybrid weaves) are not modelled by the builder');
   const sc = { length_to_m: L, force_to_n: Fn };
   const mat = d.material || {};
   for (const k of ['E', 'G', 'density', 'E90', 'fc90', 'H']) if (has(mat, k)) sc[k] = +mat[k];

@@ -8,7 +8,8 @@ builder.  The instance is then built by wsm_from_mesh — the reference's buildT
 / CpuSolver._buildGoals pipeline — exactly as for a mesh given directly.
 
 Not modelled by the builder, and refused with an error rather than approximated: lath groups that differ from the
-scene's own material / section / beam (hybrid weaves; a hybrid block whose groups all equal the first is accepted),
+scene's own material / section / beam (hybrid weaves; strands assigned to a group equal to the scene's own are
+accepted),
 and the 'region' anchor selector (graph distance on the mesh, model.js:127-145).
 """
 import json
@@ -80,14 +81,40 @@ def _flag(v):
     return 1 if v else 0
 
 
-def _groups_match(d, group):
-    """a hybrid group equal to the scene's own material / section / beam (so the weave is effectively one group)"""
-    for key in ("material", "section", "beam"):
-        a, b = d.get(key) or {}, group.get(key) or {}
-        for k in set(a) | set(b):
-            if k in a and k in b and a[k] != b[k]:
-                return False
+# params.js:6-19: what a section of a scene or of a lath group holds when the file leaves a key out
+_DEFAULTS = {
+    "material": {"E": 10000.0, "G": 625.0, "density": 700.0, "E90": 800.0, "fc90": 8.0, "H": 30.0},
+    "section": {"width": 60.0, "height": 30.0, "Ix": None, "Iy": None, "J": None},
+    "beam": {"mode": "simple", "activation": 1.0, "stiffness": 1.0, "activation_flat": 1.0, "activation_edge": 1.0,
+             "activation_twist": 1.0, "stiffness_flat": 1.0, "stiffness_edge": 1.0, "stiffness_twist": 1.0,
+             "stiffness_axial": 1.0, "stiffness_shear": 1.0},
+}
+
+
+def _groups_match(d, group, stress):
+    """a lath group equal to the scene's own material / section / beam: every key at its default when left out, and
+    the scene's own E90 / fc90 / H defaults in its stress unit (a group's are not rescaled, scene.js:144-155)"""
+    for key, dflt in _DEFAULTS.items():
+        a = dict(dflt, **(d.get(key) or {}))
+        b = dict(dflt, **(group.get(key) or {}))
+        if key == "material":
+            for k, v0 in _MATERIAL_MM_N.items():
+                if k not in (d.get(key) or {}):
+                    a[k] = v0 / stress
+        if a != b:
+            return False
     return True
+
+
+def _hybrid_is_one_group(d, stress):
+    """no strand or piece of a discretisation is assigned to a lath group that differs from the scene's own
+    (scene.js:53-72: group 1 is the scene's material / section / beam, assignments name groups 2 …)"""
+    h = d.get("hybrid") or {}
+    if not h.get("enabled"):
+        return True
+    groups = h.get("groups") or []
+    used = {int(v) for v in list((h.get("assign") or {}).values()) + list((h.get("pieces") or {}).values())}
+    return all(_groups_match(d, groups[g - 2], stress) for g in used if 2 <= g <= 1 + len(groups))
 
 
 def scene_from_dict(d):
@@ -105,12 +132,9 @@ def scene_from_dict(d):
     c = sc.c
     c.length_to_m, c.force_to_n = L, Fn
 
-    h = d.get("hybrid") or {}
-    if h.get("enabled"):
-        for g in h.get("groups") or []:
-            if not _groups_match(d, g):
-                raise NotImplementedError("lath groups with their own material, section or beam (hybrid weaves) are "
-                                          "not modelled by the builder")
+    if not _hybrid_is_one_group(d, Fn / (L * L) / 1e6):
+        raise NotImplementedError("lath groups with their own material, section or beam (hybrid weaves) are not "
+                                  "modelled by the builder")
 
     mat = d.get("material") or {}
     g = c.grp
