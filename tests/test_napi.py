@@ -169,3 +169,33 @@ console.log(JSON.stringify(res));""")
         assert np.array_equal(np.array(js["free"], dtype=np.uint8), py.arr("node_free"))
         for k, v in js["info"]["params"].items():
             assert v == getattr(py.md.par, k), (name, k, v, getattr(py.md.par, k))
+
+
+@pytest.mark.gpu
+def test_js_scene_on_the_device_matches_python_path(W, gpu_ctx):
+    """the reference's tree scene (scene.json + STL, augmented-Lagrangian contact, UDL per length) read by the JS
+    reader and relaxed 800 iterations by the JS GpuSolver equals the Python reader + Python path, bitwise"""
+    import importlib
+    SC = importlib.import_module("weave-sim_amd.scene")
+    path = os.path.join(ROOT, "tests", "golden", "scenes", "tree_cm.json")
+    scene_js = os.path.join(ROOT, "weave-sim_amd", "js", "scene_mi355x.js")
+    out = run_node(f"""
+(async () => {{
+  const S = require({json.dumps(scene_js)});
+  const dev = await W.getDevice();
+  if (!dev) throw new Error('no device');
+  const m = S.loadScene({json.dumps(path)});
+  const g = new W.GpuSolver(dev, [m], {{ perSubmit: 64, tile: 64 }});
+  const c = await g.step(800);
+  await g.download(0);
+  g.dispose();
+  console.log(JSON.stringify({{ctl: c[0], X: Array.from(m.st.X), lam: Array.from(m.st.lam)}}));
+}})().catch((e) => {{ console.error(e); process.exit(1); }});""")
+    d = json.loads(out)
+    m = SC.load_scene(path)
+    g = W.GpuSolver([m], precision="fp32", tile=64)
+    c = g.step(800)[0]
+    s = g.state(0)
+    g.close()
+    assert d["ctl"]["iter"] == c["iter"] == 800 and d["ctl"]["ke"] == c["ke"]
+    assert np.array_equal(np.array(d["X"]), s["X"]) and np.array_equal(np.array(d["lam"]), s["lam"])
