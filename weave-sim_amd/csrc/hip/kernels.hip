@@ -434,6 +434,15 @@ __global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(2
   const double uA = gl(I.u)[2 * c], uB = gl(I.u)[2 * c + 1];
   const int noffA = gl(I.k_noff)[2 * c], nsegA = gl(I.k_nseg)[2 * c];
   if (!live) return;                               // uniform over the workgroup (one instance per block)
+  // every other per-crossing value the law reads (sections, classification constants, the previous normal, the slip
+  // total, the side records' slots), issued here with the first loads: read where it is used, each would cost a
+  // memory latency of its own in the middle of the arithmetic
+  const Real wk0 = gl(I.wk)[2 * c], wk1 = gl(I.wk)[2 * c + 1], hk0 = gl(I.hk)[2 * c], hk1 = gl(I.hk)[2 * c + 1];
+  const Real Hs0 = gl(I.Hs)[2 * c], Hs1 = gl(I.Hs)[2 * c + 1], fc0 = gl(I.fc90)[2 * c], fc1 = gl(I.fc90)[2 * c + 1];
+  const Real E900 = gl(I.E90)[2 * c], E901 = gl(I.E90)[2 * c + 1], kwc = gl(I.kw)[c], khc = gl(I.kh)[c];
+  const Real np0 = gl(I.nprev)[3 * c], np1 = gl(I.nprev)[3 * c + 1], np2 = gl(I.nprev)[3 * c + 2];
+  const double slip0 = gl(I.slip)[c];
+  const int nsegB = gl(I.k_nseg)[2 * c + 1], spos0 = gl(I.side_pos)[2 * c], spos1 = gl(I.side_pos)[2 * c + 1];
   const double uOwn = side == 0 ? uA : uB;
   const int o = noffA + int(fmin(fmax(floor(uA), 0.0), double(max(nsegA - 1, 0))));
   const auto Xc = gl(buf(I.X, cur));
@@ -456,7 +465,7 @@ __global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(2
     tA[k] = side == 0 ? E.tg[k] : oT[k]; tB[k] = side == 0 ? oT[k] : E.tg[k];
   }
   for (int k = 0; k < 6; k++) { AR[k] = side == 0 ? E.R[k] : oR[k]; BR[k] = side == 0 ? oR[k] : E.R[k]; }
-  const G wA = gl(I.wk)[2 * c], wB = gl(I.wk)[2 * c + 1], hA = gl(I.hk)[2 * c], hB = gl(I.hk)[2 * c + 1];
+  const G wA = wk0, wB = wk1, hA = hk0, hB = hk1;
   // ---- AL multiplier update deferred from the end of the previous iteration (updateMultipliers, :499-506)
   G lam = aug ? G(gl(I.lam)[c]) : G(0);
   G penacc = aug ? G(gl(I.penacc)[c]) : G(0);
@@ -464,7 +473,7 @@ __global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(2
     lam = al_updated(I, c, lam, penacc, pen_n);
     penacc = G(0);
   }
-  G np[3] = {gl(I.nprev)[3 * c], gl(I.nprev)[3 * c + 1], gl(I.nprev)[3 * c + 2]};
+  G np[3] = {np0, np1, np2};
   G n[3] = {tA[1] * tB[2] - tA[2] * tB[1], tA[2] * tB[0] - tA[0] * tB[2], tA[0] * tB[1] - tA[1] * tB[0]};
   G nn = fsqrt_(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
   if (nn < G(1e-6)) { n[0] = np[0]; n[1] = np[1]; n[2] = np[2]; }
@@ -564,18 +573,18 @@ __global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(2
   int faceSide = nEdge == 1 ? (edge[1] ? 0 : 1) : -1;
   H Fa = H(fabs(Fn));
   H Aa = cface[0] * cface[1] / sinT, pres = Fa / dmax(Aa, H(1e-12));
-  H ut = pres / dmax(H(dmin(gl(I.Hs)[2 * c], gl(I.Hs)[2 * c + 1])), H(1e-12));
+  H ut = pres / dmax(H(dmin(Hs0, Hs1)), H(1e-12));
   if (nEdge == 1) {
     int fs = faceSide, es = 1 - fs;
-    H ell = cface[fs] / sinT, kap = H(gl(I.E90)[2 * c + fs]) / dmax(z0[fs], H(1e-12));
+    H ell = cface[fs] / sinT, kap = H(fs ? E901 : E900) / dmax(z0[fs], H(1e-12));
     H phiE = dmax(phiv[es], H(1e-4));
     H bw = fsqrt_(H(2) * Fa / dmax(ell * kap * phiE, H(1e-12)));
     bool full = bw >= cface[es];
     H Afe = full ? cface[0] * cface[1] / sinT : bw * ell;
     H pfe = full ? Fa / dmax(Afe, H(1e-12)) : kap * phiE * bw;
-    Aa = Afe; pres = pfe; ut = pfe / dmax(H(gl(I.fc90)[2 * c + es]), H(1e-12));
+    Aa = Afe; pres = pfe; ut = pfe / dmax(H(es ? fc1 : fc0), H(1e-12));
   } else if (nEdge == 2) {
-    H fc = H(dmin(gl(I.fc90)[2 * c], gl(I.fc90)[2 * c + 1]));
+    H fc = H(dmin(fc0, fc1));
     Aa = Fa / dmax(fc, H(1e-12)); pres = Fa > H(0) ? fc : H(0); ut = Fa > H(0) ? H(1) : H(0);
   }
   // ---- W3 seat update (_seatUpdate, :317-344)
@@ -599,14 +608,14 @@ __global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(2
       H sp0 = fsin_(phiv[0]) * fcos_(phiv[0]), sp1 = fsin_(phiv[1]) * fcos_(phiv[1]);
       H sT = fsin_(theta);
       if (nEdge == 2 && act && !locked) {
-        H fc = H(dmin(gl(I.fc90)[2 * c], gl(I.fc90)[2 * c + 1]));
+        H fc = H(dmin(fc0, fc1));
         H dEst = fsqrt_(Fa / dmax(fc, H(1e-12)) * sT * sp0 * sp1);
         for (int j = 0; j < 2; j++) { H cu = dent[j]; dent[j] = dmax(cu, dmin(cu + H(I.dent_rate) * dEst, dEst)); }
       }
       if (nEdge == 1 && act && !locked) {
         int fs = faceSide, es = 1 - fs;
         H ell = cface[fs] / dmax(sT, H(1e-6));
-        H dE = Fa / dmax(H(gl(I.fc90)[2 * c + es]) * ell, H(1e-12)) * (es == 0 ? sp0 : sp1);
+        H dE = Fa / dmax(H(es ? fc1 : fc0) * ell, H(1e-12)) * (es == 0 ? sp0 : sp1);
         dente = dmax(dente, dE);
       }
     }
@@ -647,10 +656,10 @@ __global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(2
   }
   // ---- righting stiffness (addTwistStiffness, :170-176) on the own side's segment after the slip
   G tfk = dmax(G(I.torsion_factor), G(1e-3));
-  G kth = tfk * ((Fn + kn * G(0.25) * G(gl(I.kh)[c])) * (G(0.5) * G(gl(I.kw)[c])) / G(SUPPORT_SMOOTH) + kt * (G(0.5) * G(gl(I.kh)[c])) * (G(0.5) * G(gl(I.kh)[c])));
+  G kth = tfk * ((Fn + kn * G(0.25) * G(khc)) * (G(0.5) * G(kwc)) / G(SUPPORT_SMOOTH) + kt * (G(0.5) * G(khc)) * (G(0.5) * G(khc)));
   // a side's parameter after the slip and its record
   auto finish_side = [&](int sd, const SideEv<G>& Es, double uOld) {
-    const int nsS = gl(I.k_nseg)[2 * c + sd];
+    const int nsS = sd ? nsegB : nsegA;
     double uNew = uOld;
     if (moved) {
       const double iL = rcp_(double(dmax(Es.L, G(1e-12))));
@@ -666,7 +675,7 @@ __global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(2
       rec.F[k] = Real(sd == 0 ? FA[k] : FB[k]);
       rec.T[k] = Real(sd == 0 ? TA[k] : TB[k]);
     }
-    side_store(rec, gl(I.side) + gl(I.side_pos)[2 * c + sd]);
+    side_store(rec, gl(I.side) + (sd ? spos1 : spos0));
   };
   if (valid) {
     finish_side(side, E, uOwn);
@@ -676,7 +685,7 @@ __global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(2
     gl(I.nprev)[3 * c] = Real(n[0]); gl(I.nprev)[3 * c + 1] = Real(n[1]); gl(I.nprev)[3 * c + 2] = Real(n[2]);
     if (slip) {
       G q0 = da * tA[0] + db * tB[0], q1 = da * tA[1] + db * tB[1], q2 = da * tA[2] + db * tB[2];
-      gl(I.slip)[c] += double(fsqrt_(q0 * q0 + q1 * q1 + q2 * q2));
+      gl(I.slip)[c] = slip0 + double(fsqrt_(q0 * q0 + q1 * q1 + q2 * q2));
     }
     if (aug) { gl(I.lam)[c] = lam; gl(I.penacc)[c] = penacc; }
     gl(I.fn)[c] = Real(Fn);
@@ -858,14 +867,19 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   auto joint_ok = [&](int j, int& jj) { if (closed) { if (njs <= 0) { jj = 0; return false; } jj = wrap(j, njs); return true; } jj = j; return j >= 0 && j < njs; };
 
   // ---------------------------------------------------------------- issue every independent load first
+  // the tile's cnt + 2 ≤ TILE + 1 slots of positions and frames: slot tid, and slot TILE for lane 0 of a full tile,
+  // loaded into registers (written to LDS once the parameter and side-record loads below are out too)
   const auto Xc = gl(buf(I.X, cur)), Qc = gl(buf(I.Q, cur));
-  for (int h = tid; h < cnt + 2; h += TILE) {
-    int kk;
-    if (node_ok(k0 - 1 + h, kk)) { const int i = 3 * (noff + kk); L.Xx[h] = Xc[i]; L.Xy[h] = Xc[i + 1]; L.Xz[h] = Xc[i + 2]; }
-    else { L.Xx[h] = L.Xy[h] = L.Xz[h] = 0.0; }
-    int jj;
-    if (seg_ok(k0 - 1 + h, jj)) { const int j = 4 * (soff + jj); L.Qw[h] = Qc[j]; L.Qx[h] = Qc[j + 1]; L.Qy[h] = Qc[j + 2]; L.Qz[h] = Qc[j + 3]; }
-    else { L.Qw[h] = 1.0; L.Qx[h] = L.Qy[h] = L.Qz[h] = 0.0; }
+  double sx[2], sy[2], sz[2], sw[2], qx[2], qy[2], qz[2];
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int h = tid + r * TILE;
+    int kk, jj;
+    const bool inh = h < cnt + 2;
+    if (inh && node_ok(k0 - 1 + h, kk)) { const int i = 3 * (noff + kk); sx[r] = Xc[i]; sy[r] = Xc[i + 1]; sz[r] = Xc[i + 2]; }
+    else { sx[r] = sy[r] = sz[r] = 0.0; }
+    if (inh && seg_ok(k0 - 1 + h, jj)) { const int j = 4 * (soff + jj); sw[r] = Qc[j]; qx[r] = Qc[j + 1]; qy[r] = Qc[j + 2]; qz[r] = Qc[j + 3]; }
+    else { sw[r] = 1.0; qx[r] = qy[r] = qz[r] = 0.0; }
   }
   const int t = tid;
   const int k = k0 - 1 + t;                        // node / segment / joint of this slot (strand-local)
@@ -903,6 +917,15 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   SideRec<Real> r0;
   if (tid < ns) side_load(gl(I.side) + so + tid, r0);
   if (!live) return;                               // uniform over the workgroup (one instance per tile)
+  // the staged positions and frames go to LDS only now, after every load above is out (one memory latency for all)
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int h = tid + r * TILE;
+    if (h < cnt + 2) {
+      L.Xx[h] = sx[r]; L.Xy[h] = sy[r]; L.Xz[h] = sz[r];
+      L.Qw[h] = sw[r]; L.Qx[h] = qx[r]; L.Qy[h] = qy[r]; L.Qz[h] = qz[r];
+    }
+  }
   __syncthreads();
 
   // ---------------------------------------------------------------- segments and joints
