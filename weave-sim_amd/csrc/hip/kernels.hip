@@ -1317,12 +1317,15 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   // added below in the same order as the rest)
   const int cper = (I.ncblk + it.nchunks - 1) / it.nchunks;
   const int c0 = it.chunk * cper, c1 = min(I.ncblk, c0 + cper);
-  const bool cin = c0 + tid < c1;
+  const bool cin = c0 + tid < c1, cin1 = c0 + tid + CTRL_THREADS < c1;
   // unconditional loads at an index kept in range (the array holds at least one block's partials), so the
-  // descriptor's pointer is not read behind a branch
-  const int ci = cin ? c0 + tid : 0;
+  // descriptor's pointer is not read behind a branch; two per lane, so up to 512 crossing blocks cost no second
+  // memory latency
+  const int ci = cin ? c0 + tid : 0, ci1 = cin1 ? c0 + tid + CTRL_THREADS : 0;
   const double ce0v = gl(I.cpart)[ci], cm0v = gl(I.cpart)[I.ncblk + ci];
+  const double ce1v = gl(I.cpart)[ci1], cm1v = gl(I.cpart)[I.ncblk + ci1];
   const double ce0 = cin ? ce0v : 0.0, cm0 = cin ? cm0v : 0.0;
+  const bool fon = I.floor_on != 0;
   for (int w0 = p0; w0 < p1; w0 += CTRL_THREADS * CTRL_PER) {
     double x[CTRL_PER][SPART];
 #pragma unroll
@@ -1332,7 +1335,7 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
       const auto sp = gl(I.spart) + (in ? w : p0);
 #pragma unroll
       for (int q = 0; q < SPART - 1; q++) x[k][q] = in ? sp[size_t(q) * np] : 0.0;
-      x[k][SPART - 1] = (in && I.floor_on) ? sp[size_t(SPART - 1) * np] : 0.0;
+      x[k][SPART - 1] = (in && fon) ? sp[size_t(SPART - 1) * np] : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < CTRL_PER; k++) {
@@ -1344,7 +1347,8 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
     }
   }
   if (cin) { acc[6] += ce0; acc[10] = fmax(acc[10], cm0); }
-  for (int w = c0 + tid + CTRL_THREADS; w < c1; w += CTRL_THREADS) {
+  if (cin1) { acc[6] += ce1v; acc[10] = fmax(acc[10], cm1v); }
+  for (int w = c0 + tid + 2 * CTRL_THREADS; w < c1; w += CTRL_THREADS) {
     acc[6] += gl(I.cpart)[w];
     acc[10] = fmax(acc[10], gl(I.cpart)[I.ncblk + w]);
   }
