@@ -23,10 +23,10 @@
 
 namespace wsd {
 template <typename Real> __global__ void k_begin(Ctl*, const int32_t*, const int32_t*, int32_t, int32_t, int32_t);
-template <typename Real, int LANES> __global__ void k_cross(const Inst<Real>*, const CrossItem*, int32_t, const Ctl*);
-template <typename Real, int TILE, bool RB> __global__ void k_strand(const Inst<Real>*, const TileItem*, int32_t, const Ctl*);
+template <typename Real, int LANES, bool ONE> __global__ void k_cross(const Inst<Real>*, const CrossItem*, int32_t, const Ctl*, int32_t);
+template <typename Real, int TILE, bool RB, bool ONE> __global__ void k_strand(const Inst<Real>*, const TileItem*, int32_t, const Ctl*);
 template <typename Real, int TILE> __global__ void k_rigid(const Inst<Real>*, const TileItem*, int32_t, const Ctl*);
-template <typename Real> __global__ void k_ctrl(const Inst<Real>*, const CtrlItem*, int32_t, Ctl*);
+template <typename Real, bool ONE> __global__ void k_ctrl(const Inst<Real>*, const CtrlItem*, int32_t, Ctl*);
 __global__ void k_copy(const uint4*, uint4*, int64_t);
 template <typename Real> __global__ void k_al_preview(const Inst<Real>*, int32_t, double*);
 template <typename Real> __global__ void k_set_xq(const Inst<Real>*, int32_t, const double*, const double*);
@@ -550,29 +550,37 @@ struct Batch : BatchBase {
   }
 
 
+  // a batch of one instance launches the kernels' single-instance form (work item from the block index, descriptor
+  // and control record of slot 0 loaded beside it)
+  bool one_inst() const { return B == 1 && inst[0] != nullptr; }
   void launch_cross(hipStream_t s) {
     if (!ncross) return;
-    if (lanes == 1) hipLaunchKernelGGL((k_cross<Real, 1>), dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls);
-    else hipLaunchKernelGGL((k_cross<Real, 2>), dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls);
+    const bool one = one_inst();
+    const int32_t K = one ? inst[0]->K : 0;
+    if (lanes == 1) {
+      if (one) hipLaunchKernelGGL((k_cross<Real, 1, true>), dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls, K);
+      else hipLaunchKernelGGL((k_cross<Real, 1, false>), dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls, K);
+    } else {
+      if (one) hipLaunchKernelGGL((k_cross<Real, 2, true>), dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls, K);
+      else hipLaunchKernelGGL((k_cross<Real, 2, false>), dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls, K);
+    }
+  }
+  template <int T, bool RB> void strand_as(hipStream_t s) {
+    if (one_inst()) hipLaunchKernelGGL((k_strand<Real, T, RB, true>), dim3(ntiles), dim3(T), 0, s, d_inst, d_tiles, ntiles, d_ctls);
+    else hipLaunchKernelGGL((k_strand<Real, T, RB, false>), dim3(ntiles), dim3(T), 0, s, d_inst, d_tiles, ntiles, d_ctls);
   }
   void launch_strand(hipStream_t s) {
     switch (tile) {
-      case 64:
-        if (has_rigid) hipLaunchKernelGGL((k_strand<Real, 64, true>), dim3(ntiles), dim3(64), 0, s, d_inst, d_tiles, ntiles, d_ctls);
-        else hipLaunchKernelGGL((k_strand<Real, 64, false>), dim3(ntiles), dim3(64), 0, s, d_inst, d_tiles, ntiles, d_ctls);
-        break;
-      case 128:
-        if (has_rigid) hipLaunchKernelGGL((k_strand<Real, 128, true>), dim3(ntiles), dim3(128), 0, s, d_inst, d_tiles, ntiles, d_ctls);
-        else hipLaunchKernelGGL((k_strand<Real, 128, false>), dim3(ntiles), dim3(128), 0, s, d_inst, d_tiles, ntiles, d_ctls);
-        break;
-      default:
-        if (has_rigid) hipLaunchKernelGGL((k_strand<Real, 256, true>), dim3(ntiles), dim3(256), 0, s, d_inst, d_tiles, ntiles, d_ctls);
-        else hipLaunchKernelGGL((k_strand<Real, 256, false>), dim3(ntiles), dim3(256), 0, s, d_inst, d_tiles, ntiles, d_ctls);
-        break;
+      case 64: if (has_rigid) strand_as<64, true>(s); else strand_as<64, false>(s); break;
+      case 128: if (has_rigid) strand_as<128, true>(s); else strand_as<128, false>(s); break;
+      default: if (has_rigid) strand_as<256, true>(s); else strand_as<256, false>(s); break;
     }
   }
   void launch_ctrl(hipStream_t s) {
-    if (nctrl) hipLaunchKernelGGL(k_ctrl<Real>, dim3(nctrl), dim3(CTRL_THREADS), 0, s, d_inst, d_ctrl, nctrl, d_ctls);
+    if (nctrl) {
+      if (one_inst()) hipLaunchKernelGGL((k_ctrl<Real, true>), dim3(nctrl), dim3(CTRL_THREADS), 0, s, d_inst, d_ctrl, nctrl, d_ctls);
+      else hipLaunchKernelGGL((k_ctrl<Real, false>), dim3(nctrl), dim3(CTRL_THREADS), 0, s, d_inst, d_ctrl, nctrl, d_ctls);
+    }
     if (has_rigid && ntiles) {
       switch (tile) {
         case 64: hipLaunchKernelGGL((k_rigid<Real, 64>), dim3(ntiles), dim3(64), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;

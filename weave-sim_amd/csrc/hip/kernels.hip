@@ -397,9 +397,12 @@ __device__ __forceinline__ int xchg(int v) { return __shfl_xor(v, 1, 64); }
 // instructions per crossing, for batches with enough crossings to fill the chip.  Both layouts run the same
 // arithmetic on each side, so they give the same results.  Contact kinematics run in f64 in both forms (the penetration
 // D − g cancels mm-scale lengths); the classification, the lever arms and the W2 / W3 trigonometry run in Real.
-template <typename Real, int LANES>
+// ONE: the batch holds a single instance (slot 0) and block b takes its crossings [b·n, b·n + n), n the crossings per
+// block, K = oneK: the work item follows from the block index and the descriptor and control record from slot 0, so
+// they load together instead of one after the other
+template <typename Real, int LANES, bool ONE>
 __global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_cross(const Inst<Real>* __restrict__ insts, const CrossItem* __restrict__ items,
-                                                        int32_t nitems, const Ctl* __restrict__ ctls) {
+                                                        int32_t nitems, const Ctl* __restrict__ ctls, int32_t oneK) {
   static_assert(LANES == 1 || LANES == 2, "one or two lanes per crossing");
   using G = double;
   using H = Real;
@@ -408,9 +411,10 @@ __global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(2
   // the work list holds blocks of occupied slots only (it.inst >= 0); the liveness test waits until the first
   // crossing loads are out, so the item, the control record with the descriptor, and the crossing state cost three
   // memory latencies, not one per test
-  const CrossItem it = items[bid];
-  const Inst<Real>& I = insts[it.inst];
-  const Ctl* ctl = ctls + it.inst;
+  constexpr int PER_BLK = CROSS_THREADS / LANES;
+  const CrossItem it = ONE ? CrossItem{0, bid * PER_BLK, min(PER_BLK, oneK - bid * PER_BLK), bid} : items[bid];
+  const Inst<Real>& I = insts[ONE ? 0 : it.inst];
+  const Ctl* ctl = ctls + (ONE ? 0 : it.inst);
   const CtlHead ch = ctl_head(ctl);
   const int cur = ch.cur, iter = ch.iter;
   const bool live = (ch.done == 0) & (iter < ch.stop_at);
@@ -1287,7 +1291,8 @@ __device__ __forceinline__ void block_reduce(double (*red)[TILE / 64], double* v
 // ================================================================================================= k_ctrl
 // one workgroup per instance: fixed-order reduction of the tile and crossing-block partials, then kinetic damping,
 // the residual, the stop test and the history ring (solver_cpu.js:809-837; wgsl.js:990-1047)
-template <typename Real>
+// ONE: the batch holds a single instance (slot 0), workgroup b reduces its chunk b of gridDim.x chunks
+template <typename Real, bool ONE>
 __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restrict__ insts, const CtrlItem* __restrict__ items,
                                                       int32_t nitems, Ctl* __restrict__ ctls) {
   // accumulator: [0] Σm|v|² + ΣI3ω², [1..4] stretch / shear / bending / torsion energy, [5] ground energy,
@@ -1299,9 +1304,9 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   __shared__ double red_rb[RBP][CTRL_THREADS / 64];
   __shared__ int last;
   if (int(blockIdx.x) >= nitems) return;
-  const CtrlItem it = items[blockIdx.x];
-  const Inst<Real>& I = insts[it.inst];
-  Ctl* ctl = ctls + it.inst;                       // read beside the descriptor, not through it
+  const CtrlItem it = ONE ? CtrlItem{0, int(blockIdx.x), int(gridDim.x), 0} : items[blockIdx.x];
+  const Inst<Real>& I = insts[ONE ? 0 : it.inst];
+  Ctl* ctl = ctls + (ONE ? 0 : it.inst);           // read beside the descriptor, not through it
   const int tid = threadIdx.x;
   Ctl c = *ctl;
   // this workgroup's share of the strand-tile and crossing-block partials (value-major: coalesced loads).  A round
@@ -1524,7 +1529,8 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
 
 // RB: the batch holds an instance that only the ground holds (its tiles also leave the rigid-body sums); the other
 // batches run the form without them
-template <typename Real, int TILE, bool RB>
+// ONE: the batch holds a single instance (slot 0): its descriptor and control record load beside the tile item
+template <typename Real, int TILE, bool RB, bool ONE>
 __global__ __launch_bounds__(TILE) void k_strand(const Inst<Real>* __restrict__ insts, const TileItem* __restrict__ tiles, int32_t ntiles,
                                                  const Ctl* __restrict__ ctls) {
   __shared__ StrandLDS<Real, TILE> L;
@@ -1534,7 +1540,8 @@ __global__ __launch_bounds__(TILE) void k_strand(const Inst<Real>* __restrict__ 
   // loads are out (strand_tile), so the item, the control record, the descriptor and the state cost three memory
   // latencies, not one per test
   const TileItem tl = tiles[lt];
-  strand_tile<Real, TILE, false, RB>(insts[tl.inst], ctls + tl.inst, tl, L, OutPtrs{});
+  const int b = ONE ? 0 : tl.inst;
+  strand_tile<Real, TILE, false, RB>(insts[b], ctls + b, tl, L, OutPtrs{});
 }
 
 // The rigid step of a structure only the ground holds (CpuSolver.step, solver_cpu.js:791-805), after the control
@@ -1639,19 +1646,23 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
 
 // explicit instantiations
 #define WSD_INST(R, T) \
-  template __global__ void k_strand<R, T, false>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
-  template __global__ void k_strand<R, T, true>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
+  template __global__ void k_strand<R, T, false, false>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
+  template __global__ void k_strand<R, T, true, false>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
+  template __global__ void k_strand<R, T, false, true>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
+  template __global__ void k_strand<R, T, true, true>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
   template __global__ void k_rigid<R, T>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
   template __global__ void k_outputs<R, T>(const Inst<R>*, const TileItem*, int32_t, OutPtrs); \
   template __global__ void k_evaluate<R, T>(const Inst<R>*, const TileItem*, int32_t, double*, double*, double*, const int64_t*, const int64_t*);
 WSD_INST(float, 64) WSD_INST(float, 128) WSD_INST(float, 256)
 WSD_INST(double, 64) WSD_INST(double, 128) WSD_INST(double, 256)
-template __global__ void k_cross<float, 1>(const Inst<float>*, const CrossItem*, int32_t, const Ctl*);
-template __global__ void k_cross<float, 2>(const Inst<float>*, const CrossItem*, int32_t, const Ctl*);
-template __global__ void k_cross<double, 1>(const Inst<double>*, const CrossItem*, int32_t, const Ctl*);
-template __global__ void k_cross<double, 2>(const Inst<double>*, const CrossItem*, int32_t, const Ctl*);
-template __global__ void k_ctrl<float>(const Inst<float>*, const CtrlItem*, int32_t, Ctl*);
-template __global__ void k_ctrl<double>(const Inst<double>*, const CtrlItem*, int32_t, Ctl*);
+#define WSD_CROSS(R, N) \
+  template __global__ void k_cross<R, N, false>(const Inst<R>*, const CrossItem*, int32_t, const Ctl*, int32_t); \
+  template __global__ void k_cross<R, N, true>(const Inst<R>*, const CrossItem*, int32_t, const Ctl*, int32_t);
+WSD_CROSS(float, 1) WSD_CROSS(float, 2) WSD_CROSS(double, 1) WSD_CROSS(double, 2)
+template __global__ void k_ctrl<float, false>(const Inst<float>*, const CtrlItem*, int32_t, Ctl*);
+template __global__ void k_ctrl<float, true>(const Inst<float>*, const CtrlItem*, int32_t, Ctl*);
+template __global__ void k_ctrl<double, false>(const Inst<double>*, const CtrlItem*, int32_t, Ctl*);
+template __global__ void k_ctrl<double, true>(const Inst<double>*, const CtrlItem*, int32_t, Ctl*);
 template __global__ void k_set_xq<float>(const Inst<float>*, int32_t, const double*, const double*);
 template __global__ void k_set_xq<double>(const Inst<double>*, int32_t, const double*, const double*);
 template __global__ void k_al_preview<float>(const Inst<float>*, int32_t, double*);
