@@ -199,3 +199,33 @@ def test_js_scene_on_the_device_matches_python_path(W, gpu_ctx):
     g.close()
     assert d["ctl"]["iter"] == c["iter"] == 800 and d["ctl"]["ke"] == c["ke"]
     assert np.array_equal(np.array(d["X"]), s["X"]) and np.array_equal(np.array(d["lam"]), s["lam"])
+
+
+@pytest.mark.gpu
+def test_js_server_matches_python_path(W, gpu_ctx):
+    """three requests through the JS Server (two slots: the third waits for a free one), each relaxed 300 iterations,
+    equal the same models relaxed alone through the Python path, bitwise"""
+    out = run_node("""
+(async () => {
+  const dev = await W.getDevice();
+  if (!dev) throw new Error('no device');
+  const srv = new W.Server(dev, { chunk: 64, tile: 64, maxSlots: 2 });
+  for (const seed of [1, 2, 3]) {
+    srv.submit(W.Model.hexDome({ span: 1200, spacing: 300, segment: 50, contraction: 0.15, noise: 1, seed },
+                               { E: 10000, G: 600, density: 500, width: 40, height: 4, mu: 0.3, tol: 1e-6 }), 300);
+  }
+  const res = [];
+  for (let r = 0; r < 100 && res.length < 3; r++) { await srv.pump(); for (const x of srv.poll()) res.push(x); }
+  srv.dispose();
+  console.log(JSON.stringify(res.map((x) => ({ id: x.id, iterations: x.iterations, status: x.status, X: Array.from(x.model.st.X) }))));
+})().catch((e) => { console.error(e); process.exit(1); });""")
+    d = sorted(json.loads(out), key=lambda x: x["id"])
+    assert [x["id"] for x in d] == [1, 2, 3]
+    for seed, x in zip((1, 2, 3), d):
+        m = cases.config_small(seed=seed)
+        g = W.GpuSolver([m], precision="fp32", tile=64)
+        c = g.step(300)[0]
+        s = g.state(0)
+        g.close()
+        assert x["iterations"] == c["iter"] == 300
+        assert np.array_equal(np.array(x["X"]), s["X"]), seed
