@@ -1529,10 +1529,12 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
 
 // RB: the batch holds an instance that only the ground holds (its tiles also leave the rigid-body sums); the other
 // batches run the form without them
-// ONE: the batch holds a single instance (slot 0): its descriptor and control record load beside the tile item
-template <typename Real, int TILE, bool RB, bool ONE>
-__global__ __launch_bounds__(TILE) void k_strand(const Inst<Real>* __restrict__ insts, const TileItem* __restrict__ tiles, int32_t ntiles,
-                                                 const Ctl* __restrict__ ctls) {
+// ONE: the batch holds a single instance (slot 0): its descriptor and control record load beside the tile item.
+// WPE > 1 asks for that many resident wavefronts per SIMD (the FP64 form of the 64-lane tile: 168 registers and a
+// 36-byte spill for three wavefronts instead of two; faster on batches of thousands of tiles, slower on small ones)
+template <typename Real, int TILE, bool RB, bool ONE, int WPE>
+__global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(WPE))) void k_strand(const Inst<Real>* __restrict__ insts,
+                                                 const TileItem* __restrict__ tiles, int32_t ntiles, const Ctl* __restrict__ ctls) {
   __shared__ StrandLDS<Real, TILE> L;
   int lt = xcd_tile(blockIdx.x, gridDim.x);
   if (lt >= ntiles) return;
@@ -1645,11 +1647,14 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
 }
 
 // explicit instantiations
+#define WSD_STRAND(R, T, W) \
+  template __global__ void k_strand<R, T, false, false, W>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
+  template __global__ void k_strand<R, T, true, false, W>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
+  template __global__ void k_strand<R, T, false, true, W>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
+  template __global__ void k_strand<R, T, true, true, W>(const Inst<R>*, const TileItem*, int32_t, const Ctl*);
+WSD_STRAND(double, 64, 3)
 #define WSD_INST(R, T) \
-  template __global__ void k_strand<R, T, false, false>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
-  template __global__ void k_strand<R, T, true, false>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
-  template __global__ void k_strand<R, T, false, true>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
-  template __global__ void k_strand<R, T, true, true>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
+  WSD_STRAND(R, T, 1) \
   template __global__ void k_rigid<R, T>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
   template __global__ void k_outputs<R, T>(const Inst<R>*, const TileItem*, int32_t, OutPtrs); \
   template __global__ void k_evaluate<R, T>(const Inst<R>*, const TileItem*, int32_t, double*, double*, double*, const int64_t*, const int64_t*);
