@@ -223,7 +223,7 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   A.add(d.ctype, K); A.add(d.face, K); A.add(d.side, 2 * K);
   d.ctl = ctl; A.add(d.hist, size_t(std::max(hist_len, 1)) * HIST_STRIDE);
   d.tile_waves = tile / 64;
-  A.add(d.spart, size_t(d.ntiles) * SPART); A.add(d.cpart, size_t(std::max(H.ncblk, 1)) * CPART);
+  A.add(d.spart, size_t(d.ntiles) * SPART_STRIDE); A.add(d.cpart, size_t(std::max(H.ncblk, 1)) * CPART);
   {
     const int np = std::max(1, d.ntiles);
     d.ctrl_chunk = ctrl_chunk > 0 ? ctrl_chunk : (np <= CTRL_ONE ? np : CTRL_CHUNK);

@@ -13,7 +13,8 @@
 namespace wsd {
 
 constexpr int HIST_STRIDE = 9;   // history ring entry: ke, strain, bending, torsion, stretch, shear, contact, residual, iteration
-constexpr int SPART = 9;         // doubles per strand-tile partial (stored value-major)
+constexpr int SPART = 9;         // doubles per strand-tile partial
+constexpr int SPART_STRIDE = 10; // stored tile-major, padded to 80 bytes: five 16-byte loads per tile
 constexpr int CPART = 2;         // doubles per crossing-block partial (stored value-major)
 constexpr int CTRL_THREADS = 256;  // lanes of the control kernel's workgroup
 constexpr int CTRL_PER = 8;         // strand partials per lane per round (all loads of a round in flight together)
@@ -139,7 +140,7 @@ struct Inst {
   // control
   Ctl* ctl;
   double* hist;              // [hist_len * HIST_STRIDE]
-  double* spart;             // [SPART][ntiles] one partial per strand tile
+  double* spart;             // [ntiles][SPART_STRIDE] one partial per strand tile
   double* cpart;             // [CPART][ncblk]
   double* rpart;             // [ctrl_chunks][RPART] control-chunk partials (ctrl_chunks > 1)
   double* rbpart;            // [RBP][ntiles] rigid-body sums per strand tile (rigid only)

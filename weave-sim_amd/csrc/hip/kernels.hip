@@ -1246,10 +1246,10 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
     };
     const int q = lane >> 3;
     const double x = group(q, q < 5);
-    if (k == 0) gl(I.spart)[size_t(q) * I.ntiles + tl.tile_local] = x;   // [value][tile]: coalesced for k_ctrl
+    if (k == 0) gl(I.spart)[size_t(tl.tile_local) * SPART_STRIDE + q] = x;   // [tile][value]: vector loads in k_ctrl
     if (I.floor_on) {
       const double g = group(8, true);
-      if (lane == 0) gl(I.spart)[size_t(8) * I.ntiles + tl.tile_local] = g;
+      if (lane == 0) gl(I.spart)[size_t(tl.tile_local) * SPART_STRIDE + 8] = g;
     }
   }
   if (RB && I.rigid) {
@@ -1337,10 +1337,15 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
     for (int k = 0; k < CTRL_PER; k++) {
       const int w = w0 + k * CTRL_THREADS + tid;
       const bool in = w < p1;
-      const auto sp = gl(I.spart) + (in ? w : p0);
+      // one tile's record: five 16-byte loads (the ninth value is the ground's energy, the tenth padding)
+      typedef double dvec2 __attribute__((ext_vector_type(2)));
+      const auto sp = gl(reinterpret_cast<const dvec2*>(I.spart + size_t(in ? w : p0) * SPART_STRIDE));
+      dvec2 v2[SPART_STRIDE / 2];
 #pragma unroll
-      for (int q = 0; q < SPART - 1; q++) x[k][q] = in ? sp[size_t(q) * np] : 0.0;
-      x[k][SPART - 1] = (in && fon) ? sp[size_t(SPART - 1) * np] : 0.0;
+      for (int h = 0; h < SPART_STRIDE / 2; h++) v2[h] = sp[h];
+#pragma unroll
+      for (int q = 0; q < SPART - 1; q++) x[k][q] = in ? (q & 1 ? v2[q >> 1].y : v2[q >> 1].x) : 0.0;
+      x[k][SPART - 1] = (in && fon) ? v2[(SPART - 1) >> 1].x : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < CTRL_PER; k++) {
