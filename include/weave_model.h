@@ -105,6 +105,14 @@ WS_API int wsm_clone(wsm_model* mdl, wsm_model** out);
  * (without a joint or clamp for Mx, My) taking its neighbour's value; Mx, My, T [m] */
 WS_API int wsm_segment_moments(const ws_model* md, const double* M, const double* cM, double* Mx, double* My, double* T);
 WS_API int wsm_mesh_counts(wsm_model* mdl, int32_t* nV, int32_t* nF);
+/* the stability check's layout (dofList, stencil, colour; stability.js:17-23, 57-108): the DOFs in order (the free
+ * components of every node, then the three rotations of every segment: kind 0 / 1, node or segment, component), each
+ * DOF's stencil row (the nodes' and segments' DOFs within W along its lath; a crossing joins the windows of its two
+ * sides within W + 2 of floor(u)) as CSR row_ptr [n_dofs + 1] / cols [nnz], and a greedy distance-2 colouring in DOF
+ * order.  Call with dof_kind null for the sizes, then with arrays of at least those sizes (*n_dofs, *nnz in). */
+WS_API int wsm_stability_layout(const ws_model* md, const double* u, int32_t W, int32_t* n_dofs, int64_t* nnz,
+                                int32_t* n_colours, int32_t* dof_kind, int32_t* dof_index, int32_t* dof_comp,
+                                int64_t* row_ptr, int32_t* cols, int32_t* colour);
 WS_API const char* wsm_last_error(void);
 /* sizeof of the ABI structs, for binding checks: ws_params, ws_model, ws_state, ws_control, ws_batch_opts,
  * ws_server_opts, ws_result, wsm_scene, wsm_anchor, wsm_point_load, wsm_prep, wsm_hex_spec, wsm_info */

@@ -75,3 +75,36 @@ def test_lateral_torsional_buckling():
     assert n0 == 0 and l0 > 0 and n1 >= 1 and l1 < 0
     cross = p0 + (p1 - p0) * l0 / (l0 - l1)                           # λ = 0 by linear interpolation
     assert abs(cross / Pcr - 1) < 0.01, (cross, Pcr)
+
+
+def test_layout_stencil_and_distance2_colouring():
+    """the DOF layout of a woven dome (its crossings join the stencils of both laths): free node components first, then
+    three rotations per segment; symmetric stencil rows that hold the DOF itself; no two columns of one colour share a
+    row; and each DOF takes the smallest colour left free by the earlier DOFs within distance 2 (stability.js:97-108)"""
+    import time
+    m = cases.config_small()
+    u = m["u"].copy()
+    t = time.perf_counter()
+    L = ST._Layout(m, u)
+    dt = time.perf_counter() - t
+    assert m.K > 0
+    free = m.arr("node_free").astype(bool)
+    nt = int(free.sum())
+    assert len(L) == nt + 3 * m.m
+    assert (L.kind[:nt] == 0).all() and (L.kind[nt:] == 1).all()
+    assert np.array_equal(3 * L.index[:nt] + L.comp[:nt], np.flatnonzero(free))
+    rows = [set(L.row(d).tolist()) for d in range(len(L))]
+    for d, r in enumerate(rows):
+        assert d in r
+        for a in r:
+            assert d in rows[a], (d, a)
+    for r in rows:
+        cs = L.col[list(r)]
+        assert len(set(cs.tolist())) == len(cs)                 # distance-2: one colour per row at most once
+    for d in range(len(L)):
+        used = {int(L.col[q]) for a in rows[d] for q in rows[a] if q < d}
+        c = 0
+        while c in used:
+            c += 1
+        assert L.col[d] == c, d
+    assert L.ncol == L.col.max() + 1 and dt < 2.0, dt

@@ -159,6 +159,8 @@ def model_lib():
         L.wsm_set_strength.argtypes = [vp, d, d]
         L.wsm_clone.argtypes = [vp, C.POINTER(vp)]
         L.wsm_segment_moments.argtypes = [C.POINTER(ws_model), P_d, P_d, P_d, P_d, P_d]
+        L.wsm_stability_layout.argtypes = [C.POINTER(ws_model), P_d, i32, C.POINTER(i32), C.POINTER(i64), C.POINTER(i32),
+                                           P_i32, P_i32, P_i32, C.POINTER(i64), P_i32, P_i32]
         L.wsm_last_error.restype = C.c_char_p
         L.wsm_free.argtypes = [vp]
         L.wsm_free.restype = None
@@ -224,7 +226,7 @@ WEAVE_H_SYMBOLS = [
 WEAVE_MODEL_H_SYMBOLS = [
     "wsm_scene_defaults", "wsm_from_mesh", "wsm_from_polylines", "wsm_hex_dome", "wsm_get", "wsm_info_get",
     "wsm_add_load", "wsm_set_strength", "wsm_clone", "wsm_mesh_counts", "wsm_last_error", "wsm_free",
-    "wsm_abi_sizes", "wsm_segment_moments",
+    "wsm_abi_sizes", "wsm_segment_moments", "wsm_stability_layout",
 ]
 ABI_STRUCTS = ["ws_params", "ws_model", "ws_state", "ws_control", "ws_batch_opts", "ws_server_opts", "ws_result",
                "wsm_scene", "wsm_anchor", "wsm_point_load", "wsm_prep", "wsm_hex_spec", "wsm_info", "ws_tune_entry"]

@@ -1127,6 +1127,117 @@ WS_API int wsm_segment_moments(const ws_model* md, const double* M, const double
   return WS_OK;
 }
 
+// the stability check's DOFs, stencil and colouring (dofList, stencil, colour; stability.js:17-23, 57-108)
+namespace {
+struct StabLayout {
+  std::vector<int32_t> kind, index, comp, cols, colour;
+  std::vector<int64_t> rowPtr;
+  int32_t ncol = 0;
+};
+StabLayout stabilityLayout(const ws_model& md, const double* u, int W) {
+  StabLayout L;
+  const int n = md.n, m = md.m, S = md.S;
+  std::vector<std::vector<int32_t>> nodeDofs(n), segDofs(m);
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++)
+      if (md.node_free[3 * i + k]) { nodeDofs[i].push_back(int32_t(L.kind.size())); L.kind.push_back(0); L.index.push_back(i); L.comp.push_back(k); }
+  for (int j = 0; j < m; j++)
+    for (int k = 0; k < 3; k++) { segDofs[j].push_back(int32_t(L.kind.size())); L.kind.push_back(1); L.index.push_back(j); L.comp.push_back(k); }
+  const int nd = int(L.kind.size());
+  std::vector<int32_t> nodeStrand(n), segStrand(m);
+  for (int s = 0; s < S; s++) {
+    for (int i = md.node_offset[s]; i < md.node_offset[s + 1]; i++) nodeStrand[i] = s;
+    for (int j = md.seg_offset[s]; j < md.seg_offset[s + 1]; j++) segStrand[j] = s;
+  }
+  // the nodes and segments within w of position loc along lath s (modulo the lath when it is closed)
+  auto window = [&](int s, int loc, int w, std::vector<int32_t>& out) {
+    const int nn = md.node_offset[s + 1] - md.node_offset[s], ns = md.seg_offset[s + 1] - md.seg_offset[s];
+    const bool closed = nn == ns;
+    auto wrap = [](int v, int q) { int r = v % q; return r < 0 ? r + q : r; };
+    for (int o = -w; o <= w; o++) {
+      int a = loc + o;
+      if (closed && nn) a = wrap(a, nn);
+      if (a >= 0 && a < nn) for (int d : nodeDofs[md.node_offset[s] + a]) out.push_back(d);
+      int b = loc + o;
+      if (closed && ns) b = wrap(b, ns);
+      if (b >= 0 && b < ns) for (int d : segDofs[md.seg_offset[s] + b]) out.push_back(d);
+    }
+  };
+  std::vector<std::vector<int32_t>> N(nd);
+  std::vector<int32_t> st;
+  for (int d = 0; d < nd; d++) {
+    const int s = L.kind[d] == 0 ? nodeStrand[L.index[d]] : segStrand[L.index[d]];
+    const int loc = L.index[d] - (L.kind[d] == 0 ? md.node_offset[s] : md.seg_offset[s]);
+    window(s, loc, W, N[d]);
+  }
+  if (md.par.contacts && md.K) {
+    for (int c = 0; c < md.K; c++) {
+      st.clear();
+      window(md.k_strand[2 * c], int(std::floor(u[2 * c])), W + 2, st);
+      window(md.k_strand[2 * c + 1], int(std::floor(u[2 * c + 1])), W + 2, st);
+      for (int a : st) N[a].insert(N[a].end(), st.begin(), st.end());
+    }
+  }
+  L.rowPtr.assign(nd + 1, 0);
+  for (int d = 0; d < nd; d++) {
+    std::sort(N[d].begin(), N[d].end());
+    N[d].erase(std::unique(N[d].begin(), N[d].end()), N[d].end());
+    L.rowPtr[d + 1] = L.rowPtr[d] + int64_t(N[d].size());
+  }
+  L.cols.reserve(size_t(L.rowPtr[nd]));
+  for (int d = 0; d < nd; d++) L.cols.insert(L.cols.end(), N[d].begin(), N[d].end());
+  // greedy distance-2 colouring in DOF order: the smallest colour no DOF sharing a row with d has taken.  Every row
+  // keeps the set of colours taken in it (a bitset): d's forbidden colours are the union over its rows, since q
+  // shares row r with d exactly when both lie in the (symmetric) stencil row r
+  L.colour.assign(nd, -1);
+  int words = 8;
+  std::vector<uint64_t> taken(size_t(nd) * words, 0);
+  std::vector<uint64_t> forbid;
+  for (int d = 0; d < nd; d++) {
+    forbid.assign(words, 0);
+    for (int r : N[d]) {
+      const uint64_t* t = &taken[size_t(r) * words];
+      for (int w = 0; w < words; w++) forbid[w] |= t[w];
+    }
+    int c = -1;
+    for (int w = 0; w < words && c < 0; w++)
+      if (~forbid[w]) c = 64 * w + __builtin_ctzll(~forbid[w]);
+    if (c < 0) {
+      // every colour of the bitsets is taken: widen them
+      c = 64 * words;
+      std::vector<uint64_t> wider(size_t(nd) * 2 * words, 0);
+      for (size_t r = 0; r < size_t(nd); r++) std::copy(&taken[r * words], &taken[r * words] + words, &wider[r * 2 * words]);
+      taken.swap(wider);
+      words *= 2;
+    }
+    L.colour[d] = c;
+    L.ncol = std::max(L.ncol, c + 1);
+    for (int r : N[d]) taken[size_t(r) * words + (c >> 6)] |= uint64_t(1) << (c & 63);
+  }
+  return L;
+}
+}  // namespace
+
+WS_API int wsm_stability_layout(const ws_model* md, const double* u, int32_t W, int32_t* n_dofs, int64_t* nnz,
+                                int32_t* n_colours, int32_t* dof_kind, int32_t* dof_index, int32_t* dof_comp,
+                                int64_t* row_ptr, int32_t* cols, int32_t* colour) {
+  try {
+    if (!md || !n_dofs || !nnz || !n_colours || (md->par.contacts && md->K && !u) || W < 0)
+      throw std::runtime_error("wsm_stability_layout: bad arguments");
+    const StabLayout L = stabilityLayout(*md, u, W);
+    const int nd = int(L.kind.size());
+    if (dof_kind) {
+      if (*n_dofs < nd || *nnz < L.rowPtr[nd] || !dof_index || !dof_comp || !row_ptr || !cols || !colour)
+        throw std::runtime_error("wsm_stability_layout: output arrays smaller than the layout");
+      std::copy(L.kind.begin(), L.kind.end(), dof_kind); std::copy(L.index.begin(), L.index.end(), dof_index);
+      std::copy(L.comp.begin(), L.comp.end(), dof_comp); std::copy(L.rowPtr.begin(), L.rowPtr.end(), row_ptr);
+      std::copy(L.cols.begin(), L.cols.end(), cols); std::copy(L.colour.begin(), L.colour.end(), colour);
+    }
+    *n_dofs = nd; *nnz = L.rowPtr[nd]; *n_colours = L.ncol;
+    return WS_OK;
+  } catch (const std::exception& e) { g_err = e.what(); return WS_ERR_ARG; }
+}
+
 WS_API int wsm_add_load(wsm_model* mdl, int32_t node, const double* F) {
   if (!mdl || !F || node < 0 || node >= mdl->md->n) { g_err = "wsm_add_load: bad node"; return WS_ERR_ARG; }
   for (int c = 0; c < 3; c++) mdl->md->loadConst[3 * node + c] += F[c];

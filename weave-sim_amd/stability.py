@@ -12,89 +12,65 @@ is a sparse LDLᵀ with a symmetric fill-reducing ordering (SuperLU in symmetric
 negative pivots is the number of negative eigenvalues (Sylvester's law of inertia), and inverse iteration on the factor
 gives the eigenvalue nearest zero (Rayleigh quotient), as the reference does with its own LDLᵀ
 (stability.js:98-225)."""
+import ctypes as C
 import time
 
 import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
+from . import _abi as A
+
 
 def _qexp(v):
-    a = np.linalg.norm(v)
+    """exp of rotation vectors v [k, 3] → unit quaternions [k, 4] (w x y z)"""
+    a = np.linalg.norm(v, axis=1)
     half = 0.5 * a
-    s = np.sin(half) / a if a > 1e-12 else 0.5
-    return np.array([np.cos(half), v[0] * s, v[1] * s, v[2] * s])
+    s = np.where(a > 1e-12, np.sin(half) / np.where(a > 1e-12, a, 1.0), 0.5)
+    return np.column_stack([np.cos(half), v[:, 0] * s, v[:, 1] * s, v[:, 2] * s])
 
 
 def _qmul(a, b):
-    aw, ax, ay, az = a
-    bw, bx, by, bz = b
-    return np.array([aw * bw - ax * bx - ay * by - az * bz, aw * bx + ax * bw + ay * bz - az * by,
-                     aw * by - ax * bz + ay * bw + az * bx, aw * bz + ax * by - ay * bx + az * bw])
+    """products of quaternion rows a, b [k, 4]"""
+    aw, ax, ay, az = a.T
+    bw, bx, by, bz = b.T
+    return np.column_stack([aw * bw - ax * bx - ay * by - az * bz, aw * bx + ax * bw + ay * bz - az * by,
+                            aw * by - ax * bz + ay * bw + az * bx, aw * bz + ax * by - ay * bx + az * bw])
 
 
 class _Layout:
-    """DOFs and the stencil of a model (dofList, stencil; stability.js:17-23, 57-94)"""
+    """DOFs, stencil rows and colouring of a model (dofList, stencil, colour; stability.js:17-23, 57-108), built by the
+    host model library (wsm_stability_layout)"""
 
     def __init__(self, model, u, W=3):
-        md = model.md
-        n, m, S = md.n, md.m, md.S
-        free = np.ctypeslib.as_array(md.node_free, shape=(3 * n,)).astype(bool)
-        self.dofs = [(0, i, k) for i in range(n) for k in range(3) if free[3 * i + k]] + \
-                    [(1, j, k) for j in range(m) for k in range(3)]
-        nd = len(self.dofs)
-        node_dofs = [[] for _ in range(n)]
-        seg_dofs = [[] for _ in range(m)]
-        for d, (kind, i, _) in enumerate(self.dofs):
-            (node_dofs if kind == 0 else seg_dofs)[i].append(d)
-        noff = np.ctypeslib.as_array(md.node_offset, shape=(S + 1,))
-        soff = np.ctypeslib.as_array(md.seg_offset, shape=(S + 1,))
-        node_strand = np.repeat(np.arange(S), np.diff(noff))
-        seg_strand = np.repeat(np.arange(S), np.diff(soff))
-        closed = (np.diff(soff) == np.diff(noff))
+        L = A.model_lib()
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        nd, nnz, ncol = C.c_int32(0), C.c_int64(0), C.c_int32(0)
+        pu = u.ctypes.data_as(A.P_d) if u.size else None
+        rc = L.wsm_stability_layout(C.byref(model.md), pu, W, C.byref(nd), C.byref(nnz), C.byref(ncol),
+                                    None, None, None, None, None, None)
+        if rc:
+            raise ValueError(L.wsm_last_error().decode())
+        kind, idx, comp = (np.zeros(max(nd.value, 1), dtype=np.int32) for _ in range(3))
+        row_ptr = np.zeros(nd.value + 1, dtype=np.int64)
+        cols = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        col = np.zeros(max(nd.value, 1), dtype=np.int32)
+        rc = L.wsm_stability_layout(C.byref(model.md), pu, W, C.byref(nd), C.byref(nnz), C.byref(ncol),
+                                    kind.ctypes.data_as(A.P_i32), idx.ctypes.data_as(A.P_i32),
+                                    comp.ctypes.data_as(A.P_i32), row_ptr.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    cols.ctypes.data_as(A.P_i32), col.ctypes.data_as(A.P_i32))
+        if rc:
+            raise ValueError(L.wsm_last_error().decode())
+        n = nd.value
+        self.kind, self.index, self.comp = kind[:n].astype(np.int64), idx[:n].astype(np.int64), comp[:n].astype(np.int64)
+        self.row_ptr, self.cols = row_ptr, cols[:nnz.value].astype(np.int64)
+        self.col, self.ncol = col[:n].astype(np.int64), ncol.value
 
-        def window(s, loc, w):
-            out = []
-            nn, ns = noff[s + 1] - noff[s], soff[s + 1] - soff[s]
-            for o in range(-w, w + 1):
-                a = loc + o
-                if closed[s]:
-                    a %= nn
-                if 0 <= a < nn:
-                    out += node_dofs[noff[s] + a]
-                b = loc + o
-                if closed[s] and ns:
-                    b %= ns
-                if 0 <= b < ns:
-                    out += seg_dofs[soff[s] + b]
-            return out
+    def __len__(self):
+        return len(self.kind)
 
-        N = [set() for _ in range(nd)]
-        for d, (kind, i, _) in enumerate(self.dofs):
-            s = node_strand[i] if kind == 0 else seg_strand[i]
-            loc = i - (noff[s] if kind == 0 else soff[s])
-            N[d].update(window(s, loc, W))
-        if md.par.contacts and md.K:
-            ks = np.ctypeslib.as_array(md.k_strand, shape=(2 * md.K,))
-            for c in range(md.K):
-                st = window(ks[2 * c], int(np.floor(u[2 * c])), W + 2) + window(ks[2 * c + 1], int(np.floor(u[2 * c + 1])), W + 2)
-                for a in st:
-                    N[a].update(st)
-        self.N = [np.array(sorted(x), dtype=np.int64) for x in N]
-        # greedy distance-2 colouring: columns of one colour never touch a common row (colour, stability.js:97-108)
-        col = np.full(nd, -1, dtype=np.int64)
-        mark = np.full(nd + 1, -1, dtype=np.int64)
-        ncol = 0
-        for d in range(nd):
-            for r in self.N[d]:
-                cs = col[self.N[r]]
-                mark[cs[cs >= 0]] = d
-            c = 0
-            while mark[c] == d:
-                c += 1
-            col[d] = c
-            ncol = max(ncol, c + 1)
-        self.col, self.ncol = col, ncol
+    def row(self, d):
+        return self.cols[self.row_ptr[d]:self.row_ptr[d + 1]]
 
 
 def device_evaluator(W, model, base=None, batch=256):
@@ -136,34 +112,35 @@ def stability(model, evaluate, X=None, Q=None, u=None, iters=40, tol=1e-9):
     Q0 = np.array(model["Q"] if Q is None else Q, dtype=np.float64).reshape(-1)
     U = np.array(model["u"] if u is None else u, dtype=np.float64)
     L = _Layout(model, U)
-    dofs, nd = L.dofs, len(L.dofs)
+    nd = len(L)
     if nd == 0:
         return {"lam": float("nan"), "neg": 0, "ndof": 0}
     P = X0.reshape(-1, 3)
     size = max(float(np.linalg.norm(P.max(0) - P.min(0))), 1e-9)
     hT, hR = 1e-6 * size, 1e-6
-    by_col = [np.flatnonzero(L.col == c) for c in range(L.ncol)]
+    kind, idx_i, idx_k = L.kind, L.index, L.comp
+    order = np.argsort(L.col, kind="stable")
+    by_col = np.split(order, np.cumsum(np.bincount(L.col, minlength=L.ncol))[:-1])
 
     def perturbed(ds, sgn):
+        # a colour holds each node component and each segment at most once (they share stencil rows)
         X, Q = X0.copy(), Q0.copy()
-        for d in ds:
-            kind, i, k = dofs[d]
-            if kind == 0:
-                X[3 * i + k] += sgn * hT
-            else:
-                v = np.zeros(3)
-                v[k] = sgn * hR
-                q = _qmul(Q[4 * i:4 * i + 4], _qexp(v))
-                Q[4 * i:4 * i + 4] = q / np.linalg.norm(q)
+        t = ds[kind[ds] == 0]
+        X[3 * idx_i[t] + idx_k[t]] += sgn * hT
+        r = ds[kind[ds] == 1]
+        if r.size:
+            j = idx_i[r]
+            v = np.zeros((r.size, 3))
+            v[np.arange(r.size), idx_k[r]] = sgn * hR
+            Qr = Q.reshape(-1, 4)
+            q = _qmul(Qr[j], _qexp(v))
+            Qr[j] = q / np.linalg.norm(q, axis=1)[:, None]
         return X, Q
 
     states = [(X0, Q0)]
     for c in range(L.ncol):
         states += [perturbed(by_col[c], +1), perturbed(by_col[c], -1)]
     evals = evaluate(states)
-    kind = np.array([d[0] for d in dofs])
-    idx_i = np.array([d[1] for d in dofs])
-    idx_k = np.array([d[2] for d in dofs])
 
     tr, ro = kind == 0, kind == 1
     flat_t, flat_r = 3 * idx_i[tr] + idx_k[tr], 3 * idx_i[ro] + idx_k[ro]
@@ -183,12 +160,14 @@ def stability(model, evaluate, X=None, Q=None, u=None, iters=40, tol=1e-9):
     for c in range(L.ncol):
         gp = gvec(*evals[1 + 2 * c][:2])
         gm = gvec(*evals[2 + 2 * c][:2])
-        for d in by_col[c]:
-            h = hT if kind[d] == 0 else hR
-            r = L.N[d]
-            v = -(gp[r] - gm[r]) / (2 * h)
-            nz = v != 0
-            rows.append(r[nz]); cols.append(np.full(int(nz.sum()), d)); vals.append(v[nz])
+        ds = by_col[c]
+        start, lens = L.row_ptr[ds], L.row_ptr[ds + 1] - L.row_ptr[ds]
+        # every stencil row of the colour's columns, column by column (CSR slices gathered at once)
+        pos = np.arange(int(lens.sum())) - np.repeat(np.cumsum(lens) - lens, lens) + np.repeat(start, lens)
+        r, d = L.cols[pos], np.repeat(ds, lens)
+        v = -(gp[r] - gm[r]) / (2 * np.where(kind[d] == 0, hT, hR))
+        nz = v != 0
+        rows.append(r[nz]); cols.append(d[nz]); vals.append(v[nz])
     rows, cols, vals = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
     sc = 1.0 / np.sqrt(np.maximum(mdof, 1e-300))
     K = sp.coo_matrix((vals * sc[rows] * sc[cols], (rows, cols)), shape=(nd, nd)).tocsc()
