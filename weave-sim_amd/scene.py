@@ -81,6 +81,13 @@ def _flag(v):
     return 1 if v else 0
 
 
+def _sec(d, key):
+    """a section of the file as a dictionary: anything else reads as empty (a property of a non-object is undefined in
+    the reference's reader)"""
+    v = d.get(key) if isinstance(d, dict) else None
+    return v if isinstance(v, dict) else {}
+
+
 # params.js:6-19: what a section of a scene or of a lath group holds when the file leaves a key out
 _DEFAULTS = {
     "material": {"E": 10000.0, "G": 625.0, "density": 700.0, "E90": 800.0, "fc90": 8.0, "H": 30.0},
@@ -95,11 +102,11 @@ def _groups_match(d, group, stress):
     """a lath group equal to the scene's own material / section / beam: every key at its default when left out, and
     the scene's own E90 / fc90 / H defaults in its stress unit (a group's are not rescaled, scene.js:144-155)"""
     for key, dflt in _DEFAULTS.items():
-        a = dict(dflt, **(d.get(key) or {}))
-        b = dict(dflt, **(group.get(key) or {}))
+        a = dict(dflt, **_sec(d, key))
+        b = dict(dflt, **_sec(group, key))
         if key == "material":
             for k, v0 in _MATERIAL_MM_N.items():
-                if k not in (d.get(key) or {}):
+                if k not in _sec(d, key):
                     a[k] = v0 / stress
         if a != b:
             return False
@@ -109,7 +116,7 @@ def _groups_match(d, group, stress):
 def _hybrid_is_one_group(d, stress):
     """no strand or piece of a discretisation is assigned to a lath group that differs from the scene's own
     (scene.js:53-72: group 1 is the scene's material / section / beam, assignments name groups 2 …)"""
-    h = d.get("hybrid") or {}
+    h = _sec(d, "hybrid")
     if not h.get("enabled"):
         return True
     groups = h.get("groups") or []
@@ -123,7 +130,7 @@ def scene_from_dict(d):
     v = d.get("version", SCENE_VERSION)
     if v > SCENE_VERSION:
         raise ValueError(f"scene version {v} newer than supported {SCENE_VERSION}")
-    units = d.get("units") or {}
+    units = _sec(d, "units")
     lu, fu = units.get("length", "mm"), units.get("force", "N")
     if lu not in LENGTH_TO_M or fu not in FORCE_TO_N:
         raise ValueError(f"unknown units {lu}, {fu}")
@@ -136,7 +143,7 @@ def scene_from_dict(d):
         raise NotImplementedError("lath groups with their own material, section or beam (hybrid weaves) are not "
                                   "modelled by the builder")
 
-    mat = d.get("material") or {}
+    mat = _sec(d, "material")
     g = c.grp
     for k in ("E", "G", "density", "E90", "fc90", "H"):
         if k in mat and mat[k] is not None:
@@ -145,13 +152,13 @@ def scene_from_dict(d):
     for k, v0 in _MATERIAL_MM_N.items():
         if k not in mat:
             setattr(g, k, v0 / stress)
-    sec = d.get("section") or {}
+    sec = _sec(d, "section")
     for k in ("width", "height"):
         if k in sec and sec[k] is not None:
             setattr(g, k, float(sec[k]))
     for k in ("Ix", "Iy", "J"):
         setattr(g, k, float(sec[k]) if sec.get(k) is not None else 0.0)
-    beam = d.get("beam") or {}
+    beam = _sec(d, "beam")
     g.beam_advanced = _flag(beam.get("mode", "simple") == "advanced")
     for src, dst in (("activation", "activation"), ("stiffness", "stiffness"), ("activation_flat", "act_flat"),
                      ("activation_edge", "act_edge"), ("activation_twist", "act_twist"), ("stiffness_axial", "k_axial"),
@@ -161,7 +168,7 @@ def scene_from_dict(d):
             setattr(g, dst, float(beam[src]))
 
     p = c.par
-    sol = d.get("solver") or {}
+    sol = _sec(d, "solver")
     for k in ("mass_scale", "damping", "tol"):
         if k in sol:
             setattr(p, k, float(sol[k]))
@@ -171,7 +178,7 @@ def scene_from_dict(d):
     if "kinetic_damping" in sol:
         p.kinetic_damping = _flag(sol["kinetic_damping"])
     c.prestress_asbuilt = _flag(sol.get("prestress", "flat") == "asbuilt")
-    con = d.get("contact") or {}
+    con = _sec(d, "contact")
     for k in ("mu", "strength", "stick_strength", "torsion_factor", "epsilon", "slip_rate", "rot_factor", "dent_rate",
               "edge_threshold"):
         if k in con:
@@ -187,7 +194,7 @@ def scene_from_dict(d):
     if "dent_after_iter" in con:
         p.dent_after = int(con["dent_after_iter"])
     c.locked = _flag(con.get("locked", False))
-    fl = d.get("floor") or {}
+    fl = _sec(d, "floor")
     fz = fl.get("z")
     fmu = fl.get("mu")
     fmu = float(fmu) if fmu is not None and float(fmu) >= 0 else FLOOR_MU
@@ -195,7 +202,7 @@ def scene_from_dict(d):
     p.floor_z = float(fz) if fz is not None else 0.0
     p.floor_mu = fmu
 
-    wv = d.get("weave") or {}
+    wv = _sec(d, "weave")
     if "subdivisions" in wv:
         c.subdivisions = int(wv["subdivisions"])
     if "overhang" in wv:
@@ -205,11 +212,11 @@ def scene_from_dict(d):
         c.seed_strand_phase = int(wv["seed_strand_phase"])
     flips = [(int(f[0]), int(f[1])) for f in wv.get("flip_knots") or []]
 
-    lo = d.get("loads") or {}
-    gr = lo.get("gravity") or {}
+    lo = _sec(d, "loads")
+    gr = _sec(lo, "gravity")
     c.gravity = _flag(gr.get("enabled", True))
     c.gravity_dir[:] = [float(x) for x in gr.get("direction", [0, 0, -1])]
-    udl = lo.get("udl") or {}
+    udl = _sec(lo, "udl")
     c.udl = 0 if not udl.get("enabled") else (1 if udl.get("per", "length") == "length" else 2)
     c.udl_q[:] = [float(x) for x in udl.get("q", [0, 0, 0])]
     loads = [(int(pl["frame"][0]), int(pl["frame"][1]), [float(x) for x in pl["F"]], pl.get("space", "world") == "local")
@@ -232,7 +239,7 @@ def scene_from_dict(d):
             raise NotImplementedError("the 'region' anchor selector (graph distance on the mesh) is not modelled")
         else:
             raise ValueError(f"unknown selector type {t}")
-    mesh = d.get("mesh") or {}
+    mesh = _sec(d, "mesh")
     return sc, anchors, loads, flips, {"path": mesh.get("path", ""), "scale": float(mesh.get("scale", 1.0))}
 
 
