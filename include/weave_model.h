@@ -86,6 +86,17 @@ WS_API int wsm_from_mesh(const double* V, int32_t nV, const int32_t* F, int32_t 
                          const wsm_scene* sc, const wsm_anchor* anchors, int32_t n_anchors,
                          const wsm_point_load* loads, int32_t n_loads, const int32_t* flip_knots, int32_t n_flips,
                          const wsm_prep* prep, wsm_model** out);
+/* a hybrid weave (Scene.hybrid with strand assignments, scene.js:50-62): groups[0 .. n_groups) are lath groups
+ * 2, 3, …; strand_group[2a], strand_group[2a+1] = strand id, group (n_assign pairs); an unassigned strand, or one
+ * assigned outside 2 … 1 + n_groups, is in group 1 = sc->grp.  Each segment's stiffness, activation and self-weight,
+ * each crossing side's section and material, each node's floor section and each knot's half-height lift follow
+ * the group of the strand (model.js:203-252, solver_cpu.js:72-78 and 558-560, topology.js:225).  Per-piece groups
+ * (Scene.hybrid.pieces) are not taken. */
+WS_API int wsm_from_mesh_groups(const double* V, int32_t nV, const int32_t* F, int32_t nF, int32_t weld,
+                                const wsm_scene* sc, const wsm_anchor* anchors, int32_t n_anchors,
+                                const wsm_point_load* loads, int32_t n_loads, const int32_t* flip_knots,
+                                int32_t n_flips, const wsm_prep* prep, const wsm_group* groups, int32_t n_groups,
+                                const int32_t* strand_group, int32_t n_assign, wsm_model** out);
 /* stand-alone laths (topologyFromPolylines, topology.js:348-363) with optional explicit crossings
  * (knots[4c..4c+3] = sA, kA, sB, kB frame ids) as the test fixtures of web/tests/cases.mjs build them */
 WS_API int wsm_from_polylines(const double* P, const int32_t* counts, int32_t n_strands,

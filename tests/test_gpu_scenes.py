@@ -1,11 +1,13 @@
 """The reference's example scenes on the device against the f64 oracle (scene.json + mesh kept as input fixtures
 under tests/golden): the dome with point loads in world and frame axes and a flipped knot (dome_basic), the v5 dome
 of the reference's own GPU checks (v5dome_cm, selftest.html:38-43), the tree with augmented-Lagrangian contact and a
-UDL per length (tree_cm), the dual v5 dome with rotational friction, the seat law and a UDL per area (dualv5dome_cm).
+UDL per length (tree_cm), the dual v5 dome with rotational friction, the seat law and a UDL per area (dualv5dome_cm), and its hybrid weave
+with six laths in a second lath group of another material and section (dualv5dome_cm_hybrid: crossing sides of
+different sections, per-segment stiffness and weight).
 
 Windows after the 300-iteration warm-up (nothing moves during it).  Bars: FP64 within 1e-9 × span of the oracle with
 identical stick / slip states through 1,000 iterations (an oracle run perturbed by 1e-12 of the span stays within
-1e-11 on all four); FP32+FP64 within 1e-4 × span, with identical stick / slip states wherever an oracle run perturbed
+1e-11 on all five); FP32+FP64 within 1e-4 × span, with identical stick / slip states wherever an oracle run perturbed
 at FP32's rounding level (1e-7 of the span) keeps them, and past that within the spread of that run.  The reference
 reports its own GPU form on the v5 dome as tracking f64 to 1.2e-5 of the displacement through 1,000 iterations
 (web/README.md:80-82); the FP32+FP64 figure here is printed beside the bar."""
@@ -24,7 +26,7 @@ HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scene
 WINDOW = [400, 600, 1000]
 
 
-@pytest.mark.parametrize("name", ["dome_basic", "v5dome_cm", "tree_cm", "dualv5dome_cm"])
+@pytest.mark.parametrize("name", ["dome_basic", "v5dome_cm", "tree_cm", "dualv5dome_cm", "dualv5dome_cm_hybrid"])
 @pytest.mark.parametrize("precision,tol,eps", [("fp64", 1e-9, 1e-12), ("fp32", 1e-4, 1e-7)])
 def test_example_scene_matches_oracle(W, gpu_ctx, name, precision, tol, eps):
     m = SC.load_scene(os.path.join(HERE, name + ".json"))

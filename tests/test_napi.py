@@ -144,7 +144,7 @@ def test_js_scene_reader_matches_python():
     import importlib
     SC = importlib.import_module("weave-sim_amd.scene")
     golden = os.path.join(ROOT, "tests", "golden", "scenes")
-    names = ["dome_basic", "v5dome_cm", "tree_cm", "dualv5dome_cm"]
+    names = ["dome_basic", "v5dome_cm", "tree_cm", "dualv5dome_cm", "dualv5dome_cm_hybrid"]
     scene_js = os.path.join(ROOT, "weave-sim_amd", "js", "scene_mi355x.js")
     out = run_node(f"""
 const S = require({json.dumps(scene_js)});
@@ -155,11 +155,13 @@ for (const name of {json.dumps(names)}) {{
     X: Array.from(m.st.X), u: Array.from(m.st.u), loads: Array.from(m.info.loadConst), free: Array.from(m.info.nodeFree) }};
 }}
 let refused = '';
-try {{ S.loadScene(require('path').join({json.dumps(golden)}, 'dualv5dome_cm_hybrid.json')); }} catch (e) {{ refused = e.message; }}
+const hy = JSON.parse(require('fs').readFileSync(require('path').join({json.dumps(golden)}, 'dualv5dome_cm_hybrid.json'), 'utf8'));
+hy.hybrid.pieces = {{ '3:1': 2 }};
+try {{ S.sceneFromJSON(hy); }} catch (e) {{ refused = e.message; }}
 res.refused = refused;
 console.log(JSON.stringify(res));""")
     d = json.loads(out)
-    assert "hybrid" in d["refused"]
+    assert "pieces" in d["refused"]
     for name in names:
         py = SC.load_scene(os.path.join(golden, name + ".json"))
         js = d[name]

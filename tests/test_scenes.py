@@ -91,23 +91,115 @@ def test_scene_parameters_and_units():
     assert (s.c.grp.E90, s.c.grp.fc90, s.c.grp.H) == pytest.approx((80000.0, 800.0, 3000.0))   # the file's own values
 
 
-def test_hybrid_groups_that_differ_are_refused_and_equal_ones_accepted():
-    with pytest.raises(NotImplementedError):
-        SC.scene_from_dict(scene_json("dualv5dome_cm_hybrid"))
+def _strand_of(m, offsets, count):
+    off = m.arr(offsets)
+    return np.repeat(np.arange(len(off) - 1), np.diff(off))[:count]
+
+
+def _build(d, name="dualv5dome_cm_hybrid"):
+    return SC.model_from_scene(d, base_dir=HERE)
+
+
+def test_hybrid_groups_per_strand_in_closed_form():
+    """dualv5dome_cm_hybrid: six strands in group 2 (E 5e6, 6 × 2 section) and the rest in the scene's own (E 1e6,
+    4 × 0.8).  Each segment's stiffness is its strand's group's section stiffness × its beam factors, each crossing
+    side's section and material its strand's group's (a group's E90 is taken as given, the scene's own default is
+    rescaled to its stress unit), each crossing's section the mean of its sides, each node's floor section its strand's
+    group's, and the self-weight Σ over segments of ρ·A·g of its group × l0 (model.js:203-252, solver_cpu.js:66-78,
+    scene.js:57-61)."""
+    d = scene_json("dualv5dome_cm_hybrid")
+    m = _build(d)
+    g1, g2 = d, d["hybrid"]["groups"][0]
+    grp = {int(s): int(g) for s, g in d["hybrid"]["assign"].items()}
+    S = m.md.S
+    gs = np.array([grp.get(s, 1) for s in range(S)])
+    assert (gs == 2).sum() == 6
+    E = np.where(gs == 2, g2["material"]["E"], g1["material"]["E"])
+    w = np.where(gs == 2, g2["section"]["width"], g1["section"]["width"])
+    h = np.where(gs == 2, g2["section"]["height"], g1["section"]["height"])
+    rho = np.where(gs == 2, g2["material"]["density"], g1["material"]["density"])
+    k = np.where(gs == 2, g2["beam"]["stiffness"], g1["beam"]["stiffness"])
+    E90 = np.where(gs == 2, 800.0, 80000.0)          # group 2 leaves E90 out: 800 as given; the scene's own: 800 N/mm² in N/cm²
+    sj = _strand_of(m, "seg_offset", m.m)
+    assert np.allclose(m.arr("EA"), (E * w * h)[sj], rtol=1e-14)
+    assert np.allclose(m.arr("EIx"), (k * E * w * h ** 3 / 12)[sj], rtol=1e-14)
+    assert np.allclose(m.arr("EIy"), (k * E * h * w ** 3 / 12)[sj], rtol=1e-14)
+    ks = m.arr("k_strand")
+    assert np.array_equal(m.arr("k_wk"), w[ks]) and np.array_equal(m.arr("k_hk"), h[ks])
+    assert np.array_equal(m.arr("k_E90"), E90[ks])
+    assert np.array_equal(m.arr("k_w"), 0.5 * (w[ks[0::2]] + w[ks[1::2]]))
+    assert np.array_equal(m.arr("k_h"), 0.5 * (h[ks[0::2]] + h[ks[1::2]]))
+    assert len(set(zip(gs[ks[0::2]], gs[ks[1::2]]))) >= 3           # crossings of 1 × 1, 1 × 2 and 2 × 2 laths
+    si = _strand_of(m, "node_offset", m.n)
+    assert np.array_equal(m.arr("node_w"), w[si]) and np.array_equal(m.arr("node_h"), h[si])
+    L = SC.LENGTH_TO_M[d["units"]["length"]]
+    qline = rho * (w * h * L * L) * 9.80665 * L                     # units.js:21-25, force unit N
+    gdir = np.array(d["loads"]["gravity"]["direction"], dtype=float)
+    expect = (m.arr("l0") * qline[sj]).sum() * gdir / np.linalg.norm(gdir)
+    udl = d["loads"]["udl"]
+    if udl["enabled"]:
+        expect += np.array(udl["q"]) * (m.arr("l0").sum() if udl["per"] == "length" else mesh_area("dualv5dome_cm_hybrid"))
+    got = m.arr("load_const").reshape(-1, 3).sum(0)
+    assert np.allclose(got, expect, rtol=1e-12, atol=1e-12 * np.abs(expect).max()), (got, expect)
+
+
+def test_hybrid_group_assigned_everywhere_equals_a_scene_of_that_group():
+    """every strand in group 2 builds the very model of a scene whose own material / section / beam are group 2's
+    (the nodes between knots are lifted by the scene's own half-height, topology.js:236-249, so that scene keeps
+    group 2's height), and the knot lift of a crossing is the mean of its laths' group heights (topology.js:225): a
+    crossing of two group-1 laths sits where it does with no groups, one of two group-2 laths where it does with
+    every lath in group 2, and a mixed one halfway between"""
+    d = scene_json("dualv5dome_cm_hybrid")
+    g2 = d["hybrid"]["groups"][0]
+    for g in (d, g2):
+        g["material"].update({"E90": 800.0, "fc90": 8.0, "H": 30.0})
+    S = SC.model_from_scene(dict(d, hybrid={"enabled": False}), base_dir=HERE).md.S
+    every = _build(dict(d, section=dict(d["section"], height=g2["section"]["height"]),
+                        hybrid=dict(d["hybrid"], assign={str(s): 2 for s in range(S)})))
+    own = _build(dict(d, material=g2["material"], section=g2["section"], beam=g2["beam"], hybrid={"enabled": False}))
+    names = ["l0", "j_len", "EA", "GA", "EIx", "EIy", "GJ", "omega0", "load_const", "node_free", "k_wk", "k_hk",
+             "k_w", "k_h", "k_fc90", "k_E90", "k_Hs", "node_w", "node_h", "mass"]
+    for nm in names:
+        assert np.array_equal(every.arr(nm), own.arr(nm)), nm
+    assert np.array_equal(every["X"], own["X"]) and np.array_equal(every["Q"], own["Q"])
+    plain = _build(dict(d, hybrid={"enabled": False}))
+    mixed = _build(d)
+    gs = np.ones(S, dtype=int)
+    for s, g in d["hybrid"]["assign"].items():
+        gs[int(s)] = g
+    ks = mixed.arr("k_strand")
+    both = gs[ks[0::2]] + gs[ks[1::2]]                              # 2, 3 or 4
+
+    def knots(x):
+        """each side's knot node: u starts at the knot row, projected onto the lath to within a fraction of a segment"""
+        assert np.array_equal(x.arr("k_strand"), ks)
+        row = np.rint(x["u"])
+        assert np.abs(x["u"] - row).max() < 0.4
+        return x["X"].reshape(-1, 3)[x.arr("node_offset")[ks] + row.astype(int)].reshape(-1, 2, 3)
+    Xp, Xe, Xm = knots(plain), knots(every), knots(mixed)
+    span = plain.span
+    assert np.array_equal(Xm[both == 2], Xp[both == 2]) and np.array_equal(Xm[both == 4], Xe[both == 4])
+    assert np.abs(Xm[both == 3] - 0.5 * (Xp[both == 3] + Xe[both == 3])).max() < 1e-14 * span
+    assert np.abs(Xp[both == 3] - Xe[both == 3]).max() > 1e-3 * span
+
+
+def test_hybrid_pieces_refused_and_one_group_accepted():
     SC.scene_from_dict(scene_json("v5dome_cm"))              # its second group equals the first
+    d = scene_json("dualv5dome_cm_hybrid")
+    d["hybrid"]["pieces"] = {"3:1": 2}
+    with pytest.raises(NotImplementedError):
+        SC.scene_from_dict(d)
     d = scene_json("dome_basic")
     d["anchors"].append({"select": {"type": "region", "id": [0, 0], "radius": 10.0}, "fix": [1, 1, 1, 0, 0, 0]})
     with pytest.raises(NotImplementedError):
         SC.scene_from_dict(d)
     with pytest.raises(ValueError):
         SC.scene_from_dict({"version": 2})
-    # a group that leaves a key out takes its default, not the scene's value; an unassigned group is not used
+    # a group that leaves a key out takes its default, not the scene's value
     d = scene_json("v5dome_cm")
-    del d["hybrid"]["groups"][0]["section"]["width"]                  # default 60 ≠ the scene's 2.0
-    with pytest.raises(NotImplementedError):
-        SC.scene_from_dict(d)
-    d["hybrid"]["assign"] = {}
-    SC.scene_from_dict(d)
+    del d["hybrid"]["groups"][0]["section"]["width"]
+    sc, _, _, _, _ = SC.scene_from_dict(d)
+    assert sc.groups[0].width == 60.0 and sc.c.grp.width == 2.0 and len(sc.assign) > 0
 
 
 def test_binary_and_ascii_stl_agree():
@@ -125,7 +217,7 @@ def test_binary_and_ascii_stl_agree():
     assert len(Vo) == 3 * 696 and Fo.min() == 0 and Fo.max() == 695
 
 
-@pytest.mark.parametrize("name", SCENES)
+@pytest.mark.parametrize("name", SCENES + ["dualv5dome_cm_hybrid"])
 def test_oracle_relaxes_every_scene(name):
     m = SC.load_scene(os.path.join(HERE, name + ".json"))
     o = Oracle(m)

@@ -209,6 +209,7 @@ struct Topology {
   const Mesh* mesh = nullptr;
   int subdivisions = 4; double overhang = 0.0; bool halfHeight = true; int seedPhase = 0;
   double thickness = 30.0;
+  std::vector<double> strandH;                // hybrid weave: the height of each strand's lath group; empty otherwise
   std::vector<Strand> strands;
   std::vector<int> knotOrder;                 // edge ids in insertion order (topology.js Map order)
   std::map<int, Knot> knots;                  // edge id → knot
@@ -381,7 +382,10 @@ void buildPolylines(Topology& topo) {
       int e = st.edges[k];
       Nn[k] = en(e);
       sign[k] = isOverFrame(topo.knots.at(e), s, k) ? 1.0 : -1.0;
-      Kp[k] = add3(mid(e), scl3(Nn[k], 0.5 * h * sign[k]));
+      // a hybrid weave lifts each knot by the mean height of its two laths' groups (topology.js:225)
+      const Knot& kn = topo.knots.at(e);
+      const double he = (!topo.strandH.empty() && topo.halfHeight) ? 0.5 * (topo.strandH[kn.as] + topo.strandH[kn.bs]) : h;
+      Kp[k] = add3(mid(e), scl3(Nn[k], 0.5 * he * sign[k]));
       K0[k] = mid(e);
       off[k] = (0.5 * h) * sign[k];
     }
@@ -503,6 +507,13 @@ struct Setup {
   wsm_prep prep{};
   bool hasPrep = false;
   double meshArea = 0;     // area of the welded mesh (a UDL per area spreads over it); 0 without a mesh
+  std::vector<wsm_group> extra;        // lath groups 2, 3, … of a hybrid weave; group 1 is sc.grp (scene.js:53-56)
+  std::vector<int32_t> strandGroup;    // the group of each strand; empty: every strand in group 1
+  // scene.js:57-61: an assignment outside 2 … 1 + #extra falls back to group 1
+  const wsm_group& groupOfStrand(int s) const {
+    const int g = s < int(strandGroup.size()) ? strandGroup[s] : 1;
+    return (g >= 2 && g - 2 < int(extra.size())) ? extra[g - 2] : sc.grp;
+  }
 };
 
 // Σ of the triangle areas (Mesh.area, mesh.js:50-58)
@@ -737,15 +748,27 @@ std::unique_ptr<Model> buildModel(const Built& tp, const Setup& su) {
   }
   md->X0 = md->X;
 
-  // ---- strandStiffness (model.js:236-252), one lath group
-  Stiff k = groupStiffness(sc.grp);
-  md->EA.assign(m, k.EA); md->GA.assign(m, k.GA); md->EIx.assign(m, k.EIx); md->EIy.assign(m, k.EIy); md->GJ.assign(m, k.GJ);
+  // the strand of every segment and node, whose lath group sets its section, stiffness and weight
+  std::vector<int> segS(m), nodeS(n);
+  for (int s = 0; s < S; s++) {
+    for (int j = md->segOffset[s]; j < md->segOffset[s + 1]; j++) segS[j] = s;
+    for (int i = md->nodeOffset[s]; i < md->nodeOffset[s + 1]; i++) nodeS[i] = s;
+  }
+  // ---- strandStiffness (model.js:236-252): each segment takes its strand's group, each joint the activation of the
+  // group of its first segment
+  md->EA.resize(m); md->GA.resize(m); md->EIx.resize(m); md->EIy.resize(m); md->GJ.resize(m);
+  std::vector<Stiff> segK(m);
+  for (int j = 0; j < m; j++) {
+    const Stiff k = groupStiffness(su.groupOfStrand(segS[j]));
+    segK[j] = k; md->EA[j] = k.EA; md->GA[j] = k.GA; md->EIx[j] = k.EIx; md->EIy[j] = k.EIy; md->GJ[j] = k.GJ;
+  }
   // ---- CosseratRod (rod_cosserat.js:49-72): Ω0 = (1 − act)·asBuilt(Q0)
-  double act[3] = {k.act[0], k.act[1], k.act[2]};
-  if (sc.prestress_asbuilt) act[0] = act[1] = act[2] = 0.0;
-  if (p && (act[0] < 1.0 || act[1] < 1.0 || act[2] < 1.0)) {
-    double psi[3];
-    for (int j = 0; j < p; j++) {
+  for (int j = 0; j < p; j++) {
+    const Stiff& k = segK[md->jSeg0[j]];
+    double act[3] = {k.act[0], k.act[1], k.act[2]};
+    if (sc.prestress_asbuilt) act[0] = act[1] = act[2] = 0.0;
+    if (act[0] < 1.0 || act[1] < 1.0 || act[2] < 1.0) {
+      double psi[3];
       relRotation(&md->Q0[4 * md->jSeg0[j]], &md->Q0[4 * md->jSeg1[j]], psi);
       for (int c = 0; c < 3; c++) md->omega0[3 * j + c] = (1.0 - act[c]) * (psi[c] / md->jLen[j]);
     }
@@ -760,8 +783,10 @@ std::unique_ptr<Model> buildModel(const Built& tp, const Setup& su) {
     double gl = std::max(std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]), 1e-12);
     for (double& v : g) v /= gl;
     double L = sc.length_to_m, F = sc.force_to_n;
-    double areaSI = sc.grp.width * sc.grp.height * L * L, qSI = sc.grp.density * areaSI * 9.80665, qline = qSI / F * L;  // units.js:21-25
     for (int j = 0; j < m; j++) {
+      // self-weight of the segment's group (model.js:203-211; units.js:21-25)
+      const wsm_group& gr = su.groupOfStrand(segS[j]);
+      const double areaSI = gr.width * gr.height * L * L, qSI = gr.density * areaSI * 9.80665, qline = qSI / F * L;
       double q = 0.5 * md->l0[j] * qline;
       for (int i : {md->segI0[j], md->segI1[j]}) for (int c = 0; c < 3; c++) md->loadConst[3 * i + c] += q * g[c];
     }
@@ -830,13 +855,20 @@ std::unique_ptr<Model> buildModel(const Built& tp, const Setup& su) {
   par.contacts = (sc.contacts && K > 0) ? 1 : 0;
   if (par.dent_after < 0) par.dent_after = par.warmup_iter + par.ramp_iter;   // solver_cpu.js:686
   md->kStrand.assign(2 * K, 0); md->kSegOff.assign(2 * K, 0); md->kNseg.assign(2 * K, 0);
-  md->kwk.assign(2 * K, sc.grp.width); md->khk.assign(2 * K, sc.grp.height);
-  md->kfc90.assign(2 * K, sc.grp.fc90); md->kE90.assign(2 * K, sc.grp.E90); md->kHs.assign(2 * K, sc.grp.H);
-  md->kw.assign(K, sc.grp.width); md->kh.assign(K, sc.grp.height);
+  md->kwk.assign(2 * K, 0); md->khk.assign(2 * K, 0);
+  md->kfc90.assign(2 * K, 0); md->kE90.assign(2 * K, 0); md->kHs.assign(2 * K, 0);
+  md->kw.assign(K, 0); md->kh.assign(K, 0);
   md->kEnabled.assign(K, 1); md->kLocked.assign(K, sc.locked ? 1 : 0);
   for (int c = 0; c < K; c++) {
     md->kStrand[2 * c] = tp.knots[c].as; md->kStrand[2 * c + 1] = tp.knots[c].bs;
-    for (int side = 0; side < 2; side++) { int s = md->kStrand[2 * c + side]; md->kSegOff[2 * c + side] = md->segOffset[s]; md->kNseg[2 * c + side] = md->segOffset[s + 1] - md->segOffset[s]; }
+    for (int side = 0; side < 2; side++) {
+      const int i = 2 * c + side, s = md->kStrand[i];
+      md->kSegOff[i] = md->segOffset[s]; md->kNseg[i] = md->segOffset[s + 1] - md->segOffset[s];
+      // each side's section and material: its strand's group (solver_cpu.js:72-78)
+      const wsm_group& gr = su.groupOfStrand(s);
+      md->kwk[i] = gr.width; md->khk[i] = gr.height; md->kfc90[i] = gr.fc90; md->kE90[i] = gr.E90; md->kHs[i] = gr.H;
+    }
+    md->kw[c] = 0.5 * (md->kwk[2 * c] + md->kwk[2 * c + 1]); md->kh[c] = 0.5 * (md->khk[2 * c] + md->khk[2 * c + 1]);
     if (tp.overhang <= 0 && tp.knots[c].isBoundary) md->kEnabled[c] = 0;
     md->kw[c] = 0.5 * (md->kwk[2 * c] + md->kwk[2 * c + 1]); md->kh[c] = 0.5 * (md->khk[2 * c] + md->khk[2 * c + 1]);
   }
@@ -892,7 +924,8 @@ std::unique_ptr<Model> buildModel(const Built& tp, const Setup& su) {
   md->lRef = std::isfinite(lmin) ? lmin : 1.0;
   md->par = par;
   // floor goal sections and stick points (Floor constructor, solver_cpu.js:536-554)
-  md->nodeW.assign(n, sc.grp.width); md->nodeH.assign(n, sc.grp.height);
+  md->nodeW.resize(n); md->nodeH.resize(n);
+  for (int i = 0; i < n; i++) { const wsm_group& gr = su.groupOfStrand(nodeS[i]); md->nodeW[i] = gr.width; md->nodeH[i] = gr.height; }
   md->floorA.assign(2 * n, 0);
   for (int i = 0; i < n; i++) { md->floorA[2 * i] = md->X[3 * i]; md->floorA[2 * i + 1] = md->X[3 * i + 1]; }
   // outputs
@@ -970,20 +1003,34 @@ WS_API void wsm_scene_defaults(wsm_scene* sc) {
   sc->udl = 0;
 }
 
-WS_API int wsm_from_mesh(const double* V, int32_t nV, const int32_t* F, int32_t nF, int32_t do_weld,
-                         const wsm_scene* sc, const wsm_anchor* anchors, int32_t n_anchors,
-                         const wsm_point_load* loads, int32_t n_loads, const int32_t* flips, int32_t n_flips,
-                         const wsm_prep* prep, wsm_model** out) {
+namespace {
+int fromMesh(const double* V, int32_t nV, const int32_t* F, int32_t nF, int32_t do_weld, const wsm_scene* sc,
+             const wsm_anchor* anchors, int32_t n_anchors, const wsm_point_load* loads, int32_t n_loads,
+             const int32_t* flips, int32_t n_flips, const wsm_prep* prep, bool hybrid, const wsm_group* groups,
+             int32_t n_groups, const int32_t* assign, int32_t n_assign, wsm_model** out) {
   try {
     if (!V || !F || !sc || !out || nV <= 0 || nF <= 0) throw std::runtime_error("wsm_from_mesh: bad arguments");
+    if (hybrid && ((n_groups > 0 && !groups) || (n_assign > 0 && !assign) || n_groups < 0 || n_assign < 0))
+      throw std::runtime_error("wsm_from_mesh_groups: bad group arguments");
     std::vector<double> Vv(V, V + 3 * size_t(nV)); std::vector<int32_t> Fv(F, F + 3 * size_t(nF));
     if (do_weld) weld(Vv, Fv);
     Mesh mesh = meshFromArrays(Vv, Fv);
     Topology topo; topo.mesh = &mesh;
     topo.subdivisions = sc->subdivisions; topo.overhang = sc->overhang; topo.halfHeight = sc->offset_half_height != 0;
     topo.seedPhase = sc->seed_strand_phase; topo.thickness = sc->grp.height;
-    traceStrands(topo); assignParity(topo); applyFlips(topo, flips, n_flips); buildPolylines(topo); consistentFrames(topo);
+    traceStrands(topo); assignParity(topo); applyFlips(topo, flips, n_flips);
     Setup su; su.sc = *sc;
+    if (hybrid) {
+      su.extra.assign(groups, groups + n_groups);
+      su.strandGroup.assign(topo.nStrands(), 1);
+      for (int a = 0; a < n_assign; a++) {
+        const int s = assign[2 * a];
+        if (s >= 0 && s < topo.nStrands()) su.strandGroup[s] = assign[2 * a + 1];
+      }
+      topo.strandH.resize(topo.nStrands());
+      for (int s = 0; s < topo.nStrands(); s++) topo.strandH[s] = su.groupOfStrand(s).height;
+    }
+    buildPolylines(topo); consistentFrames(topo);
     su.anchors.assign(anchors, anchors + n_anchors); su.loads.assign(loads, loads + n_loads);
     if (prep) { su.prep = *prep; su.hasPrep = true; }
     su.meshArea = meshArea(mesh);
@@ -993,6 +1040,24 @@ WS_API int wsm_from_mesh(const double* V, int32_t nV, const int32_t* F, int32_t 
     *out = new wsm_model{std::move(md)};
     return WS_OK;
   } catch (const std::exception& e) { g_err = e.what(); return WS_ERR_ARG; }
+}
+}  // namespace
+
+WS_API int wsm_from_mesh(const double* V, int32_t nV, const int32_t* F, int32_t nF, int32_t do_weld,
+                         const wsm_scene* sc, const wsm_anchor* anchors, int32_t n_anchors,
+                         const wsm_point_load* loads, int32_t n_loads, const int32_t* flips, int32_t n_flips,
+                         const wsm_prep* prep, wsm_model** out) {
+  return fromMesh(V, nV, F, nF, do_weld, sc, anchors, n_anchors, loads, n_loads, flips, n_flips, prep, false, nullptr,
+                  0, nullptr, 0, out);
+}
+
+WS_API int wsm_from_mesh_groups(const double* V, int32_t nV, const int32_t* F, int32_t nF, int32_t do_weld,
+                                const wsm_scene* sc, const wsm_anchor* anchors, int32_t n_anchors,
+                                const wsm_point_load* loads, int32_t n_loads, const int32_t* flips, int32_t n_flips,
+                                const wsm_prep* prep, const wsm_group* groups, int32_t n_groups,
+                                const int32_t* strand_group, int32_t n_assign, wsm_model** out) {
+  return fromMesh(V, nV, F, nF, do_weld, sc, anchors, n_anchors, loads, n_loads, flips, n_flips, prep, true, groups,
+                  n_groups, strand_group, n_assign, out);
 }
 
 WS_API int wsm_from_polylines(const double* P, const int32_t* counts, int32_t n_strands, const int32_t* knot_rows,

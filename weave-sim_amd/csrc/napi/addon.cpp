@@ -154,13 +154,25 @@ void get3(napi_env env, napi_value o, const char* k, double* out) {
   }
 }
 
+// a lath group's material, section and beam factors, over what g holds
+void group_from(napi_env env, napi_value o, wsm_group& g) {
+  if (!is_object(env, o)) return;
+#define D(field, key) g.field = getd(env, o, key, g.field)
+  D(E, "E"); D(G, "G"); D(density, "density"); D(width, "width"); D(height, "height");
+  D(E90, "E90"); D(fc90, "fc90"); D(H, "H");
+  D(Ix, "Ix"); D(Iy, "Iy"); D(J, "J"); g.beam_advanced = int32_t(getd(env, o, "beam_advanced", g.beam_advanced));
+  D(activation, "activation"); D(stiffness, "stiffness"); D(act_flat, "act_flat"); D(act_edge, "act_edge");
+  D(act_twist, "act_twist"); D(k_axial, "k_axial"); D(k_flat, "k_flat"); D(k_edge, "k_edge");
+  D(k_twist, "k_twist"); D(k_shear, "k_shear");
+#undef D
+}
+
 void scene_from(napi_env env, napi_value o, wsm_scene& sc) {
   wsm_scene_defaults(&sc);
   if (!is_object(env, o)) return;
+  group_from(env, o, sc.grp);
 #define D(field, key) sc.field = getd(env, o, key, sc.field)
 #define I(field, key) sc.field = int32_t(getd(env, o, key, sc.field))
-  D(grp.E, "E"); D(grp.G, "G"); D(grp.density, "density"); D(grp.width, "width"); D(grp.height, "height");
-  D(grp.E90, "E90"); D(grp.fc90, "fc90"); D(grp.H, "H");
   D(par.mass_scale, "mass_scale"); D(par.damping, "damping"); D(par.tol, "tol"); D(par.mu, "mu");
   D(par.strength, "strength"); D(par.stick_strength, "stick_strength"); D(par.torsion_factor, "torsion_factor");
   D(par.epsilon, "epsilon"); D(par.slip_rate, "slip_rate"); D(par.rot_factor, "rot_factor"); D(par.dent_rate, "dent_rate");
@@ -169,12 +181,7 @@ void scene_from(napi_env env, napi_value o, wsm_scene& sc) {
   I(par.ramp_iter, "ramp_iter"); I(par.rot_friction, "rot_friction"); I(par.seat_law, "seat_law"); I(par.seat_gap, "seat_gap");
   I(par.dent_after, "dent_after"); I(par.augmented, "augmented"); I(par.al_every, "al_every");
   I(contacts, "contacts"); I(locked, "locked"); I(gravity, "gravity");
-  // the rest of a scene (scene.json through the JS reader): section constants, beam factors, ground, weave, units,
-  // UDL; vectors as 3-element arrays
-  D(grp.Ix, "Ix"); D(grp.Iy, "Iy"); D(grp.J, "J"); I(grp.beam_advanced, "beam_advanced");
-  D(grp.activation, "activation"); D(grp.stiffness, "stiffness"); D(grp.act_flat, "act_flat"); D(grp.act_edge, "act_edge");
-  D(grp.act_twist, "act_twist"); D(grp.k_axial, "k_axial"); D(grp.k_flat, "k_flat"); D(grp.k_edge, "k_edge");
-  D(grp.k_twist, "k_twist"); D(grp.k_shear, "k_shear");
+  // the rest of a scene (scene.json through the JS reader): ground, weave, units, UDL; vectors as 3-element arrays
   I(par.floor_on, "floor_on"); D(par.floor_z, "floor_z"); D(par.floor_mu, "floor_mu");
   I(prestress_asbuilt, "prestress_asbuilt"); I(subdivisions, "subdivisions"); D(overhang, "overhang");
   I(offset_half_height, "offset_half_height"); I(seed_strand_phase, "seed_strand_phase");
@@ -227,7 +234,7 @@ napi_value ModelHexDome(napi_env env, napi_callback_info info) {
 // modelFromMesh(V: Float64Array [3nV], F: Int32Array [3nF], scene, anchors: [{type, s, k, fix[6]}],
 //               loads: [{s, k, F[3], local}], flips: [[s, k], ...], weld) — buildTopology … _buildGoals on a mesh
 napi_value ModelFromMesh(napi_env env, napi_callback_info info) {
-  napi_value a[7]; get_args(env, info, 7, a);
+  napi_value a[9]; get_args(env, info, 9, a);
   size_t nV = 0, nF = 0;
   const double* V = typed<double>(env, a[0], napi_float64_array, &nV);
   const int32_t* F = typed<int32_t>(env, a[1], napi_int32_array, &nF);
@@ -263,8 +270,25 @@ napi_value ModelFromMesh(napi_env env, napi_callback_info info) {
   bool weld = true; napi_valuetype wt; napi_typeof(env, a[6], &wt);
   if (wt == napi_boolean) napi_get_value_bool(env, a[6], &weld);
   wsm_model* h = nullptr;
-  int rc = wsm_from_mesh(V, int32_t(nV / 3), F, int32_t(nF / 3), weld ? 1 : 0, &sc, an.data(), int32_t(an.size()),
-                         ld.data(), int32_t(ld.size()), fl.data(), int32_t(fl.size() / 2), nullptr, &h);
+  int rc;
+  bool hybrid = false;
+  napi_is_array(env, a[7], &hybrid);
+  if (hybrid) {
+    // a hybrid weave: lath groups 2, 3, … (objects with the scene's group keys) and [strand, group] pairs
+    std::vector<wsm_group> gr(array_len(env, a[7]));
+    for (uint32_t i = 0; i < gr.size(); i++) { wsm_scene d; wsm_scene_defaults(&d); gr[i] = d.grp; group_from(env, elem(env, a[7], i), gr[i]); }
+    std::vector<int32_t> asg;
+    for (uint32_t i = 0, n = array_len(env, a[8]); i < n; i++) {
+      napi_value e = elem(env, a[8], i);
+      asg.push_back(int32_t(num_at(env, e, 0))); asg.push_back(int32_t(num_at(env, e, 1)));
+    }
+    rc = wsm_from_mesh_groups(V, int32_t(nV / 3), F, int32_t(nF / 3), weld ? 1 : 0, &sc, an.data(), int32_t(an.size()),
+                              ld.data(), int32_t(ld.size()), fl.data(), int32_t(fl.size() / 2), nullptr, gr.data(),
+                              int32_t(gr.size()), asg.data(), int32_t(asg.size() / 2), &h);
+  } else {
+    rc = wsm_from_mesh(V, int32_t(nV / 3), F, int32_t(nF / 3), weld ? 1 : 0, &sc, an.data(), int32_t(an.size()),
+                       ld.data(), int32_t(ld.size()), fl.data(), int32_t(fl.size() / 2), nullptr, &h);
+  }
   if (rc) { napi_throw_error(env, nullptr, wsm_last_error()); return nullptr; }
   return wrap_model(env, h);
 }

@@ -3,8 +3,9 @@
 // Node.js host: Scene.fromJSON (web/js/scene.js:136-187) — every section from the params.js defaults plus the keys
 // the file holds, E90 / fc90 / H of older files from their N/mm² defaults in the scene's stress unit — and
 // Mesh.fromBytes (web/js/mesh.js:24-31): STL (binary when the size says so, else ASCII) or OBJ (fan-triangulated),
-// scaled, then welded by the model builder.  Refused with an error rather than approximated: lath groups with their
-// own material / section / beam (hybrid weaves) and the 'region' anchor selector.
+// scaled, then welded by the model builder.  A hybrid weave's lath groups and strand assignments (scene.js:50-62) go
+// to the builder with the scene.  Refused with an error rather than approximated: groups assigned to pieces of a
+// discretisation (hybrid.pieces) unless every group in use equals the scene's own, and the 'region' anchor selector.
 //
 //   const S = require('./scene_mi355x.js');
 //   const m = S.loadScene('examples/scenes/dome_basic.json');        // a W.Model, ready for new W.GpuSolver(...)
@@ -103,7 +104,31 @@ function hybridIsOneGroup(d, stress) {
   return used.every((g) => g < 2 || g > 1 + groups.length || sameGroup(d, groups[g - 2], stress));
 }
 
-/** { scene, anchors, loads, flips, mesh } of a scene.json object: the scene as the addon's builder reads it */
+/** the builder's keys of a material / section / beam, over `into` */
+function fillGroup(into, mat, sec, beam) {
+  for (const k of ['E', 'G', 'density', 'E90', 'fc90', 'H']) if (has(mat, k)) into[k] = +mat[k];
+  for (const k of ['width', 'height']) if (has(sec, k)) into[k] = +sec[k];
+  for (const k of ['Ix', 'Iy', 'J']) into[k] = has(sec, k) ? +sec[k] : 0;
+  into.beam_advanced = flag(beam.mode === 'advanced');
+  const bmap = { activation: 'activation', stiffness: 'stiffness', activation_flat: 'act_flat', activation_edge: 'act_edge',
+    activation_twist: 'act_twist', stiffness_axial: 'k_axial', stiffness_flat: 'k_flat', stiffness_edge: 'k_edge',
+    stiffness_twist: 'k_twist', stiffness_shear: 'k_shear' };
+  for (const k of Object.keys(bmap)) if (has(beam, k)) into[bmap[k]] = +beam[k];
+  return into;
+}
+
+/** groups 2, 3, … and [strand, group] pairs of an enabled hybrid weave, or null: each group from the params.js
+ *  defaults (fromDict, scene.js:167), its E90 / fc90 / H as given (not rescaled) */
+function hybridGroups(d) {
+  const h = d.hybrid || {};
+  if (!h.enabled) return null;
+  const groups = (h.groups || []).map((g) => fillGroup(fillGroup({}, DEFAULTS.material, DEFAULTS.section, DEFAULTS.beam),
+    g.material || {}, g.section || {}, g.beam || {}));
+  const assign = Object.entries(h.assign || {}).map(([k, v]) => [+k | 0, +v | 0]);
+  return { groups, assign };
+}
+
+/** { scene, anchors, loads, flips, mesh, hybrid } of a scene.json object: the scene as the addon's builder reads it */
 function sceneFromJSON(d) {
   d = d || {};
   const v = has(d, 'version') ? d.version : SCENE_VERSION;
@@ -112,21 +137,14 @@ function sceneFromJSON(d) {
   const lu = units.length || 'mm', fu = units.force || 'N';
   if (!(lu in LENGTH_TO_M) || !(fu in FORCE_TO_N)) throw new Error(`unknown units ${lu}, ${fu}`);
   const L = LENGTH_TO_M[lu], Fn = FORCE_TO_N[fu];
-  if (!hybridIsOneGroup(d, Fn / (L * L) / 1e6)) throw new Error('lath groups with their own material, section or beam (hybrid weaves) are not modelled by the builder');
+  const stress = Fn / (L * L) / 1e6;
+  const hy = d.hybrid || {};
+  const pieces = Object.entries(hy.pieces || {}).filter(([k, v]) => /^\d+:\d+$/.test(String(k)) && (v | 0) >= 2);   // scene.js:11, 169
+  if (hy.enabled && pieces.length && !hybridIsOneGroup(d, stress)) throw new Error('lath groups assigned to pieces of a discretisation (hybrid.pieces) are not modelled by the builder');
   const sc = { length_to_m: L, force_to_n: Fn };
   const mat = d.material || {};
-  for (const k of ['E', 'G', 'density', 'E90', 'fc90', 'H']) if (has(mat, k)) sc[k] = +mat[k];
-  const stress = Fn / (L * L) / 1e6;
+  fillGroup(sc, mat, d.section || {}, d.beam || {});
   for (const k of Object.keys(MATERIAL_MM_N)) if (!(k in mat)) sc[k] = MATERIAL_MM_N[k] / stress;
-  const sec = d.section || {};
-  for (const k of ['width', 'height']) if (has(sec, k)) sc[k] = +sec[k];
-  for (const k of ['Ix', 'Iy', 'J']) sc[k] = has(sec, k) ? +sec[k] : 0;
-  const beam = d.beam || {};
-  sc.beam_advanced = flag(beam.mode === 'advanced');
-  const bmap = { activation: 'activation', stiffness: 'stiffness', activation_flat: 'act_flat', activation_edge: 'act_edge',
-    activation_twist: 'act_twist', stiffness_axial: 'k_axial', stiffness_flat: 'k_flat', stiffness_edge: 'k_edge',
-    stiffness_twist: 'k_twist', stiffness_shear: 'k_shear' };
-  for (const k of Object.keys(bmap)) if (has(beam, k)) sc[bmap[k]] = +beam[k];
   const sol = d.solver || {};
   for (const k of ['mass_scale', 'damping', 'tol', 'warmup_iter', 'max_iter']) if (has(sol, k)) sc[k] = +sol[k];
   if (has(sol, 'kinetic_damping')) sc.kinetic_damping = flag(sol.kinetic_damping);
@@ -169,14 +187,16 @@ function sceneFromJSON(d) {
     else throw new Error(`unknown selector type ${t}`);
   }
   const mesh = d.mesh || {};
-  return { scene: sc, anchors, loads, flips, mesh: { path: mesh.path || '', scale: has(mesh, 'scale') ? +mesh.scale : 1.0 } };
+  return { scene: sc, anchors, loads, flips, mesh: { path: mesh.path || '', scale: has(mesh, 'scale') ? +mesh.scale : 1.0 },
+    hybrid: hybridGroups(d) };
 }
 
 /** the host model of a scene.json object and its mesh file's bytes */
 function modelFromScene(d, meshBytes, meshName) {
   const s = sceneFromJSON(d);
   const { V, F } = readMesh(meshBytes, meshName || s.mesh.path, s.mesh.scale);
-  return Model.fromMesh(V, F, s.scene, { anchors: s.anchors, loads: s.loads, flips: s.flips, weld: true });
+  return Model.fromMesh(V, F, s.scene, { anchors: s.anchors, loads: s.loads, flips: s.flips, weld: true,
+    groups: s.hybrid ? s.hybrid.groups : null, assign: s.hybrid ? s.hybrid.assign : [] });
 }
 
 /** the host model of a scene.json file; the mesh path inside it is relative to the file's own folder */

@@ -45,9 +45,11 @@ class Model {
   /** a model on a triangle mesh (buildTopology … CpuSolver._buildGoals): V Float64Array [3nV], F Int32Array [3nF],
    *  scene fields as the addon names them, anchors [{ type, s, k, fix }], point loads [{ s, k, F, local }], flipped
    *  knots [[s, k], ...]; duplicate vertices are welded unless weld is false (scene_mi355x.js reads scene.json) */
+  /** opts: anchors, loads, flips, weld, and for a hybrid weave groups (lath groups 2, 3, … with the scene's group
+   *  keys) and assign ([strand, group] pairs; strands left out are in group 1, the scene's own) */
   static fromMesh(V, F, scene = {}, opts = {}) {
     return new Model(native().modelFromMesh(Float64Array.from(V), Int32Array.from(F), scene, opts.anchors || [],
-      opts.loads || [], opts.flips || [], opts.weld !== false));
+      opts.loads || [], opts.flips || [], opts.weld !== false, opts.groups || null, opts.assign || []));
   }
   /** md.protocol of the reference (solver_gpu.js:256-318, calibrate.js:345-365): { mode: 'force' | 'displacement' |
    *  null, probe, axis, sched, settle, avg }; the schedule is used as given (flatten the load cycles and drop

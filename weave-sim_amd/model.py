@@ -111,7 +111,10 @@ class Model:
         return cls._check(A.model_lib().wsm_hex_dome(C.byref(spec), C.byref(sc.c), C.byref(out)), out)
 
     @classmethod
-    def from_mesh(cls, V, F, scene=None, anchors=(), loads=(), flips=(), noise=0.0, seed=1, weld=True, wavelength=0.0):
+    def from_mesh(cls, V, F, scene=None, anchors=(), loads=(), flips=(), noise=0.0, seed=1, weld=True, wavelength=0.0,
+                  groups=None, assign=()):
+        """Kagome weave of a triangle mesh.  A hybrid weave passes `groups` (wsm_group of lath groups 2, 3, …) and
+        `assign` = [(strand, group), ...]; strands left out are in group 1, the scene's own (scene.js:50-62)."""
         sc = scene or Scene()
         V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1)
         F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1)
@@ -120,9 +123,14 @@ class Model:
         fl = np.ascontiguousarray(np.array(flips, dtype=np.int32).reshape(-1))
         prep = A.wsm_prep(noise, wavelength, seed, 0)
         out = C.c_void_p()
-        rc = A.model_lib().wsm_from_mesh(V.ctypes.data_as(A.P_d), len(V) // 3, F.ctypes.data_as(A.P_i32), len(F) // 3,
-                                         1 if weld else 0, C.byref(sc.c), an, len(anchors), ld, len(loads),
-                                         fl.ctypes.data_as(A.P_i32), len(fl) // 2, C.byref(prep), C.byref(out))
+        head = (V.ctypes.data_as(A.P_d), len(V) // 3, F.ctypes.data_as(A.P_i32), len(F) // 3, 1 if weld else 0,
+                C.byref(sc.c), an, len(anchors), ld, len(loads), fl.ctypes.data_as(A.P_i32), len(fl) // 2, C.byref(prep))
+        if groups is None:
+            return cls._check(A.model_lib().wsm_from_mesh(*head, C.byref(out)), out)
+        gr = (A.wsm_group * max(len(groups), 1))(*groups)
+        asg = np.ascontiguousarray(np.array(list(assign), dtype=np.int32).reshape(-1))
+        rc = A.model_lib().wsm_from_mesh_groups(*head, gr, len(groups), asg.ctypes.data_as(A.P_i32), len(asg) // 2,
+                                                C.byref(out))
         return cls._check(rc, out)
 
     @classmethod
@@ -208,7 +216,8 @@ class Model:
                  "load_const": 3 * self.n, "node_free": 3 * self.n, "k_strand": 2 * self.K, "k_seg_off": 2 * self.K,
                  "k_nseg": 2 * self.K, "k_wk": 2 * self.K, "k_hk": 2 * self.K, "k_w": self.K, "k_h": self.K,
                  "k_enabled": self.K, "k_locked": self.K, "c_seg": self.md.C, "c_len": self.md.C, "c_B": 3 * self.md.C,
-                 "c_Q": 4 * self.md.C, "k_fc90": 2 * self.K, "k_E90": 2 * self.K, "k_Hs": 2 * self.K}
+                 "c_Q": 4 * self.md.C, "k_fc90": 2 * self.K, "k_E90": 2 * self.K, "k_Hs": 2 * self.K,
+                 "node_w": self.n, "node_h": self.n}
         ptr = getattr(self.md, name)
         t = type(ptr)
         dt = {A.P_d: np.float64, A.P_u8: np.uint8, A.P_i32: np.int32}[t]

@@ -7,10 +7,12 @@ else ASCII, parseSTL :62-86) or OBJ (fan-triangulated polygons, parseOBJ :88-101
 builder.  The instance is then built by wsm_from_mesh — the reference's buildTopology / buildState / Anchors / Loads
 / CpuSolver._buildGoals pipeline — exactly as for a mesh given directly.
 
-Not modelled by the builder, and refused with an error rather than approximated: lath groups that differ from the
-scene's own material / section / beam (hybrid weaves; strands assigned to a group equal to the scene's own are
-accepted),
-and the 'region' anchor selector (graph distance on the mesh, model.js:127-145).
+A hybrid weave's lath groups (Scene.hybrid, scene.js:50-62) go to wsm_from_mesh_groups with the strand assignments:
+each strand's segments, crossing sides, nodes and knot lifts take its group's material, section and beam.
+
+Not modelled by the builder, and refused with an error rather than approximated: groups assigned to pieces of a
+discretisation (Scene.hybrid.pieces) unless every group in use equals the scene's own, and the 'region' anchor
+selector (graph distance on the mesh, model.js:127-145).
 """
 import json
 import os
@@ -19,6 +21,7 @@ import struct
 
 import numpy as np
 
+from . import _abi as A
 from .model import Model, Scene
 
 SCENE_VERSION = 1                                   # scene.js:13
@@ -113,6 +116,48 @@ def _groups_match(d, group, stress):
     return True
 
 
+def _int0(v):
+    """JavaScript's `v | 0` of a key or value as the reader takes it (scene.js:168): 0 where it is not a number"""
+    try:
+        return int(float(v))
+    except (TypeError, ValueError):
+        return 0
+
+
+def _fill_group(g, mat, sec, beam):
+    """a wsm_group from the material / section / beam dictionaries, over what g holds already"""
+    for k in ("E", "G", "density", "E90", "fc90", "H"):
+        if k in mat and mat[k] is not None:
+            setattr(g, k, float(mat[k]))
+    for k in ("width", "height"):
+        if k in sec and sec[k] is not None:
+            setattr(g, k, float(sec[k]))
+    for k in ("Ix", "Iy", "J"):
+        setattr(g, k, float(sec[k]) if sec.get(k) is not None else 0.0)
+    g.beam_advanced = _flag(beam.get("mode", "simple") == "advanced")
+    for src, dst in (("activation", "activation"), ("stiffness", "stiffness"), ("activation_flat", "act_flat"),
+                     ("activation_edge", "act_edge"), ("activation_twist", "act_twist"), ("stiffness_axial", "k_axial"),
+                     ("stiffness_flat", "k_flat"), ("stiffness_edge", "k_edge"), ("stiffness_twist", "k_twist"),
+                     ("stiffness_shear", "k_shear")):
+        if src in beam:
+            setattr(g, dst, float(beam[src]))
+    return g
+
+
+def _hybrid_groups(d):
+    """(groups 2, 3, … as wsm_group, [(strand, group)]) of an enabled Scene.hybrid, or None: each group starts from
+    the params.js defaults (fromDict, scene.js:167), its E90 / fc90 / H in N/mm² as given (not rescaled)"""
+    h = _sec(d, "hybrid")
+    if not h.get("enabled"):
+        return None
+    groups = []
+    for gd in h.get("groups") or []:
+        g = _fill_group(A.wsm_group(), _DEFAULTS["material"], _DEFAULTS["section"], _DEFAULTS["beam"])
+        groups.append(_fill_group(g, _sec(gd, "material"), _sec(gd, "section"), _sec(gd, "beam")))
+    assign = [(_int0(k), _int0(v)) for k, v in (h.get("assign") or {}).items()]
+    return groups, assign
+
+
 def _hybrid_is_one_group(d, stress):
     """no strand or piece of a discretisation is assigned to a lath group that differs from the scene's own
     (scene.js:53-72: group 1 is the scene's material / section / beam, assignments name groups 2 …)"""
@@ -139,33 +184,19 @@ def scene_from_dict(d):
     c = sc.c
     c.length_to_m, c.force_to_n = L, Fn
 
-    if not _hybrid_is_one_group(d, Fn / (L * L) / 1e6):
-        raise NotImplementedError("lath groups with their own material, section or beam (hybrid weaves) are not "
+    stress = Fn / (L * L) / 1e6                    # scene stress unit in N/mm²
+    h = _sec(d, "hybrid")
+    pieces = [k for k, v in (h.get("pieces") or {}).items() if re.match(r"^\d+:\d+$", str(k)) and _int0(v) >= 2]
+    if h.get("enabled") and pieces and not _hybrid_is_one_group(d, stress):     # keys as parsePieceKey, scene.js:11
+        raise NotImplementedError("lath groups assigned to pieces of a discretisation (hybrid.pieces) are not "
                                   "modelled by the builder")
+    sc.groups, sc.assign = _hybrid_groups(d) or (None, [])
 
     mat = _sec(d, "material")
-    g = c.grp
-    for k in ("E", "G", "density", "E90", "fc90", "H"):
-        if k in mat and mat[k] is not None:
-            setattr(g, k, float(mat[k]))
-    stress = Fn / (L * L) / 1e6                    # scene stress unit in N/mm²
+    g = _fill_group(c.grp, mat, _sec(d, "section"), _sec(d, "beam"))
     for k, v0 in _MATERIAL_MM_N.items():
         if k not in mat:
             setattr(g, k, v0 / stress)
-    sec = _sec(d, "section")
-    for k in ("width", "height"):
-        if k in sec and sec[k] is not None:
-            setattr(g, k, float(sec[k]))
-    for k in ("Ix", "Iy", "J"):
-        setattr(g, k, float(sec[k]) if sec.get(k) is not None else 0.0)
-    beam = _sec(d, "beam")
-    g.beam_advanced = _flag(beam.get("mode", "simple") == "advanced")
-    for src, dst in (("activation", "activation"), ("stiffness", "stiffness"), ("activation_flat", "act_flat"),
-                     ("activation_edge", "act_edge"), ("activation_twist", "act_twist"), ("stiffness_axial", "k_axial"),
-                     ("stiffness_flat", "k_flat"), ("stiffness_edge", "k_edge"), ("stiffness_twist", "k_twist"),
-                     ("stiffness_shear", "k_shear")):
-        if src in beam:
-            setattr(g, dst, float(beam[src]))
 
     p = c.par
     sol = _sec(d, "solver")
@@ -259,7 +290,8 @@ def model_from_scene(d, base_dir=".", mesh=None):
         V, F = read_mesh(os.path.join(base_dir, m["path"]), m["scale"])
     else:
         V, F = mesh
-    return Model.from_mesh(V, F, scene=sc, anchors=anchors, loads=loads, flips=flips, weld=True)
+    return Model.from_mesh(V, F, scene=sc, anchors=anchors, loads=loads, flips=flips, weld=True, groups=sc.groups,
+                           assign=sc.assign)
 
 
 __all__ = ["parse_stl", "parse_obj", "read_mesh", "scene_from_dict", "load_scene", "model_from_scene"]
