@@ -174,12 +174,14 @@ console.log(JSON.stringify(res));""")
 
 
 @pytest.mark.gpu
-def test_js_scene_on_the_device_matches_python_path(W, gpu_ctx):
-    """the reference's tree scene (scene.json + STL, augmented-Lagrangian contact, UDL per length) read by the JS
-    reader and relaxed 800 iterations by the JS GpuSolver equals the Python reader + Python path, bitwise"""
+@pytest.mark.parametrize("name", ["tree_cm", "dualv5dome_cm_hybrid"])
+def test_js_scene_on_the_device_matches_python_path(W, gpu_ctx, name):
+    """the reference's tree scene (scene.json + STL, augmented-Lagrangian contact, UDL per length) and its hybrid dual
+    dome (two lath groups: their stiffness, activation and contact sections come through the JS reader too) read by
+    the JS reader and relaxed 800 iterations by the JS GpuSolver equal the Python reader + Python path, bitwise"""
     import importlib
     SC = importlib.import_module("weave-sim_amd.scene")
-    path = os.path.join(ROOT, "tests", "golden", "scenes", "tree_cm.json")
+    path = os.path.join(ROOT, "tests", "golden", "scenes", name + ".json")
     scene_js = os.path.join(ROOT, "weave-sim_amd", "js", "scene_mi355x.js")
     out = run_node(f"""
 (async () => {{
