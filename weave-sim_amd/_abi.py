@@ -5,6 +5,7 @@ sizes the libraries report.
 """
 import ctypes as C
 import os
+import threading
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(HERE, "lib")
@@ -141,8 +142,26 @@ _model_lib = None
 _hip_lib = None
 
 
+_load_lock = threading.Lock()     # the loaders below may be first called from several threads at once
+
+
 def model_lib():
     """The host model builder (libweave_model.so)."""
+    if _model_lib is None:
+        with _load_lock:
+            return _open_model_lib()
+    return _model_lib
+
+
+def hip_lib():
+    """The device solver (libweave_hip.so).  Raises when it is not built: there is no CPU fallback."""
+    if _hip_lib is None:
+        with _load_lock:
+            return _open_hip_lib()
+    return _hip_lib
+
+
+def _open_model_lib():
     global _model_lib
     if _model_lib is None:
         L = _load("libweave_model.so")
@@ -171,8 +190,7 @@ def model_lib():
     return _model_lib
 
 
-def hip_lib():
-    """The device solver (libweave_hip.so).  Raises when it is not built: there is no CPU fallback."""
+def _open_hip_lib():
     global _hip_lib
     if _hip_lib is None:
         L = _load("libweave_hip.so")
