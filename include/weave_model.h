@@ -15,7 +15,8 @@
 #define WEAVE_MODEL_H
 #include "weave.h"
 
-/* lath group: MATERIAL, SECTION and BEAM of params.js:6-19 (one group; hybrid groups are not modelled) */
+/* lath group: MATERIAL, SECTION and BEAM of params.js:6-19 (the scene's own in wsm_scene.grp; the further groups of a
+ * hybrid weave go to wsm_from_mesh_groups) */
 typedef struct wsm_group {
   double E, G, density, E90, fc90, H;      /* N/mm², kg/m³ (density is SI, as units.js) */
   double width, height;                    /* section (mm) */
