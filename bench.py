@@ -1,19 +1,22 @@
 """Benchmark of the MI355X dynamic-relaxation path: node-iterations per second of woven kagome gridshells.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dome_10m] [--precision fp32|fp64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload variants] [--precision fp32|fp64]
 
 A step is one dynamic-relaxation iteration (crossing kernel, strand kernel, control kernel) over the whole input.
-The default workload is the single large instance: a kagome dome of 10 m span, 0.15 m lath spacing, 20 mm
-segments, 12 % boundary contraction, seeded ±2 mm imperfection (78,606 nodes, 9,900 crossings), FP32 forces with
-FP64 positions, frames, kinetic energy and residual.  Other workloads: dome_3m (~1.4k nodes), dome_30m (~0.8M
-nodes, μ = 0.05), variants (a ragged batch of design variants, 2k–50k nodes each).
+The default workload is the batched one: 256 seeded design variants (span 2-8 m, spacing 0.2-0.4 m, mu 0.1-0.5,
+contraction 8-20 %, thickness 3-6 mm, 2k-50k target nodes each; 3.74M nodes in all) relaxed in the same launches,
+strand tile, graph length and crossing layout picked by the autotuner, FP32 forces with FP64 positions, frames,
+kinetic energy and residual.  Other workloads: dome_3m (~1.4k nodes), dome_10m (78,606 nodes), dome_30m (~0.8M
+nodes, mu = 0.05).
 
 Prints one JSON line.  `value` is whole-job node-iterations/s with the inputs resident in HBM.  `roofline` is the
-strand kernel (the dominant launch): algorithmic bytes per launch over its mean duration (HIP events on the
-solver's stream) against the 8 TB/s HBM peak, and `traffic` the HBM bytes per launch read by two rocprofv3
-counter passes (FETCH_SIZE, WRITE_SIZE) scaled by what the same counters read for a plain copy kernel.
-`cpu_baseline` is the f64 oracle (a C++ restatement of the reference's CpuSolver) on this host: one thread on the
-single-instance workloads, every host thread (one instance each, at most 16) in `--mode solves`.
+strand kernel (the dominant launch): algorithmic bytes per launch over its mean duration inside the iteration
+sequence (HIP events between consecutive kernels on the solver's stream) against the 8 TB/s HBM peak, and `traffic`
+the HBM bytes per launch read by two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) scaled by what the same
+counters read for a plain copy kernel.  On the variants workload the line also carries `solves`: the same batch
+relaxed to the stop criterion against the plain single-instance FP64 path.  `cpu_baseline` is the f64 oracle (a
+C++ restatement of the reference's CpuSolver) on this host: every host thread (at most 16), one instance each, with
+the one-thread figure beside it.
 """
 import argparse
 import csv
@@ -37,9 +40,14 @@ WORKLOADS = {
     "dome_3m": "kagome dome 3 m span / 0.3 m spacing / 50 mm segments, 15% contraction, +-1 mm imperfection, mu 0.3",
     "dome_10m": "kagome dome 10 m span / 0.15 m spacing / 20 mm segments, 12% contraction, +-2 mm imperfection, mu 0.3",
     "dome_30m": "kagome dome 30 m span / 0.2 m spacing / 15 mm segments, 18% contraction, +-2 mm imperfection, mu 0.05",
-    "variants": "ragged batch of seeded design variants (span 2-8 m, spacing 0.2-0.4 m, mu 0.1-0.5, "
-                "contraction 8-20%, thickness 3-6 mm, 2k-50k nodes each)",
+    "variants": "ragged batch of seeded design variants (256 by default; span 2-8 m, spacing 0.2-0.4 m, mu 0.1-0.5, "
+                "contraction 8-20%, thickness 3-6 mm, 2k-50k target nodes each, seeds 3000+i)",
 }
+
+
+def log(msg):
+    """a progress line on stderr (the JSON line stays alone on stdout)"""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
 def scenes():
@@ -65,6 +73,24 @@ def strand_cross_bytes(models, precision):
     return sb, cb
 
 
+def host_info():
+    """the host the CPU baseline ran on: CPU model, the CPUs this process may use (what nproc prints), all CPUs"""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": nproc, "cpu_count": os.cpu_count()}
+
+
 def cpu_baseline(model, budget_s=12.0):
     """the f64 oracle on the same instance from its initial state, one host thread, a bounded sample"""
     from oracle.bind import Oracle
@@ -76,19 +102,17 @@ def cpu_baseline(model, budget_s=12.0):
     t = time.perf_counter()
     o.step(its)
     el = time.perf_counter() - t
-    return {"value": model.n * its / el, "unit": "node-iterations/s", "cores": 1, "kind": "port",
-            "sample": f"{its} iterations of the same {model.n}-node instance from its initial state, f64 oracle "
-                      f"(CpuSolver restated in C++, -O3, no fast-math), one host thread, {el:.1f} s"}
+    out = {"value": model.n * its / el, "unit": "node-iterations/s", "cores": 1, "kind": "port",
+           "sample": f"{its} iterations of the same {model.n}-node instance from its initial state, f64 oracle "
+                     f"(CpuSolver restated in C++, -O3, no fast-math), one host thread, {el:.1f} s"}
+    out.update(host_info())
+    return out
 
 
 def host_threads():
     """host threads for the all-cores baseline: the CPUs this process may run on, at most 16 (one GPU's share of a
     shared host)"""
-    try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    return max(1, min(16, host_info()["nproc"]))
 
 
 def cpu_solves_baseline(models, iters, budget_s=20.0, per_thread=2):
@@ -121,33 +145,75 @@ def cpu_solves_baseline(models, iters, budget_s=20.0, per_thread=2):
     wall = time.perf_counter() - t0
     solve_s = [el / its * iters[i] for i, its, el in res]
     mean_solve = sum(solve_s) / len(solve_s)
-    return {"value": T / mean_solve, "unit": "solves/s", "cores": T, "kind": "port",
-            "node_iters_per_s": sum(models[i].n * its for i, its, _ in res) / wall,
-            "single_core_solves_per_s": 1.0 / mean_solve,
-            "sample": f"{len(idx)} of the {B} variants (evenly spaced), {T} host threads, one instance per thread, "
-                      f"each run from its initial state for up to {per_inst:.0f} s of f64 oracle iterations (CpuSolver "
-                      f"restated in C++, -O3, no fast-math); solve time = time per iteration x the batched run's "
-                      f"iterations to the stop; {wall:.1f} s wall"}
+    out = {"value": T / mean_solve, "unit": "solves/s", "cores": T, "kind": "port",
+           "node_iters_per_s": sum(models[i].n * its for i, its, _ in res) / wall,
+           "single_core_solves_per_s": 1.0 / mean_solve,
+           "sample": f"{len(idx)} of the {B} variants (evenly spaced), {T} host threads, one instance per thread, "
+                     f"each run from its initial state for up to {per_inst:.0f} s of f64 oracle iterations (CpuSolver "
+                     f"restated in C++, -O3, no fast-math); solve time = time per iteration x the batched run's "
+                     f"iterations to the stop; {wall:.1f} s wall"}
+    out.update(host_info())
+    return out
+
+
+def cpu_variants_baseline(models, iters):
+    """node-iterations/s of the f64 oracle over the batch on every host thread (one instance per thread, a sample of
+    the variants) — the baseline of the batched line — with the one-thread figure on the median-size variant and the
+    all-thread solves/s beside it"""
+    allc = cpu_solves_baseline(models, iters)
+    med = sorted(models, key=lambda m: m.n)[len(models) // 2]
+    one = cpu_baseline(med, budget_s=8.0)
+    out = {"value": allc["node_iters_per_s"], "unit": "node-iterations/s", "cores": allc["cores"], "kind": "port",
+           "sample": allc["sample"], "single_thread": {"value": one["value"], "unit": one["unit"], "cores": 1,
+                                                       "sample": one["sample"]},
+           "solves_per_s": allc["value"], "single_core_solves_per_s": allc["single_core_solves_per_s"]}
+    out.update(host_info())
+    return out
 
 
 # ------------------------------------------------------------------------------------------------ counter passes
+def solver_opts(args):
+    """the batch options of this run: the tile, graph length and crossing layout given on the command line (0 = the
+    library's choice by size)"""
+    return {"tile": args.tile, "per_submit": args.per_submit, "cross_lanes": args.cross_lanes}
+
+
 def kernel_only(args):
     """the workload of one counter pass: the calibration copy, then warm-up iterations and strand launches"""
     W = importlib.import_module("weave-sim_amd")
     ctx = W.solver.default_context()
     ctx.copy_probe(COPY_BYTES, 3)
     models = build_models(args.workload, args.batch)
-    g = W.GpuSolver(models, precision=args.precision, tile=args.tile)
+    g = W.GpuSolver(models, precision=args.precision, **solver_opts(args))
     g.step(64)
     g.kernel_bench(10)
     g.close()
+
+
+def tune_only(args):
+    """the autotuner's pick for the workload, printed as one JSON line (run as a child process, so that the counter
+    passes run before the measuring process touches the GPU)"""
+    W = importlib.import_module("weave-sim_amd")
+    models = build_models(args.workload, args.batch)
+    best, rows = W.solver.autotune(models, precision=args.precision, iters=256)
+    print(json.dumps({"best": best, "rows": rows}))
+
+
+def run_child(args, extra, timeout):
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", args.workload, "--precision", args.precision,
+           "--batch", str(args.batch)] + extra
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"{' '.join(extra)}: exit {r.returncode}: {r.stderr[-400:]}")
+    return r.stdout
 
 
 def counter_pass(args, counter, outdir):
     """one rocprofv3 pass over kernel_only; returns {kernel: mean counter value per dispatch}"""
     cmd = ["timeout", "-s", "KILL", "150", "rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", outdir,
            "-o", "pass", "--", sys.executable, os.path.abspath(__file__), "--kernel-only", "--workload", args.workload,
-           "--precision", args.precision, "--tile", str(args.tile), "--batch", str(args.batch)]
+           "--precision", args.precision, "--tile", str(args.tile), "--per-submit", str(args.per_submit),
+           "--cross-lanes", str(args.cross_lanes), "--batch", str(args.batch)]
     env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
     r = subprocess.run(cmd, cwd=tempfile.gettempdir(), env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
@@ -186,49 +252,59 @@ def measure_traffic(args):
 
 
 # ------------------------------------------------------------------------------------------------ solves/s
-def run_solves(args):
-    """design variants relaxed to the stop criterion (residual ≤ tol, or the iteration limit): the batched FP32+FP64
-    form (all variants in one solver, autotuned tile) against the plain single-instance FP64 path, timed on a
-    sample of the same variants (per-iteration time × iterations to the stop, summed over the set)"""
-    W = importlib.import_module("weave-sim_amd")
-    S = scenes()
-    models = [S.design_variant(i) for i in range(args.batch)]
+def solves_compare(W, models, opts, limit, plain_sample):
+    """the batch relaxed to the stop criterion (residual <= tol, or `limit` iterations) in the batched FP32+FP64 form
+    (all instances in one solver, the given options) against the plain single-instance FP64 path: every sampled
+    instance (evenly spaced over the set) run alone from its initial state for as many iterations as it took in the
+    batch, one after another.  Both sides are timed with HIP events around the step (graph instantiation excluded),
+    the batched wall clock beside it.  Plain solves/s = 1 / the mean plain solve time of the sample."""
     B = len(models)
-    limit = args.max_iter
-    g = W.GpuSolver(models, precision="fp32", tile="autotune")
-    opts = g.opts
+    g = W.GpuSolver(models, precision="fp32", **opts)
     t0 = time.perf_counter()
     ctl = g.step(limit)
     wall = time.perf_counter() - t0
+    dev_s = g.timing()["ms_last_step"] * 1e-3
     iters = [c["iter"] for c in ctl]
     converged = sum(1 for c in ctl if c["done"])
     g.close()
-    batched = B / wall
-    # plain path: one instance per solver, FP64, one after another
-    idx = sorted(set(int(round(k * (B - 1) / max(args.plain_sample - 1, 1))) for k in range(args.plain_sample)))
+    batched = B / dev_s
+    idx = sorted(set(int(round(k * (B - 1) / max(plain_sample - 1, 1))) for k in range(plain_sample)))
     per_solve = []
     for i in idx:
         p = W.GpuSolver([models[i]], precision="fp64", tile=0)
-        p.step(64)
-        t = time.perf_counter()
-        c = p.step(args.plain_iters)[0]
-        dt = (time.perf_counter() - t) / max(c["iter"] - 64, 1)
+        c = p.step(iters[i])[0]
+        if c["iter"] != iters[i]:
+            raise RuntimeError(f"variant {i}: the plain path stopped at {c['iter']}, the batch at {iters[i]}")
+        per_solve.append(p.timing()["ms_last_step"] * 1e-3)
         p.close()
-        per_solve.append(dt * iters[i])
     plain = 1.0 / (sum(per_solve) / len(per_solve))
-    nodes = sum(m.n for m in models)
+    return {"batched_fp32_fp64_solves_per_s": batched, "batched_wall_s": wall, "batched_device_s": dev_s,
+            "batched_wall_solves_per_s": B / wall,
+            "node_iters_per_s": sum(m.n * k for m, k in zip(models, iters)) / dev_s,
+            "iteration_limit": limit, "converged": converged, "iterations_mean": sum(iters) / B,
+            "plain_single_instance_fp64_solves_per_s": plain,
+            "plain_sample": {"instances": idx, "iterations": "each as many as in the batch (whole solves)",
+                             "solve_ms_mean": 1e3 * sum(per_solve) / len(per_solve)},
+            "batched_over_plain": batched / plain}, iters
+
+
+def run_solves(args):
+    """design variants relaxed to the stop criterion: the batched FP32+FP64 form (autotuned) against the plain
+    single-instance FP64 path (solves_compare)"""
+    W = importlib.import_module("weave-sim_amd")
+    models = build_models("variants", args.batch)
+    B = len(models)
+    best, _ = W.solver.autotune(models, precision="fp32", iters=256)
+    cmp, iters = solves_compare(W, models, best, args.max_iter, args.plain_sample)
     out = {
-        "metric": "solves/s", "value": batched, "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded design variants)",
+        "metric": "solves/s", "value": cmp["batched_fp32_fp64_solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded design variants)",
         "config": {"workload": f"{B} design variants ({WORKLOADS['variants']}), relaxed to residual <= tol or "
-                               f"{limit} iterations", "instances": B, "nodes": nodes, "tile": opts["tile"],
-                   "per_submit": opts["per_submit"]},
-        "wall_s": wall, "node_iters_per_s": sum(m.n * k for m, k in zip(models, iters)) / wall,
-        "converged": converged, "iterations_mean": sum(iters) / B,
-        "plain_single_instance_fp64_solves_per_s": plain,
-        "plain_sample": {"instances": idx, "timed_iterations_each": args.plain_iters},
-        "batched_over_plain": batched / plain,
+                               f"{args.max_iter} iterations", "instances": B, "nodes": sum(m.n for m in models),
+                   **best},
     }
+    out.update(cmp)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_solves_baseline(models, iters)
     return out
@@ -365,23 +441,28 @@ def run_stability(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3200)
-    ap.add_argument("--warmup", type=int, default=320)
-    ap.add_argument("--workload", default="dome_10m", choices=sorted(WORKLOADS))
-    ap.add_argument("--batch", type=int, default=64, help="instances of the variants workload")
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--workload", default="variants", choices=sorted(WORKLOADS))
+    ap.add_argument("--batch", type=int, default=256, help="instances of the variants workload")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
-    ap.add_argument("--tile", type=int, default=0, help="strand tile (64/128/256); 0 = the default (64)")
+    ap.add_argument("--tile", type=int, default=None,
+                    help="strand tile (64/128/256); 0 = the default (64); unset: the autotuner's pick on the variants "
+                         "workload, the default elsewhere")
+    ap.add_argument("--per-submit", type=int, default=0, help="iterations per captured graph (0 = the default, 64)")
+    ap.add_argument("--cross-lanes", type=int, default=0, help="lanes per crossing, 1 or 2 (0 = by size)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true")
+    ap.add_argument("--no-solves", action="store_true", help="leave out the batched-vs-plain solves comparison")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--kernel-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--tune-only", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--mode", default="throughput", choices=["throughput", "solves", "serve", "stability"],
                     help="throughput: node-iterations/s of one workload; solves: design variants to the stop "
                          "criterion, batched vs plain; serve: a Poisson request stream through the server; "
                          "stability: the stability check of a relaxed dome, device-evaluated vs the oracle")
     ap.add_argument("--max-iter", type=int, default=5000, help="iteration limit of a solve (solves, serve)")
-    ap.add_argument("--plain-sample", type=int, default=16, help="variants timed on the plain path (solves)")
-    ap.add_argument("--plain-iters", type=int, default=512, help="iterations timed per plain-path variant")
+    ap.add_argument("--plain-sample", type=int, default=32, help="variants solved on the plain path (solves)")
     ap.add_argument("--rate", type=float, default=60.0, help="requests per second (serve)")
     ap.add_argument("--duration", type=float, default=120.0, help="seconds of arrivals (serve)")
     ap.add_argument("--chunk", type=int, default=64, help="iterations per server round (serve)")
@@ -396,9 +477,10 @@ def main():
     if args.kernel_only:
         kernel_only(args)
         return
+    if args.tune_only:
+        tune_only(args)
+        return
     if args.mode == "solves":
-        if args.batch == 64 and "--batch" not in sys.argv:
-            args.batch = 256
         print(json.dumps(run_solves(args)))
         return
     if args.mode == "serve":
@@ -407,15 +489,30 @@ def main():
     if args.mode == "stability":
         print(json.dumps(run_stability(args)))
         return
-    # counter passes first, as child processes, before this process touches the GPU
+    # the autotuner and the counter passes run as child processes, before this process touches the GPU
+    tune = None
+    if args.tile is None:
+        if args.workload == "variants":
+            log("autotuning (child process)")
+            tune = json.loads(run_child(args, ["--tune-only"], 900).strip().splitlines()[-1])
+            log(f"autotuner's pick: {tune['best']}")
+            args.tile, args.per_submit, args.cross_lanes = (tune["best"]["tile"], tune["best"]["per_submit"],
+                                                            tune["best"]["cross_lanes"])
+        else:
+            args.tile = 0
+    if not args.no_traffic:
+        log("counter passes (FETCH_SIZE, WRITE_SIZE)")
     traffic, traffic_detail = (None, {"skipped": True}) if args.no_traffic else measure_traffic(args)
 
+    log("building the workload")
     W = importlib.import_module("weave-sim_amd")
     models = build_models(args.workload, args.batch)
     n_nodes = sum(m.n for m in models)
     n_cross = sum(m.K for m in models)
-    g = W.GpuSolver(models, precision=args.precision, tile=args.tile)
+    g = W.GpuSolver(models, precision=args.precision, **solver_opts(args))
+    opts = g.opts
     dev = g.ctx.info()
+    log(f"{len(models)} instances, {n_nodes} nodes on the device; {args.warmup} warm-up and {args.steps} timed steps")
     g.step(args.warmup)
     it0 = sum(c["iter"] for c in g.read_control())
     t0 = time.perf_counter()
@@ -426,6 +523,7 @@ def main():
     if it1 - it0 != args.steps * len(models):
         raise SystemExit("an instance stopped inside the timed region; rerun with fewer steps")
     value = n_nodes * args.steps / wall
+    step_dev_ms = g.timing()["ms_last_step"]
     # per-kernel durations inside the iteration sequence: the kernels launched one by one, a HIP event between
     # consecutive kernels on the solver's stream
     g.set_profile(True)
@@ -436,9 +534,8 @@ def main():
     # and each kernel alone, launched back to back between two events
     kb = g.kernel_bench(args.kernel_reps)
     sb, cb = strand_cross_bytes(models, args.precision)
-    achieved = sb / (kb["ms_strand"] * 1e-3) / 1e9
+    achieved = sb / (kin["ms_strand"] * 1e-3) / 1e9
     alg = g.bytes()["alg_bytes_per_iter"]
-    tile = g.tile
     g.close()
     line = {
         "metric": "node-iters/s",
@@ -452,19 +549,36 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64" if args.precision == "fp32" else "f64",
-        "data": "synthetic (seeded kagome domes from the host model builder)",
+        "data": "synthetic (seeded kagome gridshells from the host model builder)",
         "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]}", "instances": len(models),
-                   "nodes": n_nodes, "crossings": n_cross, "precision": args.precision, "strand_tile": tile,
-                   "device": dev["name"]},
+                   "nodes": n_nodes, "crossings": n_cross, "precision": args.precision, "strand_tile": opts["tile"],
+                   "per_submit": opts["per_submit"], "cross_lanes": opts["cross_lanes"],
+                   "autotuned": tune is not None, "device": dev["name"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_strand",
-                     "alg_bytes_per_launch": sb, "ms_per_launch": kb["ms_strand"], "traffic_detail": traffic_detail},
-        "kernels_ms": kb,
+                     "alg_bytes_per_launch": sb, "ms_per_launch": kin["ms_strand"],
+                     "duration_from": "HIP events between consecutive kernels of the iteration sequence",
+                     "traffic_detail": traffic_detail},
+        "ms_per_step_device": step_dev_ms / args.steps,
         "kernels_ms_in_iteration": kin,
+        "kernels_ms_in_iteration_sum": kin["ms_cross"] + kin["ms_strand"] + kin["ms_ctrl"],
+        "kernels_ms_alone": kb,
         "alg_bytes_per_iter": alg,
+        "alg_bytes_frac_of_peak_per_iteration": alg / (wall / args.steps) / (HBM_PEAK_GBS * 1e9),
     }
+    if tune is not None:
+        line["autotune"] = tune["rows"]
+    iters = None
+    if args.workload == "variants" and not args.no_solves:
+        log(f"solves: the batch to the stop criterion ({args.max_iter} iterations at most), then the plain path")
+        cmp, iters = solves_compare(W, models, solver_opts(args), args.max_iter, args.plain_sample)
+        line["solves"] = cmp
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(max(models, key=lambda m: m.n))
+        log("CPU baseline (f64 oracle on the host)")
+        if args.workload == "variants":
+            line["cpu_baseline"] = cpu_variants_baseline(models, iters or [args.max_iter] * len(models))
+        else:
+            line["cpu_baseline"] = cpu_baseline(max(models, key=lambda m: m.n))
     print(json.dumps(line))
 
 

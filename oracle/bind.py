@@ -50,6 +50,18 @@ def lib():
         L.oracle_replay.restype = None
         L.oracle_pairwise_sum.argtypes = [A.P_d, C.c_int64]
         L.oracle_pairwise_sum.restype = C.c_double
+        L.oracle_probe_classify.argtypes = [C.c_void_p, C.c_int32, A.P_d, A.P_d, A.P_d, C.c_double, A.P_d]
+        L.oracle_probe_classify.restype = None
+        L.oracle_probe_support.argtypes = [C.c_void_p, C.c_int32, A.P_d, A.P_d, C.c_double, C.c_int32, A.P_d]
+        L.oracle_probe_support.restype = C.c_double
+        L.oracle_probe_seat.argtypes = [C.c_void_p, C.c_int32, A.P_d, A.P_d, A.P_d, A.P_d, A.P_d, A.P_d]
+        L.oracle_probe_seat.restype = C.c_double
+        L.oracle_probe_rot.argtypes = [C.c_void_p, C.c_int32, A.P_d, A.P_d, A.P_d, C.c_double, C.c_double, C.c_int32,
+                                       C.c_double, A.P_d]
+        L.oracle_probe_rot.restype = None
+        L.oracle_probe_al_value.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32,
+                                            C.c_double]
+        L.oracle_probe_al_value.restype = C.c_double
         _lib = L
     return _lib
 
@@ -145,6 +157,56 @@ class Oracle:
                                    Qp.ctypes.data_as(A.P_d) if Qp is not None else None,
                                    F.ctypes.data_as(A.P_d), T.ctypes.data_as(A.P_d))
         return e, F, T
+
+
+def _vec(v):
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1))
+
+
+class ContactProbe:
+    """single contact laws of the oracle on constructed geometry: frames RA, RB (rows x, y, t of each side), the
+    contact normal n, for crossing c of an oracle built on `model`"""
+
+    def __init__(self, model, c=0):
+        self.o = Oracle(model)
+        self.c = c
+
+    def classify(self, RA, RB, n, fn):
+        out = np.zeros(10)
+        RA, RB, n = _vec(RA), _vec(RB), _vec(n)
+        lib().oracle_probe_classify(self.o.h, self.c, RA.ctypes.data_as(A.P_d), RB.ctypes.data_as(A.P_d),
+                                    n.ctypes.data_as(A.P_d), float(fn), out.ctypes.data_as(A.P_d))
+        return {"ctype": int(out[0]), "face_side": int(out[1]), "theta": out[2], "phi": (out[3], out[4]),
+                "Anom": out[5], "pressure": out[6], "util": out[7], "cface": (out[8], out[9])}
+
+    def support(self, R, n, sign, side):
+        r = np.zeros(3)
+        R, n = _vec(R), _vec(n)
+        h = lib().oracle_probe_support(self.o.h, self.c, R.ctypes.data_as(A.P_d), n.ctypes.data_as(A.P_d), float(sign),
+                                       int(side), r.ctypes.data_as(A.P_d))
+        return h, r
+
+    def seat(self, RA, RB, n, St, dents):
+        tp = np.zeros(2)
+        RA, RB, n, St, dents = _vec(RA), _vec(RB), _vec(n), _vec(St), _vec(dents)
+        mu = lib().oracle_probe_seat(self.o.h, self.c, RA.ctypes.data_as(A.P_d), RB.ctypes.data_as(A.P_d),
+                                     n.ctypes.data_as(A.P_d), St.ctypes.data_as(A.P_d), dents.ctypes.data_as(A.P_d),
+                                     tp.ctypes.data_as(A.P_d))
+        return mu, tp
+
+    def rot(self, RA, RB, n, fn, theta0, active, kt):
+        out = np.zeros(3)
+        RA, RB, n = _vec(RA), _vec(RB), _vec(n)
+        lib().oracle_probe_rot(self.o.h, self.c, RA.ctypes.data_as(A.P_d), RB.ctypes.data_as(A.P_d),
+                               n.ctypes.data_as(A.P_d), float(fn), float(theta0), 1 if active else 0, float(kt),
+                               out.ctypes.data_as(A.P_d))
+        return {"M": out[0], "theta0": out[1], "rot_slip": out[2]}
+
+
+def al_value(lam, pen_mean, kn, locked, enabled, pinned, cap):
+    """PolylineContact.alValue (solver_cpu.js:492-497) as the oracle computes it"""
+    return lib().oracle_probe_al_value(float(lam), float(pen_mean), float(kn), int(locked), int(enabled), int(pinned),
+                                       float(cap))
 
 
 def pairwise_sum(a):

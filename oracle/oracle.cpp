@@ -19,8 +19,15 @@
 //   Floor.apply              floor_check.mjs invariants: ΣF = weight, cone ratio μ, μ = 0 frictionless
 //   step / KD / residual     Σ reactions + Σ loads = 0 at convergence; kinetic-energy monotonicity between resets;
 //                            pairwiseSum against numpy.sum order; bit-identical reruns
-//   W2 / W3 / AL branches    parity unpinned against the reference (no fixture or closed form pins their trajectories);
-//                            checked for invariants only (finite state, λ ≥ 0, cap respected)
+//   _classify / _support     constructed crossings (tests/test_oracle_contact_laws.py): face / edge / edge-on-edge
+//                            types from the section rolls, the strip overlap area wA·wB/sinθ against a polygon
+//                            clipping, the edge-on-face Winkler wedge (Fn = ½ κ φ b² ℓ), the support half-extent and
+//                            point against the section's corners
+//   W3 _seatFriction/_walls  the inclined-wall friction law μ_eff = tan(atan μ + ψ) on a constructed wall angle
+//   W2 _rotationalFriction   stick below and return mapping onto the Coulomb moment μ Fn r_eq above it; r_eq against
+//                            the friction radius of a uniformly pressed disk (⅔√(A/π)) and of a line contact (ℓ/4)
+//   AL updateMultipliers     a pressed crossing's penetration driven to zero with λ = the load, and λ held at the
+//                            crushing cap fc90·wA·wB/sinθ with the penetration carrying the rest
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -931,5 +938,74 @@ WS_API double oracle_rod_apply(oracle_t* h, const double* X, const double* Q, do
 }
 // numpy-order pairwise sum, exposed for its known-answer test
 WS_API double oracle_pairwise_sum(const double* a, int64_t n) { return pairwiseSum(a, size_t(n)); }
+
+// ---- probes of single contact laws on constructed geometry (the closed-form tests of _classify, _support,
+// _seatFriction / _walls, _rotationalFriction and alValue).  RA, RB: the evaluated frames of both sides of crossing c,
+// rows x (width axis), y (height axis), t (tangent), as _eval leaves them (solver_cpu.js:219-226); nrm: the contact
+// normal of crossing c.
+static void probe_sides(Oracle& o, int c, const double* RA, const double* RB) {
+  for (int s = 0; s < 2; s++) {
+    const double* R = s == 0 ? RA : RB;
+    std::copy(R, R + 9, o.ev[s].R.begin() + 9 * c);
+    std::copy(R + 6, R + 9, o.ev[s].t.begin() + 3 * c);
+  }
+}
+static std::vector<double> probe_normal(const Oracle& o, int c, const double* nrm) {
+  std::vector<double> nv(3 * o.K, 0.0);
+  std::copy(nrm, nrm + 3, nv.begin() + 3 * c);
+  return nv;
+}
+// _classify (solver_cpu.js:231-271) with Fn[c] = fn: out = ctype, faceSide, theta, phi A, phi B, Anom, pressure,
+// util, the face extent cFace of A and B
+WS_API void oracle_probe_classify(oracle_t* h, int32_t c, const double* RA, const double* RB, const double* nrm, double fn,
+                                  double* out) {
+  Oracle& o = h->o;
+  probe_sides(o, c, RA, RB);
+  o.Fn[c] = fn;
+  std::vector<double> cFace;
+  o.classify(probe_normal(o, c, nrm), cFace);
+  const double v[10] = {double(o.ctype[c]), double(o.faceSide[c]), o.theta[c], o.phi[2 * c], o.phi[2 * c + 1], o.Anom[c],
+                        o.pressure[c], o.util[c], cFace[2 * c], cFace[2 * c + 1]};
+  std::copy(v, v + 10, out);
+}
+// _support (solver_cpu.js:274-282): the half-extent of the section of `side` towards sign·nrm, its support point r
+WS_API double oracle_probe_support(oracle_t* h, int32_t c, const double* R, const double* nrm, double sign, int32_t side,
+                                   double* r) {
+  return h->o.support(R, nrm, c, sign, side, r);
+}
+// _seatFriction (solver_cpu.js:295-315) with dents dent2 on both sides of crossing c and tangential offset St:
+// returns the directional friction coefficient, tan psi of both sides in tanpsi2
+WS_API double oracle_probe_seat(oracle_t* h, int32_t c, const double* RA, const double* RB, const double* nrm,
+                                const double* St, const double* dent2, double* tanpsi2) {
+  Oracle& o = h->o;
+  probe_sides(o, c, RA, RB);
+  o.dent[2 * c] = dent2[0]; o.dent[2 * c + 1] = dent2[1];
+  o.tanpsi[2 * c] = 1.0; o.tanpsi[2 * c + 1] = 1.0;
+  const double mu = o.seatFriction(c, St, nrm, RA + 6, RB + 6);
+  tanpsi2[0] = o.tanpsi[2 * c]; tanpsi2[1] = o.tanpsi[2 * c + 1];
+  return mu;
+}
+// _rotationalFriction (solver_cpu.js:347-379) of crossing c after _classify, with Fn[c] = fn, the reference angle
+// theta0 (NaN: unset) and the stick spring kt: out = M about n, theta0 after, rotational slip of the call
+WS_API void oracle_probe_rot(oracle_t* h, int32_t c, const double* RA, const double* RB, const double* nrm, double fn,
+                             double theta0, int32_t active, double kt, double* out) {
+  Oracle& o = h->o;
+  probe_sides(o, c, RA, RB);
+  o.Fn[c] = fn;
+  const std::vector<double> nv = probe_normal(o, c, nrm);
+  std::vector<double> cFace;
+  o.classify(nv, cFace);
+  o.theta0[c] = theta0; o.rotSlip[c] = 0.0;
+  std::vector<uint8_t> act(o.K, 0);
+  act[c] = active ? 1 : 0;
+  std::vector<double> Tv(3 * o.m, 0.0);
+  o.rotationalFriction(Tv, nv, act, kt);
+  out[0] = o.Mrot[c]; out[1] = o.theta0[c]; out[2] = o.rotSlip[c];
+}
+// PolylineContact.alValue (solver_cpu.js:492-497)
+WS_API double oracle_probe_al_value(double lam, double pen_mean, double kn, int32_t locked, int32_t enabled, int32_t pinned,
+                                    double cap) {
+  return Oracle::alValue(lam, pen_mean, kn, locked != 0, enabled != 0, pinned != 0, cap);
+}
 
 }  // extern "C"

@@ -392,7 +392,11 @@ struct BatchBase {
   int cross_lanes = 0;                  // lanes per crossing (0: by size at every relist)
   int lanes = 2;                        // ... as the current work lists use it
   hipStream_t stream = nullptr;
-  hipGraphExec_t graph = nullptr;
+  // captured iteration sequences by length: per_submit iterations, and every power of two below it, so a step of
+  // n iterations replays whole per_submit graphs and then the binary pieces of the remainder — only live iterations
+  // are launched
+  std::map<int, hipGraphExec_t> graphs;
+  bool eager_graphs = true;             // build every length at the first step (the request server builds lazily)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double ms_last = 0, ms_cross = 0, ms_strand = 0, ms_ctrl = 0;
   bool profile = false;
@@ -432,7 +436,7 @@ struct Batch : BatchBase {
   std::vector<TileItem> tiles;
 
   ~Batch() override {
-    if (graph) (void)hipGraphExecDestroy(graph);
+    drop_graphs();
     for (auto& h : inst) if (h) h->arena.release();
     for (void* p : scratch_ptr) if (p) (void)hipFree(p);
     if (d_inst) (void)hipFree(d_inst);
@@ -546,7 +550,7 @@ struct Batch : BatchBase {
     std::vector<Inst<Real>> table(B);
     for (int b = 0; b < B; b++) table[b] = inst[b] ? inst[b]->d : Inst<Real>{};
     put_list(d_inst, cap_inst, table);
-    if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
+    drop_graphs();
   }
 
 
@@ -598,14 +602,30 @@ struct Batch : BatchBase {
   }
   void launch_iteration(hipStream_t s) { launch_cross(s); launch_strand(s); launch_ctrl(s); }
 
-  void ensure_graph() {
-    if (graph) return;
+  void drop_graphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+  }
+  // the graph of `len` iterations, captured, instantiated and uploaded on first use
+  hipGraphExec_t graph_of(int len) {
+    auto f = graphs.find(len);
+    if (f != graphs.end()) return f->second;
     hipGraph_t g;
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    for (int it = 0; it < per_submit; it++) launch_iteration(stream);
+    for (int it = 0; it < len; it++) launch_iteration(stream);
     HIPCHK(hipStreamEndCapture(stream, &g));
-    HIPCHK(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
-    HIPCHK(hipGraphDestroy(g));
+    hipGraphExec_t e = nullptr;
+    const hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(r);
+    graphs[len] = e;
+    HIPCHK(hipGraphUpload(e, stream));
+    return e;
+  }
+  void ensure_graphs() {
+    if (!eager_graphs || graphs.count(per_submit)) return;
+    graph_of(per_submit);
+    for (int len = 1; len < per_submit; len *= 2) graph_of(len);
   }
 
   void fetch_ctls() {
@@ -633,6 +653,7 @@ struct Batch : BatchBase {
     }
     if (counts) { n_iter = 0; for (int q = 0; q < nsel; q++) n_iter = std::max(n_iter, counts[q]); }
     if (n_iter > 0 && !sel.empty()) {
+      if (!profile) ensure_graphs();   // instantiated and uploaded before the timed region
       HIPCHK(hipMemcpyAsync(d_sel, sel.data(), sizeof(int32_t) * sel.size(), hipMemcpyHostToDevice, stream));
       if (counts) HIPCHK(hipMemcpyAsync(d_cnt, counts, sizeof(int32_t) * sel.size(), hipMemcpyHostToDevice, stream));
       hipLaunchKernelGGL(k_begin<Real>, dim3((sel.size() + 63) / 64), dim3(64), 0, stream, d_ctls, d_sel, counts ? d_cnt : nullptr,
@@ -664,15 +685,19 @@ struct Batch : BatchBase {
         ms_cross = sc / n_iter; ms_strand = ss / n_iter; ms_ctrl = sk / n_iter;
         for (auto& e : ev) (void)hipEventDestroy(e);
       } else {
-        ensure_graph();
-        int launches = (n_iter + per_submit - 1) / per_submit;
-        for (int r = 0; r < launches; r++) {
-          HIPCHK(hipGraphLaunch(graph, stream));
-          if (launches > 8 && (r % 8) == 7 && r + 1 < launches) {
+        const int full = n_iter / per_submit, rem = n_iter % per_submit;
+        for (int r = 0; r < full; r++) {
+          HIPCHK(hipGraphLaunch(graph_of(per_submit), stream));
+          if (full > 8 && (r % 8) == 7 && r + 1 < full) {
             HIPCHK(hipStreamSynchronize(stream));
             if (all_done(sel.data(), int(sel.size()))) break;
           }
         }
+        // the remainder in binary pieces, largest first
+        int hb = 1;
+        while (hb * 2 <= rem) hb *= 2;
+        for (int len = hb; rem > 0 && len >= 1; len /= 2)
+          if (rem & len) HIPCHK(hipGraphLaunch(graph_of(len), stream));
       }
       HIPCHK(hipEventRecord(ev1, stream));
       HIPCHK(hipStreamSynchronize(stream));
@@ -1036,6 +1061,7 @@ void autotune(ws_ctx* ctx, const ws_model* const* models, const ws_state* const*
   std::vector<ws_control> c0(B), c1(B);
   for (int t : tiles) for (int ps : subs) {
     auto b = make_batch(ctx, models, states, B, prec, t, ps, std::max(history, 1));
+    b->eager_graphs = false;   // the candidate steps whole graphs only
     b->step(2 * ps, nullptr, 0, 1, c0.data());
     const int n = std::max(ps, (iters + ps - 1) / ps * ps);
     b->step(n, nullptr, 0, 1, c1.data());
@@ -1050,6 +1076,7 @@ void autotune(ws_ctx* ctx, const ws_model* const* models, const ws_state* const*
   {
     const int other = best.cross_lanes == 1 ? 2 : 1, t = best.tile, ps = best.per_submit;
     auto b = make_batch(ctx, models, states, B, prec, t, ps, std::max(history, 1), 0, other);
+    b->eager_graphs = false;
     b->step(2 * ps, nullptr, 0, 1, c0.data());
     const int n = std::max(ps, (iters + ps - 1) / ps * ps);
     b->step(n, nullptr, 0, 1, c1.data());

@@ -129,6 +129,7 @@ WS_API int ws_server_create(ws_ctx* ctx, const ws_server_opts* opts, ws_server**
     }
     if (o.precision == WS_FP64) s->batch.reset(new Batch<double>()); else s->batch.reset(new Batch<float>());
     s->batch->ctx = ctx; s->batch->precision = o.precision; s->batch->tile = o.tile; s->batch->per_submit = o.chunk;
+    s->batch->eager_graphs = false;   // relisted as requests come and go: each length is captured when a round needs it
     s->batch->create_slots(o.max_slots);
     s->slot.resize(o.max_slots);
     s->t0 = std::chrono::steady_clock::now();
