@@ -225,7 +225,7 @@ def counter_pass(args, counter, outdir):
             if row["Counter_Name"] != counter:
                 continue
             name = row["Kernel_Name"]
-            key = "k_strand" if "k_strand" in name else "k_copy" if "k_copy" in name else None
+            key = next((k for k in ("k_strand", "k_cross", "k_ctrl", "k_copy") if k in name), None)
             if key:
                 acc.setdefault(key, []).append(float(row["Counter_Value"]))
     return {k: sum(v) / len(v) for k, v in acc.items()}
@@ -243,8 +243,11 @@ def measure_traffic(args):
         f_fetch = COPY_BYTES / (fetch["k_copy"] * 1024.0)
         f_write = COPY_BYTES / (write["k_copy"] * 1024.0)
         t = fetch["k_strand"] * 1024.0 * f_fetch + write["k_strand"] * 1024.0 * f_write
+        other = {k: fetch[k] * 1024.0 * f_fetch + write[k] * 1024.0 * f_write
+                 for k in ("k_cross", "k_ctrl") if k in fetch and k in write}
         return t, {"fetch_kib": fetch["k_strand"], "write_kib": write["k_strand"], "fetch_factor": f_fetch,
-                   "write_factor": f_write, "copy_fetch_kib": fetch["k_copy"], "copy_write_kib": write["k_copy"]}
+                   "write_factor": f_write, "copy_fetch_kib": fetch["k_copy"], "copy_write_kib": write["k_copy"],
+                   "other_kernels_bytes_per_launch": other}
     except Exception as e:   # the throughput line stands without the counter passes
         return None, {"error": str(e)[:200]}
     finally:
@@ -516,9 +519,9 @@ def main():
     g.step(args.warmup)
     it0 = sum(c["iter"] for c in g.read_control())
     t0 = time.perf_counter()
-    ctl = g.step(args.steps)          # returns after a synchronisation of the solver's stream
+    g.step(args.steps, control=False)   # returns after a synchronisation of the solver's stream
     t1 = time.perf_counter()
-    it1 = sum(c["iter"] for c in ctl)
+    it1 = sum(c["iter"] for c in g.read_control())
     wall = t1 - t0
     if it1 - it0 != args.steps * len(models):
         raise SystemExit("an instance stopped inside the timed region; rerun with fewer steps")
@@ -564,6 +567,7 @@ def main():
         "kernels_ms_in_iteration_sum": kin["ms_cross"] + kin["ms_strand"] + kin["ms_ctrl"],
         "kernels_ms_alone": kb,
         "alg_bytes_per_iter": alg,
+        "alg_bytes_per_launch": {"k_strand": sb, "k_cross": cb},
         "alg_bytes_frac_of_peak_per_iteration": alg / (wall / args.steps) / (HBM_PEAK_GBS * 1e9),
     }
     if tune is not None:

@@ -132,15 +132,16 @@ class GpuSolver:
         except Exception:
             pass
 
-    def step(self, n_iter, instances=None, force=True):
-        """run up to n_iter iterations on the selected instances; returns the control records"""
+    def step(self, n_iter, instances=None, force=True, control=True):
+        """run up to n_iter iterations on the selected instances; returns the control records (None with
+        control=False: nothing is read back)"""
         if instances is not None:
             sel = (C.c_int32 * len(instances))(*instances)
             ns = len(instances)
         else:
             sel, ns = None, 0
-        self.ctx.check(self.lib.ws_step(self.h, int(n_iter), sel, ns, 1 if force else 0, self._ctl))
-        return [_ctl_dict(c) for c in self._ctl]
+        self.ctx.check(self.lib.ws_step(self.h, int(n_iter), sel, ns, 1 if force else 0, self._ctl if control else None))
+        return [_ctl_dict(c) for c in self._ctl] if control else None
 
     def read_control(self):
         self.ctx.check(self.lib.ws_read_control(self.h, self._ctl))
