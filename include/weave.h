@@ -229,6 +229,14 @@ WS_API int ws_download(ws_batch* batch, int32_t b, ws_state* out);              
 WS_API int ws_read_outputs(ws_batch* batch, int32_t b, double* out, int32_t cap_rows, int32_t* n_rows);
                           /* readOutputs(b) :408-411: per recorded schedule value (travel, reaction, value, iteration) */
 WS_API int ws_internalize(ws_batch* batch, int32_t b);                              /* internalize(b) :507-520 */
+/* ws_download without device allocation — no counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The same outputs as
+ * ws_download, staged in one fixed region of WS_NOALLOC_STAGE_BYTES per GPU that the library holds from
+ * ws_ctx_create until the process ends; an instance whose staging needs more (about 72 bytes per node: instances up
+ * to ~930k nodes) is refused with WS_ERR_CAPACITY.  Calls that overlap on one GPU share that region and race: one
+ * host thread per GPU at a time. */
+#define WS_NOALLOC_STAGE_BYTES (64LL << 20)
+WS_API int ws_download_noalloc(ws_batch* batch, int32_t b, ws_state* out);
 /* generalised forces of every instance at its current state — the evaluator of the reference's stability check
  * (stability.js:27-52, on the CPU there; no GpuSolver counterpart): node forces F [3n] (rod, constant and frame-local
  * loads, contacts, ground; before the anchors), body-axis torques T [3m] and the frame inertia I3 [3m] of that state,

@@ -215,6 +215,7 @@ def _open_hip_lib():
         L.ws_set_positions.argtypes = [vp, i32, P_d, P_d]
         L.ws_download.argtypes = [vp, i32, C.POINTER(ws_state)]
         L.ws_internalize.argtypes = [vp, i32]
+        L.ws_download_noalloc.argtypes = [vp, i32, C.POINTER(ws_state)]
         L.ws_batch_timing.argtypes = [vp, P_d, P_d, P_d, P_d]
         L.ws_batch_bytes.argtypes = [vp, P_d, P_d]
         L.ws_batch_profile.argtypes = [vp, i32]
@@ -240,7 +241,7 @@ def _open_hip_lib():
 WEAVE_H_SYMBOLS = [
     "ws_ctx_create", "ws_ctx_destroy", "ws_last_error", "ws_device_info", "ws_copy_probe", "ws_batch_create", "ws_batch_destroy",
     "ws_batch_size", "ws_upload", "ws_update_params", "ws_step", "ws_read_control", "ws_read_history", "ws_read_outputs", "ws_evaluate", "ws_set_positions", "ws_download",
-    "ws_internalize", "ws_batch_timing", "ws_batch_bytes", "ws_batch_profile", "ws_batch_kernel_bench",
+    "ws_internalize", "ws_download_noalloc", "ws_batch_timing", "ws_batch_bytes", "ws_batch_profile", "ws_batch_kernel_bench",
     "ws_batch_get_opts", "ws_autotune", "ws_server_create", "ws_server_destroy", "ws_submit",
     "ws_cancel", "ws_pump", "ws_poll", "ws_fetch", "ws_server_clock_ms", "ws_server_stats",
 ]

@@ -411,7 +411,10 @@ struct BatchBase {
   virtual int read_outputs(int b, double* out, int cap, int* n) = 0;
   virtual int evaluate(double* F, double* T, double* I3) = 0;
   virtual int set_xq(int count, const double* X, const double* Q) = 0;
-  virtual int download(int b, ws_state& out) = 0;
+  // stage: device memory for the outputs the download computes on the device (section forces, moments, reactions,
+  // a pending AL update), stage_cap bytes; null: buffers are allocated for the call
+  virtual int download(int b, ws_state& out, char* stage = nullptr, size_t stage_cap = 0) = 0;
+  virtual size_t download_stage_bytes(int b) = 0;
   virtual int internalize(int b) = 0;
   virtual void kernel_bench(int reps, double* out) = 0;
   virtual double alg_bytes() = 0;
@@ -845,10 +848,30 @@ struct Batch : BatchBase {
     return WS_OK;
   }
 
-  int download(int b, ws_state& o) override {
+  // the device staging one download of instance b needs: the pending AL multipliers [K], N [m], V [2m], M [3p],
+  // clamp moments [2C] and reactions [3n], each 256-byte aligned
+  size_t download_stage_bytes(int b) override {
+    const InstHost<Real>& H = at(b);
+    return align256(sizeof(double) * std::max(H.K, 1)) + align256(sizeof(double) * std::max(H.m, 1)) +
+           align256(sizeof(double) * std::max(2 * H.m, 1)) + align256(sizeof(double) * std::max(3 * H.p, 1)) +
+           align256(sizeof(double) * std::max(2 * H.C, 1)) + align256(sizeof(double) * std::max(3 * H.n, 1));
+  }
+
+  int download(int b, ws_state& o, char* stage = nullptr, size_t stage_cap = 0) override {
     InstHost<Real>& H = at(b);
     Inst<Real>& d = H.d;
     const int n = H.n, m = H.m, p = H.p, K = H.K, C = H.C;
+    if (stage && download_stage_bytes(b) > stage_cap)
+      throw Fail{WS_ERR_CAPACITY, "download without allocation: the instance needs " + std::to_string(download_stage_bytes(b)) +
+                                  " bytes of device staging, more than the " + std::to_string(stage_cap) + " the library holds for it"};
+    // carves the staging region in the order download_stage_bytes counts it
+    size_t stage_off = 0;
+    auto carve = [&](size_t count) {
+      double* q = reinterpret_cast<double*>(stage + stage_off);
+      stage_off += align256(sizeof(double) * std::max<size_t>(count, 1));
+      return q;
+    };
+    double* st_lam = stage ? carve(K) : nullptr;
     HIPCHK(hipStreamSynchronize(stream));
     Ctl c;
     HIPCHK(hipMemcpy(&c, d.ctl, sizeof(Ctl), hipMemcpyDeviceToHost));
@@ -882,13 +905,13 @@ struct Batch : BatchBase {
       if (c.al_update && d.aug) {
         // the multiplier update decided at the end of the last iteration runs at the start of the next one here;
         // the host solver has already applied it (updateMultipliers, solver_cpu.js:499-506)
-        double* dl = nullptr;
-        HIPCHK(hipMalloc(&dl, sizeof(double) * K));
+        double* dl = st_lam;
+        if (!stage) HIPCHK(hipMalloc(&dl, sizeof(double) * K));
         hipLaunchKernelGGL(k_al_preview<Real>, dim3((K + 255) / 256), dim3(256), 0, stream, d_inst, b, dl);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream));
         if (o.lam) HIPCHK(hipMemcpy(o.lam, dl, sizeof(double) * K, hipMemcpyDeviceToHost));
-        (void)hipFree(dl);
+        if (!stage) (void)hipFree(dl);
         if (o.pen_acc) std::fill(o.pen_acc, o.pen_acc + K, 0.0);
       } else {
         if (o.lam) { auto v = d2h(d.lam, K); std::copy(v.begin(), v.end(), o.lam); }
@@ -908,12 +931,21 @@ struct Batch : BatchBase {
     // section forces, moments and reactions of the last evaluated state
     if (o.N || o.Vsh || o.M || o.cM || o.reactions) {
       double *dN = nullptr, *dV = nullptr, *dM = nullptr, *dC = nullptr, *dR = nullptr;
-      HIPCHK(hipMalloc(&dN, sizeof(double) * std::max(m, 1))); HIPCHK(hipMalloc(&dV, sizeof(double) * std::max(2 * m, 1)));
-      HIPCHK(hipMalloc(&dM, sizeof(double) * std::max(3 * p, 1))); HIPCHK(hipMalloc(&dC, sizeof(double) * std::max(2 * C, 1)));
-      HIPCHK(hipMalloc(&dR, sizeof(double) * std::max(3 * n, 1)));
-      HIPCHK(hipMemset(dN, 0, sizeof(double) * std::max(m, 1))); HIPCHK(hipMemset(dV, 0, sizeof(double) * std::max(2 * m, 1)));
-      HIPCHK(hipMemset(dM, 0, sizeof(double) * std::max(3 * p, 1))); HIPCHK(hipMemset(dC, 0, sizeof(double) * std::max(2 * C, 1)));
-      HIPCHK(hipMemset(dR, 0, sizeof(double) * std::max(3 * n, 1)));
+      if (stage) {
+        // the staging region keeps the last call's values: clear the outputs (k_outputs leaves entries untouched)
+        dN = carve(m); dV = carve(2 * m); dM = carve(3 * p); dC = carve(2 * C); dR = carve(3 * n);
+        HIPCHK(hipMemsetAsync(dN, 0, sizeof(double) * std::max(m, 1), stream)); HIPCHK(hipMemsetAsync(dV, 0, sizeof(double) * std::max(2 * m, 1), stream));
+        HIPCHK(hipMemsetAsync(dM, 0, sizeof(double) * std::max(3 * p, 1), stream)); HIPCHK(hipMemsetAsync(dC, 0, sizeof(double) * std::max(2 * C, 1), stream));
+        HIPCHK(hipMemsetAsync(dR, 0, sizeof(double) * std::max(3 * n, 1), stream));
+        HIPCHK(hipStreamSynchronize(stream));
+      } else {
+        HIPCHK(hipMalloc(&dN, sizeof(double) * std::max(m, 1))); HIPCHK(hipMalloc(&dV, sizeof(double) * std::max(2 * m, 1)));
+        HIPCHK(hipMalloc(&dM, sizeof(double) * std::max(3 * p, 1))); HIPCHK(hipMalloc(&dC, sizeof(double) * std::max(2 * C, 1)));
+        HIPCHK(hipMalloc(&dR, sizeof(double) * std::max(3 * n, 1)));
+        HIPCHK(hipMemset(dN, 0, sizeof(double) * std::max(m, 1))); HIPCHK(hipMemset(dV, 0, sizeof(double) * std::max(2 * m, 1)));
+        HIPCHK(hipMemset(dM, 0, sizeof(double) * std::max(3 * p, 1))); HIPCHK(hipMemset(dC, 0, sizeof(double) * std::max(2 * C, 1)));
+        HIPCHK(hipMemset(dR, 0, sizeof(double) * std::max(3 * n, 1)));
+      }
       if (c.live_flag) {
         switch (tile) {
           case 64: hipLaunchKernelGGL((k_outputs<Real, 64>), dim3(d.ntiles), dim3(64), 0, stream, d_inst, d_tiles, b, OutPtrs{dN, dV, dM, dC, dR}); break;
@@ -925,7 +957,7 @@ struct Batch : BatchBase {
       }
       auto cp = [](double* dst, double* src, size_t nn) { if (dst && nn) HIPCHK(hipMemcpy(dst, src, nn * sizeof(double), hipMemcpyDeviceToHost)); };
       cp(o.N, dN, m); cp(o.Vsh, dV, 2 * m); cp(o.M, dM, 3 * p); cp(o.cM, dC, 2 * C); cp(o.reactions, dR, 3 * n);
-      (void)hipFree(dN); (void)hipFree(dV); (void)hipFree(dM); (void)hipFree(dC); (void)hipFree(dR);
+      if (!stage) { (void)hipFree(dN); (void)hipFree(dV); (void)hipFree(dM); (void)hipFree(dC); (void)hipFree(dR); }
     }
     o.iteration = c.iter; o.ke_prev = c.ke_prev; o.residual = c.resid; o.pen_n = (c.al_update && d.aug && K) ? 0 : c.pen_n;
     o.converged = (c.done == 1 && c.resid <= H.tol) ? 1 : 0;

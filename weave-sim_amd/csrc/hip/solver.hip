@@ -2,6 +2,38 @@
 // web/js/solver_gpu.js:62-523).  Host code only; the kernels are in kernels.hip.
 #include "batch.hpp"
 
+#include <mutex>
+
+namespace {
+
+// The staging region of ws_download_noalloc: section forces, moments, reactions and pending AL multipliers of one
+// download.  A file-scope device array, so the entry point allocates nothing; every GPU the library runs on holds
+// its own copy until the process ends.  Host code uses the per-device address from hipGetSymbolAddress, never the
+// array's name (a host-side stand-in).
+static __device__ __align__(256) char scratch[WS_NOALLOC_STAGE_BYTES];
+
+std::mutex g_stage_mu;
+std::map<int, char*> g_stage_ptr;   // device → the address of its copy of `scratch`
+
+// once per device, with that device current (ws_ctx_create)
+void stage_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  if (g_stage_ptr.count(device)) return;
+  void* ptr = nullptr;
+  HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) throw Fail{WS_ERR_DEVICE, "hipGetSymbolAddress returned no address for the download staging region"};
+  g_stage_ptr[device] = static_cast<char*>(ptr);
+}
+
+char* stage_of(int device) {
+  std::lock_guard<std::mutex> lk(g_stage_mu);
+  auto f = g_stage_ptr.find(device);
+  if (f == g_stage_ptr.end()) throw Fail{WS_ERR_STATE, "download staging region not set up for this device"};
+  return f->second;
+}
+
+}  // namespace
+
 
 // ================================================================================================= C ABI
 extern "C" {
@@ -15,6 +47,12 @@ WS_API int ws_ctx_create(int32_t device, ws_ctx** out) {
   auto* c = new ws_ctx();
   c->device = device;
   (void)hipGetDeviceProperties(&c->prop, device);
+  try {
+    stage_setup(device);
+  } catch (const Fail& f) {
+    delete c;
+    return f.code;
+  }
   *out = c;
   return WS_OK;
 }
@@ -153,6 +191,10 @@ WS_API int ws_read_outputs(ws_batch* batch, int32_t b, double* out, int32_t cap,
 WS_API int ws_download(ws_batch* batch, int32_t b, ws_state* out) {
   if (!batch || !out || b < 0 || b >= batch->impl->B) return WS_ERR_ARG;
   BATCH_CALL(batch->impl->download(b, *out));
+}
+WS_API int ws_download_noalloc(ws_batch* batch, int32_t b, ws_state* out) {
+  if (!batch || !out || b < 0 || b >= batch->impl->B) return WS_ERR_ARG;
+  BATCH_CALL(batch->impl->download(b, *out, stage_of(batch->impl->ctx->device), WS_NOALLOC_STAGE_BYTES));
 }
 WS_API int ws_internalize(ws_batch* batch, int32_t b) {
   if (!batch || b < 0 || b >= batch->impl->B) return WS_ERR_ARG;

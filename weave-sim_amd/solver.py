@@ -147,10 +147,12 @@ class GpuSolver:
         self.ctx.check(self.lib.ws_read_control(self.h, self._ctl))
         return [_ctl_dict(c) for c in self._ctl]
 
-    def download(self, b, into=None):
-        """state of instance b into `into` (StateBuffers) or into the model's own arrays (GpuSolver.download)"""
+    def download(self, b, into=None, noalloc=False):
+        """state of instance b into `into` (StateBuffers) or into the model's own arrays (GpuSolver.download);
+        noalloc: ws_download_noalloc (no device allocation, the library's fixed staging region)"""
         tgt = into if into is not None else self.models[b]
-        self.ctx.check(self.lib.ws_download(self.h, b, C.byref(tgt.st)))
+        fn = self.lib.ws_download_noalloc if noalloc else self.lib.ws_download
+        self.ctx.check(fn(self.h, b, C.byref(tgt.st)))
         return tgt
 
     def state(self, b):
