@@ -175,7 +175,7 @@ def cpu_variants_baseline(models, iters):
 def solver_opts(args):
     """the batch options of this run: the tile, graph length and crossing layout given on the command line (0 = the
     library's choice by size)"""
-    return {"tile": args.tile, "per_submit": args.per_submit, "cross_lanes": args.cross_lanes}
+    return {"tile": args.tile, "per_submit": args.per_submit, "cross_lanes": args.cross_lanes, "ctrl_mode": args.ctrl_mode}
 
 
 def kernel_only(args):
@@ -213,7 +213,7 @@ def counter_pass(args, counter, outdir):
     cmd = ["timeout", "-s", "KILL", "150", "rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", outdir,
            "-o", "pass", "--", sys.executable, os.path.abspath(__file__), "--kernel-only", "--workload", args.workload,
            "--precision", args.precision, "--tile", str(args.tile), "--per-submit", str(args.per_submit),
-           "--cross-lanes", str(args.cross_lanes), "--batch", str(args.batch)]
+           "--cross-lanes", str(args.cross_lanes), "--ctrl-mode", str(args.ctrl_mode), "--batch", str(args.batch)]
     env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
     r = subprocess.run(cmd, cwd=tempfile.gettempdir(), env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
@@ -454,6 +454,8 @@ def main():
                          "workload, the default elsewhere")
     ap.add_argument("--per-submit", type=int, default=0, help="iterations per captured graph (0 = the default, 64)")
     ap.add_argument("--cross-lanes", type=int, default=0, help="lanes per crossing, 1 or 2 (0 = by size)")
+    ap.add_argument("--ctrl-mode", type=int, default=0,
+                    help="0: a control launch after the strand launch; 1: the strand launch runs the control step")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--no-solves", action="store_true", help="leave out the batched-vs-plain solves comparison")
@@ -539,7 +541,20 @@ def main():
     sb, cb = strand_cross_bytes(models, args.precision)
     achieved = sb / (kin["ms_strand"] * 1e-3) / 1e9
     alg = g.bytes()["alg_bytes_per_iter"]
+    # the host-buffer boundary (not part of `value`): every instance uploaded from its host model and state, and
+    # downloaded into host buffers allocated beforehand
+    bufs = [W.StateBuffers(m.md) for m in models]
+    t2 = time.perf_counter()
+    for b in range(len(models)):
+        g.upload(b)
+    t3 = time.perf_counter()
+    for b in range(len(models)):
+        g.download(b, bufs[b])
+    t4 = time.perf_counter()
     g.close()
+    solve_its = 5000
+    host = {"upload_s": t3 - t2, "download_s": t4 - t3, "iterations_per_solve": solve_its,
+            "node_iters_per_s_with_transfers": n_nodes * solve_its / (t4 - t2 + solve_its * wall / args.steps)}
     line = {
         "metric": "node-iters/s",
         "value": value,
@@ -555,7 +570,7 @@ def main():
         "data": "synthetic (seeded kagome gridshells from the host model builder)",
         "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]}", "instances": len(models),
                    "nodes": n_nodes, "crossings": n_cross, "precision": args.precision, "strand_tile": opts["tile"],
-                   "per_submit": opts["per_submit"], "cross_lanes": opts["cross_lanes"],
+                   "per_submit": opts["per_submit"], "cross_lanes": opts["cross_lanes"], "ctrl_mode": opts["ctrl_mode"],
                    "autotuned": tune is not None, "device": dev["name"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_strand",
@@ -569,6 +584,7 @@ def main():
         "alg_bytes_per_iter": alg,
         "alg_bytes_per_launch": {"k_strand": sb, "k_cross": cb},
         "alg_bytes_frac_of_peak_per_iteration": alg / (wall / args.steps) / (HBM_PEAK_GBS * 1e9),
+        "host_transfers": host,
     }
     if tune is not None:
         line["autotune"] = tune["rows"]

@@ -14,7 +14,8 @@
  *  - the host model (ws_model) is read on upload; device memory is owned by the library; downloads write
  *    into caller-owned buffers (ws_state), as GpuSolver.download overwrites the CpuSolver's typed arrays
  *    (solver_gpu.js:431-503);
- *  - one host thread per batch handle; a context may hold many batches (they run on one HIP stream each).
+ *  - calls on one batch or server handle are serialised by the library (a call made while another runs on a
+ *    second thread waits for it); a context may hold many batches (they run on one HIP stream each).
  */
 #ifndef WEAVE_H
 #define WEAVE_H
@@ -180,11 +181,15 @@ typedef struct ws_batch_opts {
   int32_t per_submit;      /* iterations per captured graph (GpuSolver perSubmit, default 64) */
   int32_t tile;            /* lanes per strand tile: 64 / 128 / 256; 0 = default (64); -1 = autotune */
   int32_t history;         /* history ring length per instance (0 = 4096, the reference's HN) */
-  int32_t ctrl_chunk;      /* strand partials per control workgroup (0 = by size: one workgroup up to 2048, else 1024
-                              each, the last to finish combining them) */
+  int32_t ctrl_chunk;      /* strand partials per control reduction group (0 = by size: with a control launch one
+                              workgroup up to 2048, else 1024 each, the last to finish combining them; 256 when the
+                              strand launch runs the control step) */
   int32_t cross_lanes;     /* lanes per crossing in the contact kernel: 2 (the sides split, shorter chains), 1 (one lane
                               takes both sides, fewer instructions per crossing), 0 = by size (1 from 131,072
                               crossings in the batch) or the autotuner's pick */
+  int32_t ctrl_mode;       /* 0: a control launch after the strand launch (the default); 1: the strand launch's last
+                              tiles per instance run the control step (no control launch; slower on gfx950, where
+                              every tile's device-scope release writes back its L2) */
 } ws_batch_opts;
 
 #define WS_TILE_AUTOTUNE (-1)
@@ -269,7 +274,7 @@ WS_API int ws_autotune(ws_ctx* ctx, const ws_model* const* models, const ws_stat
  * free and their device bytes fit the HBM budget; every pump relaxes all resident instances one chunk in shared
  * launches and retires the ones that converged, reached their iteration limit or went non-finite, writing the final
  * state into the request's ws_state.  Model and state stay caller-owned and valid until the result is polled.
- * One host thread per server. */
+ * Calls on one server are serialised like those on a batch. */
 typedef struct ws_server_opts {
   int32_t precision;       /* ws_precision */
   int32_t chunk;           /* iterations per pump (0 = 64) */

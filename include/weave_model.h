@@ -125,6 +125,18 @@ WS_API int wsm_mesh_counts(wsm_model* mdl, int32_t* nV, int32_t* nF);
 WS_API int wsm_stability_layout(const ws_model* md, const double* u, int32_t W, int32_t* n_dofs, int64_t* nnz,
                                 int32_t* n_colours, int32_t* dof_kind, int32_t* dof_index, int32_t* dof_comp,
                                 int64_t* row_ptr, int32_t* cols, int32_t* colour);
+/* the crossings' identities (st.knotEdges and topo.knots, model.js:49-63, topology.js:50, 145): per crossing its mesh
+ * edge id (the key of the reference's knot results; explicit crossings of stand-alone laths are keyed by their
+ * index), the frames [strand, k] of sides A and B, and the strand of the frame that passes over — the columns
+ * knot_edge, strand_a, k_a, strand_b, k_b, over of the reference's knots.csv.  out [6 · min(cap_rows, K)] */
+WS_API int wsm_knots(wsm_model* mdl, int32_t* out, int32_t cap_rows, int32_t* K);
+/* the frames of the model (st.nodeFrameIds, model.js:24-25): rows (strand, frame k, node) in frame-key order, the
+ * keys of the reference's reactions ("s,k", app.js:366); out [3 · min(cap_rows, n_frames)] */
+WS_API int wsm_frames(wsm_model* mdl, int32_t* out, int32_t cap_rows, int32_t* n_frames);
+/* CpuSolver.lockContacts (lock = 1) / unlockContacts (lock = 0) (solver_cpu.js:869-871): the crossings on the listed
+ * mesh edges, all of them when edges is NULL; edges without a crossing are ignored.  ws_update_params takes the new
+ * locks to the device (app.js:192 does the same with updateParams). */
+WS_API int wsm_lock_contacts(wsm_model* mdl, const int32_t* edges, int32_t n_edges, int32_t lock);
 WS_API const char* wsm_last_error(void);
 /* sizeof of the ABI structs, for binding checks: ws_params, ws_model, ws_state, ws_control, ws_batch_opts,
  * ws_server_opts, ws_result, wsm_scene, wsm_anchor, wsm_point_load, wsm_prep, wsm_hex_spec, wsm_info */

@@ -48,7 +48,8 @@ console.log(JSON.stringify({{info: m.info, x: Array.from(m.st.X.slice(0, 6)), u:
 
 @pytest.mark.gpu
 def test_js_gpu_solver_matches_python_path(W, gpu_ctx):
-    """600 iterations of the 3 m dome through the JS GpuSolver (Promises, in-place download) equal the Python path"""
+    """600 iterations of the 3 m dome through the JS GpuSolver (Promises, in-place download) equal the Python path;
+    the download reads the history rows too (solver_gpu.js:499), so a readHistory after it finds none new"""
     out = run_node(f"""
 (async () => {{
   const dev = await W.getDevice();
@@ -71,7 +72,7 @@ def test_js_gpu_solver_matches_python_path(W, gpu_ctx):
     assert d["ctl"]["iter"] == c["iter"] == 600 and d["it"] == 600
     assert np.array_equal(np.array(d["X"]), s["X"])
     assert np.array_equal(np.array(d["stuck"], dtype=np.uint8), s["stuck"])
-    assert d["ctl"]["ke"] == c["ke"] and d["hist"] == d["n"] == 600
+    assert d["ctl"]["ke"] == c["ke"] and d["hist"] == 600 and d["n"] == 0
 
 
 @pytest.mark.gpu
@@ -219,7 +220,7 @@ def test_js_server_matches_python_path(W, gpu_ctx):
                                { E: 10000, G: 600, density: 500, width: 40, height: 4, mu: 0.3, tol: 1e-6 }), 300);
   }
   const res = [];
-  for (let r = 0; r < 100 && res.length < 3; r++) { await srv.pump(); for (const x of srv.poll()) res.push(x); }
+  for (let r = 0; r < 100 && res.length < 3; r++) { await srv.pump(); for (const x of await srv.poll()) res.push(x); }
   srv.dispose();
   console.log(JSON.stringify(res.map((x) => ({ id: x.id, iterations: x.iterations, status: x.status, X: Array.from(x.model.st.X) }))));
 })().catch((e) => { console.error(e); process.exit(1); });""")
@@ -233,3 +234,182 @@ def test_js_server_matches_python_path(W, gpu_ctx):
         g.close()
         assert x["iterations"] == c["iter"] == 300
         assert np.array_equal(np.array(x["X"]), s["X"]), seed
+
+
+SHAPE = os.path.join(ROOT, "tests", "js", "cpu_solver_shape.js")
+SCENE_JS = os.path.join(ROOT, "weave-sim_amd", "js", "scene_mi355x.js")
+GOLDEN = os.path.join(ROOT, "tests", "golden", "scenes")
+
+
+def test_cpu_solver_field_map_round_trips():
+    """an object with the reference's CpuSolver fields (st.segI0, beam.cSeg, contact.enabled, loads.pointLocal, …),
+    assembled from a built model, maps onto ws_model / ws_state exactly: the native model over its arrays gives back
+    every array and scalar of the original (the 3 m dome and the reference's five example scenes: clamps, frame-local
+    loads, hybrid lath groups)"""
+    out = run_node(f"""
+const {{ asCpuSolver }} = require({json.dumps(SHAPE)});
+const S = require({json.dumps(SCENE_JS)});
+const path = require('path');
+const models = [['dome_3m', W.Model.hexDome({{span:3000,spacing:300,segment:50,contraction:0.15,noise:1,seed:1}}, {SCENE})]];
+for (const name of ['dome_basic', 'v5dome_cm', 'tree_cm', 'dualv5dome_cm', 'dualv5dome_cm_hybrid'])
+  models.push([name, S.loadScene(path.join({json.dumps(GOLDEN)}, name + '.json'))]);
+// dome_basic with its lath ends clamped (rotation anchors → clamp joints)
+const file = path.join({json.dumps(GOLDEN)}, 'dome_basic.json'), fs = require('fs');
+const d = JSON.parse(fs.readFileSync(file, 'utf8'));
+d.anchors.forEach((a) => {{ a.fix = [1, 1, 1, 1, 1, 1]; }});
+const mp = path.resolve(path.dirname(file), S.sceneFromJSON(d).mesh.path);
+models.push(['dome_basic_clamped', S.modelFromScene(d, fs.readFileSync(mp), mp)]);
+const same = (a, b) => a.length === b.length && a.every((x, i) => x === b[i] || (x !== x && b[i] !== b[i]));
+const res = {{}};
+for (const [name, m] of models) {{
+  const f = m.fields();
+  const cpu = asCpuSolver(f);
+  const g = W._internal.modelFields(W._internal.modelFromFields(W._internal.cpuSolverFields(cpu).fields));
+  const bad = [];
+  for (const k of Object.keys(f)) {{
+    if (ArrayBuffer.isView(f[k])) {{ if (!g[k] || !same(f[k], g[k])) bad.push(k); }}
+    else if (k !== 'par' && !(f[k] === g[k] || (f[k] !== f[k] && g[k] !== g[k]))) bad.push(k);
+  }}
+  for (const k of Object.keys(f.par)) if (!(f.par.floor_on === 0 && k.startsWith('floor_')) && f.par[k] !== g.par[k]) bad.push('par.' + k);
+  res[name] = {{ bad, C: f.C, npl: f.n_point_local, K: f.K }};
+}}
+console.log(JSON.stringify(res));""")
+    d = json.loads(out)
+    assert d["dome_basic_clamped"]["C"] > 0 and d["dome_basic"]["npl"] > 0 and d["dome_3m"]["K"] > 0
+    for name, v in d.items():
+        assert v["bad"] == [], (name, v["bad"])
+
+
+def test_js_tables_format_like_python():
+    """knotsCSV / segmentsCSV in Node give the Python writers' text byte for byte (floats as Python's repr) over the
+    same state, and resultsJSON keys knots by mesh edge id"""
+    import importlib
+    R = importlib.import_module("weave-sim_amd.results")
+    out = run_node(f"""
+const m = W.Model.hexDome({{span:3000,spacing:300,segment:50,contraction:0.15,noise:1,seed:1}}, {SCENE});
+const r = W.buildResults(m);
+console.log(JSON.stringify({{ knots: W.knotsCSV(m, r), segments: W.segmentsCSV(m, r), keys: Object.keys(W.resultsJSON(m, r).knot.Fn).slice(0, 5) }}));""")
+    d = json.loads(out)
+    py = cases.config1()
+    r = R.build_results(py, py)
+    assert d["knots"] == R.knots_csv(py, r) and d["segments"] == R.segments_csv(py, r)
+    assert d["keys"] == [str(int(e)) for e in py.knot_edges[:5]]
+
+
+@pytest.mark.gpu
+def test_js_cpu_solver_objects_step_like_the_package_models(W, gpu_ctx):
+    """new GpuSolver(device, [CpuSolver-shaped object]) (solver_gpu.js:65, app.js:131): 600 iterations of the 3 m dome
+    given as the reference's fields equal the same model through the Python path, bitwise, with the download written
+    into the object's own arrays (st.X, st.stuck, beam.N, anchors.reactions, contact.Fn); then a scene edit (μ, loads)
+    taken by updateParams, and a changed clamp set rejected"""
+    out = run_node(f"""
+const {{ asCpuSolver }} = require({json.dumps(SHAPE)});
+(async () => {{
+  const dev = await W.getDevice();
+  if (!dev) throw new Error('no device');
+  const m = W.Model.hexDome({{span:3000,spacing:300,segment:50,contraction:0.15,noise:1,seed:1}}, {SCENE});
+  const cpu = asCpuSolver(m.fields());
+  const X = cpu.st.X;
+  const g = new W.GpuSolver(dev, [cpu], {{ perSubmit: 64, tile: 64 }});
+  const c = await g.step(600);
+  await g.download(0);
+  const first = {{ ctl: c[0], X: Array.from(cpu.st.X), same: cpu.st.X === X, stuck: Array.from(cpu.st.stuck),
+    N: Array.from(cpu.beam.N), R: Array.from(cpu.anchors.reactions), Fn: Array.from(cpu.contact.Fn),
+    it: cpu.st.iteration, hist: cpu.st.history.kinetic.length }};
+  cpu.scene.contact.mu = 0.12;
+  for (let i = 0; i < cpu.loads.const.length; i++) cpu.loads.const[i] *= 1.5;
+  await g.updateParams(0);
+  await g.step(300);
+  await g.download(0);
+  const second = {{ X: Array.from(cpu.st.X), it: cpu.st.iteration }};
+  cpu.beam.cSeg = [0]; cpu.beam.cNode = [0]; cpu.beam.cLen = [cpu.st.l0[0] / 2]; cpu.beam.cB = [[1, 1, 1]];
+  cpu.beam.cQ = cpu.st.Q.slice(0, 4); cpu.beam.cM = new Float64Array(2);
+  let refused = '';
+  try {{ await g.updateParams(0); }} catch (e) {{ refused = e.message; }}
+  await g.dispose();
+  console.log(JSON.stringify({{ first, second, refused }}));
+}})().catch((e) => {{ console.error(e); process.exit(1); }});""")
+    d = json.loads(out)
+    m = cases.config1()
+    g = W.GpuSolver([m], precision="fp32", tile=64, per_submit=64)
+    c = g.step(600)[0]
+    s = g.state(0)
+    f = d["first"]
+    assert f["same"] and f["it"] == 600 and f["hist"] == 600 and f["ctl"]["ke"] == c["ke"]
+    assert np.array_equal(np.array(f["X"]), s["X"]) and np.array_equal(np.array(f["stuck"], dtype=np.uint8), s["stuck"])
+    assert np.array_equal(np.array(f["N"]), s["N"]) and np.array_equal(np.array(f["R"]), s["reactions"])
+    assert np.array_equal(np.array(f["Fn"]), s["Fn"])
+    m.md.par.mu = 0.12
+    m.arr("load_const")[:] *= 1.5
+    g.update_params(0)
+    g.step(300)
+    s2 = g.state(0)
+    g.close()
+    assert d["second"]["it"] == 900 and np.array_equal(np.array(d["second"]["X"]), s2["X"])
+    assert "clamps" in d["refused"]
+
+
+@pytest.mark.gpu
+def test_js_calls_made_while_a_pump_runs_wait_for_it(W, gpu_ctx):
+    """a request submitted, results polled and the solver's own calls made while a pump() or step() Promise is pending
+    run after it (one chain per handle, and the C ABI's lock): the results equal the sequential order, bitwise"""
+    out = run_node(f"""
+(async () => {{
+  const dev = await W.getDevice();
+  const dome = (seed) => W.Model.hexDome({{ span: 1200, spacing: 300, segment: 50, contraction: 0.15, noise: 1, seed }}, {SCENE});
+  const srv = new W.Server(dev, {{ chunk: 64, tile: 64, maxSlots: 4 }});
+  await srv.submit(dome(1), 300);
+  const pending = srv.pump();                   // on a worker thread
+  const id2 = srv.submit(dome(2), 300);         // made while the pump runs
+  const early = srv.poll();
+  await pending;
+  const res = [...(await early)];
+  for (let r = 0; r < 100 && res.length < 2; r++) {{ await srv.pump(); for (const x of await srv.poll()) res.push(x); }}
+  await srv.dispose();
+  // a solver: step, updateParams, download and dispose issued back to back, none awaited before the next
+  const m = dome(3);
+  const g = new W.GpuSolver(dev, [m], {{ perSubmit: 64, tile: 64 }});
+  const s1 = g.step(200); const u = g.updateParams(0); const s2 = g.step(100); const dl = g.download(0); const dp = g.dispose();
+  const [c1, , c2] = await Promise.all([s1, u, s2, dl, dp]);
+  let after = '';
+  try {{ await g.step(1); }} catch (e) {{ after = e.message; }}
+  console.log(JSON.stringify({{ id2: await id2, res: res.map((x) => ({{ id: x.id, it: x.iterations, X: Array.from(x.model.st.X) }})),
+    c1: c1[0].iter, c2: c2[0].iter, X: Array.from(m.st.X), after }}));
+}})().catch((e) => {{ console.error(e); process.exit(1); }});""")
+    d = json.loads(out)
+    res = sorted(d["res"], key=lambda x: x["id"])
+    assert [x["id"] for x in res] == [1, 2] and d["id2"] == 2
+    for seed, x in zip((1, 2), res):
+        g = W.GpuSolver([cases.config_small(seed=seed)], precision="fp32", tile=64)
+        g.step(300)
+        s = g.state(0)
+        g.close()
+        assert x["it"] == 300 and np.array_equal(np.array(x["X"]), s["X"]), seed
+    m = cases.config_small(seed=3)
+    g = W.GpuSolver([m], precision="fp32", tile=64)
+    g.step(200)
+    g.update_params(0)
+    g.step(100)
+    s = g.state(0)
+    g.close()
+    assert d["c1"] == 200 and d["c2"] == 300 and np.array_equal(np.array(d["X"]), s["X"])
+    assert "disposed" in d["after"]
+
+
+@pytest.mark.gpu
+def test_js_failed_device_call_rejects_its_promise(W, gpu_ctx):
+    """a call the C ABI refuses on the worker thread (an instance index out of range) rejects its Promise with the
+    library's message; the solver keeps working"""
+    out = run_node(f"""
+(async () => {{
+  const dev = await W.getDevice();
+  const m = W.Model.hexDome({{ span: 1200, spacing: 300, segment: 50, contraction: 0.15, noise: 1, seed: 1 }}, {SCENE});
+  const g = new W.GpuSolver(dev, [m], {{ perSubmit: 64, tile: 64 }});
+  let msg = '';
+  try {{ await g.step(10, {{ instances: [7] }}); }} catch (e) {{ msg = e.message; }}
+  const c = await g.step(10);
+  await g.dispose();
+  console.log(JSON.stringify({{ msg, it: c[0].iter }}));
+}})().catch((e) => {{ console.error(e); process.exit(1); }});""")
+    d = json.loads(out)
+    assert "out of range" in d["msg"] and d["it"] == 10

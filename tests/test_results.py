@@ -52,3 +52,45 @@ def test_results_json_document():
     rc = R.build_results(c, o.state())
     assert set(rc["knot"]) == {"Fn", "Ft", "slip", "state", "gap", "ctype", "pressure", "util", "theta", "phiA", "phiB"}
     assert len(rc["knot"]["Fn"]) == c.md.K and rc["knot"]["Fn"][0] > 0
+
+
+def _golden(name):
+    import os
+    return open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "report", name)).read().splitlines()
+
+
+def test_knots_and_segments_tables_have_the_report_layout():
+    """knots.csv / segments.csv as the reference's reports lay them out (the first lines of
+    examples/scenes/report-20261004-114809/knots.csv and segments.csv are fixtures): the same header, the same column
+    kinds (integer ids, then floats as Python's repr), one knot row per crossing in knot-edge order with its frames
+    and over strand, one segment row per segment strand by strand; the values are those of the results"""
+    m = cases.config1()
+    o = Oracle(m)
+    o.step(400)
+    st = o.state()
+    r = R.build_results(m, st)
+    kt, sg = R.knots_csv(m, r).splitlines(), R.segments_csv(m, r).splitlines()
+    gk, gs = _golden("knots_head.csv"), _golden("segments_head.csv")
+    assert kt[0] == gk[0] and sg[0] == gs[0]
+    for ours, theirs in ((kt[1], gk[1]), (sg[1], gs[1])):
+        a, b = ours.split(","), theirs.split(",")
+        assert len(a) == len(b)
+        for x, y in zip(a, b):
+            assert ("." in x or "e" in x) == ("." in y or "e" in y), (x, y)      # ids integers, values floats
+    kn = m.knots()
+    assert len(kt) == 1 + m.K and (np.diff(kn[:, 0]) > 0).all()
+    assert ((kn[:, 5] == kn[:, 1]) | (kn[:, 5] == kn[:, 3])).all()            # the over strand is one of the two
+    rows = np.array([[float(v) for v in line.split(",")] for line in kt[1:]])
+    assert np.array_equal(rows[:, :6].astype(np.int64), kn)
+    for col, name in zip(range(6, 11), ("Fn", "Ft", "gap", "slip", "state")):
+        assert np.array_equal(rows[:, col], r["knot"][name]), name
+    seg = np.array([[float(v) for v in line.split(",")] for line in sg[1:]])
+    assert len(seg) == m.md.m
+    soff = m.arr("seg_offset")
+    for s in range(m.md.S):
+        blk = seg[soff[s]:soff[s + 1]]
+        assert (blk[:, 0] == s).all() and np.array_equal(blk[:, 1], np.arange(soff[s + 1] - soff[s]))
+    for col, name in zip(range(2, 6), ("Mx", "My", "T", "N")):
+        assert np.array_equal(seg[:, col], r["segment"][name]), name
+    doc = R.results_json(m, r)
+    assert list(doc["knot"]["Fn"])[:3] == [str(int(e)) for e in kn[:3, 0]]

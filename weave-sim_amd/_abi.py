@@ -71,7 +71,7 @@ class ws_control(C.Structure):
 
 class ws_batch_opts(C.Structure):
     _fields_ = [("precision", i32), ("per_submit", i32), ("tile", i32), ("history", i32), ("ctrl_chunk", i32),
-                ("cross_lanes", i32)]
+                ("cross_lanes", i32), ("ctrl_mode", i32)]
 
 
 WS_TILE_AUTOTUNE = -1
@@ -183,6 +183,9 @@ def _open_model_lib():
         L.wsm_segment_moments.argtypes = [C.POINTER(ws_model), P_d, P_d, P_d, P_d, P_d]
         L.wsm_stability_layout.argtypes = [C.POINTER(ws_model), P_d, i32, C.POINTER(i32), C.POINTER(i64), C.POINTER(i32),
                                            P_i32, P_i32, P_i32, C.POINTER(i64), P_i32, P_i32]
+        L.wsm_knots.argtypes = [vp, P_i32, i32, C.POINTER(i32)]
+        L.wsm_frames.argtypes = [vp, P_i32, i32, C.POINTER(i32)]
+        L.wsm_lock_contacts.argtypes = [vp, P_i32, i32, i32]
         L.wsm_last_error.restype = C.c_char_p
         L.wsm_free.argtypes = [vp]
         L.wsm_free.restype = None
@@ -248,7 +251,7 @@ WEAVE_H_SYMBOLS = [
 WEAVE_MODEL_H_SYMBOLS = [
     "wsm_scene_defaults", "wsm_from_mesh", "wsm_from_mesh_groups", "wsm_from_polylines", "wsm_hex_dome", "wsm_get", "wsm_info_get",
     "wsm_add_load", "wsm_set_strength", "wsm_clone", "wsm_mesh_counts", "wsm_last_error", "wsm_free",
-    "wsm_abi_sizes", "wsm_segment_moments", "wsm_stability_layout",
+    "wsm_abi_sizes", "wsm_segment_moments", "wsm_stability_layout", "wsm_knots", "wsm_frames", "wsm_lock_contacts",
 ]
 ABI_STRUCTS = ["ws_params", "ws_model", "ws_state", "ws_control", "ws_batch_opts", "ws_server_opts", "ws_result",
                "wsm_scene", "wsm_anchor", "wsm_point_load", "wsm_prep", "wsm_hex_spec", "wsm_info", "ws_tune_entry"]

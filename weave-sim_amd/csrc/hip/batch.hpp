@@ -13,6 +13,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <type_traits>
@@ -24,7 +25,7 @@
 namespace wsd {
 template <typename Real> __global__ void k_begin(Ctl*, const int32_t*, const int32_t*, int32_t, int32_t, int32_t);
 template <typename Real, int LANES, bool ONE> __global__ void k_cross(const Inst<Real>*, const CrossItem*, int32_t, const Ctl*, int32_t);
-template <typename Real, int TILE, bool RB, bool ONE, int WPE> __global__ void k_strand(const Inst<Real>*, const TileItem*, int32_t, const Ctl*);
+template <typename Real, int TILE, bool RB, bool ONE, int WPE, bool FUSED> __global__ void k_strand(const Inst<Real>*, const TileItem*, int32_t, Ctl*);
 template <typename Real, int TILE> __global__ void k_rigid(const Inst<Real>*, const TileItem*, int32_t, const Ctl*);
 template <typename Real, bool ONE> __global__ void k_ctrl(const Inst<Real>*, const CtrlItem*, int32_t, Ctl*);
 __global__ void k_copy(const uint4*, uint4*, int64_t);
@@ -119,9 +120,11 @@ inline uint8_t fix_bits(const ws_model& md, int i) {
 }
 
 // build the device image of one host model (GpuSolver.upload, solver_gpu.js:191-310)
+// fused: the strand launch runs the control step (chunks of FUSED_CHUNK tiles by default); else the control kernel
+// (measured: fusing is slower on gfx950 — each tile's device-scope release fence writes back its XCD's L2)
 template <typename Real>
 void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, int tile, int hist_len, Ctl* ctl,
-                    int ctrl_chunk = 0, bool layout_only = false) {
+                    int ctrl_chunk = 0, bool layout_only = false, bool fused = false) {
   const int n = md.n, m = md.m, p = md.p, C = md.C, S = md.S;
   const int K = md.par.contacts ? md.K : 0;   // no contact goal: no crossings on the device
   if (n <= 0 || m < 0 || S <= 0) throw Fail{WS_ERR_ARG, "model has no nodes or strands"};
@@ -226,10 +229,11 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   A.add(d.spart, size_t(d.ntiles) * SPART_STRIDE); A.add(d.cpart, size_t(std::max(H.ncblk, 1)) * CPART);
   {
     const int np = std::max(1, d.ntiles);
-    d.ctrl_chunk = ctrl_chunk > 0 ? ctrl_chunk : (np <= CTRL_ONE ? np : CTRL_CHUNK);
+    d.ctrl_chunk = ctrl_chunk > 0 ? ctrl_chunk : (fused ? FUSED_CHUNK : (np <= CTRL_ONE ? np : CTRL_CHUNK));
     d.ctrl_chunks = (np + d.ctrl_chunk - 1) / d.ctrl_chunk;
   }
   A.add(d.rpart, size_t(d.ctrl_chunks) * RPART);
+  A.add(d.ccount, size_t(d.ctrl_chunks));
   if (d.rigid) A.add(d.rbpart, size_t(d.ntiles) * RBP);
   if (d.proto) { A.addc(d.sched, d.n_inc); A.add(d.pout, 4 * size_t(d.n_inc)); A.add(d.preact, 3); }
   if (layout_only) return;   // A.size: the device bytes the instance would take
@@ -388,6 +392,7 @@ struct BatchBase {
   ws_ctx* ctx = nullptr;
   int B = 0, precision = 0, tile = 128, per_submit = 64, hist_len = 4096;
   int ctrl_chunk = 0;                   // strand partials per control workgroup (0: by size)
+  bool fused_ctrl = false;              // the strand launch runs the control step (else the control kernel)
   bool has_rigid = false;               // an instance that only the ground holds: the rigid step runs
   int cross_lanes = 0;                  // lanes per crossing (0: by size at every relist)
   int lanes = 2;                        // ... as the current work lists use it
@@ -463,7 +468,7 @@ struct Batch : BatchBase {
     HIPCHK(hipMemset(d_ctls, 0, sizeof(Ctl) * std::max(B, 1)));
     for (int b = 0; b < B; b++) {
       inst.emplace_back(new InstHost<Real>());
-      build_instance(*inst.back(), *models[b], *states[b], tile, hist_len, d_ctls + b, ctrl_chunk);
+      build_instance(*inst.back(), *models[b], *states[b], tile, hist_len, d_ctls + b, ctrl_chunk, false, fused_ctrl);
       upload_state(*inst.back(), *states[b]);
     }
     relist();
@@ -490,7 +495,7 @@ struct Batch : BatchBase {
     if (slot < 0 || slot >= B || inst[slot]) throw Fail{WS_ERR_ARG, "add_instance: slot out of range or occupied"};
     HIPCHK(hipStreamSynchronize(stream));
     std::unique_ptr<InstHost<Real>> h(new InstHost<Real>());
-    build_instance(*h, md, st, tile, hist_len, d_ctls + slot, ctrl_chunk);
+    build_instance(*h, md, st, tile, hist_len, d_ctls + slot, ctrl_chunk, false, fused_ctrl);
     upload_state(*h, st);
     inst[slot] = std::move(h);
   }
@@ -572,9 +577,12 @@ struct Batch : BatchBase {
       else hipLaunchKernelGGL((k_cross<Real, 2, false>), dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls, K);
     }
   }
+  template <int T, bool RB, int W, bool F> void strand_wf(hipStream_t s) {
+    if (one_inst()) hipLaunchKernelGGL((k_strand<Real, T, RB, true, W, F>), dim3(ntiles), dim3(T), 0, s, d_inst, d_tiles, ntiles, d_ctls);
+    else hipLaunchKernelGGL((k_strand<Real, T, RB, false, W, F>), dim3(ntiles), dim3(T), 0, s, d_inst, d_tiles, ntiles, d_ctls);
+  }
   template <int T, bool RB, int W> void strand_w(hipStream_t s) {
-    if (one_inst()) hipLaunchKernelGGL((k_strand<Real, T, RB, true, W>), dim3(ntiles), dim3(T), 0, s, d_inst, d_tiles, ntiles, d_ctls);
-    else hipLaunchKernelGGL((k_strand<Real, T, RB, false, W>), dim3(ntiles), dim3(T), 0, s, d_inst, d_tiles, ntiles, d_ctls);
+    if (fused_ctrl) strand_wf<T, RB, W, true>(s); else strand_wf<T, RB, W, false>(s);
   }
   // the FP64 form of the 64-lane tile runs three resident wavefronts per SIMD once the batch has enough tiles to keep
   // them busy (measured: 10 % faster on the 30 m dome's 14k tiles, 4 % slower on the 10 m dome's 1,350)
@@ -591,7 +599,7 @@ struct Batch : BatchBase {
     }
   }
   void launch_ctrl(hipStream_t s) {
-    if (nctrl) {
+    if (nctrl && !fused_ctrl) {
       if (one_inst()) hipLaunchKernelGGL((k_ctrl<Real, true>), dim3(nctrl), dim3(CTRL_THREADS), 0, s, d_inst, d_ctrl, nctrl, d_ctls);
       else hipLaunchKernelGGL((k_ctrl<Real, false>), dim3(nctrl), dim3(CTRL_THREADS), 0, s, d_inst, d_ctrl, nctrl, d_ctls);
     }
@@ -973,7 +981,7 @@ struct Batch : BatchBase {
     HIPCHK(hipStreamSynchronize(stream));
     H.arena.release();
     H.arena = Arena();
-    build_instance(H, md, st, tile, hist_len, d_ctls + b, ctrl_chunk);
+    build_instance(H, md, st, tile, hist_len, d_ctls + b, ctrl_chunk, false, fused_ctrl);
     upload_state(H, st);
     relist();
     return WS_OK;
@@ -1063,7 +1071,7 @@ int fail(ws_ctx* ctx, const Fail& f) { if (ctx) ctx->err = f.msg; return f.code;
 // the batch of the given options (tile already resolved to 64 / 128 / 256)
 std::unique_ptr<BatchBase> make_batch(ws_ctx* ctx, const ws_model* const* models, const ws_state* const* states, int B,
                                       int prec, int tile, int per_submit, int history, int ctrl_chunk = 0,
-                                      int cross_lanes = 0) {
+                                      int cross_lanes = 0, bool fused_ctrl = false) {
   std::unique_ptr<BatchBase> b;
   if (prec == WS_FP64) b.reset(new Batch<double>()); else b.reset(new Batch<float>());
   b->ctx = ctx; b->precision = prec;
@@ -1071,6 +1079,7 @@ std::unique_ptr<BatchBase> make_batch(ws_ctx* ctx, const ws_model* const* models
   if (history > 0) b->hist_len = history;
   b->ctrl_chunk = std::max(ctrl_chunk, 0);
   b->cross_lanes = cross_lanes;
+  b->fused_ctrl = fused_ctrl;
   b->tile = tile;
   if (prec == WS_FP64) static_cast<Batch<double>*>(b.get())->create(models, states, B);
   else static_cast<Batch<float>*>(b.get())->create(models, states, B);
@@ -1125,4 +1134,6 @@ void autotune(ws_ctx* ctx, const ws_model* const* models, const ws_state* const*
 
 }  // namespace
 
-struct ws_batch { std::unique_ptr<BatchBase> impl; };
+// a batch handle: calls on one handle are serialised by its mutex (a JS host may issue one from the main thread while
+// another runs on a worker)
+struct ws_batch { std::unique_ptr<BatchBase> impl; std::mutex mu; };

@@ -25,6 +25,8 @@ constexpr int CTRL_ONE = 2048;
 constexpr int CTRL_CHUNK = 1024;
 constexpr int CTRL_STAGE = 64;    // chunk partials the combining workgroup stages in LDS at a time
 constexpr int RPART = 11;         // doubles per control-chunk partial
+constexpr int FUSED_CHUNK = 256;  // strand tiles per chunk when the strand launch runs the control step (four partials
+                                  // per lane of a 64-lane tile: one memory latency for the chunk's reduction)
 // rigid-body sums per strand tile: resultant of loads + ground (3), its moment about the centroid (3), Σ d² of the
 // ground-contact nodes per axis (3), their count (1), Σ x of the new positions (3)
 constexpr int RBP = 13;
@@ -143,6 +145,8 @@ struct Inst {
   double* spart;             // [ntiles][SPART_STRIDE] one partial per strand tile
   double* cpart;             // [CPART][ncblk]
   double* rpart;             // [ctrl_chunks][RPART] control-chunk partials (ctrl_chunks > 1)
+  int32_t* ccount;           // [ctrl_chunks] tiles of each chunk finished in this launch (the control step fused into
+                             // the strand launch; zero between launches)
   double* rbpart;            // [RBP][ntiles] rigid-body sums per strand tile (rigid only)
   const double* sched;       // [n_inc] replay schedule (proto != 0)
   double* pout;              // [4 n_inc] replay outputs: travel, reaction, value, iteration

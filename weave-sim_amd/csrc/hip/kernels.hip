@@ -1,17 +1,21 @@
-// HIP kernels (gfx950) of the dynamic-relaxation iteration.  One iteration is two launches:
+// HIP kernels (gfx950) of the dynamic-relaxation iteration.  One iteration is three launches (two when a batch is
+// built with ctrl_mode 1, the control step then run by the strand launch; a rigid-step launch after them when a batch
+// holds a structure only the ground holds):
 //
 //   k_cross   one or two lanes per crossing: node tangents and mean node frames around both sides' contact points,
 //             C¹ Hermite point / tangent / frame on each side, contact normal with side persistence, penalty normal
 //             force, Coulomb stick / slip with the u advance, W1 classification, W2 / W3 / AL branches; writes the
-//             crossing state and one record per crossing side (force, lever torque, righting stiffness) — no atomics.
-//             Follows PolylineContact.apply (web/js/solver_cpu.js:381-478) and the crossing branch of k_force
-//             (web/js/wgsl.js:571-777).
+//             crossing state and one record per crossing side (force, lever torque, righting stiffness) and a partial
+//             (contact energy, max Fn) per block — no atomics.  Follows PolylineContact.apply
+//             (web/js/solver_cpu.js:381-478) and the crossing branch of k_force (web/js/wgsl.js:571-777).
 //   k_strand  one workgroup per strand tile (a run of consecutive nodes of one lath): positions and frames of the
 //             tile plus a one-node halo staged in LDS; segment stretch / shear forces (rod_cosserat.js:118-133), joint
 //             bending / twist torques with J_l⁻ᵀ and the bending-coupling inertia (:136-159), clamp joints (:161-180);
 //             the crossing-side records of the tile gathered in a fixed order; node and frame integration
-//             (solver_cpu.js:751-808); FP64 partial sums; the last tile of an instance reduces the partials in index
-//             order and runs kinetic damping, the residual and the stop test (solver_cpu.js:809-837).
+//             (solver_cpu.js:751-808); one FP64 partial per tile (with ctrl_mode 1 the last tiles of an instance to
+//             finish reduce the partials and run the control step: strand_control_tail).
+//   k_ctrl    per instance, one or more workgroups: the partials reduced in a fixed order, then kinetic damping,
+//             the residual, the stop test and the history ring (solver_cpu.js:809-837; control_step).
 //
 // The kinetic-damping roll-back needs no copy: positions and frames are double-buffered and `cur` simply does not
 // flip on a roll-back, while the stale velocities are read as zero by the next iteration.
@@ -850,7 +854,11 @@ __device__ __forceinline__ void joint_torque(StrandLDS<Real, TILE>& L, int i, Re
 // at them; lane t works on "slot" t: node k0−1+t, segment k0−1+t and joint k0−1+t.  Slot 0 is the halo (the segment
 // and joint just before the tile, whose forces the first owned node and frame need): every lane computes one segment
 // and one joint, and lanes 1 .. cnt integrate their node and frame.
-template <typename Real, int TILE, bool OUT, bool RB>
+template <typename Real, int TILE, bool RB>
+__device__ void strand_control_tail(const Inst<Real>& I, Ctl* ctl, const TileItem& tl, StrandLDS<Real, TILE>& L);
+
+// FUSED: the control step runs in this launch, in the last tile of the instance to finish (strand_control_tail)
+template <typename Real, int TILE, bool OUT, bool RB, bool FUSED = false>
 __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem& tl, StrandLDS<Real, TILE>& L,
                             const OutPtrs& out) {
   const int tid = threadIdx.x;
@@ -920,7 +928,11 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   const int so = tl.so, ns = tl.ns;
   SideRec<Real> r0;
   if (tid < ns) side_load(gl(I.side) + so + tid, r0);
-  if (!live) return;                               // uniform over the workgroup (one instance per tile)
+  if (!live) {                                     // uniform over the workgroup (one instance per tile)
+    // no iteration, so no rigid step after it (the control kernel clears the flag where it runs)
+    if (FUSED && RB && I.rigid && tl.tile_local == 0 && tid == 0) const_cast<Ctl*>(ctl)->rb_move = 0;
+    return;
+  }
   // the staged positions and frames go to LDS only now, after every load above is out (one memory latency for all)
 #pragma unroll
   for (int r = 0; r < 2; r++) {
@@ -1266,6 +1278,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       gl(I.rbpart)[size_t(tid) * I.ntiles + tl.tile_local] = x;
     }
   }
+  if (FUSED) strand_control_tail<Real, TILE, RB>(I, const_cast<Ctl*>(ctl), tl, L);
 }
 
 // fixed-order block reduction of NV doubles (the first nsum summed, the rest max-reduced over non-negative values);
@@ -1286,6 +1299,224 @@ __device__ __forceinline__ void block_reduce(double (*red)[TILE / 64], double* v
     v[k] = x;
   }
   __syncthreads();
+}
+
+// The control step of one instance after its reductions (lane 0; solver_cpu.js:809-837; wgsl.js:990-1047): acc
+// holds [0] Σm|v|² + ΣI3ω², [1..4] stretch / shear / bending / torsion energy, [5] ground energy, [6] contact energy
+// (sums), [7] max |F_free|², [8] max |t_m|², [9] max(|load|, |N|, |M|/l̄_min, ground Fn), [10] max Fn (maxima); rbs the
+// instance's rigid-body sums.  c is the control record as the iteration found it; the new record is written to ctl.
+template <typename Real>
+__device__ void control_step(const Inst<Real>& I, Ctl* ctl, Ctl& c, const double* acc, const double* rbs) {
+  const double eCont = acc[6], eGround = acc[5];
+  double ke = 0.5 * acc[0];
+  const int iterPre = c.iter;
+  // rigid-body step (CpuSolver.step, solver_cpu.js:784-790; rigidMasses :565-572): velocities from this iteration's
+  // resultant, v ← clip((v + Q/M)·damping), zero while warming up; their kinetic energy counts for the roll-back
+  double rv[3] = {0, 0, 0}, rw[3] = {0, 0, 0};
+  if (I.rigid) {
+    const int rampN = max(I.ramp_iter, 0);
+    const double rmp = rampN <= 0 ? 1.0 : fmin(1.0, 0.01 + 0.99 * iterPre / rampN);
+    const double kn = rmp * I.strength, kt = rmp * I.stick, f = I.mass_scale * RB_MASS;
+    const double nc = fmax(rbs[9], 1.0), rmin = (RB_LEVER * I.rb_R) * (RB_LEVER * I.rb_R) * kn;
+    double Mr[6];
+    Mr[0] = fmax(f * nc * kt, 1e-30); Mr[1] = Mr[0]; Mr[2] = fmax(f * nc * kn, 1e-30);
+    Mr[3] = fmax(f * fmax(kn * rbs[7] + kt * rbs[8], rmin), 1e-30);
+    Mr[4] = fmax(f * fmax(kn * rbs[6] + kt * rbs[8], rmin), 1e-30);
+    Mr[5] = fmax(f * fmax(kt * (rbs[6] + rbs[7]), rmin), 1e-30);
+    if (!(iterPre < I.warmup && I.contacts)) {
+      for (int k = 0; k < 3; k++) {
+        rv[k] = fmin(fmax((c.rv[k] + rbs[k] / Mr[k]) * I.damping, -I.rb_vmax), I.rb_vmax);
+        rw[k] = fmin(fmax((c.rw[k] + rbs[3 + k] / Mr[3 + k]) * I.damping, -I.rb_wmax), I.rb_wmax);
+      }
+    }
+    for (int k = 0; k < 3; k++) ke += 0.5 * (Mr[k] * rv[k] * rv[k] + Mr[3 + k] * rw[k] * rw[k]);
+  }
+  int rollback = 0;
+  if (I.kin_damp && ke < c.ke_prev) {
+    rollback = 1; ke = 0.0;
+    for (int k = 0; k < 3; k++) { rv[k] = 0.0; rw[k] = 0.0; }
+  }
+  c.rollback = rollback;
+  c.ke_prev = ke; c.ke = ke;
+  c.iter = iterPre + 1;
+  if (I.aug && I.contacts) {
+    if (c.al_update) c.pen_n = 0;
+    if (iterPre >= max(I.ramp_iter, 0)) c.pen_n += 1;
+    const int rampN = max(I.ramp_iter, 0);
+    c.al_update = (I.al_every > 0 && c.iter > rampN && (c.iter - rampN) % I.al_every == 0) ? 1 : 0;
+  } else {
+    c.al_update = 0;
+  }
+  double fref = acc[9];
+  if (I.contacts) fref = fmax(fref, acc[10]);
+  fref = fmax(fref, 1e-12);
+  // the strand partials carry max |F_free|² and max |t_m|²: the square root of the maximum is the maximum norm
+  const double rf = I.n ? sqrt(acc[7]) / fref : 0.0, rt = I.m ? sqrt(acc[8]) / (fref * I.l_ref) : 0.0;
+  double resid = fmax(rf, rt);
+  if (I.rigid) {
+    // the rigid part of the residual (rigidResidual, :575), then the rigid record: the next reference point is the
+    // centroid of the new positions, moved by the rigid translation (the rotation turns them about the old one)
+    const double rrb = fmax(sqrt(rbs[0] * rbs[0] + rbs[1] * rbs[1] + rbs[2] * rbs[2]) / fref,
+                            sqrt(rbs[3] * rbs[3] + rbs[4] * rbs[4] + rbs[5] * rbs[5]) / (fref * I.rb_R));
+    resid = fmax(resid, rrb);
+    const bool move = rv[0] != 0 || rv[1] != 0 || rv[2] != 0 || rw[0] != 0 || rw[1] != 0 || rw[2] != 0;
+    c.rb_move = move ? 1 : 0;
+    for (int k = 0; k < 3; k++) {
+      c.rc_prev[k] = c.rc[k];
+      if (!rollback) c.rc[k] = rbs[10 + k] / I.n + (move ? rv[k] : 0.0);
+      c.rv[k] = rv[k]; c.rw[k] = rw[k];
+    }
+    c.res_rb = rrb;
+  }
+  if (!(c.iter > I.warmup)) resid = INFINITY;
+  c.resid = resid; c.fref = fref; c.res_f = rf; c.res_t = rt;
+  c.e_stretch = acc[1]; c.e_shear = acc[2]; c.e_bend = acc[3]; c.e_tors = acc[4]; c.e_cont = eCont;
+  const bool conv = resid <= I.tol;
+  if (I.proto == 0 && conv) c.done = 1;
+  if (!rollback) c.cur = c.cur ^ 1;
+  if (I.proto != 0) {
+    // UTM replay (the protocol branch of the reference's control kernel, wgsl.js:1048-1093; calibrate.js:154-215):
+    // settle at most `settle` iterations (ending at the residual), then average the probe's travel from P0 and its
+    // reaction along the axis over `avg` iterations — the state after each iteration, i.e. before it when rolled
+    // back — record them, and go on with the next schedule value
+    c.it_phase += 1;
+    if (c.phase == 0) {
+      if (conv || c.it_phase >= I.settle) {
+        c.phase = 1; c.it_phase = 0; c.acc_n = 0;
+        c.ad[0] = c.ad[1] = c.ad[2] = 0.0; c.ar = 0.0;
+      }
+    } else if (c.phase == 1) {
+      const int pr = I.probe;
+      c.ad[0] += gl(I.X[c.cur])[3 * pr] - I.p0x; c.ad[1] += gl(I.X[c.cur])[3 * pr + 1] - I.p0y; c.ad[2] += gl(I.X[c.cur])[3 * pr + 2] - I.p0z;
+      if (I.proto == 2) c.ar += gl(I.preact)[0] * I.pax + gl(I.preact)[1] * I.pay + gl(I.preact)[2] * I.paz;
+      c.acc_n += 1;
+      if (c.it_phase >= I.avg) {
+        const double nn = double(c.acc_n);
+        const auto o = gl(I.pout) + size_t(c.inc) * 4;
+        o[0] = (c.ad[0] * I.pax + c.ad[1] * I.pay + c.ad[2] * I.paz) / nn;
+        o[1] = c.ar / nn; o[2] = c.cur_f; o[3] = double(c.iter);
+        c.inc += 1; c.it_phase = 0; c.phase = 0;
+        if (c.inc >= I.n_inc) { c.phase = 2; c.done = 1; }
+        else {
+          const double v = gl(I.sched)[c.inc];
+          c.cur_f = v;
+          if (I.proto == 1) { c.pf[0] = I.pax * v; c.pf[1] = I.pay * v; c.pf[2] = I.paz * v; }
+          else {
+            // displacement control: the probe node at P0 + v·axis, at rest, in the state the next iteration reads
+            gl(I.X[c.cur])[3 * pr] = I.p0x + I.pax * v; gl(I.X[c.cur])[3 * pr + 1] = I.p0y + I.pay * v; gl(I.X[c.cur])[3 * pr + 2] = I.p0z + I.paz * v;
+            gl(I.vx)[pr] = Real(0); gl(I.vy)[pr] = Real(0); gl(I.vz)[pr] = Real(0);
+          }
+        }
+      }
+    }
+  }
+  c.live_flag = 1;   // an iteration has been evaluated
+  if (I.hist_len > 0) {
+    const auto h = gl(I.hist) + size_t((c.iter - 1) % I.hist_len) * HIST_STRIDE;
+    h[0] = ke; h[1] = acc[1] + acc[2] + acc[3] + acc[4] + eCont + eGround; h[2] = acc[3]; h[3] = acc[4];
+    h[4] = acc[1]; h[5] = acc[2]; h[6] = eCont; h[7] = resid; h[8] = double(c.iter);
+  }
+  c.hist_n += 1;
+  *ctl = c;
+}
+
+// The control step fused into the strand launch.  Every tile counts itself into its chunk (ctrl_chunk consecutive
+// tiles of the instance); the last tile of a chunk to finish reduces the chunk's tile partials in tile order, and the
+// chunk's share of the crossing-block partials, into a chunk partial; with several chunks the last chunk to finish
+// combines the chunk partials in chunk order.  That workgroup then runs the control step.  Every order is fixed: the
+// result does not depend on which workgroup arrives last.  Counters are reset by the workgroup that read them last,
+// so the next iteration starts from zero.  Partials of this launch are read with device-scope loads after an
+// acquire fence (the writers release theirs with a fence before counting in).
+template <typename Real, int TILE, bool RB>
+__device__ void strand_control_tail(const Inst<Real>& I, Ctl* ctl, const TileItem& tl, StrandLDS<Real, TILE>& L) {
+  constexpr int NACC = RPART, NSUM = 7, FPER = 4;
+  constexpr int FSTAGE = (SPART * TILE) / RPART;   // chunk partials staged at a time in the tile's dead partial space
+  __shared__ double red[NACC][TILE / 64];
+  __shared__ double red_rb[RBP][TILE / 64];
+  __shared__ int last;
+  const int tid = threadIdx.x;
+  const int np = I.ntiles, per = I.ctrl_chunk, nch = I.ctrl_chunks;
+  const int chunk = tl.tile_local / per;
+  const int p0 = chunk * per, p1 = min(np, p0 + per);
+  __threadfence();                                 // release this tile's partial
+  __syncthreads();
+  if (tid == 0) last = atomicAdd(I.ccount + chunk, 1) == p1 - p0 - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();                                 // acquire the chunk's partials
+  auto ld = [](const double* q) { return __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
+  const bool fon = I.floor_on != 0;
+  double acc[NACC];
+#pragma unroll
+  for (int q = 0; q < NACC; q++) acc[q] = 0.0;
+  for (int w0 = p0; w0 < p1; w0 += TILE * FPER) {
+    double x[FPER][SPART];
+#pragma unroll
+    for (int k = 0; k < FPER; k++) {
+      const int w = w0 + k * TILE + tid;
+      const bool in = w < p1;
+      const double* sp = I.spart + size_t(in ? w : p0) * SPART_STRIDE;
+#pragma unroll
+      for (int q = 0; q < SPART - 1; q++) x[k][q] = in ? ld(sp + q) : 0.0;
+      x[k][SPART - 1] = (in && fon) ? ld(sp + SPART - 1) : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < FPER; k++) {
+#pragma unroll
+      for (int q = 0; q < 5; q++) acc[q] += x[k][q];
+#pragma unroll
+      for (int q = 5; q < 8; q++) acc[q + 2] = fmax(acc[q + 2], x[k][q]);
+      acc[5] += x[k][8];
+    }
+  }
+  // the chunk's share of the crossing-block partials (written by the crossing launch before this one)
+  const int cper = (I.ncblk + nch - 1) / nch;
+  const int c0 = chunk * cper, c1 = min(I.ncblk, c0 + cper);
+  for (int w = c0 + tid; w < c1; w += TILE) {
+    acc[6] += gl(I.cpart)[w];
+    acc[10] = fmax(acc[10], gl(I.cpart)[I.ncblk + w]);
+  }
+  block_reduce<TILE, NACC>(red, acc, NSUM);
+  if (tid == 0) gl(I.ccount)[chunk] = 0;
+  if (nch > 1) {
+    if (tid == 0) {
+      for (int q = 0; q < NACC; q++) gl(I.rpart)[size_t(chunk) * RPART + q] = acc[q];
+      __threadfence();                             // release the chunk partial before counting in
+      last = atomicAdd(&ctl->counter, 1) == nch - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();                               // acquire every chunk partial
+    double* stage = &L.part[0][0];
+    for (int q = 0; q < NACC; q++) acc[q] = 0.0;
+    for (int k0 = 0; k0 < nch; k0 += FSTAGE) {
+      const int nk = min(FSTAGE, nch - k0);
+      __syncthreads();
+      for (int v = tid; v < nk * RPART; v += TILE) stage[v] = ld(I.rpart + size_t(k0) * RPART + v);
+      __syncthreads();
+      if (tid == 0) {
+        for (int k = 0; k < nk; k++) {
+          for (int q = 0; q < NSUM; q++) acc[q] += stage[k * RPART + q];
+          for (int q = NSUM; q < NACC; q++) acc[q] = fmax(acc[q], stage[k * RPART + q]);
+        }
+      }
+    }
+  }
+  // rigid-body sums of the instance's tiles (a structure only the ground holds), in tile order per lane, then the
+  // fixed block order
+  double rbs[RBP];
+#pragma unroll
+  for (int q = 0; q < RBP; q++) rbs[q] = 0.0;
+  if (RB && I.rigid) {
+    for (int w = tid; w < np; w += TILE)
+#pragma unroll
+      for (int q = 0; q < RBP; q++) rbs[q] += ld(I.rbpart + size_t(q) * np + w);
+    block_reduce<TILE, RBP>(red_rb, rbs, RBP);
+  }
+  if (tid != 0) return;
+  Ctl c = *ctl;
+  c.counter = 0;
+  control_step(I, ctl, c, acc, rbs);
 }
 
 // ================================================================================================= k_ctrl
@@ -1408,117 +1639,7 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
     block_reduce<CTRL_THREADS, RBP>(red_rb, rbs, RBP);
   }
   if (tid != 0) return;
-  const double eCont = acc[6], eGround = acc[5];
-  double ke = 0.5 * acc[0];
-  const int iterPre = c.iter;
-  // rigid-body step (CpuSolver.step, solver_cpu.js:784-790; rigidMasses :565-572): velocities from this iteration's
-  // resultant, v ← clip((v + Q/M)·damping), zero while warming up; their kinetic energy counts for the roll-back
-  double rv[3] = {0, 0, 0}, rw[3] = {0, 0, 0};
-  if (I.rigid) {
-    const int rampN = max(I.ramp_iter, 0);
-    const double rmp = rampN <= 0 ? 1.0 : fmin(1.0, 0.01 + 0.99 * iterPre / rampN);
-    const double kn = rmp * I.strength, kt = rmp * I.stick, f = I.mass_scale * RB_MASS;
-    const double nc = fmax(rbs[9], 1.0), rmin = (RB_LEVER * I.rb_R) * (RB_LEVER * I.rb_R) * kn;
-    double Mr[6];
-    Mr[0] = fmax(f * nc * kt, 1e-30); Mr[1] = Mr[0]; Mr[2] = fmax(f * nc * kn, 1e-30);
-    Mr[3] = fmax(f * fmax(kn * rbs[7] + kt * rbs[8], rmin), 1e-30);
-    Mr[4] = fmax(f * fmax(kn * rbs[6] + kt * rbs[8], rmin), 1e-30);
-    Mr[5] = fmax(f * fmax(kt * (rbs[6] + rbs[7]), rmin), 1e-30);
-    if (!(iterPre < I.warmup && I.contacts)) {
-      for (int k = 0; k < 3; k++) {
-        rv[k] = fmin(fmax((c.rv[k] + rbs[k] / Mr[k]) * I.damping, -I.rb_vmax), I.rb_vmax);
-        rw[k] = fmin(fmax((c.rw[k] + rbs[3 + k] / Mr[3 + k]) * I.damping, -I.rb_wmax), I.rb_wmax);
-      }
-    }
-    for (int k = 0; k < 3; k++) ke += 0.5 * (Mr[k] * rv[k] * rv[k] + Mr[3 + k] * rw[k] * rw[k]);
-  }
-  int rollback = 0;
-  if (I.kin_damp && ke < c.ke_prev) {
-    rollback = 1; ke = 0.0;
-    for (int k = 0; k < 3; k++) { rv[k] = 0.0; rw[k] = 0.0; }
-  }
-  c.rollback = rollback;
-  c.ke_prev = ke; c.ke = ke;
-  c.iter = iterPre + 1;
-  if (I.aug && I.contacts) {
-    if (c.al_update) c.pen_n = 0;
-    if (iterPre >= max(I.ramp_iter, 0)) c.pen_n += 1;
-    const int rampN = max(I.ramp_iter, 0);
-    c.al_update = (I.al_every > 0 && c.iter > rampN && (c.iter - rampN) % I.al_every == 0) ? 1 : 0;
-  } else {
-    c.al_update = 0;
-  }
-  double fref = acc[9];
-  if (I.contacts) fref = fmax(fref, acc[10]);
-  fref = fmax(fref, 1e-12);
-  // the strand partials carry max |F_free|² and max |t_m|²: the square root of the maximum is the maximum norm
-  const double rf = I.n ? sqrt(acc[7]) / fref : 0.0, rt = I.m ? sqrt(acc[8]) / (fref * I.l_ref) : 0.0;
-  double resid = fmax(rf, rt);
-  if (I.rigid) {
-    // the rigid part of the residual (rigidResidual, :575), then the rigid record: the next reference point is the
-    // centroid of the new positions, moved by the rigid translation (the rotation turns them about the old one)
-    const double rrb = fmax(sqrt(rbs[0] * rbs[0] + rbs[1] * rbs[1] + rbs[2] * rbs[2]) / fref,
-                            sqrt(rbs[3] * rbs[3] + rbs[4] * rbs[4] + rbs[5] * rbs[5]) / (fref * I.rb_R));
-    resid = fmax(resid, rrb);
-    const bool move = rv[0] != 0 || rv[1] != 0 || rv[2] != 0 || rw[0] != 0 || rw[1] != 0 || rw[2] != 0;
-    c.rb_move = move ? 1 : 0;
-    for (int k = 0; k < 3; k++) {
-      c.rc_prev[k] = c.rc[k];
-      if (!rollback) c.rc[k] = rbs[10 + k] / I.n + (move ? rv[k] : 0.0);
-      c.rv[k] = rv[k]; c.rw[k] = rw[k];
-    }
-    c.res_rb = rrb;
-  }
-  if (!(c.iter > I.warmup)) resid = INFINITY;
-  c.resid = resid; c.fref = fref; c.res_f = rf; c.res_t = rt;
-  c.e_stretch = acc[1]; c.e_shear = acc[2]; c.e_bend = acc[3]; c.e_tors = acc[4]; c.e_cont = eCont;
-  const bool conv = resid <= I.tol;
-  if (I.proto == 0 && conv) c.done = 1;
-  if (!rollback) c.cur = c.cur ^ 1;
-  if (I.proto != 0) {
-    // UTM replay (the protocol branch of the reference's control kernel, wgsl.js:1048-1093; calibrate.js:154-215):
-    // settle at most `settle` iterations (ending at the residual), then average the probe's travel from P0 and its
-    // reaction along the axis over `avg` iterations — the state after each iteration, i.e. before it when rolled
-    // back — record them, and go on with the next schedule value
-    c.it_phase += 1;
-    if (c.phase == 0) {
-      if (conv || c.it_phase >= I.settle) {
-        c.phase = 1; c.it_phase = 0; c.acc_n = 0;
-        c.ad[0] = c.ad[1] = c.ad[2] = 0.0; c.ar = 0.0;
-      }
-    } else if (c.phase == 1) {
-      const int pr = I.probe;
-      c.ad[0] += gl(I.X[c.cur])[3 * pr] - I.p0x; c.ad[1] += gl(I.X[c.cur])[3 * pr + 1] - I.p0y; c.ad[2] += gl(I.X[c.cur])[3 * pr + 2] - I.p0z;
-      if (I.proto == 2) c.ar += gl(I.preact)[0] * I.pax + gl(I.preact)[1] * I.pay + gl(I.preact)[2] * I.paz;
-      c.acc_n += 1;
-      if (c.it_phase >= I.avg) {
-        const double nn = double(c.acc_n);
-        const auto o = gl(I.pout) + size_t(c.inc) * 4;
-        o[0] = (c.ad[0] * I.pax + c.ad[1] * I.pay + c.ad[2] * I.paz) / nn;
-        o[1] = c.ar / nn; o[2] = c.cur_f; o[3] = double(c.iter);
-        c.inc += 1; c.it_phase = 0; c.phase = 0;
-        if (c.inc >= I.n_inc) { c.phase = 2; c.done = 1; }
-        else {
-          const double v = gl(I.sched)[c.inc];
-          c.cur_f = v;
-          if (I.proto == 1) { c.pf[0] = I.pax * v; c.pf[1] = I.pay * v; c.pf[2] = I.paz * v; }
-          else {
-            // displacement control: the probe node at P0 + v·axis, at rest, in the state the next iteration reads
-            gl(I.X[c.cur])[3 * pr] = I.p0x + I.pax * v; gl(I.X[c.cur])[3 * pr + 1] = I.p0y + I.pay * v; gl(I.X[c.cur])[3 * pr + 2] = I.p0z + I.paz * v;
-            gl(I.vx)[pr] = Real(0); gl(I.vy)[pr] = Real(0); gl(I.vz)[pr] = Real(0);
-          }
-        }
-      }
-    }
-  }
-  c.live_flag = 1;   // an iteration has been evaluated
-  if (I.hist_len > 0) {
-    const auto h = gl(I.hist) + size_t((c.iter - 1) % I.hist_len) * HIST_STRIDE;
-    h[0] = ke; h[1] = acc[1] + acc[2] + acc[3] + acc[4] + eCont + eGround; h[2] = acc[3]; h[3] = acc[4];
-    h[4] = acc[1]; h[5] = acc[2]; h[6] = eCont; h[7] = resid; h[8] = double(c.iter);
-  }
-  c.hist_n += 1;
-  *ctl = c;
+  control_step(I, ctl, c, acc, rbs);
 }
 
 // XCD-aware block mapping: consecutive logical tiles (neighbouring tiles of one lath share halo nodes) run on the
@@ -1537,9 +1658,11 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
 // ONE: the batch holds a single instance (slot 0): its descriptor and control record load beside the tile item.
 // WPE > 1 asks for that many resident wavefronts per SIMD (the FP64 form of the 64-lane tile: 168 registers and a
 // 36-byte spill for three wavefronts instead of two; faster on batches of thousands of tiles, slower on small ones)
-template <typename Real, int TILE, bool RB, bool ONE, int WPE>
+// FUSED: the last tiles of an instance to finish run its control step (strand_control_tail) in this launch, and no
+// control launch follows
+template <typename Real, int TILE, bool RB, bool ONE, int WPE, bool FUSED>
 __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(WPE))) void k_strand(const Inst<Real>* __restrict__ insts,
-                                                 const TileItem* __restrict__ tiles, int32_t ntiles, const Ctl* __restrict__ ctls) {
+                                                 const TileItem* __restrict__ tiles, int32_t ntiles, Ctl* __restrict__ ctls) {
   __shared__ StrandLDS<Real, TILE> L;
   int lt = xcd_tile(blockIdx.x, gridDim.x);
   if (lt >= ntiles) return;
@@ -1548,7 +1671,7 @@ __global__ __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(WPE))) voi
   // latencies, not one per test
   const TileItem tl = tiles[lt];
   const int b = ONE ? 0 : tl.inst;
-  strand_tile<Real, TILE, false, RB>(insts[b], ctls + b, tl, L, OutPtrs{});
+  strand_tile<Real, TILE, false, RB, FUSED>(insts[b], ctls + b, tl, L, OutPtrs{});
 }
 
 // The rigid step of a structure only the ground holds (CpuSolver.step, solver_cpu.js:791-805), after the control
@@ -1652,11 +1775,12 @@ __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uin
 }
 
 // explicit instantiations
-#define WSD_STRAND(R, T, W) \
-  template __global__ void k_strand<R, T, false, false, W>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
-  template __global__ void k_strand<R, T, true, false, W>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
-  template __global__ void k_strand<R, T, false, true, W>(const Inst<R>*, const TileItem*, int32_t, const Ctl*); \
-  template __global__ void k_strand<R, T, true, true, W>(const Inst<R>*, const TileItem*, int32_t, const Ctl*);
+#define WSD_STRAND_F(R, T, W, F) \
+  template __global__ void k_strand<R, T, false, false, W, F>(const Inst<R>*, const TileItem*, int32_t, Ctl*); \
+  template __global__ void k_strand<R, T, true, false, W, F>(const Inst<R>*, const TileItem*, int32_t, Ctl*); \
+  template __global__ void k_strand<R, T, false, true, W, F>(const Inst<R>*, const TileItem*, int32_t, Ctl*); \
+  template __global__ void k_strand<R, T, true, true, W, F>(const Inst<R>*, const TileItem*, int32_t, Ctl*);
+#define WSD_STRAND(R, T, W) WSD_STRAND_F(R, T, W, false) WSD_STRAND_F(R, T, W, true)
 WSD_STRAND(double, 64, 3)
 #define WSD_INST(R, T) \
   WSD_STRAND(R, T, 1) \

@@ -33,6 +33,7 @@ struct ws_server {
     double t_submit = 0, t_admit = 0;
   };
   ws_ctx* ctx = nullptr;
+  std::mutex mu;                  // serialises the calls on this handle (submit / poll from one thread during a pump)
   ws_server_opts opts{};
   std::unique_ptr<BatchBase> batch;
   std::deque<Req> queue;
@@ -139,12 +140,15 @@ WS_API int ws_server_create(ws_ctx* ctx, const ws_server_opts* opts, ws_server**
 }
 
 WS_API int ws_server_destroy(ws_server* srv) {
+  if (!srv) return WS_OK;
+  { std::lock_guard<std::mutex> lk(srv->mu); }   // a call running on another thread finishes first
   delete srv;
   return WS_OK;
 }
 
 WS_API int ws_submit(ws_server* srv, const ws_model* model, ws_state* state, int32_t max_iter, int64_t* id_out) {
   if (!srv || !model || !state) return WS_ERR_ARG;
+  std::lock_guard<std::mutex> lk(srv->mu);
   try {
     ws_server::Req r;
     r.md = model; r.st = state;
@@ -163,6 +167,7 @@ WS_API int ws_submit(ws_server* srv, const ws_model* model, ws_state* state, int
 
 WS_API int ws_cancel(ws_server* srv, int64_t id) {
   if (!srv) return WS_ERR_ARG;
+  std::lock_guard<std::mutex> lk(srv->mu);
   try {
     for (auto it = srv->queue.begin(); it != srv->queue.end(); ++it) {
       if (it->id != id) continue;
@@ -186,6 +191,7 @@ WS_API int ws_cancel(ws_server* srv, int64_t id) {
 
 WS_API int ws_pump(ws_server* srv, int32_t* n_active, int32_t* n_queued) {
   if (!srv) return WS_ERR_ARG;
+  std::lock_guard<std::mutex> lk(srv->mu);
   try {
     HIPCHK(hipSetDevice(srv->ctx->device));
     srv->pump(n_active, n_queued);
@@ -195,6 +201,7 @@ WS_API int ws_pump(ws_server* srv, int32_t* n_active, int32_t* n_queued) {
 
 WS_API int ws_poll(ws_server* srv, ws_result* out, int32_t cap, int32_t* n_out) {
   if (!srv || (cap > 0 && !out)) return WS_ERR_ARG;
+  std::lock_guard<std::mutex> lk(srv->mu);
   int n = std::min<int>(cap, int(srv->finished.size()));
   for (int q = 0; q < n; q++) out[q] = srv->finished[q];
   srv->finished.erase(srv->finished.begin(), srv->finished.begin() + n);
@@ -204,6 +211,7 @@ WS_API int ws_poll(ws_server* srv, ws_result* out, int32_t cap, int32_t* n_out) 
 
 WS_API int ws_fetch(ws_server* srv, int64_t id, ws_state* out) {
   if (!srv || !out) return WS_ERR_ARG;
+  std::lock_guard<std::mutex> lk(srv->mu);
   try {
     for (int q = 0; q < int(srv->slot.size()); q++)
       if (srv->slot[q].id == id) { srv->batch->download(q, *out); return WS_OK; }
@@ -215,6 +223,7 @@ WS_API double ws_server_clock_ms(ws_server* srv) { return srv ? now_ms(srv->t0) 
 
 WS_API int ws_server_stats(ws_server* srv, int32_t* n_active, int32_t* n_queued, int64_t* bytes_in_use, int64_t* hbm_budget) {
   if (!srv) return WS_ERR_ARG;
+  std::lock_guard<std::mutex> lk(srv->mu);
   if (n_active) *n_active = srv->active();
   if (n_queued) *n_queued = int32_t(srv->queue.size());
   if (bytes_in_use) *bytes_in_use = int64_t(srv->bytes_in_use);

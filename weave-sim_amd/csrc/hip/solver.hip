@@ -118,7 +118,9 @@ WS_API int ws_batch_create(ws_ctx* ctx, const ws_model* const* models, const ws_
       t = best.tile; if (ps <= 0) ps = best.per_submit;
       if (lanes == 0) lanes = best.cross_lanes;
     }
-    *out = new ws_batch{make_batch(ctx, models, states, B, prec, t, ps, hist, opts ? opts->ctrl_chunk : 0, lanes)};
+    const int cm = opts ? opts->ctrl_mode : 0;
+    if (cm < 0 || cm > 1) throw Fail{WS_ERR_ARG, "ctrl_mode must be 0 (a control launch) or 1 (in the strand launch)"};
+    *out = new ws_batch{make_batch(ctx, models, states, B, prec, t, ps, hist, opts ? opts->ctrl_chunk : 0, lanes, cm == 1)};
     return WS_OK;
   } catch (const Fail& f) { return fail(ctx, f); }
 }
@@ -142,18 +144,26 @@ WS_API int ws_autotune(ws_ctx* ctx, const ws_model* const* models, const ws_stat
 
 WS_API int ws_batch_get_opts(ws_batch* batch, ws_batch_opts* out) {
   if (!batch || !out) return WS_ERR_ARG;
+  std::lock_guard<std::mutex> lk(batch->mu);
   out->precision = batch->impl->precision; out->per_submit = batch->impl->per_submit;
   out->tile = batch->impl->tile; out->history = batch->impl->hist_len; out->ctrl_chunk = batch->impl->ctrl_chunk;
   out->cross_lanes = batch->impl->lanes;
+  out->ctrl_mode = batch->impl->fused_ctrl ? 1 : 0;
   return WS_OK;
 }
 
-WS_API int ws_batch_destroy(ws_batch* batch) { delete batch; return WS_OK; }
+WS_API int ws_batch_destroy(ws_batch* batch) {
+  if (!batch) return WS_OK;
+  { std::lock_guard<std::mutex> lk(batch->mu); }   // a call running on another thread finishes first
+  delete batch;
+  return WS_OK;
+}
 WS_API int ws_batch_size(ws_batch* batch, int32_t* B) { if (!batch || !B) return WS_ERR_ARG; *B = batch->impl->B; return WS_OK; }
 
 // every call makes the batch's device current on the calling thread (the N-API addon calls from worker threads)
 #define BATCH_CALL(expr) \
   if (!batch) return WS_ERR_ARG; \
+  std::lock_guard<std::mutex> lk_(batch->mu); \
   try { HIPCHK(hipSetDevice(batch->impl->ctx->device)); return (expr); } catch (const Fail& f) { return fail(batch->impl->ctx, f); }
 
 WS_API int ws_upload(ws_batch* batch, int32_t b, const ws_model* model, const ws_state* state) {
@@ -202,6 +212,7 @@ WS_API int ws_internalize(ws_batch* batch, int32_t b) {
 }
 WS_API int ws_batch_timing(ws_batch* batch, double* ms_last, double* ms_cross, double* ms_strand, double* ms_ctrl) {
   if (!batch) return WS_ERR_ARG;
+  std::lock_guard<std::mutex> lk(batch->mu);
   if (ms_ctrl) *ms_ctrl = batch->impl->ms_ctrl;
   if (ms_last) *ms_last = batch->impl->ms_last;
   if (ms_cross) *ms_cross = batch->impl->ms_cross;
@@ -210,11 +221,13 @@ WS_API int ws_batch_timing(ws_batch* batch, double* ms_last, double* ms_cross, d
 }
 WS_API int ws_batch_profile(ws_batch* batch, int32_t on) {
   if (!batch) return WS_ERR_ARG;
+  std::lock_guard<std::mutex> lk(batch->mu);
   batch->impl->profile = on != 0;
   return WS_OK;
 }
 WS_API int ws_batch_kernel_bench(ws_batch* batch, int32_t reps, double* ms_cross, double* ms_strand, double* ms_ctrl) {
   if (!batch || reps <= 0) return WS_ERR_ARG;
+  std::lock_guard<std::mutex> lk(batch->mu);
   try {
     double out[3] = {0, 0, 0};
     batch->impl->kernel_bench(reps, out);
@@ -226,6 +239,7 @@ WS_API int ws_batch_kernel_bench(ws_batch* batch, int32_t reps, double* ms_cross
 }
 WS_API int ws_batch_bytes(ws_batch* batch, double* alg, double* dev) {
   if (!batch) return WS_ERR_ARG;
+  std::lock_guard<std::mutex> lk(batch->mu);
   if (alg) *alg = batch->impl->alg_bytes();
   if (dev) *dev = batch->impl->dev_bytes();
   return WS_OK;

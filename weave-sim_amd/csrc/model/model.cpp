@@ -12,6 +12,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <set>
 #include <memory>
 #include <numeric>
 #include <stdexcept>
@@ -489,6 +490,7 @@ struct Model {
   int n = 0, m = 0, p = 0, K = 0, C = 0;
   std::vector<int32_t> nodeOffset, segOffset, jointOffset, segI0, segI1, jSeg0, jSeg1, jNode, cSeg, cNode;
   std::vector<int32_t> kStrand, kSegOff, kNseg, plNode;
+  std::vector<int32_t> kKnot;            // per crossing: mesh edge id, strand_a, k_a, strand_b, k_b, over strand [6K]
   std::vector<uint8_t> strandClosed, nodeFree, kEnabled, kLocked;
   std::vector<double> l0, jLen, EA, GA, EIx, EIy, GJ, omega0, cLen, cB, cQ, mass, krot, loadConst, plF;
   std::vector<double> kwk, khk, kfc90, kE90, kHs, kw, kh, nodeW, nodeH;
@@ -662,7 +664,7 @@ struct Built {
   std::vector<std::vector<double>> polylines, nodeFrames;
   std::vector<uint8_t> closed;
   std::vector<int> nKnots;                                // frames per strand
-  struct KnotRec { int as, ak, bs, bk; bool isBoundary; V3 edgeNormal; };
+  struct KnotRec { int as, ak, bs, bk; bool isBoundary; V3 edgeNormal; int edge = 0, overSide = 0; };
   std::vector<KnotRec> knots;                             // sorted by knot id (edge id)
   std::map<int64_t, int> knotNode;                        // fkey → row
   double overhang = 0;
@@ -859,6 +861,12 @@ std::unique_ptr<Model> buildModel(const Built& tp, const Setup& su) {
   md->kfc90.assign(2 * K, 0); md->kE90.assign(2 * K, 0); md->kHs.assign(2 * K, 0);
   md->kw.assign(K, 0); md->kh.assign(K, 0);
   md->kEnabled.assign(K, 1); md->kLocked.assign(K, sc.locked ? 1 : 0);
+  md->kKnot.assign(6 * size_t(K), 0);
+  for (int c = 0; c < K; c++) {
+    const auto& kr = tp.knots[c];
+    const int32_t row[6] = {kr.edge, kr.as, kr.ak, kr.bs, kr.bk, kr.overSide ? kr.bs : kr.as};
+    std::copy(row, row + 6, md->kKnot.begin() + 6 * size_t(c));
+  }
   for (int c = 0; c < K; c++) {
     md->kStrand[2 * c] = tp.knots[c].as; md->kStrand[2 * c + 1] = tp.knots[c].bs;
     for (int side = 0; side < 2; side++) {
@@ -944,7 +952,7 @@ Built fromTopology(const Topology& topo) {
   for (auto& kv : topo.knots) {  // sorted edge order (model.js:50)
     const Knot& kn = kv.second;
     V3 en{topo.mesh->edgeN[3 * kn.edge], topo.mesh->edgeN[3 * kn.edge + 1], topo.mesh->edgeN[3 * kn.edge + 2]};
-    b.knots.push_back({kn.as, kn.ak, kn.bs, kn.bk, kn.isBoundary, en});
+    b.knots.push_back({kn.as, kn.ak, kn.bs, kn.bk, kn.isBoundary, en, kn.edge, kn.overSide});
   }
   return b;
 }
@@ -1081,7 +1089,8 @@ WS_API int wsm_from_polylines(const double* P, const int32_t* counts, int32_t n_
       roff += nk; off += n;
     }
     std::vector<std::pair<int, Built::KnotRec>> ks;
-    for (int c = 0; c < n_knots; c++) b.knots.push_back({knots[4 * c], knots[4 * c + 1], knots[4 * c + 2], knots[4 * c + 3], false, V3{0, 0, 1}});
+    // explicit crossings have no mesh edge: each is keyed by its index
+    for (int c = 0; c < n_knots; c++) b.knots.push_back({knots[4 * c], knots[4 * c + 1], knots[4 * c + 2], knots[4 * c + 3], false, V3{0, 0, 1}, c, 0});
     Setup su; su.sc = *sc;
     su.anchors.assign(anchors, anchors + n_anchors); su.loads.assign(loads, loads + n_loads);
     auto md = buildModel(b, su);
@@ -1325,6 +1334,39 @@ WS_API int wsm_mesh_counts(wsm_model* mdl, int32_t* nV, int32_t* nF) {
   if (!mdl) return WS_ERR_ARG;
   if (nV) *nV = mdl->md->info.nV;
   if (nF) *nF = mdl->md->info.nF;
+  return WS_OK;
+}
+
+WS_API int wsm_knots(wsm_model* mdl, int32_t* out, int32_t cap_rows, int32_t* K) {
+  if (!mdl) return WS_ERR_ARG;
+  const Model& md = *mdl->md;
+  if (K) *K = md.K;
+  if (out) std::copy(md.kKnot.begin(), md.kKnot.begin() + 6 * size_t(std::min(std::max(cap_rows, 0), md.K)), out);
+  return WS_OK;
+}
+
+WS_API int wsm_frames(wsm_model* mdl, int32_t* out, int32_t cap_rows, int32_t* n_frames) {
+  if (!mdl) return WS_ERR_ARG;
+  const Model& md = *mdl->md;
+  if (n_frames) *n_frames = int32_t(md.nodeFrameIds.size());
+  if (!out) return WS_OK;
+  int32_t r = 0;
+  for (const auto& kv : md.nodeFrameIds) {
+    if (r >= cap_rows) break;
+    out[3 * r] = int32_t(kv.first / 1048576); out[3 * r + 1] = int32_t(kv.first % 1048576); out[3 * r + 2] = kv.second;
+    r++;
+  }
+  return WS_OK;
+}
+
+// CpuSolver.lockContacts / unlockContacts (solver_cpu.js:869-871, PolylineContact.lock / unlock :141-142): the
+// crossings on the listed mesh edges (all of them when edges is null); edges without a crossing are ignored
+WS_API int wsm_lock_contacts(wsm_model* mdl, const int32_t* edges, int32_t n_edges, int32_t lock) {
+  if (!mdl || n_edges < 0 || (n_edges > 0 && !edges)) return WS_ERR_ARG;
+  Model& md = *mdl->md;
+  std::set<int32_t> want(edges ? edges : nullptr, edges ? edges + n_edges : nullptr);
+  for (int c = 0; c < md.K; c++)
+    if (!edges || want.count(md.kKnot[6 * size_t(c)])) md.kLocked[c] = lock ? 1 : 0;
   return WS_OK;
 }
 

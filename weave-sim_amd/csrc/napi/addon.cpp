@@ -6,6 +6,7 @@
 #include <node_api.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <functional>
@@ -29,12 +30,20 @@ namespace {
   } while (0)
 
 struct Model {
-  wsm_model* h = nullptr;
+  wsm_model* h = nullptr;      // the C++ builder's model (null: a model over the caller's typed arrays)
   ws_model md{};
   ws_state st{};
   wsm_info info{};
   std::vector<double> sched;   // the replay schedule md.proto_sched points at
+  std::vector<napi_ref> refs;  // the caller's typed arrays md / st point into (modelFromFields), kept alive
 };
+
+void model_finalize(napi_env env, void* p, void*) {
+  Model* mm = static_cast<Model*>(p);
+  for (napi_ref r : mm->refs) napi_delete_reference(env, r);
+  if (mm->h) wsm_free(mm->h);
+  delete mm;
+}
 struct Batch { ws_ctx* ctx = nullptr; ws_batch* b = nullptr; int32_t B = 0; };
 struct Server { ws_ctx* ctx = nullptr; ws_server* s = nullptr; };
 
@@ -95,6 +104,7 @@ struct Job {
   std::function<int()> run;                               // worker thread: C ABI calls only
   std::function<napi_value(napi_env)> done;               // main thread: builds the resolved value
   std::function<std::string()> error;                     // main thread: message of a failed run
+  std::vector<napi_ref> keep;                             // the handles (batch, server, models) the run uses
   int rc = 0;
 };
 
@@ -111,9 +121,16 @@ void job_complete(napi_env env, napi_status, void* data) {
     napi_reject_deferred(env, j->deferred, err);
   }
   napi_delete_async_work(env, j->work);
+  for (napi_ref r : j->keep) napi_delete_reference(env, r);
   delete j;
 }
-napi_value queue_job(napi_env env, Job* j, const char* name) {
+// queue the job on a libuv worker; `keep` are the JS handles whose native objects the run uses: referenced until
+// the job completes, so a handle dropped meanwhile is not finalised under a running call
+napi_value queue_job(napi_env env, Job* j, const char* name, std::initializer_list<napi_value> keep = {}) {
+  for (napi_value v : keep) {
+    napi_ref r = nullptr;
+    if (napi_create_reference(env, v, 1, &r) == napi_ok) j->keep.push_back(r);
+  }
   napi_value promise, res;
   NAPI_CALL(env, napi_create_promise(env, &j->deferred, &promise));
   NAPI_CALL(env, napi_create_string_utf8(env, name, NAPI_AUTO_LENGTH, &res));
@@ -213,9 +230,200 @@ napi_value wrap_model(napi_env env, wsm_model* h) {
   wsm_get(h, &m->md, &m->st);
   wsm_info_get(h, &m->info);
   napi_value v;
-  NAPI_CALL(env, napi_create_external(env, m, [](napi_env, void* p, void*) {
-    Model* mm = static_cast<Model*>(p); wsm_free(mm->h); delete mm; }, nullptr, &v));
+  NAPI_CALL(env, napi_create_external(env, m, model_finalize, nullptr, &v));
   return v;
+}
+
+// modelFromFields(f): a model over the caller's typed arrays, no copy — the binding of the reference's CpuSolver
+// objects (solver_mi355x.js maps a CpuSolver's fields onto f).  f holds the sizes, `par` (the ws_params fields), the
+// ws_model arrays and the ws_state arrays under their include/weave.h names, each a typed array of the C type and
+// length; state arrays are written in place by a download, as GpuSolver.download overwrites the CpuSolver's arrays
+// (web/js/solver_gpu.js:431-503).  The arrays are referenced for the life of the model.
+napi_value ModelFromFields(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  napi_value f = a[0];
+  if (!is_object(env, f)) { napi_throw_type_error(env, nullptr, "modelFromFields(fields)"); return nullptr; }
+  std::unique_ptr<Model> m(new Model());
+  ws_model& md = m->md;
+  ws_state& st = m->st;
+  auto I = [&](const char* k) { return int32_t(getd(env, f, k, 0)); };
+  md.n = I("n"); md.m = I("m"); md.p = I("p"); md.K = I("K"); md.C = I("C"); md.S = I("S");
+  md.n_point_local = I("n_point_local"); md.l_ref = getd(env, f, "l_ref", 1.0);
+  const size_t n = size_t(std::max(md.n, 0)), mm = size_t(std::max(md.m, 0)), p = size_t(std::max(md.p, 0));
+  const size_t K = size_t(std::max(md.K, 0)), C = size_t(std::max(md.C, 0)), S = size_t(std::max(md.S, 0));
+  const size_t npl = size_t(std::max(md.n_point_local, 0));
+  std::string bad;
+  // binds key to *slot: a typed array of type `want` and `count` elements (absent: null, unless required)
+  auto bind = [&](const char* key, void** slot, napi_typedarray_type want, size_t count, bool required) {
+    if (!bad.empty()) return;
+    bool has = false; napi_value v;
+    if (napi_has_named_property(env, f, key, &has) != napi_ok || !has || napi_get_named_property(env, f, key, &v) != napi_ok) {
+      if (required && count) bad = std::string(key) + " is missing";
+      return;
+    }
+    napi_valuetype t; napi_typeof(env, v, &t);
+    if (t == napi_undefined || t == napi_null) { if (required && count) bad = std::string(key) + " is missing"; return; }
+    size_t len = 0;
+    void* data = typed<void>(env, v, want, &len);
+    if (!data && count) { bad = std::string(key) + " has the wrong typed-array type"; return; }
+    if (len != count) { bad = std::string(key) + " has " + std::to_string(len) + " elements, expected " + std::to_string(count); return; }
+    napi_ref r = nullptr;
+    if (napi_create_reference(env, v, 1, &r) == napi_ok) m->refs.push_back(r);
+    *slot = data;
+  };
+#define BM(field, type, count, req) bind(#field, reinterpret_cast<void**>(const_cast<void**>(reinterpret_cast<const void* const*>(&md.field))), type, count, req)
+#define BS(field, type, count) bind(#field, reinterpret_cast<void**>(&st.field), type, count, false)
+  const auto F64 = napi_float64_array, I32 = napi_int32_array, U8 = napi_uint8_array;
+  BM(node_offset, I32, S + 1, true); BM(seg_offset, I32, S + 1, true); BM(joint_offset, I32, S + 1, true);
+  BM(strand_closed, U8, S, true); BM(seg_i0, I32, mm, true); BM(seg_i1, I32, mm, true);
+  BM(j_seg0, I32, p, true); BM(j_seg1, I32, p, true); BM(j_node, I32, p, true); BM(l0, F64, mm, true); BM(j_len, F64, p, true);
+  BM(EA, F64, mm, true); BM(GA, F64, mm, true); BM(EIx, F64, mm, true); BM(EIy, F64, mm, true); BM(GJ, F64, mm, true);
+  BM(omega0, F64, 3 * p, true);
+  BM(c_seg, I32, C, true); BM(c_node, I32, C, true); BM(c_len, F64, C, true); BM(c_B, F64, 3 * C, true); BM(c_Q, F64, 4 * C, true);
+  BM(mass, F64, n, true); BM(krot_static, F64, 3 * mm, true); BM(load_const, F64, 3 * n, true);
+  BM(point_local_node, I32, npl, true); BM(point_local_F, F64, 3 * npl, true); BM(node_free, U8, 3 * n, true);
+  BM(k_strand, I32, 2 * K, true); BM(k_seg_off, I32, 2 * K, true); BM(k_nseg, I32, 2 * K, true);
+  BM(k_wk, F64, 2 * K, true); BM(k_hk, F64, 2 * K, true); BM(k_fc90, F64, 2 * K, true); BM(k_E90, F64, 2 * K, true);
+  BM(k_Hs, F64, 2 * K, true); BM(k_w, F64, K, true); BM(k_h, F64, K, true); BM(k_enabled, U8, K, true); BM(k_locked, U8, K, true);
+  BM(node_w, F64, n, false); BM(node_h, F64, n, false);
+  // parameters (params.js SOLVER / CONTACT as the reference's _uploadParams reads them, solver_gpu.js:246-284)
+  napi_value par; bool hp = false;
+  if (napi_has_named_property(env, f, "par", &hp) == napi_ok && hp && napi_get_named_property(env, f, "par", &par) == napi_ok) {
+    ws_params& P = md.par;
+#define PD(k) P.k = getd(env, par, #k, P.k)
+#define PI(k) P.k = int32_t(getd(env, par, #k, P.k))
+    PD(mass_scale); PD(damping); PD(tol); PD(mu); PD(strength); PD(stick_strength); PD(torsion_factor); PD(epsilon);
+    PD(slip_rate); PD(rot_factor); PD(dent_rate); PD(edge_threshold); PD(floor_z); PD(floor_mu);
+    PI(kinetic_damping); PI(warmup_iter); PI(max_iter); PI(contacts); PI(ramp_iter); PI(rot_friction); PI(seat_law);
+    PI(seat_gap); PI(dent_after); PI(augmented); PI(al_every); PI(floor_on);
+#undef PD
+#undef PI
+  } else bad = "par is missing";
+  if (md.par.floor_on && (!md.node_w || !md.node_h)) bad = "node_w and node_h are needed with the ground";
+  // the replay protocol (md.protocol)
+  md.proto_mode = I("proto_mode");
+  if (md.proto_mode) {
+    md.proto_probe = I("proto_probe"); md.proto_n = I("proto_n"); md.proto_settle = I("proto_settle"); md.proto_avg = I("proto_avg");
+    get3(env, f, "proto_axis", md.proto_axis);
+    BM(proto_sched, F64, size_t(std::max(md.proto_n, 0)), true);
+  }
+  // state: read on upload, written by a download
+  BS(X, F64, 3 * n); BS(X0, F64, 3 * n); BS(V, F64, 3 * n); BS(Q, F64, 4 * mm); BS(Q0, F64, 4 * mm); BS(W, F64, 3 * mm);
+  BS(u, F64, 2 * K); BS(n_prev, F64, 3 * K); BS(slip, F64, K); BS(stuck, U8, K); BS(lam, F64, K); BS(pen_acc, F64, K);
+  BS(theta0, F64, K); BS(rot_slip, F64, K); BS(dent, F64, 2 * K); BS(climb, F64, 2 * K); BS(tanpsi, F64, 2 * K);
+  BS(dent_e, F64, K); BS(floor_a, F64, 2 * n);
+  BS(N, F64, mm); BS(Vsh, F64, 2 * mm); BS(M, F64, 3 * p); BS(cM, F64, 2 * C); BS(reactions, F64, 3 * n);
+  BS(Fn, F64, K); BS(Ft, F64, K); BS(gap, F64, K); BS(kstate, F64, K); BS(theta, F64, K); BS(Anom, F64, K);
+  BS(pressure, F64, K); BS(util, F64, K); BS(Mrot, F64, K); BS(phi, F64, 2 * K); BS(mid, F64, 3 * K);
+  BS(ctype, I32, K); BS(face_side, I32, K);
+#undef BM
+#undef BS
+  if (bad.empty() && (!st.X || !st.Q || (K && !st.u))) bad = "X, Q (and u with crossings) are needed";
+  st.iteration = I("iteration"); st.pen_n = I("pen_n"); st.ke_prev = getd(env, f, "ke_prev", 0.0);
+  st.residual = getd(env, f, "residual", INFINITY);
+  if (!bad.empty()) {
+    for (napi_ref r : m->refs) napi_delete_reference(env, r);
+    napi_throw_type_error(env, nullptr, ("modelFromFields: " + bad).c_str());
+    return nullptr;
+  }
+  m->info.n = md.n; m->info.m = md.m; m->info.p = md.p; m->info.K = md.K; m->info.C = md.C; m->info.S = md.S;
+  napi_value v;
+  Model* raw = m.release();
+  if (napi_create_external(env, raw, model_finalize, nullptr, &v) != napi_ok) { model_finalize(env, raw, nullptr); return nullptr; }
+  return v;
+}
+
+// modelFields(model): copies of a built model's ws_model / ws_state arrays and scalars under their include/weave.h
+// names (the shape modelFromFields takes), e.g. to assemble an object with the fields of the reference's CpuSolver
+template <typename T> napi_value copy_typed(napi_env env, const T* p, size_t n, napi_typedarray_type type) {
+  napi_value ab, ta; void* data = nullptr;
+  napi_create_arraybuffer(env, n * sizeof(T), &data, &ab);
+  if (n && p) std::memcpy(data, p, n * sizeof(T));
+  else if (n) std::memset(data, 0, n * sizeof(T));
+  napi_create_typedarray(env, type, n, ab, 0, &ta);
+  return ta;
+}
+napi_value ModelFields(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  Model* m = ext<Model>(env, a[0]);
+  if (!m) { napi_throw_type_error(env, nullptr, "modelFields(model)"); return nullptr; }
+  const ws_model& md = m->md; const ws_state& st = m->st;
+  const size_t n = md.n, mm = md.m, p = md.p, K = md.par.contacts ? md.K : 0, C = md.C, S = md.S, npl = md.n_point_local;
+  napi_value o; napi_create_object(env, &o);
+  set(env, o, "n", i32(env, md.n)); set(env, o, "m", i32(env, md.m)); set(env, o, "p", i32(env, md.p));
+  set(env, o, "K", i32(env, int32_t(K))); set(env, o, "C", i32(env, md.C)); set(env, o, "S", i32(env, md.S));
+  set(env, o, "n_point_local", i32(env, md.n_point_local)); set(env, o, "l_ref", num(env, md.l_ref));
+  napi_value par; napi_create_object(env, &par);
+  const ws_params& P = md.par;
+#define PD(k) set(env, par, #k, num(env, P.k))
+#define PI(k) set(env, par, #k, i32(env, P.k))
+  PD(mass_scale); PD(damping); PD(tol); PD(mu); PD(strength); PD(stick_strength); PD(torsion_factor); PD(epsilon);
+  PD(slip_rate); PD(rot_factor); PD(dent_rate); PD(edge_threshold); PD(floor_z); PD(floor_mu);
+  PI(kinetic_damping); PI(warmup_iter); PI(max_iter); PI(contacts); PI(ramp_iter); PI(rot_friction); PI(seat_law);
+  PI(seat_gap); PI(dent_after); PI(augmented); PI(al_every); PI(floor_on);
+#undef PD
+#undef PI
+  set(env, o, "par", par);
+  const auto F64 = napi_float64_array, I32 = napi_int32_array, U8 = napi_uint8_array;
+#define CI(key, ptr, cnt) set(env, o, key, copy_typed<int32_t>(env, ptr, cnt, I32))
+#define CD(key, ptr, cnt) set(env, o, key, copy_typed<double>(env, ptr, cnt, F64))
+#define CU(key, ptr, cnt) set(env, o, key, copy_typed<uint8_t>(env, ptr, cnt, U8))
+  CI("node_offset", md.node_offset, S + 1); CI("seg_offset", md.seg_offset, S + 1); CI("joint_offset", md.joint_offset, S + 1);
+  CU("strand_closed", md.strand_closed, S); CI("seg_i0", md.seg_i0, mm); CI("seg_i1", md.seg_i1, mm);
+  CI("j_seg0", md.j_seg0, p); CI("j_seg1", md.j_seg1, p); CI("j_node", md.j_node, p); CD("l0", md.l0, mm); CD("j_len", md.j_len, p);
+  CD("EA", md.EA, mm); CD("GA", md.GA, mm); CD("EIx", md.EIx, mm); CD("EIy", md.EIy, mm); CD("GJ", md.GJ, mm);
+  CD("omega0", md.omega0, 3 * p); CI("c_seg", md.c_seg, C); CI("c_node", md.c_node, C); CD("c_len", md.c_len, C);
+  CD("c_B", md.c_B, 3 * C); CD("c_Q", md.c_Q, 4 * C); CD("mass", md.mass, n); CD("krot_static", md.krot_static, 3 * mm);
+  CD("load_const", md.load_const, 3 * n); CI("point_local_node", md.point_local_node, npl);
+  CD("point_local_F", md.point_local_F, 3 * npl); CU("node_free", md.node_free, 3 * n);
+  CI("k_strand", md.k_strand, 2 * K); CI("k_seg_off", md.k_seg_off, 2 * K); CI("k_nseg", md.k_nseg, 2 * K);
+  CD("k_wk", md.k_wk, 2 * K); CD("k_hk", md.k_hk, 2 * K); CD("k_fc90", md.k_fc90, 2 * K); CD("k_E90", md.k_E90, 2 * K);
+  CD("k_Hs", md.k_Hs, 2 * K); CD("k_w", md.k_w, K); CD("k_h", md.k_h, K); CU("k_enabled", md.k_enabled, K); CU("k_locked", md.k_locked, K);
+  CD("X", st.X, 3 * n); CD("X0", st.X0, 3 * n); CD("V", st.V, 3 * n); CD("Q", st.Q, 4 * mm); CD("Q0", st.Q0, 4 * mm);
+  CD("W", st.W, 3 * mm); CD("u", st.u, 2 * K); CD("n_prev", st.n_prev, 3 * K); CD("slip", st.slip, K); CU("stuck", st.stuck, K);
+  CD("lam", st.lam, K); CD("pen_acc", st.pen_acc, K); CD("theta0", st.theta0, K); CD("rot_slip", st.rot_slip, K);
+  CD("dent", st.dent, 2 * K); CD("climb", st.climb, 2 * K); CD("tanpsi", st.tanpsi, 2 * K); CD("dent_e", st.dent_e, K);
+  CD("Fn", st.Fn, K); CD("Ft", st.Ft, K); CD("gap", st.gap, K); CD("kstate", st.kstate, K); CD("theta", st.theta, K);
+  CD("Anom", st.Anom, K); CD("pressure", st.pressure, K); CD("util", st.util, K); CD("Mrot", st.Mrot, K);
+  CD("phi", st.phi, 2 * K); CD("mid", st.mid, 3 * K); CI("ctype", st.ctype, K); CI("face_side", st.face_side, K);
+#undef CI
+#undef CD
+#undef CU
+  set(env, o, "iteration", i32(env, st.iteration)); set(env, o, "residual", num(env, st.residual));
+  set(env, o, "ke_prev", num(env, st.ke_prev)); set(env, o, "pen_n", i32(env, st.pen_n));
+  return o;
+}
+
+// modelKnots(model) → Int32Array [6K] (wsm_knots); modelFrames(model) → Int32Array [3·frames] (wsm_frames);
+// modelLockContacts(model, edges | null, lock) (wsm_lock_contacts)
+napi_value ModelKnots(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  Model* m = ext<Model>(env, a[0]);
+  if (!m || !m->h) { napi_throw_type_error(env, nullptr, "modelKnots(model of the builder)"); return nullptr; }
+  int32_t K = 0; wsm_knots(m->h, nullptr, 0, &K);
+  std::vector<int32_t> v(6 * size_t(K)); wsm_knots(m->h, v.data(), K, &K);
+  return copy_typed<int32_t>(env, v.data(), v.size(), napi_int32_array);
+}
+napi_value ModelFrames(napi_env env, napi_callback_info info) {
+  napi_value a[1]; get_args(env, info, 1, a);
+  Model* m = ext<Model>(env, a[0]);
+  if (!m || !m->h) { napi_throw_type_error(env, nullptr, "modelFrames(model of the builder)"); return nullptr; }
+  int32_t nf = 0; wsm_frames(m->h, nullptr, 0, &nf);
+  std::vector<int32_t> v(3 * size_t(nf)); wsm_frames(m->h, v.data(), nf, &nf);
+  return copy_typed<int32_t>(env, v.data(), v.size(), napi_int32_array);
+}
+napi_value ModelLockContacts(napi_env env, napi_callback_info info) {
+  napi_value a[3]; get_args(env, info, 3, a);
+  Model* m = ext<Model>(env, a[0]);
+  if (!m || !m->h) { napi_throw_type_error(env, nullptr, "modelLockContacts(model of the builder, edges, lock)"); return nullptr; }
+  std::vector<int32_t> e; bool all = true;
+  uint32_t ne = array_len(env, a[1]);
+  bool isarr = false; napi_is_array(env, a[1], &isarr);
+  if (isarr) { all = false; for (uint32_t i = 0; i < ne; i++) e.push_back(int32_t(num_at(env, a[1], i))); }
+  bool lock = true; napi_valuetype t; napi_typeof(env, a[2], &t); if (t == napi_boolean) napi_get_value_bool(env, a[2], &lock);
+  int rc = wsm_lock_contacts(m->h, all ? nullptr : e.data(), all ? 0 : int32_t(e.size()), lock ? 1 : 0);
+  if (rc) return throw_status(env, rc, "wsm_lock_contacts");
+  return undef(env);
 }
 
 napi_value ModelHexDome(napi_env env, napi_callback_info info) {
@@ -435,7 +643,7 @@ napi_value scalars(napi_env env, const ws_state& s) {
   set(env, o, "resForce", num(env, s.res_force)); set(env, o, "resTorque", num(env, s.res_torque)); set(env, o, "fref", num(env, s.fref));
   set(env, o, "eStretch", num(env, s.e_stretch)); set(env, o, "eShear", num(env, s.e_shear)); set(env, o, "eBend", num(env, s.e_bend));
   set(env, o, "eTwist", num(env, s.e_twist)); set(env, o, "eContact", num(env, s.e_contact));
-  set(env, o, "resRigid", num(env, s.res_rigid));
+  set(env, o, "resRigid", num(env, s.res_rigid)); set(env, o, "penN", i32(env, s.pen_n));
   return o;
 }
 
@@ -463,6 +671,7 @@ napi_value BatchCreate(napi_env env, napi_callback_info info) {
   o.tile = int32_t(getd(env, a[2], "tile", 0)); o.history = int32_t(getd(env, a[2], "history", 0));
   o.ctrl_chunk = int32_t(getd(env, a[2], "ctrlChunk", 0));
   o.cross_lanes = int32_t(getd(env, a[2], "crossLanes", 0));
+  o.ctrl_mode = int32_t(getd(env, a[2], "ctrlMode", 0));
   std::vector<const ws_model*> mp; std::vector<const ws_state*> sp;
   for (Model* m : ms) { mp.push_back(&m->md); sp.push_back(&m->st); }
   Batch* b = new Batch(); b->ctx = ctx; b->B = int32_t(ms.size());
@@ -516,7 +725,7 @@ napi_value BatchStep(napi_env env, napi_callback_info info) {
     return arr;
   };
   j->error = ctx_error(b->ctx);
-  return queue_job(env, j, "ws_step");
+  return queue_job(env, j, "ws_step", {a[0]});
 }
 
 napi_value BatchReadControl(napi_env env, napi_callback_info info) {
@@ -533,7 +742,7 @@ napi_value BatchReadControl(napi_env env, napi_callback_info info) {
     return arr;
   };
   j->error = ctx_error(b->ctx);
-  return queue_job(env, j, "ws_read_control");
+  return queue_job(env, j, "ws_read_control", {a[0]});
 }
 
 napi_value BatchReadHistory(napi_env env, napi_callback_info info) {
@@ -556,7 +765,7 @@ napi_value BatchReadHistory(napi_env env, napi_callback_info info) {
     return ta;
   };
   j->error = ctx_error(b->ctx);
-  return queue_job(env, j, "ws_read_history");
+  return queue_job(env, j, "ws_read_history", {a[0]});
 }
 
 napi_value BatchReadOutputs(napi_env env, napi_callback_info info) {
@@ -579,7 +788,7 @@ napi_value BatchReadOutputs(napi_env env, napi_callback_info info) {
     return ta;
   };
   j->error = ctx_error(b->ctx);
-  return queue_job(env, j, "ws_read_outputs");
+  return queue_job(env, j, "ws_read_outputs", {a[0]});
 }
 
 napi_value BatchDownload(napi_env env, napi_callback_info info) {
@@ -593,7 +802,7 @@ napi_value BatchDownload(napi_env env, napi_callback_info info) {
   j->run = [=]() { return ws_download(bh, idx, &m->st); };
   j->done = [=](napi_env e) { return scalars(e, m->st); };
   j->error = ctx_error(b->ctx);
-  return queue_job(env, j, "ws_download");
+  return queue_job(env, j, "ws_download", {a[0], a[2]});
 }
 
 napi_value BatchUpload(napi_env env, napi_callback_info info) {
@@ -679,7 +888,7 @@ napi_value ServerPump(napi_env env, napi_callback_info info) {
     return o;
   };
   j->error = ctx_error(s->ctx);
-  return queue_job(env, j, "ws_pump");
+  return queue_job(env, j, "ws_pump", {a[0]});
 }
 
 napi_value ServerPoll(napi_env env, napi_callback_info info) {
@@ -723,7 +932,7 @@ napi_value Init(napi_env env, napi_value exports) {
   struct Fn { const char* name; napi_callback cb; };
   static const Fn fns[] = {
       {"deviceCreate", DeviceCreate}, {"deviceInfo", DeviceInfo},
-      {"modelHexDome", ModelHexDome}, {"modelFromMesh", ModelFromMesh}, {"modelClone", ModelClone}, {"modelInfo", ModelInfo}, {"modelState", ModelState},
+      {"modelHexDome", ModelHexDome}, {"modelFromMesh", ModelFromMesh}, {"modelFromFields", ModelFromFields}, {"modelFields", ModelFields}, {"modelKnots", ModelKnots}, {"modelFrames", ModelFrames}, {"modelLockContacts", ModelLockContacts}, {"modelClone", ModelClone}, {"modelInfo", ModelInfo}, {"modelState", ModelState},
       {"batchCreate", BatchCreate}, {"batchDestroy", BatchDestroy}, {"batchOpts", BatchOpts}, {"batchStep", BatchStep},
       {"batchReadControl", BatchReadControl}, {"batchReadHistory", BatchReadHistory}, {"batchDownload", BatchDownload},
       {"batchReadOutputs", BatchReadOutputs}, {"modelSetProtocol", ModelSetProtocol},

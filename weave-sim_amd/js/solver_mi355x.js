@@ -11,8 +11,9 @@
 //   await g.download(0);                                // m.st.X, m.st.Q, ... overwritten in place
 //   g.dispose();
 //
-// Device calls that wait for the GPU return Promises; calls on one solver are serialised (one host thread per
-// batch handle), as the reference's app serialises them (app.js:164-166).
+// Every call on a solver or server returns a Promise and runs after the calls made before it on the same handle (the
+// C ABI serialises calls on a handle too), as the reference's app serialises them (app.js:164-166).  The models may be
+// the reference's own CpuSolver objects (their typed arrays are bound without a copy) or this package's.
 
 const path = require('path');
 
@@ -56,28 +57,173 @@ class Model {
    *  non-positive displacements first, as CalibrationSet's jobs do) */
   setProtocol(p) { native().modelSetProtocol(this.h, p || {}); this.protocol = p || null; return this; }
   clone() { return new Model(native().modelClone(this.h)); }
+  /** copies of the model's ws_model / ws_state arrays and scalars under their include/weave.h names */
+  fields() { return native().modelFields(this.h); }
+  /** per crossing: mesh edge id, strand_a, k_a, strand_b, k_b, over strand (Int32Array [6K]) */
+  knots() { return native().modelKnots(this.h); }
+  /** the frames (strand, k, node) in frame-key order (Int32Array [3·frames]) */
+  frames() { return native().modelFrames(this.h); }
+  /** CpuSolver.lockContacts(edges) / unlockContacts(edges) (solver_cpu.js:869-870): the crossings on the given mesh
+   *  edges, all when edges is null; GpuSolver.updateParams takes the locks to the device */
+  lockContacts(edges = null) { native().modelLockContacts(this.h, edges ? Array.from(edges) : null, true); return this; }
+  unlockContacts(edges = null) { native().modelLockContacts(this.h, edges ? Array.from(edges) : null, false); return this; }
   get n() { return this.info.n; }
   get m() { return this.info.m; }
   get K() { return this.info.K; }
   get span() { return this.info.span; }
 }
 
+// ------------------------------------------------------------------------------------------ the reference's models
+/** the reference's CpuSolver (web/js/solver_cpu.js:662-720): its state `st` (model.js:10-75), rod `beam`
+ *  (rod_cosserat.js:47-100), goals `mass` / `kRotStatic` / `lRef` / `loads` / `anchors`, contact goal `contact`
+ *  (solver_cpu.js:59-98), ground `floor`, scene parameters and replay `protocol` */
+function isCpuSolver(md) { return !!md && !md.h && md.st && md.beam && md.scene; }
+
+const asType = (x, T) => (x instanceof T ? x : T.from(x || []));
+
+/** the fields of ws_model / ws_state (include/weave.h) over a CpuSolver's arrays: typed arrays of the C type are
+ *  passed as they are (no copy; a download overwrites them in place, as GpuSolver.download does,
+ *  solver_gpu.js:431-503), the rest converted; `back` lists the state arrays a download must copy back */
+function cpuSolverFields(md) {
+  const st = md.st, rod = md.beam, ct = md.contact || null, sc = md.scene, sp = sc.solver, cp = sc.contact;
+  const fl = md.floor || null, pr = md.protocol || null;
+  const S = st.S, n = st.n, m = st.m, p = st.p, K = ct ? st.K : 0, C = rod.cSeg.length;
+  // strands: a closed lath has as many segments as nodes, its joints wrap (buildState, model.js:36-37)
+  const closed = new Uint8Array(S), jOff = new Int32Array(S + 1);
+  for (let s = 0; s < S; s++) {
+    const nn = st.nodeOffset[s + 1] - st.nodeOffset[s], ns = st.segOffset[s + 1] - st.segOffset[s];
+    closed[s] = nn > 0 && ns === nn ? 1 : 0;
+    jOff[s + 1] = jOff[s] + (closed[s] ? ns : Math.max(ns - 1, 0));
+  }
+  const cB = new Float64Array(3 * C);
+  rod.cB.forEach((b, c) => { for (let k = 0; k < 3; k++) cB[3 * c + k] = b[k]; });
+  const pl = Array.from(md.loads.pointLocal || []);
+  const plNode = Int32Array.from(pl, ([nid]) => nid), plF = new Float64Array(3 * pl.length);
+  pl.forEach(([, Fv], q) => { for (let k = 0; k < 3; k++) plF[3 * q + k] = +Fv[k]; });
+  const back = [];
+  const stateArr = (x, T) => { if (x instanceof T) return x; const y = T.from(x); back.push([y, x]); return y; };
+  const f = {
+    n, m, p, K, C, S, n_point_local: pl.length, l_ref: md.lRef,
+    par: {
+      mass_scale: +sp.mass_scale, damping: +sp.damping, tol: +sp.tol, kinetic_damping: sp.kinetic_damping ? 1 : 0,
+      warmup_iter: sp.warmup_iter | 0, max_iter: sp.max_iter | 0, contacts: ct ? 1 : 0,
+      ramp_iter: Math.max(cp.ramp_iter | 0, 0), augmented: cp.augmented ? 1 : 0, al_every: cp.al_every | 0, mu: +cp.mu,
+      strength: +cp.strength, stick_strength: +cp.stick_strength, torsion_factor: +cp.torsion_factor,
+      epsilon: +cp.epsilon, slip_rate: +cp.slip_rate, rot_friction: cp.rot_friction ? 1 : 0, rot_factor: +cp.rot_factor,
+      seat_law: cp.seat_law && cp.seat_law !== 'off' ? 1 : 0, seat_gap: cp.seat_gap ? 1 : 0, dent_rate: +cp.dent_rate,
+      dent_after: cp.dent_after_iter >= 0 ? cp.dent_after_iter : (ct ? ct.dentAfterDefault : 0),
+      edge_threshold: +cp.edge_threshold, floor_on: fl ? 1 : 0, floor_z: fl ? +fl.p.z : 0,
+      floor_mu: fl ? Math.max(+fl.p.mu || 0, 0) : 0,
+    },
+    node_offset: asType(st.nodeOffset, Int32Array), seg_offset: asType(st.segOffset, Int32Array), joint_offset: jOff,
+    strand_closed: closed, seg_i0: asType(st.segI0, Int32Array), seg_i1: asType(st.segI1, Int32Array),
+    j_seg0: asType(st.jSeg0, Int32Array), j_seg1: asType(st.jSeg1, Int32Array), j_node: asType(st.jNode, Int32Array),
+    l0: asType(st.l0, Float64Array), j_len: asType(st.jLen, Float64Array),
+    EA: asType(rod.segEA, Float64Array), GA: asType(rod.segGA, Float64Array), EIx: asType(rod.segEIx, Float64Array),
+    EIy: asType(rod.segEIy, Float64Array), GJ: asType(rod.segGJ, Float64Array), omega0: asType(st.Omega0, Float64Array),
+    c_seg: asType(rod.cSeg, Int32Array), c_node: asType(rod.cNode, Int32Array), c_len: asType(rod.cLen, Float64Array),
+    c_B: cB, c_Q: asType(rod.cQ, Float64Array),
+    mass: asType(md.mass, Float64Array), krot_static: asType(md.kRotStatic, Float64Array),
+    load_const: asType(md.loads.const, Float64Array), point_local_node: plNode, point_local_F: plF,
+    node_free: asType(md.anchors.nodeFree, Uint8Array),
+    proto_mode: pr ? pr.mode : 0,
+    // the state, read on upload and written by a download
+    X: stateArr(st.X, Float64Array), X0: stateArr(st.X0, Float64Array), V: stateArr(st.V, Float64Array),
+    Q: stateArr(st.Q, Float64Array), Q0: stateArr(st.Q0, Float64Array), W: stateArr(st.Wr, Float64Array),
+    N: stateArr(rod.N, Float64Array), Vsh: stateArr(rod.V, Float64Array), M: stateArr(rod.M, Float64Array),
+    cM: stateArr(rod.cM, Float64Array), reactions: stateArr(md.anchors.reactions, Float64Array),
+    iteration: st.iteration | 0, residual: st.residual, ke_prev: md.kePrev || 0, pen_n: ct ? ct.penN | 0 : 0,
+  };
+  if (pr) Object.assign(f, { proto_probe: pr.probe, proto_n: pr.sched.length, proto_settle: pr.settle, proto_avg: pr.avg,
+    proto_axis: Array.from(pr.axis || [0, 0, -1]), proto_sched: Float64Array.from(pr.sched) });
+  if (ct) {
+    Object.assign(f, {
+      k_strand: asType(ct.strand, Int32Array), k_seg_off: asType(ct.segOff, Int32Array), k_nseg: asType(ct.nseg, Int32Array),
+      k_wk: asType(ct.wk, Float64Array), k_hk: asType(ct.hk, Float64Array), k_fc90: asType(ct.fc90, Float64Array),
+      k_E90: asType(ct.E90, Float64Array), k_Hs: asType(ct.Hs, Float64Array), k_w: asType(ct.w, Float64Array),
+      k_h: asType(ct.h, Float64Array), k_enabled: asType(ct.enabled, Uint8Array), k_locked: asType(ct.locked, Uint8Array),
+      u: stateArr(st.u, Float64Array), n_prev: stateArr(st.nPrev, Float64Array), slip: stateArr(st.slip, Float64Array),
+      stuck: stateArr(st.stuck, Uint8Array), lam: stateArr(st.lam, Float64Array), pen_acc: stateArr(ct.penAcc, Float64Array),
+      theta0: stateArr(ct.theta0, Float64Array), rot_slip: stateArr(ct.rotSlip, Float64Array),
+      dent: stateArr(ct.dent, Float64Array), climb: stateArr(ct.climb, Float64Array), tanpsi: stateArr(ct.tanpsi, Float64Array),
+      dent_e: stateArr(ct.dentE, Float64Array), Fn: stateArr(ct.Fn, Float64Array), Ft: stateArr(ct.Ft, Float64Array),
+      gap: stateArr(ct.gap, Float64Array), kstate: stateArr(ct.state, Float64Array), theta: stateArr(ct.theta, Float64Array),
+      Anom: stateArr(ct.Anom, Float64Array), pressure: stateArr(ct.pressure, Float64Array), util: stateArr(ct.util, Float64Array),
+      Mrot: stateArr(ct.Mrot, Float64Array), phi: stateArr(ct.phi, Float64Array), mid: stateArr(ct.mid, Float64Array),
+      ctype: stateArr(ct.ctype, Int32Array), face_side: stateArr(ct.faceSide, Int32Array),
+    });
+  }
+  let floorA = null;
+  if (fl) {
+    floorA = new Float64Array(2 * n);
+    for (let i = 0; i < n; i++) { floorA[2 * i] = fl.ax[i]; floorA[2 * i + 1] = fl.ay[i]; }
+    Object.assign(f, { node_w: asType(fl.w, Float64Array), node_h: asType(fl.h, Float64Array), floor_a: floorA });
+  }
+  return { fields: f, back, floorA };
+}
+
+/** one CpuSolver as a device instance: a native model over its arrays, rebuilt when the CpuSolver's goals were
+ *  rebuilt (upload, updateParams) */
+class CpuSolverInstance {
+  constructor(md) { this.cpu = md; this.bind(); }
+  bind() {
+    const { fields, back, floorA } = cpuSolverFields(this.cpu);
+    this.back = back; this.floorA = floorA;
+    this.h = native().modelFromFields(fields);
+  }
+  /** after a download: converted arrays copied back, the scalars and derived fields the reference's download sets
+   *  (solver_gpu.js:474-497) */
+  afterDownload(s) {
+    const md = this.cpu, st = md.st, ct = md.contact, rod = md.beam;
+    for (const [from, to] of this.back) for (let i = 0; i < to.length; i++) to[i] = from[i];
+    if (this.floorA && md.floor) for (let i = 0; i < st.n; i++) { md.floor.ax[i] = this.floorA[2 * i]; md.floor.ay[i] = this.floorA[2 * i + 1]; }
+    st.iteration = s.iteration; st.residual = s.residual; md.kePrev = s.kePrev;
+    md.converged = !!s.converged;
+    md.residualParts = { force: s.resForce, torque: s.resTorque, rigid: s.resRigid };
+    rod.energyParts = { bending: s.eBend, torsion: s.eTwist, stretch: s.eStretch, shear: s.eShear };
+    rod.energy = s.eStretch + s.eShear + s.eBend + s.eTwist;
+    if (ct) { ct.energy = s.eContact; ct.penN = s.penN; }
+    // the frame inertia of the state (CpuSolver.step), when the object carries the reference's methods
+    if (md.inertia3 && typeof rod.updateBendCoupling === 'function') {
+      const M = st.m, kTw = new Float64Array(M);
+      if (ct && typeof ct.addTwistStiffness === 'function') ct.addTwistStiffness(st, kTw);
+      const ms = md.scene.solver.mass_scale, kD = rod.updateBendCoupling(st);
+      for (let j = 0; j < M; j++) for (let k = 0; k < 3; k++) md.inertia3[3 * j + k] = ms * Math.max(md.kRotStatic[3 * j + k] + kTw[j] + kD[3 * j + k], 1e-12);
+    }
+    return md;
+  }
+}
+
+/** a per-handle chain: every call on a solver or server runs after the previous one has finished, so a call made
+ *  while a step or a pump is on a worker thread waits for it (the reference's app serialises with A.busy,
+ *  app.js:164-166) */
+class Chain {
+  constructor() { this.tail = Promise.resolve(); }
+  run(fn) {
+    const p = this.tail.then(fn);
+    this.tail = p.catch(() => {});
+    return p;
+  }
+}
+
 class GpuSolver {
-  /** new GpuSolver(device, models, { perSubmit, tile, precision }) (solver_gpu.js:65-123).  Instances may differ in
-   *  topology; precision 'fp32' (FP32 forces, FP64 positions / frames / energy / residual) or 'fp64' */
+  /** new GpuSolver(device, models, { perSubmit, tile, precision }) (solver_gpu.js:65-123).  models: the reference's
+   *  CpuSolver objects, or models of this package's builder; instances may differ in topology; precision 'fp32'
+   *  (FP32 forces, FP64 positions / frames / energy / residual) or 'fp64' */
   constructor(device, models, opts = {}) {
     if (!device || !device.ctx) throw new Error('GpuSolver: no device');
     this.dev = device;
     this.models = models;
     this.B = models.length;
+    this.inst = models.map((md) => (isCpuSolver(md) ? new CpuSolverInstance(md) : md));
     const precision = opts.precision === 'fp64' ? FP64 : FP32_FP64;
-    this.h = native().batchCreate(device.ctx, models.map((m) => m.h),
+    this.h = native().batchCreate(device.ctx, this.inst.map((m) => m.h),
       { precision, perSubmit: opts.perSubmit || 0, tile: opts.tile === 'autotune' ? -1 : (opts.tile || 0), history: opts.history || 0,
         crossLanes: opts.crossLanes || 0 });
     this.perSubmit = native().batchOpts(this.h).perSubmit;
     this.ctlHost = new Array(this.B).fill(null);
-    this._chain = Promise.resolve();
-    this.sizes = models.map((m) => ({ n: m.n, m: m.m, K: m.K }));
+    this._chain = new Chain();
+    this.sizes = models.map((m) => (isCpuSolver(m) ? { n: m.st.n, m: m.st.m, K: m.contact ? m.st.K : 0 } : { n: m.n, m: m.m, K: m.K }));
     // fixed-point scales of the reference (solver_gpu.js:132-159) have no use here: positions and frames are FP64
     this.PB = 0; this.FB = 0; this.TB = 0; this.KB = 0;
   }
@@ -89,17 +235,28 @@ class GpuSolver {
   async compileInfo() { return []; }
 
   _serial(fn) {
-    const p = this._chain.then(fn);
-    this._chain = p.catch(() => {});
-    return p;
+    if (!this.h) return Promise.reject(new Error('GpuSolver: disposed'));
+    return this._chain.run(() => { if (!this.h) throw new Error('GpuSolver: disposed'); return fn(); });
   }
 
-  /** upload(b) (solver_gpu.js:191): the model's state and goals to the device */
-  upload(b) { native().batchUpload(this.h, b, this.models[b].h); }
+  /** upload(b) (solver_gpu.js:191): the model's state and goals to the device (after the calls queued before it) */
+  upload(b) {
+    return this._serial(() => {
+      const it = this.inst[b];
+      if (it instanceof CpuSolverInstance) it.bind();
+      native().batchUpload(this.h, b, it.h);
+    });
+  }
 
-  /** updateParams(b) (solver_gpu.js:323-333): new goals and parameters; throws if the rotation clamps changed */
-  updateParams(b) { native().batchUpdateParams(this.h, b, this.models[b].h); }
-  _uploadParams(b) { this.updateParams(b); }
+  /** updateParams(b) (solver_gpu.js:323-333): new goals and parameters; rejects if the rotation clamps changed */
+  updateParams(b) {
+    return this._serial(() => {
+      const it = this.inst[b];
+      if (it instanceof CpuSolverInstance) it.bind();    // a rebuild replaced the CpuSolver's goal arrays
+      native().batchUpdateParams(this.h, b, it.h);
+    });
+  }
+  _uploadParams(b) { return this.updateParams(b); }
 
   /** step(nIter, { instances, force }) (solver_gpu.js:338-361) → control records */
   step(nIter, opts = {}) {
@@ -114,9 +271,11 @@ class GpuSolver {
   }
 
   /** readHistory(b) (solver_gpu.js:389-405): the rows written since the last call, appended to model.st.history */
-  readHistory(b) {
-    return this._serial(() => native().batchReadHistory(this.h, b, 4096).then((rows) => {
-      const H = this.models[b].st.history;
+  readHistory(b) { return this._serial(() => this._readHistory(b)); }
+  _readHistory(b) {
+    return native().batchReadHistory(this.h, b, 4096).then((rows) => {
+      const st = this.models[b].st;
+      const H = st.history || (st.history = { kinetic: [], strain: [], bending: [], torsion: [], stretch: [], shear: [], contact: [], residual: [] });
       const n = rows.length / 9;
       for (let k = 0; k < n; k++) {
         const o = 9 * k;
@@ -124,30 +283,34 @@ class GpuSolver {
         H.stretch.push(rows[o + 4]); H.shear.push(rows[o + 5]); H.contact.push(rows[o + 6]); H.residual.push(rows[o + 7]);
       }
       return n;
+    });
+  }
+
+  /** download(b, { history }) (solver_gpu.js:431-503): the device state of instance b into its model's arrays, in
+   *  place, then (history, default true) the history rows written since the last read */
+  download(b, opts = {}) {
+    const it = this.inst[b], md = this.models[b];
+    return this._serial(() => native().batchDownload(this.h, b, it.h).then((s) => {
+      if (it instanceof CpuSolverInstance) it.afterDownload(s);
+      else {
+        Object.assign(md.st, { iteration: s.iteration, residual: s.residual });
+        md.converged = !!s.converged; md.kePrev = s.kePrev;
+        md.residualParts = { force: s.resForce, torque: s.resTorque, fref: s.fref };
+        md.energies = { stretch: s.eStretch, shear: s.eShear, bending: s.eBend, torsion: s.eTwist, contact: s.eContact };
+      }
+      return opts.history === false ? md : this._readHistory(b).then(() => md);
     }));
   }
 
-  /** download(b) (solver_gpu.js:431-503): the device state of instance b into its model's arrays, in place */
-  download(b) {
-    const m = this.models[b];
-    return this._serial(() => native().batchDownload(this.h, b, m.h).then((s) => {
-      Object.assign(m.st, { iteration: s.iteration, residual: s.residual });
-      m.converged = !!s.converged; m.kePrev = s.kePrev;
-      m.residualParts = { force: s.resForce, torque: s.resTorque, fref: s.fref };
-      m.energies = { stretch: s.eStretch, shear: s.eShear, bending: s.eBend, torsion: s.eTwist, contact: s.eContact };
-      return m;
-    }));
-  }
-
-  /** downloadAll(instances) (solver_gpu.js:418-428) */
-  async downloadAll(instances = null) {
+  /** downloadAll(instances, opts) (solver_gpu.js:418-428) */
+  async downloadAll(instances = null, opts = {}) {
     const out = [];
-    for (const b of instances || [...Array(this.B).keys()]) out.push(await this.download(b));
+    for (const b of instances || [...Array(this.B).keys()]) out.push(await this.download(b, opts));
     return out;
   }
 
   /** internalize(b) (solver_gpu.js:507-520): the current shape becomes the displacement origin, slip lengths reset */
-  internalize(b) { native().batchInternalize(this.h, b); }
+  internalize(b) { return this._serial(() => { native().batchInternalize(this.h, b); }); }
 
   /** readOutputs(b) (solver_gpu.js:408-411): the replay outputs recorded so far, a Float64Array of rows (travel,
    *  reaction, value, iteration) */
@@ -160,13 +323,16 @@ class GpuSolver {
     return out;
   }
 
-  /** dispose() (solver_gpu.js:522) */
-  dispose() { if (this.h) { native().batchDestroy(this.h); this.h = null; } }
+  /** dispose() (solver_gpu.js:522): after the calls queued before it */
+  dispose() {
+    if (!this.h) return Promise.resolve();
+    return this._chain.run(() => { if (this.h) { native().batchDestroy(this.h); this.h = null; } });
+  }
 }
 
 /** buildResults (solver_cpu.js:878-922) over a downloaded model: per-segment Mx My T N Vx Vy, per-crossing contact
- *  layers, per-node displacement, reactions of the anchored nodes keyed "strand,node", energies.  Knots are indexed
- *  by crossing; the ground's per-node force is not part of a download. */
+ *  layers, per-node displacement, reactions of the anchored frames keyed "strand,k" (a fixed node that is no frame:
+ *  "node,strand,node"), energies.  The ground's per-node force is not part of a download. */
 function buildResults(model) {
   const st = model.st, info = model.info, n = info.n, m = info.m, K = info.K;
   const mom = native().modelSegmentMoments(model.h);
@@ -184,10 +350,18 @@ function buildResults(model) {
     if (info.augmented) knot.lambda = pick(st.lam);
   } else for (const k of ['Fn', 'Ft', 'slip', 'state']) knot[k] = new Float64Array(K);
   const reactions = new Map();
+  const fixed = (i) => !(info.nodeFree[3 * i] && info.nodeFree[3 * i + 1] && info.nodeFree[3 * i + 2]);
+  const fr = model.frames(), framed = new Set();
+  for (let r = 0; r < fr.length / 3; r++) {
+    const s = fr[3 * r], k = fr[3 * r + 1], i = fr[3 * r + 2];
+    if (!fixed(i)) continue;
+    reactions.set(`${s},${k}`, [st.reactions[3 * i], st.reactions[3 * i + 1], st.reactions[3 * i + 2], 0, 0, 0]);
+    framed.add(i);
+  }
   for (let s = 0; s < info.S; s++) {
     for (let i = info.nodeOffset[s]; i < info.nodeOffset[s + 1]; i++) {
-      if (info.nodeFree[3 * i] && info.nodeFree[3 * i + 1] && info.nodeFree[3 * i + 2]) continue;
-      reactions.set(`${s},${i - info.nodeOffset[s]}`, [st.reactions[3 * i], st.reactions[3 * i + 1], st.reactions[3 * i + 2], 0, 0, 0]);
+      if (!fixed(i) || framed.has(i)) continue;
+      reactions.set(`node,${s},${i - info.nodeOffset[s]}`, [st.reactions[3 * i], st.reactions[3 * i + 1], st.reactions[3 * i + 2], 0, 0, 0]);
     }
   }
   const e = model.energies || {};
@@ -197,20 +371,57 @@ function buildResults(model) {
       shear: e.shear || 0, contact: e.contact || 0, beam, strain: beam + (e.contact || 0) }, rod: 'cosserat' };
 }
 
-/** resultsJSON (app.js:360-390): per-strand segment arrays, knot values by crossing index, per-strand displacements,
- *  reactions */
+/** resultsJSON (app.js:360-390): per-strand segment arrays, knot values by the knot's mesh edge id, per-strand
+ *  displacements, reactions */
 function resultsJSON(model, r) {
   const info = model.info;
   const perStrand = (a) => { const out = []; for (let s = 0; s < info.S; s++) out.push(Array.from(a.slice(info.segOffset[s], info.segOffset[s + 1]))); return out; };
   const segment = {}; for (const k of ['Mx', 'My', 'T', 'N', 'Vx', 'Vy']) segment[k] = perStrand(r.segment[k]);
-  const knot = {}; for (const [k, v] of Object.entries(r.knot)) { const o = {}; v.forEach((x, c) => { o[String(c)] = x; }); knot[k] = o; }
+  const kn = info.K ? model.knots() : new Int32Array(0);
+  const knot = {}; for (const [k, v] of Object.entries(r.knot)) { const o = {}; v.forEach((x, c) => { o[String(kn[6 * c])] = x; }); knot[k] = o; }
   const disp = [];
   for (let s = 0; s < info.S; s++) { const rows = []; for (let i = info.nodeOffset[s]; i < info.nodeOffset[s + 1]; i++) rows.push([r.disp[3 * i], r.disp[3 * i + 1], r.disp[3 * i + 2]]); disp.push(rows); }
   const reactions = {}; for (const [k, v] of r.reactions) reactions[k] = v;
   return { iterations: r.iterations, converged: r.converged, residual: r.residual, segment, knot, disp, reactions, energies: r.energies };
 }
 
-/** Continuous-batching request server (include/weave.h, serving): submit solves as they arrive, pump, poll */
+/** a float as Python's repr writes it (the reports' tables were written by Python): the shortest round-trip digits,
+ *  positional for decimal exponents −4 … 15, else d.ddde±XX */
+function fmt(x) {
+  const v = +x;
+  if (!Number.isFinite(v)) return Number.isNaN(v) ? 'nan' : (v > 0 ? 'inf' : '-inf');
+  if (v === 0) return Object.is(v, -0) ? '-0.0' : '0.0';
+  const parts = v.toExponential().split('e'), exp = parseInt(parts[1], 10);
+  const neg = parts[0][0] === '-', digits = parts[0].replace('-', '').replace('.', '');
+  let body;
+  if (exp < -4 || exp >= 16) {
+    body = (digits.length > 1 ? digits[0] + '.' + digits.slice(1) : digits) + 'e' + (exp < 0 ? '-' : '+') + String(Math.abs(exp)).padStart(2, '0');
+  } else if (exp >= 0) {
+    body = digits.length <= exp + 1 ? digits + '0'.repeat(exp + 1 - digits.length) + '.0' : digits.slice(0, exp + 1) + '.' + digits.slice(exp + 1);
+  } else body = '0.' + '0'.repeat(-exp - 1) + digits;
+  return (neg ? '-' : '') + body;
+}
+
+/** the knots table of the reference's reports (examples/scenes/report-DATE/knots.csv): one row per crossing in knot-edge order */
+function knotsCSV(model, r) {
+  const kn = model.knots(), K = kn.length / 6, out = ['knot_edge,strand_a,k_a,strand_b,k_b,over,Fn,Ft,gap,slip,state'];
+  for (let c = 0; c < K; c++) {
+    const ids = Array.from(kn.slice(6 * c, 6 * c + 6)).join(',');
+    out.push(ids + ',' + ['Fn', 'Ft', 'gap', 'slip', 'state'].map((k) => fmt(r.knot[k][c])).join(','));
+  }
+  return out.join('\n') + '\n';
+}
+
+/** the segments table of the reference's reports (examples/scenes/report-DATE/segments.csv): strand, segment, Mx, My, T, N */
+function segmentsCSV(model, r) {
+  const info = model.info, out = ['strand,segment,Mx,My,T,N'];
+  for (let s = 0; s < info.S; s++) for (let j = info.segOffset[s]; j < info.segOffset[s + 1]; j++)
+    out.push(`${s},${j - info.segOffset[s]},` + ['Mx', 'My', 'T', 'N'].map((k) => fmt(r.segment[k][j])).join(','));
+  return out.join('\n') + '\n';
+}
+
+/** Continuous-batching request server (include/weave.h, serving): submit solves as they arrive, pump, poll.  Every
+ *  call is queued on the server's chain, so a request submitted while a pump runs on a worker joins after it */
 class Server {
   constructor(device, opts = {}) {
     this.h = native().serverCreate(device.ctx, {
@@ -218,19 +429,45 @@ class Server {
       maxSlots: opts.maxSlots || 0, hbmBudget: opts.hbmBudget || 0,
     });
     this.keep = new Map();
+    this._chain = new Chain();
   }
+  _serial(fn) {
+    if (!this.h) return Promise.reject(new Error('Server: disposed'));
+    return this._chain.run(() => { if (!this.h) throw new Error('Server: disposed'); return fn(); });
+  }
+  /** → Promise<request id>; the model (a package model or a CpuSolver) stays referenced until its result is polled */
   submit(model, maxIter = 0) {
-    const id = native().serverSubmit(this.h, model.h, maxIter);
-    this.keep.set(id, model);
-    return id;
+    return this._serial(() => {
+      const it = isCpuSolver(model) ? new CpuSolverInstance(model) : model;
+      const id = native().serverSubmit(this.h, it.h, maxIter);
+      this.keep.set(id, { model, it });
+      return id;
+    });
   }
-  cancel(id) { native().serverCancel(this.h, id); }
-  pump() { return native().serverPump(this.h); }
+  cancel(id) { return this._serial(() => { native().serverCancel(this.h, id); }); }
+  /** → Promise<{ active, queued }>: admit, relax one chunk, retire */
+  pump() { return this._serial(() => native().serverPump(this.h)); }
+  /** → Promise<results>: each with its model, whose arrays hold the final state */
   poll() {
-    return native().serverPoll(this.h).map((r) => { r.model = this.keep.get(r.id); this.keep.delete(r.id); return r; });
+    return this._serial(() => native().serverPoll(this.h).map((r) => {
+      const k = this.keep.get(r.id);
+      this.keep.delete(r.id);
+      r.model = k ? k.model : undefined;
+      if (k && k.it instanceof CpuSolverInstance) {
+        const md = k.model;
+        for (const [from, to] of k.it.back) for (let i = 0; i < to.length; i++) to[i] = from[i];
+        md.st.iteration = r.iterations; md.st.residual = r.residual;
+      }
+      return r;
+    }));
   }
   clockMs() { return native().serverClockMs(this.h); }
-  dispose() { if (this.h) { native().serverDestroy(this.h); this.h = null; } }
+  dispose() {
+    if (!this.h) return Promise.resolve();
+    return this._chain.run(() => { if (this.h) { native().serverDestroy(this.h); this.h = null; } });
+  }
 }
 
-module.exports = { getDevice, Model, GpuSolver, Server, buildResults, resultsJSON, FP32_FP64, FP64 };
+module.exports = { getDevice, Model, GpuSolver, Server, buildResults, resultsJSON, knotsCSV, segmentsCSV, FP32_FP64, FP64,
+  // the CpuSolver binding, for its round-trip test: the fields it builds and the native model over them
+  _internal: { cpuSolverFields, modelFromFields: (f) => native().modelFromFields(f), modelFields: (h) => native().modelFields(h) } };

@@ -223,6 +223,42 @@ class Model:
         dt = {A.P_d: np.float64, A.P_u8: np.uint8, A.P_i32: np.int32}[t]
         return _view(ptr, count if count is not None else sizes[name], dt)
 
+    def knots(self):
+        """per crossing: mesh edge id, strand_a, k_a, strand_b, k_b, over strand (K × 6; wsm_knots)"""
+        K = C.c_int32()
+        A.model_lib().wsm_knots(self.h, None, 0, C.byref(K))
+        out = np.zeros((K.value, 6), dtype=np.int32)
+        if K.value:
+            A.model_lib().wsm_knots(self.h, out.ctypes.data_as(A.P_i32), K.value, C.byref(K))
+        return out
+
+    def frames(self):
+        """the model's frames: rows (strand, frame k, node) in frame-key order (st.nodeFrameIds; wsm_frames)"""
+        n = C.c_int32()
+        A.model_lib().wsm_frames(self.h, None, 0, C.byref(n))
+        out = np.zeros((n.value, 3), dtype=np.int32)
+        if n.value:
+            A.model_lib().wsm_frames(self.h, out.ctypes.data_as(A.P_i32), n.value, C.byref(n))
+        return out
+
+    @property
+    def knot_edges(self):
+        """the mesh edge id of each crossing (st.knotEdges)"""
+        return self.knots()[:, 0].copy()
+
+    def lock_contacts(self, edges=None, lock=True):
+        """CpuSolver.lockContacts(edges) / unlockContacts (lock=False): the crossings on the given mesh edges, all
+        when edges is None; GpuSolver.update_params takes the locks to the device"""
+        e = None if edges is None else np.ascontiguousarray(np.asarray(edges, dtype=np.int32).reshape(-1))
+        rc = A.model_lib().wsm_lock_contacts(self.h, e.ctypes.data_as(A.P_i32) if e is not None else None,
+                                             0 if e is None else len(e), 1 if lock else 0)
+        if rc:
+            raise ValueError("wsm_lock_contacts failed")
+        return self
+
+    def unlock_contacts(self, edges=None):
+        return self.lock_contacts(edges, lock=False)
+
     def add_load(self, node, F):
         f = (C.c_double * 3)(*F)
         A.model_lib().wsm_add_load(self.h, int(node), f)

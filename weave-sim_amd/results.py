@@ -1,11 +1,11 @@
-"""Results of a relaxed instance, as the reference's app reports them: buildResults (web/js/solver_cpu.js:878-922)
-and the results JSON export (resultsJSON, web/js/app.js:360-390), over a downloaded state.
+"""Results of a relaxed instance, as the reference's app reports them: buildResults (web/js/solver_cpu.js:878-922),
+the results JSON export (resultsJSON, web/js/app.js:360-390) and the knots / segments tables of the reference's
+reports (examples/scenes/report-*/knots.csv, segments.csv), over a downloaded state.
 
     r = build_results(model, state)        # state: a StateBuffers filled by GpuSolver.download (or the model itself)
-    doc = results_json(model, r)           # per-strand segment arrays, knot values, displacements, reactions
+    doc = results_json(model, r)           # per-strand segment arrays, knot values by mesh edge id, reactions by frame
+    knots_csv(model, r), segments_csv(model, r)
 
-Differences from the reference, by design: knot values are keyed by crossing index (the reference keys them by the
-mesh edge id of the knot); reactions by "strand,node" (the strand-local node index; the reference by frame key).
 The ground's per-node force is not part of a download, so `floor` is absent."""
 import ctypes as C
 
@@ -73,11 +73,20 @@ def build_results(model, state, contact_options=None):
         knot = {k: np.zeros(K) for k in ("Fn", "Ft", "slip", "state")}
     free = np.ctypeslib.as_array(md.node_free, shape=(3 * n,)).reshape(n, 3)
     R = _arr(state, "reactions", 3 * n).reshape(n, 3)
-    noff = np.ctypeslib.as_array(md.node_offset, shape=(md.S + 1,))
+    # the reactions of the anchored frames, keyed by frame (strand, k) as buildResults keys them by frame key
+    # (solver_cpu.js:913); a fixed node that is no frame is reported under its strand-local node index
     reactions = {}
+    framed = set()
+    for s, k, i in model.frames():
+        if not free[i].all():
+            reactions[(int(s), int(k))] = [float(R[i, 0]), float(R[i, 1]), float(R[i, 2]), 0.0, 0.0, 0.0]
+            framed.add(int(i))
+    noff = np.ctypeslib.as_array(md.node_offset, shape=(md.S + 1,))
     for i in np.flatnonzero(~free.all(axis=1)):
+        if int(i) in framed:
+            continue
         s = int(np.searchsorted(noff, i, side="right") - 1)
-        reactions[(s, int(i - noff[s]))] = [float(R[i, 0]), float(R[i, 1]), float(R[i, 2]), 0.0, 0.0, 0.0]
+        reactions[("node", s, int(i - noff[s]))] = [float(R[i, 0]), float(R[i, 1]), float(R[i, 2]), 0.0, 0.0, 0.0]
     energies = {"kinetic": st.ke, "bending": st.e_bend, "torsion": st.e_twist, "stretch": st.e_stretch,
                 "shear": st.e_shear, "contact": st.e_contact,
                 "beam": st.e_bend + st.e_twist + st.e_stretch + st.e_shear}
@@ -88,16 +97,47 @@ def build_results(model, state, contact_options=None):
 
 
 def results_json(model, r):
-    """the export document (resultsJSON, app.js:360-390): per-strand segment arrays, knot values by crossing index,
-    per-strand node displacements, reactions by "strand,node" """
+    """the export document (resultsJSON, app.js:360-390): per-strand segment arrays, knot values by the knot's mesh
+    edge id, per-strand node displacements, reactions by "strand,k" of the anchored frame"""
     md = model.md
     soff = np.ctypeslib.as_array(md.seg_offset, shape=(md.S + 1,))
     noff = np.ctypeslib.as_array(md.node_offset, shape=(md.S + 1,))
     per_strand = lambda a: [a[soff[s]:soff[s + 1]].tolist() for s in range(md.S)]
     seg = {k: per_strand(r["segment"][k]) for k in SEG_LAYERS if k in r["segment"]}
-    knot = {k: {str(c): float(v[c]) for c in range(len(v))} for k, v in r["knot"].items()}
+    edges = model.knot_edges if md.K else np.zeros(0, dtype=np.int32)
+    knot = {k: {str(int(edges[c])): float(v[c]) for c in range(len(v))} for k, v in r["knot"].items()}
     d = r["disp"].reshape(-1, 3)
     disp = [d[noff[s]:noff[s + 1]].tolist() for s in range(md.S)]
-    reactions = {f"{s},{k}": v for (s, k), v in r["reactions"].items()}
+    reactions = {",".join(str(x) for x in key): v for key, v in r["reactions"].items()}
     return {"iterations": int(r["iterations"]), "converged": r["converged"], "residual": float(r["residual"]),
             "segment": seg, "knot": knot, "disp": disp, "reactions": reactions, "energies": r["energies"]}
+
+
+def _fmt(x):
+    return repr(float(x))
+
+
+def knots_csv(model, r):
+    """the knots table of the reference's reports (report-*/knots.csv): one row per crossing in knot-edge order,
+    knot_edge,strand_a,k_a,strand_b,k_b,over,Fn,Ft,gap,slip,state"""
+    kn = model.knots()
+    lines = ["knot_edge,strand_a,k_a,strand_b,k_b,over,Fn,Ft,gap,slip,state"]
+    k = r["knot"]
+    for c in range(len(kn)):
+        ids = ",".join(str(int(v)) for v in kn[c])
+        vals = ",".join(_fmt(k[name][c]) for name in ("Fn", "Ft", "gap", "slip", "state"))
+        lines.append(f"{ids},{vals}")
+    return "\n".join(lines) + "\n"
+
+
+def segments_csv(model, r):
+    """the segments table of the reference's reports (report-*/segments.csv): one row per segment, strand by strand,
+    strand,segment,Mx,My,T,N"""
+    md = model.md
+    soff = np.ctypeslib.as_array(md.seg_offset, shape=(md.S + 1,))
+    seg = r["segment"]
+    lines = ["strand,segment,Mx,My,T,N"]
+    for s in range(md.S):
+        for j in range(soff[s], soff[s + 1]):
+            lines.append(f"{s},{j - soff[s]}," + ",".join(_fmt(seg[name][j]) for name in ("Mx", "My", "T", "N")))
+    return "\n".join(lines) + "\n"
