@@ -166,6 +166,10 @@ typedef struct ws_state {
   double *phi;              /* [2K] (out) */
   double *mid;              /* [3K] crossing mid points (out) */
   int32_t *ctype, *face_side; /* [K] (out) */
+  /* the ground (floor_on; Floor.F / state / energy / FnMax, filled by download, solver_gpu.js:454-461) */
+  double *floor_F;          /* [3n] the ground's force on every node, world axes (out) */
+  uint8_t *floor_state;     /* [n] 0 clear of the ground, 1 stick, 2 slip (out) */
+  double e_floor, fn_floor; /* the ground's energy and largest normal force of the last iteration (out) */
 } ws_state;
 
 /* Per-instance control record, the reference's CT struct as readControl returns it (solver_gpu.js:374-386). */
@@ -174,6 +178,7 @@ typedef struct ws_control {
                                                                              finished 2; schedule values recorded */
   double ke_prev, ke, resid, fref, e_stretch, e_bend, e_tors, e_cont, e_shear, res_t, res_f;
   double res_rb, rv[3], rw[3];   /* rigid-body freedoms (ground only): residual part, translation / rotation per iteration */
+  double e_floor, fn_floor;      /* the ground's energy and largest normal force (eFloor, fnFloor) */
 } ws_control;
 
 typedef struct ws_batch_opts {
@@ -237,8 +242,8 @@ WS_API int ws_internalize(ws_batch* batch, int32_t b);                          
 /* ws_download without device allocation — no counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The same outputs as
  * ws_download, staged in one fixed region of WS_NOALLOC_STAGE_BYTES per GPU that the library holds from
- * ws_ctx_create until the process ends; an instance whose staging needs more (about 72 bytes per node: instances up
- * to ~930k nodes) is refused with WS_ERR_CAPACITY.  Calls that overlap on one GPU share that region and race: one
+ * ws_ctx_create until the process ends; an instance whose staging needs more (about 72 bytes per node, 97 with the
+ * ground: instances up to ~930k nodes, ~690k with the ground) is refused with WS_ERR_CAPACITY.  Calls that overlap on one GPU share that region and race: one
  * host thread per GPU at a time. */
 #define WS_NOALLOC_STAGE_BYTES (64LL << 20)
 WS_API int ws_download_noalloc(ws_batch* batch, int32_t b, ws_state* out);

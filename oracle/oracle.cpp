@@ -877,6 +877,13 @@ struct Oracle {
     put(o.pressure, pressure); put(o.util, util); put(o.Mrot, Mrot); put(o.phi, phi); put(o.mid, mid);
     if (o.ctype) std::copy(ctype.begin(), ctype.end(), o.ctype);
     if (o.face_side) std::copy(faceSide.begin(), faceSide.end(), o.face_side);
+    if (floorOn) {                 // Floor.F / state / energy / FnMax of the last apply
+      put(o.floor_F, fF);
+      if (o.floor_state) std::copy(fState.begin(), fState.end(), o.floor_state);
+      o.e_floor = fEnergy; o.fn_floor = fFnMax;
+    } else {
+      o.e_floor = 0; o.fn_floor = 0;
+    }
   }
 };
 

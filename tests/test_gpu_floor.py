@@ -114,3 +114,69 @@ def test_unanchored_lath_rests_on_the_ground(W, gpu_ctx):
     assert np.abs(o.state()["X"] - b["X"]).max() <= 1e-4 * 100.0
     X = b["X"].reshape(-1, 3)
     assert abs(X[0, 2] - X[-1, 2]) < 1e-4
+
+
+# the ground block of a download (Floor.F / state / energy / FnMax, solver_gpu.js:454-461)
+GROUND_OUT = {
+    "pinned_ramped": dict(anchor=True, tilt=0.0, lift=1.5, ramp_iter=200),   # read inside the ground's ramp too
+    "pinned_pushed_slides": dict(anchor=True, tilt=0.0, lift=1.5, push=12.0, push_dir=(0.0, 1.0, 0.0)),
+    "unanchored_pushed_slides": dict(push=6.0),
+}
+
+
+@pytest.mark.parametrize("case", sorted(GROUND_OUT))
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_ground_block_of_a_download_matches_the_oracle(W, gpu_ctx, case, precision):
+    """the ground's force on every node, its stick / slip state, energy and largest Fn after the same iterations as the
+    oracle's Floor, inside the ramp and past it.  The force bound follows from the measured position error: ten times
+    (k_n + k_t) times it; the contact column of the history holds the ground's energy (solver_cpu.js:828)"""
+    m = cases.ground_lath(**GROUND_OUT[case])
+    kn, kt = m.md.par.strength, m.md.par.stick_strength
+    o = Oracle(m)
+    g = W.GpuSolver([m], precision=precision)
+    for n in (60, 150, 1500):
+        o.step(n - o.iteration)
+        c = g.step(n - g.read_control()[0]["iter"])[0]
+        a, b = o.state(), g.state(0)
+        dX = float(np.abs(a["X"] - b["X"]).max())
+        Fmax = float(np.abs(a["floor_F"]).max())
+        bound = 10.0 * (kn + kt) * dX + 1e-12 * Fmax
+        dF = float(np.abs(a["floor_F"] - b["floor_F"]).max())
+        print(f"\n{case} {precision} it {n}: |dX| {dX:.2e}, |dF| {dF:.2e} of {Fmax:.3e} (bound {bound:.2e}), "
+              f"on the ground {int((a['floor_state'] > 0).sum())}, slipping {int((a['floor_state'] == 2).sum())}")
+        assert dF <= bound, (n, dF, bound)
+        mism = int((a["floor_state"] != b["floor_state"]).sum())
+        assert mism <= (0 if precision == "fp64" else 1), (n, mism)
+        assert abs(b.st.fn_floor - a.st.fn_floor) <= 10.0 * kn * dX + 1e-12 * a.st.fn_floor
+        assert abs(b.st.e_floor - a.st.e_floor) <= 10.0 * m.n * Fmax * dX + 1e-9 * a.st.e_floor
+        assert b.st.e_floor == c["e_floor"] and b.st.fn_floor == c["fn_floor"]
+        h = g.read_history(0)[-1]
+        assert h[8] == c["iter"] and h[6] == c["e_cont"] + c["e_floor"]   # the lath may stop at its tolerance first
+    assert (b["floor_state"] > 0).sum() > 0
+    if "slides" in case:
+        assert (b["floor_state"] == 2).sum() > 0
+    g.close()
+
+
+def test_ground_block_in_the_results(W, gpu_ctx):
+    """buildResults' floor block and the export's floor section (solver_cpu.js:914-915; app.js:377-382) from a
+    download: the summary counts and resultant follow the per-node forces and states"""
+    import importlib
+    R = importlib.import_module("weave-sim_amd.results")
+    m = cases.ground_lath(anchor=True, tilt=0.0, lift=1.5, push=12.0, push_dir=(0.0, 1.0, 0.0))
+    g = W.GpuSolver([m], precision="fp64")
+    g.step(1500)
+    s = g.state(0)
+    g.close()
+    r = R.build_results(m, s)
+    fl = r["floor"]
+    on = s["floor_state"] > 0
+    assert fl["nodes"] == int(on.sum()) > 0 and fl["slipping"] == int((s["floor_state"] == 2).sum())
+    assert np.allclose(fl["sum"], s["floor_F"].reshape(-1, 3)[on].sum(0), rtol=1e-12, atol=1e-12)
+    assert fl["FnMax"] == s.st.fn_floor == s["floor_F"].reshape(-1, 3)[:, 2].max()
+    assert r["energies"]["floor"] == s.st.e_floor > 0
+    doc = R.results_json(m, r, s)
+    assert doc["floor"]["nodes"] == fl["nodes"] and len(doc["floor"]["contacts"]) == fl["nodes"]
+    first = doc["floor"]["contacts"][0]
+    i = int(np.flatnonzero(on)[0])
+    assert first["F"] == s["floor_F"][3 * i:3 * i + 3].tolist() and first["sliding"] == bool(s["floor_state"][i] == 2)

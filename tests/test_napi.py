@@ -139,6 +139,41 @@ def test_js_build_results_match_python(W, gpu_ctx):
     assert d["doc"]["segment"]["My"] == doc["segment"]["My"] and d["doc"]["reactions"].keys() == doc["reactions"].keys()
 
 
+@pytest.mark.gpu
+def test_js_ground_block_matches_python(W, gpu_ctx):
+    """the 3 m dome on a ground just above its anchors (22 nodes rest on it): the download's ground arrays (floorF, floorState), its energy
+    and largest Fn, buildResults' floor block and resultsJSON's floor section in Node equal the Python path's"""
+    import importlib
+    R = importlib.import_module("weave-sim_amd.results")
+    out = run_node(f"""
+(async () => {{
+  const dev = await W.getDevice();
+  const sc = Object.assign({SCENE}, {{ floor_on: 1, floor_z: 63, floor_mu: 0.4 }});
+  const m = W.Model.hexDome({{span:3000,spacing:300,segment:50,contraction:0.15,noise:1,seed:1}}, sc);
+  const g = new W.GpuSolver(dev, [m], {{ perSubmit: 64, tile: 64 }});
+  await g.step(300);
+  await g.download(0);
+  g.dispose();
+  const r = W.buildResults(m);
+  const doc = W.resultsJSON(m, r);
+  console.log(JSON.stringify({{ F: Array.from(r.floor.F), state: Array.from(r.floor.state), nodes: r.floor.nodes,
+    slipping: r.floor.slipping, sum: r.floor.sum, FnMax: r.floor.FnMax, e: r.energies.floor, doc: doc.floor }}));
+}})().catch((e) => {{ console.error(e); process.exit(1); }});""")
+    d = json.loads(out)
+    py = cases.config1(floor_on=1, floor_z=63.0, floor_mu=0.4)
+    g = W.GpuSolver([py], precision="fp32", tile=64, per_submit=64)
+    g.step(300)
+    s = g.state(0)
+    g.close()
+    r = R.build_results(py, s)
+    fl = r["floor"]
+    assert fl["nodes"] > 0 and d["nodes"] == fl["nodes"] and d["slipping"] == fl["slipping"]
+    assert np.array_equal(np.array(d["F"]), fl["F"]) and np.array_equal(np.array(d["state"]), fl["state"])
+    assert d["sum"] == fl["sum"] and d["FnMax"] == fl["FnMax"] and d["e"] == r["energies"]["floor"]
+    doc = json.loads(json.dumps(R.results_json(py, r, s)))
+    assert d["doc"] == doc["floor"]
+
+
 def test_js_scene_reader_matches_python():
     """the reference's example scenes read by the JS reader (scene_mi355x.js: scene.json, STL) and by the Python one
     build the same model: sizes, initial state, anchors, loads and solver parameters, bit for bit"""

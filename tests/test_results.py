@@ -45,7 +45,7 @@ def test_results_json_document():
     doc = json.loads(json.dumps(R.results_json(m, r)))
     assert doc["converged"] and set(doc["segment"]) == set(R.SEG_LAYERS)
     assert len(doc["segment"]["Mx"]) == 1 and len(doc["segment"]["Mx"][0]) == 10
-    assert len(doc["disp"][0]) == 11 and doc["reactions"]["0,0"][2] > 0
+    assert len(doc["node"]["disp"][0]) == 11 and doc["reactions"]["0,0"][2] > 0
     c = cases.crossing(**cases.CROSS_CASES["cross_stick"])
     o = Oracle(c)
     o.step(500)
@@ -94,3 +94,41 @@ def test_knots_and_segments_tables_have_the_report_layout():
         assert np.array_equal(seg[:, col], r["segment"][name]), name
     doc = R.results_json(m, r)
     assert list(doc["knot"]["Fn"])[:3] == [str(int(e)) for e in kn[:3, 0]]
+
+
+def test_ground_block_follows_the_floor_laws():
+    """buildResults' floor block (solver_cpu.js:914-915) over the oracle's pinned lath pushed sideways above μ·W
+    (floor_check.mjs: every free node slides, on the friction cone |F_t| = μ F_n), and the export's floor section
+    (app.js:377-382): the resultant is the sum of the resting nodes' forces, each contact names its strand and node"""
+    m = cases.ground_lath(anchor=True, tilt=0.0, lift=1.5, push=12.0, push_dir=(0.0, 1.0, 0.0))
+    o = Oracle(m)
+    o.step(2000)
+    st = o.state()
+    r = R.build_results(m, st)
+    fl = r["floor"]
+    F = fl["F"].reshape(-1, 3)
+    assert fl["nodes"] == m.n and fl["slipping"] == m.n - 1     # the pinned node rests without sliding
+    slid = fl["state"] == 2
+    ft, fn = np.hypot(F[slid, 0], F[slid, 1]), F[slid, 2]
+    assert np.allclose(ft, m.md.par.floor_mu * fn, rtol=1e-9)
+    assert np.allclose(fl["sum"], F[fl["state"] > 0].sum(0), rtol=1e-12) and fl["FnMax"] == F[:, 2].max()
+    assert r["energies"]["floor"] == st.st.e_floor > 0
+    assert r["energies"]["strain"] == r["energies"]["beam"] + r["energies"]["contact"] + st.st.e_floor
+    doc = json.loads(json.dumps(R.results_json(m, r, st)))
+    f = doc["floor"]
+    assert (f["z"], f["mu"], f["nodes"], f["sliding"]) == (0.0, m.md.par.floor_mu, fl["nodes"], fl["slipping"])
+    assert len(f["contacts"]) == fl["nodes"] and all(c["strand"] == 0 for c in f["contacts"])
+    assert [c["sliding"] for c in f["contacts"]] == [False] + [True] * (m.n - 1)
+    assert [c["node"] for c in f["contacts"]] == [int(i) for i in np.flatnonzero(fl["state"])]
+    # without a ground the block is absent
+    m2, st2 = converged_cantilever()
+    r2 = R.build_results(m2, st2)
+    assert r2["floor"] is None and "floor" not in R.results_json(m2, r2) and r2["energies"]["floor"] == 0.0
+
+
+def test_export_rounds_points_as_javascript_to_fixed():
+    """+v.toFixed(4) (values printed by Node): the nearest multiple of 1e-4 to the binary value, exact ties away
+    from zero"""
+    assert R._fixed4(1.03125) == 1.0313 and R._fixed4(-1.03125) == -1.0313 and R._fixed4(-2.00015) == -2.0002
+    assert R._fixed4(0.12344999) == 0.1234 and R._fixed4(2.5) == 2.5 and R._fixed4(-0.00004) == 0.0
+    assert R._fixed4(1.00005) == 1.0001 and R._fixed4(123.45678) == 123.4568

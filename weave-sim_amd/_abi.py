@@ -51,7 +51,8 @@ STATE_ARRAYS = [
     ("N", P_d, "m"), ("Vsh", P_d, "2m"), ("M", P_d, "3p"), ("cM", P_d, "2C"), ("reactions", P_d, "3n"),
     ("Fn", P_d, "K"), ("Ft", P_d, "K"), ("gap", P_d, "K"), ("kstate", P_d, "K"), ("theta", P_d, "K"),
     ("Anom", P_d, "K"), ("pressure", P_d, "K"), ("util", P_d, "K"), ("Mrot", P_d, "K"), ("phi", P_d, "2K"),
-    ("mid", P_d, "3K"), ("ctype", P_i32, "K"), ("face_side", P_i32, "K"),
+    ("mid", P_d, "3K"), ("ctype", P_i32, "K"), ("face_side", P_i32, "K"), ("floor_F", P_d, "3n"),
+    ("floor_state", P_u8, "n"),
 ]
 
 
@@ -59,14 +60,14 @@ class ws_state(C.Structure):
     _fields_ = [(n, i32) for n in ("iteration", "converged", "pen_n", "pad0")] + \
                [(n, d) for n in ("ke_prev", "residual", "res_force", "res_torque", "fref", "ke", "e_stretch",
                                  "e_shear", "e_bend", "e_twist", "e_contact", "res_rigid")] + \
-               [(n, t) for (n, t, _) in STATE_ARRAYS]
+               [(n, t) for (n, t, _) in STATE_ARRAYS] + [("e_floor", d), ("fn_floor", d)]
 
 
 class ws_control(C.Structure):
     _fields_ = [(n, i32) for n in ("iter", "done", "rollback", "al_update", "stop_at", "hist_n", "phase", "inc")] + \
                [(n, d) for n in ("ke_prev", "ke", "resid", "fref", "e_stretch", "e_bend", "e_tors", "e_cont",
                                  "e_shear", "res_t", "res_f", "res_rb")] + \
-               [("rv", d * 3), ("rw", d * 3)]
+               [("rv", d * 3), ("rw", d * 3), ("e_floor", d), ("fn_floor", d)]
 
 
 class ws_batch_opts(C.Structure):

@@ -214,7 +214,7 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   A.add(d.wx, m); A.add(d.wy, m); A.add(d.wz, m);
   A.addc(d.l0, m); A.addc(d.EA, m); A.addc(d.GA, m); A.addc(d.EIx, m); A.addc(d.EIy, m); A.addc(d.GJ, m);
   A.addc(d.krx, m); A.addc(d.kry, m); A.addc(d.krz, m);
-  if (d.floor_on) { A.addc(d.nodew, n); A.addc(d.nodeh, n); A.add(d.fax, n); A.add(d.fay, n); }
+  if (d.floor_on) { A.addc(d.nodew, n); A.addc(d.nodeh, n); A.add(d.fax, n); A.add(d.fay, n); A.add(d.gstate, n); }
   A.addc(d.jlen, p); A.addc(d.om0, 3 * p);
   if (C) { A.addc(d.c_len, C); A.addc(d.c_B, 3 * C); A.addc(d.c_Q, 4 * C); }
   A.addc(d.k_seg_off, 2 * K); A.addc(d.k_nseg, 2 * K); A.addc(d.k_noff, 2 * K); A.addc(d.k_nn, 2 * K); A.addc(d.wk, 2 * K); A.addc(d.hk, 2 * K);
@@ -728,6 +728,7 @@ struct Batch : BatchBase {
     o.res_rb = c.res_rb;
     for (int k = 0; k < 3; k++) { o.rv[k] = c.rv[k]; o.rw[k] = c.rw[k]; }
     o.phase = c.phase; o.inc = c.inc;
+    o.e_floor = c.e_floor; o.fn_floor = c.fn_floor;
     return o;
   }
 
@@ -860,9 +861,11 @@ struct Batch : BatchBase {
   // clamp moments [2C] and reactions [3n], each 256-byte aligned
   size_t download_stage_bytes(int b) override {
     const InstHost<Real>& H = at(b);
-    return align256(sizeof(double) * std::max(H.K, 1)) + align256(sizeof(double) * std::max(H.m, 1)) +
-           align256(sizeof(double) * std::max(2 * H.m, 1)) + align256(sizeof(double) * std::max(3 * H.p, 1)) +
-           align256(sizeof(double) * std::max(2 * H.C, 1)) + align256(sizeof(double) * std::max(3 * H.n, 1));
+    size_t s = align256(sizeof(double) * std::max(H.K, 1)) + align256(sizeof(double) * std::max(H.m, 1)) +
+               align256(sizeof(double) * std::max(2 * H.m, 1)) + align256(sizeof(double) * std::max(3 * H.p, 1)) +
+               align256(sizeof(double) * std::max(2 * H.C, 1)) + align256(sizeof(double) * std::max(3 * H.n, 1));
+    if (H.d.floor_on) s += align256(sizeof(double) * std::max(3 * H.n, 1)) + align256(sizeof(double) * std::max((H.n + 7) / 8, 1));
+    return s;
   }
 
   int download(int b, ws_state& o, char* stage = nullptr, size_t stage_cap = 0) override {
@@ -936,41 +939,52 @@ struct Batch : BatchBase {
         if (o.face_side) for (int q = 0; q < K; q++) o.face_side[q] = fs[q];
       }
     }
-    // section forces, moments and reactions of the last evaluated state
-    if (o.N || o.Vsh || o.M || o.cM || o.reactions) {
-      double *dN = nullptr, *dV = nullptr, *dM = nullptr, *dC = nullptr, *dR = nullptr;
+    // section forces, moments, reactions and the ground's forces of the last evaluated state
+    const bool ground = d.floor_on && (o.floor_F || o.floor_state);
+    if (o.N || o.Vsh || o.M || o.cM || o.reactions || ground) {
+      double *dN = nullptr, *dV = nullptr, *dM = nullptr, *dC = nullptr, *dR = nullptr, *dG = nullptr;
+      uint8_t* dGS = nullptr;
+      const size_t bN = sizeof(double) * std::max(m, 1), bV = sizeof(double) * std::max(2 * m, 1), bM = sizeof(double) * std::max(3 * p, 1),
+                   bC = sizeof(double) * std::max(2 * C, 1), bR = sizeof(double) * std::max(3 * n, 1), bGS = size_t(std::max(n, 1));
       if (stage) {
         // the staging region keeps the last call's values: clear the outputs (k_outputs leaves entries untouched)
         dN = carve(m); dV = carve(2 * m); dM = carve(3 * p); dC = carve(2 * C); dR = carve(3 * n);
-        HIPCHK(hipMemsetAsync(dN, 0, sizeof(double) * std::max(m, 1), stream)); HIPCHK(hipMemsetAsync(dV, 0, sizeof(double) * std::max(2 * m, 1), stream));
-        HIPCHK(hipMemsetAsync(dM, 0, sizeof(double) * std::max(3 * p, 1), stream)); HIPCHK(hipMemsetAsync(dC, 0, sizeof(double) * std::max(2 * C, 1), stream));
-        HIPCHK(hipMemsetAsync(dR, 0, sizeof(double) * std::max(3 * n, 1), stream));
-        HIPCHK(hipStreamSynchronize(stream));
+        if (ground) { dG = carve(3 * n); dGS = reinterpret_cast<uint8_t*>(carve((n + 7) / 8)); }
       } else {
-        HIPCHK(hipMalloc(&dN, sizeof(double) * std::max(m, 1))); HIPCHK(hipMalloc(&dV, sizeof(double) * std::max(2 * m, 1)));
-        HIPCHK(hipMalloc(&dM, sizeof(double) * std::max(3 * p, 1))); HIPCHK(hipMalloc(&dC, sizeof(double) * std::max(2 * C, 1)));
-        HIPCHK(hipMalloc(&dR, sizeof(double) * std::max(3 * n, 1)));
-        HIPCHK(hipMemset(dN, 0, sizeof(double) * std::max(m, 1))); HIPCHK(hipMemset(dV, 0, sizeof(double) * std::max(2 * m, 1)));
-        HIPCHK(hipMemset(dM, 0, sizeof(double) * std::max(3 * p, 1))); HIPCHK(hipMemset(dC, 0, sizeof(double) * std::max(2 * C, 1)));
-        HIPCHK(hipMemset(dR, 0, sizeof(double) * std::max(3 * n, 1)));
+        HIPCHK(hipMalloc(&dN, bN)); HIPCHK(hipMalloc(&dV, bV)); HIPCHK(hipMalloc(&dM, bM)); HIPCHK(hipMalloc(&dC, bC));
+        HIPCHK(hipMalloc(&dR, bR));
+        if (ground) { HIPCHK(hipMalloc(&dG, bR)); HIPCHK(hipMalloc(&dGS, bGS)); }
       }
+      HIPCHK(hipMemsetAsync(dN, 0, bN, stream)); HIPCHK(hipMemsetAsync(dV, 0, bV, stream)); HIPCHK(hipMemsetAsync(dM, 0, bM, stream));
+      HIPCHK(hipMemsetAsync(dC, 0, bC, stream)); HIPCHK(hipMemsetAsync(dR, 0, bR, stream));
+      if (ground) { HIPCHK(hipMemsetAsync(dG, 0, bR, stream)); HIPCHK(hipMemsetAsync(dGS, 0, bGS, stream)); }
+      HIPCHK(hipStreamSynchronize(stream));
       if (c.live_flag) {
+        const OutPtrs op{dN, dV, dM, dC, dR, nullptr, nullptr, nullptr, dG, dGS};
         switch (tile) {
-          case 64: hipLaunchKernelGGL((k_outputs<Real, 64>), dim3(d.ntiles), dim3(64), 0, stream, d_inst, d_tiles, b, OutPtrs{dN, dV, dM, dC, dR}); break;
-          case 128: hipLaunchKernelGGL((k_outputs<Real, 128>), dim3(d.ntiles), dim3(128), 0, stream, d_inst, d_tiles, b, OutPtrs{dN, dV, dM, dC, dR}); break;
-          default: hipLaunchKernelGGL((k_outputs<Real, 256>), dim3(d.ntiles), dim3(256), 0, stream, d_inst, d_tiles, b, OutPtrs{dN, dV, dM, dC, dR}); break;
+          case 64: hipLaunchKernelGGL((k_outputs<Real, 64>), dim3(d.ntiles), dim3(64), 0, stream, d_inst, d_tiles, b, op); break;
+          case 128: hipLaunchKernelGGL((k_outputs<Real, 128>), dim3(d.ntiles), dim3(128), 0, stream, d_inst, d_tiles, b, op); break;
+          default: hipLaunchKernelGGL((k_outputs<Real, 256>), dim3(d.ntiles), dim3(256), 0, stream, d_inst, d_tiles, b, op); break;
         }
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream));
       }
       auto cp = [](double* dst, double* src, size_t nn) { if (dst && nn) HIPCHK(hipMemcpy(dst, src, nn * sizeof(double), hipMemcpyDeviceToHost)); };
       cp(o.N, dN, m); cp(o.Vsh, dV, 2 * m); cp(o.M, dM, 3 * p); cp(o.cM, dC, 2 * C); cp(o.reactions, dR, 3 * n);
-      if (!stage) { (void)hipFree(dN); (void)hipFree(dV); (void)hipFree(dM); (void)hipFree(dC); (void)hipFree(dR); }
+      if (ground) {
+        cp(o.floor_F, dG, 3 * n);
+        if (o.floor_state && n) HIPCHK(hipMemcpy(o.floor_state, dGS, n, hipMemcpyDeviceToHost));
+      }
+      if (!stage) {
+        (void)hipFree(dN); (void)hipFree(dV); (void)hipFree(dM); (void)hipFree(dC); (void)hipFree(dR);
+        if (ground) { (void)hipFree(dG); (void)hipFree(dGS); }
+      }
     }
     o.iteration = c.iter; o.ke_prev = c.ke_prev; o.residual = c.resid; o.pen_n = (c.al_update && d.aug && K) ? 0 : c.pen_n;
     o.converged = (c.done == 1 && c.resid <= H.tol) ? 1 : 0;
     o.res_force = c.res_f; o.res_torque = c.res_t; o.fref = c.fref; o.ke = c.ke; o.res_rigid = c.res_rb;
     o.e_stretch = c.e_stretch; o.e_shear = c.e_shear; o.e_bend = c.e_bend; o.e_twist = c.e_tors; o.e_contact = c.e_cont;
+    o.e_floor = d.floor_on ? c.e_floor : 0.0; o.fn_floor = d.floor_on ? c.fn_floor : 0.0;
     return WS_OK;
   }
 

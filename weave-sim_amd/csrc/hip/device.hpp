@@ -13,8 +13,8 @@
 namespace wsd {
 
 constexpr int HIST_STRIDE = 9;   // history ring entry: ke, strain, bending, torsion, stretch, shear, contact, residual, iteration
-constexpr int SPART = 9;         // doubles per strand-tile partial
-constexpr int SPART_STRIDE = 10; // stored tile-major, padded to 80 bytes: five 16-byte loads per tile
+constexpr int SPART = 9;         // doubles per strand-tile partial (a tenth, the largest ground Fn, when the ground is on)
+constexpr int SPART_STRIDE = 10; // stored tile-major, 80 bytes: five 16-byte loads per tile
 constexpr int CPART = 2;         // doubles per crossing-block partial (stored value-major)
 constexpr int CTRL_THREADS = 256;  // lanes of the control kernel's workgroup
 constexpr int CTRL_PER = 8;         // strand partials per lane per round (all loads of a round in flight together)
@@ -24,7 +24,7 @@ constexpr int CTRL_PER = 8;         // strand partials per lane per round (all l
 constexpr int CTRL_ONE = 2048;
 constexpr int CTRL_CHUNK = 1024;
 constexpr int CTRL_STAGE = 64;    // chunk partials the combining workgroup stages in LDS at a time
-constexpr int RPART = 11;         // doubles per control-chunk partial
+constexpr int RPART = 12;         // doubles per control-chunk partial
 constexpr int FUSED_CHUNK = 256;  // strand tiles per chunk when the strand launch runs the control step (four partials
                                   // per lane of a 64-lane tile: one memory latency for the chunk's reduction)
 // rigid-body sums per strand tile: resultant of loads + ground (3), its moment about the centroid (3), Σ d² of the
@@ -55,6 +55,7 @@ struct Ctl {
   // Σ reaction along the axis over the averaging window
   int32_t phase, it_phase, inc, acc_n;
   double pf[3], cur_f, ad[3], ar;
+  double e_floor, fn_floor;   // the ground's energy and largest normal force of the last iteration (Floor.energy, FnMax)
 };
 
 // one crossing side's contribution of this iteration, written by the crossing kernel, gathered per strand tile
@@ -114,6 +115,7 @@ struct Inst {
   // ground (floor_on): node section width / height, stick points (f64)
   const Real* nodew; const Real* nodeh;
   double* fax; double* fay;
+  uint8_t* gstate;           // [n] the ground's state of the last iteration per node: 0 clear, 1 stick, 2 slip
   // joints
   const Real* jlen; const Real* om0;                          // om0 [3p]
   // clamps
@@ -154,9 +156,9 @@ struct Inst {
 };
 
 // output arrays of the strand tile in output mode (downloads, evaluations): N [m], V [2m], M [3p], clamp moments
-// [2C], reactions [3n]; for an evaluation also the node forces F [3n], body-axis torques T [3m] and frame inertia
-// I3 [3m] (null otherwise)
-struct OutPtrs { double *N, *V, *M, *CM, *R, *F, *T, *I3; };
+// [2C], reactions [3n], the ground's force on every node G [3n] and its state GS [n] (ground on); for an evaluation
+// also the node forces F [3n], body-axis torques T [3m] and frame inertia I3 [3m] (null otherwise)
+struct OutPtrs { double *N, *V, *M, *CM, *R, *F, *T, *I3, *G; uint8_t* GS; };
 
 // one strand tile: strand-local node range [k0, k1) plus the strand's layout, so that a tile reads its geometry in
 // one 64-byte load

@@ -740,6 +740,7 @@ struct StrandLDS {
   int wcnt[TILE / 64];
   Real bnd[TILE / 64][9];            // last lane of each wavefront: segment force, second-segment torque, kDyn
   double rbw[RBP][TILE / 64];        // rigid-body sums per wavefront (rigid instances)
+  double gmx[TILE / 64];             // largest ground Fn per wavefront (ground on, tiles of several wavefronts)
 };
 
 // the value of lane − 1 (DPP wave_shr:1; lane 0 reads 0)
@@ -1080,8 +1081,11 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       // the ground (Floor.apply, solver_cpu.js:583-627): penalty against the plane under the section's lowest
       // point, Coulomb friction against a stick point that follows a sliding node.  In f64 (positions, stick
       // points); the output pass recomputes the force from the moved stick point, which gives the same force.
+      // a download evaluates the state the last iteration evaluated, at that iteration's ramp (Floor.rampAt(iter − 1),
+      // solver_gpu.js:462)
       const int rampN = max(I.ramp_iter, 0);
-      const double rmp = rampN <= 0 ? 1.0 : fmin(1.0, 0.01 + 0.99 * iter / rampN);
+      const int itg = (OUT && !out.F) ? max(iter - 1, 0) : iter;
+      const double rmp = rampN <= 0 ? 1.0 : fmin(1.0, 0.01 + 0.99 * itg / rampN);
       const double kn = rmp * I.strength, kt = rmp * I.stick;
       const double pen = (I.floor_z - L.Xz[t]) + (0.5 * double(gl(I.nodew)[i]) * fabs(Rn[2]) + 0.5 * double(gl(I.nodeh)[i]) * fabs(Rn[5]));
       double fg[3] = {0, 0, 0};
@@ -1100,12 +1104,19 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
         const double scale = Ftv > limit ? 1.0 - limit / fmax(Ftv, 1e-12) : 0.0;
         const double dax = sx * scale, day = sy * scale;
         if (!OUT && scale > 0) { gl(I.fax)[i] = ax + dax; gl(I.fay)[i] = ay + day; }
+        if (!OUT) gl(I.gstate)[i] = scale > 0 ? 2 : 1;
         const double ex = sx - dax, ey = sy - day;
         fg[0] = -kt * ex; fg[1] = -kt * ey; fg[2] = fn;
         eG = 0.5 * kn * pen * pen + 0.5 * kt * (ex * ex + ey * ey);
         mxG = fn;
       } else if (!OUT) {
         gl(I.fax)[i] = L.Xx[t]; gl(I.fay)[i] = L.Xy[t];
+        gl(I.gstate)[i] = 0;
+      }
+      if (OUT && out.G) {
+        // the stick point has moved with a slide already, so the state is the one the iteration recorded
+        out.G[3 * i] = fg[0]; out.G[3 * i + 1] = fg[1]; out.G[3 * i + 2] = fg[2];
+        out.GS[i] = pen > 0 ? gl(I.gstate)[i] : 0;
       }
       for (int c = 0; c < 3; c++) F[c] += Real(fg[c]);
       if (RB && I.rigid && !OUT) {
@@ -1264,6 +1275,21 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       if (lane == 0) gl(I.spart)[size_t(tl.tile_local) * SPART_STRIDE + 8] = g;
     }
   }
+  if (I.floor_on) {
+    // the tile's largest ground Fn (an exact maximum, so any order): each wavefront's, then the wavefronts'
+    const double g = wave_reduce63<false>(mxG);
+    if (TILE == 64) {
+      if (lane == 63) gl(I.spart)[size_t(tl.tile_local) * SPART_STRIDE + 9] = g;
+    } else {
+      if (lane == 63) L.gmx[wv] = g;
+      __syncthreads();
+      if (tid == 0) {
+        double x = L.gmx[0];
+        for (int w = 1; w < TILE / 64; w++) x = fmax(x, L.gmx[w]);
+        gl(I.spart)[size_t(tl.tile_local) * SPART_STRIDE + 9] = x;
+      }
+    }
+  }
   if (RB && I.rigid) {
     // the tile's rigid-body sums: each wavefront's in a fixed DPP order, then the wavefronts in order
 #pragma unroll
@@ -1303,7 +1329,8 @@ __device__ __forceinline__ void block_reduce(double (*red)[TILE / 64], double* v
 
 // The control step of one instance after its reductions (lane 0; solver_cpu.js:809-837; wgsl.js:990-1047): acc
 // holds [0] Σm|v|² + ΣI3ω², [1..4] stretch / shear / bending / torsion energy, [5] ground energy, [6] contact energy
-// (sums), [7] max |F_free|², [8] max |t_m|², [9] max(|load|, |N|, |M|/l̄_min, ground Fn), [10] max Fn (maxima); rbs the
+// (sums), [7] max |F_free|², [8] max |t_m|², [9] max(|load|, |N|, |M|/l̄_min, ground Fn), [10] max Fn, [11] max
+// ground Fn (maxima); rbs the
 // instance's rigid-body sums.  c is the control record as the iteration found it; the new record is written to ctl.
 template <typename Real>
 __device__ void control_step(const Inst<Real>& I, Ctl* ctl, Ctl& c, const double* acc, const double* rbs) {
@@ -1371,6 +1398,7 @@ __device__ void control_step(const Inst<Real>& I, Ctl* ctl, Ctl& c, const double
   if (!(c.iter > I.warmup)) resid = INFINITY;
   c.resid = resid; c.fref = fref; c.res_f = rf; c.res_t = rt;
   c.e_stretch = acc[1]; c.e_shear = acc[2]; c.e_bend = acc[3]; c.e_tors = acc[4]; c.e_cont = eCont;
+  c.e_floor = eGround; c.fn_floor = acc[11];
   const bool conv = resid <= I.tol;
   if (I.proto == 0 && conv) c.done = 1;
   if (!rollback) c.cur = c.cur ^ 1;
@@ -1414,7 +1442,7 @@ __device__ void control_step(const Inst<Real>& I, Ctl* ctl, Ctl& c, const double
   if (I.hist_len > 0) {
     const auto h = gl(I.hist) + size_t((c.iter - 1) % I.hist_len) * HIST_STRIDE;
     h[0] = ke; h[1] = acc[1] + acc[2] + acc[3] + acc[4] + eCont + eGround; h[2] = acc[3]; h[3] = acc[4];
-    h[4] = acc[1]; h[5] = acc[2]; h[6] = eCont; h[7] = resid; h[8] = double(c.iter);
+    h[4] = acc[1]; h[5] = acc[2]; h[6] = eCont + eGround; h[7] = resid; h[8] = double(c.iter);
   }
   c.hist_n += 1;
   *ctl = c;
@@ -1450,7 +1478,7 @@ __device__ void strand_control_tail(const Inst<Real>& I, Ctl* ctl, const TileIte
 #pragma unroll
   for (int q = 0; q < NACC; q++) acc[q] = 0.0;
   for (int w0 = p0; w0 < p1; w0 += TILE * FPER) {
-    double x[FPER][SPART];
+    double x[FPER][SPART_STRIDE];
 #pragma unroll
     for (int k = 0; k < FPER; k++) {
       const int w = w0 + k * TILE + tid;
@@ -1459,6 +1487,7 @@ __device__ void strand_control_tail(const Inst<Real>& I, Ctl* ctl, const TileIte
 #pragma unroll
       for (int q = 0; q < SPART - 1; q++) x[k][q] = in ? ld(sp + q) : 0.0;
       x[k][SPART - 1] = (in && fon) ? ld(sp + SPART - 1) : 0.0;
+      x[k][SPART] = (in && fon) ? ld(sp + SPART) : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < FPER; k++) {
@@ -1467,6 +1496,7 @@ __device__ void strand_control_tail(const Inst<Real>& I, Ctl* ctl, const TileIte
 #pragma unroll
       for (int q = 5; q < 8; q++) acc[q + 2] = fmax(acc[q + 2], x[k][q]);
       acc[5] += x[k][8];
+      acc[11] = fmax(acc[11], x[k][9]);
     }
   }
   // the chunk's share of the crossing-block partials (written by the crossing launch before this one)
@@ -1528,7 +1558,7 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
                                                       int32_t nitems, Ctl* __restrict__ ctls) {
   // accumulator: [0] Σm|v|² + ΣI3ω², [1..4] stretch / shear / bending / torsion energy, [5] ground energy,
   // [6] contact energy (sums); [7] max |F_free|², [8] max |t_m|², [9] max(|load|, |N|, |M|/l̄_min, ground Fn),
-  // [10] max Fn (maxima)
+  // [10] max Fn, [11] max ground Fn (maxima)
   constexpr int NACC = RPART, NSUM = 7;
   __shared__ double red[NACC][CTRL_THREADS / 64];
   __shared__ double stage[CTRL_STAGE * RPART];
@@ -1563,12 +1593,12 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   const double ce0 = cin ? ce0v : 0.0, cm0 = cin ? cm0v : 0.0;
   const bool fon = I.floor_on != 0;
   for (int w0 = p0; w0 < p1; w0 += CTRL_THREADS * CTRL_PER) {
-    double x[CTRL_PER][SPART];
+    double x[CTRL_PER][SPART_STRIDE];
 #pragma unroll
     for (int k = 0; k < CTRL_PER; k++) {
       const int w = w0 + k * CTRL_THREADS + tid;
       const bool in = w < p1;
-      // one tile's record: five 16-byte loads (the ninth value is the ground's energy, the tenth padding)
+      // one tile's record: five 16-byte loads (the ninth value is the ground's energy, the tenth its largest Fn)
       typedef double dvec2 __attribute__((ext_vector_type(2)));
       const auto sp = gl(reinterpret_cast<const dvec2*>(I.spart + size_t(in ? w : p0) * SPART_STRIDE));
       dvec2 v2[SPART_STRIDE / 2];
@@ -1577,6 +1607,7 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
 #pragma unroll
       for (int q = 0; q < SPART - 1; q++) x[k][q] = in ? (q & 1 ? v2[q >> 1].y : v2[q >> 1].x) : 0.0;
       x[k][SPART - 1] = (in && fon) ? v2[(SPART - 1) >> 1].x : 0.0;
+      x[k][SPART] = (in && fon) ? v2[SPART >> 1].y : 0.0;
     }
 #pragma unroll
     for (int k = 0; k < CTRL_PER; k++) {
@@ -1585,6 +1616,7 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
 #pragma unroll
       for (int q = 5; q < 8; q++) acc[q + 2] = fmax(acc[q + 2], x[k][q]);
       acc[5] += x[k][8];
+      acc[11] = fmax(acc[11], x[k][9]);
     }
   }
   if (cin) { acc[6] += ce0; acc[10] = fmax(acc[10], cm0); }

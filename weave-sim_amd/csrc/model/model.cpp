@@ -496,7 +496,8 @@ struct Model {
   std::vector<double> kwk, khk, kfc90, kE90, kHs, kw, kh, nodeW, nodeH;
   // state
   std::vector<double> X, X0, V, Q, Q0, W, u, nPrev, slip, lam, penAcc, theta0, rotSlip, dent, climb, tanpsi, dentE, floorA;
-  std::vector<uint8_t> stuck;
+  std::vector<uint8_t> stuck, floorState;
+  std::vector<double> floorF;
   std::vector<double> N, Vsh, M, cM, reactions, Fn, Ft, gap, kstate, theta, Anom, pressure, util, Mrot, phi, mid;
   std::vector<int32_t> ctype, faceSide;
   int32_t iteration = 0; double kePrev = 0.0, residual = INFINITY;
@@ -936,6 +937,7 @@ std::unique_ptr<Model> buildModel(const Built& tp, const Setup& su) {
   for (int i = 0; i < n; i++) { const wsm_group& gr = su.groupOfStrand(nodeS[i]); md->nodeW[i] = gr.width; md->nodeH[i] = gr.height; }
   md->floorA.assign(2 * n, 0);
   for (int i = 0; i < n; i++) { md->floorA[2 * i] = md->X[3 * i]; md->floorA[2 * i + 1] = md->X[3 * i + 1]; }
+  md->floorF.assign(3 * n, 0); md->floorState.assign(n, 0);
   // outputs
   md->N.assign(m, 0); md->Vsh.assign(2 * m, 0); md->M.assign(3 * p, 0); md->cM.assign(2 * md->C, 0); md->reactions.assign(3 * n, 0);
   md->info.n = n; md->info.m = m; md->info.p = p; md->info.K = K; md->info.C = md->C; md->info.S = S;
@@ -982,6 +984,7 @@ ws_model describe(Model& md, ws_state* st) {
     st->Fn = md.Fn.data(); st->Ft = md.Ft.data(); st->gap = md.gap.data(); st->kstate = md.kstate.data(); st->theta = md.theta.data();
     st->Anom = md.Anom.data(); st->pressure = md.pressure.data(); st->util = md.util.data(); st->Mrot = md.Mrot.data(); st->phi = md.phi.data();
     st->mid = md.mid.data(); st->ctype = md.ctype.data(); st->face_side = md.faceSide.data();
+    st->floor_F = md.floorF.data(); st->floor_state = md.floorState.data();
   }
   return o;
 }

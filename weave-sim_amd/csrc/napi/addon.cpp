@@ -94,6 +94,7 @@ napi_value ctl_object(napi_env env, const ws_control& c) {
   napi_value rv, rw; napi_create_array_with_length(env, 3, &rv); napi_create_array_with_length(env, 3, &rw);
   for (uint32_t k = 0; k < 3; k++) { napi_set_element(env, rv, k, num(env, c.rv[k])); napi_set_element(env, rw, k, num(env, c.rw[k])); }
   set(env, o, "rv", rv); set(env, o, "rw", rw);
+  set(env, o, "eFloor", num(env, c.e_floor)); set(env, o, "fnFloor", num(env, c.fn_floor));
   return o;
 }
 
@@ -315,7 +316,7 @@ napi_value ModelFromFields(napi_env env, napi_callback_info info) {
   BS(N, F64, mm); BS(Vsh, F64, 2 * mm); BS(M, F64, 3 * p); BS(cM, F64, 2 * C); BS(reactions, F64, 3 * n);
   BS(Fn, F64, K); BS(Ft, F64, K); BS(gap, F64, K); BS(kstate, F64, K); BS(theta, F64, K); BS(Anom, F64, K);
   BS(pressure, F64, K); BS(util, F64, K); BS(Mrot, F64, K); BS(phi, F64, 2 * K); BS(mid, F64, 3 * K);
-  BS(ctype, I32, K); BS(face_side, I32, K);
+  BS(ctype, I32, K); BS(face_side, I32, K); BS(floor_F, F64, 3 * n); BS(floor_state, U8, n);
 #undef BM
 #undef BS
   if (bad.empty() && (!st.X || !st.Q || (K && !st.u))) bad = "X, Q (and u with crossings) are needed";
@@ -576,6 +577,7 @@ napi_value ModelInfo(napi_env env, napi_callback_info info) {
   const ws_params& P = m->md.par;
   set(env, o, "contacts", i32(env, P.contacts)); set(env, o, "seatLaw", i32(env, P.seat_law));
   set(env, o, "rotFriction", i32(env, P.rot_friction)); set(env, o, "augmented", i32(env, P.augmented));
+  set(env, o, "floorOn", i32(env, P.floor_on)); set(env, o, "floorZ", num(env, P.floor_z)); set(env, o, "floorMu", num(env, P.floor_mu));
   // strand layout and free freedoms (copies): what the results need to split arrays per strand and find anchors
   auto copy = [&](const char* name, const void* src, size_t count, napi_typedarray_type t, size_t esz) {
     napi_value ab, ta; void* data = nullptr;
@@ -631,6 +633,7 @@ napi_value ModelState(napi_env env, napi_callback_info info) {
   arr("util", s.util, K, napi_float64_array, 8); arr("phi", s.phi, 2 * K, napi_float64_array, 8);
   arr("mid", s.mid, 3 * K, napi_float64_array, 8); arr("ctype", s.ctype, K, napi_int32_array, 4);
   arr("faceSide", s.face_side, K, napi_int32_array, 4);
+  if (md.par.floor_on) { arr("floorF", s.floor_F, 3 * n, napi_float64_array, 8); arr("floorState", s.floor_state, n, napi_uint8_array, 1); }
   set(env, o, "iteration", i32(env, s.iteration)); set(env, o, "residual", num(env, s.residual));
   set(env, o, "converged", i32(env, s.converged));
   return o;
@@ -644,6 +647,7 @@ napi_value scalars(napi_env env, const ws_state& s) {
   set(env, o, "eStretch", num(env, s.e_stretch)); set(env, o, "eShear", num(env, s.e_shear)); set(env, o, "eBend", num(env, s.e_bend));
   set(env, o, "eTwist", num(env, s.e_twist)); set(env, o, "eContact", num(env, s.e_contact));
   set(env, o, "resRigid", num(env, s.res_rigid)); set(env, o, "penN", i32(env, s.pen_n));
+  set(env, o, "eFloor", num(env, s.e_floor)); set(env, o, "fnFloor", num(env, s.fn_floor));
   return o;
 }
 

@@ -157,7 +157,9 @@ function cpuSolverFields(md) {
   if (fl) {
     floorA = new Float64Array(2 * n);
     for (let i = 0; i < n; i++) { floorA[2 * i] = fl.ax[i]; floorA[2 * i + 1] = fl.ay[i]; }
-    Object.assign(f, { node_w: asType(fl.w, Float64Array), node_h: asType(fl.h, Float64Array), floor_a: floorA });
+    Object.assign(f, { node_w: asType(fl.w, Float64Array), node_h: asType(fl.h, Float64Array), floor_a: floorA,
+      floor_F: stateArr(fl.F || new Float64Array(3 * n), Float64Array),
+      floor_state: stateArr(fl.state || new Uint8Array(n), Uint8Array) });
   }
   return { fields: f, back, floorA };
 }
@@ -183,6 +185,12 @@ class CpuSolverInstance {
     rod.energyParts = { bending: s.eBend, torsion: s.eTwist, stretch: s.eStretch, shear: s.eShear };
     rod.energy = s.eStretch + s.eShear + s.eBend + s.eTwist;
     if (ct) { ct.energy = s.eContact; ct.penN = s.penN; }
+    if (md.floor) {
+      // the ground's energy, largest Fn and the ramp of the iteration that evaluated the state (solver_gpu.js:460-462)
+      const fl = md.floor, rampN = Math.max(md.scene.contact.ramp_iter | 0, 0), it = Math.max(s.iteration - 1, 0);
+      fl.energy = s.eFloor; fl.FnMax = s.fnFloor;
+      fl.ramp = rampN <= 0 ? 1.0 : Math.min(1.0, 0.01 + 0.99 * it / rampN);
+    }
     // the frame inertia of the state (CpuSolver.step), when the object carries the reference's methods
     if (md.inertia3 && typeof rod.updateBendCoupling === 'function') {
       const M = st.m, kTw = new Float64Array(M);
@@ -296,7 +304,9 @@ class GpuSolver {
         Object.assign(md.st, { iteration: s.iteration, residual: s.residual });
         md.converged = !!s.converged; md.kePrev = s.kePrev;
         md.residualParts = { force: s.resForce, torque: s.resTorque, fref: s.fref };
-        md.energies = { stretch: s.eStretch, shear: s.eShear, bending: s.eBend, torsion: s.eTwist, contact: s.eContact };
+        md.energies = { stretch: s.eStretch, shear: s.eShear, bending: s.eBend, torsion: s.eTwist, contact: s.eContact,
+          floor: s.eFloor };
+        md.floorFnMax = s.fnFloor;
       }
       return opts.history === false ? md : this._readHistory(b).then(() => md);
     }));
@@ -332,7 +342,7 @@ class GpuSolver {
 
 /** buildResults (solver_cpu.js:878-922) over a downloaded model: per-segment Mx My T N Vx Vy, per-crossing contact
  *  layers, per-node displacement, reactions of the anchored frames keyed "strand,k" (a fixed node that is no frame:
- *  "node,strand,node"), energies.  The ground's per-node force is not part of a download. */
+ *  "node,strand,node"), the ground (its force on every node, state, summary; null without one), energies */
 function buildResults(model) {
   const st = model.st, info = model.info, n = info.n, m = info.m, K = info.K;
   const mom = native().modelSegmentMoments(model.h);
@@ -366,13 +376,22 @@ function buildResults(model) {
   }
   const e = model.energies || {};
   const beam = (e.bending || 0) + (e.torsion || 0) + (e.stretch || 0) + (e.shear || 0);
-  return { iterations: st.iteration, converged: !!model.converged, residual: st.residual, segment, knot, disp, reactions,
+  let floor = null;
+  if (info.floorOn) {
+    // Floor.summary (solver_cpu.js:629-633): nodes on the ground, the slipping ones, the resultant of their forces
+    const F = Float64Array.from(st.floorF), state = Uint8Array.from(st.floorState), sum = [0, 0, 0];
+    let nodes = 0, slipping = 0;
+    for (let i = 0; i < n; i++) if (state[i]) { nodes++; if (state[i] === 2) slipping++; for (let k = 0; k < 3; k++) sum[k] += F[3 * i + k]; }
+    floor = { z: info.floorZ, mu: info.floorMu, F, state, nodes, slipping, sum, FnMax: model.floorFnMax || 0 };
+  }
+  const ef = info.floorOn ? (e.floor || 0) : 0;
+  return { iterations: st.iteration, converged: !!model.converged, residual: st.residual, segment, knot, disp, reactions, floor,
     energies: { kinetic: model.kePrev || 0, bending: e.bending || 0, torsion: e.torsion || 0, stretch: e.stretch || 0,
-      shear: e.shear || 0, contact: e.contact || 0, beam, strain: beam + (e.contact || 0) }, rod: 'cosserat' };
+      shear: e.shear || 0, contact: e.contact || 0, floor: ef, beam, strain: beam + (e.contact || 0) + ef }, rod: 'cosserat' };
 }
 
 /** resultsJSON (app.js:360-390): per-strand segment arrays, knot values by the knot's mesh edge id, per-strand
- *  displacements, reactions */
+ *  displacements, reactions, and with a ground its plane and the nodes resting on it */
 function resultsJSON(model, r) {
   const info = model.info;
   const perStrand = (a) => { const out = []; for (let s = 0; s < info.S; s++) out.push(Array.from(a.slice(info.segOffset[s], info.segOffset[s + 1]))); return out; };
@@ -382,7 +401,21 @@ function resultsJSON(model, r) {
   const disp = [];
   for (let s = 0; s < info.S; s++) { const rows = []; for (let i = info.nodeOffset[s]; i < info.nodeOffset[s + 1]; i++) rows.push([r.disp[3 * i], r.disp[3 * i + 1], r.disp[3 * i + 2]]); disp.push(rows); }
   const reactions = {}; for (const [k, v] of r.reactions) reactions[k] = v;
-  return { iterations: r.iterations, converged: r.converged, residual: r.residual, segment, knot, disp, reactions, energies: r.energies };
+  const doc = { iterations: r.iterations, converged: r.converged, residual: r.residual, energies: r.energies, segment, knot,
+    node: { disp }, reactions };
+  if (r.floor) {
+    // the plane and its force on every node resting on it (app.js:377-382)
+    const contacts = [], X = model.st.X;
+    for (let s = 0; s < info.S; s++) {
+      for (let i = info.nodeOffset[s]; i < info.nodeOffset[s + 1]; i++) {
+        if (!r.floor.state[i]) continue;
+        contacts.push({ strand: s, node: i - info.nodeOffset[s], point: [X[3 * i], X[3 * i + 1], X[3 * i + 2]].map(v => +v.toFixed(4)),
+          F: [r.floor.F[3 * i], r.floor.F[3 * i + 1], r.floor.F[3 * i + 2]], sliding: r.floor.state[i] === 2 });
+      }
+    }
+    doc.floor = { z: r.floor.z, mu: r.floor.mu, nodes: r.floor.nodes, sliding: r.floor.slipping, resultant: r.floor.sum, contacts };
+  }
+  return doc;
 }
 
 /** a float as Python's repr writes it (the reports' tables were written by Python): the shortest round-trip digits,

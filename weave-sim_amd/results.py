@@ -4,10 +4,9 @@ reports (examples/scenes/report-*/knots.csv, segments.csv), over a downloaded st
 
     r = build_results(model, state)        # state: a StateBuffers filled by GpuSolver.download (or the model itself)
     doc = results_json(model, r)           # per-strand segment arrays, knot values by mesh edge id, reactions by frame
-    knots_csv(model, r), segments_csv(model, r)
-
-The ground's per-node force is not part of a download, so `floor` is absent."""
+    knots_csv(model, r), segments_csv(model, r)"""
 import ctypes as C
+import decimal
 
 import numpy as np
 
@@ -87,18 +86,45 @@ def build_results(model, state, contact_options=None):
             continue
         s = int(np.searchsorted(noff, i, side="right") - 1)
         reactions[("node", s, int(i - noff[s]))] = [float(R[i, 0]), float(R[i, 1]), float(R[i, 2]), 0.0, 0.0, 0.0]
+    # the ground (Floor.F / state and Floor.summary, solver_cpu.js:629-633, 914-915): None without one
+    floor = None
+    ef = st.e_floor if par.floor_on else 0.0
+    if par.floor_on:
+        FG = _arr(state, "floor_F", 3 * n)
+        GS = _arr(state, "floor_state", n, np.uint8)
+        on = GS != 0
+        floor = {"z": float(par.floor_z), "mu": float(par.floor_mu), "F": FG, "state": GS, "nodes": int(on.sum()),
+                 "slipping": int((GS == 2).sum()), "sum": [float(v) for v in _seq_sum(FG.reshape(n, 3)[on])],
+                 "FnMax": float(st.fn_floor)}
     energies = {"kinetic": st.ke, "bending": st.e_bend, "torsion": st.e_twist, "stretch": st.e_stretch,
-                "shear": st.e_shear, "contact": st.e_contact,
+                "shear": st.e_shear, "contact": st.e_contact, "floor": ef,
                 "beam": st.e_bend + st.e_twist + st.e_stretch + st.e_shear}
-    energies["strain"] = energies["beam"] + st.e_contact
+    energies["strain"] = energies["beam"] + st.e_contact + ef
     return {"iterations": st.iteration, "converged": bool(st.converged), "residual": st.residual,
-            "segment": seg, "knot": knot, "disp": disp, "reactions": reactions, "energies": energies,
+            "segment": seg, "knot": knot, "disp": disp, "reactions": reactions, "floor": floor, "energies": energies,
             "rod": "cosserat"}
 
 
-def results_json(model, r):
+def _seq_sum(rows):
+    """column sums of rows added one after another, in row order (the reference's summary loop)"""
+    acc = [0.0, 0.0, 0.0]
+    for r in rows:
+        for k in range(3):
+            acc[k] += float(r[k])
+    return acc
+
+
+def _fixed4(v):
+    """a number as JavaScript's +v.toFixed(4) gives it: the nearest multiple of 1e-4 to the exact binary value,
+    exact ties away from zero (toFixed rounds |v| and keeps the sign)"""
+    d = decimal.Decimal(float(v))
+    return float(d.quantize(decimal.Decimal("0.0001"), rounding=decimal.ROUND_HALF_UP))
+
+
+def results_json(model, r, state=None):
     """the export document (resultsJSON, app.js:360-390): per-strand segment arrays, knot values by the knot's mesh
-    edge id, per-strand node displacements, reactions by "strand,k" of the anchored frame"""
+    edge id, per-strand node displacements, reactions by "strand,k" of the anchored frame, and with a ground its plane
+    and the nodes resting on it (positions from `state`, else the model's own arrays)"""
     md = model.md
     soff = np.ctypeslib.as_array(md.seg_offset, shape=(md.S + 1,))
     noff = np.ctypeslib.as_array(md.node_offset, shape=(md.S + 1,))
@@ -109,8 +135,20 @@ def results_json(model, r):
     d = r["disp"].reshape(-1, 3)
     disp = [d[noff[s]:noff[s + 1]].tolist() for s in range(md.S)]
     reactions = {",".join(str(x) for x in key): v for key, v in r["reactions"].items()}
-    return {"iterations": int(r["iterations"]), "converged": r["converged"], "residual": float(r["residual"]),
-            "segment": seg, "knot": knot, "disp": disp, "reactions": reactions, "energies": r["energies"]}
+    doc = {"iterations": int(r["iterations"]), "converged": r["converged"], "residual": float(r["residual"]),
+           "energies": r["energies"], "segment": seg, "knot": knot, "node": {"disp": disp}, "reactions": reactions}
+    fl = r.get("floor")
+    if fl is not None:
+        # the plane and its force on every node resting on it (app.js:377-382)
+        X = _arr(state if state is not None else model, "X", 3 * md.n)
+        contacts = []
+        for i in np.flatnonzero(fl["state"]):
+            s = int(np.searchsorted(noff, i, side="right") - 1)
+            contacts.append({"strand": s, "node": int(i - noff[s]), "point": [_fixed4(X[3 * i + k]) for k in range(3)],
+                             "F": [float(fl["F"][3 * i + k]) for k in range(3)], "sliding": bool(fl["state"][i] == 2)})
+        doc["floor"] = {"z": fl["z"], "mu": fl["mu"], "nodes": fl["nodes"], "sliding": fl["slipping"],
+                        "resultant": fl["sum"], "contacts": contacts}
+    return doc
 
 
 def _fmt(x):
