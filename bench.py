@@ -35,6 +35,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 COPY_BYTES = 256 << 20          # the calibration copy: 256 MiB read, 256 MiB written
+STREAM_BYTES = 1 << 30          # the plain copy timed beside k_strand: 1 GiB read, 1 GiB written (the batch's size)
 
 WORKLOADS = {
     "dome_3m": "kagome dome 3 m span / 0.3 m spacing / 50 mm segments, 15% contraction, +-1 mm imperfection, mu 0.3",
@@ -540,6 +541,8 @@ def main():
     kb = g.kernel_bench(args.kernel_reps)
     sb, cb = strand_cross_bytes(models, args.precision)
     achieved = sb / (kin["ms_strand"] * 1e-3) / 1e9
+    # what a plain copy kernel (16-byte loads and stores, grid-stride) reaches on this GPU: the practical HBM ceiling
+    copy_gbs = 2 * STREAM_BYTES / (g.ctx.copy_probe(STREAM_BYTES, 20) * 1e-3) / 1e9
     alg = g.bytes()["alg_bytes_per_iter"]
     # the host-buffer boundary (not part of `value`): every instance uploaded from its host model and state, and
     # downloaded into host buffers allocated beforehand
@@ -576,6 +579,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_strand",
                      "alg_bytes_per_launch": sb, "ms_per_launch": kin["ms_strand"],
                      "duration_from": "HIP events between consecutive kernels of the iteration sequence",
+                     "copy_kernel_GBps": copy_gbs, "frac_of_copy_kernel": achieved / copy_gbs,
                      "traffic_detail": traffic_detail},
         "ms_per_step_device": step_dev_ms / args.steps,
         "kernels_ms_in_iteration": kin,

@@ -119,6 +119,33 @@ inline uint8_t fix_bits(const ws_model& md, int i) {
   return bits;
 }
 
+// the section stiffness per strand, when every segment of each strand carries the same (EA, GA, EIx, EIy, GJ) bit for
+// bit in the device's precision (a lath's section is its group's, model.js:236-252): the strand kernel then reads 20
+// bytes per strand instead of per segment.  And the axes along which some constant load is not +0: the others are
+// not read.  Uploaded at build and by every parameter update (a scene edit may change either).
+template <typename Real>
+void upload_sections(Inst<Real>& d, const ws_model& md) {
+  const int S = md.S;
+  const double* src[5] = {md.EA, md.GA, md.EIx, md.EIy, md.GJ};
+  std::vector<Real> tab(5 * size_t(S), Real(0));
+  bool uni = true;
+  for (int s = 0; s < S && uni; s++) {
+    const int j0 = md.seg_offset[s], j1 = md.seg_offset[s + 1];
+    for (int q = 0; q < 5 && uni; q++) {
+      const Real v = j1 > j0 ? Real(src[q][j0]) : Real(0);
+      tab[5 * size_t(s) + q] = v;
+      for (int j = j0 + 1; j < j1 && uni; j++) { const Real w = Real(src[q][j]); uni = std::memcmp(&w, &v, sizeof(Real)) == 0; }
+    }
+  }
+  d.sec_strand = uni ? 1 : 0;
+  if (uni) h2d(d.sec, tab);
+  int axes = 0;
+  const Real zero = Real(0);
+  for (int i = 0; i < md.n && axes != 7; i++)
+    for (int c = 0; c < 3; c++) { const Real v = Real(md.load_const[3 * i + c]); if (std::memcmp(&v, &zero, sizeof(Real)) != 0) axes |= 1 << c; }
+  d.load_axes = axes;
+}
+
 // build the device image of one host model (GpuSolver.upload, solver_gpu.js:191-310)
 // fused: the strand launch runs the control step (chunks of FUSED_CHUNK tiles by default); else the control kernel
 // (measured: fusing is slower on gfx950 — each tile's device-scope release fence writes back its XCD's L2)
@@ -213,6 +240,7 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   for (int b = 0; b < 2; b++) A.add(d.Q[b], 4 * size_t(m));
   A.add(d.wx, m); A.add(d.wy, m); A.add(d.wz, m);
   A.addc(d.l0, m); A.addc(d.EA, m); A.addc(d.GA, m); A.addc(d.EIx, m); A.addc(d.EIy, m); A.addc(d.GJ, m);
+  A.addc(d.sec, 5 * size_t(S));
   A.addc(d.krx, m); A.addc(d.kry, m); A.addc(d.krz, m);
   if (d.floor_on) { A.addc(d.nodew, n); A.addc(d.nodeh, n); A.add(d.fax, n); A.add(d.fay, n); A.add(d.gstate, n); }
   A.addc(d.jlen, p); A.addc(d.om0, 3 * p);
@@ -287,6 +315,7 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   h2d(d.l0, std::vector<double>(md.l0, md.l0 + m));
   h2d(d.EA, toReal<Real>(md.EA, m)); h2d(d.GA, toReal<Real>(md.GA, m)); h2d(d.EIx, toReal<Real>(md.EIx, m));
   h2d(d.EIy, toReal<Real>(md.EIy, m)); h2d(d.GJ, toReal<Real>(md.GJ, m));
+  upload_sections(d, md);
   std::vector<Real> kx(m), ky(m), kz(m);
   for (int j = 0; j < m; j++) { kx[j] = Real(md.krot_static[3 * j]); ky[j] = Real(md.krot_static[3 * j + 1]); kz[j] = Real(md.krot_static[3 * j + 2]); }
   h2d(d.krx, kx); h2d(d.kry, ky); h2d(d.krz, kz);
@@ -1029,6 +1058,7 @@ struct Batch : BatchBase {
     h2d(d.fix, fx);
     h2d(d.EA, toReal<Real>(md.EA, m)); h2d(d.GA, toReal<Real>(md.GA, m)); h2d(d.EIx, toReal<Real>(md.EIx, m));
     h2d(d.EIy, toReal<Real>(md.EIy, m)); h2d(d.GJ, toReal<Real>(md.GJ, m));
+    upload_sections(d, md);
     std::vector<Real> kx(m), ky(m), kz(m);
     for (int j = 0; j < m; j++) { kx[j] = Real(md.krot_static[3 * j]); ky[j] = Real(md.krot_static[3 * j + 1]); kz[j] = Real(md.krot_static[3 * j + 2]); }
     h2d(d.krx, kx); h2d(d.kry, ky); h2d(d.krz, kz);

@@ -82,6 +82,8 @@ struct Inst {
   double edge_thr, l_ref, jlen_min, inv_jlen_min;   // inv_jlen_min: 1/l̄_min, 0 without joints
   int32_t kin_damp, warmup, contacts, ramp_iter, rot_fric, seat, seat_gap, dent_after, aug, al_every, hist_len, floor_on;
   int32_t rigid, pad_r;              // rigid: the ground holds the structure alone (no anchor, no clamp)
+  int32_t sec_strand;                // every strand's segments share (EA, GA, EIx, EIy, GJ): read from sec, not per segment
+  int32_t load_axes;                 // bit c: some node's constant load has a component along axis c (else not read)
   double floor_z, floor_mu;          // the ground plane
   double rb_R, rb_vmax, rb_wmax;     // rigid body: model radius about the centroid, speed limits per iteration
   int32_t proto, probe, n_inc, settle, avg, pad_p;   // UTM replay: mode (0 off, 1 force, 2 displacement), probe node, …
@@ -111,6 +113,7 @@ struct Inst {
   Real* wx; Real* wy; Real* wz;
   const double* l0;
   const Real* EA; const Real* GA; const Real* EIx; const Real* EIy; const Real* GJ;
+  const Real* sec;           // [5S] EA, GA, EIx, EIy, GJ per strand (sec_strand)
   const Real* krx; const Real* kry; const Real* krz;
   // ground (floor_on): node section width / height, stick points (f64)
   const Real* nodew; const Real* nodeh;

@@ -404,8 +404,11 @@ __device__ __forceinline__ int xchg(int v) { return __shfl_xor(v, 1, 64); }
 // ONE: the batch holds a single instance (slot 0) and block b takes its crossings [b·n, b·n + n), n the crossings per
 // block, K = oneK: the work item follows from the block index and the descriptor and control record from slot 0, so
 // they load together instead of one after the other
+#ifndef WS_CROSS_WPE
+#define WS_CROSS_WPE 2   // resident wavefronts per SIMD the crossing kernel is compiled for
+#endif
 template <typename Real, int LANES, bool ONE>
-__global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(2))) void k_cross(const Inst<Real>* __restrict__ insts, const CrossItem* __restrict__ items,
+__global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(WS_CROSS_WPE))) void k_cross(const Inst<Real>* __restrict__ insts, const CrossItem* __restrict__ items,
                                                         int32_t nitems, const Ctl* __restrict__ ctls, int32_t oneK) {
   static_assert(LANES == 1 || LANES == 2, "one or two lanes per crossing");
   using G = double;
@@ -907,14 +910,26 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   // node parameters and velocity
   uint8_t fx = 0; Real ms = 1, ldx = 0, ldy = 0, ldz = 0, llx = 0, lly = 0, llz = 0, vx = 0, vy = 0, vz = 0;
   if (ownNode) {
-    fx = gl(I.fix)[i]; ms = gl(I.mass)[i]; ldx = gl(I.lx)[i]; ldy = gl(I.ly)[i]; ldz = gl(I.lz)[i];
+    fx = gl(I.fix)[i]; ms = gl(I.mass)[i];
+    const int ax = I.load_axes;                    // the load components that are +0 everywhere are not read
+    if (ax & 1) ldx = gl(I.lx)[i];
+    if (ax & 2) ldy = gl(I.ly)[i];
+    if (ax & 4) ldz = gl(I.lz)[i];
     if (I.npl) { llx = gl(I.llx)[i]; lly = gl(I.lly)[i]; llz = gl(I.llz)[i]; }
     if (!OUT && !rbk) { vx = gl(I.vx)[i]; vy = gl(I.vy)[i]; vz = gl(I.vz)[i]; }
   }
-  // segment and joint parameters of the slot
+  // segment and joint parameters of the slot; a strand whose segments share their section reads it once per tile
+  // (a uniform load) instead of per segment
+  const bool secS = I.sec_strand != 0;
+  Real sEA = 0, sGA = 0, sBx = 0, sBy = 0, sBz = 0;
+  if (secS) {
+    const auto sp = gl(I.sec) + 5 * tl.strand;
+    sEA = sp[0]; sGA = sp[1]; sBx = sp[2]; sBy = sp[3]; sBz = sp[4];
+  }
   double l0 = 1.0; Real EA = 0, GA = 0, Bx = 0, By = 0, Bz = 0, krx = 0, kry = 0, krz = 0, wx = 0, wy = 0, wz = 0;
   if (hasSeg) {
-    l0 = gl(I.l0)[j]; EA = gl(I.EA)[j]; GA = gl(I.GA)[j];
+    l0 = gl(I.l0)[j];
+    if (secS) { EA = sEA; GA = sGA; } else { EA = gl(I.EA)[j]; GA = gl(I.GA)[j]; }
     if (ownSeg) {
       krx = gl(I.krx)[j]; kry = gl(I.kry)[j]; krz = gl(I.krz)[j];
       if (!OUT && !rbk) { wx = gl(I.wx)[j]; wy = gl(I.wy)[j]; wz = gl(I.wz)[j]; }
@@ -924,7 +939,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
   if (hasJoint) {
     const int jg = joff + jjo;
     lb = gl(I.jlen)[jg]; om[0] = gl(I.om0)[3 * jg]; om[1] = gl(I.om0)[3 * jg + 1]; om[2] = gl(I.om0)[3 * jg + 2];
-    Bx = gl(I.EIx)[j]; By = gl(I.EIy)[j]; Bz = gl(I.GJ)[j];
+    if (secS) { Bx = sBx; By = sBy; Bz = sBz; } else { Bx = gl(I.EIx)[j]; By = gl(I.EIy)[j]; Bz = gl(I.GJ)[j]; }
   }
   const int so = tl.so, ns = tl.ns;
   SideRec<Real> r0;
