@@ -1166,11 +1166,14 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       const int nx = cur ^ 1;
       const double xn = L.Xx[t] + double(V[0]), yn = L.Xy[t] + double(V[1]), zn = L.Xz[t] + double(V[2]);
       const auto Xn = gl(buf(I.X, nx));
-      Xn[3 * i] = xn;
-      Xn[3 * i + 1] = yn;
-      Xn[3 * i + 2] = zn;
+      // the new state is not read again in this launch: streaming stores (measured: the 10 m dome's strand launch
+      // 13.1 → 12.1 µs, the 256 variants' 167.2 → 166.2 µs)
+      __builtin_nontemporal_store(xn, &Xn[3 * i]);
+      __builtin_nontemporal_store(yn, &Xn[3 * i + 1]);
+      __builtin_nontemporal_store(zn, &Xn[3 * i + 2]);
       if (RB) { rb[10] = xn; rb[11] = yn; rb[12] = zn; }
-      gl(I.vx)[i] = V[0]; gl(I.vy)[i] = V[1]; gl(I.vz)[i] = V[2];
+      __builtin_nontemporal_store(V[0], &gl(I.vx)[i]); __builtin_nontemporal_store(V[1], &gl(I.vy)[i]);
+      __builtin_nontemporal_store(V[2], &gl(I.vz)[i]);
       keN = double(ms) * (double(V[0]) * double(V[0]) + double(V[1]) * double(V[1]) + double(V[2]) * double(V[2]));
     }
   }
@@ -1220,7 +1223,8 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       W[2] = clampv(W[2], Real(-TWIST_RATE_MAX), Real(TWIST_RATE_MAX));
       if (kTw > Real(0)) W[2] *= Real(ROLL_DAMP);
       if (damping != Real(1)) for (int c = 0; c < 3; c++) W[c] *= damping;
-      gl(I.wx)[j] = W[0]; gl(I.wy)[j] = W[1]; gl(I.wz)[j] = W[2];
+      __builtin_nontemporal_store(W[0], &gl(I.wx)[j]); __builtin_nontemporal_store(W[1], &gl(I.wy)[j]);
+      __builtin_nontemporal_store(W[2], &gl(I.wz)[j]);
       // q ⊗ exp(ω): the increment exp(ω) − 1 = (−2 sin²(a/4), ω sin(a/2)/a) carries the precision of the small
       // rotation itself (the delta form of wgsl.js:962-980); the product and the renormalisation run in f64
       double e0, e1, e2, e3;
@@ -1251,7 +1255,8 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       double inv = n2 > 0 ? rsqrt_d(n2) : 1.0;
       const int nx = cur ^ 1;
       const auto Qn = gl(buf(I.Q, nx));
-      Qn[4 * j] = rw * inv; Qn[4 * j + 1] = rx * inv; Qn[4 * j + 2] = ry * inv; Qn[4 * j + 3] = rz * inv;
+      __builtin_nontemporal_store(rw * inv, &Qn[4 * j]); __builtin_nontemporal_store(rx * inv, &Qn[4 * j + 1]);
+      __builtin_nontemporal_store(ry * inv, &Qn[4 * j + 2]); __builtin_nontemporal_store(rz * inv, &Qn[4 * j + 3]);
       keR = double(I3[0]) * double(W[0]) * double(W[0]) + double(I3[1]) * double(W[1]) * double(W[1]) + double(I3[2]) * double(W[2]) * double(W[2]);
     }
   }
