@@ -176,7 +176,8 @@ def cpu_variants_baseline(models, iters):
 def solver_opts(args):
     """the batch options of this run: the tile, graph length and crossing layout given on the command line (0 = the
     library's choice by size)"""
-    return {"tile": args.tile, "per_submit": args.per_submit, "cross_lanes": args.cross_lanes, "ctrl_mode": args.ctrl_mode}
+    return {"tile": args.tile, "per_submit": args.per_submit, "cross_lanes": args.cross_lanes, "ctrl_mode": args.ctrl_mode,
+            "streams": args.streams}
 
 
 def kernel_only(args):
@@ -214,7 +215,8 @@ def counter_pass(args, counter, outdir):
     cmd = ["timeout", "-s", "KILL", "150", "rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", outdir,
            "-o", "pass", "--", sys.executable, os.path.abspath(__file__), "--kernel-only", "--workload", args.workload,
            "--precision", args.precision, "--tile", str(args.tile), "--per-submit", str(args.per_submit),
-           "--cross-lanes", str(args.cross_lanes), "--ctrl-mode", str(args.ctrl_mode), "--batch", str(args.batch)]
+           "--cross-lanes", str(args.cross_lanes), "--ctrl-mode", str(args.ctrl_mode), "--streams", str(args.streams),
+           "--batch", str(args.batch)]
     env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
     r = subprocess.run(cmd, cwd=tempfile.gettempdir(), env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True)
@@ -455,6 +457,8 @@ def main():
                          "workload, the default elsewhere")
     ap.add_argument("--per-submit", type=int, default=0, help="iterations per captured graph (0 = the default, 64)")
     ap.add_argument("--cross-lanes", type=int, default=0, help="lanes per crossing, 1 or 2 (0 = by size)")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="streams of launches: 1, or 2 (the instances in two overlapping groups); 0 = by size")
     ap.add_argument("--ctrl-mode", type=int, default=0,
                     help="0: a control launch after the strand launch; 1: the strand launch runs the control step")
     ap.add_argument("--no-cpu", action="store_true")
@@ -574,6 +578,7 @@ def main():
         "config": {"workload": f"{args.workload}: {WORKLOADS[args.workload]}", "instances": len(models),
                    "nodes": n_nodes, "crossings": n_cross, "precision": args.precision, "strand_tile": opts["tile"],
                    "per_submit": opts["per_submit"], "cross_lanes": opts["cross_lanes"], "ctrl_mode": opts["ctrl_mode"],
+                   "streams": opts["streams"],
                    "autotuned": tune is not None, "device": dev["name"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_strand",

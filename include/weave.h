@@ -195,6 +195,10 @@ typedef struct ws_batch_opts {
   int32_t ctrl_mode;       /* 0: a control launch after the strand launch (the default); 1: the strand launch's last
                               tiles per instance run the control step (no control launch; slower on gfx950, where
                               every tile's device-scope release writes back its L2) */
+  int32_t streams;         /* 1: every launch on the batch's stream; 2: the instances in two groups of about equal
+                              work, each replaying its own iteration graphs on its own stream (one group's
+                              latency-bound launches overlap the other's strand launch; results as with one);
+                              0 = by size (2 from two instances and 8,192 strand tiles) */
 } ws_batch_opts;
 
 #define WS_TILE_AUTOTUNE (-1)

@@ -72,7 +72,7 @@ class ws_control(C.Structure):
 
 class ws_batch_opts(C.Structure):
     _fields_ = [("precision", i32), ("per_submit", i32), ("tile", i32), ("history", i32), ("ctrl_chunk", i32),
-                ("cross_lanes", i32), ("ctrl_mode", i32)]
+                ("cross_lanes", i32), ("ctrl_mode", i32), ("streams", i32)]
 
 
 WS_TILE_AUTOTUNE = -1

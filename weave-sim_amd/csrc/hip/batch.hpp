@@ -425,11 +425,18 @@ struct BatchBase {
   bool has_rigid = false;               // an instance that only the ground holds: the rigid step runs
   int cross_lanes = 0;                  // lanes per crossing (0: by size at every relist)
   int lanes = 2;                        // ... as the current work lists use it
-  hipStream_t stream = nullptr;
-  // captured iteration sequences by length: per_submit iterations, and every power of two below it, so a step of
-  // n iterations replays whole per_submit graphs and then the binary pieces of the remainder — only live iterations
-  // are launched
-  std::map<int, hipGraphExec_t> graphs;
+  // the instances split into two groups of about equal work, each replaying its own iteration graphs on its own
+  // stream: one group's latency-bound launches (crossings, control) and kernel boundaries overlap the other's
+  // bandwidth-bound strand launch.  Instances never read each other's memory, so the results are those of one stream.
+  int streams_opt = 0;                  // 0: by size (two from SPLIT_TILES strand tiles and two instances), 1 or 2
+  int nstreams = 1;                     // ... as the current work lists use it
+  hipStream_t stream = nullptr;         // the batch's stream (group 0); every call outside a step uses it alone
+  hipStream_t stream2 = nullptr;        // group 1's stream during a step (joined back into `stream` before it ends)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // captured iteration sequences by length, per group: per_submit iterations, and every power of two below it, so a
+  // step of n iterations replays whole per_submit graphs and then the binary pieces of the remainder — only live
+  // iterations are launched
+  std::map<int, hipGraphExec_t> graphs, graphs2;
   bool eager_graphs = true;             // build every length at the first step (the request server builds lazily)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double ms_last = 0, ms_cross = 0, ms_strand = 0, ms_ctrl = 0;
@@ -485,13 +492,23 @@ struct Batch : BatchBase {
     if (d_ctls) (void)hipFree(d_ctls);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
     if (stream) (void)hipStreamDestroy(stream);
+    if (stream2) (void)hipStreamDestroy(stream2);
+  }
+
+  void make_streams() {
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&stream2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&ev0)); HIPCHK(hipEventCreate(&ev1));
+    HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
   }
 
   void create(const ws_model* const* models, const ws_state* const* states, int nB) {
     B = nB;
-    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    HIPCHK(hipEventCreate(&ev0)); HIPCHK(hipEventCreate(&ev1));
+    make_streams();
     // the control records of all instances sit side by side: one copy reads every instance's state
     HIPCHK(hipMalloc(&d_ctls, sizeof(Ctl) * std::max(B, 1)));
     HIPCHK(hipMemset(d_ctls, 0, sizeof(Ctl) * std::max(B, 1)));
@@ -508,8 +525,7 @@ struct Batch : BatchBase {
   // kernel skips it
   void create_slots(int capacity) override {
     B = capacity;
-    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    HIPCHK(hipEventCreate(&ev0)); HIPCHK(hipEventCreate(&ev1));
+    make_streams();
     HIPCHK(hipMalloc(&d_ctls, sizeof(Ctl) * std::max(B, 1)));
     std::vector<Ctl> idle(std::max(B, 1));
     for (auto& c : idle) { c = Ctl{}; c.done = 1; }
@@ -559,28 +575,58 @@ struct Batch : BatchBase {
   }
   int cap_tiles = 0, cap_cross = 0, cap_ctrl = 0, cap_inst = 0;
 
+  // a contiguous range of the work lists: every item of the instances of one group (or of all of them)
+  struct Range { int t0 = 0, nt = 0, c0 = 0, nc = 0, k0 = 0, nk = 0; bool rigid = false; };
+  Range group[2];
+  Range all_range() const {
+    Range r; r.nt = ntiles; r.nc = ncross; r.nk = nctrl; r.rigid = has_rigid; return r;
+  }
+  // two streams from this many strand tiles: each group then still fills the chip (65,370 tiles of 256 variants:
+  // 6.5 % faster per iteration with two than with one; four streams were no faster than one)
+  static constexpr long SPLIT_TILES = 8192;
+
   void relist() override {
     tiles.clear();
     std::vector<CrossItem> cross;
     std::vector<CtrlItem> ctrl;
-    long total_k = 0;
-    for (int b = 0; b < B; b++) if (inst[b]) total_k += inst[b]->K;
+    long total_k = 0, total_t = 0;
+    int ninst = 0;
+    for (int b = 0; b < B; b++) if (inst[b]) { total_k += inst[b]->K; total_t += long(inst[b]->tiles.size()); ninst++; }
     lanes = cross_lanes > 0 ? cross_lanes : (total_k >= CROSS_ONE_LANE ? 1 : 2);
+    nstreams = streams_opt > 0 ? streams_opt : (total_t >= SPLIT_TILES ? 2 : 1);
+    if (ninst < 2) nstreams = 1;
+    // groups of about equal strand tiles: largest instance first into the lighter group
+    std::vector<int> grp(std::max(B, 1), 0);
+    if (nstreams == 2) {
+      std::vector<int> order;
+      for (int b = 0; b < B; b++) if (inst[b]) order.push_back(b);
+      std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return inst[a]->tiles.size() > inst[c]->tiles.size(); });
+      long load[2] = {0, 0};
+      for (int b : order) { const int g = load[1] < load[0] ? 1 : 0; grp[b] = g; load[g] += long(inst[b]->tiles.size()); }
+    }
     const int per_blk = CROSS_THREADS / lanes;
-    for (int b = 0; b < B; b++) {
-      if (!inst[b]) continue;
-      InstHost<Real>& H = at(b);
-      H.d.tile0 = int(tiles.size());
-      for (auto t : H.tiles) { t.inst = b; tiles.push_back(t); }
-      H.d.cblk0 = int(cross.size());
-      H.d.ncblk = (H.K + per_blk - 1) / per_blk;   // cpart holds H.ncblk ≥ this many blocks
-      for (int q = 0; q < H.d.ncblk; q++) cross.push_back(CrossItem{b, q * per_blk, std::min(per_blk, H.K - q * per_blk), q});
-      for (int q = 0; q < H.d.ctrl_chunks; q++) ctrl.push_back(CtrlItem{b, q, H.d.ctrl_chunks, 0});
+    for (int g = 0; g < nstreams; g++) {
+      Range& R = group[g];
+      R = Range{};
+      R.t0 = int(tiles.size()); R.c0 = int(cross.size()); R.k0 = int(ctrl.size());
+      for (int b = 0; b < B; b++) {
+        if (!inst[b] || grp[b] != g) continue;
+        InstHost<Real>& H = at(b);
+        H.d.tile0 = int(tiles.size());
+        for (auto t : H.tiles) { t.inst = b; tiles.push_back(t); }
+        H.d.cblk0 = int(cross.size());
+        H.d.ncblk = (H.K + per_blk - 1) / per_blk;   // cpart holds H.ncblk ≥ this many blocks
+        for (int q = 0; q < H.d.ncblk; q++) cross.push_back(CrossItem{b, q * per_blk, std::min(per_blk, H.K - q * per_blk), q});
+        for (int q = 0; q < H.d.ctrl_chunks; q++) ctrl.push_back(CtrlItem{b, q, H.d.ctrl_chunks, 0});
+        if (H.d.rigid) R.rigid = true;
+      }
+      R.nt = int(tiles.size()) - R.t0; R.nc = int(cross.size()) - R.c0; R.nk = int(ctrl.size()) - R.k0;
     }
     ntiles = int(tiles.size()); ncross = int(cross.size()); nctrl = int(ctrl.size());
     has_rigid = false;
     for (int b = 0; b < B; b++) if (inst[b] && inst[b]->d.rigid) has_rigid = true;
     HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipStreamSynchronize(stream2));
     put_list(d_tiles, cap_tiles, tiles);
     put_list(d_cross, cap_cross, cross);
     put_list(d_ctrl, cap_ctrl, ctrl);
@@ -594,78 +640,95 @@ struct Batch : BatchBase {
   // a batch of one instance launches the kernels' single-instance form (work item from the block index, descriptor
   // and control record of slot 0 loaded beside it)
   bool one_inst() const { return B == 1 && inst[0] != nullptr; }
-  void launch_cross(hipStream_t s) {
-    if (!ncross) return;
+  // the kernels over a range of the work lists (the single-instance forms take the whole batch of one)
+  void launch_cross(hipStream_t s, const Range& R) {
+    if (!R.nc) return;
     const bool one = one_inst();
     const int32_t K = one ? inst[0]->K : 0;
+    const CrossItem* items = d_cross + R.c0;
     if (lanes == 1) {
-      if (one) hipLaunchKernelGGL((k_cross<Real, 1, true>), dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls, K);
-      else hipLaunchKernelGGL((k_cross<Real, 1, false>), dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls, K);
+      if (one) hipLaunchKernelGGL((k_cross<Real, 1, true>), dim3(R.nc), dim3(CROSS_THREADS), 0, s, d_inst, items, R.nc, d_ctls, K);
+      else hipLaunchKernelGGL((k_cross<Real, 1, false>), dim3(R.nc), dim3(CROSS_THREADS), 0, s, d_inst, items, R.nc, d_ctls, K);
     } else {
-      if (one) hipLaunchKernelGGL((k_cross<Real, 2, true>), dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls, K);
-      else hipLaunchKernelGGL((k_cross<Real, 2, false>), dim3(ncross), dim3(CROSS_THREADS), 0, s, d_inst, d_cross, ncross, d_ctls, K);
+      if (one) hipLaunchKernelGGL((k_cross<Real, 2, true>), dim3(R.nc), dim3(CROSS_THREADS), 0, s, d_inst, items, R.nc, d_ctls, K);
+      else hipLaunchKernelGGL((k_cross<Real, 2, false>), dim3(R.nc), dim3(CROSS_THREADS), 0, s, d_inst, items, R.nc, d_ctls, K);
     }
   }
-  template <int T, bool RB, int W, bool F> void strand_wf(hipStream_t s) {
-    if (one_inst()) hipLaunchKernelGGL((k_strand<Real, T, RB, true, W, F>), dim3(ntiles), dim3(T), 0, s, d_inst, d_tiles, ntiles, d_ctls);
-    else hipLaunchKernelGGL((k_strand<Real, T, RB, false, W, F>), dim3(ntiles), dim3(T), 0, s, d_inst, d_tiles, ntiles, d_ctls);
+  template <int T, bool RB, int W, bool F> void strand_wf(hipStream_t s, const Range& R) {
+    const TileItem* items = d_tiles + R.t0;
+    if (one_inst()) hipLaunchKernelGGL((k_strand<Real, T, RB, true, W, F>), dim3(R.nt), dim3(T), 0, s, d_inst, items, R.nt, d_ctls);
+    else hipLaunchKernelGGL((k_strand<Real, T, RB, false, W, F>), dim3(R.nt), dim3(T), 0, s, d_inst, items, R.nt, d_ctls);
   }
-  template <int T, bool RB, int W> void strand_w(hipStream_t s) {
-    if (fused_ctrl) strand_wf<T, RB, W, true>(s); else strand_wf<T, RB, W, false>(s);
+  template <int T, bool RB, int W> void strand_w(hipStream_t s, const Range& R) {
+    if (fused_ctrl) strand_wf<T, RB, W, true>(s, R); else strand_wf<T, RB, W, false>(s, R);
   }
   // the FP64 form of the 64-lane tile runs three resident wavefronts per SIMD once the batch has enough tiles to keep
   // them busy (measured: 10 % faster on the 30 m dome's 14k tiles, 4 % slower on the 10 m dome's 1,350)
   static constexpr int FP64_WIDE_TILES = 4096;
-  template <int T, bool RB> void strand_as(hipStream_t s) {
-    if (std::is_same<Real, double>::value && T == 64 && ntiles >= FP64_WIDE_TILES) strand_w<T, RB, (T == 64 && std::is_same<Real, double>::value) ? 3 : 1>(s);
-    else strand_w<T, RB, 1>(s);
+  template <int T, bool RB> void strand_as(hipStream_t s, const Range& R) {
+    if (std::is_same<Real, double>::value && T == 64 && ntiles >= FP64_WIDE_TILES) strand_w<T, RB, (T == 64 && std::is_same<Real, double>::value) ? 3 : 1>(s, R);
+    else strand_w<T, RB, 1>(s, R);
   }
-  void launch_strand(hipStream_t s) {
+  void launch_strand(hipStream_t s, const Range& R) {
+    if (!R.nt) return;
     switch (tile) {
-      case 64: if (has_rigid) strand_as<64, true>(s); else strand_as<64, false>(s); break;
-      case 128: if (has_rigid) strand_as<128, true>(s); else strand_as<128, false>(s); break;
-      default: if (has_rigid) strand_as<256, true>(s); else strand_as<256, false>(s); break;
+      case 64: if (R.rigid) strand_as<64, true>(s, R); else strand_as<64, false>(s, R); break;
+      case 128: if (R.rigid) strand_as<128, true>(s, R); else strand_as<128, false>(s, R); break;
+      default: if (R.rigid) strand_as<256, true>(s, R); else strand_as<256, false>(s, R); break;
     }
   }
-  void launch_ctrl(hipStream_t s) {
-    if (nctrl && !fused_ctrl) {
-      if (one_inst()) hipLaunchKernelGGL((k_ctrl<Real, true>), dim3(nctrl), dim3(CTRL_THREADS), 0, s, d_inst, d_ctrl, nctrl, d_ctls);
-      else hipLaunchKernelGGL((k_ctrl<Real, false>), dim3(nctrl), dim3(CTRL_THREADS), 0, s, d_inst, d_ctrl, nctrl, d_ctls);
+  void launch_ctrl(hipStream_t s, const Range& R) {
+    if (R.nk && !fused_ctrl) {
+      const CtrlItem* items = d_ctrl + R.k0;
+      if (one_inst()) hipLaunchKernelGGL((k_ctrl<Real, true>), dim3(R.nk), dim3(CTRL_THREADS), 0, s, d_inst, items, R.nk, d_ctls);
+      else hipLaunchKernelGGL((k_ctrl<Real, false>), dim3(R.nk), dim3(CTRL_THREADS), 0, s, d_inst, items, R.nk, d_ctls);
     }
-    if (has_rigid && ntiles) {
+    if (R.rigid && R.nt) {
+      const TileItem* items = d_tiles + R.t0;
       switch (tile) {
-        case 64: hipLaunchKernelGGL((k_rigid<Real, 64>), dim3(ntiles), dim3(64), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
-        case 128: hipLaunchKernelGGL((k_rigid<Real, 128>), dim3(ntiles), dim3(128), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
-        default: hipLaunchKernelGGL((k_rigid<Real, 256>), dim3(ntiles), dim3(256), 0, s, d_inst, d_tiles, ntiles, d_ctls); break;
+        case 64: hipLaunchKernelGGL((k_rigid<Real, 64>), dim3(R.nt), dim3(64), 0, s, d_inst, items, R.nt, d_ctls); break;
+        case 128: hipLaunchKernelGGL((k_rigid<Real, 128>), dim3(R.nt), dim3(128), 0, s, d_inst, items, R.nt, d_ctls); break;
+        default: hipLaunchKernelGGL((k_rigid<Real, 256>), dim3(R.nt), dim3(256), 0, s, d_inst, items, R.nt, d_ctls); break;
       }
     }
   }
-  void launch_iteration(hipStream_t s) { launch_cross(s); launch_strand(s); launch_ctrl(s); }
+  void launch_cross(hipStream_t s) { launch_cross(s, all_range()); }
+  void launch_strand(hipStream_t s) { launch_strand(s, all_range()); }
+  void launch_ctrl(hipStream_t s) { launch_ctrl(s, all_range()); }
+  void launch_iteration(hipStream_t s, const Range& R) { launch_cross(s, R); launch_strand(s, R); launch_ctrl(s, R); }
 
   void drop_graphs() {
-    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
-    graphs.clear();
+    for (auto* m : {&graphs, &graphs2}) {
+      for (auto& g : *m) (void)hipGraphExecDestroy(g.second);
+      m->clear();
+    }
   }
-  // the graph of `len` iterations, captured, instantiated and uploaded on first use
-  hipGraphExec_t graph_of(int len) {
-    auto f = graphs.find(len);
-    if (f != graphs.end()) return f->second;
-    hipGraph_t g;
-    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    for (int it = 0; it < len; it++) launch_iteration(stream);
-    HIPCHK(hipStreamEndCapture(stream, &g));
+  // the graph of `len` iterations of group g (the whole batch with one stream), captured on that group's stream,
+  // instantiated and uploaded on first use
+  hipGraphExec_t graph_of(int len, int g = 0) {
+    auto& m = g ? graphs2 : graphs;
+    auto f = m.find(len);
+    if (f != m.end()) return f->second;
+    hipStream_t s = g ? stream2 : stream;
+    const Range R = nstreams == 1 ? all_range() : group[g];
+    hipGraph_t gr;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int it = 0; it < len; it++) launch_iteration(s, R);
+    HIPCHK(hipStreamEndCapture(s, &gr));
     hipGraphExec_t e = nullptr;
-    const hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
+    const hipError_t r = hipGraphInstantiate(&e, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
     HIPCHK(r);
-    graphs[len] = e;
-    HIPCHK(hipGraphUpload(e, stream));
+    m[len] = e;
+    HIPCHK(hipGraphUpload(e, s));
     return e;
   }
   void ensure_graphs() {
     if (!eager_graphs || graphs.count(per_submit)) return;
-    graph_of(per_submit);
-    for (int len = 1; len < per_submit; len *= 2) graph_of(len);
+    for (int g = 0; g < nstreams; g++) {
+      graph_of(per_submit, g);
+      for (int len = 1; len < per_submit; len *= 2) graph_of(len, g);
+    }
   }
 
   void fetch_ctls() {
@@ -725,11 +788,15 @@ struct Batch : BatchBase {
         ms_cross = sc / n_iter; ms_strand = ss / n_iter; ms_ctrl = sk / n_iter;
         for (auto& e : ev) (void)hipEventDestroy(e);
       } else {
+        // with two groups, group 1's stream starts after k_begin and `stream` waits for it before the step ends
+        const int ns = nstreams;
+        hipStream_t ss[2] = {stream, stream2};
+        if (ns == 2) { HIPCHK(hipEventRecord(ev_fork, stream)); HIPCHK(hipStreamWaitEvent(stream2, ev_fork, 0)); }
         const int full = n_iter / per_submit, rem = n_iter % per_submit;
         for (int r = 0; r < full; r++) {
-          HIPCHK(hipGraphLaunch(graph_of(per_submit), stream));
+          for (int g = 0; g < ns; g++) HIPCHK(hipGraphLaunch(graph_of(per_submit, g), ss[g]));
           if (full > 8 && (r % 8) == 7 && r + 1 < full) {
-            HIPCHK(hipStreamSynchronize(stream));
+            for (int g = 0; g < ns; g++) HIPCHK(hipStreamSynchronize(ss[g]));
             if (all_done(sel.data(), int(sel.size()))) break;
           }
         }
@@ -737,7 +804,8 @@ struct Batch : BatchBase {
         int hb = 1;
         while (hb * 2 <= rem) hb *= 2;
         for (int len = hb; rem > 0 && len >= 1; len /= 2)
-          if (rem & len) HIPCHK(hipGraphLaunch(graph_of(len), stream));
+          if (rem & len) for (int g = 0; g < ns; g++) HIPCHK(hipGraphLaunch(graph_of(len, g), ss[g]));
+        if (ns == 2) { HIPCHK(hipEventRecord(ev_join, stream2)); HIPCHK(hipStreamWaitEvent(stream, ev_join, 0)); }
       }
       HIPCHK(hipEventRecord(ev1, stream));
       HIPCHK(hipStreamSynchronize(stream));
@@ -1115,7 +1183,7 @@ int fail(ws_ctx* ctx, const Fail& f) { if (ctx) ctx->err = f.msg; return f.code;
 // the batch of the given options (tile already resolved to 64 / 128 / 256)
 std::unique_ptr<BatchBase> make_batch(ws_ctx* ctx, const ws_model* const* models, const ws_state* const* states, int B,
                                       int prec, int tile, int per_submit, int history, int ctrl_chunk = 0,
-                                      int cross_lanes = 0, bool fused_ctrl = false) {
+                                      int cross_lanes = 0, bool fused_ctrl = false, int streams = 0) {
   std::unique_ptr<BatchBase> b;
   if (prec == WS_FP64) b.reset(new Batch<double>()); else b.reset(new Batch<float>());
   b->ctx = ctx; b->precision = prec;
@@ -1124,6 +1192,7 @@ std::unique_ptr<BatchBase> make_batch(ws_ctx* ctx, const ws_model* const* models
   b->ctrl_chunk = std::max(ctrl_chunk, 0);
   b->cross_lanes = cross_lanes;
   b->fused_ctrl = fused_ctrl;
+  b->streams_opt = streams;
   b->tile = tile;
   if (prec == WS_FP64) static_cast<Batch<double>*>(b.get())->create(models, states, B);
   else static_cast<Batch<float>*>(b.get())->create(models, states, B);

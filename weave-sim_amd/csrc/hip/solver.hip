@@ -120,7 +120,9 @@ WS_API int ws_batch_create(ws_ctx* ctx, const ws_model* const* models, const ws_
     }
     const int cm = opts ? opts->ctrl_mode : 0;
     if (cm < 0 || cm > 1) throw Fail{WS_ERR_ARG, "ctrl_mode must be 0 (a control launch) or 1 (in the strand launch)"};
-    *out = new ws_batch{make_batch(ctx, models, states, B, prec, t, ps, hist, opts ? opts->ctrl_chunk : 0, lanes, cm == 1)};
+    const int nsr = opts ? opts->streams : 0;
+    if (nsr < 0 || nsr > 2) throw Fail{WS_ERR_ARG, "streams must be 0 (by size), 1 or 2"};
+    *out = new ws_batch{make_batch(ctx, models, states, B, prec, t, ps, hist, opts ? opts->ctrl_chunk : 0, lanes, cm == 1, nsr)};
     return WS_OK;
   } catch (const Fail& f) { return fail(ctx, f); }
 }
@@ -149,6 +151,7 @@ WS_API int ws_batch_get_opts(ws_batch* batch, ws_batch_opts* out) {
   out->tile = batch->impl->tile; out->history = batch->impl->hist_len; out->ctrl_chunk = batch->impl->ctrl_chunk;
   out->cross_lanes = batch->impl->lanes;
   out->ctrl_mode = batch->impl->fused_ctrl ? 1 : 0;
+  out->streams = batch->impl->nstreams;
   return WS_OK;
 }
 

@@ -676,6 +676,7 @@ napi_value BatchCreate(napi_env env, napi_callback_info info) {
   o.ctrl_chunk = int32_t(getd(env, a[2], "ctrlChunk", 0));
   o.cross_lanes = int32_t(getd(env, a[2], "crossLanes", 0));
   o.ctrl_mode = int32_t(getd(env, a[2], "ctrlMode", 0));
+  o.streams = int32_t(getd(env, a[2], "streams", 0));
   std::vector<const ws_model*> mp; std::vector<const ws_state*> sp;
   for (Model* m : ms) { mp.push_back(&m->md); sp.push_back(&m->st); }
   Batch* b = new Batch(); b->ctx = ctx; b->B = int32_t(ms.size());
@@ -702,7 +703,8 @@ napi_value BatchOpts(napi_env env, napi_callback_info info) {
   napi_value r; napi_create_object(env, &r);
   set(env, r, "precision", i32(env, o.precision)); set(env, r, "perSubmit", i32(env, o.per_submit));
   set(env, r, "tile", i32(env, o.tile)); set(env, r, "history", i32(env, o.history));
-  set(env, r, "crossLanes", i32(env, o.cross_lanes));
+  set(env, r, "crossLanes", i32(env, o.cross_lanes)); set(env, r, "ctrlMode", i32(env, o.ctrl_mode));
+  set(env, r, "streams", i32(env, o.streams));
   return r;
 }
 

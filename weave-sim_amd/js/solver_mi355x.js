@@ -217,7 +217,8 @@ class Chain {
 class GpuSolver {
   /** new GpuSolver(device, models, { perSubmit, tile, precision }) (solver_gpu.js:65-123).  models: the reference's
    *  CpuSolver objects, or models of this package's builder; instances may differ in topology; precision 'fp32'
-   *  (FP32 forces, FP64 positions / frames / energy / residual) or 'fp64' */
+   *  (FP32 forces, FP64 positions / frames / energy / residual) or 'fp64'; also crossLanes, ctrlMode and streams
+   *  (include/weave.h ws_batch_opts) */
   constructor(device, models, opts = {}) {
     if (!device || !device.ctx) throw new Error('GpuSolver: no device');
     this.dev = device;
@@ -227,7 +228,7 @@ class GpuSolver {
     const precision = opts.precision === 'fp64' ? FP64 : FP32_FP64;
     this.h = native().batchCreate(device.ctx, this.inst.map((m) => m.h),
       { precision, perSubmit: opts.perSubmit || 0, tile: opts.tile === 'autotune' ? -1 : (opts.tile || 0), history: opts.history || 0,
-        crossLanes: opts.crossLanes || 0 });
+        crossLanes: opts.crossLanes || 0, ctrlMode: opts.ctrlMode || 0, streams: opts.streams || 0 });
     this.perSubmit = native().batchOpts(this.h).perSubmit;
     this.ctlHost = new Array(this.B).fill(null);
     this._chain = new Chain();

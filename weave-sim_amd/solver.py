@@ -89,11 +89,12 @@ class GpuSolver:
     """B instances (each its own topology allowed) relaxed in the same launches."""
 
     def __init__(self, models, precision="fp32", tile=0, per_submit=0, history=4096, ctx=None, states=None, ctrl_chunk=0,
-                 cross_lanes=0, ctrl_mode=0):
+                 cross_lanes=0, ctrl_mode=0, streams=0):
         """tile: 64 / 128 / 256 lanes per strand tile, 0 = the default (64), "autotune" (or -1) = the autotuner's pick;
         per_submit: iterations per captured graph (0 = 64, or the autotuner's pick);
         cross_lanes: lanes per crossing in the contact kernel, 1 or 2 (0 = by size, or the autotuner's pick);
-        ctrl_mode: 0 = a control launch after the strand launch, 1 = the strand launch runs the control step"""
+        ctrl_mode: 0 = a control launch after the strand launch, 1 = the strand launch runs the control step;
+        streams: 1 or 2 streams of launches (two groups of instances overlapping), 0 = by size"""
         self.ctx = ctx or default_context()
         self.lib = self.ctx.lib
         self.models = list(models)
@@ -104,7 +105,8 @@ class GpuSolver:
         sp = (C.POINTER(A.ws_state) * self.B)(*[C.pointer(s.st) for s in sts])
         if tile == "autotune":
             tile = A.WS_TILE_AUTOTUNE
-        opts = A.ws_batch_opts(PRECISIONS[precision], per_submit, tile, history, ctrl_chunk, cross_lanes, ctrl_mode)
+        opts = A.ws_batch_opts(PRECISIONS[precision], per_submit, tile, history, ctrl_chunk, cross_lanes, ctrl_mode,
+                               streams)
         self.h = C.c_void_p()
         self.ctx.check(self.lib.ws_batch_create(self.ctx.h, mp, sp, self.B, C.byref(opts), C.byref(self.h)))
         self._ctl = (A.ws_control * self.B)()
@@ -116,7 +118,8 @@ class GpuSolver:
         o = A.ws_batch_opts()
         self.ctx.check(self.lib.ws_batch_get_opts(self.h, C.byref(o)))
         return {"precision": o.precision, "per_submit": o.per_submit, "tile": o.tile, "history": o.history,
-                "ctrl_chunk": o.ctrl_chunk, "cross_lanes": o.cross_lanes, "ctrl_mode": o.ctrl_mode}
+                "ctrl_chunk": o.ctrl_chunk, "cross_lanes": o.cross_lanes, "ctrl_mode": o.ctrl_mode,
+                "streams": o.streams}
 
     @property
     def tile(self):
