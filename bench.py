@@ -181,12 +181,13 @@ def solver_opts(args):
 
 
 def kernel_only(args):
-    """the workload of one counter pass: the calibration copy, then warm-up iterations and strand launches"""
+    """the workload of one counter pass: the calibration copy, then warm-up iterations and strand launches — on one
+    stream, so that every dispatch covers the whole batch (per-dispatch counters are then per launch of the line)"""
     W = importlib.import_module("weave-sim_amd")
     ctx = W.solver.default_context()
     ctx.copy_probe(COPY_BYTES, 3)
     models = build_models(args.workload, args.batch)
-    g = W.GpuSolver(models, precision=args.precision, **solver_opts(args))
+    g = W.GpuSolver(models, precision=args.precision, **dict(solver_opts(args), streams=1))
     g.step(64)
     g.kernel_bench(10)
     g.close()
