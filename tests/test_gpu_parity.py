@@ -114,6 +114,29 @@ def test_dome_trajectory_window(W, gpu_ctx, precision, tol, window, lanes):
             assert abs(c["resid"] - o.residual) <= 1e-3 * o.residual + (1e-6 if precision == "fp64" else 1e-3)
 
 
+@pytest.mark.parametrize("precision,tol_f,tol_c,mism", [("fp64", 1e-6, 1e-6, 0), ("fp32", 1e-3, 1e-2, 0.01)])
+def test_dome_classification_window(W, gpu_ctx, precision, tol_f, tol_c, mism):
+    """the W1 outputs of every crossing of the 3 m dome (solver_cpu.js:231-271) and its contact layers through the
+    first 600 iterations against the oracle: contact type exactly (FP32: at most 1 % of the crossings, where θ or
+    the roll sits on an edge threshold); Fn and gap within tol_f of each layer's largest value, Ft within tol_f of the
+    largest Fn; θ, roll angles, nominal area, pressure and utilisation within tol_c (the FP32+FP64 form classifies in
+    FP32, and an edge contact's area follows its small roll angle: FP32 pressures sit ~1.5e-3 of the largest off)"""
+    m = cases.config1()
+    for n, a, b, o, c in run_pair(W, m, [100, 300, 600], precision):
+        same = a["ctype"] == b["ctype"]
+        assert (~same).sum() <= mism * m.K, (n, int((~same).sum()))
+        for k in ("theta", "phi", "Anom", "pressure", "util", "Fn", "Ft", "gap"):
+            x, y = a[k], b[k]
+            if k in ("Anom", "pressure", "util", "phi"):     # set by the contact type
+                sel = np.repeat(same, 2) if k == "phi" else same
+                x, y = x[sel], y[sel]
+            # the friction force against the normal force's scale (|Ft| ≤ μ Fn; tiny while the contacts ramp in)
+            scale = max(np.abs(a["Fn"] if k == "Ft" else x).max(), 1e-9)
+            tol = tol_f if k in ("Fn", "Ft", "gap") else tol_c
+            assert np.abs(x - y).max() <= tol * scale, (n, k, np.abs(x - y).max(), scale)
+        assert (a["Fn"] > 0).sum() > 0.25 * m.K            # a third of the crossings press by iteration 100
+
+
 def test_dome_divergence_is_chaos_not_error(W, gpu_ctx):
     """past the chaotic horizon the device FP64 trajectory stays within the spread of a 1e-12-perturbed oracle run
     (the reference's perturbed-CPU yardstick, gpu_parity.html:5-13) — or both stay within 1e-4 span"""
