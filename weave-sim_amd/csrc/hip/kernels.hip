@@ -784,7 +784,7 @@ __device__ __forceinline__ Real relrot_R(double aw, double ax, double ay, double
     return th;
   } else {
     float xf = float(x), yf = float(y), zf = float(z), wf = float(w);
-    const float s2 = xf * xf + yf * yf + zf * zf, s = sqrtf(s2);
+    const float s2 = xf * xf + yf * yf + zf * zf, s = __builtin_amdgcn_sqrtf(s2);   // hardware square root (1 ulp)
     const float iw = rcp_(fmaxf(wf, 1e-12f));
     const float r2 = s2 * iw * iw;                 // (|v| / w)²: w ≥ 0 here
     float f, th;
@@ -801,18 +801,23 @@ __device__ __forceinline__ Real relrot_R(double aw, double ax, double ay, double
   }
 }
 
+// R(q)ᵀ v: the components of v along the axes X | Y | Z of the frame of q/|q| (qframe's columns).  With s = 2/|q|²
+// the frame is I + s(w [q]× + [q]×²), so Rᵀ v = v + s (q × (q × v) − w q × v): two cross products instead of the nine
+// matrix entries
+template <typename Real>
+__device__ __forceinline__ void rotT_q(Real w, Real x, Real y, Real z, const Real* v, Real* out) {
+  const Real n2 = w * w + x * x + y * y + z * z;
+  const Real s = n2 > Real(0) ? Real(2) * rcp_(n2) : Real(2);
+  const Real c0 = y * v[2] - z * v[1], c1 = z * v[0] - x * v[2], c2 = x * v[1] - y * v[0];
+  const Real e0 = y * c2 - z * c1, e1 = z * c0 - x * c2, e2 = x * c1 - y * c0;
+  out[0] = v[0] + s * (e0 - w * c0);
+  out[1] = v[1] + s * (e1 - w * c1);
+  out[2] = v[2] + s * (e2 - w * c2);
+}
 // R(d)ᵀ v of a relative rotation d (unit up to rounding)
 template <typename Real>
 __device__ __forceinline__ void rotT2(const double* d, const Real* v, Real* out) {
-  Real w = Real(d[0]), x = Real(d[1]), y = Real(d[2]), z = Real(d[3]);
-  Real n2 = w * w + x * x + y * y + z * z;
-  Real s = n2 > Real(0) ? Real(2) * rcp_(n2) : Real(2);
-  Real R0 = 1 - s * (y * y + z * z), R1 = s * (x * y + z * w), R2 = s * (x * z - y * w);
-  Real R3 = s * (x * y - z * w), R4 = 1 - s * (x * x + z * z), R5 = s * (y * z + x * w);
-  Real R6 = s * (x * z + y * w), R7 = s * (y * z - x * w), R8 = 1 - s * (x * x + y * y);
-  out[0] = R0 * v[0] + R1 * v[1] + R2 * v[2];
-  out[1] = R3 * v[0] + R4 * v[1] + R5 * v[2];
-  out[2] = R6 * v[0] + R7 * v[1] + R8 * v[2];
+  rotT_q(Real(d[0]), Real(d[1]), Real(d[2]), Real(d[3]), v, out);
 }
 
 // one segment's stretch / shear (CosseratRod.apply, rod_cosserat.js:118-133): world force sf; body torque
@@ -1199,11 +1204,10 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
       }
     }
     if (Tc[0] != Real(0) || Tc[1] != Real(0) || Tc[2] != Real(0)) {
-      double Rs[9];
-      qframe_d(L.Qw[t], L.Qx[t], L.Qy[t], L.Qz[t], Rs);
-      tm[0] += Real(Rs[0]) * Tc[0] + Real(Rs[1]) * Tc[1] + Real(Rs[2]) * Tc[2];
-      tm[1] += Real(Rs[3]) * Tc[0] + Real(Rs[4]) * Tc[1] + Real(Rs[5]) * Tc[2];
-      tm[2] += Real(Rs[6]) * Tc[0] + Real(Rs[7]) * Tc[1] + Real(Rs[8]) * Tc[2];
+      // the crossings' world torques on the segment's body axes (in Real: the torque is Real)
+      Real tb[3];
+      rotT_q(Real(L.Qw[t]), Real(L.Qx[t]), Real(L.Qy[t]), Real(L.Qz[t]), Tc, tb);
+      tm[0] += tb[0]; tm[1] += tb[1]; tm[2] += tb[2];
     }
     mxT = double(tm[0]) * double(tm[0]) + double(tm[1]) * double(tm[1]) + double(tm[2]) * double(tm[2]);   // squared
     if (OUT && out.T) {
