@@ -1,0 +1,6 @@
+// the batch C ABI (solver.hip) over the CPU stand-in runtime and the emulated kernels (test infrastructure)
+#include <hip/hip_runtime.h>
+#include "../../weave-sim_amd/csrc/hip/batch.hpp"
+#define HOSTSTUB_DEFINE_COPY
+#include "emu_kernels.hpp"
+#include "../../weave-sim_amd/csrc/hip/solver.hip"

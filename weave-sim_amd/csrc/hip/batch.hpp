@@ -1247,6 +1247,3 @@ void autotune(ws_ctx* ctx, const ws_model* const* models, const ws_state* const*
 
 }  // namespace
 
-// a batch handle: calls on one handle are serialised by its mutex (a JS host may issue one from the main thread while
-// another runs on a worker)
-struct ws_batch { std::unique_ptr<BatchBase> impl; std::mutex mu; };

@@ -4,6 +4,10 @@
 
 #include <mutex>
 
+// a batch handle: calls on one handle are serialised by its mutex (a JS host may issue one from the main thread while
+// another runs on a worker).  Defined in this file only: its member's type lives in batch.hpp's per-file namespace.
+struct ws_batch { std::unique_ptr<BatchBase> impl; std::mutex mu; };
+
 namespace {
 
 // The staging region of ws_download_noalloc: section forces, moments, reactions and pending AL multipliers of one
