@@ -193,8 +193,9 @@ typedef struct ws_batch_opts {
                               takes both sides, fewer instructions per crossing), 0 = by size (1 from 131,072
                               crossings in the batch) or the autotuner's pick */
   int32_t ctrl_mode;       /* 0: a control launch after the strand launch (the default); 1: the strand launch's last
-                              tiles per instance run the control step (no control launch; slower on gfx950, where
-                              every tile's device-scope release writes back its L2) */
+                              tiles per instance run the control step, no control launch (the partials written
+                              through the L2 with agent-scope stores and read with agent-scope loads; a UTM replay
+                              instance releases them with device-scope fences).  Slower than 0 on gfx950 */
   int32_t streams;         /* 1: every launch on the batch's stream; 2: the instances in two groups of about equal
                               work, each replaying its own iteration graphs on its own stream (one group's
                               latency-bound launches overlap the other's strand launch; results as with one);

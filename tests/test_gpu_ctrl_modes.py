@@ -1,8 +1,10 @@
-"""The two forms of the control step (solver_cpu.js:809-837: partial sums reduced, kinetic damping, residual, stop
+"""The forms of the control step (solver_cpu.js:809-837: partial sums reduced, kinetic damping, residual, stop
 test, history): a control launch after the strand launch (ctrl_mode 0, the default) and the strand launch's last
-tiles of each instance running it (ctrl_mode 1).  Both against the oracle, on instances whose strand partials fit one
+tiles of each instance running it (ctrl_mode 1: the partials written through the L2, or fenced on a UTM replay, whose
+control step reads the probe's new position).  Each against the oracle, on instances whose strand partials fit one
 reduction group and on instances split over several (the 10 m dome: ~1,400 tiles; the fused form's groups are 256
-tiles), on a batch of mixed sizes and on the unanchored lath whose rigid-body step the control step also carries."""
+tiles), on a batch of mixed sizes, on the unanchored lath whose rigid-body step the control step also carries, and on
+replays batched with a plain relaxation."""
 import numpy as np
 import pytest
 
@@ -42,25 +44,26 @@ def test_control_forms_match_the_oracle(W, gpu_ctx, scene, precision, tol, mode)
 
 
 def test_control_forms_agree_on_a_mixed_batch(W, gpu_ctx):
-    """six small domes and the 3 m dome in one batch, both forms: the same states to the reduction-order level, and
-    each instance of the fused form bitwise as that instance alone in the fused form"""
+    """six small domes and the 3 m dome in one batch, the control launch and a fused form: the same states to the
+    reduction-order level, and each instance of the fused form bitwise as that instance alone in the fused form"""
     models = [cases.config_small(seed=s) for s in range(1, 7)] + [cases.config1()]
     st = {}
-    for mode in (0, 1):
+    fused = 1
+    for mode in (0, fused):
         g = W.GpuSolver([mm.clone() for mm in models], precision="fp64", tile=64, ctrl_mode=mode)
         g.step(500)
         st[mode] = [g.state(b) for b in range(len(models))]
         g.close()
     for b, mm in enumerate(models):
-        assert np.abs(st[0][b]["X"] - st[1][b]["X"]).max() <= 1e-9 * mm.span, b
-        assert np.array_equal(st[0][b]["stuck"], st[1][b]["stuck"]), b
+        assert np.abs(st[0][b]["X"] - st[fused][b]["X"]).max() <= 1e-9 * mm.span, b
+        assert np.array_equal(st[0][b]["stuck"], st[fused][b]["stuck"]), b
     for b in (0, 6):
-        g = W.GpuSolver([models[b].clone()], precision="fp64", tile=64, ctrl_mode=1)
+        g = W.GpuSolver([models[b].clone()], precision="fp64", tile=64, ctrl_mode=fused)
         g.step(500)
         alone = g.state(0)
         g.close()
         for k in ("X", "Q", "u", "stuck"):
-            assert np.array_equal(alone[k], st[1][b][k]), (b, k)
+            assert np.array_equal(alone[k], st[fused][b][k]), (b, k)
 
 
 @pytest.mark.parametrize("mode", [0, 1])
@@ -83,3 +86,28 @@ def test_ctrl_mode_out_of_range_is_refused(W, gpu_ctx):
     with pytest.raises(W.DeviceError) as e:
         W.GpuSolver([cases.config_small(seed=1)], ctrl_mode=2)
     assert "ctrl_mode" in str(e.value)
+
+
+def test_control_forms_run_the_replays(W, gpu_ctx):
+    """a force-control and a displacement-control cantilever replay batched with a plain dome, control in the strand
+    launch: the replays' read-outs (travel, reaction, value) as with the control launch.  The replay instances take
+    the fenced path (their control step reads the probe's new position) and the dome the write-through one, in the
+    same launches."""
+    mode = 1
+    from test_gpu_replay import CANT_D, CANT_F, cant
+    outs = {}
+    for m_ in (0, mode):
+        models = [cant().set_protocol(*CANT_F), cant().set_protocol(*CANT_D), cases.config_small()]
+        g = W.GpuSolver(models, precision="fp64", ctrl_mode=m_)
+        for _ in range(200):     # every step is forced (clears `done`), so a finished replay is told by its phase
+            if all(c["phase"] == 2 for c in g.step(2000)[:2]):
+                break
+        ctl = g.read_control()
+        outs[m_] = [g.read_outputs(b) for b in range(2)]
+        g.close()
+        assert all(c["phase"] == 2 and c["inc"] == n for c, n in zip(ctl, (len(CANT_F[3]), len(CANT_D[3])))), ctl[:2]
+    for a, b in zip(outs[0], outs[mode]):
+        assert a.shape == b.shape
+        assert np.array_equal(a[:, 2], b[:, 2])
+        assert np.abs(a[:, 0] - b[:, 0]).max() <= 1e-9 * 1000.0
+        assert np.abs(a[:, 1] - b[:, 1]).max() <= 1e-6 * max(np.abs(a[:, 1]).max(), 1.0)

@@ -421,7 +421,7 @@ struct BatchBase {
   ws_ctx* ctx = nullptr;
   int B = 0, precision = 0, tile = 128, per_submit = 64, hist_len = 4096;
   int ctrl_chunk = 0;                   // strand partials per control workgroup (0: by size)
-  bool fused_ctrl = false;              // the strand launch runs the control step (else the control kernel)
+  int ctrl_mode = 0;                    // 0: the control kernel; 1: the strand launch runs the control step
   bool has_rigid = false;               // an instance that only the ground holds: the rigid step runs
   int cross_lanes = 0;                  // lanes per crossing (0: by size at every relist)
   int lanes = 2;                        // ... as the current work lists use it
@@ -514,7 +514,7 @@ struct Batch : BatchBase {
     HIPCHK(hipMemset(d_ctls, 0, sizeof(Ctl) * std::max(B, 1)));
     for (int b = 0; b < B; b++) {
       inst.emplace_back(new InstHost<Real>());
-      build_instance(*inst.back(), *models[b], *states[b], tile, hist_len, d_ctls + b, ctrl_chunk, false, fused_ctrl);
+      build_instance(*inst.back(), *models[b], *states[b], tile, hist_len, d_ctls + b, ctrl_chunk, false, ctrl_mode != 0);
       upload_state(*inst.back(), *states[b]);
     }
     relist();
@@ -540,7 +540,7 @@ struct Batch : BatchBase {
     if (slot < 0 || slot >= B || inst[slot]) throw Fail{WS_ERR_ARG, "add_instance: slot out of range or occupied"};
     HIPCHK(hipStreamSynchronize(stream));
     std::unique_ptr<InstHost<Real>> h(new InstHost<Real>());
-    build_instance(*h, md, st, tile, hist_len, d_ctls + slot, ctrl_chunk, false, fused_ctrl);
+    build_instance(*h, md, st, tile, hist_len, d_ctls + slot, ctrl_chunk, false, ctrl_mode != 0);
     upload_state(*h, st);
     inst[slot] = std::move(h);
   }
@@ -660,7 +660,7 @@ struct Batch : BatchBase {
     else hipLaunchKernelGGL((k_strand<Real, T, RB, false, W, F>), dim3(R.nt), dim3(T), 0, s, d_inst, items, R.nt, d_ctls);
   }
   template <int T, bool RB, int W> void strand_w(hipStream_t s, const Range& R) {
-    if (fused_ctrl) strand_wf<T, RB, W, true>(s, R); else strand_wf<T, RB, W, false>(s, R);
+    if (ctrl_mode == 1) strand_wf<T, RB, W, true>(s, R); else strand_wf<T, RB, W, false>(s, R);
   }
   // the FP64 form of the 64-lane tile runs three resident wavefronts per SIMD once the batch has enough tiles to keep
   // them busy (measured: 10 % faster on the 30 m dome's 14k tiles, 4 % slower on the 10 m dome's 1,350)
@@ -678,7 +678,7 @@ struct Batch : BatchBase {
     }
   }
   void launch_ctrl(hipStream_t s, const Range& R) {
-    if (R.nk && !fused_ctrl) {
+    if (R.nk && ctrl_mode == 0) {
       const CtrlItem* items = d_ctrl + R.k0;
       if (one_inst()) hipLaunchKernelGGL((k_ctrl<Real, true>), dim3(R.nk), dim3(CTRL_THREADS), 0, s, d_inst, items, R.nk, d_ctls);
       else hipLaunchKernelGGL((k_ctrl<Real, false>), dim3(R.nk), dim3(CTRL_THREADS), 0, s, d_inst, items, R.nk, d_ctls);
@@ -1092,7 +1092,7 @@ struct Batch : BatchBase {
     HIPCHK(hipStreamSynchronize(stream));
     H.arena.release();
     H.arena = Arena();
-    build_instance(H, md, st, tile, hist_len, d_ctls + b, ctrl_chunk, false, fused_ctrl);
+    build_instance(H, md, st, tile, hist_len, d_ctls + b, ctrl_chunk, false, ctrl_mode != 0);
     upload_state(H, st);
     relist();
     return WS_OK;
@@ -1183,7 +1183,7 @@ int fail(ws_ctx* ctx, const Fail& f) { if (ctx) ctx->err = f.msg; return f.code;
 // the batch of the given options (tile already resolved to 64 / 128 / 256)
 std::unique_ptr<BatchBase> make_batch(ws_ctx* ctx, const ws_model* const* models, const ws_state* const* states, int B,
                                       int prec, int tile, int per_submit, int history, int ctrl_chunk = 0,
-                                      int cross_lanes = 0, bool fused_ctrl = false, int streams = 0) {
+                                      int cross_lanes = 0, int ctrl_mode = 0, int streams = 0) {
   std::unique_ptr<BatchBase> b;
   if (prec == WS_FP64) b.reset(new Batch<double>()); else b.reset(new Batch<float>());
   b->ctx = ctx; b->precision = prec;
@@ -1191,7 +1191,7 @@ std::unique_ptr<BatchBase> make_batch(ws_ctx* ctx, const ws_model* const* models
   if (history > 0) b->hist_len = history;
   b->ctrl_chunk = std::max(ctrl_chunk, 0);
   b->cross_lanes = cross_lanes;
-  b->fused_ctrl = fused_ctrl;
+  b->ctrl_mode = ctrl_mode;
   b->streams_opt = streams;
   b->tile = tile;
   if (prec == WS_FP64) static_cast<Batch<double>*>(b.get())->create(models, states, B);

@@ -866,6 +866,15 @@ __device__ __forceinline__ void joint_torque(StrandLDS<Real, TILE>& L, int i, Re
 template <typename Real, int TILE, bool RB>
 __device__ void strand_control_tail(const Inst<Real>& I, Ctl* ctl, const TileItem& tl, StrandLDS<Real, TILE>& L);
 
+// a value the control step reduces (a tile's partial, a rigid-body sum).  When the strand launch runs the control
+// step (FUSED) it is an agent-scope store, written through the L2, so the tile that reduces it reads it from any XCD
+// with agent-scope loads and no tile has to write back its L2; otherwise a plain store.
+template <bool FUSED>
+__device__ __forceinline__ void part_store(double* p, double x) {
+  if (FUSED) __hip_atomic_store(gl(p), x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else *gl(p) = x;
+}
+
 // FUSED: the control step runs in this launch, in the last tile of the instance to finish (strand_control_tail)
 template <typename Real, int TILE, bool OUT, bool RB, bool FUSED = false>
 __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem& tl, StrandLDS<Real, TILE>& L,
@@ -1293,24 +1302,24 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
     };
     const int q = lane >> 3;
     const double x = group(q, q < 5);
-    if (k == 0) gl(I.spart)[size_t(tl.tile_local) * SPART_STRIDE + q] = x;   // [tile][value]: vector loads in k_ctrl
+    if (k == 0) part_store<FUSED>(I.spart + size_t(tl.tile_local) * SPART_STRIDE + q, x);   // [tile][value]: vector loads in k_ctrl
     if (I.floor_on) {
       const double g = group(8, true);
-      if (lane == 0) gl(I.spart)[size_t(tl.tile_local) * SPART_STRIDE + 8] = g;
+      if (lane == 0) part_store<FUSED>(I.spart + size_t(tl.tile_local) * SPART_STRIDE + 8, g);
     }
   }
   if (I.floor_on) {
     // the tile's largest ground Fn (an exact maximum, so any order): each wavefront's, then the wavefronts'
     const double g = wave_reduce63<false>(mxG);
     if (TILE == 64) {
-      if (lane == 63) gl(I.spart)[size_t(tl.tile_local) * SPART_STRIDE + 9] = g;
+      if (lane == 63) part_store<FUSED>(I.spart + size_t(tl.tile_local) * SPART_STRIDE + 9, g);
     } else {
       if (lane == 63) L.gmx[wv] = g;
       __syncthreads();
       if (tid == 0) {
         double x = L.gmx[0];
         for (int w = 1; w < TILE / 64; w++) x = fmax(x, L.gmx[w]);
-        gl(I.spart)[size_t(tl.tile_local) * SPART_STRIDE + 9] = x;
+        part_store<FUSED>(I.spart + size_t(tl.tile_local) * SPART_STRIDE + 9, x);
       }
     }
   }
@@ -1325,7 +1334,7 @@ __device__ void strand_tile(const Inst<Real>& I, const Ctl* ctl, const TileItem&
     if (tid < RBP) {
       double x = L.rbw[tid][0];
       for (int w = 1; w < TILE / 64; w++) x += L.rbw[tid][w];
-      gl(I.rbpart)[size_t(tid) * I.ntiles + tl.tile_local] = x;
+      part_store<FUSED>(I.rbpart + size_t(tid) * I.ntiles + tl.tile_local, x);
     }
   }
   if (FUSED) strand_control_tail<Real, TILE, RB>(I, const_cast<Ctl*>(ctl), tl, L);
@@ -1472,13 +1481,28 @@ __device__ void control_step(const Inst<Real>& I, Ctl* ctl, Ctl& c, const double
   *ctl = c;
 }
 
+// The release before a tile (or a chunk) counts in and the acquire after its count.  The values the control step
+// reduces were written through the L2 (part_store), so the release only waits for this wavefront's stores to
+// complete (s_waitcnt vmcnt(0), the part of a release that orders agent-scope stores) and the reads — agent-scope
+// loads — need no acquire.  An instance whose control step reads positions this launch wrote (a UTM replay: `fence`)
+// takes device-scope fences, which on gfx950 write back the XCD's L2 (as every tile doing so was measured to cost
+// ~15x the strand launch on the 256 design variants, only those instances pay it).
+constexpr int WAIT_VMEM = 0x0F70;   // s_waitcnt vmcnt(0), expcnt and lgkmcnt left alone (gfx9 encoding)
+__device__ __forceinline__ void part_release(bool fence) {
+  if (fence) __threadfence();
+  else __builtin_amdgcn_s_waitcnt(WAIT_VMEM);
+}
+__device__ __forceinline__ void part_acquire(bool fence) {
+  if (fence) __threadfence();
+}
+
 // The control step fused into the strand launch.  Every tile counts itself into its chunk (ctrl_chunk consecutive
 // tiles of the instance); the last tile of a chunk to finish reduces the chunk's tile partials in tile order, and the
 // chunk's share of the crossing-block partials, into a chunk partial; with several chunks the last chunk to finish
 // combines the chunk partials in chunk order.  That workgroup then runs the control step.  Every order is fixed: the
 // result does not depend on which workgroup arrives last.  Counters are reset by the workgroup that read them last,
-// so the next iteration starts from zero.  Partials of this launch are read with device-scope loads after an
-// acquire fence (the writers release theirs with a fence before counting in).
+// so the next iteration starts from zero.  Partials of this launch are written through the L2 and read with
+// device-scope loads; the writers release theirs before counting in (part_release).
 template <typename Real, int TILE, bool RB>
 __device__ void strand_control_tail(const Inst<Real>& I, Ctl* ctl, const TileItem& tl, StrandLDS<Real, TILE>& L) {
   constexpr int NACC = RPART, NSUM = 7, FPER = 4;
@@ -1490,12 +1514,13 @@ __device__ void strand_control_tail(const Inst<Real>& I, Ctl* ctl, const TileIte
   const int np = I.ntiles, per = I.ctrl_chunk, nch = I.ctrl_chunks;
   const int chunk = tl.tile_local / per;
   const int p0 = chunk * per, p1 = min(np, p0 + per);
-  __threadfence();                                 // release this tile's partial
+  const bool fence = I.proto != 0;
+  part_release(fence);                      // release this tile's partial
   __syncthreads();
   if (tid == 0) last = atomicAdd(I.ccount + chunk, 1) == p1 - p0 - 1;
   __syncthreads();
   if (!last) return;
-  __threadfence();                                 // acquire the chunk's partials
+  part_acquire(fence);                      // acquire the chunk's partials
   auto ld = [](const double* q) { return __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); };
   const bool fon = I.floor_on != 0;
   double acc[NACC];
@@ -1534,13 +1559,13 @@ __device__ void strand_control_tail(const Inst<Real>& I, Ctl* ctl, const TileIte
   if (tid == 0) gl(I.ccount)[chunk] = 0;
   if (nch > 1) {
     if (tid == 0) {
-      for (int q = 0; q < NACC; q++) gl(I.rpart)[size_t(chunk) * RPART + q] = acc[q];
-      __threadfence();                             // release the chunk partial before counting in
+      for (int q = 0; q < NACC; q++) part_store<true>(I.rpart + size_t(chunk) * RPART + q, acc[q]);
+      part_release(fence);                  // release the chunk partial before counting in
       last = atomicAdd(&ctl->counter, 1) == nch - 1;
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();                               // acquire every chunk partial
+    part_acquire(fence);                    // acquire every chunk partial
     double* stage = &L.part[0][0];
     for (int q = 0; q < NACC; q++) acc[q] = 0.0;
     for (int k0 = 0; k0 < nch; k0 += FSTAGE) {

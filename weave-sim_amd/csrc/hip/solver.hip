@@ -126,7 +126,7 @@ WS_API int ws_batch_create(ws_ctx* ctx, const ws_model* const* models, const ws_
     if (cm < 0 || cm > 1) throw Fail{WS_ERR_ARG, "ctrl_mode must be 0 (a control launch) or 1 (in the strand launch)"};
     const int nsr = opts ? opts->streams : 0;
     if (nsr < 0 || nsr > 2) throw Fail{WS_ERR_ARG, "streams must be 0 (by size), 1 or 2"};
-    *out = new ws_batch{make_batch(ctx, models, states, B, prec, t, ps, hist, opts ? opts->ctrl_chunk : 0, lanes, cm == 1, nsr)};
+    *out = new ws_batch{make_batch(ctx, models, states, B, prec, t, ps, hist, opts ? opts->ctrl_chunk : 0, lanes, cm, nsr)};
     return WS_OK;
   } catch (const Fail& f) { return fail(ctx, f); }
 }
@@ -154,7 +154,7 @@ WS_API int ws_batch_get_opts(ws_batch* batch, ws_batch_opts* out) {
   out->precision = batch->impl->precision; out->per_submit = batch->impl->per_submit;
   out->tile = batch->impl->tile; out->history = batch->impl->hist_len; out->ctrl_chunk = batch->impl->ctrl_chunk;
   out->cross_lanes = batch->impl->lanes;
-  out->ctrl_mode = batch->impl->fused_ctrl ? 1 : 0;
+  out->ctrl_mode = batch->impl->ctrl_mode;
   out->streams = batch->impl->nstreams;
   return WS_OK;
 }
