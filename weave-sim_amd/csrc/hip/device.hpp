@@ -37,6 +37,7 @@ constexpr int CROSS_BLOCK = CROSS_THREADS / 2;    // crossings per crossing-kern
 // at the kernel's two-wavefront occupancy (1,024 SIMDs × 2 × 64 lanes); below it the two-lane layout's shorter chains
 // measured faster (10 m dome, 30 m dome, 64 variants)
 constexpr long CROSS_ONE_LANE = 131072;
+constexpr int COPY_PER = 2048;   // 16-byte elements per workgroup of the copy kernel (eight per lane)
 constexpr int MAX_REL = 1024;    // crossing sides a strand tile may gather (checked on the host per strand)
 
 // instance control record, device-written (the reference's CT struct, wgsl.js:154-172)

@@ -1851,8 +1851,25 @@ __global__ __launch_bounds__(256) void k_al_preview(const Inst<Real>* __restrict
   out[c] = (I.aug && I.contacts && ctl->al_update) ? al_updated(I, c, lam, gl(I.penacc)[c], ctl->pen_n) : lam;
 }
 
+// the plain copy the benchmark measures beside the strand kernel (the practical HBM ceiling) and calibrates the
+// traffic counters with: each workgroup moves COPY_PER 16-byte elements, every load of a lane issued before its
+// stores, streaming loads and stores (measured on MI355X at 1 GiB: 6.2 TB/s, where a grid-stride loop of 16,384
+// workgroups reached 4.6)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
-  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) dst[i] = src[i];
+  constexpr int U = COPY_PER / 256;
+  const u32x4* s = reinterpret_cast<const u32x4*>(src);
+  u32x4* d = reinterpret_cast<u32x4*>(dst);
+  const int64_t b0 = int64_t(blockIdx.x) * COPY_PER, base = b0 + threadIdx.x;
+  if (b0 + COPY_PER <= n) {                        // a whole block: every load in flight before the stores
+    u32x4 v[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) v[k] = __builtin_nontemporal_load(&s[base + k * 256]);
+#pragma unroll
+    for (int k = 0; k < U; k++) __builtin_nontemporal_store(v[k], &d[base + k * 256]);
+  } else {                                         // the last, partial block
+    for (int64_t i = base; i < n; i += 256) d[i] = s[i];
+  }
 }
 
 // explicit instantiations

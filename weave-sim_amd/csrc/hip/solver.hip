@@ -86,7 +86,9 @@ WS_API int ws_copy_probe(ws_ctx* ctx, int64_t bytes, int32_t reps, double* ms) {
     HIPCHK(hipMemset(a, 1, n * 16));
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
-    int grid = int(std::min<int64_t>((n + 255) / 256, 256 * 64));
+    const int64_t blocks = (n + COPY_PER - 1) / COPY_PER;
+    if (blocks > INT32_MAX) throw Fail{WS_ERR_ARG, "ws_copy_probe: size too large"};
+    const int grid = int(blocks);
     hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n);
     HIPCHK(hipEventRecord(e0, 0));
     for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, a, b, n);
