@@ -149,3 +149,36 @@ def test_config5_dome_30m_both_forms_against_the_oracle(W, gpu_ctx):
     print(f"\nconfig 5: {sliding} of {m.K} crossings sliding at 600 iterations")
     for prec in ("fp32", "fp64"):
         _check(f"dome_30m {prec} @600", m, a, dev[prec])
+
+
+@pytest.mark.timeout(1200)
+def test_config3_solve_horizon_both_forms_against_the_oracle(W, gpu_ctx):
+    """the solves comparison runs 5,000 iterations per variant: four variants evenly spaced by size (the largest
+    included), batched in the FP32+FP64 form and each alone on the plain FP64 path, both against the oracle at
+    5,000 iterations.  Past the chaotic horizon both forms are held to one yardstick: 1e-4 x span, or 20 x the
+    oracle's own spread under a perturbation of 1e-7 of the span (the FP32 rounding level)."""
+    models = [cases.config3_variant(i) for i in range(256)]
+    order = np.argsort([m.n for m in models], kind="stable")
+    pick = [int(order[round(k * 255 / 3)]) for k in range(4)]
+    assert models[pick[-1]].n == max(m.n for m in models)
+    its = 5000
+    with ThreadPoolExecutor(max_workers=1) as ex:
+        fut = ex.submit(lambda: _oracle_runs([(models[b], its, 0.0) for b in pick] +
+                                             [(models[b], its, 1e-7) for b in pick]))
+        g = W.GpuSolver([models[b] for b in pick], precision="fp32", tile=64)
+        c = g.step(its)
+        assert all(x["iter"] == its for x in c)
+        d32 = [g.state(q) for q in range(len(pick))]
+        g.close()
+        d64 = []
+        for b in pick:
+            g = W.GpuSolver([models[b]], precision="fp64")
+            assert g.step(its)[0]["iter"] == its
+            d64.append(g.state(0))
+            g.close()
+        runs = fut.result()
+    for q, b in enumerate(pick):
+        a, ap = runs[q][0], runs[q + len(pick)][0]
+        spread = float(np.abs(a["X"] - ap["X"]).max())
+        for tag, d in (("fp32+fp64 batched", d32[q]), ("fp64 plain", d64[q])):
+            _check(f"variant {b} {tag} @{its} (perturbed-oracle spread {spread:.2e} mm)", models[b], a, d, spread)
