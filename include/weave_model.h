@@ -48,8 +48,12 @@ typedef struct wsm_scene {
   double udl_q[3];
 } wsm_scene;
 
-enum wsm_selector { WSM_FRAME = 0, WSM_ALL_START = 1, WSM_ALL_END = 2, WSM_ALL_ENDS = 3, WSM_STRAND = 4, WSM_BOUNDARY = 5 };
-typedef struct wsm_anchor { int32_t type; int32_t s, k; int32_t fix[6]; } wsm_anchor;     /* model.js:115-189 */
+enum wsm_selector { WSM_FRAME = 0, WSM_ALL_START = 1, WSM_ALL_END = 2, WSM_ALL_ENDS = 3, WSM_STRAND = 4, WSM_BOUNDARY = 5,
+                    WSM_REGION = 6 };
+/* model.js:115-189.  WSM_FRAME: frame (s, k); WSM_STRAND: strand s; WSM_REGION: the frames of every knot whose mesh
+ * edge lies within `radius` of the knot of frame (s, k), by graph distance along the mesh edges (regionFrames,
+ * model.js:127-145; no frames for a model without a mesh) */
+typedef struct wsm_anchor { int32_t type; int32_t s, k; int32_t fix[6]; int32_t pad; double radius; } wsm_anchor;
 typedef struct wsm_point_load { int32_t s, k; int32_t local; int32_t pad; double F[3]; } wsm_point_load;
 
 /* synthetic preparation applied after buildState and before the goals: seeded out-of-plane noise, a smooth field

@@ -195,9 +195,18 @@ const hy = JSON.parse(require('fs').readFileSync(require('path').join({json.dump
 hy.hybrid.pieces = {{ '3:1': 2 }};
 try {{ S.sceneFromJSON(hy); }} catch (e) {{ refused = e.message; }}
 res.refused = refused;
+const rg = JSON.parse(require('fs').readFileSync(require('path').join({json.dumps(golden)}, 'dome_basic.json'), 'utf8'));
+rg.anchors = [{{ select: {{ type: 'region', id: [3, 2], radius: 900 }}, fix: [1, 1, 1, 0, 0, 0] }}];
+const mr = S.modelFromScene(rg, require('fs').readFileSync(require('path').join({json.dumps(golden)}, rg.mesh.path)), rg.mesh.path);
+res.region = Array.from(mr.info.nodeFree);
 console.log(JSON.stringify(res));""")
     d = json.loads(out)
     assert "pieces" in d["refused"]
+    # the 'region' selector through both readers (model.js:127-145)
+    rg = json.load(open(os.path.join(golden, "dome_basic.json")))
+    rg["anchors"] = [{"select": {"type": "region", "id": [3, 2], "radius": 900}, "fix": [1, 1, 1, 0, 0, 0]}]
+    pr = SC.model_from_scene(rg, base_dir=golden).arr("node_free")
+    assert np.array_equal(np.array(d["region"], dtype=np.uint8), pr) and 0 < (pr == 0).sum() < len(pr)
     for name in names:
         py = SC.load_scene(os.path.join(golden, name + ".json"))
         js = d[name]

@@ -189,10 +189,6 @@ def test_hybrid_pieces_refused_and_one_group_accepted():
     d["hybrid"]["pieces"] = {"3:1": 2}
     with pytest.raises(NotImplementedError):
         SC.scene_from_dict(d)
-    d = scene_json("dome_basic")
-    d["anchors"].append({"select": {"type": "region", "id": [0, 0], "radius": 10.0}, "fix": [1, 1, 1, 0, 0, 0]})
-    with pytest.raises(NotImplementedError):
-        SC.scene_from_dict(d)
     with pytest.raises(ValueError):
         SC.scene_from_dict({"version": 2})
     # a group that leaves a key out takes its default, not the scene's value
@@ -225,3 +221,53 @@ def test_oracle_relaxes_every_scene(name):
     st = o.state()
     assert np.isfinite(st["X"]).all() and np.isfinite(o.residual) and o.residual > 0
     assert np.abs(st["X"] - m["X"]).max() > 0                # it moved once the warm-up ended
+
+
+def _mesh_graph(V, F):
+    """the builder's welded mesh and its edges, restated with numpy: vertices quantised at 1e-9 of the model size and
+    deduplicated in lexicographic order (weld, mesh.js:106-129), edges the sorted unique vertex pairs (edgeTopology,
+    mesh.js:133-150; edge id = row)"""
+    V = V.reshape(-1, 3)
+    q = max(np.ptp(V, axis=0).max(), 1.0) * 1e-9
+    _, first, inv = np.unique(np.round(V / q), axis=0, return_index=True, return_inverse=True)
+    Vw = V[first]
+    Fw = inv.reshape(-1)[F.reshape(-1, 3)]
+    Fw = Fw[(Fw[:, 0] != Fw[:, 1]) & (Fw[:, 1] != Fw[:, 2]) & (Fw[:, 0] != Fw[:, 2])]
+    rows = np.concatenate([np.sort(Fw[:, [(i + 1) % 3, (i + 2) % 3]], axis=1) for i in range(3)])
+    return Vw, np.unique(rows, axis=0)
+
+
+@pytest.mark.parametrize("radius", [0.0, 450.0, 1200.0, 1e12])
+def test_region_selector_by_mesh_graph_distance(radius):
+    """the 'region' anchor (regionFrames, model.js:127-145): the frames of every knot whose mesh edge's two vertices
+    lie within `radius` on average, by shortest paths along the mesh edges from the seed knot's edge.  Held to a
+    restatement with SciPy's Dijkstra on the numpy-welded mesh; radius 0 anchors the seed knot alone and an
+    unbounded radius every knot"""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import dijkstra
+    d = scene_json("dome_basic")
+    V, F = SC.read_mesh(os.path.join(HERE, d["mesh"]["path"]), d["mesh"]["scale"])
+    Vw, E = _mesh_graph(V, F)
+    probe = SC.load_scene(os.path.join(HERE, "dome_basic.json"))
+    kn = probe.knots()
+    assert len(E) == probe.info.nE
+    seed = kn[len(kn) // 2]                          # a knot in the middle of the list: frame (strand_a, k_a)
+    d["anchors"] = [{"select": {"type": "region", "id": [int(seed[1]), int(seed[2])], "radius": radius},
+                     "fix": [1, 1, 1, 0, 0, 0]}]
+    m = SC.model_from_scene(d, base_dir=HERE)
+    fixed = set(np.nonzero((m.arr("node_free").reshape(-1, 3) == 0).all(1))[0].tolist())
+    # the restatement
+    L = np.linalg.norm(Vw[E[:, 0]] - Vw[E[:, 1]], axis=1)
+    G = coo_matrix((np.r_[L, L], (np.r_[E[:, 0], E[:, 1]], np.r_[E[:, 1], E[:, 0]])), shape=(len(Vw), len(Vw))).tocsr()
+    dist = dijkstra(G, indices=[E[seed[0], 0], E[seed[0], 1]], min_only=True)
+    near = 0.5 * (dist[E[kn[:, 0], 0]] + dist[E[kn[:, 0], 1]]) <= radius
+    node = {(s, k): nd for s, k, nd in m.frames()}
+    want = {node[(a, b)] for e, a, b, _, _, _ in kn[near]} | {node[(a, b)] for e, _, _, a, b, _ in kn[near]}
+    print(f"\nradius {radius}: {int(near.sum())} of {len(kn)} knots, {len(fixed)} nodes anchored")
+    assert fixed == want
+    if radius == 0.0:
+        assert near.sum() == 1
+    if radius > 1e9:
+        assert near.all()
+    if 0.0 < radius < 1e9:
+        assert 1 < near.sum() < len(kn)

@@ -107,7 +107,7 @@ class wsm_scene(C.Structure):
 
 
 class wsm_anchor(C.Structure):
-    _fields_ = [("type", i32), ("s", i32), ("k", i32), ("fix", i32 * 6)]
+    _fields_ = [("type", i32), ("s", i32), ("k", i32), ("fix", i32 * 6), ("pad", i32), ("radius", d)]
 
 
 class wsm_point_load(C.Structure):
@@ -128,7 +128,7 @@ class wsm_info(C.Structure):
                                    "parity_conflicts")] + [("span_plan", d), ("rise", d)]
 
 
-WSM_FRAME, WSM_ALL_START, WSM_ALL_END, WSM_ALL_ENDS, WSM_STRAND, WSM_BOUNDARY = range(6)
+WSM_FRAME, WSM_ALL_START, WSM_ALL_END, WSM_ALL_ENDS, WSM_STRAND, WSM_BOUNDARY, WSM_REGION = range(7)
 WS_FP32_FP64, WS_FP64 = 0, 1
 
 

@@ -15,6 +15,7 @@
 #include <set>
 #include <memory>
 #include <numeric>
+#include <queue>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -669,7 +670,47 @@ struct Built {
   std::vector<KnotRec> knots;                             // sorted by knot id (edge id)
   std::map<int64_t, int> knotNode;                        // fkey → row
   double overhang = 0;
+  // the mesh graph, for the 'region' selector: vertices, edges, and the knots' edges in the topology's insertion order
+  std::vector<double> meshV;
+  std::vector<int32_t> meshE;
+  std::vector<int> knotOrder;                             // indices into `knots`
 };
+
+// regionFrames (model.js:127-145): the mesh graph distance from the two vertices of the seed frame's knot edge
+// (Dijkstra over the edges, lengths in scene units), then both frames of every knot whose edge's two vertex
+// distances average at most r, in the topology's knot order
+std::vector<int64_t> regionFrames(const Built& tp, int s0, int k0, double r) {
+  std::vector<int64_t> out;
+  const int nV = int(tp.meshV.size() / 3), nE = int(tp.meshE.size() / 2);
+  if (nE == 0) return out;
+  int seed = -1;
+  for (const auto& kr : tp.knots)
+    if ((kr.as == s0 && kr.ak == k0) || (kr.bs == s0 && kr.bk == k0)) { seed = kr.edge; break; }
+  if (seed < 0 || seed >= nE) return out;
+  std::vector<std::vector<std::pair<int, double>>> nbr(nV);
+  for (int e = 0; e < nE; e++) {
+    const int a = tp.meshE[2 * e], b = tp.meshE[2 * e + 1];
+    const double l = std::hypot(tp.meshV[3 * a] - tp.meshV[3 * b], tp.meshV[3 * a + 1] - tp.meshV[3 * b + 1],
+                                tp.meshV[3 * a + 2] - tp.meshV[3 * b + 2]);
+    nbr[a].push_back({b, l}); nbr[b].push_back({a, l});
+  }
+  std::vector<double> dist(nV, INFINITY);
+  using Item = std::pair<double, int>;
+  std::priority_queue<Item, std::vector<Item>, std::greater<Item>> heap;
+  for (int v : {tp.meshE[2 * seed], tp.meshE[2 * seed + 1]}) { dist[v] = 0.0; heap.push({0.0, v}); }
+  while (!heap.empty()) {
+    const auto [d, v] = heap.top(); heap.pop();
+    if (d > dist[v]) continue;
+    for (const auto& [w, l] : nbr[v]) if (d + l < dist[w]) { dist[w] = d + l; heap.push({d + l, w}); }
+  }
+  for (int q : tp.knotOrder) {
+    const auto& kr = tp.knots[q];
+    if (0.5 * (dist[tp.meshE[2 * kr.edge]] + dist[tp.meshE[2 * kr.edge + 1]]) <= r) {
+      out.push_back(fkey(kr.as, kr.ak)); out.push_back(fkey(kr.bs, kr.bk));
+    }
+  }
+  return out;
+}
 
 std::unique_ptr<Model> buildModel(const Built& tp, const Setup& su) {
   auto md = std::make_unique<Model>();
@@ -830,6 +871,7 @@ std::unique_ptr<Model> buildModel(const Built& tp, const Setup& su) {
         break;
       case WSM_STRAND: if (an.s >= 0 && an.s < S) for (int kk = 0; kk < tp.nKnots[an.s]; kk++) keys.push_back(fkey(an.s, kk)); break;
       case WSM_BOUNDARY: for (auto& bf : md->boundaryFrames) { keys.push_back(bf.first); keys.push_back(bf.second); } break;
+      case WSM_REGION: keys = regionFrames(tp, an.s, an.k, an.radius); break;
       default: throw std::runtime_error("unknown selector type");
     }
     for (int64_t key : keys) addMask(key, an.fix);
@@ -956,6 +998,10 @@ Built fromTopology(const Topology& topo) {
     V3 en{topo.mesh->edgeN[3 * kn.edge], topo.mesh->edgeN[3 * kn.edge + 1], topo.mesh->edgeN[3 * kn.edge + 2]};
     b.knots.push_back({kn.as, kn.ak, kn.bs, kn.bk, kn.isBoundary, en, kn.edge, kn.overSide});
   }
+  b.meshV = topo.mesh->V; b.meshE = topo.mesh->E;
+  std::map<int, int> row;
+  for (int q = 0; q < int(b.knots.size()); q++) row[b.knots[q].edge] = q;
+  for (int e : topo.knotOrder) b.knotOrder.push_back(row.at(e));
   return b;
 }
 

@@ -456,6 +456,7 @@ napi_value ModelFromMesh(napi_env env, napi_callback_info info) {
   for (uint32_t i = 0; i < an.size(); i++) {
     napi_value e = elem(env, a[3], i);
     an[i].type = int32_t(getd(env, e, "type", 0)); an[i].s = int32_t(getd(env, e, "s", 0)); an[i].k = int32_t(getd(env, e, "k", 0));
+    an[i].pad = 0; an[i].radius = getd(env, e, "radius", 0);
     napi_value fx; bool has = false;
     napi_has_named_property(env, e, "fix", &has);
     if (has && napi_get_named_property(env, e, "fix", &fx) == napi_ok)

@@ -4,8 +4,9 @@
 // the file holds, E90 / fc90 / H of older files from their N/mm² defaults in the scene's stress unit — and
 // Mesh.fromBytes (web/js/mesh.js:24-31): STL (binary when the size says so, else ASCII) or OBJ (fan-triangulated),
 // scaled, then welded by the model builder.  A hybrid weave's lath groups and strand assignments (scene.js:50-62) go
-// to the builder with the scene.  Refused with an error rather than approximated: groups assigned to pieces of a
-// discretisation (hybrid.pieces) unless every group in use equals the scene's own, and the 'region' anchor selector.
+// to the builder with the scene, and so do anchors by every selector, the 'region' one included.  Refused with an error
+// rather than approximated: groups assigned to pieces of a discretisation (hybrid.pieces) unless every group in use
+// equals the scene's own.
 //
 //   const S = require('./scene_mi355x.js');
 //   const m = S.loadScene('examples/scenes/dome_basic.json');        // a W.Model, ready for new W.GpuSolver(...)
@@ -19,7 +20,7 @@ const LENGTH_TO_M = { mm: 1e-3, cm: 1e-2, m: 1.0 };
 const FORCE_TO_N = { N: 1.0, kN: 1e3 };
 const FLOOR_MU = 0.4;
 const MATERIAL_MM_N = { E90: 800.0, fc90: 8.0, H: 30.0 };
-const SELECTOR = { frame: 0, all_start: 1, all_end: 2, all_ends: 3, strand: 4, boundary: 5 };
+const SELECTOR = { frame: 0, all_start: 1, all_end: 2, all_ends: 3, strand: 4, boundary: 5, region: 6 };
 
 /** triangle soup { V, F } of an STL file's bytes (Buffer / Uint8Array) */
 function parseSTL(buf) {
@@ -182,8 +183,9 @@ function sceneFromJSON(d) {
     if (t === 'frame') anchors.push({ type: SELECTOR.frame, s: sel.id[0] | 0, k: sel.id[1] | 0, fix });
     else if (t === 'knots') for (const i of sel.ids || []) anchors.push({ type: SELECTOR.frame, s: i[0] | 0, k: i[1] | 0, fix });
     else if (t === 'strand') anchors.push({ type: SELECTOR.strand, s: sel.strand | 0, k: 0, fix });
+    // regionFrames (model.js:127-145): the knots within `radius` of the seed frame's knot along the mesh edges
+    else if (t === 'region') anchors.push({ type: SELECTOR.region, s: sel.id[0] | 0, k: sel.id[1] | 0, fix, radius: +(sel.radius || 0) });
     else if (t in SELECTOR) anchors.push({ type: SELECTOR[t], s: 0, k: 0, fix });
-    else if (t === 'region') throw new Error("the 'region' anchor selector (graph distance on the mesh) is not modelled");
     else throw new Error(`unknown selector type ${t}`);
   }
   const mesh = d.mesh || {};

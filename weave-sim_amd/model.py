@@ -272,12 +272,13 @@ class Model:
 
 
 def _anchor(a):
-    """(type, s, k, fix[6]) or dict(type=..., s=, k=, fix=)"""
+    """(type, s, k, fix[6][, radius]) or dict(type=..., s=, k=, fix=[, radius=]); radius: the region selector's"""
     if isinstance(a, dict):
-        t, s, k, fix = a["type"], a.get("s", 0), a.get("k", 0), a["fix"]
+        t, s, k, fix, r = a["type"], a.get("s", 0), a.get("k", 0), a["fix"], a.get("radius", 0.0)
     else:
-        t, s, k, fix = a
-    return A.wsm_anchor(t, s, k, (C.c_int32 * 6)(*[1 if f else 0 for f in fix]))
+        t, s, k, fix = a[:4]
+        r = a[4] if len(a) > 4 else 0.0
+    return A.wsm_anchor(t, s, k, (C.c_int32 * 6)(*[1 if f else 0 for f in fix]), 0, float(r))
 
 
 def _load(x):

@@ -10,9 +10,11 @@ builder.  The instance is then built by wsm_from_mesh — the reference's buildT
 A hybrid weave's lath groups (Scene.hybrid, scene.js:50-62) go to wsm_from_mesh_groups with the strand assignments:
 each strand's segments, crossing sides, nodes and knot lifts take its group's material, section and beam.
 
+Anchors take every selector of resolveSelector (model.js:115-125), the 'region' one (graph distance along the mesh
+edges from a seed knot, model.js:127-145) included.
+
 Not modelled by the builder, and refused with an error rather than approximated: groups assigned to pieces of a
-discretisation (Scene.hybrid.pieces) unless every group in use equals the scene's own, and the 'region' anchor
-selector (graph distance on the mesh, model.js:127-145).
+discretisation (Scene.hybrid.pieces) unless every group in use equals the scene's own.
 """
 import json
 import os
@@ -29,7 +31,7 @@ LENGTH_TO_M = {"mm": 1e-3, "cm": 1e-2, "m": 1.0}    # units.js:2
 FORCE_TO_N = {"N": 1.0, "kN": 1e3}                  # units.js:3
 FLOOR_MU = 0.4                                      # scene.js:14
 _MATERIAL_MM_N = {"E90": 800.0, "fc90": 8.0, "H": 30.0}   # params.js:6, N/mm²
-_SELECTOR = {"frame": 0, "all_start": 1, "all_end": 2, "all_ends": 3, "strand": 4, "boundary": 5}   # wsm_selector
+_SELECTOR = {"frame": 0, "all_start": 1, "all_end": 2, "all_ends": 3, "strand": 4, "boundary": 5, "region": 6}   # wsm_selector
 
 
 # ------------------------------------------------------------------------------------------------ meshes
@@ -264,10 +266,10 @@ def scene_from_dict(d):
             anchors.extend((_SELECTOR["frame"], int(i[0]), int(i[1]), fix) for i in sel.get("ids") or [])
         elif t == "strand":
             anchors.append((_SELECTOR["strand"], int(sel.get("strand", 0)), 0, fix))
+        elif t == "region":     # regionFrames (model.js:127-145): seed frame id, radius in scene length units
+            anchors.append((_SELECTOR["region"], int(sel["id"][0]), int(sel["id"][1]), fix, float(sel.get("radius") or 0)))
         elif t in _SELECTOR:
             anchors.append((_SELECTOR[t], 0, 0, fix))
-        elif t == "region":
-            raise NotImplementedError("the 'region' anchor selector (graph distance on the mesh) is not modelled")
         else:
             raise ValueError(f"unknown selector type {t}")
     mesh = _sec(d, "mesh")
