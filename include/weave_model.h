@@ -96,12 +96,25 @@ WS_API int wsm_from_mesh(const double* V, int32_t nV, const int32_t* F, int32_t 
  * assigned outside 2 … 1 + n_groups, is in group 1 = sc->grp.  Each segment's stiffness, activation and self-weight,
  * each crossing side's section and material, each node's floor section and each knot's half-height lift follow
  * the group of the strand (model.js:203-252, solver_cpu.js:72-78 and 558-560, topology.js:225).  Per-piece groups
- * (Scene.hybrid.pieces) are not taken. */
+ * (Scene.hybrid.pieces): wsm_from_mesh_pieces. */
 WS_API int wsm_from_mesh_groups(const double* V, int32_t nV, const int32_t* F, int32_t nF, int32_t weld,
                                 const wsm_scene* sc, const wsm_anchor* anchors, int32_t n_anchors,
                                 const wsm_point_load* loads, int32_t n_loads, const int32_t* flip_knots,
                                 int32_t n_flips, const wsm_prep* prep, const wsm_group* groups, int32_t n_groups,
                                 const int32_t* strand_group, int32_t n_assign, wsm_model** out);
+/* a hybrid weave whose groups are also assigned to pieces of a discretisation (Scene.joints and Scene.hybrid.pieces,
+ * scene.js:57-90; discretize.js:334-358): joint q sits at arc length joint_s[q] along strand joint_strand[q];
+ * piece_group[3q..3q+2] = strand, piece (0-based, counted from the strand's start between its sorted joints),
+ * group.  A segment takes the group of the piece that holds its midpoint, a node (floor section, crossing side
+ * section) the piece that holds it, by arc length on the built positions; a piece not assigned, or assigned
+ * outside 2 … 1 + n_groups, takes its strand's group.  With n_pieces = 0 the model is wsm_from_mesh_groups'. */
+WS_API int wsm_from_mesh_pieces(const double* V, int32_t nV, const int32_t* F, int32_t nF, int32_t weld,
+                                const wsm_scene* sc, const wsm_anchor* anchors, int32_t n_anchors,
+                                const wsm_point_load* loads, int32_t n_loads, const int32_t* flip_knots,
+                                int32_t n_flips, const wsm_prep* prep, const wsm_group* groups, int32_t n_groups,
+                                const int32_t* strand_group, int32_t n_assign, const int32_t* joint_strand,
+                                const double* joint_s, int32_t n_joints, const int32_t* piece_group, int32_t n_pieces,
+                                wsm_model** out);
 /* stand-alone laths (topologyFromPolylines, topology.js:348-363) with optional explicit crossings
  * (knots[4c..4c+3] = sA, kA, sB, kB frame ids) as the test fixtures of web/tests/cases.mjs build them */
 WS_API int wsm_from_polylines(const double* P, const int32_t* counts, int32_t n_strands,

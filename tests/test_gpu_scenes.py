@@ -3,7 +3,9 @@ under tests/golden): the dome with point loads in world and frame axes and a fli
 of the reference's own GPU checks (v5dome_cm, selftest.html:38-43), the tree with augmented-Lagrangian contact and a
 UDL per length (tree_cm), the dual v5 dome with rotational friction, the seat law and a UDL per area (dualv5dome_cm), and its hybrid weave
 with six laths in a second lath group of another material and section (dualv5dome_cm_hybrid: crossing sides of
-different sections, per-segment stiffness and weight).
+different sections, per-segment stiffness and weight), and that weave with groups also assigned to pieces of a
+discretisation (joints on two laths, their pieces in the second group: stiffness, weight and sections change along
+a lath).
 
 Windows after the 300-iteration warm-up (nothing moves during it).  Bars: FP64 within 1e-9 × span of the oracle with
 identical stick / slip states through 1,000 iterations (an oracle run perturbed by 1e-12 of the span stays within
@@ -12,6 +14,7 @@ at FP32's rounding level (1e-7 of the span) keeps them, and past that within the
 reports its own GPU form on the v5 dome as tracking f64 to 1.2e-5 of the displacement through 1,000 iterations
 (web/README.md:80-82); the FP32+FP64 figure here is printed beside the bar."""
 import importlib
+import json
 import os
 
 import numpy as np
@@ -26,10 +29,21 @@ HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "scene
 WINDOW = [400, 600, 1000]
 
 
-@pytest.mark.parametrize("name", ["dome_basic", "v5dome_cm", "tree_cm", "dualv5dome_cm", "dualv5dome_cm_hybrid"])
+def load(name):
+    if name != "dualv5dome_cm_pieces":
+        return SC.load_scene(os.path.join(HERE, name + ".json"))
+    with open(os.path.join(HERE, "dualv5dome_cm_hybrid.json")) as f:
+        d = json.load(f)
+    d["joints"] = {"5": [120.0, 40.0], "9": [10.5]}
+    d["hybrid"]["pieces"] = {"5:1": 2, "9:0": 2, "9:1": 2}
+    return SC.model_from_scene(d, base_dir=HERE)
+
+
+@pytest.mark.parametrize("name", ["dome_basic", "v5dome_cm", "tree_cm", "dualv5dome_cm", "dualv5dome_cm_hybrid",
+                                  "dualv5dome_cm_pieces"])
 @pytest.mark.parametrize("precision,tol,eps", [("fp64", 1e-9, 1e-12), ("fp32", 1e-4, 1e-7)])
 def test_example_scene_matches_oracle(W, gpu_ctx, name, precision, tol, eps):
-    m = SC.load_scene(os.path.join(HERE, name + ".json"))
+    m = load(name)
     span = m.span
     X0 = m["X"].copy()
     o = Oracle(m)

@@ -190,18 +190,27 @@ for (const name of {json.dumps(names)}) {{
   res[name] = {{ info: {{ n: m.info.n, m: m.info.m, K: m.info.K, S: m.info.S, params: m.info.params }},
     X: Array.from(m.st.X), u: Array.from(m.st.u), loads: Array.from(m.info.loadConst), free: Array.from(m.info.nodeFree) }};
 }}
-let refused = '';
 const hy = JSON.parse(require('fs').readFileSync(require('path').join({json.dumps(golden)}, 'dualv5dome_cm_hybrid.json'), 'utf8'));
-hy.hybrid.pieces = {{ '3:1': 2 }};
-try {{ S.sceneFromJSON(hy); }} catch (e) {{ refused = e.message; }}
-res.refused = refused;
+hy.joints = {{ '5': [120.0, 40.0], '9': [10.5] }};
+hy.hybrid.pieces = {{ '5:1': 2, '9:0': 2, '9:1': 2, '5:01': 2, '7:0': 9 }};
+const mp = S.modelFromScene(hy, require('fs').readFileSync(require('path').join({json.dumps(golden)}, hy.mesh.path)), hy.mesh.path);
+const fp = mp.fields();
+res.pieces = {{ EA: Array.from(fp.EA), k_wk: Array.from(fp.k_wk), k_E90: Array.from(fp.k_E90), loads: Array.from(mp.info.loadConst) }};
 const rg = JSON.parse(require('fs').readFileSync(require('path').join({json.dumps(golden)}, 'dome_basic.json'), 'utf8'));
 rg.anchors = [{{ select: {{ type: 'region', id: [3, 2], radius: 900 }}, fix: [1, 1, 1, 0, 0, 0] }}];
 const mr = S.modelFromScene(rg, require('fs').readFileSync(require('path').join({json.dumps(golden)}, rg.mesh.path)), rg.mesh.path);
 res.region = Array.from(mr.info.nodeFree);
 console.log(JSON.stringify(res));""")
     d = json.loads(out)
-    assert "pieces" in d["refused"]
+    # groups of pieces of a discretisation through both readers (scene.js:57-90)
+    hy = json.load(open(os.path.join(golden, "dualv5dome_cm_hybrid.json")))
+    hy["joints"] = {"5": [120.0, 40.0], "9": [10.5]}
+    hy["hybrid"]["pieces"] = {"5:1": 2, "9:0": 2, "9:1": 2, "5:01": 2, "7:0": 9}
+    pp = SC.model_from_scene(hy, base_dir=golden)
+    plain = SC.load_scene(os.path.join(golden, "dualv5dome_cm_hybrid.json"))
+    for k, nm in (("EA", "EA"), ("k_wk", "k_wk"), ("k_E90", "k_E90"), ("loads", "load_const")):
+        assert np.array_equal(np.array(d["pieces"][k]), pp.arr(nm)), k
+    assert not np.array_equal(pp.arr("EA"), plain.arr("EA"))
     # the 'region' selector through both readers (model.js:127-145)
     rg = json.load(open(os.path.join(golden, "dome_basic.json")))
     rg["anchors"] = [{"select": {"type": "region", "id": [3, 2], "radius": 900}, "fix": [1, 1, 1, 0, 0, 0]}]

@@ -183,12 +183,85 @@ def test_hybrid_group_assigned_everywhere_equals_a_scene_of_that_group():
     assert np.abs(Xp[both == 3] - Xe[both == 3]).max() > 1e-3 * span
 
 
-def test_hybrid_pieces_refused_and_one_group_accepted():
-    SC.scene_from_dict(scene_json("v5dome_cm"))              # its second group equals the first
+def _piece_scene():
+    """dualv5dome_cm_hybrid with joints on two strands and groups assigned to pieces: strand s0 (group 1) gets joints
+    at 65 % and 30 % of its length (unsorted) and its middle piece in group 2; strand s2 (group 1, no joints) its only
+    piece in group 2; keys that are not in pieceKey's exact form, or name a group outside 2 … 1 + groups, apply to
+    nothing"""
     d = scene_json("dualv5dome_cm_hybrid")
-    d["hybrid"]["pieces"] = {"3:1": 2}
-    with pytest.raises(NotImplementedError):
-        SC.scene_from_dict(d)
+    base = _build(d)
+    S = base.md.S
+    grp = {int(s): int(g) for s, g in d["hybrid"]["assign"].items()}
+    no = base.arr("node_offset")
+    free = [s for s in range(S) if grp.get(s, 1) == 1 and no[s + 1] - no[s] > 20 and not base.arr("strand_closed")[s]]
+    s0, s2 = free[0], free[1]
+    X = base["X"].reshape(-1, 3)
+    P = X[no[s0]:no[s0 + 1]]
+    Sa = np.r_[0.0, np.cumsum(np.linalg.norm(np.diff(P, axis=0), axis=1))]
+    d["joints"] = {str(s0): [0.65 * Sa[-1], 0.3 * Sa[-1]]}
+    d["hybrid"]["pieces"] = {f"{s0}:1": 2, f"{s2}:0": 2, f"{s0}:02": 2, f"{s0}:2": 7, f"{s0}:0": 1}
+    return d, base, s0, s2, Sa
+
+
+def test_hybrid_piece_groups_by_arc_length():
+    """groups assigned to pieces of a discretisation (Scene.hybrid.pieces with Scene.joints; scene.js:57-90,
+    discretize.js:334-358): a segment takes the group of the piece that holds its midpoint and a node the piece that
+    holds it, by arc length along the strand on the built positions, for its stiffness, self-weight, floor section and
+    crossing-side sections; every other strand is bitwise as in the hybrid model without pieces, and so is the
+    geometry (the knot lift follows the strands' groups, topology.js:225)"""
+    d, base, s0, s2, Sa = _piece_scene()
+    m = _build(d)
+    g1, g2 = d, d["hybrid"]["groups"][0]
+    EA = {1: g1["material"]["E"] * g1["section"]["width"] * g1["section"]["height"],
+          2: g2["material"]["E"] * g2["section"]["width"] * g2["section"]["height"]}
+    W = {1: g1["section"]["width"], 2: g2["section"]["width"]}
+    assert np.array_equal(m["X"], base["X"]) and np.array_equal(m.arr("l0"), base.arr("l0"))
+    no, so = m.arr("node_offset"), m.arr("seg_offset")
+    js = np.sort(np.array(d["joints"][str(s0)]))
+    X = m["X"].reshape(-1, 3)
+    i0, j0, j1 = no[s0], so[s0], so[s0 + 1]
+    a, b = m.arr("seg_i0")[j0:j1], m.arr("seg_i1")[j0:j1]
+    mid = Sa[a - i0] + 0.5 * np.linalg.norm(X[b] - X[a], axis=1)
+    gseg = np.where(np.searchsorted(js, mid, "right") == 1, 2, 1)
+    gnode = np.where(np.searchsorted(js, Sa, "right") == 1, 2, 1)
+    assert set(gseg) == {1, 2} and set(gnode) == {1, 2}
+    assert np.allclose(m.arr("EA")[j0:j1], [EA[g] for g in gseg], rtol=1e-14)
+    assert np.array_equal(m.arr("node_w")[i0:no[s0 + 1]], [W[g] for g in gnode])
+    # the whole joint-less strand s2 in group 2, every other strand as before
+    j2 = slice(so[s2], so[s2 + 1])
+    assert np.allclose(m.arr("EA")[j2], EA[2], rtol=1e-14) and np.allclose(base.arr("EA")[j2], EA[1], rtol=1e-14)
+    rest = np.ones(m.m, bool); rest[j0:j1] = False; rest[j2] = False
+    assert np.array_equal(m.arr("EA")[rest], base.arr("EA")[rest])
+    # crossing sides take the group of the piece at their knot node
+    ks, fr = m.arr("k_strand"), {(s, k): nd for s, k, nd in m.frames()}
+    kn = m.knots()
+    node = np.array([[fr[(r[1], r[2])], fr[(r[3], r[4])]] for r in kn]).reshape(-1)
+    on = ks == s0
+    assert np.array_equal(m.arr("k_wk")[on], [W[g] for g in gnode[node[on] - i0]])
+    assert np.array_equal(m.arr("k_wk")[ks == s2], np.full((ks == s2).sum(), W[2]))
+    other = (ks != s0) & (ks != s2)
+    assert np.array_equal(m.arr("k_wk")[other], base.arr("k_wk")[other])
+    # self-weight: ρ·A·g of each segment's group × l0, half to each end
+    L = SC.LENGTH_TO_M[d["units"]["length"]]
+    q = {g: gd["material"]["density"] * gd["section"]["width"] * gd["section"]["height"] * L * L * 9.80665 * L for g, gd in ((1, g1), (2, g2))}
+    dl = (m.arr("load_const") - base.arr("load_const")).reshape(-1, 3).sum(0)
+    l0 = m.arr("l0")
+    gdir = np.array(d["loads"]["gravity"]["direction"], dtype=float)
+    changed = l0[j0:j1][gseg == 2].sum() + l0[j2].sum()
+    assert np.allclose(dl, changed * (q[2] - q[1]) * gdir / np.linalg.norm(gdir), rtol=1e-9, atol=1e-9 * q[2] * changed)
+
+
+def test_hybrid_pieces_of_every_strand_equal_strand_groups():
+    """every piece of every strand (joint-less: piece 0) assigned as the strands are builds the hybrid model itself"""
+    d = scene_json("dualv5dome_cm_hybrid")
+    base = _build(d)
+    d["hybrid"]["pieces"] = {f"{s}:0": g for s, g in d["hybrid"]["assign"].items()}
+    m = _build(d)
+    for nm in ["EA", "GA", "EIx", "EIy", "GJ", "omega0", "load_const", "k_wk", "k_hk", "k_E90", "node_w", "node_h", "mass"]:
+        assert np.array_equal(m.arr(nm), base.arr(nm)), nm
+
+
+def test_scene_version_and_group_defaults():
     with pytest.raises(ValueError):
         SC.scene_from_dict({"version": 2})
     # a group that leaves a key out takes its default, not the scene's value

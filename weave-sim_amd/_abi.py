@@ -173,6 +173,10 @@ def _open_model_lib():
         L.wsm_from_mesh_groups.argtypes = [P_d, i32, P_i32, i32, i32, C.POINTER(wsm_scene), C.POINTER(wsm_anchor), i32,
                                            C.POINTER(wsm_point_load), i32, P_i32, i32, C.POINTER(wsm_prep),
                                            C.POINTER(wsm_group), i32, P_i32, i32, C.POINTER(vp)]
+        L.wsm_from_mesh_pieces.argtypes = [P_d, i32, P_i32, i32, i32, C.POINTER(wsm_scene), C.POINTER(wsm_anchor), i32,
+                                           C.POINTER(wsm_point_load), i32, P_i32, i32, C.POINTER(wsm_prep),
+                                           C.POINTER(wsm_group), i32, P_i32, i32, P_i32, P_d, i32, P_i32, i32,
+                                           C.POINTER(vp)]
         L.wsm_from_polylines.argtypes = [P_d, P_i32, i32, P_i32, P_i32, P_d, P_i32, i32, C.POINTER(wsm_scene),
                                          C.POINTER(wsm_anchor), i32, C.POINTER(wsm_point_load), i32, C.POINTER(vp)]
         L.wsm_hex_dome.argtypes = [C.POINTER(wsm_hex_spec), C.POINTER(wsm_scene), C.POINTER(vp)]
@@ -250,7 +254,7 @@ WEAVE_H_SYMBOLS = [
     "ws_cancel", "ws_pump", "ws_poll", "ws_fetch", "ws_server_clock_ms", "ws_server_stats",
 ]
 WEAVE_MODEL_H_SYMBOLS = [
-    "wsm_scene_defaults", "wsm_from_mesh", "wsm_from_mesh_groups", "wsm_from_polylines", "wsm_hex_dome", "wsm_get", "wsm_info_get",
+    "wsm_scene_defaults", "wsm_from_mesh", "wsm_from_mesh_groups", "wsm_from_mesh_pieces", "wsm_from_polylines", "wsm_hex_dome", "wsm_get", "wsm_info_get",
     "wsm_add_load", "wsm_set_strength", "wsm_clone", "wsm_mesh_counts", "wsm_last_error", "wsm_free",
     "wsm_abi_sizes", "wsm_segment_moments", "wsm_stability_layout", "wsm_knots", "wsm_frames", "wsm_lock_contacts",
 ]

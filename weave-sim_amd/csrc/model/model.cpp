@@ -513,12 +513,31 @@ struct Setup {
   double meshArea = 0;     // area of the welded mesh (a UDL per area spreads over it); 0 without a mesh
   std::vector<wsm_group> extra;        // lath groups 2, 3, … of a hybrid weave; group 1 is sc.grp (scene.js:53-56)
   std::vector<int32_t> strandGroup;    // the group of each strand; empty: every strand in group 1
+  // pieces of a discretisation (Scene.joints, Scene.hybrid.pieces): each strand's joint arc lengths, ascending, and
+  // the group assigned to piece p of strand s (key fkey(s, p)); pieceGroups: the scene assigns pieces at all
+  std::vector<std::vector<double>> joints;
+  std::map<int64_t, int> pieces;
+  bool pieceGroups = false;
+  bool valid(int g) const { return g >= 2 && g - 2 < int(extra.size()); }
   // scene.js:57-61: an assignment outside 2 … 1 + #extra falls back to group 1
-  const wsm_group& groupOfStrand(int s) const {
+  int groupIdOfStrand(int s) const {
     const int g = s < int(strandGroup.size()) ? strandGroup[s] : 1;
-    return (g >= 2 && g - 2 < int(extra.size())) ? extra[g - 2] : sc.grp;
+    return valid(g) ? g : 1;
   }
+  // pieceGroup (scene.js:66-70): the piece's own assignment, else the strand's group
+  int groupIdOfPiece(int s, int p) const {
+    const auto it = pieces.find(fkey(s, p));
+    const int g = it == pieces.end() ? 0 : it->second;
+    return valid(g) ? g : groupIdOfStrand(s);
+  }
+  const wsm_group& group(int g) const { return valid(g) ? extra[g - 2] : sc.grp; }
+  const wsm_group& groupOfStrand(int s) const { return group(groupIdOfStrand(s)); }
 };
+
+// np.searchsorted(a, v, 'right') on an ascending array (searchsorted, fields.js:88-93)
+int searchsortedRight(const std::vector<double>& a, double v) {
+  return int(std::upper_bound(a.begin(), a.end(), v) - a.begin());
+}
 
 // Σ of the triangle areas (Mesh.area, mesh.js:50-58)
 double meshArea(const Mesh& mesh) {
@@ -798,12 +817,43 @@ std::unique_ptr<Model> buildModel(const Built& tp, const Setup& su) {
     for (int j = md->segOffset[s]; j < md->segOffset[s + 1]; j++) segS[j] = s;
     for (int i = md->nodeOffset[s]; i < md->nodeOffset[s + 1]; i++) nodeS[i] = s;
   }
-  // ---- strandStiffness (model.js:236-252): each segment takes its strand's group, each joint the activation of the
-  // group of its first segment
+  // the lath group of every segment and node (Scene.segmentGroups / nodeGroups, scene.js:75-90): the strand's, or,
+  // where the scene assigns pieces of a discretisation, the group of the piece that holds the segment's midpoint /
+  // the node by arc length along the strand on the built positions (segmentPieceIndex / pieceIndex,
+  // discretize.js:334-358; a strand without joints is piece 0 throughout)
+  std::vector<int> gSeg(m), gNode(n);
+  if (!su.pieceGroups) {
+    for (int j = 0; j < m; j++) gSeg[j] = su.groupIdOfStrand(segS[j]);
+    for (int i = 0; i < n; i++) gNode[i] = su.groupIdOfStrand(nodeS[i]);
+  } else {
+    for (int s = 0; s < S; s++) {
+      const int i0 = md->nodeOffset[s], nn = md->nodeOffset[s + 1] - i0;
+      const std::vector<double> none;
+      const std::vector<double>& js = s < int(su.joints.size()) ? su.joints[s] : none;
+      std::vector<double> Sarc(nn, 0.0);
+      const auto& X = md->X;
+      for (int i = 1; i < nn; i++) {
+        const int a = 3 * (i0 + i - 1), b = 3 * (i0 + i);
+        Sarc[i] = Sarc[i - 1] + std::hypot(X[b] - X[a], X[b + 1] - X[a + 1], X[b + 2] - X[a + 2]);
+      }
+      for (int j = md->segOffset[s]; j < md->segOffset[s + 1]; j++) {
+        int pc = 0;
+        if (!js.empty()) {
+          const int a = md->segI0[j], b = md->segI1[j];
+          const double len = std::hypot(X[3 * b] - X[3 * a], X[3 * b + 1] - X[3 * a + 1], X[3 * b + 2] - X[3 * a + 2]);
+          pc = searchsortedRight(js, Sarc[a - i0] + 0.5 * len);
+        }
+        gSeg[j] = su.groupIdOfPiece(s, pc);
+      }
+      for (int i = 0; i < nn; i++) gNode[i0 + i] = su.groupIdOfPiece(s, js.empty() ? 0 : searchsortedRight(js, Sarc[i]));
+    }
+  }
+  // ---- strandStiffness (model.js:236-252): each segment takes its group, each joint the activation of the group of
+  // its first segment
   md->EA.resize(m); md->GA.resize(m); md->EIx.resize(m); md->EIy.resize(m); md->GJ.resize(m);
   std::vector<Stiff> segK(m);
   for (int j = 0; j < m; j++) {
-    const Stiff k = groupStiffness(su.groupOfStrand(segS[j]));
+    const Stiff k = groupStiffness(su.group(gSeg[j]));
     segK[j] = k; md->EA[j] = k.EA; md->GA[j] = k.GA; md->EIx[j] = k.EIx; md->EIy[j] = k.EIy; md->GJ[j] = k.GJ;
   }
   // ---- CosseratRod (rod_cosserat.js:49-72): Ω0 = (1 − act)·asBuilt(Q0)
@@ -829,7 +879,7 @@ std::unique_ptr<Model> buildModel(const Built& tp, const Setup& su) {
     double L = sc.length_to_m, F = sc.force_to_n;
     for (int j = 0; j < m; j++) {
       // self-weight of the segment's group (model.js:203-211; units.js:21-25)
-      const wsm_group& gr = su.groupOfStrand(segS[j]);
+      const wsm_group& gr = su.group(gSeg[j]);
       const double areaSI = gr.width * gr.height * L * L, qSI = gr.density * areaSI * 9.80665, qline = qSI / F * L;
       double q = 0.5 * md->l0[j] * qline;
       for (int i : {md->segI0[j], md->segI1[j]}) for (int c = 0; c < 3; c++) md->loadConst[3 * i + c] += q * g[c];
@@ -915,8 +965,10 @@ std::unique_ptr<Model> buildModel(const Built& tp, const Setup& su) {
     for (int side = 0; side < 2; side++) {
       const int i = 2 * c + side, s = md->kStrand[i];
       md->kSegOff[i] = md->segOffset[s]; md->kNseg[i] = md->segOffset[s + 1] - md->segOffset[s];
-      // each side's section and material: its strand's group (solver_cpu.js:72-78)
-      const wsm_group& gr = su.groupOfStrand(s);
+      // each side's section and material: the group at the side's crossing node, its strand's or its piece's
+      // (solver_cpu.js:66-78, 683)
+      const auto kn = md->nodeFrameIds.find(side ? fkey(tp.knots[c].bs, tp.knots[c].bk) : fkey(tp.knots[c].as, tp.knots[c].ak));
+      const wsm_group& gr = su.group(kn != md->nodeFrameIds.end() ? gNode[kn->second] : su.groupIdOfStrand(s));
       md->kwk[i] = gr.width; md->khk[i] = gr.height; md->kfc90[i] = gr.fc90; md->kE90[i] = gr.E90; md->kHs[i] = gr.H;
     }
     md->kw[c] = 0.5 * (md->kwk[2 * c] + md->kwk[2 * c + 1]); md->kh[c] = 0.5 * (md->khk[2 * c] + md->khk[2 * c + 1]);
@@ -976,7 +1028,7 @@ std::unique_ptr<Model> buildModel(const Built& tp, const Setup& su) {
   md->par = par;
   // floor goal sections and stick points (Floor constructor, solver_cpu.js:536-554)
   md->nodeW.resize(n); md->nodeH.resize(n);
-  for (int i = 0; i < n; i++) { const wsm_group& gr = su.groupOfStrand(nodeS[i]); md->nodeW[i] = gr.width; md->nodeH[i] = gr.height; }
+  for (int i = 0; i < n; i++) { const wsm_group& gr = su.group(gNode[i]); md->nodeW[i] = gr.width; md->nodeH[i] = gr.height; }
   md->floorA.assign(2 * n, 0);
   for (int i = 0; i < n; i++) { md->floorA[2 * i] = md->X[3 * i]; md->floorA[2 * i + 1] = md->X[3 * i + 1]; }
   md->floorF.assign(3 * n, 0); md->floorState.assign(n, 0);
@@ -1064,11 +1116,15 @@ namespace {
 int fromMesh(const double* V, int32_t nV, const int32_t* F, int32_t nF, int32_t do_weld, const wsm_scene* sc,
              const wsm_anchor* anchors, int32_t n_anchors, const wsm_point_load* loads, int32_t n_loads,
              const int32_t* flips, int32_t n_flips, const wsm_prep* prep, bool hybrid, const wsm_group* groups,
-             int32_t n_groups, const int32_t* assign, int32_t n_assign, wsm_model** out) {
+             int32_t n_groups, const int32_t* assign, int32_t n_assign, wsm_model** out,
+             const int32_t* joint_strand = nullptr, const double* joint_s = nullptr, int32_t n_joints = 0,
+             const int32_t* piece_group = nullptr, int32_t n_pieces = -1) {
   try {
     if (!V || !F || !sc || !out || nV <= 0 || nF <= 0) throw std::runtime_error("wsm_from_mesh: bad arguments");
     if (hybrid && ((n_groups > 0 && !groups) || (n_assign > 0 && !assign) || n_groups < 0 || n_assign < 0))
       throw std::runtime_error("wsm_from_mesh_groups: bad group arguments");
+    if (n_pieces >= 0 && ((n_joints > 0 && (!joint_strand || !joint_s)) || (n_pieces > 0 && !piece_group) || n_joints < 0))
+      throw std::runtime_error("wsm_from_mesh_pieces: bad joint or piece arguments");
     std::vector<double> Vv(V, V + 3 * size_t(nV)); std::vector<int32_t> Fv(F, F + 3 * size_t(nF));
     if (do_weld) weld(Vv, Fv);
     Mesh mesh = meshFromArrays(Vv, Fv);
@@ -1086,6 +1142,18 @@ int fromMesh(const double* V, int32_t nV, const int32_t* F, int32_t nF, int32_t 
       }
       topo.strandH.resize(topo.nStrands());
       for (int s = 0; s < topo.nStrands(); s++) topo.strandH[s] = su.groupOfStrand(s).height;
+    }
+    if (n_pieces >= 0) {
+      // Scene.hasPieceGroups (scene.js:71): the scene assigns pieces; joints of strands the topology does not have
+      // are ignored, as jointPoints and the piece indices do (discretize.js:309, 337)
+      su.pieceGroups = n_pieces > 0;
+      su.joints.assign(topo.nStrands(), {});
+      for (int q = 0; q < n_joints; q++) {
+        const int s = joint_strand[q];
+        if (s >= 0 && s < topo.nStrands()) su.joints[s].push_back(joint_s[q]);
+      }
+      for (auto& js : su.joints) std::sort(js.begin(), js.end());
+      for (int q = 0; q < n_pieces; q++) su.pieces[fkey(piece_group[3 * q], piece_group[3 * q + 1])] = piece_group[3 * q + 2];
     }
     buildPolylines(topo); consistentFrames(topo);
     su.anchors.assign(anchors, anchors + n_anchors); su.loads.assign(loads, loads + n_loads);
@@ -1115,6 +1183,18 @@ WS_API int wsm_from_mesh_groups(const double* V, int32_t nV, const int32_t* F, i
                                 const int32_t* strand_group, int32_t n_assign, wsm_model** out) {
   return fromMesh(V, nV, F, nF, do_weld, sc, anchors, n_anchors, loads, n_loads, flips, n_flips, prep, true, groups,
                   n_groups, strand_group, n_assign, out);
+}
+
+WS_API int wsm_from_mesh_pieces(const double* V, int32_t nV, const int32_t* F, int32_t nF, int32_t do_weld,
+                                const wsm_scene* sc, const wsm_anchor* anchors, int32_t n_anchors,
+                                const wsm_point_load* loads, int32_t n_loads, const int32_t* flips, int32_t n_flips,
+                                const wsm_prep* prep, const wsm_group* groups, int32_t n_groups,
+                                const int32_t* strand_group, int32_t n_assign, const int32_t* joint_strand,
+                                const double* joint_s, int32_t n_joints, const int32_t* piece_group, int32_t n_pieces,
+                                wsm_model** out) {
+  if (n_pieces < 0) { g_err = "wsm_from_mesh_pieces: negative piece count"; return WS_ERR_ARG; }
+  return fromMesh(V, nV, F, nF, do_weld, sc, anchors, n_anchors, loads, n_loads, flips, n_flips, prep, true, groups,
+                  n_groups, strand_group, n_assign, out, joint_strand, joint_s, n_joints, piece_group, n_pieces);
 }
 
 WS_API int wsm_from_polylines(const double* P, const int32_t* counts, int32_t n_strands, const int32_t* knot_rows,

@@ -443,7 +443,7 @@ napi_value ModelHexDome(napi_env env, napi_callback_info info) {
 // modelFromMesh(V: Float64Array [3nV], F: Int32Array [3nF], scene, anchors: [{type, s, k, fix[6]}],
 //               loads: [{s, k, F[3], local}], flips: [[s, k], ...], weld) — buildTopology … _buildGoals on a mesh
 napi_value ModelFromMesh(napi_env env, napi_callback_info info) {
-  napi_value a[9]; get_args(env, info, 9, a);
+  napi_value a[11]; get_args(env, info, 11, a);
   size_t nV = 0, nF = 0;
   const double* V = typed<double>(env, a[0], napi_float64_array, &nV);
   const int32_t* F = typed<int32_t>(env, a[1], napi_int32_array, &nF);
@@ -492,9 +492,25 @@ napi_value ModelFromMesh(napi_env env, napi_callback_info info) {
       napi_value e = elem(env, a[8], i);
       asg.push_back(int32_t(num_at(env, e, 0))); asg.push_back(int32_t(num_at(env, e, 1)));
     }
-    rc = wsm_from_mesh_groups(V, int32_t(nV / 3), F, int32_t(nF / 3), weld ? 1 : 0, &sc, an.data(), int32_t(an.size()),
-                              ld.data(), int32_t(ld.size()), fl.data(), int32_t(fl.size() / 2), nullptr, gr.data(),
-                              int32_t(gr.size()), asg.data(), int32_t(asg.size() / 2), &h);
+    // groups of pieces of a discretisation: joints [[strand, arc length], ...] and [strand, piece, group] triples
+    std::vector<int32_t> jst, pcs; std::vector<double> jsv;
+    for (uint32_t i = 0, n = array_len(env, a[9]); i < n; i++) {
+      napi_value e = elem(env, a[9], i);
+      jst.push_back(int32_t(num_at(env, e, 0))); jsv.push_back(num_at(env, e, 1));
+    }
+    for (uint32_t i = 0, n = array_len(env, a[10]); i < n; i++) {
+      napi_value e = elem(env, a[10], i);
+      for (uint32_t c = 0; c < 3; c++) pcs.push_back(int32_t(num_at(env, e, c)));
+    }
+    if (pcs.empty())
+      rc = wsm_from_mesh_groups(V, int32_t(nV / 3), F, int32_t(nF / 3), weld ? 1 : 0, &sc, an.data(), int32_t(an.size()),
+                                ld.data(), int32_t(ld.size()), fl.data(), int32_t(fl.size() / 2), nullptr, gr.data(),
+                                int32_t(gr.size()), asg.data(), int32_t(asg.size() / 2), &h);
+    else
+      rc = wsm_from_mesh_pieces(V, int32_t(nV / 3), F, int32_t(nF / 3), weld ? 1 : 0, &sc, an.data(), int32_t(an.size()),
+                                ld.data(), int32_t(ld.size()), fl.data(), int32_t(fl.size() / 2), nullptr, gr.data(),
+                                int32_t(gr.size()), asg.data(), int32_t(asg.size() / 2), jst.data(), jsv.data(),
+                                int32_t(jst.size()), pcs.data(), int32_t(pcs.size() / 3), &h);
   } else {
     rc = wsm_from_mesh(V, int32_t(nV / 3), F, int32_t(nF / 3), weld ? 1 : 0, &sc, an.data(), int32_t(an.size()),
                        ld.data(), int32_t(ld.size()), fl.data(), int32_t(fl.size() / 2), nullptr, &h);

@@ -4,9 +4,8 @@
 // the file holds, E90 / fc90 / H of older files from their N/mm² defaults in the scene's stress unit — and
 // Mesh.fromBytes (web/js/mesh.js:24-31): STL (binary when the size says so, else ASCII) or OBJ (fan-triangulated),
 // scaled, then welded by the model builder.  A hybrid weave's lath groups and strand assignments (scene.js:50-62) go
-// to the builder with the scene, and so do anchors by every selector, the 'region' one included.  Refused with an error
-// rather than approximated: groups assigned to pieces of a discretisation (hybrid.pieces) unless every group in use
-// equals the scene's own.
+// to the builder with the scene, and so do groups assigned to pieces of a discretisation (hybrid.pieces, with the
+// scene's joints) and anchors by every selector, the 'region' one included.
 //
 //   const S = require('./scene_mi355x.js');
 //   const m = S.loadScene('examples/scenes/dome_basic.json');        // a W.Model, ready for new W.GpuSolver(...)
@@ -84,27 +83,6 @@ const DEFAULTS = {
     stiffness_flat: 1.0, stiffness_edge: 1.0, stiffness_twist: 1.0, stiffness_axial: 1.0, stiffness_shear: 1.0 },
 };
 
-/** a lath group equal to the scene's own material / section / beam: every key at its default when left out, and the
- *  scene's own E90 / fc90 / H defaults in its stress unit (a group's are not rescaled, scene.js:144-155) */
-function sameGroup(d, g, stress) {
-  for (const key of Object.keys(DEFAULTS)) {
-    const a = Object.assign({}, DEFAULTS[key], d[key] || {}), b = Object.assign({}, DEFAULTS[key], g[key] || {});
-    if (key === 'material') for (const k of Object.keys(MATERIAL_MM_N)) if (!(k in (d[key] || {}))) a[k] = MATERIAL_MM_N[k] / stress;
-    const keys = new Set(Object.keys(a).concat(Object.keys(b)));
-    for (const k of keys) if (a[k] !== b[k] && !(a[k] == null && b[k] == null)) return false;
-  }
-  return true;
-}
-
-/** no strand or piece is assigned to a lath group that differs from the scene's own (scene.js:53-72) */
-function hybridIsOneGroup(d, stress) {
-  const h = d.hybrid || {};
-  if (!h.enabled) return true;
-  const groups = h.groups || [];
-  const used = Object.values(h.assign || {}).concat(Object.values(h.pieces || {})).map((v) => v | 0);
-  return used.every((g) => g < 2 || g > 1 + groups.length || sameGroup(d, groups[g - 2], stress));
-}
-
 /** the builder's keys of a material / section / beam, over `into` */
 function fillGroup(into, mat, sec, beam) {
   for (const k of ['E', 'G', 'density', 'E90', 'fc90', 'H']) if (has(mat, k)) into[k] = +mat[k];
@@ -126,7 +104,16 @@ function hybridGroups(d) {
   const groups = (h.groups || []).map((g) => fillGroup(fillGroup({}, DEFAULTS.material, DEFAULTS.section, DEFAULTS.beam),
     g.material || {}, g.section || {}, g.beam || {}));
   const assign = Object.entries(h.assign || {}).map(([k, v]) => [+k | 0, +v | 0]);
-  return { groups, assign };
+  // groups of pieces of a discretisation (scene.js:57-90): a piece is looked up by its key "s:p" (pieceKey,
+  // scene.js:10), so only keys in that exact form apply; the joints are Scene.joints, arc lengths along each strand
+  const pieces = [];
+  for (const [k, v] of Object.entries(h.pieces || {})) {
+    const m = /^(\d+):(\d+)$/.exec(String(k));
+    if (m && String(k) === `${+m[1]}:${+m[2]}`) pieces.push([+m[1], +m[2], +v | 0]);
+  }
+  const joints = [];
+  for (const [k, vs] of Object.entries(d.joints || {})) if (/^\d+$/.test(String(k)) && vs) for (const x of vs) joints.push([+k, +x]);
+  return { groups, assign, joints, pieces };
 }
 
 /** { scene, anchors, loads, flips, mesh, hybrid } of a scene.json object: the scene as the addon's builder reads it */
@@ -139,9 +126,6 @@ function sceneFromJSON(d) {
   if (!(lu in LENGTH_TO_M) || !(fu in FORCE_TO_N)) throw new Error(`unknown units ${lu}, ${fu}`);
   const L = LENGTH_TO_M[lu], Fn = FORCE_TO_N[fu];
   const stress = Fn / (L * L) / 1e6;
-  const hy = d.hybrid || {};
-  const pieces = Object.entries(hy.pieces || {}).filter(([k, v]) => /^\d+:\d+$/.test(String(k)) && (v | 0) >= 2);   // scene.js:11, 169
-  if (hy.enabled && pieces.length && !hybridIsOneGroup(d, stress)) throw new Error('lath groups assigned to pieces of a discretisation (hybrid.pieces) are not modelled by the builder');
   const sc = { length_to_m: L, force_to_n: Fn };
   const mat = d.material || {};
   fillGroup(sc, mat, d.section || {}, d.beam || {});
@@ -198,7 +182,8 @@ function modelFromScene(d, meshBytes, meshName) {
   const s = sceneFromJSON(d);
   const { V, F } = readMesh(meshBytes, meshName || s.mesh.path, s.mesh.scale);
   return Model.fromMesh(V, F, s.scene, { anchors: s.anchors, loads: s.loads, flips: s.flips, weld: true,
-    groups: s.hybrid ? s.hybrid.groups : null, assign: s.hybrid ? s.hybrid.assign : [] });
+    groups: s.hybrid ? s.hybrid.groups : null, assign: s.hybrid ? s.hybrid.assign : [],
+    joints: s.hybrid ? s.hybrid.joints : [], pieces: s.hybrid ? s.hybrid.pieces : [] });
 }
 
 /** the host model of a scene.json file; the mesh path inside it is relative to the file's own folder */

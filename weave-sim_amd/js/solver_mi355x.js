@@ -50,7 +50,8 @@ class Model {
    *  keys) and assign ([strand, group] pairs; strands left out are in group 1, the scene's own) */
   static fromMesh(V, F, scene = {}, opts = {}) {
     return new Model(native().modelFromMesh(Float64Array.from(V), Int32Array.from(F), scene, opts.anchors || [],
-      opts.loads || [], opts.flips || [], opts.weld !== false, opts.groups || null, opts.assign || []));
+      opts.loads || [], opts.flips || [], opts.weld !== false, opts.groups || null, opts.assign || [],
+      opts.joints || [], opts.pieces || []));
   }
   /** md.protocol of the reference (solver_gpu.js:256-318, calibrate.js:345-365): { mode: 'force' | 'displacement' |
    *  null, probe, axis, sched, settle, avg }; the schedule is used as given (flatten the load cycles and drop

@@ -112,9 +112,11 @@ class Model:
 
     @classmethod
     def from_mesh(cls, V, F, scene=None, anchors=(), loads=(), flips=(), noise=0.0, seed=1, weld=True, wavelength=0.0,
-                  groups=None, assign=()):
+                  groups=None, assign=(), joints=None, pieces=()):
         """Kagome weave of a triangle mesh.  A hybrid weave passes `groups` (wsm_group of lath groups 2, 3, …) and
-        `assign` = [(strand, group), ...]; strands left out are in group 1, the scene's own (scene.js:50-62)."""
+        `assign` = [(strand, group), ...]; strands left out are in group 1, the scene's own (scene.js:50-62).
+        Groups of pieces of a discretisation: `joints` = {strand: [arc lengths]} and `pieces` = [(strand, piece,
+        group), ...] (wsm_from_mesh_pieces; scene.js:57-90)."""
         sc = scene or Scene()
         V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1)
         F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1)
@@ -129,8 +131,17 @@ class Model:
             return cls._check(A.model_lib().wsm_from_mesh(*head, C.byref(out)), out)
         gr = (A.wsm_group * max(len(groups), 1))(*groups)
         asg = np.ascontiguousarray(np.array(list(assign), dtype=np.int32).reshape(-1))
-        rc = A.model_lib().wsm_from_mesh_groups(*head, gr, len(groups), asg.ctypes.data_as(A.P_i32), len(asg) // 2,
-                                                C.byref(out))
+        if not pieces:
+            rc = A.model_lib().wsm_from_mesh_groups(*head, gr, len(groups), asg.ctypes.data_as(A.P_i32), len(asg) // 2,
+                                                    C.byref(out))
+            return cls._check(rc, out)
+        js = [(int(s), float(v)) for s, vs in (joints or {}).items() for v in vs]
+        jst = np.ascontiguousarray(np.array([s for s, _ in js], dtype=np.int32))
+        jsv = np.ascontiguousarray(np.array([v for _, v in js], dtype=np.float64))
+        pg = np.ascontiguousarray(np.array(list(pieces), dtype=np.int32).reshape(-1))
+        rc = A.model_lib().wsm_from_mesh_pieces(*head, gr, len(groups), asg.ctypes.data_as(A.P_i32), len(asg) // 2,
+                                                jst.ctypes.data_as(A.P_i32), jsv.ctypes.data_as(A.P_d), len(js),
+                                                pg.ctypes.data_as(A.P_i32), len(pg) // 3, C.byref(out))
         return cls._check(rc, out)
 
     @classmethod

@@ -13,8 +13,9 @@ each strand's segments, crossing sides, nodes and knot lifts take its group's ma
 Anchors take every selector of resolveSelector (model.js:115-125), the 'region' one (graph distance along the mesh
 edges from a seed knot, model.js:127-145) included.
 
-Not modelled by the builder, and refused with an error rather than approximated: groups assigned to pieces of a
-discretisation (Scene.hybrid.pieces) unless every group in use equals the scene's own.
+Groups assigned to pieces of a discretisation (Scene.hybrid.pieces, with the joints of Scene.joints) go to
+wsm_from_mesh_pieces: a segment takes the group of the piece that holds its midpoint, a node the piece that holds
+it, by arc length along the strand (discretize.js:334-358).
 """
 import json
 import os
@@ -103,21 +104,6 @@ _DEFAULTS = {
 }
 
 
-def _groups_match(d, group, stress):
-    """a lath group equal to the scene's own material / section / beam: every key at its default when left out, and
-    the scene's own E90 / fc90 / H defaults in its stress unit (a group's are not rescaled, scene.js:144-155)"""
-    for key, dflt in _DEFAULTS.items():
-        a = dict(dflt, **_sec(d, key))
-        b = dict(dflt, **_sec(group, key))
-        if key == "material":
-            for k, v0 in _MATERIAL_MM_N.items():
-                if k not in _sec(d, key):
-                    a[k] = v0 / stress
-        if a != b:
-            return False
-    return True
-
-
 def _int0(v):
     """JavaScript's `v | 0` of a key or value as the reader takes it (scene.js:168): 0 where it is not a number"""
     try:
@@ -160,15 +146,23 @@ def _hybrid_groups(d):
     return groups, assign
 
 
-def _hybrid_is_one_group(d, stress):
-    """no strand or piece of a discretisation is assigned to a lath group that differs from the scene's own
-    (scene.js:53-72: group 1 is the scene's material / section / beam, assignments name groups 2 …)"""
+def _piece_groups(d):
+    """(joints {strand: [arc lengths]}, [(strand, piece, group)]) of an enabled Scene.hybrid that assigns pieces of a
+    discretisation (scene.js:57-90).  A piece is looked up by its key "s:p" (pieceKey, scene.js:10), so only keys in
+    that exact form can apply; joints are Scene.joints (scene.js:164), arc lengths in scene units along each strand"""
     h = _sec(d, "hybrid")
     if not h.get("enabled"):
-        return True
-    groups = h.get("groups") or []
-    used = {int(v) for v in list((h.get("assign") or {}).values()) + list((h.get("pieces") or {}).values())}
-    return all(_groups_match(d, groups[g - 2], stress) for g in used if 2 <= g <= 1 + len(groups))
+        return None, []
+    pieces = []
+    for k, v in (h.get("pieces") or {}).items():
+        m = re.match(r"^(\d+):(\d+)$", str(k))
+        if m and str(k) == f"{int(m.group(1))}:{int(m.group(2))}":
+            pieces.append((int(m.group(1)), int(m.group(2)), _int0(v)))
+    joints = {}
+    for k, vs in (d.get("joints") or {}).items():
+        if re.match(r"^\d+$", str(k)) and vs:
+            joints[int(k)] = [float(x) for x in vs]
+    return joints, pieces
 
 
 def scene_from_dict(d):
@@ -187,12 +181,8 @@ def scene_from_dict(d):
     c.length_to_m, c.force_to_n = L, Fn
 
     stress = Fn / (L * L) / 1e6                    # scene stress unit in N/mm²
-    h = _sec(d, "hybrid")
-    pieces = [k for k, v in (h.get("pieces") or {}).items() if re.match(r"^\d+:\d+$", str(k)) and _int0(v) >= 2]
-    if h.get("enabled") and pieces and not _hybrid_is_one_group(d, stress):     # keys as parsePieceKey, scene.js:11
-        raise NotImplementedError("lath groups assigned to pieces of a discretisation (hybrid.pieces) are not "
-                                  "modelled by the builder")
     sc.groups, sc.assign = _hybrid_groups(d) or (None, [])
+    sc.joints, sc.pieces = _piece_groups(d)
 
     mat = _sec(d, "material")
     g = _fill_group(c.grp, mat, _sec(d, "section"), _sec(d, "beam"))
@@ -293,7 +283,7 @@ def model_from_scene(d, base_dir=".", mesh=None):
     else:
         V, F = mesh
     return Model.from_mesh(V, F, scene=sc, anchors=anchors, loads=loads, flips=flips, weld=True, groups=sc.groups,
-                           assign=sc.assign)
+                           assign=sc.assign, joints=sc.joints, pieces=sc.pieces)
 
 
 __all__ = ["parse_stl", "parse_obj", "read_mesh", "scene_from_dict", "load_scene", "model_from_scene"]
