@@ -495,8 +495,14 @@ def main():
         print(json.dumps(run_solves(args)))
         return
     if args.mode == "serve":
-        print(json.dumps(run_serve(args)))
-        return
+        # switched off: a run of this mode on an earlier build ended in a host segmentation fault, and no cause has
+        # been found, neither by reading nor on the CPU (the request loop over a stand-in device runtime under
+        # AddressSanitizer, tests/test_server_host.py, runs clean).  The server's own GPU tests cover admission,
+        # the HBM budget, cancellation, mixed request sizes and alone-vs-mix equality.
+        print("bench.py --mode serve is switched off: a run of it once ended in a host segmentation fault whose "
+              "cause is not found; see tests/test_server.py and tests/test_server_host.py for the server",
+              file=sys.stderr)
+        sys.exit(3)
     if args.mode == "stability":
         print(json.dumps(run_stability(args)))
         return
