@@ -143,7 +143,7 @@ typedef struct ws_state {
   double residual;          /* st.residual (inf while warming up) */
   double res_force, res_torque, fref;               /* residualParts and F_ref of the last iteration (out) */
   double ke, e_stretch, e_shear, e_bend, e_twist, e_contact;  /* energies of the last iteration (out) */
-  double res_rigid;         /* residualParts.rigid (out; the rigid velocities restart at rest on an upload) */
+  double res_rigid;         /* residualParts.rigid (out) */
   double *X, *X0, *V;       /* [3n] */
   double *Q, *Q0, *W;       /* [4m], [4m], [3m] */
   double *u;                /* [2K] contact parameters (segment index + fraction) */
@@ -170,6 +170,14 @@ typedef struct ws_state {
   double *floor_F;          /* [3n] the ground's force on every node, world axes (out) */
   uint8_t *floor_state;     /* [n] 0 clear of the ground, 1 stick, 2 slip (out) */
   double e_floor, fn_floor; /* the ground's energy and largest normal force of the last iteration (out) */
+  /* the rigid-body freedoms of a structure only the ground holds (Floor.rv / rw / c / Rb; the reference's upload
+     writes rv and rw, solver_gpu.js:228-240, and its download reads all three back, :454-461).  An upload takes the
+     velocities, and the model radius when positive (0: from the uploaded positions, as the Floor constructor
+     computes it, solver_cpu.js:546-550); the reference point starts at the centroid of the uploaded positions.  A
+     download returns all four; they are 0 for an instance without these freedoms. */
+  double rb_v[3], rb_w[3];  /* rigid translation / rotation per iteration (in/out) */
+  double rb_c[3];           /* reference point of the rigid rotation: the centroid of the positions (out) */
+  double rb_radius;         /* model radius about the centroid (in/out; 0 on input: from X) */
 } ws_state;
 
 /* Per-instance control record, the reference's CT struct as readControl returns it (solver_gpu.js:374-386). */

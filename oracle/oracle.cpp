@@ -191,6 +191,9 @@ struct Oracle {
       for (int i = 0; i < n; i++) R2 = std::max(R2, (X[3 * i] - c[0]) * (X[3 * i] - c[0]) + (X[3 * i + 1] - c[1]) * (X[3 * i + 1] - c[1]) + (X[3 * i + 2] - c[2]) * (X[3 * i + 2] - c[2]));
       for (int k = 0; k < 3; k++) fc[k] = c[k];
       fRb = std::max(std::sqrt(R2), 1e-9);
+      // a state that carries the rigid freedoms (a download of them: Floor.rv / rw, and Rb as the Floor kept it)
+      if (st.rb_radius > 0) fRb = st.rb_radius;
+      for (int k = 0; k < 3; k++) { rv[k] = st.rb_v[k]; rw[k] = st.rb_w[k]; }
       double hmin = INFINITY;
       for (int i = 0; i < n; i++) hmin = std::min(hmin, std::min(nodeH[i], nodeW[i]));
       vmax = RB_VMAX * (std::isfinite(hmin) && hmin > 0 ? hmin : 1.0); wmax = vmax / fRb;
@@ -884,6 +887,16 @@ struct Oracle {
     } else {
       o.e_floor = 0; o.fn_floor = 0;
     }
+    // the rigid freedoms (Floor.rv / rw, Rb) and the reference point the next evaluation takes: the centroid of the
+    // current positions (Floor.apply, solver_cpu.js:590)
+    const bool rb = floorOn && !anyAnchor;
+    double cs[3] = {0, 0, 0};
+    if (rb && n) {
+      for (int i = 0; i < n; i++) for (int k = 0; k < 3; k++) cs[k] += X[3 * i + k];
+      for (int k = 0; k < 3; k++) cs[k] /= n;
+    }
+    for (int k = 0; k < 3; k++) { o.rb_v[k] = rb ? rv[k] : 0.0; o.rb_w[k] = rb ? rw[k] : 0.0; o.rb_c[k] = cs[k]; }
+    o.rb_radius = rb ? fRb : 0.0;
   }
 };
 

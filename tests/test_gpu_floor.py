@@ -95,6 +95,47 @@ def test_unanchored_lath_matches_oracle(W, gpu_ctx, case, precision, tol):
     g.close()
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 1e-5)])
+def test_download_upload_carries_the_rigid_freedoms(W, gpu_ctx, precision, tol):
+    """a lath sliding as a whole, downloaded after 300 iterations and uploaded into a new batch, goes on as the oracle
+    does without the break: the download carries the rigid velocities, the reference point and the model radius
+    (Floor.rv / rw / c / Rb, solver_gpu.js:461) and an upload takes the velocities and the radius back
+    (solver_gpu.js:236-237).  Uploaded with the velocities zeroed, the same state leaves the oracle's trajectory."""
+    m = cases.ground_lath(push=6.0)
+    o = Oracle(m)
+    o.step(300)
+    a = o.state()
+    g = W.GpuSolver([m], precision=precision)
+    g.step(300)
+    s = g.state(0)
+    g.close()
+    X = s["X"].reshape(-1, 3)
+    rv, rva = np.array(s.st.rb_v[:]), np.array(a.st.rb_v[:])
+    rw, rwa = np.array(s.st.rb_w[:]), np.array(a.st.rb_w[:])
+    print(f"{precision}: rb_v {rv} (oracle {rva}), rb_w {rw} (oracle {rwa}), rb_radius {s.st.rb_radius}", flush=True)
+    assert abs(rv[0]) > 1e-3                                   # sliding along the push
+    assert np.abs(rv - rva).max() <= tol * 10.0 and np.abs(rw - rwa).max() <= tol * 0.1
+    assert np.abs(np.array(s.st.rb_c[:]) - X.mean(0)).max() <= 1e-12 * 100.0
+    assert s.st.rb_radius == a.st.rb_radius
+    o.step(400)
+    ref = o.state()["X"]
+    h = W.GpuSolver([m], precision=precision)
+    h.upload(0, m, s)
+    h.step(400)
+    b = h.state(0)
+    for k in range(3):
+        s.st.rb_v[k] = 0.0
+        s.st.rb_w[k] = 0.0
+    h.upload(0, m, s)
+    h.step(400)
+    z = h.state(0)
+    h.close()
+    dX, dZ = np.abs(ref - b["X"]).max(), np.abs(ref - z["X"]).max()
+    print(f"{precision}: resumed |dX| vs oracle {dX:.3e}, resumed at rest {dZ:.3e}", flush=True)
+    assert b.st.iteration == 700 and dX <= tol * 100.0
+    assert dZ > 10.0 * tol * 100.0
+
+
 def test_unanchored_lath_rests_on_the_ground(W, gpu_ctx):
     """relaxed to the 1e-6 residual with the rigid freedoms on: the ground carries the whole weight (floor_check.mjs
     rest case) and the device stops where the oracle does"""

@@ -333,6 +333,32 @@ console.log(JSON.stringify(res));""")
         assert v["bad"] == [], (name, v["bad"])
 
 
+def test_cpu_solver_ground_rigid_freedoms_map_both_ways():
+    """a CpuSolver-shaped object with a ground (the reference's Floor: rv, rw, c, Rb) maps its rigid velocities and
+    model radius onto ws_state (rb_v, rb_w, rb_radius; read on upload, as _uploadFloor writes rv and rw,
+    solver_gpu.js:236-237), and a download's rigid freedoms land in the Floor's own arrays (solver_gpu.js:461)"""
+    out = run_node(f"""
+const {{ asCpuSolver }} = require({json.dumps(SHAPE)});
+const m = W.Model.hexDome({{span:3000,spacing:300,segment:50,contraction:0.15,noise:1,seed:1}}, {SCENE});
+const cpu = asCpuSolver(m.fields());
+const n = cpu.st.n;
+const ax = new Float64Array(n), ay = new Float64Array(n);
+for (let i = 0; i < n; i++) {{ ax[i] = cpu.st.X[3 * i]; ay[i] = cpu.st.X[3 * i + 1]; }}
+const rv = Float64Array.of(0.25, -0.5, 0.125), rw = Float64Array.of(1e-3, -2e-3, 3e-3), c = new Float64Array(3);
+cpu.floor = {{ p: {{ z: 0, mu: 0.4 }}, w: new Float64Array(n).fill(40), h: new Float64Array(n).fill(4), ax, ay,
+  F: new Float64Array(3 * n), state: new Uint8Array(n), rv, rw, c, Rb: 1234.5 }};
+const g = W._internal.modelFields(W._internal.modelFromFields(W._internal.cpuSolverFields(cpu).fields));
+const inst = new W._internal.CpuSolverInstance(cpu);
+inst.afterDownload({{ iteration: 7, residual: 1, kePrev: 0, converged: 0, resForce: 0, resTorque: 0, resRigid: 0.5,
+  eBend: 0, eTwist: 0, eStretch: 0, eShear: 0, eContact: 0, penN: 0, eFloor: 0, fnFloor: 0,
+  rbV: [1, 2, 3], rbW: [4, 5, 6], rbC: [7, 8, 9] }});
+console.log(JSON.stringify({{ v: Array.from(g.rb_v), w: Array.from(g.rb_w), R: g.rb_radius, same: cpu.floor.rv === rv && cpu.floor.c === c,
+  rv: Array.from(cpu.floor.rv), rw: Array.from(cpu.floor.rw), c: Array.from(cpu.floor.c) }}));""")
+    d = json.loads(out)
+    assert d["v"] == [0.25, -0.5, 0.125] and d["w"] == [1e-3, -2e-3, 3e-3] and d["R"] == 1234.5
+    assert d["same"] and d["rv"] == [1, 2, 3] and d["rw"] == [4, 5, 6] and d["c"] == [7, 8, 9]
+
+
 def test_js_tables_format_like_python():
     """knotsCSV / segmentsCSV in Node give the Python writers' text byte for byte (floats as Python's repr) over the
     same state, and resultsJSON keys knots by mesh edge id"""

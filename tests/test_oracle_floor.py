@@ -65,3 +65,33 @@ def test_pinned():
     R = o.state()["reactions"].reshape(-1, 3)
     assert o.converged and rigid == 0.0
     assert abs(F[:, 2].sum() + R[:, 2].sum() + W[2]) < 1e-3 * abs(W[2])
+
+
+def test_resume_carries_the_rigid_freedoms():
+    """a state exported while the lath slides as a whole and imported into a new oracle continues the same trajectory
+    bit for bit: the rigid velocities and the model radius travel with the state (Floor.rv / rw / Rb; the reference's
+    upload writes rv and rw, solver_gpu.js:236-237), and the reference point is the centroid of the positions
+    (Floor.apply, solver_cpu.js:590).  Imported without the velocities, the same state departs from that trajectory."""
+    m = cases.ground_lath(push=6.0)
+    a = Oracle(m)
+    a.step(700)
+    b = Oracle(m)
+    b.step(300)
+    s = b.state()
+    X = s["X"].reshape(-1, 3)
+    rv = np.array(s.st.rb_v[:])
+    assert abs(rv[0]) > 1e-3                                   # sliding along the push: the rigid translation is live
+    assert np.abs(np.array(s.st.rb_c[:]) - X.mean(0)).max() < 1e-12 * np.abs(X).max()
+    X0 = m["X"].reshape(-1, 3)
+    R = np.sqrt(((X0 - X0.mean(0)) ** 2).sum(1)).max()
+    assert abs(s.st.rb_radius - R) < 1e-12 * R                 # Rb of the Floor's construction
+    c = Oracle(m, state=s)
+    c.step(400)
+    assert c.iteration == a.iteration
+    assert np.array_equal(c.state()["X"], a.state()["X"])
+    for k in range(3):
+        s.st.rb_v[k] = 0.0
+        s.st.rb_w[k] = 0.0
+    d = Oracle(m, state=s)
+    d.step(400)
+    assert np.abs(d.state()["X"] - a.state()["X"]).max() > 1e-3

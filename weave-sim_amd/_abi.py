@@ -60,7 +60,8 @@ class ws_state(C.Structure):
     _fields_ = [(n, i32) for n in ("iteration", "converged", "pen_n", "pad0")] + \
                [(n, d) for n in ("ke_prev", "residual", "res_force", "res_torque", "fref", "ke", "e_stretch",
                                  "e_shear", "e_bend", "e_twist", "e_contact", "res_rigid")] + \
-               [(n, t) for (n, t, _) in STATE_ARRAYS] + [("e_floor", d), ("fn_floor", d)]
+               [(n, t) for (n, t, _) in STATE_ARRAYS] + [("e_floor", d), ("fn_floor", d)] + \
+               [("rb_v", d * 3), ("rb_w", d * 3), ("rb_c", d * 3), ("rb_radius", d)]
 
 
 class ws_control(C.Structure):

@@ -193,8 +193,9 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
   d.pax = md.proto_axis[0]; d.pay = md.proto_axis[1]; d.paz = md.proto_axis[2];
   if (md.proto_mode) { d.p0x = st.X[3 * md.proto_probe]; d.p0y = st.X[3 * md.proto_probe + 1]; d.p0z = st.X[3 * md.proto_probe + 2]; }
   if (rigid) {
-    // model radius about the centroid and the rigid speed limits, from the state uploaded at construction (the
-    // Floor constructor, solver_cpu.js:546-562)
+    // model radius about the centroid and the rigid speed limits (the Floor constructor, solver_cpu.js:546-562): the
+    // radius from the uploaded positions unless the state carries one (a downloaded state's, as the reference's
+    // md.floor keeps Rb from its construction)
     double c[3] = {0, 0, 0};
     for (int i = 0; i < n; i++) for (int k = 0; k < 3; k++) c[k] += st.X[3 * i + k] / std::max(n, 1);
     double R2 = 0, hmin = INFINITY;
@@ -203,7 +204,7 @@ void build_instance(InstHost<Real>& H, const ws_model& md, const ws_state& st, i
       R2 = std::max(R2, dx * dx + dy * dy + dz * dz);
       hmin = std::min(hmin, std::min(md.node_h[i], md.node_w[i]));
     }
-    d.rb_R = std::max(std::sqrt(R2), 1e-9);
+    d.rb_R = st.rb_radius > 0 ? st.rb_radius : std::max(std::sqrt(R2), 1e-9);
     d.rb_vmax = RB_VMAX * (std::isfinite(hmin) && hmin > 0 ? hmin : 1.0);
     d.rb_wmax = d.rb_vmax / d.rb_R;
   }
@@ -405,11 +406,11 @@ void upload_state(InstHost<Real>& H, const ws_state& st) {
     if (d.proto == 1) { c.pf[0] = d.pax * v0; c.pf[1] = d.pay * v0; c.pf[2] = d.paz * v0; }
   }
   if (d.rigid) {
-    // the reference point of the first evaluation: the centroid of the uploaded nodes (Floor.apply, :588); the rigid
-    // velocities start at rest
+    // the reference point of the first evaluation: the centroid of the uploaded nodes (Floor.apply,
+    // solver_cpu.js:590); the rigid velocities as the state carries them (_uploadFloor, solver_gpu.js:236-237)
     double s[3] = {0, 0, 0};
     for (int i = 0; i < n; i++) for (int k = 0; k < 3; k++) s[k] += st.X[3 * i + k];
-    for (int k = 0; k < 3; k++) { c.rc[k] = s[k] / n; c.rc_prev[k] = c.rc[k]; }
+    for (int k = 0; k < 3; k++) { c.rc[k] = s[k] / n; c.rc_prev[k] = c.rc[k]; c.rv[k] = st.rb_v[k]; c.rw[k] = st.rb_w[k]; }
   }
   HIPCHK(hipMemcpy(d.ctl, &c, sizeof(Ctl), hipMemcpyHostToDevice));
   H.hist_read = 0;
@@ -1082,6 +1083,11 @@ struct Batch : BatchBase {
     o.res_force = c.res_f; o.res_torque = c.res_t; o.fref = c.fref; o.ke = c.ke; o.res_rigid = c.res_rb;
     o.e_stretch = c.e_stretch; o.e_shear = c.e_shear; o.e_bend = c.e_bend; o.e_twist = c.e_tors; o.e_contact = c.e_cont;
     o.e_floor = d.floor_on ? c.e_floor : 0.0; o.fn_floor = d.floor_on ? c.fn_floor : 0.0;
+    // the rigid freedoms (Floor.rv / rw / c, solver_gpu.js:461) and the model radius the instance runs with
+    for (int k = 0; k < 3; k++) {
+      o.rb_v[k] = d.rigid ? c.rv[k] : 0.0; o.rb_w[k] = d.rigid ? c.rw[k] : 0.0; o.rb_c[k] = d.rigid ? c.rc[k] : 0.0;
+    }
+    o.rb_radius = d.rigid ? d.rb_R : 0.0;
     return WS_OK;
   }
 

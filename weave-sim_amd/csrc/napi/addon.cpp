@@ -162,10 +162,23 @@ napi_value DeviceInfo(napi_env env, napi_callback_info info) {
 }
 
 // ---------------------------------------------------------------------------------------------- host models
+napi_value arr3(napi_env env, const double* v) {
+  napi_value a; napi_create_array_with_length(env, 3, &a);
+  for (uint32_t k = 0; k < 3; k++) napi_set_element(env, a, k, num(env, v[k]));
+  return a;
+}
+// three numbers under key k: an array or a typed array of length 3 (absent or another shape: out unchanged)
 void get3(napi_env env, napi_value o, const char* k, double* out) {
-  bool has = false; napi_value v; bool is = false; uint32_t n = 0;
+  bool has = false; napi_value v; bool is = false, ta = false; uint32_t n = 0;
   if (napi_has_named_property(env, o, k, &has) != napi_ok || !has || napi_get_named_property(env, o, k, &v) != napi_ok) return;
-  if (napi_is_array(env, v, &is) != napi_ok || !is || napi_get_array_length(env, v, &n) != napi_ok || n != 3) return;
+  if (napi_is_array(env, v, &is) == napi_ok && is) {
+    if (napi_get_array_length(env, v, &n) != napi_ok) return;
+  } else if (napi_is_typedarray(env, v, &ta) == napi_ok && ta) {
+    size_t len = 0;
+    if (napi_get_typedarray_info(env, v, nullptr, &len, nullptr, nullptr, nullptr) != napi_ok) return;
+    n = uint32_t(len);
+  }
+  if (n != 3) return;
   for (uint32_t i = 0; i < 3; i++) {
     napi_value e; double x = 0;
     if (napi_get_element(env, v, i, &e) == napi_ok && napi_get_value_double(env, e, &x) == napi_ok) out[i] = x;
@@ -322,6 +335,8 @@ napi_value ModelFromFields(napi_env env, napi_callback_info info) {
   if (bad.empty() && (!st.X || !st.Q || (K && !st.u))) bad = "X, Q (and u with crossings) are needed";
   st.iteration = I("iteration"); st.pen_n = I("pen_n"); st.ke_prev = getd(env, f, "ke_prev", 0.0);
   st.residual = getd(env, f, "residual", INFINITY);
+  // the rigid freedoms of a structure only the ground holds (Floor.rv / rw / Rb): [3] arrays and a number
+  get3(env, f, "rb_v", st.rb_v); get3(env, f, "rb_w", st.rb_w); st.rb_radius = getd(env, f, "rb_radius", 0.0);
   if (!bad.empty()) {
     for (napi_ref r : m->refs) napi_delete_reference(env, r);
     napi_throw_type_error(env, nullptr, ("modelFromFields: " + bad).c_str());
@@ -387,11 +402,13 @@ napi_value ModelFields(napi_env env, napi_callback_info info) {
   CD("Fn", st.Fn, K); CD("Ft", st.Ft, K); CD("gap", st.gap, K); CD("kstate", st.kstate, K); CD("theta", st.theta, K);
   CD("Anom", st.Anom, K); CD("pressure", st.pressure, K); CD("util", st.util, K); CD("Mrot", st.Mrot, K);
   CD("phi", st.phi, 2 * K); CD("mid", st.mid, 3 * K); CI("ctype", st.ctype, K); CI("face_side", st.face_side, K);
+  CD("rb_v", st.rb_v, 3); CD("rb_w", st.rb_w, 3);
 #undef CI
 #undef CD
 #undef CU
   set(env, o, "iteration", i32(env, st.iteration)); set(env, o, "residual", num(env, st.residual));
   set(env, o, "ke_prev", num(env, st.ke_prev)); set(env, o, "pen_n", i32(env, st.pen_n));
+  set(env, o, "rb_radius", num(env, st.rb_radius));
   return o;
 }
 
@@ -665,6 +682,8 @@ napi_value scalars(napi_env env, const ws_state& s) {
   set(env, o, "eTwist", num(env, s.e_twist)); set(env, o, "eContact", num(env, s.e_contact));
   set(env, o, "resRigid", num(env, s.res_rigid)); set(env, o, "penN", i32(env, s.pen_n));
   set(env, o, "eFloor", num(env, s.e_floor)); set(env, o, "fnFloor", num(env, s.fn_floor));
+  set(env, o, "rbV", arr3(env, s.rb_v)); set(env, o, "rbW", arr3(env, s.rb_w)); set(env, o, "rbC", arr3(env, s.rb_c));
+  set(env, o, "rbRadius", num(env, s.rb_radius));
   return o;
 }
 

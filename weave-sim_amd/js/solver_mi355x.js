@@ -160,7 +160,9 @@ function cpuSolverFields(md) {
     for (let i = 0; i < n; i++) { floorA[2 * i] = fl.ax[i]; floorA[2 * i + 1] = fl.ay[i]; }
     Object.assign(f, { node_w: asType(fl.w, Float64Array), node_h: asType(fl.h, Float64Array), floor_a: floorA,
       floor_F: stateArr(fl.F || new Float64Array(3 * n), Float64Array),
-      floor_state: stateArr(fl.state || new Uint8Array(n), Uint8Array) });
+      floor_state: stateArr(fl.state || new Uint8Array(n), Uint8Array),
+      // the rigid freedoms the Floor carries (uploaded as _uploadFloor writes them, solver_gpu.js:236-237) and its Rb
+      rb_v: Array.from(fl.rv || [0, 0, 0]), rb_w: Array.from(fl.rw || [0, 0, 0]), rb_radius: +fl.Rb || 0 });
   }
   return { fields: f, back, floorA };
 }
@@ -191,6 +193,12 @@ class CpuSolverInstance {
       const fl = md.floor, rampN = Math.max(md.scene.contact.ramp_iter | 0, 0), it = Math.max(s.iteration - 1, 0);
       fl.energy = s.eFloor; fl.FnMax = s.fnFloor;
       fl.ramp = rampN <= 0 ? 1.0 : Math.min(1.0, 0.01 + 0.99 * it / rampN);
+      // its rigid velocities and reference point (solver_gpu.js:461)
+      for (let k = 0; k < 3; k++) {
+        if (fl.rv) fl.rv[k] = s.rbV[k];
+        if (fl.rw) fl.rw[k] = s.rbW[k];
+        if (fl.c) fl.c[k] = s.rbC[k];
+      }
     }
     // the frame inertia of the state (CpuSolver.step), when the object carries the reference's methods
     if (md.inertia3 && typeof rod.updateBendCoupling === 'function') {
@@ -505,4 +513,5 @@ class Server {
 
 module.exports = { getDevice, Model, GpuSolver, Server, buildResults, resultsJSON, knotsCSV, segmentsCSV, FP32_FP64, FP64,
   // the CpuSolver binding, for its round-trip test: the fields it builds and the native model over them
-  _internal: { cpuSolverFields, modelFromFields: (f) => native().modelFromFields(f), modelFields: (h) => native().modelFields(h) } };
+  _internal: { cpuSolverFields, CpuSolverInstance, modelFromFields: (f) => native().modelFromFields(f),
+    modelFields: (h) => native().modelFields(h) } };
