@@ -187,6 +187,7 @@ typedef struct ws_control {
   double ke_prev, ke, resid, fref, e_stretch, e_bend, e_tors, e_cont, e_shear, res_t, res_f;
   double res_rb, rv[3], rw[3];   /* rigid-body freedoms (ground only): residual part, translation / rotation per iteration */
   double e_floor, fn_floor;      /* the ground's energy and largest normal force (eFloor, fnFloor) */
+  double rc[3];                  /* reference point of the rigid rotation (rc; ground only, else 0) */
 } ws_control;
 
 typedef struct ws_batch_opts {

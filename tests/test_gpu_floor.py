@@ -106,10 +106,11 @@ def test_download_upload_carries_the_rigid_freedoms(W, gpu_ctx, precision, tol):
     o.step(300)
     a = o.state()
     g = W.GpuSolver([m], precision=precision)
-    g.step(300)
+    c = g.step(300)[0]
     s = g.state(0)
     g.close()
     X = s["X"].reshape(-1, 3)
+    assert c["rv"] == list(s.st.rb_v) and c["rw"] == list(s.st.rb_w) and c["rc"] == list(s.st.rb_c)   # readControl
     rv, rva = np.array(s.st.rb_v[:]), np.array(a.st.rb_v[:])
     rw, rwa = np.array(s.st.rb_w[:]), np.array(a.st.rb_w[:])
     print(f"{precision}: rb_v {rv} (oracle {rva}), rb_w {rw} (oracle {rwa}), rb_radius {s.st.rb_radius}", flush=True)

@@ -68,7 +68,7 @@ class ws_control(C.Structure):
     _fields_ = [(n, i32) for n in ("iter", "done", "rollback", "al_update", "stop_at", "hist_n", "phase", "inc")] + \
                [(n, d) for n in ("ke_prev", "ke", "resid", "fref", "e_stretch", "e_bend", "e_tors", "e_cont",
                                  "e_shear", "res_t", "res_f", "res_rb")] + \
-               [("rv", d * 3), ("rw", d * 3), ("e_floor", d), ("fn_floor", d)]
+               [("rv", d * 3), ("rw", d * 3), ("e_floor", d), ("fn_floor", d), ("rc", d * 3)]
 
 
 class ws_batch_opts(C.Structure):

@@ -824,7 +824,7 @@ struct Batch : BatchBase {
     o.ke_prev = c.ke_prev; o.ke = c.ke; o.resid = c.resid; o.fref = c.fref; o.e_stretch = c.e_stretch; o.e_bend = c.e_bend;
     o.e_tors = c.e_tors; o.e_cont = c.e_cont; o.e_shear = c.e_shear; o.res_t = c.res_t; o.res_f = c.res_f;
     o.res_rb = c.res_rb;
-    for (int k = 0; k < 3; k++) { o.rv[k] = c.rv[k]; o.rw[k] = c.rw[k]; }
+    for (int k = 0; k < 3; k++) { o.rv[k] = c.rv[k]; o.rw[k] = c.rw[k]; o.rc[k] = c.rc[k]; }
     o.phase = c.phase; o.inc = c.inc;
     o.e_floor = c.e_floor; o.fn_floor = c.fn_floor;
     return o;

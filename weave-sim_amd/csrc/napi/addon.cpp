@@ -50,6 +50,11 @@ struct Server { ws_ctx* ctx = nullptr; ws_server* s = nullptr; };
 // ---------------------------------------------------------------------------------------------- helpers
 napi_value undef(napi_env env) { napi_value v; napi_get_undefined(env, &v); return v; }
 napi_value num(napi_env env, double x) { napi_value v; napi_create_double(env, x, &v); return v; }
+napi_value arr3(napi_env env, const double* v) {
+  napi_value a; napi_create_array_with_length(env, 3, &a);
+  for (uint32_t k = 0; k < 3; k++) napi_set_element(env, a, k, num(env, v[k]));
+  return a;
+}
 napi_value i32(napi_env env, int32_t x) { napi_value v; napi_create_int32(env, x, &v); return v; }
 napi_value str(napi_env env, const char* s) { napi_value v; napi_create_string_utf8(env, s, NAPI_AUTO_LENGTH, &v); return v; }
 void set(napi_env env, napi_value o, const char* k, napi_value v) { napi_set_named_property(env, o, k, v); }
@@ -95,6 +100,7 @@ napi_value ctl_object(napi_env env, const ws_control& c) {
   for (uint32_t k = 0; k < 3; k++) { napi_set_element(env, rv, k, num(env, c.rv[k])); napi_set_element(env, rw, k, num(env, c.rw[k])); }
   set(env, o, "rv", rv); set(env, o, "rw", rw);
   set(env, o, "eFloor", num(env, c.e_floor)); set(env, o, "fnFloor", num(env, c.fn_floor));
+  set(env, o, "rc", arr3(env, c.rc));
   return o;
 }
 
@@ -162,11 +168,6 @@ napi_value DeviceInfo(napi_env env, napi_callback_info info) {
 }
 
 // ---------------------------------------------------------------------------------------------- host models
-napi_value arr3(napi_env env, const double* v) {
-  napi_value a; napi_create_array_with_length(env, 3, &a);
-  for (uint32_t k = 0; k < 3; k++) napi_set_element(env, a, k, num(env, v[k]));
-  return a;
-}
 // three numbers under key k: an array or a typed array of length 3 (absent or another shape: out unchanged)
 void get3(napi_env env, napi_value o, const char* k, double* out) {
   bool has = false; napi_value v; bool is = false, ta = false; uint32_t n = 0;
