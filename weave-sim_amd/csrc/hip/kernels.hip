@@ -392,6 +392,18 @@ __device__ __forceinline__ double xchg(double v) { return __shfl_xor(v, 1, 64); 
 __device__ __forceinline__ float xchg(float v) { return __shfl_xor(v, 1, 64); }
 __device__ __forceinline__ int xchg(int v) { return __shfl_xor(v, 1, 64); }
 
+// XCD-aware block mapping: consecutive logical blocks run on the same XCD and its L2 (the hardware deals workgroups
+// round-robin over the 8 XCDs).  Neighbouring strand tiles of one lath share halo nodes; the work lists are ordered by
+// instance, so an instance's crossing blocks and control workgroups land about where its strand tiles ran
+__device__ __forceinline__ int xcd_tile(int bid, int nblk) {
+  const int nx = 8;
+  const int per = (nblk + nx - 1) / nx;
+  const int x = bid % nx, r = bid / nx;
+  const int full = nblk % nx == 0 ? nx : nblk % nx;   // XCDs that receive `per` blocks
+  int base = x < full ? x * per : full * per + (x - full) * (per - 1);
+  return base + r;
+}
+
 // LANES = 2: lane 2j evaluates side A of crossing c0 + j, lane 2j + 1 side B.  The side evaluations (the neighbour
 // loads, tangents, node frames and the Hermite point) run in parallel; the lanes then swap the partner's point,
 // tangent and frame and both compute the crossing's normal, penetration and friction law (the same instructions on
@@ -413,7 +425,7 @@ __global__ __launch_bounds__(CROSS_THREADS) __attribute__((amdgpu_waves_per_eu(W
   static_assert(LANES == 1 || LANES == 2, "one or two lanes per crossing");
   using G = double;
   using H = Real;
-  int bid = blockIdx.x;
+  const int bid = xcd_tile(blockIdx.x, gridDim.x);
   if (bid >= nitems) return;
   // the work list holds blocks of occupied slots only (it.inst >= 0); the liveness test waits until the first
   // crossing loads are out, so the item, the control record with the descriptor, and the crossing state cost three
@@ -1613,8 +1625,9 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   __shared__ double stage[CTRL_STAGE * RPART];
   __shared__ double red_rb[RBP][CTRL_THREADS / 64];
   __shared__ int last;
-  if (int(blockIdx.x) >= nitems) return;
-  const CtrlItem it = ONE ? CtrlItem{0, int(blockIdx.x), int(gridDim.x), 0} : items[blockIdx.x];
+  const int cb = xcd_tile(blockIdx.x, gridDim.x);
+  if (cb >= nitems) return;
+  const CtrlItem it = ONE ? CtrlItem{0, cb, int(gridDim.x), 0} : items[cb];
   const Inst<Real>& I = insts[ONE ? 0 : it.inst];
   Ctl* ctl = ctls + (ONE ? 0 : it.inst);           // read beside the descriptor, not through it
   const int tid = threadIdx.x;
@@ -1723,16 +1736,6 @@ __global__ __launch_bounds__(CTRL_THREADS) void k_ctrl(const Inst<Real>* __restr
   control_step(I, ctl, c, acc, rbs);
 }
 
-// XCD-aware block mapping: consecutive logical tiles (neighbouring tiles of one lath share halo nodes) run on the
-// same XCD and its L2; the hardware deals workgroups round-robin over the 8 XCDs
-__device__ __forceinline__ int xcd_tile(int bid, int nblk) {
-  const int nx = 8;
-  const int per = (nblk + nx - 1) / nx;
-  const int x = bid % nx, r = bid / nx;
-  const int full = nblk % nx == 0 ? nx : nblk % nx;   // XCDs that receive `per` blocks
-  int base = x < full ? x * per : full * per + (x - full) * (per - 1);
-  return base + r;
-}
 
 // RB: the batch holds an instance that only the ground holds (its tiles also leave the rigid-body sums); the other
 // batches run the form without them
