@@ -458,8 +458,8 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="instances of the variants workload")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--tile", type=int, default=None,
-                    help="strand tile (64/128/256); 0 = the default (64); unset: the autotuner's pick on the variants "
-                         "workload, the default elsewhere")
+                    help="strand tile (64/128/256); 0 = the default (64); unset: the autotuner's pick (tile, graph "
+                         "length and crossing layout)")
     ap.add_argument("--per-submit", type=int, default=0, help="iterations per captured graph (0 = the default, 64)")
     ap.add_argument("--cross-lanes", type=int, default=0, help="lanes per crossing, 1 or 2 (0 = by size)")
     ap.add_argument("--streams", type=int, default=0,
@@ -513,14 +513,11 @@ def main():
     # the autotuner and the counter passes run as child processes, before this process touches the GPU
     tune = None
     if args.tile is None:
-        if args.workload == "variants":
-            log("autotuning (child process)")
-            tune = json.loads(run_child(args, ["--tune-only"], 900).strip().splitlines()[-1])
-            log(f"autotuner's pick: {tune['best']}")
-            args.tile, args.per_submit, args.cross_lanes = (tune["best"]["tile"], tune["best"]["per_submit"],
-                                                            tune["best"]["cross_lanes"])
-        else:
-            args.tile = 0
+        log("autotuning (child process)")
+        tune = json.loads(run_child(args, ["--tune-only"], 900).strip().splitlines()[-1])
+        log(f"autotuner's pick: {tune['best']}")
+        args.tile, args.per_submit, args.cross_lanes = (tune["best"]["tile"], tune["best"]["per_submit"],
+                                                        tune["best"]["cross_lanes"])
     if not args.no_traffic:
         log("counter passes (FETCH_SIZE, WRITE_SIZE)")
     traffic, traffic_detail = (None, {"skipped": True}) if args.no_traffic else measure_traffic(args)
