@@ -202,6 +202,30 @@ def test_step_subsets_and_stop_at(W, gpu_ctx):
     g.close()
 
 
+def test_graph_build_replay_leaves_every_instance_alone(W, gpu_ctx):
+    """building the iteration graphs replays each of them once with no live iteration: after an upload (which
+    rebuilds the graphs) a step of one selected instance leaves the other's iteration and state untouched, and the
+    stepped instance's state is bitwise that of a batch whose graphs were never rebuilt"""
+    ms = [cases.config_small(seed=s) for s in (1, 2)]   # past the 300-iteration contact warm-up: the nodes move
+    ref = W.GpuSolver(ms, precision="fp32")
+    ref.step(400)
+    ref.step(37, instances=[1])
+    r0, r1 = ref.state(0), ref.state(1)
+    ref.close()
+    g = W.GpuSolver(ms, precision="fp32")
+    g.step(400)
+    s0 = g.state(0)
+    g.upload(0, ms[0], s0)                       # the instance's own state back: the graphs are captured again
+    c = g.step(37, instances=[1])
+    a0, a1 = g.state(0), g.state(1)
+    g.close()
+    assert [x["iter"] for x in c] == [400, 437]
+    for k in ("X", "Q", "V", "u"):
+        assert np.array_equal(a0[k], s0[k]), k
+        assert np.array_equal(a0[k], r0[k]), k
+        assert np.array_equal(a1[k], r1[k]), k
+
+
 def test_outputs_match_oracle(W, gpu_ctx):
     """section forces, joint moments and reactions of the last evaluated state (download) against the oracle"""
     m = cases.crossing(**cases.CROSS_CASES["cross_stick"])
