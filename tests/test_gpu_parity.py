@@ -114,6 +114,24 @@ def test_dome_trajectory_window(W, gpu_ctx, precision, tol, window, lanes):
             assert abs(c["resid"] - o.residual) <= 1e-3 * o.residual + (1e-6 if precision == "fp64" else 1e-3)
 
 
+@pytest.mark.parametrize("tile", [128, 256])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 1e-4)])
+def test_dome_window_in_wider_strand_tiles(W, gpu_ctx, tile, precision, tol):
+    """the single-instance form with strand tiles of two and four wavefronts (the autotuner's pick for one dome):
+    the 3 m dome against the oracle through 600 iterations, positions and stick/slip"""
+    m = cases.config1()
+    o = Oracle(m)
+    g = W.GpuSolver([m], precision=precision, tile=tile)
+    assert g.opts["tile"] == tile
+    for n in (300, 600):
+        o.step(n - o.iteration)
+        g.step(n - g.read_control()[0]["iter"])
+        a, b = o.state(), g.state(0)
+        assert np.abs(a["X"] - b["X"]).max() <= tol * m.span, (n, np.abs(a["X"] - b["X"]).max())
+        assert np.array_equal(a["stuck"], b["stuck"]), n
+    g.close()
+
+
 @pytest.mark.parametrize("precision,tol_f,tol_c,mism", [("fp64", 1e-6, 1e-6, 0), ("fp32", 1e-3, 1e-2, 0.01)])
 def test_dome_classification_window(W, gpu_ctx, precision, tol_f, tol_c, mism):
     """the W1 outputs of every crossing of the 3 m dome (solver_cpu.js:231-271) and its contact layers through the
